@@ -1,0 +1,164 @@
+"""Benchmark: grid-cell RK3 steps per second of the dry + warm-rain step.
+
+Workload (the 512x512x64 warm-rain convection case): dx = dy = 1 km, 64
+levels stretched from 100 m to 500 m up to 16 km, doubly periodic, flat;
+theta = 300 K + 43 K (z/12 km)^1.25 below 12 km and isothermal above, RH =
+1 - 0.75 (z/12 km)^1.25 capped at qv = 14 g/kg, seeded +/-0.5 K theta noise
+below 1 km; dt = 5 s, 6 acoustic substeps, warm rain with 12 m/s fallout.
+A step is one full moist step: dry SSPRK3 (3 slow stages, 18 fast
+substeps), tracer SSPRK3 for qv/qc/qr, warm rain + fallout.  Inputs are
+resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is cell-steps/s of the FP32 step on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gpu-wm-next_amd"))
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--nx", type=int, default=512)
+    p.add_argument("--ny", type=int, default=512)
+    p.add_argument("--nz", type=int, default=64)
+    p.add_argument("--fp64-steps", type=int, default=5,
+                   help="timed steps of the FP64 plain path (0 to skip)")
+    p.add_argument("--cpu-baseline", type=int, default=1,
+                   help="time the FP64 CPU oracle on a bounded sample (0 to skip)")
+    p.add_argument("--cpu-threads", type=int, default=16)
+    return p.parse_args()
+
+
+def make_case(G, precision, nx, ny, nz):
+    ctx = G.Context(0, precision)
+    zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
+    grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    st = G.State(grid, 3)
+    st.init_convection(1.0e5, 42, 0.5, 1000.0)
+    return ctx, grid, st, zi
+
+
+def run(G, precision, nx, ny, nz, steps, warmup, timing):
+    ctx, grid, st, zi = make_case(G, precision, nx, ny, nz)
+    cfg = G.step_config(dt=5.0, fast_substeps=6)
+    wr = G.warm_rain_config(dt=5.0, rain_terminal_velocity=12.0)
+    for _ in range(warmup):
+        st.moist_step(cfg, wr)
+    ctx.sync()
+    if timing:
+        ctx.reset_kernel_stats()
+        ctx.set_kernel_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.moist_step(cfg, wr)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    stats = {}
+    if timing:
+        for name in ("fast_substep", "slow_stage", "level_partials", "level_finalize",
+                     "tracer_stage", "warm_rain", "column_reference"):
+            n, ms = ctx.kernel_stats(name)
+            if n:
+                stats[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n}
+        ctx.set_kernel_timing(False)
+    summary = st.summary()
+    return secs, stats, summary
+
+
+def cpu_baseline(nx, ny, nz, threads):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import pyoracle as O
+    import gwm_amd as G
+
+    # Bounded sample of the same workload: a quarter of the horizontal domain
+    # (identical columns, levels and physics), one moist step, FP64 oracle.
+    nx, ny = max(nx // 2, 8), max(ny // 2, 8)
+    zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
+    ctx, grid, st, _ = make_case(G, G.F64, nx, ny, nz)
+    w = O.World(nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi, n_tracers=3, precision=8)
+    for f in (O.RHO, O.RHO_THETA, O.MOM_U, O.MOM_V, O.MOM_W, O.TRACER0):
+        w.set(f, st.get(f))
+    del st, grid, ctx
+    steps = 1
+    secs = w.time_steps(O.step_cfg(dt=5.0, fast_substeps=6),
+                        O.warm_rain_cfg(dt=5.0, rain_terminal_velocity=12.0), moist=True,
+                        steps=steps, threads=threads)
+    _ = np.float64(secs)
+    return {
+        "value": nx * ny * nz * steps / secs,
+        "unit": "cell-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} full moist step of the {nx}x{ny}x{nz} case (FP64 CPU oracle, "
+                  f"{threads} OpenMP threads, {secs:.1f} s)",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU decomposition is described in DESIGN.md)"}))
+        sys.exit(2)
+    import gwm_amd as G
+
+    cells = a.nx * a.ny * a.nz
+    secs, stats, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True)
+    value = cells * a.steps / secs
+    ns = 6
+    # Dominant kernel: the fast substep.  Algorithmic elements per cell per
+    # launch: read rho, theta, u, v, w, u_base, v_base, w_base (8), write rho,
+    # u, v, w (4), plus rho*theta on the last substep of each subcycle.
+    fs = stats.get("fast_substep")
+    roofline = None
+    if fs:
+        elems = (12 * (ns - 1) + 13) / ns
+        bytes_per_launch = elems * 4 * cells
+        achieved = bytes_per_launch / (fs["avg_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                    "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                    "kernel": "fast_substep", "avg_ms": round(fs["avg_ms"], 5),
+                    "algorithmic_bytes": int(bytes_per_launch)}
+    fp64 = None
+    if a.fp64_steps > 0:
+        s64, st64, _ = run(G, G.F64, a.nx, a.ny, a.nz, a.fp64_steps, 1, False)
+        fp64 = cells * a.fp64_steps / s64
+    cpu = cpu_baseline(a.nx, a.ny, a.nz, a.cpu_threads) if a.cpu_baseline else None
+    out = {
+        "metric": "cell-steps/s per GPU, % HBM roofline; FP32+surrogate vs FP64 plain at set error",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": secs / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"warm-rain convection {a.nx}x{a.ny}x{a.nz}, dx 1 km, stretched "
+                               "100-500 m to 16 km, dt 5 s, 6 substeps, plain warm rain",
+                   "global_batch": 1, "parallelism": "single GPU"},
+        "fp64_plain_value": fp64,
+        "fp32_over_fp64": (value / fp64) if fp64 else None,
+        "dry_mass": summary["total_dry_mass"],
+        "kernels": stats,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1770 @@
+// gwm_capi.cpp -- host side of the C-ABI (include/gwm_hip.h): contexts, grids,
+// device states, the SSPRK3 / fast-mode / tracer / warm-rain orchestration,
+// initial states and compensated budgets.  All compute runs in the HIP
+// kernels of gwm_kernels.hip; this file only allocates, launches and copies.
+#include "../../include/gwm_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gwm_kernels.hpp"
+
+using gwm::DMetrics;
+using gwm::DState;
+using gwm::Geo;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void require(bool c, const std::string& m) {
+  if (!c) throw Error(GWM_E_INVALID, m);
+}
+
+void hipchk(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess)
+    throw Error(GWM_E_HIP, std::string("HIP error at ") + file + ":" + std::to_string(line) +
+                               " for " + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hipchk((x), #x, __FILE__, __LINE__)
+
+template <class F>
+int guard(F f) {
+  try {
+    f();
+    return GWM_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of memory";
+    return GWM_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return GWM_E_INVALID;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct gwm_ctx {
+  int device = 0;
+  int prec = GWM_F32;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  struct Stat {
+    long launches = 0;
+    double ms = 0.0;
+  };
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::map<std::string, Stat> stats;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  double* partials = nullptr;  // device scratch for reductions
+  size_t partials_n = 0;
+  int refs = 0;            // live grids and states
+  bool closing = false;    // gwm_ctx_destroy called while refs > 0
+
+  size_t esize() const { return prec == GWM_F64 ? 8 : 4; }
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  void resolve() {
+    for (auto& p : pending) {
+      HIPCHK(hipEventSynchronize(p.b));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+      auto& s = stats[p.name];
+      s.launches += 1;
+      s.ms += ms;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  double* scratch(size_t n) {
+    if (n > partials_n) {
+      if (partials) HIPCHK(hipFree(partials));
+      HIPCHK(hipMalloc(&partials, n * sizeof(double)));
+      partials_n = n;
+    }
+    return partials;
+  }
+};
+
+namespace {
+
+// Wraps one kernel launch with HIP events on the context stream when
+// per-kernel timing is on.
+struct Timed {
+  gwm_ctx* c;
+  hipEvent_t a = nullptr;
+  const char* name;
+  Timed(gwm_ctx* ctx, const char* n) : c(ctx), name(n) {
+    if (c->timing) {
+      a = c->ev();
+      HIPCHK(hipEventRecord(a, c->stream));
+    }
+  }
+  ~Timed() noexcept(false) {
+    if (std::uncaught_exceptions() > 0) return;
+    HIPCHK(hipGetLastError());
+    if (c->timing) {
+      hipEvent_t b = c->ev();
+      HIPCHK(hipEventRecord(b, c->stream));
+      c->pending.push_back({name, a, b});
+    }
+  }
+};
+
+void* dalloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) return nullptr;
+  HIPCHK(hipMalloc(&p, bytes));
+  return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Grid / metrics
+// ---------------------------------------------------------------------------
+namespace {
+void ctx_free(gwm_ctx* ctx);
+void ctx_release(gwm_ctx* ctx) {
+  if (!ctx) return;
+  ctx->refs -= 1;
+  if (ctx->closing && ctx->refs <= 0) ctx_free(ctx);
+}
+}  // namespace
+
+struct gwm_grid {
+  gwm_ctx* ctx = nullptr;
+  Geo geo{};
+  bool terrain_nonzero = false;
+  bool terrain_active = false;  // |h| > 1e-6 somewhere (dry_fast_modes.cu:484-492)
+  std::vector<double> z_if, z_c, w_if, w_c;
+  void *zif_m = nullptr, *zc_m = nullptr, *idz_c = nullptr, *idz_f = nullptr;
+  void *sx = nullptr, *sy = nullptr, *terr = nullptr;
+  void *lev_idz_c = nullptr, *lev_idz_f = nullptr, *lev_zc = nullptr;
+  ~gwm_grid() {
+    for (void* p : {zif_m, zc_m, idz_c, idz_f, sx, sy, terr, lev_idz_c, lev_idz_f, lev_zc})
+      if (p) (void)hipFree(p);
+    ctx_release(ctx);
+  }
+  template <class R>
+  DMetrics<R> dm() const {
+    return DMetrics<R>{static_cast<const R*>(zc_m), static_cast<const R*>(idz_c),
+                       static_cast<const R*>(idz_f), static_cast<const R*>(lev_idz_c),
+                       static_cast<const R*>(lev_idz_f), static_cast<const R*>(lev_zc)};
+  }
+};
+
+namespace {
+
+// Terrain taper weight (grid_metrics.hpp:126-144).
+double taper_weight(double eta, double taper) {
+  if (eta <= 0.0) return 1.0;
+  if (eta >= 1.0) return 0.0;
+  if (taper <= 0.0) {
+    const double s = std::clamp(eta, 0.0, 1.0);
+    return 1.0 - s * s * (3.0 - 2.0 * s);
+  }
+  if (eta <= taper) return 1.0;
+  const double s = std::clamp((eta - taper) / std::max(1.0e-9, 1.0 - taper), 0.0, 1.0);
+  return 1.0 - s * s * (3.0 - 2.0 * s);
+}
+
+template <class R>
+void build_grid(gwm_grid* G, const double* zin, const float* terrain) {
+  const Geo& g = G->geo;
+  const int nz = g.nz;
+  (void)zin;  // the flat columns were filled by gwm_grid_create
+  if (terrain) {
+    for (size_t a = 0; a < static_cast<size_t>(g.nx) * g.ny; ++a) {
+      if (terrain[a] != 0.0f) G->terrain_nonzero = true;
+      if (std::fabs(static_cast<double>(terrain[a])) > 1.0e-6) G->terrain_active = true;
+    }
+  }
+  const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+  const size_t es = sizeof(R);
+  G->zif_m = dalloc(n2 * (nz + 1) * es);
+  G->zc_m = dalloc(n2 * nz * es);
+  G->idz_c = dalloc(n2 * nz * es);
+  G->idz_f = dalloc(n2 * (nz + 1) * es);
+  G->sx = dalloc(n2 * es);
+  G->sy = dalloc(n2 * es);
+  G->terr = dalloc(n2 * es);
+  double *d_zif = nullptr, *d_zc = nullptr, *d_wif = nullptr, *d_wc = nullptr;
+  float* d_terr = nullptr;
+  d_zif = static_cast<double*>(dalloc((nz + 1) * sizeof(double)));
+  d_zc = static_cast<double*>(dalloc(nz * sizeof(double)));
+  d_wif = static_cast<double*>(dalloc((nz + 1) * sizeof(double)));
+  d_wc = static_cast<double*>(dalloc(nz * sizeof(double)));
+  hipStream_t st = G->ctx->stream;
+  HIPCHK(hipMemcpyAsync(d_zif, G->z_if.data(), (nz + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_zc, G->z_c.data(), nz * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_wif, G->w_if.data(), (nz + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_wc, G->w_c.data(), nz * sizeof(double), hipMemcpyHostToDevice, st));
+  if (terrain) {
+    d_terr = static_cast<float*>(dalloc(n2 * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(d_terr, terrain, n2 * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  {
+    Timed t(G->ctx, "metrics");
+    gwm::launch_metrics<R>(st, g, d_zif, d_zc, d_wif, d_wc, d_terr, static_cast<R*>(G->zif_m),
+                           static_cast<R*>(G->zc_m), static_cast<R*>(G->idz_c),
+                           static_cast<R*>(G->idz_f), static_cast<R*>(G->sx),
+                           static_cast<R*>(G->sy), static_cast<R*>(G->terr));
+  }
+  // Per-level columns for flat terrain, taken from column (0,0) of the
+  // device build so they are identical to the 3-D fields.
+  G->lev_idz_c = dalloc(nz * es);
+  G->lev_idz_f = dalloc((nz + 1) * es);
+  G->lev_zc = dalloc(nz * es);
+  HIPCHK(hipMemcpy2DAsync(G->lev_idz_c, es, G->idz_c, n2 * es, es, nz, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(G->lev_idz_f, es, G->idz_f, n2 * es, es, nz + 1, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(G->lev_zc, es, G->zc_m, n2 * es, es, nz, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (void* p : {static_cast<void*>(d_zif), static_cast<void*>(d_zc), static_cast<void*>(d_wif),
+                  static_cast<void*>(d_wc), static_cast<void*>(d_terr)})
+    if (p) HIPCHK(hipFree(p));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// State
+// ---------------------------------------------------------------------------
+struct gwm_state {
+  gwm_ctx* ctx = nullptr;
+  const gwm_grid* grid = nullptr;
+  int ntr = 0;
+  bool view = false;        // stage view handed to a boundary callback
+  gwm_state* owner = nullptr;
+  int nsets = 0;
+  void* set[5][5] = {};     // buffer sets: rho, rho_theta (or theta), u, v, w
+  int cur = 0;              // canonical set index
+  void* tend[5] = {};
+  void* pressure = nullptr;
+  std::vector<void*> tr;             // canonical tracers
+  std::vector<void*> tr_scr[3];      // tracer stage buffers
+  void* precip = nullptr;
+  void* bref[5] = {};
+  std::vector<void*> bref_tr;
+  bool have_boundary = false;
+  void* theta_ref = nullptr;  // nz
+  void* rref_lev = nullptr;   // nz
+  void* pref_lev = nullptr;   // nz
+  void* rref3 = nullptr;      // nx*ny*nz
+  void* pref3 = nullptr;
+  void* theta_scr = nullptr;  // cell field
+  bool faces_synced = false;
+  // view of a stage set
+  void* vset[5] = {};
+
+  const Geo& g() const { return grid->geo; }
+  size_t es() const { return ctx->esize(); }
+  size_t fsize(int f) const {
+    const Geo& G = g();
+    switch (f) {
+      case GWM_MOM_U:
+      case GWM_TEND_U:
+        return G.xf_total();
+      case GWM_MOM_V:
+      case GWM_TEND_V:
+        return G.yf_total();
+      case GWM_MOM_W:
+      case GWM_TEND_W:
+        return G.zf_total();
+      case GWM_PRECIP:
+        return static_cast<size_t>(G.nx) * G.ny;
+      default:
+        return G.cell_total();
+    }
+  }
+  void* alloc_set_field(int f) { return dalloc(fsize(f) * es()); }
+  void ensure_sets(int n) {
+    for (int s = nsets; s < n; ++s)
+      for (int f = 0; f < 5; ++f) set[s][f] = alloc_set_field(f);
+    nsets = std::max(nsets, n);
+  }
+  void** cur_set() { return view ? vset : set[cur]; }
+  void* field(int f) {
+    if (f >= GWM_RHO && f <= GWM_MOM_W) return cur_set()[f];
+    if (f == GWM_PRESSURE) {
+      if (!pressure) pressure = dalloc(g().cell_total() * es());
+      return pressure;
+    }
+    if (f >= GWM_TEND_RHO && f <= GWM_TEND_W) {
+      const int a = f - GWM_TEND_RHO;
+      if (!tend[a]) tend[a] = dalloc(fsize(f) * es());
+      return tend[a];
+    }
+    if (f == GWM_PRECIP) return precip;
+    const int t = f - GWM_TRACER0;
+    require(t >= 0 && t < ntr, "bad field id");
+    return view ? owner->tr[t] : tr[t];
+  }
+  ~gwm_state() {
+    if (view) return;
+    struct Release {
+      gwm_ctx* c;
+      ~Release() { ctx_release(c); }
+    } release{ctx};
+    for (auto& s : set)
+      for (void* p : s)
+        if (p) (void)hipFree(p);
+    for (void* p : tend)
+      if (p) (void)hipFree(p);
+    for (void* p : bref)
+      if (p) (void)hipFree(p);
+    for (auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr})
+      for (void* p : *v)
+        if (p) (void)hipFree(p);
+    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, rref3, pref3, theta_scr})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+gwm_state* S_(gwm_state* s) {
+  require(s != nullptr, "null state");
+  return s;
+}
+
+template <class R>
+DState<R> dstate(void* const* f) {
+  return DState<R>{static_cast<R*>(f[0]), static_cast<R*>(f[1]), static_cast<R*>(f[2]),
+                   static_cast<R*>(f[3]), static_cast<R*>(f[4])};
+}
+
+// Wrap-face synchronisation of the canonical faces (makes the reference's
+// owned-interface averaging an identity inside the step).
+template <class R>
+void ensure_synced(gwm_state* S, void** fs) {
+  if (S->faces_synced) return;
+  const Geo& g = S->g();
+  if (g.per_x) {
+    Timed t(S->ctx, "sync_interfaces");
+    gwm::launch_sync_interfaces<R>(S->ctx->stream, g, 1, static_cast<R*>(fs[GWM_MOM_U]));
+  }
+  if (g.per_y) {
+    Timed t(S->ctx, "sync_interfaces");
+    gwm::launch_sync_interfaces<R>(S->ctx->stream, g, 2, static_cast<R*>(fs[GWM_MOM_V]));
+  }
+  S->faces_synced = true;
+}
+
+template <class R>
+void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
+  const Geo& g = S->g();
+  if (!S->theta_ref) S->theta_ref = dalloc(g.nz * sizeof(R));
+  double* part = S->ctx->scratch(static_cast<size_t>(g.nz) * ((g.ny + 7) / 8) * 2);
+  int nch = 0;
+  {
+    Timed t(S->ctx, "level_partials");
+    gwm::launch_level_partials<R>(S->ctx->stream, g, rho, rt, 0, part, &nch);
+  }
+  {
+    Timed t(S->ctx, "level_finalize");
+    gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nch, part, 0,
+                                  static_cast<double>(g.nx) * g.ny,
+                                  static_cast<R*>(S->theta_ref), nullptr);
+  }
+}
+
+// One fast subcycle (apply_local_split_explicit_fast_modes) from
+// (rho_in, faces_in) with fixed theta and base faces; returns the set index
+// holding the result (rho, rho*theta, faces).  out sets are a[0], a[1].
+template <class R>
+int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u_in,
+             const R* v_in, const R* w_in, const R* theta, const R* ub, const R* vb, const R* wb,
+             int z0, int z1) {
+  const Geo& g = S->g();
+  const gwm_grid* G = S->grid;
+  const int ns = std::max(1, c.fast_substeps);
+  const R dtf = static_cast<R>(c.dt) / static_cast<R>(ns);
+  const bool colref = G->terrain_active;
+  if (!colref) {
+    if (!S->rref_lev) S->rref_lev = dalloc(g.nz * sizeof(R));
+    if (!S->pref_lev) S->pref_lev = dalloc(g.nz * sizeof(R));
+  } else {
+    if (!S->rref3) S->rref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
+    if (!S->pref3) S->pref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
+  }
+  const int outs[2] = {z0, z1};
+  const R* rho = rho_in;
+  const R *u = u_in, *v = v_in, *w = w_in;
+  int last = z0;
+  for (int s = 0; s < ns; ++s) {
+    const int o = outs[s % 2];
+    if (!colref) {
+      double* part = S->ctx->scratch(static_cast<size_t>(g.nz) * ((g.ny + 7) / 8) * 2);
+      int nch = 0;
+      {
+        Timed t(S->ctx, "level_partials");
+        gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nch);
+      }
+      Timed t(S->ctx, "level_finalize");
+      gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nch, part, 1,
+                                    static_cast<double>(g.nx) * g.ny,
+                                    static_cast<R*>(S->rref_lev), static_cast<R*>(S->pref_lev));
+    } else {
+      Timed t(S->ctx, "column_reference");
+      gwm::launch_column_reference<R>(S->ctx->stream, g, rho, theta,
+                                      static_cast<const R*>(G->zc_m), static_cast<R>(c.gravity),
+                                      static_cast<R*>(S->rref3), static_cast<R*>(S->pref3));
+    }
+    gwm::FastArgs<R> a{};
+    a.g = g;
+    a.rho = rho;
+    a.theta = theta;
+    a.u = u;
+    a.v = v;
+    a.w = w;
+    a.ub = ub;
+    a.vb = vb;
+    a.wb = wb;
+    a.rho_out = static_cast<R*>(S->set[o][0]);
+    a.rt_out = s == ns - 1 ? static_cast<R*>(S->set[o][1]) : nullptr;
+    a.u_out = static_cast<R*>(S->set[o][2]);
+    a.v_out = static_cast<R*>(S->set[o][3]);
+    a.w_out = static_cast<R*>(S->set[o][4]);
+    a.rref_lev = static_cast<const R*>(S->rref_lev);
+    a.pref_lev = static_cast<const R*>(S->pref_lev);
+    a.rref3 = static_cast<const R*>(S->rref3);
+    a.pref3 = static_cast<const R*>(S->pref3);
+    a.m = G->dm<R>();
+    a.dtf = dtf;
+    a.gravity = static_cast<R>(c.gravity);
+    a.horizontal = c.fast_update_horizontal_momentum;
+    a.density = c.fast_update_density;
+    a.col_ref = colref;
+    {
+      Timed t(S->ctx, "fast_substep");
+      gwm::launch_fast_substep<R>(S->ctx->stream, a, G->terrain_nonzero);
+    }
+    rho = a.rho_out;
+    u = a.u_out;
+    v = a.v_out;
+    w = a.w_out;
+    last = o;
+  }
+  return last;
+}
+
+template <class R>
+void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* const* q0,
+                void* const* out, int mode, R wa, R wb, int out_theta) {
+  const Geo& g = S->g();
+  level_theta_ref<R>(S, static_cast<const R*>(in[0]), static_cast<const R*>(in[1]));
+  gwm::SlowArgs<R> a{};
+  a.g = g;
+  a.s = dstate<R>(in);
+  if (q0) a.q0 = dstate<R>(q0);
+  a.out = dstate<R>(out);
+  a.m = S->grid->dm<R>();
+  a.theta_ref = static_cast<const R*>(S->theta_ref);
+  a.gravity = static_cast<R>(c.gravity);
+  a.dt = static_cast<R>(c.dt);
+  a.wa = wa;
+  a.wb = wb;
+  a.mode = mode;
+  a.out_theta = out_theta;
+  Timed t(S->ctx, "slow_stage");
+  gwm::launch_slow_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+}
+
+int orient_of(int f) {
+  switch (f) {
+    case GWM_MOM_U:
+    case GWM_TEND_U:
+      return 1;
+    case GWM_MOM_V:
+    case GWM_TEND_V:
+      return 2;
+    case GWM_MOM_W:
+    case GWM_TEND_W:
+      return 3;
+    default:
+      return 0;
+  }
+}
+
+template <class R>
+void apply_boundary_set(gwm_state* S, void* const* fs, int mode, double width) {
+  if (mode == GWM_BOUNDARY_NONE) return;
+  require(S->have_boundary, "boundary reference not captured (gwm_state_capture_boundary)");
+  const Geo& g = S->g();
+  for (int f = 0; f < 5; ++f) {
+    Timed t(S->ctx, "boundary");
+    gwm::launch_boundary<R>(S->ctx->stream, g, orient_of(f), static_cast<R*>(fs[f]),
+                            static_cast<const R*>(S->bref[f]), mode, width);
+  }
+}
+
+template <class R>
+void run_boundary(gwm_state* S, void** fs, const gwm_boundary_config* b, double sim_time) {
+  if (!b) return;
+  apply_boundary_set<R>(S, fs, b->mode, b->relax_width);
+  if (b->callback) {
+    gwm_state view;
+    view.ctx = S->ctx;
+    view.grid = S->grid;
+    view.ntr = S->ntr;
+    view.view = true;
+    view.owner = S;
+    view.faces_synced = true;
+    for (int f = 0; f < 5; ++f) view.vset[f] = fs[f];
+    HIPCHK(hipStreamSynchronize(S->ctx->stream));
+    const int rc = b->callback(&view, sim_time, b->user);
+    require(rc == 0, "boundary callback failed");
+  }
+}
+
+// advance_dry_state_ssprk3 (dry_core.cu:541-651)
+template <class R>
+void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b) {
+  const bool bnd = b && (b->mode != GWM_BOUNDARY_NONE || b->callback);
+  S->ensure_sets(bnd ? 5 : 4);
+  const int X = S->cur;
+  int others[4], n = 0;
+  for (int s = 0; s < S->nsets && n < 4; ++s)
+    if (s != X) others[n++] = s;
+  const int Y = others[0], Z0 = others[1], Z1 = others[2];
+  ensure_synced<R>(S, S->set[X]);
+  void** in = S->set[X];
+  if (bnd) {
+    const int Wb = others[3];
+    for (int f = 0; f < 5; ++f) {
+      Timed t(S->ctx, "copy");
+      gwm::launch_copy<R>(S->ctx->stream, static_cast<const R*>(S->set[X][f]),
+                          static_cast<R*>(S->set[Wb][f]), S->fsize(f));
+    }
+    in = S->set[Wb];
+    run_boundary<R>(S, in, b, 0.0);
+  }
+  auto sub = [&](int y) {
+    void** ys = S->set[y];
+    return subcycle<R>(S, c, static_cast<const R*>(ys[0]), static_cast<const R*>(ys[2]),
+                       static_cast<const R*>(ys[3]), static_cast<const R*>(ys[4]),
+                       static_cast<const R*>(ys[1]), static_cast<const R*>(ys[2]),
+                       static_cast<const R*>(ys[3]), static_cast<const R*>(ys[4]), Z0, Z1);
+  };
+  // stage 1
+  slow_stage<R>(S, c, in, nullptr, S->set[Y], 1, R(0), R(1), 1);
+  int r = sub(Y);
+  run_boundary<R>(S, S->set[r], b, c.dt);
+  // stage 2
+  slow_stage<R>(S, c, S->set[r], S->set[X], S->set[Y], 2, R(0.75), R(0.25), 1);
+  r = sub(Y);
+  run_boundary<R>(S, S->set[r], b, 0.5 * c.dt);
+  // stage 3
+  slow_stage<R>(S, c, S->set[r], S->set[X], S->set[Y], 2, R(1) / R(3), R(2) / R(3), 1);
+  r = sub(Y);
+  S->cur = r;
+}
+
+// apply_local_split_explicit_fast_modes on the canonical state.
+template <class R>
+void fast_modes(gwm_state* S, const gwm_step_config& c) {
+  S->ensure_sets(4);
+  const int X = S->cur;
+  int others[3], n = 0;
+  for (int s = 0; s < S->nsets && n < 3; ++s)
+    if (s != X) others[n++] = s;
+  ensure_synced<R>(S, S->set[X]);
+  const Geo& g = S->g();
+  if (!S->theta_scr) S->theta_scr = dalloc(g.cell_total() * sizeof(R));
+  {
+    Timed t(S->ctx, "theta");
+    gwm::launch_theta<R>(S->ctx->stream, g, static_cast<const R*>(S->set[X][0]),
+                         static_cast<const R*>(S->set[X][1]), static_cast<R*>(S->theta_scr));
+  }
+  void** xs = S->set[X];
+  const int r = subcycle<R>(S, c, static_cast<const R*>(xs[0]), static_cast<const R*>(xs[2]),
+                            static_cast<const R*>(xs[3]), static_cast<const R*>(xs[4]),
+                            static_cast<const R*>(S->theta_scr), static_cast<const R*>(xs[2]),
+                            static_cast<const R*>(xs[3]), static_cast<const R*>(xs[4]),
+                            others[0], others[1]);
+  S->cur = r;
+}
+
+template <class R>
+void slow_tendencies(gwm_state* S, const gwm_step_config& c) {
+  ensure_synced<R>(S, S->cur_set());
+  void* out[5];
+  for (int f = 0; f < 5; ++f) out[f] = S->field(GWM_TEND_RHO + f);
+  slow_stage<R>(S, c, S->cur_set(), nullptr, out, 0, R(0), R(1), 0);
+}
+
+// advance_passive_tracers_ssprk3 (dry_transport.cu:440-518)
+template <class R>
+void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
+  if (S->ntr == 0) return;
+  const Geo& g = S->g();
+  ensure_synced<R>(S, S->set[S->cur]);
+  for (int k = 0; k < 3; ++k)
+    if (S->tr_scr[k].empty())
+      for (int t = 0; t < S->ntr; ++t) S->tr_scr[k].push_back(dalloc(g.cell_total() * sizeof(R)));
+  void** ds = S->set[S->cur];
+  gwm::TracerArgs<R> a{};
+  a.g = g;
+  a.rho = static_cast<const R*>(ds[0]);
+  a.u = static_cast<const R*>(ds[2]);
+  a.v = static_cast<const R*>(ds[3]);
+  a.w = static_cast<const R*>(ds[4]);
+  a.idz_c = static_cast<const R*>(S->grid->idz_c);
+  a.lev_idz_c = static_cast<const R*>(S->grid->lev_idz_c);
+  a.ntr = S->ntr;
+  a.dt = static_cast<R>(c.dt);
+  auto strips = [&](std::vector<void*>& f) {
+    if (bmode == GWM_BOUNDARY_NONE) return;
+    require(S->have_boundary, "boundary reference not captured");
+    for (int t = 0; t < S->ntr; ++t) {
+      Timed tt(S->ctx, "boundary");
+      gwm::launch_boundary<R>(S->ctx->stream, g, 0, static_cast<R*>(f[t]),
+                              static_cast<const R*>(S->bref_tr[t]), GWM_BOUNDARY_STRIPS, 0.0);
+    }
+  };
+  std::vector<void*>& A = S->tr_scr[0];
+  std::vector<void*>& B = S->tr_scr[1];
+  std::vector<void*>& C = S->tr_scr[2];
+  std::vector<void*> in1 = S->tr;
+  if (bmode != GWM_BOUNDARY_NONE) {
+    for (int t = 0; t < S->ntr; ++t) {
+      Timed tt(S->ctx, "copy");
+      gwm::launch_copy<R>(S->ctx->stream, static_cast<const R*>(S->tr[t]), static_cast<R*>(C[t]),
+                          g.cell_total());
+    }
+    strips(C);
+    in1 = C;
+  }
+  auto stage = [&](const std::vector<void*>& in, std::vector<void*>& out, int mode, R wa, R wb) {
+    for (int t = 0; t < S->ntr; ++t) {
+      a.in[t] = static_cast<const R*>(in[t]);
+      a.q0[t] = static_cast<const R*>(S->tr[t]);
+      a.out[t] = static_cast<R*>(out[t]);
+    }
+    a.mode = mode;
+    a.wa = wa;
+    a.wb = wb;
+    Timed tt(S->ctx, "tracer_stage");
+    gwm::launch_tracer_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+  };
+  stage(in1, A, 1, R(0), R(1));
+  strips(A);
+  stage(A, B, 2, R(0.75), R(0.25));
+  strips(B);
+  stage(B, A, 2, R(1) / R(3), R(2) / R(3));
+  std::swap(S->tr, S->tr_scr[0]);
+}
+
+template <class R>
+void warm_rain(gwm_state* S, const gwm_warm_rain_config& c, int metric) {
+  require(S->ntr >= 3, "warm rain requires the qv, qc, qr tracers");
+  const Geo& g = S->g();
+  void** ds = S->cur_set();
+  gwm::WarmRainArgs<R> a{};
+  a.g = g;
+  a.rho = static_cast<const R*>(ds[0]);
+  a.rt = static_cast<R*>(ds[1]);
+  a.qv = static_cast<R*>(S->field(GWM_TRACER0 + 0));
+  a.qc = static_cast<R*>(S->field(GWM_TRACER0 + 1));
+  a.qr = static_cast<R*>(S->field(GWM_TRACER0 + 2));
+  a.precip = static_cast<R*>(S->view ? S->owner->precip : S->precip);
+  a.idz_c = static_cast<const R*>(S->grid->idz_c);
+  a.lev_idz_c = static_cast<const R*>(S->grid->lev_idz_c);
+  a.dt = static_cast<R>(c.dt);
+  a.relax = static_cast<R>(c.condensation_relaxation);
+  a.thr = static_cast<R>(c.cloud_autoconversion_threshold);
+  a.krate = static_cast<R>(c.cloud_autoconversion_rate);
+  a.erate = static_cast<R>(c.rain_evaporation_rate);
+  a.vt = static_cast<R>(c.rain_terminal_velocity);
+  a.depth = static_cast<R>(c.sedimentation_layer_depth_m);
+  a.kacc = static_cast<R>(c.accretion_rate);
+  a.heat = c.enable_latent_heating;
+  if (metric) a.fallout = a.vt > R(0) ? 2 : 0;
+  else a.fallout = (a.vt > R(0) && a.depth > R(0)) ? 1 : 0;
+  Timed t(S->ctx, "warm_rain");
+  gwm::launch_warm_rain<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+}
+
+// Compensated reduction of one field; returns {sum, min, max}.
+template <class R>
+void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, const R* rho,
+                  double out[3]) {
+  const int nb = 512;
+  double* part = S->ctx->scratch(static_cast<size_t>(nb) * 4);
+  gwm::ReduceArgs<R> a{};
+  a.f = f;
+  a.rho = rho;
+  a.idz3 = S->grid->terrain_nonzero ? static_cast<const R*>(S->grid->idz_c) : nullptr;
+  a.idz_lev = static_cast<const R*>(S->grid->lev_idz_c);
+  a.fx = fx;
+  a.fy = fy;
+  a.fz = fz;
+  a.h = S->g().h;
+  a.kind = kind;
+  a.partials = part;
+  {
+    Timed t(S->ctx, "reduce");
+    gwm::launch_reduce<R>(S->ctx->stream, a, nb);
+  }
+  std::vector<double> h(static_cast<size_t>(nb) * 4);
+  HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        S->ctx->stream));
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300;
+  for (int b = 0; b < nb; ++b) {  // Neumaier over block partials, fixed order
+    for (int q = 0; q < 2; ++q) {
+      const double v = h[b * 4 + q];
+      const double s2 = sum + v;
+      comp += std::fabs(sum) >= std::fabs(v) ? (sum - s2) + v : (v - s2) + sum;
+      sum = s2;
+    }
+    mn = std::min(mn, h[b * 4 + 2]);
+    mx = std::max(mx, h[b * 4 + 3]);
+  }
+  out[0] = sum + comp;
+  out[1] = mn;
+  out[2] = mx;
+}
+
+template <class R>
+void budget(gwm_state* S, double* o) {
+  const Geo& g = S->g();
+  for (int a = 0; a < GWM_SUMMARY_N; ++a) o[a] = 0.0;
+  void** ds = S->cur_set();
+  const R* rho = static_cast<const R*>(ds[0]);
+  double r[3];
+  reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r);
+  o[GWM_S_TOTAL_DRY_MASS] = r[0];
+  o[GWM_S_RHO_MIN] = r[1];
+  o[GWM_S_RHO_MAX] = r[2];
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 0, nullptr, r);
+  o[GWM_S_TOTAL_RHO_THETA] = r[0];
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 2, rho, r);
+  o[GWM_S_THETA_MIN] = r[1];
+  o[GWM_S_THETA_MAX] = r[2];
+  reduce_field<R>(S, static_cast<const R*>(ds[2]), g.nx + 1, g.ny, g.nz, 0, nullptr, r);
+  o[GWM_S_TOTAL_MOM_X] = r[0];
+  o[GWM_S_U_MIN] = r[1];
+  o[GWM_S_U_MAX] = r[2];
+  reduce_field<R>(S, static_cast<const R*>(ds[3]), g.nx, g.ny + 1, g.nz, 0, nullptr, r);
+  o[GWM_S_TOTAL_MOM_Y] = r[0];
+  o[GWM_S_V_MIN] = r[1];
+  o[GWM_S_V_MAX] = r[2];
+  reduce_field<R>(S, static_cast<const R*>(ds[4]), g.nx, g.ny, g.nz + 1, 0, nullptr, r);
+  o[GWM_S_TOTAL_MOM_Z] = r[0];
+  o[GWM_S_W_MIN] = r[1];
+  o[GWM_S_W_MAX] = r[2];
+  for (int t = 0; t < std::min(S->ntr, 3); ++t) {
+    const R* q = static_cast<const R*>(S->field(GWM_TRACER0 + t));
+    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 1, nullptr, r);
+    o[GWM_S_VAPOR_PATH + t] = r[0];
+    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 2, rho, r);
+    o[GWM_S_TRACER_QMIN0 + 2 * t] = r[1];
+    o[GWM_S_TRACER_QMIN0 + 2 * t + 1] = r[2];
+  }
+  const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+  std::vector<R> p(n2);
+  HIPCHK(hipMemcpyAsync(p.data(), S->view ? S->owner->precip : S->precip, n2 * sizeof(R),
+                        hipMemcpyDeviceToHost, S->ctx->stream));
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  double ps = 0.0, pc = 0.0, pm = 0.0;
+  long wet = 0;
+  for (size_t a = 0; a < n2; ++a) {
+    const double v = static_cast<double>(p[a]);
+    const double s2 = ps + v;
+    pc += std::fabs(ps) >= std::fabs(v) ? (ps - s2) + v : (v - s2) + ps;
+    ps = s2;
+    pm = std::max(pm, v);
+    if (v > 0.0) ++wet;
+  }
+  o[GWM_S_PRECIP_SUM] = ps + pc;
+  o[GWM_S_PRECIP_MEAN] = n2 ? (ps + pc) / static_cast<double>(n2) : 0.0;
+  o[GWM_S_PRECIP_MAX] = pm;
+  o[GWM_S_PRECIP_CELLS] = static_cast<double>(n2);
+  o[GWM_S_PRECIPITATING_CELLS] = static_cast<double>(wet);
+}
+
+// ---------------------------------------------------------------------------
+// Host initial states.  Built in the working precision on the host with the
+// reference's formulas, then uploaded.
+// ---------------------------------------------------------------------------
+template <class R>
+struct HostThermo;
+template <>
+struct HostThermo<float> {
+  static constexpr float Rd = 287.05f, p0 = 100000.0f, kappa = 0.2854f, g = 9.81f;
+  static constexpr float pi = 3.14159265358979323846f;
+};
+template <>
+struct HostThermo<double> {
+  static constexpr double Rd = 287.05, p0 = 100000.0, kappa = 0.2854, g = 9.81;
+  static constexpr double pi = 3.14159265358979323846;
+};
+
+template <class R>
+R h_pressure(R rho, R rt) {
+  using T = HostThermo<R>;
+  const R th = rt / std::max(rho, R(1.0e-6));
+  const R ratio = std::max(rho * T::Rd * th / T::p0, R(1.0e-6));
+  return T::p0 * std::pow(ratio, R(1) / (R(1) - T::kappa));
+}
+template <class R>
+R h_rho_from_p_theta(R p, R th) {
+  using T = HostThermo<R>;
+  const R ex = std::pow(std::max(p / T::p0, R(1.0e-6)), T::kappa);
+  return p / std::max(T::Rd * th * ex, R(1.0e-6));
+}
+// hydrostatic_rho_at_height (dry_core.cu:22-35)
+template <class R>
+R h_hydro_rho(R rho_s, R th, R z) {
+  using T = HostThermo<R>;
+  const R ps = h_pressure(rho_s, rho_s * th);
+  const R exs = std::pow(std::max(ps / T::p0, R(1.0e-6)), T::kappa);
+  const R ex = std::max(R(1.0e-4), exs - T::g * z / ((T::Rd / T::kappa) * th));
+  const R p = T::p0 * std::pow(ex, R(1) / T::kappa);
+  return h_rho_from_p_theta(p, th);
+}
+
+template <class R>
+struct HostState {
+  std::vector<R> rho, rt, u, v, w;
+  std::vector<std::vector<R>> q;
+  HostState(const gwm_state* S) {
+    const Geo& g = S->g();
+    rho.assign(g.cell_total(), R(0));
+    rt.assign(g.cell_total(), R(0));
+    u.assign(g.xf_total(), R(0));
+    v.assign(g.yf_total(), R(0));
+    w.assign(g.zf_total(), R(0));
+    q.assign(S->ntr, std::vector<R>(g.cell_total(), R(0)));
+  }
+  void upload(gwm_state* S) {
+    std::vector<R>* f[5] = {&rho, &rt, &u, &v, &w};
+    for (int a = 0; a < 5; ++a)
+      HIPCHK(hipMemcpy(S->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R), hipMemcpyHostToDevice));
+    for (int t = 0; t < S->ntr; ++t)
+      HIPCHK(hipMemcpy(S->field(GWM_TRACER0 + t), q[t].data(), q[t].size() * sizeof(R),
+                       hipMemcpyHostToDevice));
+    S->faces_synced = false;
+  }
+};
+
+template <class R>
+void fill_constant(const gwm_state* S, HostState<R>& H, R rho, R th, R u, R v, R w) {
+  (void)S;
+  std::fill(H.rho.begin(), H.rho.end(), rho);
+  std::fill(H.rt.begin(), H.rt.end(), rho * th);
+  std::fill(H.u.begin(), H.u.end(), rho * u);
+  std::fill(H.v.begin(), H.v.end(), rho * v);
+  std::fill(H.w.begin(), H.w.end(), rho * w);
+}
+
+template <class R>
+std::vector<R> host_metric(const gwm_state* S, int which) {
+  const Geo& g = S->g();
+  const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+  const size_t n = which == 0 || which == 3 ? n2 * (g.nz + 1) : n2 * g.nz;
+  std::vector<R> out(n);
+  const void* src = which == 0 ? S->grid->zif_m : which == 1 ? S->grid->zc_m
+                  : which == 2 ? S->grid->idz_c : S->grid->idz_f;
+  HIPCHK(hipMemcpy(out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost));
+  return out;
+}
+
+template <class R>
+void init_hydro(gwm_state* S, double rho_s, double th, HostState<R>& H) {
+  const Geo& g = S->g();
+  fill_constant<R>(S, H, R(0), static_cast<R>(th), R(0), R(0), R(0));
+  const std::vector<R> zc = host_metric<R>(S, 1);
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const R r = h_hydro_rho(static_cast<R>(rho_s), static_cast<R>(th), zc[g.metric(i, j, k)]);
+        H.rho[g.cell(i, j, k)] = r;
+        H.rt[g.cell(i, j, k)] = r * static_cast<R>(th);
+      }
+  std::fill(H.u.begin(), H.u.end(), R(0));
+  std::fill(H.v.begin(), H.v.end(), R(0));
+  std::fill(H.w.begin(), H.w.end(), R(0));
+}
+
+// make_bubble_like_state (idealized_cases.cpp:105-162)
+template <class R>
+void init_bubble(gwm_state* S, const double* p, int sign) {
+  using T = HostThermo<R>;
+  const Geo& g = S->g();
+  HostState<R> H(S);
+  const bool hydro = p[0] <= 0.0;
+  if (hydro) init_hydro<R>(S, -p[0], p[1], H);
+  else fill_constant<R>(S, H, static_cast<R>(p[0]), static_cast<R>(p[1]), static_cast<R>(p[2]),
+                        static_cast<R>(p[3]), static_cast<R>(p[4]));
+  const std::vector<R> zc = host_metric<R>(S, 1);
+  const double z_top = S->grid->z_if.back();
+  const R dxr = static_cast<R>(g.dx), dyr = static_cast<R>(g.dy);
+  const R x0 = static_cast<R>(p[5]) * static_cast<R>(g.nx - 1) * dxr;
+  const R y0 = static_cast<R>(p[6]) * static_cast<R>(g.ny - 1) * dyr;
+  const R z0 = static_cast<R>(p[7]) * static_cast<R>(z_top);
+  const R rx = std::max(static_cast<R>(p[8]) * static_cast<R>(g.nx) * dxr, R(1));
+  const R ry = std::max(static_cast<R>(p[9]) * static_cast<R>(g.ny) * dyr, R(1));
+  const R rz = std::max(static_cast<R>(p[10]) * static_cast<R>(z_top), R(1));
+  const R dth = static_cast<R>(sign >= 0 ? std::fabs(p[11]) : -std::fabs(p[11]));
+  const R thb = static_cast<R>(p[1]);
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j) {
+      const R y = static_cast<R>(j) * dyr;
+      for (int i = 0; i < g.nx; ++i) {
+        const R x = static_cast<R>(i) * dxr;
+        const R z = zc[g.metric(i, j, k)];
+        const R wx = (x - x0) / rx, wy = (y - y0) / ry, wz = (z - z0) / rz;
+        const R r2 = wx * wx + wy * wy + wz * wz;
+        const R wgt = r2 >= R(1) ? R(0) : R(0.5) * (R(1) + std::cos(T::pi * std::sqrt(r2)));
+        const R rb = hydro ? H.rho[g.cell(i, j, k)] : static_cast<R>(p[0]);
+        H.rt[g.cell(i, j, k)] = rb * (thb + dth * wgt);
+      }
+    }
+  H.upload(S);
+}
+
+// set_background_face_momenta (idealized_cases.cpp:36-103)
+template <class R>
+void face_momenta(gwm_state* S, HostState<R>& H, R U, R V, R W) {
+  const Geo& g = S->g();
+  auto rho = [&](int i, int j, int k) { return H.rho[g.cell(i, j, k)]; };
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i <= g.nx; ++i) {
+        R f;
+        if (g.nx == 1 || i <= 0) f = rho(0, j, k);
+        else if (i >= g.nx) f = rho(g.nx - 1, j, k);
+        else f = R(0.5) * (rho(i - 1, j, k) + rho(i, j, k));
+        H.u[g.xf(i, j, k)] = f * U;
+      }
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j <= g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        R f;
+        if (g.ny == 1 || j <= 0) f = rho(i, 0, k);
+        else if (j >= g.ny) f = rho(i, g.ny - 1, k);
+        else f = R(0.5) * (rho(i, j - 1, k) + rho(i, j, k));
+        H.v[g.yf(i, j, k)] = f * V;
+      }
+  for (int k = 0; k <= g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        R f;
+        if (g.nz == 1 || k <= 0) f = rho(i, j, 0);
+        else if (k >= g.nz) f = rho(i, j, g.nz - 1);
+        else f = R(0.5) * (rho(i, j, k - 1) + rho(i, j, k));
+        H.w[g.zf(i, j, k)] = f * W;
+      }
+}
+
+// make_mountain_wave_background_state (idealized_cases.cpp:182-223)
+template <class R>
+void init_mountain(gwm_state* S, double rho_s, double th, double u, double v, double w) {
+  const Geo& g = S->g();
+  HostState<R> H(S);
+  fill_constant<R>(S, H, R(0), static_cast<R>(th), static_cast<R>(u), static_cast<R>(v),
+                   static_cast<R>(w));
+  std::vector<R> terr(static_cast<size_t>(g.nx) * g.ny);
+  HIPCHK(hipMemcpy(terr.data(), S->grid->terr, terr.size() * sizeof(R), hipMemcpyDeviceToHost));
+  const auto& zc = S->grid->z_c;
+  const auto& wc = S->grid->w_c;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double z = zc[k] + wc[k] * static_cast<double>(terr[static_cast<size_t>(j) * g.nx + i]);
+        const R r = h_hydro_rho(static_cast<R>(rho_s), static_cast<R>(th), static_cast<R>(z));
+        H.rho[g.cell(i, j, k)] = r;
+        H.rt[g.cell(i, j, k)] = r * static_cast<R>(th);
+      }
+  face_momenta<R>(S, H, static_cast<R>(u), static_cast<R>(v), static_cast<R>(w));
+  H.upload(S);
+}
+
+// Extension: constant-N sounding on terrain.  theta(z) = theta_s exp(N^2 z/g);
+// Exner from the closed-form hydrostatic integral
+// Pi(z) = Pi_s - g^2/(c_p theta_s N^2) (1 - exp(-N^2 z / g)), in FP64.
+template <class R>
+void init_constant_n(gwm_state* S, double th_s, double N, double p_s, double u, double v) {
+  const Geo& g = S->g();
+  HostState<R> H(S);
+  const double Rd = 287.05, kap = 0.2854, cp = Rd / kap, grav = 9.81, p0 = 1.0e5;
+  const double Pis = std::pow(p_s / p0, kap);
+  const std::vector<R> zc = host_metric<R>(S, 1);
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double z = static_cast<double>(zc[g.metric(i, j, k)]);
+        const double th = th_s * std::exp(N * N * z / grav);
+        const double Pi = Pis - grav * grav / (cp * th_s * N * N) * (1.0 - std::exp(-N * N * z / grav));
+        const double p = p0 * std::pow(Pi, 1.0 / kap);
+        const double r = p / (Rd * th * Pi);
+        H.rho[g.cell(i, j, k)] = static_cast<R>(r);
+        H.rt[g.cell(i, j, k)] = static_cast<R>(r) * static_cast<R>(th);
+      }
+  face_momenta<R>(S, H, static_cast<R>(u), static_cast<R>(v), R(0));
+  H.upload(S);
+}
+
+// Counter-based uniform deviate in [-0.5, 0.5) keyed by (seed, i, j, k)
+// (splitmix64 finaliser), identical on every host.
+double noise(unsigned long long seed, int i, int j, int k) {
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull ^ (static_cast<uint64_t>(i) << 42) ^
+               (static_cast<uint64_t>(j) << 21) ^ static_cast<uint64_t>(k);
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return static_cast<double>(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+// Extension: warm-rain convection sounding (flat terrain), FP64 column
+// integration of the Exner function with a fine trapezoid rule.
+template <class R>
+void init_convection(gwm_state* S, double p_s, unsigned long long seed, double amp, double ztop_noise) {
+  const Geo& g = S->g();
+  require(!S->grid->terrain_nonzero, "the convection sounding expects flat terrain");
+  require(S->ntr >= 3, "the convection sounding needs the warm-rain tracers");
+  HostState<R> H(S);
+  const double Rd = 287.05, kap = 0.2854, cp = Rd / kap, grav = 9.81, p0 = 1.0e5, ztr = 12000.0;
+  const double Rv = 461.5, eps = Rd / Rv;
+  auto theta_at = [&](double z, double Ttr, double Pitr) {
+    if (z <= ztr) return 300.0 + 43.0 * std::pow(z / ztr, 1.25);
+    const double Pi = Pitr * std::exp(-grav * (z - ztr) / (cp * Ttr));
+    return Ttr / Pi;
+  };
+  // Exner at 12 km by integrating dPi/dz = -g/(c_p theta)
+  const double Pis = std::pow(p_s / p0, kap);
+  auto integrate = [&](double z1, double Pi0) {
+    const int n = 20000;
+    double Pi = Pi0, zz = 0.0;
+    const double h = z1 / n;
+    for (int s = 0; s < n; ++s) {
+      const double a = 300.0 + 43.0 * std::pow(std::min(zz, ztr) / ztr, 1.25);
+      const double b = 300.0 + 43.0 * std::pow(std::min(zz + h, ztr) / ztr, 1.25);
+      Pi -= 0.5 * h * grav / cp * (1.0 / a + 1.0 / b);
+      zz += h;
+    }
+    return Pi;
+  };
+  const double Pitr = integrate(ztr, Pis);
+  const double Ttr = 343.0 * Pitr;
+  const auto& zc = S->grid->z_c;
+  std::vector<double> th(g.nz), rho(g.nz), qv(g.nz);
+  for (int k = 0; k < g.nz; ++k) {
+    const double z = zc[k];
+    const double Pi = z <= ztr ? integrate(z, Pis) : Pitr * std::exp(-grav * (z - ztr) / (cp * Ttr));
+    th[k] = theta_at(z, Ttr, Pitr);
+    const double p = p0 * std::pow(Pi, 1.0 / kap);
+    rho[k] = p / (Rd * th[k] * Pi);
+    const double T = th[k] * Pi;
+    const double rh = z <= ztr ? 1.0 - 0.75 * std::pow(z / ztr, 1.25) : 0.25;
+    const double tc = T - 273.15;
+    const double es = 6.112 * std::exp(17.67 * tc / (tc + 243.5)) * 100.0;
+    const double qs = std::min(std::max(eps * es / std::max(p - (1.0 - eps) * es, 1000.0), 0.0), 0.05);
+    qv[k] = std::min(0.014, rh * qs);
+  }
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double t = th[k] + (zc[k] < ztop_noise ? 2.0 * amp * noise(seed, i, j, k) : 0.0);
+        const R r = static_cast<R>(rho[k]);
+        H.rho[g.cell(i, j, k)] = r;
+        H.rt[g.cell(i, j, k)] = r * static_cast<R>(t);
+        H.q[0][g.cell(i, j, k)] = r * static_cast<R>(qv[k]);
+      }
+  H.upload(S);
+}
+
+template <class F>
+auto dispatch(const gwm_ctx* c, F f) {
+  if (c->prec == GWM_F64) return f(double{});
+  return f(float{});
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* gwm_last_error(void) { return g_err.c_str(); }
+
+int gwm_ctx_create(int device, int precision, gwm_ctx** out) {
+  return guard([&] {
+    require(out != nullptr, "null output");
+    require(precision == GWM_F32 || precision == GWM_F64, "precision must be GWM_F32 or GWM_F64");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    require(device >= 0 && device < n, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    auto c = std::make_unique<gwm_ctx>();
+    c->device = device;
+    c->prec = precision;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+  });
+}
+
+namespace {
+void ctx_free(gwm_ctx* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->partials) (void)hipFree(ctx->partials);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+}  // namespace
+
+int gwm_ctx_destroy(gwm_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    // Grids and states still alive keep the context (and its stream) until
+    // the last of them is destroyed.
+    ctx->closing = true;
+    if (ctx->refs <= 0) ctx_free(ctx);
+  });
+}
+
+int gwm_sync(gwm_ctx* ctx) {
+  return guard([&] {
+    require(ctx, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int gwm_ctx_precision(const gwm_ctx* ctx) { return ctx ? ctx->prec : -1; }
+
+int gwm_ctx_set_kernel_timing(gwm_ctx* ctx, int enable) {
+  return guard([&] {
+    require(ctx, "null context");
+    ctx->timing = enable != 0;
+  });
+}
+
+int gwm_ctx_kernel_stats(gwm_ctx* ctx, const char* name, long* launches, double* total_ms) {
+  return guard([&] {
+    require(ctx && name, "null argument");
+    ctx->resolve();
+    auto it = ctx->stats.find(name);
+    if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.ms;
+  });
+}
+
+int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx) {
+  return guard([&] {
+    require(ctx, "null context");
+    ctx->resolve();
+    ctx->stats.clear();
+  });
+}
+
+int gwm_grid_create(gwm_ctx* ctx, const gwm_grid_desc* d, const double* z_interfaces,
+                    const float* terrain, gwm_grid** out) {
+  return guard([&] {
+    require(ctx && d && out, "null argument");
+    require(d->nx > 0 && d->ny > 0 && d->nz > 0, "grid dimensions must be positive");
+    require(d->halo >= 1, "halo must be at least 1");
+    require(d->dx > 0 && d->dy > 0, "grid spacing must be positive");
+    require(d->halo <= d->nx && d->halo <= d->ny, "halo wider than the domain");
+    auto G = std::make_unique<gwm_grid>();
+    require(!ctx->closing, "context already destroyed");
+    G->ctx = ctx;
+    ctx->refs += 1;
+    Geo& g = G->geo;
+    g.nx = d->nx;
+    g.ny = d->ny;
+    g.nz = d->nz;
+    g.h = d->halo;
+    g.per_x = d->periodic_x ? 1 : 0;
+    g.per_y = d->periodic_y ? 1 : 0;
+    g.pxc = d->nx + 2 * d->halo;
+    g.pyc = d->ny + 2 * d->halo;
+    g.pxu = d->nx + 1 + 2 * d->halo;
+    g.pyv = d->ny + 1 + 2 * d->halo;
+    g.dx = d->dx;
+    g.dy = d->dy;
+    const int nz = d->nz;
+    G->z_if.resize(nz + 1);
+    G->z_c.resize(nz);
+    G->w_if.resize(nz + 1);
+    G->w_c.resize(nz);
+    if (!z_interfaces) {  // grid_metrics.hpp:146-161
+      require(d->z_top > 0, "z_top must be positive");
+      for (int k = 0; k <= nz; ++k) {
+        const double eta = static_cast<double>(k) / static_cast<double>(nz);
+        G->w_if[k] = taper_weight(eta, d->terrain_taper_eta);
+        G->z_if[k] = eta * d->z_top;
+        if (k < nz) {
+          const double ec = (static_cast<double>(k) + 0.5) / static_cast<double>(nz);
+          G->w_c[k] = taper_weight(ec, d->terrain_taper_eta);
+          G->z_c[k] = ec * d->z_top;
+        }
+      }
+    } else {  // stretched flat interfaces (extension)
+      const double zt = z_interfaces[nz];
+      for (int k = 0; k <= nz; ++k) {
+        if (k > 0) require(z_interfaces[k] > z_interfaces[k - 1], "z_interfaces must increase");
+        G->z_if[k] = z_interfaces[k];
+        G->w_if[k] = taper_weight(z_interfaces[k] / zt, d->terrain_taper_eta);
+      }
+      for (int k = 0; k < nz; ++k) {
+        G->z_c[k] = 0.5 * (z_interfaces[k] + z_interfaces[k + 1]);
+        G->w_c[k] = taper_weight(G->z_c[k] / zt, d->terrain_taper_eta);
+      }
+    }
+    dispatch(ctx, [&](auto r) {
+      using R = decltype(r);
+      build_grid<R>(G.get(), z_interfaces, terrain);
+      return 0;
+    });
+    *out = G.release();
+  });
+}
+
+int gwm_grid_destroy(gwm_grid* grid) {
+  return guard([&] { delete grid; });
+}
+
+int gwm_grid_terrain_active(const gwm_grid* grid) { return grid && grid->terrain_active ? 1 : 0; }
+
+int gwm_grid_download_metric(gwm_grid* G, int which, void* host) {
+  return guard([&] {
+    require(G && host, "null argument");
+    const Geo& g = G->geo;
+    const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+    const size_t es = G->ctx->esize();
+    switch (which) {
+      case 0: HIPCHK(hipMemcpy(host, G->zif_m, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost)); break;
+      case 1: HIPCHK(hipMemcpy(host, G->zc_m, n2 * g.nz * es, hipMemcpyDeviceToHost)); break;
+      case 2: HIPCHK(hipMemcpy(host, G->idz_c, n2 * g.nz * es, hipMemcpyDeviceToHost)); break;
+      case 3: HIPCHK(hipMemcpy(host, G->idz_f, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost)); break;
+      case 4: HIPCHK(hipMemcpy(host, G->terr, n2 * es, hipMemcpyDeviceToHost)); break;
+      case 5: HIPCHK(hipMemcpy(host, G->sx, n2 * es, hipMemcpyDeviceToHost)); break;
+      case 6: HIPCHK(hipMemcpy(host, G->sy, n2 * es, hipMemcpyDeviceToHost)); break;
+      case 7: std::memcpy(host, G->z_if.data(), G->z_if.size() * sizeof(double)); break;
+      case 8: std::memcpy(host, G->z_c.data(), G->z_c.size() * sizeof(double)); break;
+      case 9: std::memcpy(host, G->w_if.data(), G->w_if.size() * sizeof(double)); break;
+      case 10: std::memcpy(host, G->w_c.data(), G->w_c.size() * sizeof(double)); break;
+      default: require(false, "bad metric id");
+    }
+  });
+}
+
+// idealized_domain_builder.cpp:14-35
+int gwm_terrain_cosine_mountain(int nx, int ny, double cx, double cy, double hx, double hy,
+                                double height, float* out) {
+  return guard([&] {
+    require(nx > 0 && ny > 0 && out, "bad arguments");
+    const double kPi = 3.14159265358979323846;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        float v = 0.0f;
+        if (height > 0.0) {
+          const double x = nx > 1 ? static_cast<double>(i) / (nx - 1) : 0.0;
+          const double y = ny > 1 ? static_cast<double>(j) / (ny - 1) : 0.0;
+          const double dx = (x - cx) / hx, dy = (y - cy) / hy;
+          if (std::fabs(dx) < 1.0 && std::fabs(dy) < 1.0)
+            v = static_cast<float>(height * (0.5 * (1.0 + std::cos(kPi * dx))) *
+                                   (0.5 * (1.0 + std::cos(kPi * dy))));
+        }
+        out[static_cast<size_t>(j) * nx + i] = v;
+      }
+  });
+}
+
+int gwm_terrain_agnesi_ridge(int nx, int ny, double dx, double xc, double a, double h0, float* out) {
+  return guard([&] {
+    require(nx > 0 && ny > 0 && out && a > 0, "bad arguments");
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double x = (static_cast<double>(i) + 0.5) * dx - xc;
+        out[static_cast<size_t>(j) * nx + i] = static_cast<float>(h0 / (1.0 + (x / a) * (x / a)));
+      }
+  });
+}
+
+// dz_k = dz_b + (dz_t - dz_b) s_k^p with s_k = k/(nz-1), p chosen by
+// bisection so that the column sums to z_top exactly (rescaled at the end).
+int gwm_stretched_levels(int nz, double dzb, double dzt, double ztop, double* out) {
+  return guard([&] {
+    require(nz >= 2 && dzb > 0 && dzt > 0 && ztop > 0 && out, "bad arguments");
+    auto total = [&](double p) {
+      double s = 0.0;
+      for (int k = 0; k < nz; ++k) s += dzb + (dzt - dzb) * std::pow(static_cast<double>(k) / (nz - 1), p);
+      return s;
+    };
+    double lo = 1e-3, hi = 1e3, p = 1.0;
+    const bool grows = dzt > dzb;
+    for (int it = 0; it < 200; ++it) {
+      p = std::sqrt(lo * hi);
+      const double t = total(p);
+      // larger p -> smaller total when dz grows with height
+      if ((t > ztop) == grows) lo = p;
+      else hi = p;
+    }
+    double z = 0.0;
+    out[0] = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      z += dzb + (dzt - dzb) * std::pow(static_cast<double>(k) / (nz - 1), p);
+      out[k + 1] = z;
+    }
+    const double scale = ztop / z;
+    for (int k = 0; k <= nz; ++k) out[k] *= scale;
+    out[nz] = ztop;
+  });
+}
+
+int gwm_state_create(gwm_ctx* ctx, const gwm_grid* grid, int ntr, gwm_state** out) {
+  return guard([&] {
+    require(ctx && grid && out, "null argument");
+    require(ntr >= 0 && ntr <= gwm::kMaxTracers, "tracer count out of range");
+    require(!ctx->closing, "context already destroyed");
+    auto S = std::make_unique<gwm_state>();
+    S->ctx = ctx;
+    ctx->refs += 1;
+    S->grid = grid;
+    S->ntr = ntr;
+    S->ensure_sets(1);
+    for (int t = 0; t < ntr; ++t) S->tr.push_back(dalloc(S->g().cell_total() * ctx->esize()));
+    S->precip = dalloc(static_cast<size_t>(S->g().nx) * S->g().ny * ctx->esize());
+    for (int f = 0; f < 5; ++f) HIPCHK(hipMemset(S->set[0][f], 0, S->fsize(f) * ctx->esize()));
+    for (void* p : S->tr) HIPCHK(hipMemset(p, 0, S->g().cell_total() * ctx->esize()));
+    HIPCHK(hipMemset(S->precip, 0, static_cast<size_t>(S->g().nx) * S->g().ny * ctx->esize()));
+    S->faces_synced = false;
+    *out = S.release();
+  });
+}
+
+int gwm_state_destroy(gwm_state* s) {
+  return guard([&] {
+    if (s) (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+  });
+}
+
+long gwm_state_field_size(const gwm_state* s, int f) {
+  if (!s) return -1;
+  if (f >= GWM_TRACER0 && f - GWM_TRACER0 >= s->ntr) return -1;
+  return static_cast<long>(s->fsize(f));
+}
+
+int gwm_state_upload(gwm_state* s, int f, const void* host) {
+  return guard([&] {
+    S_(s);
+    require(host, "null host pointer");
+    void* d = s->field(f);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(d, host, s->fsize(f) * s->es(), hipMemcpyHostToDevice));
+    if (f == GWM_MOM_U || f == GWM_MOM_V) s->faces_synced = false;
+  });
+}
+
+int gwm_state_download(gwm_state* s, int f, void* host) {
+  return guard([&] {
+    S_(s);
+    require(host, "null host pointer");
+    void* d = s->field(f);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(host, d, s->fsize(f) * s->es(), hipMemcpyDeviceToHost));
+  });
+}
+
+int gwm_state_fill(gwm_state* s, int f, double v) {
+  return guard([&] {
+    S_(s);
+    void* d = s->field(f);
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      Timed t(s->ctx, "fill");
+      gwm::launch_fill<R>(s->ctx->stream, static_cast<R*>(d), s->fsize(f), static_cast<R>(v));
+      return 0;
+    });
+    if (f == GWM_MOM_U || f == GWM_MOM_V) s->faces_synced = false;
+  });
+}
+
+int gwm_state_copy(gwm_state* dst, const gwm_state* src) {
+  return guard([&] {
+    require(dst && src && dst->grid == src->grid && dst->ntr == src->ntr, "state mismatch");
+    gwm_state* s = const_cast<gwm_state*>(src);
+    for (int f = 0; f < 5; ++f)
+      HIPCHK(hipMemcpyAsync(dst->field(f), s->field(f), s->fsize(f) * s->es(),
+                            hipMemcpyDeviceToDevice, dst->ctx->stream));
+    for (int t = 0; t < s->ntr; ++t)
+      HIPCHK(hipMemcpyAsync(dst->field(GWM_TRACER0 + t), s->field(GWM_TRACER0 + t),
+                            s->fsize(GWM_RHO) * s->es(), hipMemcpyDeviceToDevice, dst->ctx->stream));
+    HIPCHK(hipMemcpyAsync(dst->precip, s->precip, s->fsize(GWM_PRECIP) * s->es(),
+                          hipMemcpyDeviceToDevice, dst->ctx->stream));
+    HIPCHK(hipStreamSynchronize(dst->ctx->stream));
+    dst->faces_synced = s->faces_synced;
+  });
+}
+
+int gwm_state_capture_boundary(gwm_state* s) {
+  return guard([&] {
+    S_(s);
+    require(!s->view, "cannot capture on a stage view");
+    for (int f = 0; f < 5; ++f) {
+      if (!s->bref[f]) s->bref[f] = s->alloc_set_field(f);
+      HIPCHK(hipMemcpyAsync(s->bref[f], s->field(f), s->fsize(f) * s->es(), hipMemcpyDeviceToDevice,
+                            s->ctx->stream));
+    }
+    while (static_cast<int>(s->bref_tr.size()) < s->ntr)
+      s->bref_tr.push_back(dalloc(s->g().cell_total() * s->es()));
+    for (int t = 0; t < s->ntr; ++t)
+      HIPCHK(hipMemcpyAsync(s->bref_tr[t], s->tr[t], s->g().cell_total() * s->es(),
+                            hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    s->have_boundary = true;
+  });
+}
+
+int gwm_halo_fill(gwm_state* s, int f) {
+  return guard([&] {
+    S_(s);
+    require(f != GWM_PRECIP, "the surface bucket has no halo");
+    void* d = s->field(f);
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      Timed t(s->ctx, "halo_fill");
+      gwm::launch_halo_fill<R>(s->ctx->stream, s->g(), orient_of(f), static_cast<R*>(d));
+      return 0;
+    });
+  });
+}
+
+int gwm_sync_interfaces(gwm_state* s, int f) {
+  return guard([&] {
+    S_(s);
+    const int o = orient_of(f);
+    if (o != 1 && o != 2) return;
+    if ((o == 1 && !s->g().per_x) || (o == 2 && !s->g().per_y)) return;
+    void* d = s->field(f);
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      Timed t(s->ctx, "sync_interfaces");
+      gwm::launch_sync_interfaces<R>(s->ctx->stream, s->g(), o, static_cast<R*>(d));
+      return 0;
+    });
+  });
+}
+
+int gwm_halo_exchange_layout(gwm_ctx* ctx, int nr, const int* desc13, int px, int py, int orient,
+                             void* const* fields) {
+  return guard([&] {
+    require(ctx && desc13 && fields && nr > 0, "null argument");
+    require(orient >= 0 && orient <= 3, "bad orientation");
+    // neighbours (cartesian_topology.cpp:57-74)
+    std::vector<int> d(static_cast<size_t>(nr) * 17);
+    auto rank_at = [&](int cx, int cy) {
+      const int rx = desc13[3], ry = desc13[4];
+      if (px) cx = ((cx % rx) + rx) % rx;
+      else if (cx < 0 || cx >= rx) return -1;
+      if (py) cy = ((cy % ry) + ry) % ry;
+      else if (cy < 0 || cy >= ry) return -1;
+      for (int n = 0; n < nr; ++n)
+        if (desc13[n * 13 + 1] == cx && desc13[n * 13 + 2] == cy) return desc13[n * 13 + 0];
+      return -1;
+    };
+    int maxe = 0;
+    for (int n = 0; n < nr; ++n) {
+      const int* D = desc13 + n * 13;
+      require(D[0] == n, "ranks must be listed in order");
+      std::copy(D, D + 13, d.begin() + n * 17);
+      d[n * 17 + 13] = rank_at(D[1] - 1, D[2]);
+      d[n * 17 + 14] = rank_at(D[1] + 1, D[2]);
+      d[n * 17 + 15] = rank_at(D[1], D[2] - 1);
+      d[n * 17 + 16] = rank_at(D[1], D[2] + 1);
+      const int h = D[8], nxl = D[10] - D[9], nyl = D[12] - D[11];
+      const int fx = nxl + (orient == 1), fy = nyl + (orient == 2), fz = D[7] + (orient == 3);
+      maxe = std::max(maxe, std::max(fz * fy * h, fz * h * (fx + 2 * h)));
+    }
+    int* dd = static_cast<int*>(dalloc(d.size() * sizeof(int)));
+    void** df = static_cast<void**>(dalloc(nr * sizeof(void*)));
+    HIPCHK(hipMemcpyAsync(dd, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(df, fields, nr * sizeof(void*), hipMemcpyHostToDevice, ctx->stream));
+    dispatch(ctx, [&](auto r) {
+      using R = decltype(r);
+      for (int phase = 0; phase < 2; ++phase) {
+        Timed t(ctx, "exchange_layout");
+        gwm::launch_exchange_layout<R>(ctx->stream, phase, orient, nr, dd,
+                                       reinterpret_cast<R* const*>(df), maxe);
+      }
+      return 0;
+    });
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(dd));
+    HIPCHK(hipFree(df));
+  });
+}
+
+int gwm_device_alloc(gwm_ctx* ctx, size_t bytes, void** out) {
+  return guard([&] {
+    require(ctx && out, "null argument");
+    *out = dalloc(bytes);
+  });
+}
+int gwm_device_free(gwm_ctx* ctx, void* p) {
+  return guard([&] {
+    require(ctx, "null context");
+    if (p) HIPCHK(hipFree(p));
+  });
+}
+int gwm_memcpy_h2d(gwm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    require(ctx, "null context");
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+int gwm_memcpy_d2h(gwm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    require(ctx, "null context");
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwm_pressure(gwm_state* s) {
+  return guard([&] {
+    S_(s);
+    void* p = s->field(GWM_PRESSURE);
+    void** fs = s->cur_set();
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      Timed t(s->ctx, "pressure");
+      gwm::launch_pressure<R>(s->ctx->stream, s->g(), static_cast<const R*>(fs[0]),
+                              static_cast<const R*>(fs[1]), static_cast<R*>(p));
+      return 0;
+    });
+  });
+}
+
+int gwm_slow_tendencies(gwm_state* s, const gwm_step_config* c) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    require(!s->view, "operator not available on a stage view");
+    dispatch(s->ctx, [&](auto r) {
+      slow_tendencies<decltype(r)>(s, *c);
+      return 0;
+    });
+  });
+}
+
+int gwm_fast_modes(gwm_state* s, const gwm_step_config* c) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    require(!s->view, "operator not available on a stage view");
+    dispatch(s->ctx, [&](auto r) {
+      fast_modes<decltype(r)>(s, *c);
+      return 0;
+    });
+  });
+}
+
+int gwm_ssprk3_step(gwm_state* s, const gwm_step_config* c, const gwm_boundary_config* b) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    require(!s->view, "operator not available on a stage view");
+    dispatch(s->ctx, [&](auto r) {
+      ssprk3<decltype(r)>(s, *c, b);
+      return 0;
+    });
+  });
+}
+
+int gwm_tracers_ssprk3(gwm_state* s, const gwm_step_config* c, int bmode) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    require(!s->view, "operator not available on a stage view");
+    dispatch(s->ctx, [&](auto r) {
+      tracers_ssprk3<decltype(r)>(s, *c, bmode);
+      return 0;
+    });
+  });
+}
+
+int gwm_warm_rain(gwm_state* s, const gwm_warm_rain_config* c, int metric) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    dispatch(s->ctx, [&](auto r) {
+      warm_rain<decltype(r)>(s, *c, metric);
+      return 0;
+    });
+  });
+}
+
+int gwm_apply_boundary(gwm_state* s, int mode, double width) {
+  return guard([&] {
+    S_(s);
+    gwm_state* owner = s->view ? s->owner : s;
+    require(owner->have_boundary, "boundary reference not captured");
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      void** fs = s->cur_set();
+      gwm_state* o = owner;
+      const Geo& g = s->g();
+      if (mode == GWM_BOUNDARY_NONE) return 0;
+      for (int f = 0; f < 5; ++f) {
+        Timed t(s->ctx, "boundary");
+        gwm::launch_boundary<R>(s->ctx->stream, g, orient_of(f), static_cast<R*>(fs[f]),
+                                static_cast<const R*>(o->bref[f]), mode, width);
+      }
+      if (mode == GWM_BOUNDARY_STRIPS)
+        for (int t = 0; t < o->ntr; ++t) {
+          Timed tt(s->ctx, "boundary");
+          gwm::launch_boundary<R>(s->ctx->stream, g, 0, static_cast<R*>(o->tr[t]),
+                                  static_cast<const R*>(o->bref_tr[t]), mode, width);
+        }
+      return 0;
+    });
+    owner->faces_synced = false;
+  });
+}
+
+int gwm_moist_step(gwm_state* s, const gwm_step_config* c, const gwm_warm_rain_config* wr,
+                   const gwm_boundary_config* b) {
+  return guard([&] {
+    S_(s);
+    require(c && wr, "null config");
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      ssprk3<R>(s, *c, b);
+      tracers_ssprk3<R>(s, *c, b ? (b->mode == GWM_BOUNDARY_STRIPS ? GWM_BOUNDARY_STRIPS : GWM_BOUNDARY_NONE) : GWM_BOUNDARY_NONE);
+      warm_rain<R>(s, *wr, 1);
+      return 0;
+    });
+  });
+}
+
+int gwm_budget_sums(gwm_state* s, double* out) {
+  return guard([&] {
+    S_(s);
+    require(out, "null output");
+    dispatch(s->ctx, [&](auto r) {
+      budget<decltype(r)>(s, out);
+      return 0;
+    });
+  });
+}
+
+int gwm_init_constant(gwm_state* s, double rho, double th, double u, double v, double w) {
+  return guard([&] {
+    S_(s);
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      HostState<R> H(s);
+      fill_constant<R>(s, H, static_cast<R>(rho), static_cast<R>(th), static_cast<R>(u),
+                       static_cast<R>(v), static_cast<R>(w));
+      H.q.clear();
+      std::vector<R>* f[5] = {&H.rho, &H.rt, &H.u, &H.v, &H.w};
+      for (int a = 0; a < 5; ++a)
+        HIPCHK(hipMemcpy(s->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R), hipMemcpyHostToDevice));
+      s->faces_synced = false;
+      return 0;
+    });
+  });
+}
+
+int gwm_init_hydrostatic_rest(gwm_state* s, double rho_s, double th) {
+  return guard([&] {
+    S_(s);
+    dispatch(s->ctx, [&](auto r) {
+      using R = decltype(r);
+      HostState<R> H(s);
+      init_hydro<R>(s, rho_s, th, H);
+      H.upload(s);
+      return 0;
+    });
+  });
+}
+
+int gwm_init_bubble(gwm_state* s, const double* p, int sign) {
+  return guard([&] {
+    S_(s);
+    require(p, "null parameters");
+    dispatch(s->ctx, [&](auto r) {
+      init_bubble<decltype(r)>(s, p, sign);
+      return 0;
+    });
+  });
+}
+
+int gwm_init_mountain_wave(gwm_state* s, double rho_s, double th, double u, double v, double w) {
+  return guard([&] {
+    S_(s);
+    dispatch(s->ctx, [&](auto r) {
+      init_mountain<decltype(r)>(s, rho_s, th, u, v, w);
+      return 0;
+    });
+  });
+}
+
+int gwm_init_constant_n(gwm_state* s, double th_s, double N, double p_s, double u, double v) {
+  return guard([&] {
+    S_(s);
+    require(N > 0 && th_s > 0 && p_s > 0, "bad sounding parameters");
+    dispatch(s->ctx, [&](auto r) {
+      init_constant_n<decltype(r)>(s, th_s, N, p_s, u, v);
+      return 0;
+    });
+  });
+}
+
+int gwm_init_convection(gwm_state* s, double p_s, unsigned long long seed, double amp, double ztop) {
+  return guard([&] {
+    S_(s);
+    dispatch(s->ctx, [&](auto r) {
+      init_convection<decltype(r)>(s, p_s, seed, amp, ztop);
+      return 0;
+    });
+  });
+}
+
+}  // extern "C"

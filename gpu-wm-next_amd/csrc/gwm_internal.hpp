@@ -1,0 +1,149 @@
+// gwm_internal.hpp -- shared device/host definitions of the gfx950 path.
+//
+// Storage layout (identical to the reference's Field3D, field3d.hpp:49-54):
+//   cell   (nx  , ny  , nz  ) idx = (k*(ny+2h)   + j+h)*(nx+2h)   + i+h
+//   x-face (nx+1, ny  , nz  ) idx = (k*(ny+2h)   + j+h)*(nx+1+2h) + i+h
+//   y-face (nx  , ny+1, nz  ) idx = (k*(ny+1+2h) + j+h)*(nx+2h)   + i+h
+//   z-face (nx  , ny  , nz+1) idx as cell, nz+1 levels
+// Kernels never rely on halo storage: a neighbour outside the owned range is
+// read through the 1x1 halo map (periodic wrap, or zero across an open side),
+// which is the value the reference's exchange would have placed there
+// (src/comm/halo_exchange.cpp:32-116, 313-373).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace gwm {
+
+struct Geo {
+  int nx, ny, nz, h;
+  int per_x, per_y;
+  int pxc, pyc;   // cell pitches nx+2h, ny+2h
+  int pxu;        // x-face row pitch nx+1+2h
+  int pyv;        // y-face plane pitch ny+1+2h
+  double dx, dy;
+
+  __host__ __device__ size_t cell(int i, int j, int k) const {
+    return (static_cast<size_t>(k) * pyc + (j + h)) * pxc + (i + h);
+  }
+  __host__ __device__ size_t xf(int i, int j, int k) const {
+    return (static_cast<size_t>(k) * pyc + (j + h)) * pxu + (i + h);
+  }
+  __host__ __device__ size_t yf(int i, int j, int k) const {
+    return (static_cast<size_t>(k) * pyv + (j + h)) * pxc + (i + h);
+  }
+  __host__ __device__ size_t zf(int i, int j, int k) const { return cell(i, j, k); }
+  __host__ __device__ size_t metric(int i, int j, int k) const {
+    return (static_cast<size_t>(k) * ny + j) * nx + i;
+  }
+  __host__ __device__ size_t cell_total() const { return static_cast<size_t>(pxc) * pyc * nz; }
+  __host__ __device__ size_t xf_total() const { return static_cast<size_t>(pxu) * pyc * nz; }
+  __host__ __device__ size_t yf_total() const { return static_cast<size_t>(pxc) * pyv * nz; }
+  __host__ __device__ size_t zf_total() const { return static_cast<size_t>(pxc) * pyc * (nz + 1); }
+};
+
+// Halo map for a cell coordinate along an axis of n owned cells: returns the
+// owned source index, or -1 when the value is zero (open side).
+__host__ __device__ inline int map_cell(int i, int n, int periodic) {
+  if (i >= 0 && i < n) return i;
+  if (!periodic) return -1;
+  int m = i % n;
+  return m < 0 ? m + n : m;
+}
+// Along the face-normal axis of a face field (n+1 owned faces, faces 0 and n
+// coincide under periodicity): ghost -1 <- face n-1, ghost n+1 <- face 1.
+__host__ __device__ inline int map_face(int i, int n, int periodic) {
+  if (i >= 0 && i <= n) return i;
+  if (!periodic) return -1;
+  return i < 0 ? i + n : i - n;
+}
+
+// ---------------------------------------------------------------------------
+// Math wrappers so one template serves FP32 and FP64.
+// ---------------------------------------------------------------------------
+__device__ __host__ inline float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __host__ inline double rmax(double a, double b) { return fmax(a, b); }
+__device__ __host__ inline float rmin(float a, float b) { return fminf(a, b); }
+__device__ __host__ inline double rmin(double a, double b) { return fmin(a, b); }
+__device__ inline float rpow(float a, float b) { return powf(a, b); }
+__device__ inline double rpow(double a, double b) { return pow(a, b); }
+__device__ inline float rexp(float a) { return expf(a); }
+__device__ inline double rexp(double a) { return exp(a); }
+
+// Physical constants in the working precision (dry_thermo.hpp:17-19,
+// moist_thermo.hpp:18-21).  FP32 uses the reference's float literals.
+template <class R>
+struct Phys;
+template <>
+struct Phys<float> {
+  static constexpr float Rd = 287.05f, p0 = 100000.0f, kappa = 0.2854f;
+  static constexpr float Rv = 461.5f, Lv = 2.5e6f, eps = 287.05f / 461.5f;
+  static constexpr float mom_floor = -1.0e30f;
+};
+template <>
+struct Phys<double> {
+  static constexpr double Rd = 287.05, p0 = 100000.0, kappa = 0.2854;
+  static constexpr double Rv = 461.5, Lv = 2.5e6, eps = 287.05 / 461.5;
+  static constexpr double mom_floor = -1.0e30;
+};
+
+template <class R>
+__device__ inline R d_theta(R rho, R rt) {  // dry_thermo.hpp:21-24
+  return rt / rmax(rho, R(1.0e-6));
+}
+template <class R>
+__device__ inline R d_pressure(R rho, R rt) {  // dry_thermo.hpp:26-33
+  const R th = d_theta(rho, rt);
+  const R ratio = rmax(rho * Phys<R>::Rd * th / Phys<R>::p0, R(1.0e-6));
+  return Phys<R>::p0 * rpow(ratio, R(1) / (R(1) - Phys<R>::kappa));
+}
+template <class R>
+__device__ inline R d_rho_from_p_theta(R p, R th) {  // dry_thermo.hpp:35-40
+  const R ex = rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
+  return p / rmax(Phys<R>::Rd * th * ex, R(1.0e-6));
+}
+template <class R>
+__device__ inline R d_cp() {
+  return Phys<R>::Rd / Phys<R>::kappa;
+}
+template <class R>
+__device__ inline R d_qsat(R T, R p) {  // moist_thermo.hpp:23-36
+  const R tc = T - R(273.15);
+  const R es = R(6.112) * rexp((R(17.67) * tc) / rmax(tc + R(243.5), R(1.0e-6))) * R(100);
+  const R denom = rmax(p - (R(1) - Phys<R>::eps) * es, R(1000));
+  return rmin(rmax(Phys<R>::eps * es / denom, R(0)), R(0.05));
+}
+
+// ---------------------------------------------------------------------------
+// Device state (pointers into device memory, context precision).
+// ---------------------------------------------------------------------------
+template <class R>
+struct DState {
+  R* rho;
+  R* rt;
+  R* u;
+  R* v;
+  R* w;
+};
+
+constexpr int kMaxTracers = 8;
+
+// Metric arrays on the device (global, no halo, (k*ny+j)*nx+i) plus the
+// per-level columns used on flat terrain.
+template <class R>
+struct DMetrics {
+  const R* zc;     // z_centers_metric  nx*ny*nz
+  const R* idz_c;  // inv_dz_cell       nx*ny*nz
+  const R* idz_f;  // inv_dz_face       nx*ny*(nz+1)
+  const R* lev_idz_c;  // nz   (flat columns: identical in every column)
+  const R* lev_idz_f;  // nz+1
+  const R* lev_zc;     // nz
+};
+
+// Kernel launch record for optional per-kernel event timing.
+struct LaunchTimer;
+
+}  // namespace gwm

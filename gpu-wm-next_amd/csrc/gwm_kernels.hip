@@ -1,0 +1,1062 @@
+// gwm_kernels.hip -- gfx950 kernels of the dry + warm-rain time step.
+//
+// Each kernel states the reference operator(s) it covers.  Semantics are the
+// reference's 1x1 layout: a neighbour outside the owned range is the value
+// the reference's halo exchange places there (map_cell / map_face), so no
+// kernel depends on halo storage.  Periodic wrap faces (face 0 and face n of
+// a face field) are kept equal as a state invariant, which makes the
+// reference's owned-interface averaging (halo_exchange.cpp:217-285) an
+// identity inside the step; gwm_sync_interfaces() establishes it on entry.
+#include "gwm_kernels.hpp"
+
+namespace gwm {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Mapped loads (1x1 halo semantics).
+// ---------------------------------------------------------------------------
+template <class R>
+struct Loader {
+  Geo g;
+  __device__ R cell(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.cell(si, sj, k)];
+  }
+  __device__ R xf(const R* f, int i, int j, int k) const {
+    const int si = map_face(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.xf(si, sj, k)];
+  }
+  __device__ R yf(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_face(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.yf(si, sj, k)];
+  }
+  __device__ R zf(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.zf(si, sj, k)];
+  }
+  // Metric column index with the reference's periodic wrap
+  // (dry_fast_modes.cu:24-30); callers only pass in-range columns when the
+  // axis is open.
+  __device__ int mi(int i) const { return g.per_x ? map_cell(i, g.nx, 1) : i; }
+  __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
+};
+
+template <class R>
+__device__ inline R upw(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
+  return a >= R(0) ? a * ql : a * qr;
+}
+
+// dry_pressure_gradient.cu:37-73, with p supplied by a functor.
+template <class R, class PF>
+__device__ inline R vgrad(const PF& P, const R* zc, const Geo& g, int gi, int gj, int k, int li,
+                          int lj) {
+  if (g.nz == 1) return R(0);
+  if (k == 0) {
+    const R dz = zc[g.metric(gi, gj, 1)] - zc[g.metric(gi, gj, 0)];
+    return (P(li, lj, 1) - P(li, lj, 0)) / rmax(dz, R(1.0e-6));
+  }
+  if (k + 1 == g.nz) {
+    const R dz = zc[g.metric(gi, gj, k)] - zc[g.metric(gi, gj, k - 1)];
+    return (P(li, lj, k) - P(li, lj, k - 1)) / rmax(dz, R(1.0e-6));
+  }
+  const R dz = zc[g.metric(gi, gj, k + 1)] - zc[g.metric(gi, gj, k - 1)];
+  return (P(li, lj, k + 1) - P(li, lj, k - 1)) / rmax(dz, R(1.0e-6));
+}
+
+// Deterministic block sum of a double (256 threads).
+__device__ inline double block_sum(double v, double* sh) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int n = blockDim.x * blockDim.y * blockDim.z;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = n / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Level sums (deterministic).  mode 0: theta = rho_theta/max(rho,1e-6) of a
+// state (dry_core.cu:462-490); mode 1: rho and p(rho, rho*theta_fixed) of the
+// fast subcycle (dry_fast_modes.cu:42-67, FP32 atomics replaced by FP64
+// partials in fixed order).  Block = 256 threads over a chunk of rows of one
+// level; partials[(k*nchunk + c)*2 + {0,1}].
+// ===========================================================================
+template <class R>
+__global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, const R* aux,
+                                                         int mode, int rows_per_chunk,
+                                                         double* partials) {
+  __shared__ double sh[256];
+  const int k = blockIdx.y;
+  const int c = blockIdx.x;
+  const int nchunk = gridDim.x;
+  const int j0 = c * rows_per_chunk;
+  const int j1 = min(g.ny, j0 + rows_per_chunk);
+  const int t = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (int j = j0; j < j1; ++j)
+    for (int i = t; i < g.nx; i += 256) {
+      const size_t c0 = g.cell(i, j, k);
+      const R r = rho[c0];
+      if (mode == 0) {
+        s0 += static_cast<double>(aux[c0] / rmax(r, R(1.0e-6)));
+      } else {
+        s0 += static_cast<double>(r);
+        s1 += static_cast<double>(d_pressure(r, r * aux[c0]));
+      }
+    }
+  s0 = block_sum(s0, sh);
+  s1 = block_sum(s1, sh);
+  if (t == 0) {
+    partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 0] = s0;
+    partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 1] = s1;
+  }
+}
+
+template <class R>
+__global__ void k_level_finalize(int nz, int nchunk, const double* partials, int mode,
+                                 double count, R* out0, R* out1) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    s0 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 0];
+    s1 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 1];
+  }
+  if (mode == 0) {
+    out0[k] = static_cast<R>(s0 / count);  // dry_core.cu:485-490
+  } else {
+    const R inv = R(1) / static_cast<R>(count);  // dry_fast_modes.cu:481-482
+    out0[k] = static_cast<R>(s0) * inv;
+    out1[k] = static_cast<R>(s1) * inv;
+  }
+}
+
+// ===========================================================================
+// Pressure over the full storage (halo included) of the halo-filled state:
+// compute_dry_pressure_fields (dry_pressure_gradient.cu:177-202, K20).
+// ===========================================================================
+template <class R>
+__global__ void k_pressure(Geo g, const R* rho, const R* rt, R* p) {
+  const size_t n = g.cell_total();
+  const size_t a = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int pi = static_cast<int>(a % g.pxc);
+  const int pj = static_cast<int>((a / g.pxc) % g.pyc);
+  const int k = static_cast<int>(a / (static_cast<size_t>(g.pxc) * g.pyc));
+  Loader<R> L{g};
+  const R r = L.cell(rho, pi - g.h, pj - g.h, k);
+  const R t = L.cell(rt, pi - g.h, pj - g.h, k);
+  p[a] = d_pressure(r, t);
+}
+
+// ===========================================================================
+// Slow stage: compute_slow_tendencies (dry_core.cu:400-539) = K2 + K16-K19 +
+// K21-K22 + K3, fused with the stage update / SSP blend (dry_core.cu:572-646,
+// K4/K5).  One thread per cell (i,j,k); it also owns x-face i, y-face j and
+// z-face k, plus face nx / ny / nz on the last index.
+//   mode 0: write tendencies;  mode 1: out = max(floor, S + dt*T);
+//   mode 2: out = wa*Q0 + wb*max(floor, S + dt*T).
+// With out_theta, the cell output is theta = (rho theta)_new / max(rho_new,
+// 1e-6) (the fast-mode entry value, dry_fast_modes.cu:86-95) instead of
+// rho theta.
+// ===========================================================================
+template <class R, bool TERRAIN>
+__global__ __launch_bounds__(256) void k_slow_stage(SlowArgs<R> a) {
+  const Geo g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const Loader<R> L{g};
+  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const DState<R> s = a.s;
+  const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
+
+  auto RHO = [&](int ii, int jj, int kk) { return L.cell(s.rho, ii, jj, kk); };
+  auto RT = [&](int ii, int jj, int kk) { return L.cell(s.rt, ii, jj, kk); };
+  auto TH = [&](int ii, int jj, int kk) { return d_theta(RHO(ii, jj, kk), RT(ii, jj, kk)); };
+  auto P = [&](int ii, int jj, int kk) { return d_pressure(RHO(ii, jj, kk), RT(ii, jj, kk)); };
+  auto U = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
+  auto V = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
+  auto W = [&](int ii, int jj, int kk) { return L.zf(s.w, ii, jj, kk); };
+  // K16 cell velocities (dry_momentum_flux.cu:36-73)
+  auto UC = [&](int ii, int jj, int kk) {
+    return R(0.5) * (U(ii, jj, kk) + U(ii + 1, jj, kk)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
+  };
+  auto VC = [&](int ii, int jj, int kk) {
+    return R(0.5) * (V(ii, jj, kk) + V(ii, jj + 1, kk)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
+  };
+  auto WC = [&](int ii, int jj, int kk) {
+    return R(0.5) * (W(ii, jj, kk) + W(ii, jj, kk + 1)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
+  };
+  auto IDZC = [&](int ii, int jj, int kk) -> R {
+    if (TERRAIN) return a.m.idz_c[g.metric(L.mi(ii), L.mj(jj), kk)];
+    return a.m.lev_idz_c[kk];
+  };
+  auto IDZF = [&](int ii, int jj, int kk) -> R {
+    if (TERRAIN) return a.m.idz_f[g.metric(L.mi(ii), L.mj(jj), kk)];
+    return a.m.lev_idz_f[kk];
+  };
+
+  // ---- K2: continuity and upwind theta flux at cell (i,j,k) (dry_core.cu:62-144)
+  R t_rho, t_rt;
+  {
+    const R mw = U(i, j, k), me = U(i + 1, j, k);
+    const R ms = V(i, j, k), mn = V(i, j + 1, k);
+    const R mb = W(i, j, k), mt = W(i, j, k + 1);
+    const R idz = IDZC(i, j, k);
+    const R div = (me - mw) / dx + (mn - ms) / dy + (mt - mb) * idz;
+    t_rho = -div;
+    const R thc = TH(i, j, k);
+    const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
+    const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
+    const R thb = k > 0 ? TH(i, j, k - 1) : thc;
+    const R tht = k + 1 < g.nz ? TH(i, j, k + 1) : thc;
+    const R fw = mw * (mw >= R(0) ? thw : thc);
+    const R fe = me * (me >= R(0) ? thc : the);
+    const R fs = ms * (ms >= R(0) ? ths : thc);
+    const R fn = mn * (mn >= R(0) ? thc : thn);
+    const R fb = mb * (mb >= R(0) ? thb : thc);
+    const R ft = mt * (mt >= R(0) ? thc : tht);
+    t_rt = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+  }
+
+  // ---- x-face tendency at face fi (K17 + K21) ----
+  auto tend_u = [&](int fi) -> R {
+    if ((fi == 0 && ow) || (fi == g.nx && oe)) return R(0);
+    const R qw = U(fi - 1, j, k), qc = U(fi, j, k), qe = U(fi + 1, j, k);
+    const R qs = U(fi, j - 1, k), qn = U(fi, j + 1, k);
+    const R qb = k > 0 ? U(fi, j, k - 1) : qc;
+    const R qt = k + 1 < g.nz ? U(fi, j, k + 1) : qc;
+    const R fw = upw(UC(fi - 1, j, k), qw, qc);
+    const R fe = upw(UC(fi, j, k), qc, qe);
+    R fs = 0, fn = 0, fb = 0, ft = 0;
+    if (!(j == 0 && os)) fs = upw(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
+    if (!(j + 1 == g.ny && on)) fn = upw(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
+    if (k > 0) fb = upw(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
+    if (k + 1 < g.nz) ft = upw(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
+    const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
+    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+    // K21 (dry_pressure_gradient.cu:75-123)
+    const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
+    R term;
+    if (TERRAIN) {
+      const int gj = L.mj(j), gl = L.mi(fi - 1), gr = L.mi(fi);
+      const R zl = a.m.zc[g.metric(gl, gj, k)], zr = a.m.zc[g.metric(gr, gj, k)];
+      const R dzdx = (zr - zl) / dx;
+      const R dl = vgrad<R>(P, a.m.zc, g, gl, gj, k, fi - 1, j);
+      const R drr = vgrad<R>(P, a.m.zc, g, gr, gj, k, fi, j);
+      term = (pr - pl) / dx - dzdx * (R(0.5) * (dl + drr));
+    } else {
+      term = (pr - pl) / dx;  // dzdx == 0 exactly on flat terrain
+    }
+    return A + (-term);
+  };
+  // ---- y-face tendency at face fj (K18 + K22) ----
+  auto tend_v = [&](int fj) -> R {
+    if ((fj == 0 && os) || (fj == g.ny && on)) return R(0);
+    const R qw = V(i - 1, fj, k), qc = V(i, fj, k), qe = V(i + 1, fj, k);
+    const R qs = V(i, fj - 1, k), qn = V(i, fj + 1, k);
+    const R qb = k > 0 ? V(i, fj, k - 1) : qc;
+    const R qt = k + 1 < g.nz ? V(i, fj, k + 1) : qc;
+    R fw = 0, fe = 0, fb = 0, ft = 0;
+    if (!(i == 0 && ow)) fw = upw(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
+    if (!(i + 1 == g.nx && oe)) fe = upw(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
+    const R fs = upw(VC(i, fj - 1, k), qs, qc);
+    const R fn = upw(VC(i, fj, k), qc, qn);
+    if (k > 0) fb = upw(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
+    if (k + 1 < g.nz) ft = upw(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
+    const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
+    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+    const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
+    R term;
+    if (TERRAIN) {
+      const int gi = L.mi(i), gs = L.mj(fj - 1), gn = L.mj(fj);
+      const R zs = a.m.zc[g.metric(gi, gs, k)], zn = a.m.zc[g.metric(gi, gn, k)];
+      const R dzdy = (zn - zs) / dy;
+      const R ds = vgrad<R>(P, a.m.zc, g, gi, gs, k, i, fj - 1);
+      const R dn = vgrad<R>(P, a.m.zc, g, gi, gn, k, i, fj);
+      term = (pn - ps) / dy - dzdy * (R(0.5) * (ds + dn));
+    } else {
+      term = (pn - ps) / dy;
+    }
+    return A + (-term);
+  };
+  // ---- z-face tendency at interior face kf (K19 + K3) ----
+  auto tend_w = [&](int kf) -> R {
+    if (kf == 0 || kf == g.nz) return R(0);
+    const int kl = kf - 1, ku = kf;
+    const R qw = W(i - 1, j, kf), qc = W(i, j, kf), qe = W(i + 1, j, kf);
+    const R qs = W(i, j - 1, kf), qn = W(i, j + 1, kf);
+    const R qb = W(i, j, kf - 1), qt = W(i, j, kf + 1);
+    R fw = 0, fe = 0, fs = 0, fn = 0;
+    if (!(i == 0 && ow)) fw = upw(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
+    if (!(i + 1 == g.nx && oe)) fe = upw(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
+    if (!(j == 0 && os)) fs = upw(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
+    if (!(j + 1 == g.ny && on)) fn = upw(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
+    const R fb = upw(WC(i, j, kl), qb, qc);
+    const R ft = upw(WC(i, j, ku), qc, qt);
+    const R idz = IDZF(i, j, kf);
+    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+    // K3 buoyancy (dry_core.cu:146-187)
+    const R rl = RHO(i, j, kl), ru = RHO(i, j, ku);
+    const R tl = RT(i, j, kl) / rmax(rl, R(1.0e-6));
+    const R tu = RT(i, j, ku) / rmax(ru, R(1.0e-6));
+    const R tf = R(0.5) * (tl + tu);
+    const R trf = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
+    const R rf = R(0.5) * (rl + ru);
+    const R b = a.gravity * (tf - trf) / rmax(trf, R(1.0e-6));
+    return A + rf * b;
+  };
+
+  const R dt = a.dt;
+  auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
+  auto fin = [&](R q0v, R v) { return a.mode == 2 ? a.wa * q0v + a.wb * v : v; };
+
+  const size_t ci = g.cell(i, j, k);
+  const R tu0 = tend_u(i), tv0 = tend_v(j), tw0 = tend_w(k);
+  if (a.mode == 0) {
+    a.out.rho[ci] = t_rho;
+    a.out.rt[ci] = t_rt;
+    a.out.u[g.xf(i, j, k)] = tu0;
+    a.out.v[g.yf(i, j, k)] = tv0;
+    a.out.w[g.zf(i, j, k)] = tw0;
+    if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx);
+    if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny);
+    if (k == g.nz - 1) a.out.w[g.zf(i, j, g.nz)] = R(0);
+    return;
+  }
+  const R fl = R(1.0e-6), mf = Phys<R>::mom_floor;
+  const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(s.rho[ci], t_rho, fl));
+  const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(s.rt[ci], t_rt, fl));
+  a.out.rho[ci] = rho_new;
+  a.out.rt[ci] = a.out_theta ? rt_new / rmax(rho_new, R(1.0e-6)) : rt_new;
+  {
+    const size_t f = g.xf(i, j, k);
+    a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tu0, mf));
+  }
+  {
+    const size_t f = g.yf(i, j, k);
+    a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tv0, mf));
+  }
+  {
+    const size_t f = g.zf(i, j, k);
+    a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], tw0, mf));
+  }
+  if (i == g.nx - 1) {
+    const size_t f = g.xf(g.nx, j, k);
+    a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx), mf));
+  }
+  if (j == g.ny - 1) {
+    const size_t f = g.yf(i, g.ny, k);
+    a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny), mf));
+  }
+  if (k == g.nz - 1) {
+    const size_t f = g.zf(i, j, g.nz);
+    a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], R(0), mf));
+  }
+}
+
+// ===========================================================================
+// Column hydrostatic reference (K11, dry_fast_modes.cu:108-158), terrain
+// path.  One thread per column; writes rho_ref / p_ref as (k*ny+j)*nx+i.
+// ===========================================================================
+template <class R>
+__global__ void k_column_reference(Geo g, const R* rho, const R* theta, const R* zc, R gravity,
+                                   R* rref, R* pref) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  // K10 then K20 at level 0: rt_fast = rho*theta
+  R r0 = rho[g.cell(i, j, 0)];
+  R rt0 = r0 * theta[g.cell(i, j, 0)];
+  const R th0 = d_theta(r0, rt0);
+  R pk = d_pressure(r0, rt0);
+  R rk = d_rho_from_p_theta(pk, th0);
+  pref[g.metric(i, j, 0)] = pk;
+  rref[g.metric(i, j, 0)] = rk;
+  for (int k = 1; k < g.nz; ++k) {
+    const R r = rho[g.cell(i, j, k)];
+    const R thk = d_theta(r, r * theta[g.cell(i, j, k)]);
+    const R dz = zc[g.metric(i, j, k)] - zc[g.metric(i, j, k - 1)];
+    R pn = rmax(R(1), pk - gravity * rk * dz);
+    for (int it = 0; it < 4; ++it) {
+      const R rn = d_rho_from_p_theta(pn, thk);
+      pn = pk - gravity * R(0.5) * (rk + rn) * dz;
+      pn = rmax(pn, R(1));
+    }
+    pk = pn;
+    rk = d_rho_from_p_theta(pn, thk);
+    pref[g.metric(i, j, k)] = pk;
+    rref[g.metric(i, j, k)] = rk;
+  }
+}
+
+// ===========================================================================
+// Fast substep: one substep of apply_local_split_explicit_fast_modes
+// (dry_fast_modes.cu:495-652): K10+K20 pressure from rho and the fixed theta,
+// K12/K13 horizontal pressure gradient, K14 vertical perturbation branch,
+// face synchronisation (identity under the wrap-face invariant), K15 density
+// from the divergence of (m - m_base), K10 rho*theta rebuild (last substep).
+// One thread per cell; u/v/w of the cell's upper faces are recomputed
+// redundantly so that rho needs no second pass.  In -> out ping-pong.
+// ===========================================================================
+template <class R, bool TERRAIN>
+__global__ __launch_bounds__(256) void k_fast_substep(FastArgs<R> a) {
+  const Geo g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const Loader<R> L{g};
+  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R dtf = a.dtf, grav = a.gravity;
+  const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
+
+  auto P = [&](int ii, int jj, int kk) -> R {
+    const R r = L.cell(a.rho, ii, jj, kk);
+    return d_pressure(r, r * L.cell(a.theta, ii, jj, kk));
+  };
+  const R pc = P(i, j, k);
+  auto pgf_x = [&](int fi, R pl, R pr) -> R {
+    if (!TERRAIN) return (pr - pl) / dx;
+    const int gj = L.mj(j), gl = L.mi(fi - 1), gr = L.mi(fi);
+    const R zl = a.m.zc[g.metric(gl, gj, k)], zr = a.m.zc[g.metric(gr, gj, k)];
+    const R dzdx = (zr - zl) / dx;
+    const R dl = vgrad<R>(P, a.m.zc, g, gl, gj, k, fi - 1, j);
+    const R drr = vgrad<R>(P, a.m.zc, g, gr, gj, k, fi, j);
+    return (pr - pl) / dx - dzdx * (R(0.5) * (dl + drr));
+  };
+  auto pgf_y = [&](int fj, R ps, R pn) -> R {
+    if (!TERRAIN) return (pn - ps) / dy;
+    const int gi = L.mi(i), gs = L.mj(fj - 1), gn = L.mj(fj);
+    const R zs = a.m.zc[g.metric(gi, gs, k)], zn = a.m.zc[g.metric(gi, gn, k)];
+    const R dzdy = (zn - zs) / dy;
+    const R ds = vgrad<R>(P, a.m.zc, g, gi, gs, k, i, fj - 1);
+    const R dn = vgrad<R>(P, a.m.zc, g, gi, gn, k, i, fj);
+    return (pn - ps) / dy - dzdy * (R(0.5) * (ds + dn));
+  };
+  auto new_u = [&](int fi, R pl, R pr) -> R {
+    const R cur = a.u[g.xf(fi, j, k)];
+    if (!a.horizontal || (fi == 0 && ow) || (fi == g.nx && oe)) return cur;
+    return cur + (-dtf * pgf_x(fi, pl, pr));
+  };
+  auto new_v = [&](int fj, R ps, R pn) -> R {
+    const R cur = a.v[g.yf(i, fj, k)];
+    if (!a.horizontal || (fj == 0 && os) || (fj == g.ny && on)) return cur;
+    return cur + (-dtf * pgf_y(fj, ps, pn));
+  };
+  auto RREF = [&](int kk) -> R { return a.col_ref ? a.rref3[g.metric(i, j, kk)] : a.rref_lev[kk]; };
+  auto PREF = [&](int kk) -> R { return a.col_ref ? a.pref3[g.metric(i, j, kk)] : a.pref_lev[kk]; };
+  auto new_w = [&](int kf, R plo, R pup) -> R {
+    const R cur = a.w[g.zf(i, j, kf)];
+    if (kf == 0 || kf == g.nz) return cur;
+    const R ppl = plo - PREF(kf - 1);
+    const R ppu = pup - PREF(kf);
+    const R rpf = R(0.5) * ((a.rho[g.cell(i, j, kf - 1)] - RREF(kf - 1)) +
+                            (a.rho[g.cell(i, j, kf)] - RREF(kf)));
+    const R idz = TERRAIN ? a.m.idz_f[g.metric(i, j, kf)] : a.m.lev_idz_f[kf];
+    const R tend = -(ppu - ppl) * idz - grav * rpf;
+    return cur + dtf * tend;
+  };
+
+  const R pw = P(i - 1, j, k), pe = P(i + 1, j, k);
+  const R ps = P(i, j - 1, k), pn = P(i, j + 1, k);
+  const R u0 = new_u(i, pw, pc), u1 = new_u(i + 1, pc, pe);
+  const R v0 = new_v(j, ps, pc), v1 = new_v(j + 1, pc, pn);
+  const R pb = k > 0 ? P(i, j, k - 1) : R(0);
+  const R pt = k + 1 < g.nz ? P(i, j, k + 1) : R(0);
+  const R w0 = new_w(k, pb, pc), w1 = new_w(k + 1, pc, pt);
+
+  const size_t ci = g.cell(i, j, k);
+  R rho_new = a.rho[ci];
+  if (a.density) {
+    const R idz = TERRAIN ? a.m.idz_c[g.metric(i, j, k)] : a.m.lev_idz_c[k];
+    const R div = ((u1 - a.ub[g.xf(i + 1, j, k)]) - (u0 - a.ub[g.xf(i, j, k)])) / dx +
+                  ((v1 - a.vb[g.yf(i, j + 1, k)]) - (v0 - a.vb[g.yf(i, j, k)])) / dy +
+                  ((w1 - a.wb[g.zf(i, j, k + 1)]) - (w0 - a.wb[g.zf(i, j, k)])) * idz;
+    rho_new = rmax(R(1.0e-6), rho_new - dtf * div);
+  }
+  a.rho_out[ci] = rho_new;
+  if (a.rt_out) a.rt_out[ci] = rho_new * a.theta[ci];
+  a.u_out[g.xf(i, j, k)] = u0;
+  a.v_out[g.yf(i, j, k)] = v0;
+  a.w_out[g.zf(i, j, k)] = w0;
+  if (i == g.nx - 1) a.u_out[g.xf(g.nx, j, k)] = u1;
+  if (j == g.ny - 1) a.v_out[g.yf(i, g.ny, k)] = v1;
+  if (k == g.nz - 1) a.w_out[g.zf(i, j, g.nz)] = w1;
+}
+
+// theta = rho_theta / max(rho, 1e-6) on owned cells (K9, dry_fast_modes.cu:86-95)
+template <class R>
+__global__ void k_theta(Geo g, const R* rho, const R* rt, R* th) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const size_t c = g.cell(i, j, k);
+  th[c] = d_theta(rho[c], rt[c]);
+}
+
+// ===========================================================================
+// Passive tracers, one SSPRK3 stage for all tracers (K23,
+// dry_transport.cu:49-125 + blend 40-47) with frozen rho and face momenta.
+// mode 1: out = max(0, in - dt div);  mode 2: out = wa*q0 + wb*(...).
+// ===========================================================================
+template <class R, bool TERRAIN>
+__global__ __launch_bounds__(256) void k_tracer_stage(TracerArgs<R> a) {
+  const Geo g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const Loader<R> L{g};
+  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R rc = a.rho[g.cell(i, j, k)];
+  const R rw = L.cell(a.rho, i - 1, j, k), re = L.cell(a.rho, i + 1, j, k);
+  const R rs = L.cell(a.rho, i, j - 1, k), rn = L.cell(a.rho, i, j + 1, k);
+  const R rb = k > 0 ? a.rho[g.cell(i, j, k - 1)] : R(1);
+  const R rt_ = k + 1 < g.nz ? a.rho[g.cell(i, j, k + 1)] : R(1);
+  const R mw = a.u[g.xf(i, j, k)], me = a.u[g.xf(i + 1, j, k)];
+  const R ms = a.v[g.yf(i, j, k)], mn = a.v[g.yf(i, j + 1, k)];
+  const R mb = a.w[g.zf(i, j, k)], mt = a.w[g.zf(i, j, k + 1)];
+  const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k];
+  const size_t ci = g.cell(i, j, k);
+  for (int t = 0; t < a.ntr; ++t) {
+    const R* qi = a.in[t];
+    auto q = [&](R rq, R r) { return rq / rmax(r, R(1.0e-6)); };
+    const R qc = q(qi[ci], rc);
+    const R qw = q(L.cell(qi, i - 1, j, k), rw), qe = q(L.cell(qi, i + 1, j, k), re);
+    const R qs = q(L.cell(qi, i, j - 1, k), rs), qn = q(L.cell(qi, i, j + 1, k), rn);
+    const R qb = k > 0 ? q(qi[g.cell(i, j, k - 1)], rb) : qc;
+    const R qt = k + 1 < g.nz ? q(qi[g.cell(i, j, k + 1)], rt_) : qc;
+    const R fw = mw * (mw >= R(0) ? qw : qc);
+    const R fe = me * (me >= R(0) ? qc : qe);
+    const R fs = ms * (ms >= R(0) ? qs : qc);
+    const R fn = mn * (mn >= R(0) ? qc : qn);
+    const R fb = mb * (mb >= R(0) ? qb : qc);
+    const R ft = mt * (mt >= R(0) ? qc : qt);
+    const R div = (fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz;
+    const R v = rmax(R(0), qi[ci] - a.dt * div);
+    a.out[t][ci] = a.mode == 2 ? a.wa * a.q0[t][ci] + a.wb * v : v;
+  }
+}
+
+// ===========================================================================
+// Warm rain + fallout, one thread per column marching top-down: K24
+// (warm_rain.cu:36-120) on cell k, then the fallout sweep on the same cell
+// (K25 nominal, warm_rain.cu:122-154, or K26 metric, 156-191), which reads
+// the post-microphysics rain exactly like the reference's qr_old copy.
+// ===========================================================================
+template <class R, bool TERRAIN>
+__global__ void k_warm_rain(WarmRainArgs<R> a) {
+  const Geo g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const R dt = a.dt;
+  R inc = 0;
+  const bool nominal = a.fallout == 1, metric = a.fallout == 2;
+  const R dz_nom = rmax(a.depth, R(1.0e-3));
+  const R frac_nom = rmin(rmax(a.vt * dt / dz_nom, R(0)), R(1));
+  for (int k = g.nz - 1; k >= 0; --k) {
+    const size_t c = g.cell(i, j, k);
+    const R rho = rmax(a.rho[c], R(1.0e-6));
+    const R rt0 = a.rt[c];
+    R th = rt0 / rho;
+    R p = d_pressure(rho, rt0);
+    R T = d_theta(rho, rt0) * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
+    const R ex = rmax(T / rmax(th, R(1.0e-6)), R(1.0e-6));
+    R qv = rmax(a.qv[c] / rho, R(0));
+    R qc = rmax(a.qc[c] / rho, R(0));
+    R qr = rmax(a.qr[c] / rho, R(0));
+    R qs = d_qsat(T, p);
+    const R lfac = Phys<R>::Lv / (d_cp<R>() * rmax(ex, R(1.0e-6)));
+    if (qv > qs) {
+      const R cnd = rmin((qv - qs) * rmax(a.relax, R(0)), qv);
+      qv -= cnd;
+      qc += cnd;
+      if (a.heat && cnd > R(0)) th += lfac * cnd;
+    } else if (qv < qs && qc > R(0)) {
+      const R e = rmin((qs - qv) * rmax(a.relax, R(0)), qc);
+      qv += e;
+      qc -= e;
+      if (a.heat && e > R(0)) th -= lfac * e;
+    }
+    const R exc = rmax(qc - a.thr, R(0));
+    if (exc > R(0) && a.krate > R(0)) {
+      const R cv = rmin(exc, dt * a.krate * exc);
+      qc -= cv;
+      qr += cv;
+    }
+    if (a.kacc > R(0) && qc > R(0) && qr > R(0)) {  // accretion extension
+      const R ac = rmin(qc, dt * a.kacc * qc * qr);
+      qc -= ac;
+      qr += ac;
+    }
+    p = d_pressure(rho, rho * th);
+    T = th * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
+    qs = d_qsat(T, p);
+    if (qr > R(0) && qv < qs && a.erate > R(0)) {
+      const R e = rmin(qr, dt * a.erate * rmin(qs - qv, qr));
+      qr -= e;
+      qv += e;
+      if (a.heat && e > R(0)) th -= lfac * e;
+    }
+    qv = rmax(qv, R(0));
+    qc = rmax(qc, R(0));
+    qr = rmax(qr, R(0));
+    a.qv[c] = rho * qv;
+    a.qc[c] = rho * qc;
+    R rqr = rho * qr;
+    a.rt[c] = rho * th;
+    if (nominal) {
+      const R old = rmax(rqr, R(0)) * dz_nom;
+      const R out = frac_nom * old;
+      rqr = (old - out + inc) / dz_nom;
+      inc = out;
+    } else if (metric) {
+      const R idz = rmax(TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k], R(1.0e-6));
+      const R dz = R(1) / idz;
+      const R frac = rmin(rmax(a.vt * dt * idz, R(0)), R(1));
+      const R old = rmax(rqr, R(0)) * dz;
+      const R out = frac * old;
+      rqr = (old - out + inc) * idz;
+      inc = out;
+    }
+    a.qr[c] = rqr;
+  }
+  if (nominal || metric) a.precip[static_cast<size_t>(j) * g.nx + i] += inc;
+}
+
+// ===========================================================================
+// Halo fill / interface averaging (1x1 layout) and the multi-rank exchange.
+// ===========================================================================
+template <class R>
+__global__ void k_halo_fill(Geo g, int orient, R* f) {
+  // orient: 0 cell, 1 x-face, 2 y-face, 3 z-face
+  const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
+  const int px = fx + 2 * g.h, py = fy + 2 * g.h;
+  const size_t n = static_cast<size_t>(px) * py * fz;
+  const size_t a = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int ii = static_cast<int>(a % px) - g.h;
+  const int jj = static_cast<int>((a / px) % py) - g.h;
+  const int k = static_cast<int>(a / (static_cast<size_t>(px) * py));
+  if (ii >= 0 && ii < fx && jj >= 0 && jj < fy) return;
+  const int si = orient == 1 ? map_face(ii, g.nx, g.per_x) : map_cell(ii, g.nx, g.per_x);
+  const int sj = orient == 2 ? map_face(jj, g.ny, g.per_y) : map_cell(jj, g.ny, g.per_y);
+  f[a] = (si < 0 || sj < 0) ? R(0)
+                            : f[(static_cast<size_t>(k) * py + (sj + g.h)) * px + (si + g.h)];
+}
+
+template <class R>
+__global__ void k_sync_interfaces(Geo g, int orient, R* f) {
+  // synchronize_face_owned_interfaces_impl for a 1x1 periodic layout
+  // (halo_exchange.cpp:255-283): face n and face 0 both take their average.
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (orient == 1) {
+    if (a >= g.ny * g.nz) return;
+    const int j = a % g.ny, k = a / g.ny;
+    const size_t e = g.xf(g.nx, j, k), w = g.xf(0, j, k);
+    const R avg = R(0.5) * (f[e] + f[w]);
+    f[e] = avg;
+    f[w] = avg;
+  } else {
+    if (a >= g.nx * g.nz) return;
+    const int i = a % g.nx, k = a / g.nx;
+    const size_t nn = g.yf(i, g.ny, k), s = g.yf(i, 0, k);
+    const R avg = R(0.5) * (f[nn] + f[s]);
+    f[nn] = avg;
+    f[s] = avg;
+  }
+}
+
+// Multi-rank exchange (halo_exchange.cpp:32-116, 148-215, 313-373, 563-620):
+// phase 0 fills the x ghosts of owned rows, phase 1 the y ghost rows (which
+// include the x ghosts).  d: 13 ints per rank + 4 neighbour ranks.
+template <class R>
+__global__ void k_exchange_layout(int phase, int orient, int nranks, const int* d, R* const* F,
+                                  int maxn) {
+  const int rank = blockIdx.y;
+  if (rank >= nranks) return;
+  const int* D = d + rank * 17;
+  const int h = D[8], nxl = D[10] - D[9], nyl = D[12] - D[11];
+  const int fx = nxl + (orient == 1), fy = nyl + (orient == 2), fz = D[7] + (orient == 3);
+  const int px = fx + 2 * h, py = fy + 2 * h;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  R* f = F[rank];
+  auto at = [&](R* base, int ppx, int ppy, int i, int j, int k) -> R& {
+    return base[(static_cast<size_t>(k) * ppy + (j + h)) * ppx + (i + h)];
+  };
+  if (phase == 0) {
+    // x phase: for k < fz, j < fy, hh < h: west ghost (-h+hh) and east ghost (fx+hh)
+    const int total = fz * fy * h;
+    if (a >= total) return;
+    const int hh = a % h, j = (a / h) % fy, k = a / (h * fy);
+    const int west = D[13], east = D[14];
+    // west ghost <- west neighbour's east send
+    if (west >= 0) {
+      const int* W = d + west * 17;
+      const int wnxl = W[10] - W[9], wnyl = W[12] - W[11];
+      const int wfx = wnxl + (orient == 1), wfy = wnyl + (orient == 2);
+      const int ei = orient == 1 ? wnxl - h + hh : wfx - h + hh;
+      at(f, px, py, -h + hh, j, k) = at(F[west], wfx + 2 * h, wfy + 2 * h, ei, j, k);
+    } else {
+      at(f, px, py, -h + hh, j, k) = R(0);
+    }
+    if (east >= 0) {
+      const int* E = d + east * 17;
+      const int enxl = E[10] - E[9], enyl = E[12] - E[11];
+      const int efx = enxl + (orient == 1), efy = enyl + (orient == 2);
+      const int wi = orient == 1 ? 1 + hh : hh;
+      at(f, px, py, fx + hh, j, k) = at(F[east], efx + 2 * h, efy + 2 * h, wi, j, k);
+    } else {
+      at(f, px, py, fx + hh, j, k) = R(0);
+    }
+  } else {
+    const int total = fz * h * (fx + 2 * h);
+    if (a >= total) return;
+    const int i = a % (fx + 2 * h) - h, hh = (a / (fx + 2 * h)) % h, k = a / ((fx + 2 * h) * h);
+    const int south = D[15], north = D[16];
+    if (south >= 0) {
+      const int* S = d + south * 17;
+      const int snxl = S[10] - S[9], snyl = S[12] - S[11];
+      const int sfx = snxl + (orient == 1), sfy = snyl + (orient == 2);
+      const int nj = orient == 2 ? snyl - h + hh : sfy - h + hh;
+      at(f, px, py, i, -h + hh, k) = at(F[south], sfx + 2 * h, sfy + 2 * h, i, nj, k);
+    } else {
+      at(f, px, py, i, -h + hh, k) = R(0);
+    }
+    if (north >= 0) {
+      const int* N = d + north * 17;
+      const int nnxl = N[10] - N[9], nnyl = N[12] - N[11];
+      const int nfx = nnxl + (orient == 1), nfy = nnyl + (orient == 2);
+      const int sj = orient == 2 ? 1 + hh : hh;
+      at(f, px, py, i, fy + hh, k) = at(F[north], nfx + 2 * h, nfy + 2 * h, i, sj, k);
+    } else {
+      at(f, px, py, i, fy + hh, k) = R(0);
+    }
+  }
+  (void)maxn;
+}
+
+// ===========================================================================
+// Boundary strips / relaxation on the outermost owned strips of one field
+// (boundary_conditions.cpp:9-161; relaxation is the extension documented in
+// the header).  orient as in k_halo_fill.
+// ===========================================================================
+template <class R>
+__global__ void k_boundary(Geo g, int orient, R* f, const R* ref, int mode, R width) {
+  const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= fx * fy * fz) return;
+  const int i = a % fx, j = (a / fx) % fy, k = a / (fx * fy);
+  const bool w = !g.per_x, e = !g.per_x, s = !g.per_y, n = !g.per_y;
+  const int px = fx + 2 * g.h, py = fy + 2 * g.h;
+  const size_t idx = (static_cast<size_t>(k) * py + (j + g.h)) * px + (i + g.h);
+  if (mode == 1) {
+    if ((w && i == 0) || (e && i == fx - 1) || (s && j == 0) || (n && j == fy - 1)) f[idx] = ref[idx];
+    return;
+  }
+  double dmin = 1e30;
+  if (w) dmin = fmin(dmin, static_cast<double>(i));
+  if (e) dmin = fmin(dmin, static_cast<double>(fx - 1 - i));
+  if (s) dmin = fmin(dmin, static_cast<double>(j));
+  if (n) dmin = fmin(dmin, static_cast<double>(fy - 1 - j));
+  if (dmin >= static_cast<double>(width)) return;
+  const double a0 = 1.0 - dmin / static_cast<double>(width);
+  const R wgt = static_cast<R>(a0 * a0);
+  f[idx] = f[idx] + wgt * (ref[idx] - f[idx]);
+}
+
+// ===========================================================================
+// Budget reductions: compensated (Neumaier) FP64 partial sums, min and max
+// of a field over owned elements, optionally weighted by dz = 1/inv_dz_cell
+// (column paths, runtime_summary.cpp:70-80) or divided by rho (mixing
+// ratio, runtime_summary.cpp:82-99).  One partial record per block:
+// {sum, comp, min, max}.
+// ===========================================================================
+template <class R>
+__global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  const size_t n = static_cast<size_t>(a.fx) * a.fy * a.fz;
+  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * 256 + t; e < n;
+       e += static_cast<size_t>(gridDim.x) * 256) {
+    const int i = static_cast<int>(e % a.fx);
+    const int j = static_cast<int>((e / a.fx) % a.fy);
+    const int k = static_cast<int>(e / (static_cast<size_t>(a.fx) * a.fy));
+    const size_t idx = (static_cast<size_t>(k) * (a.fy + 2 * a.h) + (j + a.h)) * (a.fx + 2 * a.h) + (i + a.h);
+    double v = static_cast<double>(a.f[idx]);
+    if (a.kind == 1) {  // column path
+      const R idz = a.idz3 ? a.idz3[(static_cast<size_t>(k) * a.fy + j) * a.fx + i] : a.idz_lev[k];
+      v = v * (1.0 / static_cast<double>(idz));
+    } else if (a.kind == 2) {  // theta_m / mixing ratio: f / max(rho, 1e-12)
+      v = v / fmax(static_cast<double>(a.rho[idx]), 1.0e-12);
+    }
+    const double s2 = sum + v;
+    comp += fabs(sum) >= fabs(v) ? (sum - s2) + v : (v - s2) + sum;
+    sum = s2;
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  // Block combine in a fixed tree order; every node is a TwoSum whose
+  // rounding error joins the compensation, so sum + comp stays exact to
+  // working precision.
+  double* out = a.partials + static_cast<size_t>(blockIdx.x) * 4;
+  __shared__ double se[256];
+  sh[t] = sum;
+  se[t] = comp;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      const double x = sh[t], y = sh[t + s];
+      const double s2 = x + y;
+      const double bb = s2 - x;
+      const double err = (x - (s2 - bb)) + (y - bb);
+      sh[t] = s2;
+      se[t] = se[t] + se[t + s] + err;
+    }
+    __syncthreads();
+  }
+  const double bsum = sh[0];
+  const double bcomp = se[0];
+  __syncthreads();
+  sh[t] = mn;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmin(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  const double bmn = sh[0];
+  __syncthreads();
+  sh[t] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = bsum;
+    out[1] = bcomp;
+    out[2] = bmn;
+    out[3] = sh[0];
+  }
+}
+
+// ===========================================================================
+// Metric construction (grid_metrics.hpp:184-236): one thread per column,
+// FP64 arithmetic without contraction so the stored values match a host
+// build of the same formulas.
+// ===========================================================================
+template <class R>
+__global__ void k_metrics(Geo g, const double* zif, const double* zc, const double* wif,
+                          const double* wc, const float* terrain, R* zif_m, R* zc_m, R* idz_c,
+                          R* idz_f, R* slope_x, R* slope_y, R* terr_out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  auto tr = [&](int ii, int jj) -> float { return terrain ? terrain[static_cast<size_t>(jj) * g.nx + ii] : 0.0f; };
+  auto wi = [&](int ii) { return g.per_x ? map_cell(ii, g.nx, 1) : min(max(ii, 0), g.nx - 1); };
+  auto wj = [&](int jj) { return g.per_y ? map_cell(jj, g.ny, 1) : min(max(jj, 0), g.ny - 1); };
+  const float th = tr(i, j);
+  const size_t c2 = static_cast<size_t>(j) * g.nx + i;
+  terr_out[c2] = static_cast<R>(th);
+  const double ddx = (static_cast<double>(tr(wi(i + 1), j)) - static_cast<double>(tr(wi(i - 1), j))) /
+                     (g.per_x || (i > 0 && i + 1 < g.nx) ? 2.0 * g.dx : g.dx);
+  const double ddy = (static_cast<double>(tr(i, wj(j + 1))) - static_cast<double>(tr(i, wj(j - 1)))) /
+                     (g.per_y || (j > 0 && j + 1 < g.ny) ? 2.0 * g.dy : g.dy);
+  slope_x[c2] = static_cast<R>(ddx);
+  slope_y[c2] = static_cast<R>(ddy);
+  const double thd = static_cast<double>(th);
+  for (int k = 0; k <= g.nz; ++k) {
+    const double prod = wif[k] * thd;
+    zif_m[g.metric(i, j, k)] = static_cast<R>(zif[k] + prod);
+    if (k == 0 || k == g.nz) {
+      idz_f[g.metric(i, j, k)] = R(0);
+    } else {
+      const double hi = zc[k] + wc[k] * thd;
+      const double lo = zc[k - 1] + wc[k - 1] * thd;
+      const double dzf = hi - lo;
+      idz_f[g.metric(i, j, k)] = static_cast<R>(1.0 / fmax(dzf, 1.0e-6));
+    }
+  }
+  for (int k = 0; k < g.nz; ++k) {
+    const double prod = wc[k] * thd;
+    zc_m[g.metric(i, j, k)] = static_cast<R>(zc[k] + prod);
+    const double dzc = static_cast<double>(zif_m[g.metric(i, j, k + 1)]) -
+                       static_cast<double>(zif_m[g.metric(i, j, k)]);
+    idz_c[g.metric(i, j, k)] = static_cast<R>(1.0 / fmax(dzc, 1.0e-6));
+  }
+}
+
+// Copy of owned + halo storage (whole buffer).
+template <class R>
+__global__ void k_copy(const R* src, R* dst, size_t n) {
+  const size_t a = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a < n) dst[a] = src[a];
+}
+template <class R>
+__global__ void k_fill(R* dst, size_t n, R v) {
+  const size_t a = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a < n) dst[a] = v;
+}
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+namespace {
+inline dim3 cell_grid(const Geo& g, dim3 b) {
+  return dim3((g.nx + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, g.nz);
+}
+inline unsigned blocks_for(size_t n, unsigned b) { return static_cast<unsigned>((n + b - 1) / b); }
+}  // namespace
+
+template <class R>
+void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
+                           double* partials, int* nchunk_out) {
+  const int rows = 8;
+  const int nchunk = (g.ny + rows - 1) / rows;
+  hipLaunchKernelGGL(k_level_partials<R>, dim3(nchunk, g.nz), dim3(256), 0, st, g, rho, aux, mode,
+                     rows, partials);
+  *nchunk_out = nchunk;
+}
+template <class R>
+void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
+                           double count, R* o0, R* o1) {
+  hipLaunchKernelGGL(k_level_finalize<R>, dim3((nz + 63) / 64), dim3(64), 0, st, nz, nchunk,
+                     partials, mode, count, o0, o1);
+}
+template <class R>
+void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p) {
+  hipLaunchKernelGGL(k_pressure<R>, dim3(blocks_for(g.cell_total(), 256)), dim3(256), 0, st, g,
+                     rho, rt, p);
+}
+template <class R>
+void launch_slow_stage(hipStream_t st, const SlowArgs<R>& a, bool terrain) {
+  const dim3 b(64, 4, 1);
+  if (terrain)
+    hipLaunchKernelGGL((k_slow_stage<R, true>), cell_grid(a.g, b), b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_slow_stage<R, false>), cell_grid(a.g, b), b, 0, st, a);
+}
+template <class R>
+void launch_column_reference(hipStream_t st, const Geo& g, const R* rho, const R* theta,
+                             const R* zc, R gravity, R* rref, R* pref) {
+  const dim3 b(64, 4, 1);
+  hipLaunchKernelGGL(k_column_reference<R>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), b, 0, st, g,
+                     rho, theta, zc, gravity, rref, pref);
+}
+template <class R>
+void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain) {
+  const dim3 b(64, 4, 1);
+  if (terrain)
+    hipLaunchKernelGGL((k_fast_substep<R, true>), cell_grid(a.g, b), b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_fast_substep<R, false>), cell_grid(a.g, b), b, 0, st, a);
+}
+template <class R>
+void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th) {
+  const dim3 b(64, 4, 1);
+  hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th);
+}
+template <class R>
+void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain) {
+  const dim3 b(64, 4, 1);
+  if (terrain)
+    hipLaunchKernelGGL((k_tracer_stage<R, true>), cell_grid(a.g, b), b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_tracer_stage<R, false>), cell_grid(a.g, b), b, 0, st, a);
+}
+template <class R>
+void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain) {
+  const dim3 b(64, 4, 1);
+  const dim3 gr((a.g.nx + 63) / 64, (a.g.ny + 3) / 4);
+  if (terrain)
+    hipLaunchKernelGGL((k_warm_rain<R, true>), gr, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_warm_rain<R, false>), gr, b, 0, st, a);
+}
+template <class R>
+void launch_halo_fill(hipStream_t st, const Geo& g, int orient, R* f) {
+  const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
+  const size_t n = static_cast<size_t>(fx + 2 * g.h) * (fy + 2 * g.h) * fz;
+  hipLaunchKernelGGL(k_halo_fill<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, g, orient, f);
+}
+template <class R>
+void launch_sync_interfaces(hipStream_t st, const Geo& g, int orient, R* f) {
+  const int n = orient == 1 ? g.ny * g.nz : g.nx * g.nz;
+  hipLaunchKernelGGL(k_sync_interfaces<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, g, orient, f);
+}
+template <class R>
+void launch_exchange_layout(hipStream_t st, int phase, int orient, int nranks, const int* d,
+                            R* const* F, int max_elems) {
+  hipLaunchKernelGGL(k_exchange_layout<R>, dim3(blocks_for(max_elems, 256), nranks), dim3(256), 0,
+                     st, phase, orient, nranks, d, F, max_elems);
+}
+template <class R>
+void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* ref, int mode,
+                     double width) {
+  const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
+  hipLaunchKernelGGL(k_boundary<R>, dim3(blocks_for(static_cast<size_t>(fx) * fy * fz, 256)),
+                     dim3(256), 0, st, g, orient, f, ref, mode, static_cast<R>(width));
+}
+template <class R>
+void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks) {
+  hipLaunchKernelGGL(k_reduce<R>, dim3(nblocks), dim3(256), 0, st, a);
+}
+template <class R>
+void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const double* zc,
+                    const double* wif, const double* wc, const float* terrain, R* zif_m, R* zc_m,
+                    R* idz_c, R* idz_f, R* sx, R* sy, R* terr) {
+  const dim3 b(64, 4, 1);
+  hipLaunchKernelGGL(k_metrics<R>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), b, 0, st, g, zif, zc,
+                     wif, wc, terrain, zif_m, zc_m, idz_c, idz_f, sx, sy, terr);
+}
+template <class R>
+void launch_copy(hipStream_t st, const R* src, R* dst, size_t n) {
+  hipLaunchKernelGGL(k_copy<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, n);
+}
+template <class R>
+void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
+  hipLaunchKernelGGL(k_fill<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, n, v);
+}
+
+#define GWM_INSTANTIATE(R)                                                                          \
+  template void launch_level_partials<R>(hipStream_t, const Geo&, const R*, const R*, int,        \
+                                         double*, int*);                                          \
+  template void launch_level_finalize<R>(hipStream_t, int, int, const double*, int, double, R*,   \
+                                         R*);                                                     \
+  template void launch_pressure<R>(hipStream_t, const Geo&, const R*, const R*, R*);             \
+  template void launch_slow_stage<R>(hipStream_t, const SlowArgs<R>&, bool);                     \
+  template void launch_column_reference<R>(hipStream_t, const Geo&, const R*, const R*, const R*, \
+                                           R, R*, R*);                                            \
+  template void launch_fast_substep<R>(hipStream_t, const FastArgs<R>&, bool);                   \
+  template void launch_theta<R>(hipStream_t, const Geo&, const R*, const R*, R*);                \
+  template void launch_tracer_stage<R>(hipStream_t, const TracerArgs<R>&, bool);                 \
+  template void launch_warm_rain<R>(hipStream_t, const WarmRainArgs<R>&, bool);                  \
+  template void launch_halo_fill<R>(hipStream_t, const Geo&, int, R*);                           \
+  template void launch_sync_interfaces<R>(hipStream_t, const Geo&, int, R*);                     \
+  template void launch_exchange_layout<R>(hipStream_t, int, int, int, const int*, R* const*, int); \
+  template void launch_boundary<R>(hipStream_t, const Geo&, int, R*, const R*, int, double);     \
+  template void launch_reduce<R>(hipStream_t, const ReduceArgs<R>&, int);                        \
+  template void launch_metrics<R>(hipStream_t, const Geo&, const double*, const double*,         \
+                                  const double*, const double*, const float*, R*, R*, R*, R*, R*, \
+                                  R*, R*);                                                        \
+  template void launch_copy<R>(hipStream_t, const R*, R*, size_t);                               \
+  template void launch_fill<R>(hipStream_t, R*, size_t, R);
+
+GWM_INSTANTIATE(float)
+GWM_INSTANTIATE(double)
+
+}  // namespace gwm

@@ -1,0 +1,121 @@
+// gwm_kernels.hpp -- kernel argument blocks and launcher declarations.
+#pragma once
+
+#include "gwm_internal.hpp"
+
+namespace gwm {
+
+template <class R>
+struct SlowArgs {
+  Geo g;
+  DState<R> s;    // stage input state (rho, rho_theta, faces)
+  DState<R> q0;   // SSP base (mode 2)
+  DState<R> out;  // output (tendencies in mode 0)
+  DMetrics<R> m;
+  const R* theta_ref;  // nz
+  R gravity;
+  R dt;
+  R wa, wb;
+  int mode;       // 0 tendencies, 1 update, 2 update + blend
+  int out_theta;  // write theta instead of rho*theta in the cell output
+};
+
+template <class R>
+struct FastArgs {
+  Geo g;
+  const R* rho;    // current rho
+  const R* theta;  // fixed theta of the subcycle
+  const R *u, *v, *w;     // current faces
+  const R *ub, *vb, *wb;  // faces at subcycle entry
+  R *rho_out, *u_out, *v_out, *w_out;
+  R* rt_out;  // rho*theta rebuild (last substep), may be null
+  const R *rref_lev, *pref_lev;  // flat reference (nz)
+  const R *rref3, *pref3;        // column reference ((k*ny+j)*nx+i)
+  DMetrics<R> m;
+  R dtf, gravity;
+  int horizontal, density, col_ref;
+};
+
+template <class R>
+struct TracerArgs {
+  Geo g;
+  const R* rho;
+  const R *u, *v, *w;
+  const R* idz_c;
+  const R* lev_idz_c;
+  const R* in[kMaxTracers];
+  const R* q0[kMaxTracers];
+  R* out[kMaxTracers];
+  int ntr;
+  R dt, wa, wb;
+  int mode;
+};
+
+template <class R>
+struct WarmRainArgs {
+  Geo g;
+  const R* rho;
+  R* rt;
+  R *qv, *qc, *qr;
+  R* precip;
+  const R* idz_c;
+  const R* lev_idz_c;
+  R dt, relax, thr, krate, erate, vt, depth, kacc;
+  int heat;
+  int fallout;  // 0 none, 1 nominal, 2 metric
+};
+
+template <class R>
+struct ReduceArgs {
+  const R* f;
+  const R* rho;
+  const R* idz3;
+  const R* idz_lev;
+  int fx, fy, fz, h;
+  int kind;  // 0 value, 1 value*dz, 2 value/rho
+  double* partials;
+};
+
+template <class R>
+void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
+                           double* partials, int* nchunk_out);
+template <class R>
+void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
+                           double count, R* o0, R* o1);
+template <class R>
+void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p);
+template <class R>
+void launch_slow_stage(hipStream_t st, const SlowArgs<R>& a, bool terrain);
+template <class R>
+void launch_column_reference(hipStream_t st, const Geo& g, const R* rho, const R* theta,
+                             const R* zc, R gravity, R* rref, R* pref);
+template <class R>
+void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain);
+template <class R>
+void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th);
+template <class R>
+void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
+template <class R>
+void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain);
+template <class R>
+void launch_halo_fill(hipStream_t st, const Geo& g, int orient, R* f);
+template <class R>
+void launch_sync_interfaces(hipStream_t st, const Geo& g, int orient, R* f);
+template <class R>
+void launch_exchange_layout(hipStream_t st, int phase, int orient, int nranks, const int* d,
+                            R* const* F, int max_elems);
+template <class R>
+void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* ref, int mode,
+                     double width);
+template <class R>
+void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks);
+template <class R>
+void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const double* zc,
+                    const double* wif, const double* wc, const float* terrain, R* zif_m, R* zc_m,
+                    R* idz_c, R* idz_f, R* sx, R* sy, R* terr);
+template <class R>
+void launch_copy(hipStream_t st, const R* src, R* dst, size_t n);
+template <class R>
+void launch_fill(hipStream_t st, R* dst, size_t n, R v);
+
+}  // namespace gwm

@@ -1,0 +1,385 @@
+"""Python host binding of libgwm_hip.so (the MI355X dry + warm-rain step).
+
+A thin ctypes layer over the C-ABI in include/gwm_hip.h, used by the tests
+and bench.py.  There is no CPU fallback: if the HIP library is missing or no
+GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libgwm_hip.so")
+HEADER = os.path.join(REPO, "include", "gwm_hip.h")
+
+F32, F64 = 0, 1
+RHO, RHO_THETA, MOM_U, MOM_V, MOM_W, PRESSURE = 0, 1, 2, 3, 4, 5
+TEND_RHO, TEND_RHO_THETA, TEND_U, TEND_V, TEND_W = 6, 7, 8, 9, 10
+PRECIP, TRACER0 = 11, 16
+BOUNDARY_NONE, BOUNDARY_STRIPS, BOUNDARY_RELAX = 0, 1, 2
+
+SUMMARY_KEYS = [
+    "total_dry_mass", "total_rho_theta_m", "total_horizontal_momentum_x",
+    "total_horizontal_momentum_y", "total_vertical_momentum", "rho_min", "rho_max",
+    "theta_min", "theta_max", "u_min", "u_max", "v_min", "v_max", "w_min", "w_max",
+    "vapor_path", "cloud_path", "rain_path", "precip_sum", "precip_mean", "precip_max",
+    "precip_cells", "precipitating_cells", "qv_min", "qv_max", "qc_min", "qc_max",
+    "qr_min", "qr_max",
+]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("halo", ctypes.c_int), ("periodic_x", ctypes.c_int),
+                ("periodic_y", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("z_top", ctypes.c_double),
+                ("terrain_taper_eta", ctypes.c_double)]
+
+
+class StepConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("fast_substeps", ctypes.c_int),
+                ("gravity", ctypes.c_double),
+                ("fast_update_horizontal_momentum", ctypes.c_int),
+                ("fast_update_density", ctypes.c_int)]
+
+
+def step_config(dt=2.0, fast_substeps=1, gravity=9.81, horizontal=True, density=True):
+    """DryStepperConfig defaults (include/gwm/dycore/dry_core.hpp:50-56)."""
+    return StepConfig(dt, fast_substeps, gravity, int(horizontal), int(density))
+
+
+class WarmRainConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("condensation_relaxation", ctypes.c_double),
+                ("cloud_autoconversion_threshold", ctypes.c_double),
+                ("cloud_autoconversion_rate", ctypes.c_double),
+                ("rain_evaporation_rate", ctypes.c_double),
+                ("rain_terminal_velocity", ctypes.c_double),
+                ("sedimentation_layer_depth_m", ctypes.c_double),
+                ("enable_latent_heating", ctypes.c_int),
+                ("accretion_rate", ctypes.c_double)]
+
+
+def warm_rain_config(dt=2.0, condensation_relaxation=1.0, cloud_autoconversion_threshold=1.0e-4,
+                     cloud_autoconversion_rate=1.0e-3, rain_evaporation_rate=5.0e-3,
+                     rain_terminal_velocity=0.0, sedimentation_layer_depth_m=0.0,
+                     enable_latent_heating=True, accretion_rate=0.0):
+    """WarmRainConfig defaults (include/gwm/physics/warm_rain.hpp:14-23)."""
+    return WarmRainConfig(dt, condensation_relaxation, cloud_autoconversion_threshold,
+                          cloud_autoconversion_rate, rain_evaporation_rate,
+                          rain_terminal_velocity, sedimentation_layer_depth_m,
+                          int(enable_latent_heating), accretion_rate)
+
+
+BOUNDARY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
+
+
+class BoundaryConfig(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("relax_width", ctypes.c_double),
+                ("callback", BOUNDARY_FN), ("user", ctypes.c_void_p)]
+
+
+class GwmError(RuntimeError):
+    pass
+
+
+_lib = None
+_shutting_down = False
+
+
+@atexit.register
+def _mark_shutdown():
+    # Objects collected during interpreter shutdown leak their device memory
+    # to the process exit instead of calling into a runtime that may be
+    # tearing down.
+    global _shutting_down
+    _shutting_down = True
+
+
+def header_functions():
+    """Names of every function declared in include/gwm_hip.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(gwm_[a-z0-9_]+)\s*\(", text)) - {"gwm_boundary_fn"})
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GwmError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+        pvp = ctypes.POINTER(ctypes.c_void_p)
+        L.gwm_last_error.restype = cp
+        L.gwm_ctx_create.argtypes = [ip, ip, pvp]
+        L.gwm_ctx_destroy.argtypes = [vp]
+        L.gwm_sync.argtypes = [vp]
+        L.gwm_ctx_precision.argtypes = [vp]
+        L.gwm_ctx_set_kernel_timing.argtypes = [vp, ip]
+        L.gwm_ctx_kernel_stats.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_long),
+                                           ctypes.POINTER(ctypes.c_double)]
+        L.gwm_ctx_reset_kernel_stats.argtypes = [vp]
+        L.gwm_grid_create.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, pvp]
+        L.gwm_grid_destroy.argtypes = [vp]
+        L.gwm_grid_download_metric.argtypes = [vp, ip, vp]
+        L.gwm_grid_terrain_active.argtypes = [vp]
+        L.gwm_terrain_cosine_mountain.argtypes = [ip, ip, dp, dp, dp, dp, dp, vp]
+        L.gwm_terrain_agnesi_ridge.argtypes = [ip, ip, dp, dp, dp, dp, vp]
+        L.gwm_stretched_levels.argtypes = [ip, dp, dp, dp, vp]
+        L.gwm_state_create.argtypes = [vp, vp, ip, pvp]
+        L.gwm_state_destroy.argtypes = [vp]
+        L.gwm_state_field_size.restype = ctypes.c_long
+        L.gwm_state_field_size.argtypes = [vp, ip]
+        for n in ("gwm_state_upload", "gwm_state_download"):
+            getattr(L, n).argtypes = [vp, ip, vp]
+        L.gwm_state_fill.argtypes = [vp, ip, dp]
+        L.gwm_state_copy.argtypes = [vp, vp]
+        L.gwm_state_capture_boundary.argtypes = [vp]
+        L.gwm_halo_fill.argtypes = [vp, ip]
+        L.gwm_sync_interfaces.argtypes = [vp, ip]
+        L.gwm_halo_exchange_layout.argtypes = [vp, ip, vp, ip, ip, ip, vp]
+        L.gwm_device_alloc.argtypes = [vp, ctypes.c_size_t, pvp]
+        L.gwm_device_free.argtypes = [vp, vp]
+        L.gwm_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.gwm_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
+        L.gwm_pressure.argtypes = [vp]
+        L.gwm_slow_tendencies.argtypes = [vp, ctypes.POINTER(StepConfig)]
+        L.gwm_fast_modes.argtypes = [vp, ctypes.POINTER(StepConfig)]
+        L.gwm_ssprk3_step.argtypes = [vp, ctypes.POINTER(StepConfig), ctypes.POINTER(BoundaryConfig)]
+        L.gwm_tracers_ssprk3.argtypes = [vp, ctypes.POINTER(StepConfig), ip]
+        L.gwm_warm_rain.argtypes = [vp, ctypes.POINTER(WarmRainConfig), ip]
+        L.gwm_apply_boundary.argtypes = [vp, ip, dp]
+        L.gwm_moist_step.argtypes = [vp, ctypes.POINTER(StepConfig),
+                                     ctypes.POINTER(WarmRainConfig), ctypes.POINTER(BoundaryConfig)]
+        L.gwm_budget_sums.argtypes = [vp, vp]
+        L.gwm_init_constant.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.gwm_init_hydrostatic_rest.argtypes = [vp, dp, dp]
+        L.gwm_init_bubble.argtypes = [vp, vp, ip]
+        L.gwm_init_mountain_wave.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.gwm_init_constant_n.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.gwm_init_convection.argtypes = [vp, dp, ctypes.c_ulonglong, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(rc):
+    if rc != 0:
+        raise GwmError(lib().gwm_last_error().decode())
+
+
+def cosine_mountain(nx, ny, height, cx=0.5, cy=0.5, hx=0.15, hy=0.15):
+    out = np.zeros((ny, nx), np.float32)
+    check(lib().gwm_terrain_cosine_mountain(nx, ny, cx, cy, hx, hy, height, _ptr(out)))
+    return out
+
+
+def agnesi_ridge(nx, ny, dx, center_m, half_width_m, height_m):
+    out = np.zeros((ny, nx), np.float32)
+    check(lib().gwm_terrain_agnesi_ridge(nx, ny, dx, center_m, half_width_m, height_m, _ptr(out)))
+    return out
+
+
+def stretched_levels(nz, dz_bottom, dz_top, z_top):
+    out = np.zeros(nz + 1, np.float64)
+    check(lib().gwm_stretched_levels(nz, dz_bottom, dz_top, z_top, _ptr(out)))
+    return out
+
+
+class Context:
+    def __init__(self, device=0, precision=F32):
+        self.h = ctypes.c_void_p()
+        check(lib().gwm_ctx_create(device, precision, ctypes.byref(self.h)))
+        self.precision = precision
+        self.dtype = np.float64 if precision == F64 else np.float32
+
+    def close(self):
+        if self.h and not _shutting_down:
+            lib().gwm_ctx_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().gwm_sync(self.h))
+
+    def set_kernel_timing(self, on):
+        check(lib().gwm_ctx_set_kernel_timing(self.h, int(on)))
+
+    def kernel_stats(self, name):
+        n = ctypes.c_long()
+        ms = ctypes.c_double()
+        check(lib().gwm_ctx_kernel_stats(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_kernel_stats(self):
+        check(lib().gwm_ctx_reset_kernel_stats(self.h))
+
+
+class Grid:
+    def __init__(self, ctx, nx, ny, nz, halo=1, periodic_x=True, periodic_y=True, dx=1000.0,
+                 dy=1000.0, z_top=16000.0, z_interfaces=None, terrain=None, taper_eta=0.25):
+        self.ctx = ctx
+        self.nx, self.ny, self.nz, self.halo = nx, ny, nz, halo
+        self.desc = GridDesc(nx, ny, nz, halo, int(periodic_x), int(periodic_y), dx, dy, z_top,
+                             taper_eta)
+        zi = None if z_interfaces is None else np.ascontiguousarray(z_interfaces, np.float64)
+        tr = None if terrain is None else np.ascontiguousarray(terrain, np.float32).reshape(-1)
+        self.h = ctypes.c_void_p()
+        check(lib().gwm_grid_create(ctx.h, ctypes.byref(self.desc),
+                                    None if zi is None else _ptr(zi),
+                                    None if tr is None else _ptr(tr), ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and not _shutting_down:
+            lib().gwm_grid_destroy(self.h)
+        self.h = None
+
+    def metric(self, which):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        shapes = {0: (nz + 1, ny, nx), 1: (nz, ny, nx), 2: (nz, ny, nx), 3: (nz + 1, ny, nx),
+                  4: (ny, nx), 5: (ny, nx), 6: (ny, nx), 7: (nz + 1,), 8: (nz,), 9: (nz + 1,),
+                  10: (nz,)}
+        dt = np.float64 if which >= 7 else self.ctx.dtype
+        out = np.empty(shapes[which], dt)
+        check(lib().gwm_grid_download_metric(self.h, which, _ptr(out)))
+        return out
+
+    @property
+    def terrain_active(self):
+        return bool(lib().gwm_grid_terrain_active(self.h))
+
+
+class State:
+    """Device state; storage() returns the halo-padded arrays (nz, py, px)."""
+
+    def __init__(self, grid, n_tracers=0):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self.n_tracers = n_tracers
+        self.h = ctypes.c_void_p()
+        check(lib().gwm_state_create(self.ctx.h, grid.h, n_tracers, ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and not _shutting_down:
+            lib().gwm_state_destroy(self.h)
+        self.h = None
+
+    def storage_shape(self, field):
+        g = self.grid
+        h = g.halo
+        fx = g.nx + (1 if field in (MOM_U, TEND_U) else 0)
+        fy = g.ny + (1 if field in (MOM_V, TEND_V) else 0)
+        fz = g.nz + (1 if field in (MOM_W, TEND_W) else 0)
+        if field == PRECIP:
+            return (g.ny, g.nx)
+        return (fz, fy + 2 * h, fx + 2 * h)
+
+    def storage(self, field):
+        out = np.empty(self.storage_shape(field), self.ctx.dtype)
+        check(lib().gwm_state_download(self.h, field, _ptr(out)))
+        return out
+
+    def set_storage(self, field, arr):
+        a = np.ascontiguousarray(arr, self.ctx.dtype)
+        assert a.shape == self.storage_shape(field), (a.shape, self.storage_shape(field))
+        check(lib().gwm_state_upload(self.h, field, _ptr(a)))
+
+    def get(self, field):
+        """Owned values (no halo), shaped like the oracle's global arrays."""
+        s = self.storage(field)
+        if field == PRECIP:
+            return s[None]
+        h = self.grid.halo
+        return np.ascontiguousarray(s[:, h:s.shape[1] - h, h:s.shape[2] - h])
+
+    def set(self, field, arr):
+        if field == PRECIP:
+            a = np.ascontiguousarray(np.reshape(arr, self.storage_shape(field)), self.ctx.dtype)
+            check(lib().gwm_state_upload(self.h, field, _ptr(a)))
+            return
+        s = self.storage(field)
+        h = self.grid.halo
+        s[:, h:s.shape[1] - h, h:s.shape[2] - h] = arr
+        self.set_storage(field, s)
+
+    def fill(self, field, value):
+        check(lib().gwm_state_fill(self.h, field, float(value)))
+
+    def halo_fill(self, field):
+        check(lib().gwm_halo_fill(self.h, field))
+
+    def sync_interfaces(self, field):
+        check(lib().gwm_sync_interfaces(self.h, field))
+
+    def capture_boundary(self):
+        check(lib().gwm_state_capture_boundary(self.h))
+
+    # operators
+    def pressure(self):
+        check(lib().gwm_pressure(self.h))
+
+    def slow_tendencies(self, cfg):
+        check(lib().gwm_slow_tendencies(self.h, ctypes.byref(cfg)))
+
+    def fast_modes(self, cfg):
+        check(lib().gwm_fast_modes(self.h, ctypes.byref(cfg)))
+
+    def ssprk3_step(self, cfg, boundary=None):
+        b = None if boundary is None else ctypes.byref(boundary)
+        check(lib().gwm_ssprk3_step(self.h, ctypes.byref(cfg), b))
+
+    def tracers_ssprk3(self, cfg, boundary_mode=BOUNDARY_NONE):
+        check(lib().gwm_tracers_ssprk3(self.h, ctypes.byref(cfg), boundary_mode))
+
+    def warm_rain(self, cfg, metric_fallout=True):
+        check(lib().gwm_warm_rain(self.h, ctypes.byref(cfg), int(metric_fallout)))
+
+    def apply_boundary(self, mode, width=0.0):
+        check(lib().gwm_apply_boundary(self.h, mode, float(width)))
+
+    def moist_step(self, cfg, wr, boundary=None):
+        b = None if boundary is None else ctypes.byref(boundary)
+        check(lib().gwm_moist_step(self.h, ctypes.byref(cfg), ctypes.byref(wr), b))
+
+    def summary(self):
+        out = np.zeros(32, np.float64)
+        check(lib().gwm_budget_sums(self.h, _ptr(out)))
+        return dict(zip(SUMMARY_KEYS, out.tolist()))
+
+    # initial states
+    def init_constant(self, rho, theta, u=0.0, v=0.0, w=0.0):
+        check(lib().gwm_init_constant(self.h, rho, theta, u, v, w))
+
+    def init_hydrostatic_rest(self, rho_surface, theta_ref):
+        check(lib().gwm_init_hydrostatic_rest(self.h, rho_surface, theta_ref))
+
+    def init_bubble(self, rho_background=1.0, theta_background=300.0, u=0.0, v=0.0, w=0.0,
+                    center=(0.5, 0.5, 0.5), radius=(0.1, 0.1, 0.1), theta_perturbation=2.0,
+                    sign=+1):
+        p = np.array([rho_background, theta_background, u, v, w, *center, *radius,
+                      theta_perturbation], np.float64)
+        check(lib().gwm_init_bubble(self.h, _ptr(p), int(sign)))
+
+    def init_mountain_wave(self, rho_surface=1.18, theta_ref=300.0, u=15.0, v=0.0, w=0.0):
+        check(lib().gwm_init_mountain_wave(self.h, rho_surface, theta_ref, u, v, w))
+
+    def init_constant_n(self, theta_surface=288.0, brunt_vaisala=0.01, p_surface=1.0e5, u=10.0,
+                        v=0.0):
+        check(lib().gwm_init_constant_n(self.h, theta_surface, brunt_vaisala, p_surface, u, v))
+
+    def init_convection(self, p_surface=1.0e5, seed=0, noise_amplitude=0.5, noise_top_m=1000.0):
+        check(lib().gwm_init_convection(self.h, p_surface, seed, noise_amplitude, noise_top_m))

@@ -1,0 +1,286 @@
+/*
+ * gwm_hip.h -- C-ABI of the MI355X (gfx950) dry + warm-rain time step.
+ *
+ * This is the drop-in boundary for the reference's operator and state
+ * contracts (FahrenheitResearch/gpu-wm-next).  The reference exposes a C++
+ * API over CUDA managed memory; the C++ host mirror in
+ * gpu-wm-next_amd/host/ restores those signatures on top of this ABI.  Each
+ * entry point cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain C types only.  Every function returns 0 on success and a nonzero
+ *    GWM_E_* code on failure; gwm_last_error() then returns a message
+ *    (thread-local), mirroring gwm::require / GWM_CUDA_CHECK
+ *    (include/gwm/core/types.hpp:24-28, include/gwm/core/cuda_utils.hpp:10-24).
+ *  - One context per host thread, one HIP stream per context.  Calls are
+ *    stream-ordered; gwm_sync() waits.  Uploads and downloads are
+ *    synchronous.
+ *  - Device fields use exactly the reference storage layout
+ *    (include/gwm/state/field3d.hpp:49-54):
+ *      idx = (k*(ny+2h) + j+h)*(nx+2h) + i+h, x fastest, no halo in k,
+ *    with C-grid faces sized (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1)
+ *    (include/gwm/state/face_field.hpp:16-23).  Host arrays passed to
+ *    upload/download use that same layout, halo included.
+ *  - One GPU holds the whole horizontal domain (a 1x1 layout).  A virtual-
+ *    rank split of the reference is tiling only; gwm_halo_exchange_layout()
+ *    reproduces the multi-subdomain halo semantics bit-exactly for callers
+ *    that hold per-rank arrays.
+ */
+#ifndef GWM_HIP_H
+#define GWM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  GWM_OK = 0,
+  GWM_E_INVALID = 1,   /* bad argument or shape mismatch (gwm::require) */
+  GWM_E_HIP = 2,       /* HIP runtime error (GWM_CUDA_CHECK) */
+  GWM_E_NOMEM = 3,
+  GWM_E_UNSUPPORTED = 4
+};
+
+/* ---- precision --------------------------------------------------------- */
+enum { GWM_F32 = 0, GWM_F64 = 1 };
+
+/* ---- field identifiers (state, scratch outputs, tracers) --------------- */
+enum {
+  GWM_RHO = 0,          /* rho_d                    DryState::rho_d        */
+  GWM_RHO_THETA = 1,    /* rho_d * theta_m          DryState::rho_theta_m  */
+  GWM_MOM_U = 2,        /* x-face mass flux         DryState::mom_u        */
+  GWM_MOM_V = 3,        /* y-face mass flux         DryState::mom_v        */
+  GWM_MOM_W = 4,        /* z-face mass flux         DryState::mom_w        */
+  GWM_PRESSURE = 5,     /* output of gwm_pressure()                         */
+  GWM_TEND_RHO = 6,     /* DrySlowTendencies (include/gwm/dycore/dry_core.hpp:36-48) */
+  GWM_TEND_RHO_THETA = 7,
+  GWM_TEND_U = 8,
+  GWM_TEND_V = 9,
+  GWM_TEND_W = 10,
+  GWM_PRECIP = 11,      /* surface bucket, (nx, ny) float, no halo, kg m-2 */
+  GWM_TRACER0 = 16      /* tracer t is GWM_TRACER0 + t (rho_d * q)         */
+};
+
+/* Warm-rain tracer order (include/gwm/state/tracer_registry.hpp:57-63). */
+enum { GWM_QV = 0, GWM_QC = 1, GWM_QR = 2 };
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct gwm_ctx gwm_ctx;
+typedef struct gwm_grid gwm_grid;
+typedef struct gwm_state gwm_state;
+
+/* ---- configuration structs -------------------------------------------- */
+
+/* GridMetrics::make_hybrid_height (include/gwm/domain/grid_metrics.hpp:87-92)
+ * plus the RectilinearDomainConfig shape (idealized_domain_builder.hpp:12-31). */
+typedef struct {
+  int nx, ny, nz, halo;
+  int periodic_x, periodic_y;
+  double dx, dy;
+  double z_top;             /* used when z_interfaces is NULL */
+  double terrain_taper_eta; /* reference default 0.25 */
+} gwm_grid_desc;
+
+/* DryStepperConfig (include/gwm/dycore/dry_core.hpp:50-56). */
+typedef struct {
+  double dt;
+  int fast_substeps;
+  double gravity;
+  int fast_update_horizontal_momentum;
+  int fast_update_density;
+} gwm_step_config;
+
+/* WarmRainConfig (include/gwm/physics/warm_rain.hpp:14-23).  accretion_rate
+ * is an extension (0 reproduces the reference). */
+typedef struct {
+  double dt;
+  double condensation_relaxation;
+  double cloud_autoconversion_threshold;
+  double cloud_autoconversion_rate;
+  double rain_evaporation_rate;
+  double rain_terminal_velocity;
+  double sedimentation_layer_depth_m;
+  int enable_latent_heating;
+  double accretion_rate;
+} gwm_warm_rain_config;
+
+/* Boundary handling inside the stepper.  NONE == NullBoundaryUpdater
+ * (src/dycore/dry_core.cu:336-339); STRIPS == apply_reference_boundaries
+ * (src/dycore/boundary_conditions.cpp:111-161) against the state captured by
+ * gwm_state_capture_boundary(); RELAX is the relaxation-zone extension. */
+enum { GWM_BOUNDARY_NONE = 0, GWM_BOUNDARY_STRIPS = 1, GWM_BOUNDARY_RELAX = 2 };
+
+/* Host callback run once per RK stage, like BoundaryUpdater::apply
+ * (include/gwm/dycore/dry_core.hpp:58-64).  `stage` is a view of the stage
+ * state: gwm_state_download/upload and gwm_apply_boundary work on it. */
+typedef int (*gwm_boundary_fn)(gwm_state* stage, double sim_time, void* user);
+
+typedef struct {
+  int mode;             /* GWM_BOUNDARY_* */
+  double relax_width;   /* cells, RELAX only */
+  gwm_boundary_fn callback; /* optional, runs after `mode` */
+  void* user;
+} gwm_boundary_config;
+
+/* Summary slots written by gwm_budget_sums (double[GWM_SUMMARY_N]).  Sums are
+ * compensated (Neumaier) FP64 device reductions; keys follow
+ * summarize_dry_states (src/dycore/dry_diagnostics.cpp:11-96) and
+ * summarize_runtime_state (src/core/runtime_summary.cpp:27-178). */
+enum {
+  GWM_S_TOTAL_DRY_MASS = 0,
+  GWM_S_TOTAL_RHO_THETA = 1,
+  GWM_S_TOTAL_MOM_X = 2,
+  GWM_S_TOTAL_MOM_Y = 3,
+  GWM_S_TOTAL_MOM_Z = 4,
+  GWM_S_RHO_MIN = 5,
+  GWM_S_RHO_MAX = 6,
+  GWM_S_THETA_MIN = 7,
+  GWM_S_THETA_MAX = 8,
+  GWM_S_U_MIN = 9,
+  GWM_S_U_MAX = 10,
+  GWM_S_V_MIN = 11,
+  GWM_S_V_MAX = 12,
+  GWM_S_W_MIN = 13,
+  GWM_S_W_MAX = 14,
+  GWM_S_VAPOR_PATH = 15,
+  GWM_S_CLOUD_PATH = 16,
+  GWM_S_RAIN_PATH = 17,
+  GWM_S_PRECIP_SUM = 18,
+  GWM_S_PRECIP_MEAN = 19,
+  GWM_S_PRECIP_MAX = 20,
+  GWM_S_PRECIP_CELLS = 21,
+  GWM_S_PRECIPITATING_CELLS = 22,
+  GWM_S_TRACER_QMIN0 = 23, /* min/max mixing ratio of tracers 0..2 at 23..28 */
+  GWM_SUMMARY_N = 32
+};
+
+/* ---- context ------------------------------------------------------------ */
+const char* gwm_last_error(void);
+int gwm_ctx_create(int device, int precision, gwm_ctx** out);
+int gwm_ctx_destroy(gwm_ctx* ctx);
+int gwm_sync(gwm_ctx* ctx);
+int gwm_ctx_precision(const gwm_ctx* ctx);
+/* Per-kernel HIP-event timing on the context stream (off by default). */
+int gwm_ctx_set_kernel_timing(gwm_ctx* ctx, int enable);
+/* Kernel names are stable strings ("fast_substep", "slow_stage", ...).
+ * Returns launches and total milliseconds recorded since the last reset. */
+int gwm_ctx_kernel_stats(gwm_ctx* ctx, const char* name, long* launches, double* total_ms);
+int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx);
+
+/* ---- grid / metrics ------------------------------------------------------
+ * Replaces GridMetrics::make_hybrid_height (grid_metrics.hpp:87-239) and
+ * build_rectilinear_domain (src/domain/idealized_domain_builder.cpp:39-64).
+ * z_interfaces: NULL for the reference's uniform eta levels, or nz+1 flat
+ * interface heights (stretched-level extension).  terrain: NULL (flat) or
+ * nx*ny floats, x fastest.  The metric fields are built on the device. */
+int gwm_grid_create(gwm_ctx* ctx, const gwm_grid_desc* desc, const double* z_interfaces,
+                    const float* terrain, gwm_grid** out);
+int gwm_grid_destroy(gwm_grid* grid);
+/* Metric download, ids as in gwm_oracle.h: 0 z_interfaces, 1 z_centers,
+ * 2 inv_dz_cell, 3 inv_dz_face, 4 terrain, 5 slope_x, 6 slope_y (context
+ * precision, global (k*ny+j)*nx+i layout); 7..10 flat FP64 columns. */
+int gwm_grid_download_metric(gwm_grid* grid, int which, void* host);
+int gwm_grid_terrain_active(const gwm_grid* grid);
+
+/* Terrain builders of the product drivers: the reference's cosine mountain
+ * (idealized_domain_builder.cpp:14-35) and the Witch-of-Agnesi ridge
+ * h(x) = h0 / (1 + ((x - xc)/a)^2), uniform in y (extension). */
+int gwm_terrain_cosine_mountain(int nx, int ny, double cx, double cy, double hx, double hy,
+                                double height, float* out);
+int gwm_terrain_agnesi_ridge(int nx, int ny, double dx, double xc_m, double half_width_m,
+                             double height_m, float* out);
+/* Stretched flat interfaces: dz grows smoothly from dz_bottom to dz_top and
+ * the column is rescaled to end exactly at z_top (extension). */
+int gwm_stretched_levels(int nz, double dz_bottom, double dz_top, double z_top, double* out);
+
+/* ---- state ----------------------------------------------------------------
+ * DryState (include/gwm/dycore/dry_core.hpp:14-34) + TracerState
+ * (include/gwm/state/tracer_state.hpp:12-119) + WarmRainSurfaceAccumulation
+ * (include/gwm/physics/warm_rain.hpp:25-46), all on the device. */
+int gwm_state_create(gwm_ctx* ctx, const gwm_grid* grid, int n_tracers, gwm_state** out);
+int gwm_state_destroy(gwm_state* state);
+/* Element count of a field's storage (halo included). */
+long gwm_state_field_size(const gwm_state* state, int field);
+int gwm_state_upload(gwm_state* state, int field, const void* host);
+int gwm_state_download(gwm_state* state, int field, void* host);
+int gwm_state_fill(gwm_state* state, int field, double value);
+int gwm_state_copy(gwm_state* dst, const gwm_state* src);
+/* Keep a copy of the current state as the lateral-boundary reference. */
+int gwm_state_capture_boundary(gwm_state* state);
+
+/* ---- halo semantics -------------------------------------------------------
+ * HaloExchange::exchange_scalar / exchange_face and
+ * synchronize_owned_face_interfaces (include/gwm/comm/halo_exchange.hpp:65-91,
+ * src/comm/halo_exchange.cpp:148-285,563-620) for the 1x1 layout, in place on
+ * one state field (periodic copy or zero across an open side). */
+int gwm_halo_fill(gwm_state* state, int field);
+int gwm_sync_interfaces(gwm_state* state, int field);
+/* Multi-subdomain exchange for per-rank device arrays (virtual ranks):
+ * desc13 holds 13 ints per rank (rank, coord_x, coord_y, ranks_x, ranks_y,
+ * global_nx, global_ny, nz, halo, i_begin, i_end, j_begin, j_end);
+ * orientation 0 cell, 1 x-face, 2 y-face, 3 z-face.  fields[n] is a device
+ * pointer of context precision in the reference layout for rank n. */
+int gwm_halo_exchange_layout(gwm_ctx* ctx, int n_ranks, const int* desc13, int periodic_x,
+                             int periodic_y, int orientation, void* const* device_fields);
+/* Device allocation helpers for callers that hold raw per-rank arrays. */
+int gwm_device_alloc(gwm_ctx* ctx, size_t bytes, void** out);
+int gwm_device_free(gwm_ctx* ctx, void* ptr);
+int gwm_memcpy_h2d(gwm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gwm_memcpy_d2h(gwm_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- operators ------------------------------------------------------------ */
+/* compute_dry_pressure_fields (src/dycore/dry_pressure_gradient.cu:177-202)
+ * on the halo-filled state -> GWM_PRESSURE. */
+int gwm_pressure(gwm_state* state);
+/* compute_slow_tendencies (src/dycore/dry_core.cu:400-539) -> GWM_TEND_*. */
+int gwm_slow_tendencies(gwm_state* state, const gwm_step_config* cfg);
+/* apply_local_split_explicit_fast_modes (src/dycore/dry_fast_modes.cu:400-656). */
+int gwm_fast_modes(gwm_state* state, const gwm_step_config* cfg);
+/* advance_dry_state_ssprk3 (src/dycore/dry_core.cu:541-651). */
+int gwm_ssprk3_step(gwm_state* state, const gwm_step_config* cfg,
+                    const gwm_boundary_config* boundary);
+/* advance_passive_tracers_ssprk3 (src/dycore/dry_transport.cu:440-518). */
+int gwm_tracers_ssprk3(gwm_state* state, const gwm_step_config* cfg, int boundary_mode);
+/* apply_warm_rain_microphysics, both overloads (include/gwm/physics/warm_rain.hpp:57-68):
+ * metric_fallout = 0 -> nominal fallout (warm_rain.cu:293-350),
+ * metric_fallout = 1 -> layout/metric overload (warm_rain.cu:352-432). */
+int gwm_warm_rain(gwm_state* state, const gwm_warm_rain_config* cfg, int metric_fallout);
+/* apply_reference_boundaries (boundary_conditions.cpp:111-207) / relaxation. */
+int gwm_apply_boundary(gwm_state* state, int mode, double relax_width);
+/* One step of gwm_prepared_case_driver's loop (gwm_prepared_case_driver.cpp:427-463):
+ * dry SSPRK3, tracer SSPRK3, warm rain (metric fallout). */
+int gwm_moist_step(gwm_state* state, const gwm_step_config* cfg,
+                   const gwm_warm_rain_config* wr, const gwm_boundary_config* boundary);
+/* Diagnostics with compensated FP64 device sums. */
+int gwm_budget_sums(gwm_state* state, double* out);
+
+/* ---- initial states (host-built, uploaded) ---------------------------------
+ * make_constant_dry_state (dry_core.cu:348-361), make_hydrostatic_rest_state
+ * (dry_core.cu:363-398), make_warm_bubble_state / make_density_current_state
+ * (idealized_cases.cpp:105-180), make_mountain_wave_background_state
+ * (idealized_cases.cpp:182-223).  Bubble parameters p[12] = rho_background
+ * (<= 0: hydrostatic background with rho_surface = -p[0]), theta_background,
+ * u, v, w, center x/y/z fractions, radius x/y/z fractions, theta perturbation. */
+int gwm_init_constant(gwm_state* state, double rho, double theta, double u, double v, double w);
+int gwm_init_hydrostatic_rest(gwm_state* state, double rho_surface, double theta_ref);
+int gwm_init_bubble(gwm_state* state, const double* p12, int sign);
+int gwm_init_mountain_wave(gwm_state* state, double rho_surface, double theta_ref, double u,
+                           double v, double w);
+/* Extensions for the benchmark cases: a constant-N sounding
+ * theta = theta_s exp(N^2 z / g) over terrain with uniform wind, and the
+ * warm-rain convection sounding (theta = 300 + 43 (z/12km)^1.25 below 12 km,
+ * isothermal above; RH = 1 - 0.75 (z/12km)^1.25, qv <= 14 g/kg) with seeded
+ * uniform theta noise of +/- amplitude below noise_top_m. */
+int gwm_init_constant_n(gwm_state* state, double theta_surface, double brunt_vaisala,
+                        double p_surface, double u, double v);
+int gwm_init_convection(gwm_state* state, double p_surface, unsigned long long seed,
+                        double noise_amplitude, double noise_top_m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

