@@ -552,9 +552,11 @@ void run_boundary(gwm_state* S, void** fs, const gwm_boundary_config* b, double 
   }
 }
 
-// advance_dry_state_ssprk3 (dry_core.cu:541-651)
+// advance_dry_state_ssprk3 (dry_core.cu:541-651).  fast_fn != null replaces
+// the device fast modes with a host plug-in run on each stage state.
 template <class R>
-void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b) {
+void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b,
+            gwm_boundary_fn fast_fn = nullptr, void* fast_user = nullptr) {
   const bool bnd = b && (b->mode != GWM_BOUNDARY_NONE || b->callback);
   S->ensure_sets(bnd ? 5 : 4);
   const int X = S->cur;
@@ -576,6 +578,33 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
   }
   auto sub = [&](int y) {
     void** ys = S->set[y];
+    if (fast_fn) {
+      // Host plug-in: hand it the stage state (rho, rho*theta, faces) in Z0.
+      void** zs = S->set[Z0];
+      const Geo& g = S->g();
+      for (int f = 0; f < 5; ++f) {
+        Timed t(S->ctx, "copy");
+        gwm::launch_copy<R>(S->ctx->stream, static_cast<const R*>(ys[f]), static_cast<R*>(zs[f]),
+                            S->fsize(f));
+      }
+      {
+        // ys[1] holds theta; rebuild rho*theta = rho*theta (K10) for the view.
+        Timed t(S->ctx, "theta");
+        gwm::launch_rebuild_rt<R>(S->ctx->stream, g, static_cast<const R*>(ys[0]),
+                                  static_cast<const R*>(ys[1]), static_cast<R*>(zs[1]));
+      }
+      gwm_state view;
+      view.ctx = S->ctx;
+      view.grid = S->grid;
+      view.ntr = S->ntr;
+      view.view = true;
+      view.owner = S;
+      view.faces_synced = true;
+      for (int f = 0; f < 5; ++f) view.vset[f] = zs[f];
+      HIPCHK(hipStreamSynchronize(S->ctx->stream));
+      require(fast_fn(&view, 0.0, fast_user) == 0, "fast-mode callback failed");
+      return Z0;
+    }
     return subcycle<R>(S, c, static_cast<const R*>(ys[0]), static_cast<const R*>(ys[2]),
                        static_cast<const R*>(ys[3]), static_cast<const R*>(ys[4]),
                        static_cast<const R*>(ys[1]), static_cast<const R*>(ys[2]),
@@ -1612,6 +1641,20 @@ int gwm_ssprk3_step(gwm_state* s, const gwm_step_config* c, const gwm_boundary_c
     require(!s->view, "operator not available on a stage view");
     dispatch(s->ctx, [&](auto r) {
       ssprk3<decltype(r)>(s, *c, b);
+      return 0;
+    });
+  });
+}
+
+int gwm_ssprk3_step_custom_fast(gwm_state* s, const gwm_step_config* c,
+                                const gwm_boundary_config* b, gwm_boundary_fn fast_fn,
+                                void* fast_user) {
+  return guard([&] {
+    S_(s);
+    require(c && fast_fn, "null argument");
+    require(!s->view, "operator not available on a stage view");
+    dispatch(s->ctx, [&](auto r) {
+      ssprk3<decltype(r)>(s, *c, b, fast_fn, fast_user);
       return 0;
     });
   });

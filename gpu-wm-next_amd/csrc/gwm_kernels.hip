@@ -505,6 +505,17 @@ __global__ void k_theta(Geo g, const R* rho, const R* rt, R* th) {
   th[c] = d_theta(rho[c], rt[c]);
 }
 
+// rho_theta = rho * theta over owned cells (K10, dry_fast_modes.cu:97-106)
+template <class R>
+__global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const size_t c = g.cell(i, j, k);
+  rt[c] = rho[c] * th[c];
+}
+
 // ===========================================================================
 // Passive tracers, one SSPRK3 stage for all tracers (K23,
 // dry_transport.cu:49-125 + blend 40-47) with frozen rho and face momenta.
@@ -971,6 +982,11 @@ void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th
   hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th);
 }
 template <class R>
+void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt) {
+  const dim3 b(64, 4, 1);
+  hipLaunchKernelGGL(k_rebuild_rt<R>, cell_grid(g, b), b, 0, st, g, rho, th, rt);
+}
+template <class R>
 void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain) {
   const dim3 b(64, 4, 1);
   if (terrain)
@@ -1044,6 +1060,7 @@ void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
   template void launch_fast_substep<R>(hipStream_t, const FastArgs<R>&, bool);                   \
   template void launch_theta<R>(hipStream_t, const Geo&, const R*, const R*, R*);                \
   template void launch_tracer_stage<R>(hipStream_t, const TracerArgs<R>&, bool);                 \
+  template void launch_rebuild_rt<R>(hipStream_t, const Geo&, const R*, const R*, R*);           \
   template void launch_warm_rain<R>(hipStream_t, const WarmRainArgs<R>&, bool);                  \
   template void launch_halo_fill<R>(hipStream_t, const Geo&, int, R*);                           \
   template void launch_sync_interfaces<R>(hipStream_t, const Geo&, int, R*);                     \

@@ -96,6 +96,8 @@ void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th
 template <class R>
 void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
 template <class R>
+void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt);
+template <class R>
 void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain);
 template <class R>
 void launch_halo_fill(hipStream_t st, const Geo& g, int orient, R* f);

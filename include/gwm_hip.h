@@ -242,6 +242,13 @@ int gwm_fast_modes(gwm_state* state, const gwm_step_config* cfg);
 /* advance_dry_state_ssprk3 (src/dycore/dry_core.cu:541-651). */
 int gwm_ssprk3_step(gwm_state* state, const gwm_step_config* cfg,
                     const gwm_boundary_config* boundary);
+/* advance_dry_state_ssprk3 with a host FastModeIntegrator plug-in
+ * (include/gwm/dycore/dry_core.hpp:73-80): fast_fn runs on each stage state
+ * instead of the device fast modes (a callback that does nothing reproduces
+ * the prepared driver's NullFastModeIntegrator). */
+int gwm_ssprk3_step_custom_fast(gwm_state* state, const gwm_step_config* cfg,
+                                const gwm_boundary_config* boundary, gwm_boundary_fn fast_fn,
+                                void* fast_user);
 /* advance_passive_tracers_ssprk3 (src/dycore/dry_transport.cu:440-518). */
 int gwm_tracers_ssprk3(gwm_state* state, const gwm_step_config* cfg, int boundary_mode);
 /* apply_warm_rain_microphysics, both overloads (include/gwm/physics/warm_rain.hpp:57-68):

@@ -1,0 +1,131 @@
+"""The C++ host mirror and the idealized driver.
+
+CPU: the binaries are built and resolve libgwm_hip.so; the driver rejects
+bad flags like the reference driver (src/core/gwm_idealized_driver.cpp:70-190).
+GPU: the C++ mirror tests (tests/cpp/test_mirror_*.cpp) pass, and the driver's
+summary JSON matches the oracle on the same case to the six digits the JSON
+carries.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "gpu-wm-next_amd", "host", "bin")
+DRIVER = os.path.join(BIN, "gwm_idealized_driver")
+CPP_TESTS = sorted(f[:-4] for f in os.listdir(os.path.join(REPO, "tests", "cpp"))
+                   if f.startswith("test_") and f.endswith(".cpp"))
+
+
+def _need(path):
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (run __graft_entry__.build())")
+
+
+def test_binaries_link_against_the_hip_library():
+    for name in ["gwm_idealized_driver"] + CPP_TESTS:
+        path = os.path.join(BIN, name)
+        _need(path)
+        out = subprocess.run(["ldd", path], capture_output=True, text=True, check=True).stdout
+        line = [ln for ln in out.splitlines() if "libgwm_hip.so" in ln]
+        assert line and "not found" not in line[0], out
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--bogus", "1"], "Unknown argument: --bogus"),
+    (["--nx"], "Missing value for --nx"),
+    (["--periodic-x", "maybe"], "Invalid boolean value: maybe"),
+    (["--terrain-kind", "witch"], "Unsupported terrain kind: witch"),
+])
+def test_driver_rejects_bad_flags(args, msg):
+    _need(DRIVER)
+    r = subprocess.run([DRIVER] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CPP_TESTS)
+def test_cpp_mirror(name):
+    path = os.path.join(BIN, name)
+    _need(path)
+    print(f"running {name}", flush=True)
+    r = subprocess.run([path], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+CASES = {
+    "warm_bubble": dict(
+        args=["--case", "warm_bubble", "--nx", "24", "--ny", "20", "--nz", "10", "--dx", "500",
+              "--dy", "500", "--z-top", "10000", "--rho-background", "1.16",
+              "--center-z-fraction", "0.2", "--radius-x-fraction", "0.2",
+              "--radius-y-fraction", "0.2", "--radius-z-fraction", "0.2", "--dt", "2",
+              "--fast-substeps", "3", "--steps", "3"],
+        world=dict(nx=24, ny=20, nz=10, dx=500.0, dy=500.0, z_top=10000.0),
+        init=lambda w: w.init_bubble(1.16, 300.0, center=(0.5, 0.5, 0.2),
+                                     radius=(0.2, 0.2, 0.2), theta_perturbation=2.0),
+        step=dict(dt=2.0, fast_substeps=3)),
+    "mountain_wave_background": dict(
+        args=["--case", "mountain_wave_background", "--nx", "32", "--ny", "8", "--nz", "12",
+              "--dx", "1000", "--dy", "1000", "--z-top", "12000", "--terrain-kind",
+              "cosine_mountain", "--terrain-height", "400", "--terrain-half-width-x", "0.2",
+              "--terrain-half-width-y", "0.5", "--u-background", "10", "--dt", "1",
+              "--fast-substeps", "4", "--steps", "3"],
+        world=dict(nx=32, ny=8, nz=12, dx=1000.0, dy=1000.0, z_top=12000.0),
+        terrain=dict(cx=0.5, cy=0.5, hx=0.2, hy=0.5, height=400.0),
+        init=lambda w: w.init_mountain_wave(1.18, 300.0, 10.0, 0.0, 0.0),
+        step=dict(dt=1.0, fast_substeps=4)),
+}
+
+KEYS = [("total_dry_mass", "total_dry_mass"), ("total_rho_theta_m", "total_rho_theta_m"),
+        ("total_horizontal_momentum_x", "total_horizontal_momentum_x"),
+        ("total_vertical_momentum", "total_vertical_momentum")]
+RANGES = [("rho_d", "rho"), ("theta_m", "theta"), ("u_face", "u"), ("w_face", "w")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_driver_summary_matches_oracle(orc, case, precision):
+    import gwm_amd as G
+
+    _need(DRIVER)
+    spec = CASES[case]
+    print(f"driver {case} {precision}", flush=True)
+    r = subprocess.run([DRIVER, "--precision", precision] + spec["args"], capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout)
+    assert got["case"] == case and got["steps"] == 3
+
+    terrain = None
+    if "terrain" in spec:
+        t = spec["terrain"]
+        terrain = G.cosine_mountain(spec["world"]["nx"], spec["world"]["ny"], t["height"],
+                                    t["cx"], t["cy"], t["hx"], t["hy"])
+    w = orc.World(**spec["world"], terrain=terrain, precision=8 if precision == "fp64" else 4)
+    spec["init"](w)
+    initial = w.summary()
+    for _ in range(3):
+        w.ssprk3_step(orc.step_cfg(**spec["step"]))
+    final = w.summary()
+    # The JSON carries six significant digits (rel 5e-7).  FP32 adds the
+    # device/host FP32 trajectory difference, up to 2e-4 of a field's range
+    # after a few steps (the FP32 multi-step bound of the parity tests).
+    rel = 2e-6 if precision == "fp64" else 5e-4
+    for part, ref in (("initial", initial), ("final", final)):
+        s = got[part]
+        # Momentum totals can be pure round-off around zero: bound them by
+        # the momentum a 1 m/s wind would carry.
+        mom_floor = rel * abs(ref["total_dry_mass"])
+        for jk, ok in KEYS:
+            floor = mom_floor if "momentum" in jk else 0.0
+            assert abs(s[jk] - ref[ok]) <= rel * abs(ref[ok]) + floor, (part, jk, s[jk], ref[ok])
+        for jk, ok in RANGES:
+            spread = max(abs(ref[f"{ok}_max"]), abs(ref[f"{ok}_min"]), 1e-3)
+            for b in ("min", "max"):
+                assert abs(s[jk][b] - ref[f"{ok}_{b}"]) <= rel * spread, (part, jk, b)
+    assert np.isfinite(got["final"]["total_dry_mass"])
