@@ -67,8 +67,8 @@ def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0):
     secs = time.perf_counter() - t0
     stats = {}
     if timing:
-        for name in ("fast_substep", "slow_stage", "level_partials", "level_finalize",
-                     "tracer_stage", "warm_rain", "column_reference"):
+        for name in ("fast_column", "fast_substep", "slow_stage", "level_partials",
+                     "level_finalize", "tracer_stage", "warm_rain", "column_reference"):
             n, ms = ctx.kernel_stats(name)
             if n:
                 stats[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n}
@@ -118,8 +118,11 @@ def main():
     cells = a.nx * a.ny * a.nz
     import math
 
-    secs, stats, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, a.dt)
+    secs, _, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, a.dt)
     value = cells * a.steps / secs
+    # Per-kernel HIP-event timings come from a second pass, so the events do
+    # not sit inside the timed region above.
+    _, stats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, a.dt)
     finite = all(math.isfinite(summary[k]) for k in ("w_min", "w_max", "total_dry_mass"))
     control = None
     if a.control_dt > 0:
@@ -133,7 +136,8 @@ def main():
     # Dominant kernel: the fast substep.  Algorithmic elements per cell per
     # launch: read rho, theta, u, v, w, u_base, v_base, w_base (8), write rho,
     # u, v, w (4), plus rho*theta on the last substep of each subcycle.
-    fs = stats.get("fast_substep")
+    kname = "fast_column" if "fast_column" in stats else "fast_substep"
+    fs = stats.get(kname)
     roofline = None
     if fs:
         elems = (12 * (ns - 1) + 13) / ns
@@ -141,7 +145,7 @@ def main():
         achieved = bytes_per_launch / (fs["avg_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
-                    "kernel": "fast_substep", "avg_ms": round(fs["avg_ms"], 5),
+                    "kernel": kname, "avg_ms": round(fs["avg_ms"], 5),
                     "algorithmic_bytes": int(bytes_per_launch)}
     fp64 = None
     if a.fp64_steps > 0:

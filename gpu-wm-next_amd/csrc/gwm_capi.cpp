@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <map>
@@ -85,6 +87,14 @@ struct gwm_ctx {
   size_t partials_n = 0;
   int refs = 0;            // live grids and states
   bool closing = false;    // gwm_ctx_destroy called while refs > 0
+  // Stencil kernels: column march (default) or the point kernels
+  // (GWM_KERNELS=point, kept for comparison).
+  bool fast_point = false;
+  // Level chunks per column of the column kernels: chosen per (kernel,
+  // shape, precision) by timing the candidates at first use (autotuner), or
+  // fixed by gwm_ctx_set_chunks / GWM_CHUNKS.
+  int force_chunks = 0;
+  std::map<std::string, int> tuned;
 
   size_t esize() const { return prec == GWM_F64 ? 8 : 4; }
   hipEvent_t ev() {
@@ -403,6 +413,42 @@ void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
   }
 }
 
+// Autotuner for the level-chunk count of a column kernel.  `launch(n)` runs
+// the kernel with n chunks; the kernels are out-of-place, so the candidates
+// can be run on the live arguments and every candidate writes identical
+// values.  Each candidate is timed over two launches after a warm-up.
+template <class Launch>
+int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Launch&& launch) {
+  if (ctx->force_chunks > 0) return ctx->force_chunks;
+  char key[160];
+  std::snprintf(key, sizeof key, "%s:%dx%dx%d:%s:%d", kname, g.nx, g.ny, g.nz,
+                ctx->prec == GWM_F64 ? "f64" : "f32", variant);
+  auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) return it->second;
+  hipEvent_t e0 = ctx->ev(), e1 = ctx->ev();
+  int best = 1;
+  float best_ms = 1e30f;
+  for (int n : {1, 2, 4, 8}) {
+    if (n > 1 && g.nz / n < 4) continue;
+    launch(n);
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    launch(n);
+    launch(n);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = n;
+    }
+  }
+  ctx->pool.push_back(e0);
+  ctx->pool.push_back(e1);
+  ctx->tuned[key] = best;
+  return best;
+}
+
 // One fast subcycle (apply_local_split_explicit_fast_modes) from
 // (rho_in, faces_in) with fixed theta and base faces; returns the set index
 // holding the result (rho, rho*theta, faces).  out sets are a[0], a[1].
@@ -418,7 +464,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   if (!colref) {
     if (!S->rref_lev) S->rref_lev = dalloc(g.nz * sizeof(R));
     if (!S->pref_lev) S->pref_lev = dalloc(g.nz * sizeof(R));
-  } else {
+  } else if (S->ctx->fast_point) {
     if (!S->rref3) S->rref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
     if (!S->pref3) S->pref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
   }
@@ -426,20 +472,26 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   const R* rho = rho_in;
   const R *u = u_in, *v = v_in, *w = w_in;
   int last = z0;
+  // Level partials: substep 0 from a pass over the entry state, later
+  // substeps from the previous substep's column kernel.
+  const int nwave = gwm::fast_column_parts(g.nx, g.ny);
+  const int nrow = (g.ny + 7) / 8;
+  double* part = nullptr;
+  if (!colref)
+    part = S->ctx->scratch(static_cast<size_t>(g.nz) * std::max(nwave, nrow) * 2);
+  int nparts = 0;
   for (int s = 0; s < ns; ++s) {
     const int o = outs[s % 2];
     if (!colref) {
-      double* part = S->ctx->scratch(static_cast<size_t>(g.nz) * ((g.ny + 7) / 8) * 2);
-      int nch = 0;
-      {
+      if (s == 0 || S->ctx->fast_point) {
         Timed t(S->ctx, "level_partials");
-        gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nch);
+        gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nparts);
       }
       Timed t(S->ctx, "level_finalize");
-      gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nch, part, 1,
+      gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nparts, part, 1,
                                     static_cast<double>(g.nx) * g.ny,
                                     static_cast<R*>(S->rref_lev), static_cast<R*>(S->pref_lev));
-    } else {
+    } else if (S->ctx->fast_point) {
       Timed t(S->ctx, "column_reference");
       gwm::launch_column_reference<R>(S->ctx->stream, g, rho, theta,
                                       static_cast<const R*>(G->zc_m), static_cast<R>(c.gravity),
@@ -470,9 +522,20 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     a.horizontal = c.fast_update_horizontal_momentum;
     a.density = c.fast_update_density;
     a.col_ref = colref;
-    {
+    if (S->ctx->fast_point) {
       Timed t(S->ctx, "fast_substep");
       gwm::launch_fast_substep<R>(S->ctx->stream, a, G->terrain_nonzero);
+    } else {
+      const bool emit = !colref && s + 1 < ns;
+      auto run = [&](int n) {
+        gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
+                                   emit ? part : nullptr, emit ? &nparts : nullptr);
+      };
+      const int nkc = colref ? 1
+                             : pick_chunks(S->ctx, "fast_column", g,
+                                           G->terrain_nonzero ? 1 : 0, run);
+      Timed t(S->ctx, "fast_column");
+      run(nkc);
     }
     rho = a.rho_out;
     u = a.u_out;
@@ -501,8 +564,18 @@ void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* c
   a.wb = wb;
   a.mode = mode;
   a.out_theta = out_theta;
+  if (S->ctx->fast_point) {
+    Timed t(S->ctx, "slow_stage");
+    gwm::launch_slow_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+    return;
+  }
+  auto run = [&](int n) {
+    gwm::launch_slow_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
+  };
+  const int nkc = pick_chunks(S->ctx, "slow_column", g,
+                              (S->grid->terrain_nonzero ? 1 : 0) + 2 * mode, run);
   Timed t(S->ctx, "slow_stage");
-  gwm::launch_slow_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+  run(nkc);
 }
 
 int orient_of(int f) {
@@ -708,8 +781,18 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
     a.mode = mode;
     a.wa = wa;
     a.wb = wb;
+    if (S->ctx->fast_point) {
+      Timed tt(S->ctx, "tracer_stage");
+      gwm::launch_tracer_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+      return;
+    }
+    auto run = [&](int n) {
+      gwm::launch_tracer_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
+    };
+    const int nkc = pick_chunks(S->ctx, "tracer_column", g,
+                                (S->grid->terrain_nonzero ? 1 : 0) + 2 * S->ntr, run);
     Timed tt(S->ctx, "tracer_stage");
-    gwm::launch_tracer_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+    run(nkc);
   };
   stage(in1, A, 1, R(0), R(1));
   strips(A);
@@ -1167,6 +1250,10 @@ int gwm_ctx_create(int device, int precision, gwm_ctx** out) {
     auto c = std::make_unique<gwm_ctx>();
     c->device = device;
     c->prec = precision;
+    const char* fk = std::getenv("GWM_KERNELS");
+    c->fast_point = fk != nullptr && std::string(fk) == "point";
+    const char* ch = std::getenv("GWM_CHUNKS");
+    if (ch != nullptr) c->force_chunks = std::max(0, std::atoi(ch));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
   });
@@ -1222,6 +1309,26 @@ int gwm_ctx_kernel_stats(gwm_ctx* ctx, const char* name, long* launches, double*
   });
 }
 
+int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks) {
+  return guard([&] {
+    require(ctx, "null context");
+    require(chunks >= 0, "chunks must be >= 0");
+    ctx->force_chunks = chunks;
+  });
+}
+
+int gwm_ctx_tuning_report(gwm_ctx* ctx, char* buf, size_t size) {
+  return guard([&] {
+    require(ctx && buf && size > 0, "null argument");
+    std::string r;
+    for (const auto& kv : ctx->tuned) {
+      if (!r.empty()) r += ";";
+      r += kv.first + "=" + std::to_string(kv.second);
+    }
+    std::snprintf(buf, size, "%s", r.c_str());
+  });
+}
+
 int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx) {
   return guard([&] {
     require(ctx, "null context");
@@ -1238,6 +1345,11 @@ int gwm_grid_create(gwm_ctx* ctx, const gwm_grid_desc* d, const double* z_interf
     require(d->halo >= 1, "halo must be at least 1");
     require(d->dx > 0 && d->dy > 0, "grid spacing must be positive");
     require(d->halo <= d->nx && d->halo <= d->ny, "halo wider than the domain");
+    {
+      const double elems = static_cast<double>(d->nx + 1 + 2 * d->halo) *
+                           (d->ny + 1 + 2 * d->halo) * (d->nz + 1);
+      require(elems < 2147483647.0, "grid too large: a field must hold fewer than 2^31 elements");
+    }
     auto G = std::make_unique<gwm_grid>();
     require(!ctx->closing, "context already destroyed");
     G->ctx = ctx;

@@ -1,0 +1,333 @@
+// gwm_fast.hip -- column-marching fast substep for gfx950.
+//
+// One substep of apply_local_split_explicit_fast_modes
+// (src/dycore/dry_fast_modes.cu:495-652): K10 + K20 pressure from rho and the
+// fixed theta, K11 column hydrostatic reference (terrain), K12/K13
+// horizontal pressure gradient, K14 vertical perturbation branch, K15
+// density from the divergence of (m - m_base), and K10 rho*theta rebuild on
+// the last substep.  Arithmetic per value is the same as the point kernel in
+// gwm_kernels.hip (k_fast_substep + k_column_reference).
+//
+// Mapping.  A work-group of 64 x 4 threads owns a 64 x 4 column tile and
+// marches it upwards over a chunk of levels [k0, k1), one level per
+// iteration.  Pressure of the tile plus a one-cell ring is computed once per
+// cell and level into an LDS ring of four planes (levels k-1 .. k+2); the
+// plane two levels ahead and the face momenta one level ahead are loaded into
+// registers an iteration early, so their global loads overlap the current
+// level's arithmetic.  The w face above the cell and the column reference are
+// carried in registers, so each face is computed once.  One barrier per
+// level.  Splitting the column into chunks (chosen by the autotuner) gives
+// the chip more independent work-groups; a chunk recomputes its bottom w face
+// from the level below.  The column hydrostatic reference is a recurrence
+// from the ground, so terrain runs keep whole columns.  Work-group b is
+// remapped so that each XCD (which receives every eighth work-group) owns a
+// contiguous band of tiles and the ring rows shared by neighbouring tiles hit
+// the same L2.
+//
+// Flat terrain: the level means of rho and p for the next substep are
+// emitted as per-wave FP64 partial sums of the new rho (fixed order: a wave
+// butterfly, lane 0 writes), replacing a separate pass over the state.
+#include "gwm_kernels.hpp"
+
+namespace gwm {
+
+namespace {
+
+constexpr int TX = 64, TY = 4, PX = TX + 2, PY = TY + 2, PN = PX * PY;
+constexpr int NHALO = PN - TX * TY;  // 140 ring cells
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace
+
+template <class R, bool TERRAIN, bool COLREF>
+__global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
+                                                     int nkc, double* part_out) {
+  __shared__ R sp[4][PN];
+  const Geo g = a.g;
+  const Loader<R> L{g};
+  const int nz = g.nz;
+
+  // XCD-aware order (work-group b runs on XCD b % 8), then (chunk, tile).
+  const int nb = tiles * nkc;
+  int b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int chunk = b / tiles, tile = b % tiles;
+  const int k0 = (nz * chunk) / nkc, k1 = (nz * (chunk + 1)) / nkc;
+  const int ti0 = (tile % tiles_x) * TX, tj0 = (tile / tiles_x) * TY;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int t = lx + TX * ly;
+  const int i = ti0 + lx, j = tj0 + ly;
+  const bool active = i < g.nx && j < g.ny;
+  const int ci = (lx + 1) + PX * (ly + 1);
+
+  // Ring cell of this thread (t < NHALO): rows -1 and TY, then columns -1 and TX.
+  int hx = 0, hy = 0;
+  if (t < 2 * PX) {
+    hx = (t % PX) - 1;
+    hy = t < PX ? -1 : TY;
+  } else if (t < NHALO) {
+    const int q = t - 2 * PX;
+    hx = (q & 1) ? TX : -1;
+    hy = q >> 1;
+  }
+  const int hci = (hx + 1) + PX * (hy + 1);
+
+  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R dtf = a.dtf, grav = a.gravity;
+
+  // Raw loads of one plane: own cell and (t < NHALO) one ring cell.
+  R rr_c, rt_c, rr_h = R(0), rt_h = R(0);
+  auto load_plane = [&](int k) {
+    rr_c = L.cell(a.rho, i, j, k);
+    rt_c = L.cell(a.theta, i, j, k);
+    if (t < NHALO) {
+      rr_h = L.cell(a.rho, ti0 + hx, tj0 + hy, k);
+      rt_h = L.cell(a.theta, ti0 + hx, tj0 + hy, k);
+    }
+  };
+  auto stage_plane = [&](int k) {
+    sp[k & 3][ci] = d_pressure(rr_c, rr_c * rt_c);
+    if (t < NHALO) sp[k & 3][hci] = d_pressure(rr_h, rr_h * rt_h);
+  };
+
+  // Face momenta of one level for the own cell (prefetched a level ahead).
+  R f_u0 = 0, f_u1 = 0, f_v0 = 0, f_v1 = 0, f_w1 = 0, f_ub0 = 0, f_ub1 = 0, f_vb0 = 0,
+    f_vb1 = 0, f_wb1 = 0;
+  auto load_faces = [&](int k) {
+    const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k), zf1 = g.zf(i, j, k + 1);
+    f_u0 = a.u[xf0];
+    f_u1 = a.u[xf0 + 1];
+    f_v0 = a.v[yf0];
+    f_v1 = a.v[yf0 + g.pxc];
+    f_w1 = a.w[zf1];
+    f_ub0 = a.ub[xf0];
+    f_ub1 = a.ub[xf0 + 1];
+    f_vb0 = a.vb[yf0];
+    f_vb1 = a.vb[yf0 + g.pxc];
+    f_wb1 = a.wb[zf1];
+  };
+
+  // Metric column index (periodic wrap of global i, j; dry_fast_modes.cu:24-30).
+  auto mcol = [&](int ii, int jj) -> size_t {
+    return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);
+  };
+  const size_t plane_m = static_cast<size_t>(g.nx) * g.ny;
+  const size_t own_m = active ? mcol(i, j) : 0;
+
+  // ---- prologue: planes k0-1 .. k0+1, own column values, bottom w face.
+  R rho_km1 = R(0);
+  if (k0 > 0) {
+    load_plane(k0 - 1);
+    stage_plane(k0 - 1);
+    rho_km1 = rr_c;
+  }
+  R rho_k, th_k, rho_k1 = R(0), th_k1 = R(0);
+  load_plane(k0);
+  stage_plane(k0);
+  rho_k = rr_c;
+  th_k = rt_c;
+  if (k0 + 1 < nz) {
+    load_plane(k0 + 1);
+    stage_plane(k0 + 1);
+    rho_k1 = rr_c;
+    th_k1 = rt_c;
+  }
+  if (k0 + 2 < nz && k0 + 2 <= k1) load_plane(k0 + 2);
+  if (active) load_faces(k0);
+  __syncthreads();
+
+  // Column reference (K11) at level k0 (whole columns only: k0 == 0).
+  R pref_k = R(0), rref_k = R(0);
+  if (COLREF) {
+    const R rt0 = rho_k * th_k;
+    pref_k = d_pressure(rho_k, rt0);
+    rref_k = d_rho_from_p_theta(pref_k, d_theta(rho_k, rt0));
+  }
+
+  const bool ow = !g.per_x, os = !g.per_y;
+  R w0 = R(0), wb0 = R(0);
+  if (active) {
+    const size_t zf0 = g.zf(i, j, k0);
+    w0 = a.w[zf0];
+    wb0 = a.wb[zf0];
+    if (k0 > 0) {
+      // K14 at face k0 from levels k0-1 and k0 (flat references).
+      const R ppl = sp[(k0 - 1) & 3][ci] - a.pref_lev[k0 - 1];
+      const R ppu = sp[k0 & 3][ci] - a.pref_lev[k0];
+      const R rpf = R(0.5) * ((rho_km1 - a.rref_lev[k0 - 1]) + (rho_k - a.rref_lev[k0]));
+      const R idz = TERRAIN ? a.m.idz_f[own_m + k0 * plane_m] : a.m.lev_idz_f[k0];
+      const R tend = -(ppu - ppl) * idz - grav * rpf;
+      w0 = w0 + dtf * tend;
+    }
+  }
+
+  for (int k = k0; k < k1; ++k) {
+    // Stage level k+2 (loaded last iteration) and start loading level k+3.
+    R rho_k2 = R(0), th_k2 = R(0);
+    if (k + 2 < nz && k + 2 <= k1) {
+      stage_plane(k + 2);
+      rho_k2 = rr_c;
+      th_k2 = rt_c;
+      if (k + 3 < nz && k + 3 <= k1) load_plane(k + 3);
+    }
+    // Faces of this level; start loading the next level's.
+    const R uc0 = f_u0, uc1 = f_u1, vc0 = f_v0, vc1 = f_v1, wc1 = f_w1;
+    const R ub0 = f_ub0, ub1 = f_ub1, vb0 = f_vb0, vb1 = f_vb1, wb1 = f_wb1;
+    if (active && k + 1 < k1) load_faces(k + 1);
+
+    const R* pk = sp[k & 3];
+    const R* pdn = sp[(k + 3) & 3];  // level k-1
+    const R* pup = sp[(k + 1) & 3];  // level k+1
+
+    // Column reference at level k+1 (K11 recurrence, own column).
+    R pref_k1 = R(0), rref_k1 = R(0);
+    if (COLREF && k + 1 < nz && active) {
+      const R thk = d_theta(rho_k1, rho_k1 * th_k1);
+      const R dz = a.m.zc[own_m + (k + 1) * plane_m] - a.m.zc[own_m + k * plane_m];
+      R pn = rmax(R(1), pref_k - grav * rref_k * dz);
+      for (int it = 0; it < 4; ++it) {
+        const R rn = d_rho_from_p_theta(pn, thk);
+        pn = pref_k - grav * R(0.5) * (rref_k + rn) * dz;
+        pn = rmax(pn, R(1));
+      }
+      pref_k1 = pn;
+      rref_k1 = d_rho_from_p_theta(pn, thk);
+    }
+
+    R rho_new = R(0);
+    if (active) {
+      // dp/dz at a neighbour column, from the LDS planes (dry_pressure_gradient.cu:37-73).
+      auto vgrad = [&](int off, size_t m) -> R {
+        if (nz == 1) return R(0);
+        const R* zc = a.m.zc;
+        if (k == 0) {
+          const R dz = zc[m + plane_m] - zc[m];
+          return (pup[off] - pk[off]) / rmax(dz, R(1.0e-6));
+        }
+        if (k + 1 == nz) {
+          const R dz = zc[m + k * plane_m] - zc[m + (k - 1) * plane_m];
+          return (pk[off] - pdn[off]) / rmax(dz, R(1.0e-6));
+        }
+        const R dz = zc[m + (k + 1) * plane_m] - zc[m + (k - 1) * plane_m];
+        return (pup[off] - pdn[off]) / rmax(dz, R(1.0e-6));
+      };
+      auto pgf = [&](int offl, int offr, int il, int jl, int ir, int jr, R d) -> R {
+        const R pl = pk[offl], pr = pk[offr];
+        if (!TERRAIN) return (pr - pl) / d;
+        const size_t ml = mcol(il, jl), mr = mcol(ir, jr);
+        const R dzd = (a.m.zc[mr + k * plane_m] - a.m.zc[ml + k * plane_m]) / d;
+        const R gl = vgrad(offl, ml), gr = vgrad(offr, mr);
+        return (pr - pl) / d - dzd * (R(0.5) * (gl + gr));
+      };
+      // K12 / K13: faces i, i+1 and j, j+1 of the cell.
+      R u0 = uc0, u1 = uc1, v0 = vc0, v1 = vc1;
+      if (a.horizontal) {
+        if (!(i == 0 && ow)) u0 = uc0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx));
+        if (!(i + 1 == g.nx && ow)) u1 = uc1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx));
+        if (!(j == 0 && os)) v0 = vc0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy));
+        if (!(j + 1 == g.ny && os)) v1 = vc1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy));
+      }
+      // K14: w face k+1 (faces 0 and nz are left as they are).
+      R w1 = wc1;
+      if (k + 1 < nz) {
+        R rl, pl, ru, pu;
+        if (COLREF) {
+          rl = rref_k; pl = pref_k; ru = rref_k1; pu = pref_k1;
+        } else {
+          rl = a.rref_lev[k]; pl = a.pref_lev[k]; ru = a.rref_lev[k + 1]; pu = a.pref_lev[k + 1];
+        }
+        const R ppl = pk[ci] - pl;
+        const R ppu = pup[ci] - pu;
+        const R rpf = R(0.5) * ((rho_k - rl) + (rho_k1 - ru));
+        const R idz = TERRAIN ? a.m.idz_f[own_m + (k + 1) * plane_m] : a.m.lev_idz_f[k + 1];
+        const R tend = -(ppu - ppl) * idz - grav * rpf;
+        w1 = wc1 + dtf * tend;
+      }
+      // K15
+      rho_new = rho_k;
+      if (a.density) {
+        const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : a.m.lev_idz_c[k];
+        const R div = ((u1 - ub1) - (u0 - ub0)) / dx + ((v1 - vb1) - (v0 - vb0)) / dy +
+                      ((w1 - wb1) - (w0 - wb0)) * idz;
+        rho_new = rmax(R(1.0e-6), rho_new - dtf * div);
+      }
+      const size_t c0 = g.cell(i, j, k);
+      const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k);
+      a.rho_out[c0] = rho_new;
+      if (a.rt_out) a.rt_out[c0] = rho_new * th_k;
+      a.u_out[xf0] = u0;
+      a.v_out[yf0] = v0;
+      a.w_out[c0] = w0;
+      if (i == g.nx - 1) a.u_out[xf0 + 1] = u1;
+      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = v1;
+      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = w1;
+      w0 = w1;
+      wb0 = wb1;
+    }
+
+    if (part_out) {  // level sums of the next substep (flat refs, K6/K7)
+      double sr = 0.0, spp = 0.0;
+      if (active) {
+        sr = static_cast<double>(rho_new);
+        spp = static_cast<double>(d_pressure(rho_new, rho_new * th_k));
+      }
+      sr = wave_sum(sr);
+      spp = wave_sum(spp);
+      if (lx == 0) {
+        const size_t nw = static_cast<size_t>(tiles) * 4;
+        const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * 4 + ly) * 2;
+        part_out[slot] = sr;
+        part_out[slot + 1] = spp;
+      }
+    }
+
+    rho_k = rho_k1;
+    th_k = th_k1;
+    rho_k1 = rho_k2;
+    th_k1 = th_k2;
+    pref_k = pref_k1;
+    rref_k = rref_k1;
+    __syncthreads();
+  }
+}
+
+template <class R>
+void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int nkc,
+                        double* part_out, int* nparts_out) {
+  const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TY - 1) / TY;
+  const int tiles = tx * ty;
+  if (nparts_out) *nparts_out = tiles * 4;
+  if (a.col_ref || nkc < 1) nkc = 1;
+  if (nkc > a.g.nz) nkc = a.g.nz;
+  const dim3 grid(tiles * nkc), blk(TX, TY, 1);
+  if (a.col_ref) {
+    if (terrain)
+      hipLaunchKernelGGL((k_fast_column<R, true, true>), grid, blk, 0, st, a, tx, tiles, 1,
+                         nullptr);
+    else
+      hipLaunchKernelGGL((k_fast_column<R, false, true>), grid, blk, 0, st, a, tx, tiles, 1,
+                         nullptr);
+  } else if (terrain) {
+    hipLaunchKernelGGL((k_fast_column<R, true, false>), grid, blk, 0, st, a, tx, tiles, nkc,
+                       part_out);
+  } else {
+    hipLaunchKernelGGL((k_fast_column<R, false, false>), grid, blk, 0, st, a, tx, tiles, nkc,
+                       part_out);
+  }
+}
+
+int fast_column_parts(int nx, int ny) {
+  return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * 4;
+}
+
+template void launch_fast_column<float>(hipStream_t, const FastArgs<float>&, bool, int, double*,
+                                        int*);
+template void launch_fast_column<double>(hipStream_t, const FastArgs<double>&, bool, int,
+                                         double*, int*);
+
+}  // namespace gwm

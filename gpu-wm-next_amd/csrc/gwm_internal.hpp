@@ -26,18 +26,21 @@ struct Geo {
   int pyv;        // y-face plane pitch ny+1+2h
   double dx, dy;
 
+  // Element offsets are formed in 32-bit arithmetic (grids are limited to
+  // 2^31 elements per field, checked at grid creation), which keeps address
+  // arithmetic in single registers.
   __host__ __device__ size_t cell(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * pyc + (j + h)) * pxc + (i + h);
+    return static_cast<unsigned>((k * pyc + (j + h)) * pxc + (i + h));
   }
   __host__ __device__ size_t xf(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * pyc + (j + h)) * pxu + (i + h);
+    return static_cast<unsigned>((k * pyc + (j + h)) * pxu + (i + h));
   }
   __host__ __device__ size_t yf(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * pyv + (j + h)) * pxc + (i + h);
+    return static_cast<unsigned>((k * pyv + (j + h)) * pxc + (i + h));
   }
   __host__ __device__ size_t zf(int i, int j, int k) const { return cell(i, j, k); }
   __host__ __device__ size_t metric(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * ny + j) * nx + i;
+    return static_cast<unsigned>((k * ny + j) * nx + i);
   }
   __host__ __device__ size_t cell_total() const { return static_cast<size_t>(pxc) * pyc * nz; }
   __host__ __device__ size_t xf_total() const { return static_cast<size_t>(pxu) * pyc * nz; }
@@ -141,6 +144,35 @@ struct DMetrics {
   const R* lev_idz_c;  // nz   (flat columns: identical in every column)
   const R* lev_idz_f;  // nz+1
   const R* lev_zc;     // nz
+};
+
+// ---------------------------------------------------------------------------
+// Mapped loads (1x1 halo semantics).
+// ---------------------------------------------------------------------------
+template <class R>
+struct Loader {
+  Geo g;
+  __device__ R cell(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.cell(si, sj, k)];
+  }
+  __device__ R xf(const R* f, int i, int j, int k) const {
+    const int si = map_face(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.xf(si, sj, k)];
+  }
+  __device__ R yf(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_face(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.yf(si, sj, k)];
+  }
+  __device__ R zf(const R* f, int i, int j, int k) const {
+    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
+    return (si < 0 || sj < 0) ? R(0) : f[g.zf(si, sj, k)];
+  }
+  // Metric column index with the reference's periodic wrap
+  // (dry_fast_modes.cu:24-30); callers only pass in-range columns when the
+  // axis is open.
+  __device__ int mi(int i) const { return g.per_x ? map_cell(i, g.nx, 1) : i; }
+  __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
 };
 
 // Kernel launch record for optional per-kernel event timing.

@@ -13,35 +13,6 @@ namespace gwm {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Mapped loads (1x1 halo semantics).
-// ---------------------------------------------------------------------------
-template <class R>
-struct Loader {
-  Geo g;
-  __device__ R cell(const R* f, int i, int j, int k) const {
-    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.cell(si, sj, k)];
-  }
-  __device__ R xf(const R* f, int i, int j, int k) const {
-    const int si = map_face(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.xf(si, sj, k)];
-  }
-  __device__ R yf(const R* f, int i, int j, int k) const {
-    const int si = map_cell(i, g.nx, g.per_x), sj = map_face(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.yf(si, sj, k)];
-  }
-  __device__ R zf(const R* f, int i, int j, int k) const {
-    const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.zf(si, sj, k)];
-  }
-  // Metric column index with the reference's periodic wrap
-  // (dry_fast_modes.cu:24-30); callers only pass in-range columns when the
-  // axis is open.
-  __device__ int mi(int i) const { return g.per_x ? map_cell(i, g.nx, 1) : i; }
-  __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
-};
-
 template <class R>
 __device__ inline R upw(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
   return a >= R(0) ? a * ql : a * qr;
@@ -119,16 +90,24 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
   }
 }
 
+// One work-group per level: each thread sums a strided subset of the
+// partials in order, then a fixed-shape tree adds the 256 thread sums, so the
+// result does not depend on scheduling.
 template <class R>
-__global__ void k_level_finalize(int nz, int nchunk, const double* partials, int mode,
-                                 double count, R* out0, R* out1) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nz) return;
+__global__ __launch_bounds__(256) void k_level_finalize(int nz, int nchunk,
+                                                         const double* partials, int mode,
+                                                         double count, R* out0, R* out1) {
+  __shared__ double sh[256];
+  const int k = blockIdx.x;
+  const int t = threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  for (int c = 0; c < nchunk; ++c) {
+  for (int c = t; c < nchunk; c += 256) {
     s0 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 0];
     s1 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 1];
   }
+  s0 = block_sum(s0, sh);
+  s1 = block_sum(s1, sh);
+  if (t != 0) return;
   if (mode == 0) {
     out0[k] = static_cast<R>(s0 / count);  // dry_core.cu:485-490
   } else {
@@ -577,17 +556,30 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
   const bool nominal = a.fallout == 1, metric = a.fallout == 2;
   const R dz_nom = rmax(a.depth, R(1.0e-3));
   const R frac_nom = rmin(rmax(a.vt * dt / dz_nom, R(0)), R(1));
+  // The loads of the level below are issued before the current level's
+  // arithmetic, so the march overlaps memory latency with compute.
+  const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+  size_t c = g.cell(i, j, g.nz - 1);
+  R n_rho = a.rho[c], n_rt = a.rt[c], n_qv = a.qv[c], n_qc = a.qc[c], n_qr = a.qr[c];
   for (int k = g.nz - 1; k >= 0; --k) {
-    const size_t c = g.cell(i, j, k);
-    const R rho = rmax(a.rho[c], R(1.0e-6));
-    const R rt0 = a.rt[c];
+    const R rho = rmax(n_rho, R(1.0e-6));
+    const R rt0 = n_rt;
+    const R in_qv = n_qv, in_qc = n_qc, in_qr = n_qr;
+    if (k > 0) {
+      const size_t cn = c - plane;
+      n_rho = a.rho[cn];
+      n_rt = a.rt[cn];
+      n_qv = a.qv[cn];
+      n_qc = a.qc[cn];
+      n_qr = a.qr[cn];
+    }
     R th = rt0 / rho;
     R p = d_pressure(rho, rt0);
     R T = d_theta(rho, rt0) * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
     const R ex = rmax(T / rmax(th, R(1.0e-6)), R(1.0e-6));
-    R qv = rmax(a.qv[c] / rho, R(0));
-    R qc = rmax(a.qc[c] / rho, R(0));
-    R qr = rmax(a.qr[c] / rho, R(0));
+    R qv = rmax(in_qv / rho, R(0));
+    R qc = rmax(in_qc / rho, R(0));
+    R qr = rmax(in_qr / rho, R(0));
     R qs = d_qsat(T, p);
     const R lfac = Phys<R>::Lv / (d_cp<R>() * rmax(ex, R(1.0e-6)));
     if (qv > qs) {
@@ -643,6 +635,7 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
       inc = out;
     }
     a.qr[c] = rqr;
+    c -= plane;
   }
   if (nominal || metric) a.precip[static_cast<size_t>(j) * g.nx + i] += inc;
 }
@@ -945,8 +938,8 @@ void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* 
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1) {
-  hipLaunchKernelGGL(k_level_finalize<R>, dim3((nz + 63) / 64), dim3(64), 0, st, nz, nchunk,
-                     partials, mode, count, o0, o1);
+  hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(256), 0, st, nz, nchunk, partials,
+                     mode, count, o0, o1);
 }
 template <class R>
 void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p) {

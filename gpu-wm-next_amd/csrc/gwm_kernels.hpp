@@ -86,15 +86,30 @@ template <class R>
 void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p);
 template <class R>
 void launch_slow_stage(hipStream_t st, const SlowArgs<R>& a, bool terrain);
+// Column-marching slow stage (gwm_slow.hip), same modes as launch_slow_stage.
+template <class R>
+void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int nkc);
 template <class R>
 void launch_column_reference(hipStream_t st, const Geo& g, const R* rho, const R* theta,
                              const R* zc, R gravity, R* rref, R* pref);
 template <class R>
 void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain);
+// Column-marching fast substep (gwm_fast.hip).  With part_out (flat refs),
+// also writes the next substep's per-wave level partials [k][wave][2] and
+// returns their count per level in *nparts.
+// nkc: level chunks per column (1 = whole columns; terrain keeps 1).
+template <class R>
+void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int nkc,
+                        double* part_out, int* nparts);
+int fast_column_parts(int nx, int ny);
 template <class R>
 void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th);
 template <class R>
 void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
+// Column-marching tracer stage (gwm_slow.hip), same arguments as
+// launch_tracer_stage.
+template <class R>
+void launch_tracer_column(hipStream_t st, const TracerArgs<R>& a, bool terrain, int nkc);
 template <class R>
 void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt);
 template <class R>

@@ -125,6 +125,8 @@ def lib():
         L.gwm_ctx_kernel_stats.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_long),
                                            ctypes.POINTER(ctypes.c_double)]
         L.gwm_ctx_reset_kernel_stats.argtypes = [vp]
+        L.gwm_ctx_set_chunks.argtypes = [vp, ip]
+        L.gwm_ctx_tuning_report.argtypes = [vp, cp, ctypes.c_size_t]
         L.gwm_grid_create.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, pvp]
         L.gwm_grid_destroy.argtypes = [vp]
         L.gwm_grid_download_metric.argtypes = [vp, ip, vp]
@@ -227,6 +229,21 @@ class Context:
 
     def reset_kernel_stats(self):
         check(lib().gwm_ctx_reset_kernel_stats(self.h))
+
+    def set_chunks(self, chunks):
+        """Fix the level-chunk count of the column kernels (0: autotune)."""
+        check(lib().gwm_ctx_set_chunks(self.h, int(chunks)))
+
+    def tuning_report(self):
+        """{launch shape key: chunk count} chosen by the autotuner."""
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().gwm_ctx_tuning_report(self.h, buf, len(buf)))
+        out = {}
+        for item in buf.value.decode().split(";"):
+            if item:
+                k, v = item.rsplit("=", 1)
+                out[k] = int(v)
+        return out
 
 
 class Grid:

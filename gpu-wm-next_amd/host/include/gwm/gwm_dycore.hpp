@@ -298,7 +298,7 @@ inline void apply_local_split_explicit_fast_modes(
     std::vector<DryState>& states, const std::vector<domain::SubdomainDescriptor>& layout,
     const domain::GridMetrics& metrics, const DryStepperConfig& config) {
   const auto G = gwm::detail::global_of(layout, metrics);
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(), 0);
+  gwm::detail::StateLease s(metrics.device.get(), 0);
   detail::upload_dry(s, states, layout, G);
   const auto c = detail::cfg_of(config);
   detail::check(gwm_fast_modes(s, &c));
@@ -312,7 +312,7 @@ inline void compute_slow_tendencies(const std::vector<DryState>& states,
                                     std::vector<DrySlowTendencies>& out) {
   const auto G = gwm::detail::global_of(layout, metrics);
   require(out.size() == states.size(), "Tendency/state size mismatch");
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(), 0);
+  gwm::detail::StateLease s(metrics.device.get(), 0);
   detail::upload_dry(s, states, layout, G);
   const auto c = detail::cfg_of(config);
   detail::check(gwm_slow_tendencies(s, &c));
@@ -337,7 +337,7 @@ inline void advance_dry_state_ssprk3(std::vector<DryState>& states,
                                      BoundaryUpdater& boundary_updater,
                                      FastModeIntegrator& fast_modes) {
   const auto G = gwm::detail::global_of(layout, metrics);
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(), 0);
+  gwm::detail::StateLease s(metrics.device.get(), 0);
   detail::upload_dry(s, states, layout, G);
   const auto c = detail::cfg_of(config);
   detail::PluginCall call;
@@ -382,7 +382,7 @@ std::vector<DryState> device_init(const std::vector<domain::SubdomainDescriptor>
                                   const domain::GridMetrics& metrics, const std::string& prefix,
                                   Init init) {
   const auto G = gwm::detail::global_of(layout, metrics);
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(), 0);
+  gwm::detail::StateLease s(metrics.device.get(), 0);
   check(init(s));
   std::vector<DryState> out;
   for (const auto& d : layout)
@@ -653,7 +653,7 @@ inline void advance_passive_tracers_ssprk3(std::vector<state::TracerState>& trac
           "mirror; use gwm_tracers_ssprk3 with GWM_BOUNDARY_STRIPS for lateral strips");
   const auto G = gwm::detail::global_of(layout, metrics);
   const int ntr = static_cast<int>(tracers.front().size());
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(), ntr);
+  gwm::detail::StateLease s(metrics.device.get(), ntr);
   detail::upload_dry(s, dry_states, layout, G);
   detail::upload_tracers(s, tracers, layout, G);
   const auto c = detail::cfg_of(config);
@@ -791,8 +791,7 @@ inline void apply_warm_rain_microphysics(std::vector<dycore::DryState>& states,
     });
     const auto L = comm::VirtualRankLayout::build(r.nx(), r.ny(), r.nz(), r.halo(), 1, 1);
     const gwm::detail::Global G{r.nx(), r.ny(), r.nz(), r.halo()};
-    gwm_state* s =
-        gwm::detail::StateCache::get().get(g, static_cast<int>(tracers[n].size()));
+    gwm::detail::StateLease s(g, static_cast<int>(tracers[n].size()));
     std::vector<dycore::DryState> one;
     one.push_back(std::move(states[n]));
     std::vector<state::TracerState> tone;
@@ -832,7 +831,7 @@ inline void apply_warm_rain_microphysics(std::vector<dycore::DryState>& states,
                                   metrics.dx * metrics.dy);
   }
   for (const auto& t : tracers) detail::check_warm_rain_registry(t);
-  gwm_state* s = gwm::detail::StateCache::get().get(metrics.device.get(),
+  gwm::detail::StateLease s(metrics.device.get(),
                                                     static_cast<int>(tracers.front().size()));
   detail::run(s, states, tracers, layout, G, config, 1, accumulations);
 }

@@ -71,25 +71,40 @@ struct Device {
   }
 };
 // Cached device states per (grid, tracer count); destroyed before the grid.
+// A state is leased for the duration of one operator call, so an operator
+// called from inside another (a host plug-in) gets a state of its own.
 struct StateCache {
-  std::vector<std::tuple<const gwm_grid*, int, gwm_state*>> entries;
+  struct Entry {
+    const gwm_grid* grid;
+    int ntr;
+    gwm_state* state;
+    bool in_use;
+  };
+  std::vector<Entry> entries;
   ~StateCache() { clear(); }
   void clear() {
-    for (auto& e : entries) gwm_state_destroy(std::get<2>(e));
+    for (auto& e : entries) gwm_state_destroy(e.state);
     entries.clear();
   }
-  gwm_state* get(const gwm_grid* g, int ntr) {
+  gwm_state* acquire(const gwm_grid* g, int ntr) {
     for (auto& e : entries)
-      if (std::get<0>(e) == g && std::get<1>(e) == ntr) return std::get<2>(e);
+      if (e.grid == g && e.ntr == ntr && !e.in_use) {
+        e.in_use = true;
+        return e.state;
+      }
     gwm_state* s = nullptr;
     check(gwm_state_create(Device::get().ctx, g, ntr, &s));
-    entries.emplace_back(g, ntr, s);
+    entries.push_back(Entry{g, ntr, s, true});
     return s;
+  }
+  void release(gwm_state* s) {
+    for (auto& e : entries)
+      if (e.state == s) e.in_use = false;
   }
   void drop(const gwm_grid* g) {
     for (auto it = entries.begin(); it != entries.end();) {
-      if (std::get<0>(*it) == g) {
-        gwm_state_destroy(std::get<2>(*it));
+      if (it->grid == g) {
+        gwm_state_destroy(it->state);
         it = entries.erase(it);
       } else {
         ++it;
@@ -101,6 +116,18 @@ struct StateCache {
     static StateCache cache;
     return cache;
   }
+};
+
+class StateLease {
+ public:
+  StateLease(const gwm_grid* g, int ntr) : s_(StateCache::get().acquire(g, ntr)) {}
+  ~StateLease() { StateCache::get().release(s_); }
+  StateLease(const StateLease&) = delete;
+  StateLease& operator=(const StateLease&) = delete;
+  operator gwm_state*() const { return s_; }
+
+ private:
+  gwm_state* s_;
 };
 
 }  // namespace detail

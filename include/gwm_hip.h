@@ -169,6 +169,14 @@ int gwm_ctx_set_kernel_timing(gwm_ctx* ctx, int enable);
  * Returns launches and total milliseconds recorded since the last reset. */
 int gwm_ctx_kernel_stats(gwm_ctx* ctx, const char* name, long* launches, double* total_ms);
 int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx);
+/* Column kernels split each column into level chunks; by default the count
+ * is chosen per kernel, shape and precision by timing the candidates
+ * {1, 2, 4, 8} at first use (the autotuner).  chunks > 0 fixes it; 0 returns
+ * to the autotuner.  Results do not depend on the choice.  No counterpart in
+ * the reference (tuning control of this implementation). */
+int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks);
+/* "kernel:NXxNYxNZ:prec:variant=chunks;..." for every tuned launch shape. */
+int gwm_ctx_tuning_report(gwm_ctx* ctx, char* buf, size_t size);
 
 /* ---- grid / metrics ------------------------------------------------------
  * Replaces GridMetrics::make_hybrid_height (grid_metrics.hpp:87-239) and

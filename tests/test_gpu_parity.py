@@ -305,6 +305,11 @@ CASES = {
     "mountain_wave": dict(nx=32, ny=12, nz=14, dx=1000.0, dy=1000.0, z_top=18000.0, terrain=1200.0),
     "terrain_rest": dict(nx=24, ny=16, nz=12, dx=2000.0, dy=2000.0, z_top=18000.0, terrain=900.0),
     "open_bubble": dict(nx=20, ny=16, nz=10, periodic_x=False, periodic_y=False),
+    # Several 64 x 4 column tiles, a ragged last tile in x and a tile count
+    # divisible by 8 (the XCD-banded tile order of the column kernel).
+    "wide_bubble": dict(nx=130, ny=32, nz=8),
+    "wide_mountain": dict(nx=128, ny=16, nz=10, dx=1000.0, dy=1000.0, z_top=15000.0,
+                          terrain=800.0),
 }
 
 
@@ -320,7 +325,9 @@ def build_case(name, prec, ctx32, ctx64, n_tracers=0):
         bubble_world(w, +1, u=4.0)
     elif name in ("density_current", "open_bubble"):
         bubble_world(w, -1)
-    elif name == "mountain_wave":
+    elif name == "wide_bubble":
+        bubble_world(w, +1, u=3.0)
+    elif name in ("mountain_wave", "wide_mountain"):
         w.init_mountain_wave(1.18, 300.0, 15.0)
     elif name == "terrain_rest":
         w.init_hydrostatic_rest(1.18, 300.0)
@@ -458,3 +465,36 @@ def test_budget_sums(ctx32, ctx64, prec):
     a, b = gs.summary(), w.summary()
     for k in O.SUMMARY_KEYS:
         assert abs(a[k] - b[k]) <= 1e-12 * max(1.0, abs(b[k])), (k, a[k], b[k])
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_level_chunks_do_not_change_results(ctx32, ctx64, prec):
+    """The column kernels split columns into level chunks (autotuned); every
+    chunk count must give bit-identical states, tracers included."""
+    ctx = ctx_for(prec, ctx32, ctx64)
+    nx, ny, nz = 130, 16, 12
+    zi = np.linspace(0.0, 12000.0, nz + 1)
+    grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    ref = None
+    try:
+        for chunks in (1, 2, 4, 8, 0):
+            ctx.set_chunks(chunks)
+            gs = G.State(grid, 3)
+            gs.init_convection(1.0e5, 5, 0.5, 1000.0)
+            qv = gs.get(O.TRACER0)
+            gs.set(O.TRACER0 + 1, (0.02 * qv).astype(qv.dtype))
+            gs.set(O.TRACER0 + 2, (0.01 * qv).astype(qv.dtype))
+            for _ in range(2):
+                gs.moist_step(G.step_config(dt=2.0, fast_substeps=3),
+                              G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0))
+            out = [gs.get(f) for f in FIELDS + (O.TRACER0, O.TRACER0 + 1, O.TRACER0 + 2)]
+            if ref is None:
+                ref = out
+            else:
+                for a, b in zip(out, ref):
+                    assert np.array_equal(a, b), f"chunks={chunks}"
+    finally:
+        ctx.set_chunks(0)
+    report = ctx.tuning_report()
+    assert any(k.startswith("fast_column:130x16x12") for k in report), report
+    assert all(v in (1, 2, 4, 8) for v in report.values())

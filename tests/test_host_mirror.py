@@ -112,20 +112,29 @@ def test_driver_summary_matches_oracle(orc, case, precision):
     for _ in range(3):
         w.ssprk3_step(orc.step_cfg(**spec["step"]))
     final = w.summary()
-    # The JSON carries six significant digits (rel 5e-7).  FP32 adds the
-    # device/host FP32 trajectory difference, up to 2e-4 of a field's range
-    # after a few steps (the FP32 multi-step bound of the parity tests).
-    rel = 2e-6 if precision == "fp64" else 5e-4
+    # The JSON carries six significant digits: a printed value is within
+    # 5e-6 of its own magnitude.  FP32 adds the device/host FP32 trajectory
+    # difference; extremes of a field move by up to ~2e-3 of its range after
+    # three steps (the FP32 parity tests bound the rel-L2 at 2e-4).
+    digits = 6e-6
+    traj = 0.0 if precision == "fp64" else 2e-3
+    bad = []
     for part, ref in (("initial", initial), ("final", final)):
         s = got[part]
-        # Momentum totals can be pure round-off around zero: bound them by
-        # the momentum a 1 m/s wind would carry.
-        mom_floor = rel * abs(ref["total_dry_mass"])
+        scale = abs(ref["total_dry_mass"])
         for jk, ok in KEYS:
-            floor = mom_floor if "momentum" in jk else 0.0
-            assert abs(s[jk] - ref[ok]) <= rel * abs(ref[ok]) + floor, (part, jk, s[jk], ref[ok])
+            # Momentum totals can be round-off around zero: allow the momentum
+            # a 1e-6 m/s (FP64) or 1e-3 m/s (FP32) mean wind would carry.
+            floor = (1e-6 if precision == "fp64" else 1e-3) * scale if "momentum" in jk else 0.0
+            tol = digits * abs(ref[ok]) + traj * abs(ref[ok]) + floor
+            if abs(s[jk] - ref[ok]) > tol:
+                bad.append((part, jk, s[jk], ref[ok]))
         for jk, ok in RANGES:
-            spread = max(abs(ref[f"{ok}_max"]), abs(ref[f"{ok}_min"]), 1e-3)
+            spread = max(abs(ref[f"{ok}_max"]), abs(ref[f"{ok}_min"]))
             for b in ("min", "max"):
-                assert abs(s[jk][b] - ref[f"{ok}_{b}"]) <= rel * spread, (part, jk, b)
+                v = ref[f"{ok}_{b}"]
+                tol = digits * abs(v) + traj * spread + 1e-12
+                if abs(s[jk][b] - v) > tol:
+                    bad.append((part, jk, b, s[jk][b], v))
+    assert not bad, bad
     assert np.isfinite(got["final"]["total_dry_mass"])
