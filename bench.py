@@ -21,6 +21,11 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "gpu-wm-next_amd"))
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -37,6 +42,11 @@ def parse():
     p.add_argument("--dt", type=float, default=5.0)
     p.add_argument("--control-dt", type=float, default=0.25,
                    help="dt of the finite-state control run (0 to skip)")
+    p.add_argument("--surrogate-pairs", type=float, default=2e7,
+                   help="training pairs of the surrogate leg, from seeds 0-7 (0 to skip)")
+    p.add_argument("--surrogate-epochs", type=int, default=4)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run one device-to-device state copy (profiler byte calibration)")
     return p.parse_args()
 
 
@@ -49,12 +59,31 @@ def make_case(G, precision, nx, ny, nz):
     return ctx, grid, st, zi
 
 
-def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0):
-    ctx, grid, st, zi = make_case(G, precision, nx, ny, nz)
+def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None, ctx=None,
+        calibrate_copy=False, info=None):
+    if ctx is None:
+        ctx, grid, st, zi = make_case(G, precision, nx, ny, nz)
+    else:
+        zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
+        grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+        st = G.State(grid, 3)
+        st.init_convection(1.0e5, 42, 0.5, 1000.0)
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
+
+    def step():
+        if surrogate is None:
+            st.moist_step(cfg, wr)
+        else:
+            st.moist_step_surrogate(cfg, wr, surrogate)
+
+    if calibrate_copy:
+        other = G.State(grid, 3)
+        other.copy_from(st)
+        ctx.sync()
+        del other
     for _ in range(warmup):
-        st.moist_step(cfg, wr)
+        step()
     ctx.sync()
     if timing:
         ctx.reset_kernel_stats()
@@ -62,18 +91,21 @@ def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0):
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        st.moist_step(cfg, wr)
+        step()
     ctx.sync()
     secs = time.perf_counter() - t0
     stats = {}
     if timing:
         for name in ("fast_column", "fast_substep", "slow_stage", "level_partials",
-                     "level_finalize", "tracer_stage", "warm_rain", "column_reference"):
+                     "level_finalize", "tracer_stage", "warm_rain", "column_reference",
+                     "warm_rain_surrogate", "fallout"):
             n, ms = ctx.kernel_stats(name)
             if n:
                 stats[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n}
         ctx.set_kernel_timing(False)
     summary = st.summary()
+    if info is not None:
+        info["tuning"] = ctx.tuning_report()
     return secs, stats, summary
 
 
@@ -107,6 +139,51 @@ def cpu_baseline(nx, ny, nz, threads, dt):
     }
 
 
+def surrogate_leg(G, a, cells, info):
+    """Train the warm-rain surrogate on plain-scheme pairs (seeds 0-7 of the
+    case), report its error against the plain scheme at seed 42, and time the
+    FP32+surrogate step."""
+    import numpy as np
+    import gwm_amd.surrogate as SG
+
+    ctx = G.Context(0, G.F32)
+    seeds = list(range(8))
+    per = int(a.surrogate_pairs) // len(seeds)
+    log(f"surrogate: sampling {a.surrogate_pairs:.0f} pairs")
+    t0 = time.perf_counter()
+    X, Y, n = SG.generate(ctx, seeds, per, shape=(a.nx, a.ny, a.nz),
+                          wr=G.warm_rain_config(dt=a.dt, rain_terminal_velocity=12.0))
+    ctx.sync()
+    t_gen = time.perf_counter() - t0
+    log("surrogate: training")
+    t0 = time.perf_counter()
+    sur, norm, losses = SG.train(ctx, X, Y, n, seed=0, epochs=a.surrogate_epochs, batch=4096,
+                                 lr=2e-3, lr_final=5e-5)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    del X, Y
+    log(f"surrogate: epoch losses {list(losses)}; error at seed 42")
+    errs = SG.tendency_errors(ctx, sur, seed=42, shape=(min(a.nx, 256), min(a.ny, 256), a.nz),
+                              wr=G.warm_rain_config(dt=a.dt, rain_terminal_velocity=0.0))
+    # Throughput of the FP32+surrogate step at the finite-state dt.
+    dt = a.control_dt or a.dt
+    sinfo = {}
+    log("surrogate: FP32+surrogate step")
+    ss, sstats, ssum = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt,
+                           surrogate=sur, ctx=ctx)
+    _, sstats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, dt, surrogate=sur,
+                       ctx=ctx, info=sinfo)
+    return {
+        "pairs": n, "seeds": seeds, "epochs": a.surrogate_epochs,
+        "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+        "epoch_loss": [float(v) for v in losses],
+        "error_vs_plain_seed42": errs,
+        "value": cells * a.steps / ss, "dt": dt,
+        "state_finite": bool(np.isfinite(ssum["w_max"])),
+        "kernels": sstats,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -118,16 +195,21 @@ def main():
     cells = a.nx * a.ny * a.nz
     import math
 
-    secs, _, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, a.dt)
+    log("FP32 plain step")
+    secs, _, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, a.dt,
+                           calibrate_copy=a.calibrate_copy)
     value = cells * a.steps / secs
     # Per-kernel HIP-event timings come from a second pass, so the events do
     # not sit inside the timed region above.
-    _, stats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, a.dt)
+    info = {}
+    log("FP32 plain step, per-kernel timing")
+    _, stats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, a.dt, info=info)
     finite = all(math.isfinite(summary[k]) for k in ("w_min", "w_max", "total_dry_mass"))
     control = None
     if a.control_dt > 0:
         # Same per-step work at a dt the reference scheme integrates without
         # diverging, to show the throughput does not depend on the values.
+        log("FP32 plain step, finite control")
         cs, _, csum = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, a.control_dt)
         control = {"dt": a.control_dt, "value": cells * a.steps / cs,
                    "state_finite": all(math.isfinite(csum[k]) for k in ("w_min", "w_max")),
@@ -147,10 +229,30 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
                     "kernel": kname, "avg_ms": round(fs["avg_ms"], 5),
                     "algorithmic_bytes": int(bytes_per_launch)}
-    fp64 = None
+    # Other stencil kernels against their own algorithmic bytes: slow stage
+    # (mean of modes: stage 1 reads 5 writes 5, stages 2-3 read 10 write 5),
+    # tracer stage (3 tracers: rho, 3 faces, 3 in, 3 q0 except stage 1, 3 out).
+    kernel_roofline = {}
+    for name, elems in (("slow_stage", (10 + 15 + 15) / 3), ("tracer_stage", (10 + 13 + 13) / 3),
+                        ("warm_rain", 9)):
+        if name in stats:
+            b = elems * 4 * cells
+            gbs = b / (stats[name]["avg_ms"] * 1e-3) / 1e9
+            kernel_roofline[name] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / 8000.0, 4),
+                                     "algorithmic_bytes": int(b)}
+    fp64 = fp64_control = None
     if a.fp64_steps > 0:
-        s64, st64, _ = run(G, G.F64, a.nx, a.ny, a.nz, a.fp64_steps, 1, False, a.dt)
+        log("FP64 plain step")
+        s64, _, _ = run(G, G.F64, a.nx, a.ny, a.nz, a.fp64_steps, 1, False, a.dt)
         fp64 = cells * a.fp64_steps / s64
+        if a.control_dt > 0:
+            s64c, _, _ = run(G, G.F64, a.nx, a.ny, a.nz, a.fp64_steps, 1, False, a.control_dt)
+            fp64_control = cells * a.fp64_steps / s64c
+    sur = surrogate_leg(G, a, cells, info) if a.surrogate_pairs > 0 else None
+    if sur is not None and fp64_control:
+        sur["over_fp64_plain"] = sur["value"] / fp64_control
+    if a.cpu_baseline:
+        log("CPU baseline")
     cpu = (cpu_baseline(a.nx, a.ny, a.nz, a.cpu_threads, a.control_dt or a.dt)
            if a.cpu_baseline else None)
     out = {
@@ -172,9 +274,13 @@ def main():
         "state_finite": finite,
         "finite_control": control,
         "fp64_plain_value": fp64,
+        "fp64_plain_control_value": fp64_control,
         "fp32_over_fp64": (value / fp64) if fp64 else None,
+        "surrogate": sur,
         "dry_mass": summary["total_dry_mass"],
         "kernels": stats,
+        "kernel_roofline": kernel_roofline,
+        "tuning": info.get("tuning"),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -3,6 +3,7 @@
 // initial states and compensated budgets.  All compute runs in the HIP
 // kernels of gwm_kernels.hip; this file only allocates, launches and copies.
 #include "../../include/gwm_hip.h"
+#include "gwm_surrogate.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -1919,6 +1920,348 @@ int gwm_init_convection(gwm_state* s, double p_s, unsigned long long seed, doubl
       init_convection<decltype(r)>(s, p_s, seed, amp, ztop);
       return 0;
     });
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Warm-rain surrogate (no reference counterpart): parameters, training-data
+// sampling, FP32 training, BF16 matrix-core inference and the warm-rain step
+// that uses it.
+// ---------------------------------------------------------------------------
+struct gwm_surrogate {
+  gwm_ctx* ctx = nullptr;
+  gwm::SurTrain T;
+  int batch = 0;  // rows the training buffers hold
+  gwm::SurNorm norm{};
+  bool have_norm = false;
+  unsigned long long* nfall = nullptr;
+  double* loss_part = nullptr;
+  size_t loss_n = 0;
+  ~gwm_surrogate() {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    free_batch();
+    for (void* p : {static_cast<void*>(T.params), static_cast<void*>(T.grad),
+                    static_cast<void*>(T.m), static_cast<void*>(T.v),
+                    static_cast<void*>(nfall), static_cast<void*>(loss_part)})
+      if (p) (void)hipFree(p);
+    ctx_release(ctx);
+  }
+  void free_batch() {
+    std::vector<float*> ps = {T.xn, T.wpart};
+    for (auto* q : T.z) ps.push_back(q);
+    for (auto* q : T.a) ps.push_back(q);
+    for (auto* q : T.dz) ps.push_back(q);
+    for (float* q : ps)
+      if (q) (void)hipFree(q);
+    T.xn = T.wpart = nullptr;
+    for (auto& q : T.z) q = nullptr;
+    for (auto& q : T.a) q = nullptr;
+    for (auto& q : T.dz) q = nullptr;
+    batch = 0;
+  }
+  void ensure_batch(int B) {
+    if (B <= batch) return;
+    free_batch();
+    auto fa = [](size_t n) { return static_cast<float*>(dalloc(n * sizeof(float))); };
+    T.xn = fa(static_cast<size_t>(B) * 8);
+    for (int l = 0; l < 4; ++l) {
+      const size_t w = l < 3 ? 64 : gwm::kSurOut;
+      T.z[l] = fa(static_cast<size_t>(B) * w);
+      T.dz[l] = fa(static_cast<size_t>(B) * w);
+      if (l < 3) T.a[l] = fa(static_cast<size_t>(B) * 64);
+    }
+    T.wpart = fa(static_cast<size_t>(16) * 65 * 64);
+    batch = B;
+  }
+  double* loss_buf(size_t n) {
+    if (n > loss_n) {
+      if (loss_part) HIPCHK(hipFree(loss_part));
+      loss_part = static_cast<double*>(dalloc(n * sizeof(double)));
+      loss_n = n;
+    }
+    return loss_part;
+  }
+};
+
+namespace {
+
+gwm::SurNorm to_norm(const gwm_surrogate_norm& n) {
+  gwm::SurNorm s{};
+  for (int f = 0; f < gwm::kSurIn; ++f) {
+    require(n.x_std[f] > 0.0f, "feature std must be positive");
+    require(n.box_lo[f] <= n.box_hi[f], "fallback box bounds out of order");
+    s.xm[f] = n.x_mean[f];
+    s.xs[f] = n.x_std[f];
+    s.lo[f] = n.box_lo[f];
+    s.hi[f] = n.box_hi[f];
+  }
+  for (int o = 0; o < gwm::kSurOut; ++o) {
+    require(n.y_std[o] > 0.0f, "target std must be positive");
+    s.ym[o] = n.y_mean[o];
+    s.ys[o] = n.y_std[o];
+  }
+  return s;
+}
+
+gwm::MicroCfg<float> micro_cfg(const gwm_warm_rain_config& c) {
+  return gwm::MicroCfg<float>{static_cast<float>(c.dt),
+                              static_cast<float>(c.condensation_relaxation),
+                              static_cast<float>(c.cloud_autoconversion_threshold),
+                              static_cast<float>(c.cloud_autoconversion_rate),
+                              static_cast<float>(c.rain_evaporation_rate),
+                              static_cast<float>(c.accretion_rate), c.enable_latent_heating};
+}
+
+// Warm rain with the surrogate: source terms (network, or the plain scheme
+// outside the box), then the fallout sweep of warm_rain (metric or nominal).
+void warm_rain_surrogate(gwm_state* S, const gwm_warm_rain_config& c, gwm_surrogate* M,
+                         int metric, long long* nfall_out) {
+  require(S->ctx->prec == GWM_F32, "the surrogate runs on FP32 states");
+  require(S->ntr >= 3, "warm rain requires the qv, qc, qr tracers");
+  require(M->have_norm, "surrogate normalisation not set");
+  require(M->ctx == S->ctx, "surrogate and state belong to different contexts");
+  const Geo& g = S->g();
+  void** ds = S->cur_set();
+  float* rho = static_cast<float*>(ds[0]);
+  float* rt = static_cast<float*>(ds[1]);
+  float* qv = static_cast<float*>(S->field(GWM_TRACER0 + 0));
+  float* qc = static_cast<float*>(S->field(GWM_TRACER0 + 1));
+  float* qr = static_cast<float*>(S->field(GWM_TRACER0 + 2));
+  gwm::launch_zero_counter(S->ctx->stream, M->nfall);
+  {
+    Timed t(S->ctx, "warm_rain_surrogate");
+    gwm::launch_warm_rain_sur(S->ctx->stream, g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params,
+                              M->norm, M->nfall);
+  }
+  const float vt = static_cast<float>(c.rain_terminal_velocity);
+  const float depth = static_cast<float>(c.sedimentation_layer_depth_m);
+  const int fallout = metric ? (vt > 0.0f ? 2 : 0) : ((vt > 0.0f && depth > 0.0f) ? 1 : 0);
+  if (fallout) {
+    Timed t(S->ctx, "fallout");
+    gwm::launch_fallout(S->ctx->stream, g, qr,
+                        static_cast<float*>(S->view ? S->owner->precip : S->precip),
+                        static_cast<const float*>(S->grid->idz_c),
+                        static_cast<const float*>(S->grid->lev_idz_c),
+                        static_cast<float>(c.dt), vt, depth, fallout, S->grid->terrain_nonzero);
+  }
+  if (nfall_out) {
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, M->nfall, sizeof h, hipMemcpyDeviceToHost, S->ctx->stream));
+    HIPCHK(hipStreamSynchronize(S->ctx->stream));
+    *nfall_out = static_cast<long long>(h);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gwm_surrogate_create(gwm_ctx* ctx, unsigned long long seed, gwm_surrogate** out) {
+  return guard([&] {
+    require(ctx && out, "null argument");
+    require(!ctx->closing, "context already destroyed");
+    HIPCHK(hipSetDevice(ctx->device));
+    auto M = std::make_unique<gwm_surrogate>();
+    M->ctx = ctx;
+    ctx->refs += 1;
+    // Xavier-uniform weights from a seeded counter hash, zero biases.
+    std::vector<float> p(gwm::kSurParams, 0.0f);
+    const int wofs[4] = {gwm::kSurW1, gwm::kSurW2, gwm::kSurW3, gwm::kSurW4};
+    const int fin[4] = {gwm::kSurIn, 64, 64, 64};
+    const int fout[4] = {64, 64, 64, gwm::kSurOut};
+    unsigned long long ctr = seed * 0x9e3779b97f4a7c15ull + 1;
+    for (int l = 0; l < 4; ++l) {
+      const double lim = std::sqrt(6.0 / (fin[l] + fout[l]));
+      for (int q = 0; q < fin[l] * fout[l]; ++q) {
+        unsigned long long z = (ctr += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        const double u = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+        p[wofs[l] + q] = static_cast<float>((2.0 * u - 1.0) * lim);
+      }
+    }
+    auto fa = [](size_t n) { return static_cast<float*>(dalloc(n * sizeof(float))); };
+    M->T.params = fa(gwm::kSurParams);
+    M->T.grad = fa(gwm::kSurParams);
+    M->T.m = fa(gwm::kSurParams);
+    M->T.v = fa(gwm::kSurParams);
+    HIPCHK(hipMemcpy(M->T.params, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(M->T.m, 0, gwm::kSurParams * sizeof(float)));
+    HIPCHK(hipMemset(M->T.v, 0, gwm::kSurParams * sizeof(float)));
+    M->nfall = static_cast<unsigned long long*>(dalloc(sizeof(unsigned long long)));
+    *out = M.release();
+  });
+}
+
+int gwm_surrogate_destroy(gwm_surrogate* m) {
+  return guard([&] { delete m; });
+}
+
+int gwm_surrogate_set_norm(gwm_surrogate* m, const gwm_surrogate_norm* n) {
+  return guard([&] {
+    require(m && n, "null argument");
+    m->norm = to_norm(*n);
+    m->have_norm = true;
+  });
+}
+
+int gwm_surrogate_get_norm(gwm_surrogate* m, gwm_surrogate_norm* n) {
+  return guard([&] {
+    require(m && n, "null argument");
+    require(m->have_norm, "surrogate normalisation not set");
+    for (int f = 0; f < gwm::kSurIn; ++f) {
+      n->x_mean[f] = m->norm.xm[f];
+      n->x_std[f] = m->norm.xs[f];
+      n->box_lo[f] = m->norm.lo[f];
+      n->box_hi[f] = m->norm.hi[f];
+    }
+    for (int o = 0; o < gwm::kSurOut; ++o) {
+      n->y_mean[o] = m->norm.ym[o];
+      n->y_std[o] = m->norm.ys[o];
+    }
+  });
+}
+
+int gwm_surrogate_set_params(gwm_surrogate* m, const float* host) {
+  return guard([&] {
+    require(m && host, "null argument");
+    HIPCHK(hipMemcpyAsync(m->T.params, host, gwm::kSurParams * sizeof(float),
+                          hipMemcpyHostToDevice, m->ctx->stream));
+    HIPCHK(hipMemsetAsync(m->T.m, 0, gwm::kSurParams * sizeof(float), m->ctx->stream));
+    HIPCHK(hipMemsetAsync(m->T.v, 0, gwm::kSurParams * sizeof(float), m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    m->T.step = 0;
+  });
+}
+
+int gwm_surrogate_get_params(gwm_surrogate* m, float* host) {
+  return guard([&] {
+    require(m && host, "null argument");
+    HIPCHK(hipMemcpyAsync(host, m->T.params, gwm::kSurParams * sizeof(float),
+                          hipMemcpyDeviceToHost, m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  });
+}
+
+int gwm_surrogate_sample(gwm_state* s, const gwm_warm_rain_config* c, unsigned long long seed,
+                         long n, void* dev_x, void* dev_y, void* dev_cell) {
+  return guard([&] {
+    S_(s);
+    require(c && dev_x && dev_y, "null argument");
+    require(n > 0, "sample count must be positive");
+    require(s->ctx->prec == GWM_F32, "sampling runs on FP32 states");
+    require(s->ntr >= 3, "sampling requires the qv, qc, qr tracers");
+    void** ds = s->cur_set();
+    gwm::launch_sur_sample(s->ctx->stream, s->g(), static_cast<const float*>(ds[0]),
+                           static_cast<const float*>(ds[1]),
+                           static_cast<const float*>(s->field(GWM_TRACER0 + 0)),
+                           static_cast<const float*>(s->field(GWM_TRACER0 + 1)),
+                           static_cast<const float*>(s->field(GWM_TRACER0 + 2)), micro_cfg(*c),
+                           seed, n, static_cast<float*>(dev_x), static_cast<float*>(dev_y),
+                           static_cast<long long*>(dev_cell));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int gwm_surrogate_train(gwm_surrogate* m, const void* dev_x, const void* dev_y, long n, int batch,
+                        int epochs, double lr, double lr_final, double* epoch_loss) {
+  return guard([&] {
+    require(m && dev_x && dev_y, "null argument");
+    require(m->have_norm, "surrogate normalisation not set");
+    require(batch >= 64 && batch % 64 == 0, "batch must be a positive multiple of 64");
+    require(n >= batch && epochs >= 1, "need at least one batch and one epoch");
+    require(lr > 0.0 && lr_final > 0.0, "learning rates must be positive");
+    m->ensure_batch(batch);
+    const long nmb = n / batch;
+    const int per = (batch * gwm::kSurOut + 255) / 256;
+    double* part = m->loss_buf(static_cast<size_t>(nmb) * per);
+    const long total = nmb * epochs;
+    const float* X = static_cast<const float*>(dev_x);
+    const float* Y = static_cast<const float*>(dev_y);
+    std::vector<double> h(static_cast<size_t>(nmb) * per);
+    long step = 0;
+    for (int e = 0; e < epochs; ++e) {
+      for (long b = 0; b < nmb; ++b, ++step) {
+        // cosine decay from lr to lr_final over all steps
+        const double f = 0.5 * (1.0 + std::cos(M_PI * static_cast<double>(step) / total));
+        m->T.lr = static_cast<float>(lr_final + (lr - lr_final) * f);
+        int nl = 0;
+        gwm::launch_sur_train_step(m->ctx->stream, m->T, X + b * batch * gwm::kSurIn,
+                                   Y + b * batch * gwm::kSurOut, batch, m->norm, true,
+                                   part + b * per, &nl);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            m->ctx->stream));
+      HIPCHK(hipStreamSynchronize(m->ctx->stream));
+      double sum = 0.0;
+      for (double v : h) sum += v;
+      if (epoch_loss) epoch_loss[e] = sum / (static_cast<double>(nmb) * batch * gwm::kSurOut);
+    }
+  });
+}
+
+int gwm_surrogate_loss_grad(gwm_surrogate* m, const void* dev_x, const void* dev_y, int n,
+                            double* loss, float* grad_host) {
+  return guard([&] {
+    require(m && dev_x && dev_y && loss && grad_host, "null argument");
+    require(m->have_norm, "surrogate normalisation not set");
+    require(n >= 1, "need at least one row");
+    m->ensure_batch(n);
+    const int per = (n * gwm::kSurOut + 255) / 256;
+    double* part = m->loss_buf(static_cast<size_t>(per));
+    int nl = 0;
+    gwm::launch_sur_train_step(m->ctx->stream, m->T, static_cast<const float*>(dev_x),
+                               static_cast<const float*>(dev_y), n, m->norm, false, part, &nl);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(static_cast<size_t>(nl));
+    HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          m->ctx->stream));
+    HIPCHK(hipMemcpyAsync(grad_host, m->T.grad, gwm::kSurParams * sizeof(float),
+                          hipMemcpyDeviceToHost, m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    *loss = sum / (static_cast<double>(n) * gwm::kSurOut);
+  });
+}
+
+int gwm_surrogate_infer(gwm_surrogate* m, const void* dev_x, long n, void* dev_y) {
+  return guard([&] {
+    require(m && dev_x && dev_y, "null argument");
+    require(m->have_norm, "surrogate normalisation not set");
+    require(n >= 1, "need at least one row");
+    Timed t(m->ctx, "surrogate_infer");
+    gwm::launch_sur_infer(m->ctx->stream, m->T.params, m->norm, static_cast<const float*>(dev_x),
+                          n, static_cast<float*>(dev_y));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int gwm_warm_rain_surrogate(gwm_state* s, const gwm_warm_rain_config* c, gwm_surrogate* m,
+                            int metric_fallout, long long* n_fallback) {
+  return guard([&] {
+    S_(s);
+    require(c && m, "null argument");
+    warm_rain_surrogate(s, *c, m, metric_fallout, n_fallback);
+  });
+}
+
+int gwm_moist_step_surrogate(gwm_state* s, const gwm_step_config* c,
+                             const gwm_warm_rain_config* wr, gwm_surrogate* m,
+                             const gwm_boundary_config* b) {
+  return guard([&] {
+    S_(s);
+    require(c && wr && m, "null argument");
+    require(s->ctx->prec == GWM_F32, "the surrogate runs on FP32 states");
+    ssprk3<float>(s, *c, b);
+    tracers_ssprk3<float>(
+        s, *c, b ? (b->mode == GWM_BOUNDARY_STRIPS ? GWM_BOUNDARY_STRIPS : GWM_BOUNDARY_NONE)
+                 : GWM_BOUNDARY_NONE);
+    warm_rain_surrogate(s, *wr, m, 1, nullptr);
   });
 }
 

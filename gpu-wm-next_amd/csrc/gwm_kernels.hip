@@ -8,6 +8,7 @@
 // reference's owned-interface averaging (halo_exchange.cpp:217-285) an
 // identity inside the step; gwm_sync_interfaces() establishes it on entry.
 #include "gwm_kernels.hpp"
+#include "gwm_micro.hpp"
 
 namespace gwm {
 
@@ -556,6 +557,7 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
   const bool nominal = a.fallout == 1, metric = a.fallout == 2;
   const R dz_nom = rmax(a.depth, R(1.0e-3));
   const R frac_nom = rmin(rmax(a.vt * dt / dz_nom, R(0)), R(1));
+  const MicroCfg<R> mc{dt, a.relax, a.thr, a.krate, a.erate, a.kacc, a.heat};
   // The loads of the level below are issued before the current level's
   // arithmetic, so the march overlaps memory latency with compute.
   const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
@@ -573,49 +575,9 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
       n_qc = a.qc[cn];
       n_qr = a.qr[cn];
     }
-    R th = rt0 / rho;
-    R p = d_pressure(rho, rt0);
-    R T = d_theta(rho, rt0) * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
-    const R ex = rmax(T / rmax(th, R(1.0e-6)), R(1.0e-6));
-    R qv = rmax(in_qv / rho, R(0));
-    R qc = rmax(in_qc / rho, R(0));
-    R qr = rmax(in_qr / rho, R(0));
-    R qs = d_qsat(T, p);
-    const R lfac = Phys<R>::Lv / (d_cp<R>() * rmax(ex, R(1.0e-6)));
-    if (qv > qs) {
-      const R cnd = rmin((qv - qs) * rmax(a.relax, R(0)), qv);
-      qv -= cnd;
-      qc += cnd;
-      if (a.heat && cnd > R(0)) th += lfac * cnd;
-    } else if (qv < qs && qc > R(0)) {
-      const R e = rmin((qs - qv) * rmax(a.relax, R(0)), qc);
-      qv += e;
-      qc -= e;
-      if (a.heat && e > R(0)) th -= lfac * e;
-    }
-    const R exc = rmax(qc - a.thr, R(0));
-    if (exc > R(0) && a.krate > R(0)) {
-      const R cv = rmin(exc, dt * a.krate * exc);
-      qc -= cv;
-      qr += cv;
-    }
-    if (a.kacc > R(0) && qc > R(0) && qr > R(0)) {  // accretion extension
-      const R ac = rmin(qc, dt * a.kacc * qc * qr);
-      qc -= ac;
-      qr += ac;
-    }
-    p = d_pressure(rho, rho * th);
-    T = th * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
-    qs = d_qsat(T, p);
-    if (qr > R(0) && qv < qs && a.erate > R(0)) {
-      const R e = rmin(qr, dt * a.erate * rmin(qs - qv, qr));
-      qr -= e;
-      qv += e;
-      if (a.heat && e > R(0)) th -= lfac * e;
-    }
-    qv = rmax(qv, R(0));
-    qc = rmax(qc, R(0));
-    qr = rmax(qr, R(0));
+    MicroCell<R> m = micro_prepare(rho, rt0, in_qv, in_qc, in_qr);
+    micro_apply(m, mc);
+    const R th = m.th, qv = m.qv, qc = m.qc, qr = m.qr;
     a.qv[c] = rho * qv;
     a.qc[c] = rho * qc;
     R rqr = rho * qr;

@@ -1,0 +1,544 @@
+// gwm_surrogate.hip -- the warm-rain surrogate: a per-cell MLP that replaces
+// the source terms of K24 (src/physics/warm_rain.cu:36-120).  No counterpart
+// in the reference; the plain scheme (gwm_micro.hpp) generates its training
+// data, is its fallback outside the training box, and judges its error.
+//
+// Network: 7 features (T, p, rho_d, qv, qc, qr, dt; normalised) -> 64 -> 64
+// -> 64 -> 4 outputs (dqv, dqc, dqr, dtheta; normalised), GELU (tanh form)
+// after each hidden layer.  Parameters, FP32 master copy, flat:
+//   W1[7][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][4] b4[4]
+// (9092 values, W[in][out]).
+//
+// Training runs in FP32 (forward, backward and Adam kernels below), so its
+// gradients can be checked against an FP64 host restatement.  Inference runs
+// on the matrix cores: BF16 weights and activations, FP32 accumulation.
+#include "gwm_kernels.hpp"
+#include "gwm_micro.hpp"
+#include "gwm_surrogate.hpp"
+
+namespace gwm {
+
+namespace {
+
+constexpr float kGeluA = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float kGeluB = 0.044715f;
+
+__device__ inline float gelu_f(float x) {
+  const float u = kGeluA * (x + kGeluB * x * x * x);
+  return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ inline float gelu_grad_f(float x) {
+  const float u = kGeluA * (x + kGeluB * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluA * (1.0f + 3.0f * kGeluB * x * x);
+}
+// Same function as gelu_f written as x * sigmoid(2u), for inference.
+__device__ inline float gelu_fast(float x) {
+  const float u = kGeluA * (x + kGeluB * x * x * x);
+  return x / (1.0f + __expf(-2.0f * u));
+}
+
+// FP32 -> BF16 with round-to-nearest-even (NaN kept quiet).
+__device__ inline unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+
+__device__ inline unsigned long long mix64(unsigned long long z) {  // splitmix64 finaliser
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Training-data sampler: n cells drawn by a counter-based hash of (seed, s)
+// from the owned cells; features are the raw (T, p, rho_d, qv, qc, qr, dt)
+// of the cell before the adjustment, targets the plain scheme's change
+// (dqv, dqc, dqr, dtheta) over one call.
+// ===========================================================================
+__global__ void k_sur_sample(Geo g, const float* rho, const float* rt, const float* qv,
+                             const float* qc, const float* qr, MicroCfg<float> mc,
+                             unsigned long long seed, long n, float* x, float* y,
+                             long long* cell_out) {
+  const long s = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const unsigned long long cells = static_cast<unsigned long long>(g.nx) * g.ny * g.nz;
+  const unsigned long long r = mix64(seed * 0x100000001b3ull + static_cast<unsigned long long>(s));
+  const unsigned long long lin = r % cells;
+  const int i = static_cast<int>(lin % g.nx);
+  const int j = static_cast<int>((lin / g.nx) % g.ny);
+  const int k = static_cast<int>(lin / (static_cast<unsigned long long>(g.nx) * g.ny));
+  const size_t c = g.cell(i, j, k);
+  if (cell_out) cell_out[s] = static_cast<long long>(lin);
+  MicroCell<float> m = micro_prepare(rho[c], rt[c], qv[c], qc[c], qr[c]);
+  float* xs = x + s * kSurIn;
+  xs[0] = m.T;
+  xs[1] = m.p;
+  xs[2] = m.rho;
+  xs[3] = m.qv;
+  xs[4] = m.qc;
+  xs[5] = m.qr;
+  xs[6] = mc.dt;
+  const float th0 = m.th, qv0 = m.qv, qc0 = m.qc, qr0 = m.qr;
+  micro_apply(m, mc);
+  float* ys = y + s * kSurOut;
+  ys[0] = m.qv - qv0;
+  ys[1] = m.qc - qc0;
+  ys[2] = m.qr - qr0;
+  ys[3] = m.th - th0;
+}
+
+// ===========================================================================
+// FP32 training kernels.  Row-major activations [B][width].
+// ===========================================================================
+
+// x_n[b][0..7] = (x[b][f] - mean) / std, feature 7 = 0.
+__global__ void k_sur_norm_in(const float* x, long n, SurNorm nm, float* xn) {
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n * 8) return;
+  const long b = t >> 3;
+  const int f = static_cast<int>(t & 7);
+  xn[t] = f < kSurIn ? (x[b * kSurIn + f] - nm.xm[f]) / nm.xs[f] : 0.0f;
+}
+
+// z = in W + b; a = gelu(z) (act) or z.  Thread per (row, out).
+__global__ void k_dense_fwd(const float* in, int ldi, int K, const float* W, const float* bias,
+                            int N, int B, int act, float* z, float* a) {
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long>(B) * N) return;
+  const long b = t / N;
+  const int n = static_cast<int>(t % N);
+  float s = bias[n];
+  const float* row = in + b * ldi;
+  for (int k = 0; k < K; ++k) s += row[k] * W[k * N + n];
+  z[t] = s;
+  if (a) a[t] = act ? gelu_f(s) : s;
+}
+
+// dy = 2 (y_pred - y_target_n) / (B * 4); per-block partial loss sums.
+__global__ __launch_bounds__(256) void k_mse_grad(const float* yp, const float* yt, int B,
+                                                  SurNorm nm, float* dy, double* part) {
+  __shared__ double sh[256];
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (t < static_cast<long>(B) * kSurOut) {
+    const int o = static_cast<int>(t % kSurOut);
+    const float target = (yt[t] - nm.ym[o]) / nm.ys[o];
+    const float d = yp[t] - target;
+    dy[t] = 2.0f * d / static_cast<float>(B * kSurOut);
+    l = static_cast<double>(d) * d;
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// dz_in[b][k] = (sum_n dz[b][n] W[k][n]) * gelu'(z_in[b][k]).
+__global__ void k_dense_bwd_data(const float* dz, int N, const float* W, int K, const float* zin,
+                                 int B, float* dzin) {
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long>(B) * K) return;
+  const long b = t / K;
+  const int k = static_cast<int>(t % K);
+  float s = 0.0f;
+  const float* row = dz + b * N;
+  for (int n = 0; n < N; ++n) s += row[n] * W[k * N + n];
+  dzin[t] = s * gelu_grad_f(zin[t]);
+}
+
+// Partial weight gradients over a slice of the batch: out[slice][k][n]
+// (k == K is the bias row).  Fixed summation order.
+__global__ void k_dense_bwd_weight(const float* in, int ldi, int K, const float* dz, int N, int B,
+                                   int slices, float* out) {
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long per = static_cast<long>(K + 1) * N;
+  if (t >= per * slices) return;
+  const int sl = static_cast<int>(t / per);
+  const int kn = static_cast<int>(t % per);
+  const int k = kn / N, n = kn % N;
+  const int b0 = static_cast<int>((static_cast<long>(B) * sl) / slices);
+  const int b1 = static_cast<int>((static_cast<long>(B) * (sl + 1)) / slices);
+  float s = 0.0f;
+  for (int b = b0; b < b1; ++b) s += (k < K ? in[static_cast<long>(b) * ldi + k] : 1.0f) * dz[static_cast<long>(b) * N + n];
+  out[t] = s;
+}
+
+// g[W offset + k*N + n] / g[b offset + n] = sum over slices.
+__global__ void k_dense_grad_reduce(const float* part, int K, int N, int slices, float* gW,
+                                    float* gb) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = (K + 1) * N;
+  if (t >= per) return;
+  float s = 0.0f;
+  for (int sl = 0; sl < slices; ++sl) s += part[static_cast<long>(sl) * per + t];
+  const int k = t / N, n = t % N;
+  if (k < K) gW[k * N + n] = s;
+  else gb[n] = s;
+}
+
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float gt = g[t];
+  const float mt = b1 * m[t] + (1.0f - b1) * gt;
+  const float vt = b2 * v[t] + (1.0f - b2) * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+// ===========================================================================
+// BF16 MFMA inference.  A wave evaluates 16 cells at a time with the
+// transposed form h' = W^T h: the output tile of v_mfma_f32_16x16x* (lane l
+// holds rows 4*(l/16) .. +3 of column l%16) is, after GELU and rounding to
+// BF16, exactly the operand layout of the next layer's k-slice, so
+// activations never leave registers.  Layer 1 (K = 7 -> 16) uses the
+// 16x16x16 form; layers 2-4 use the gfx950 16x16x32 form with the k order
+// permuted identically in both operands: k slot v of lane group g stands for
+// neuron 32*kt + (v < 4 ? 4g + v : 16 + 4g + v - 4).  Weights are held in
+// registers (converted once per wave), biases in LDS.
+// ===========================================================================
+namespace {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct MlpRegs {
+  s16x4 a1[4];      // layer 1, per output tile
+  bf16x8 a2[4][2];  // layer 2, [out tile][k tile]
+  bf16x8 a3[4][2];
+  bf16x8 a4[2];     // layer 4, [k tile]
+};
+
+__device__ inline __bf16 bf16_of(float f) { return __builtin_bit_cast(__bf16, to_bf16(f)); }
+
+__device__ inline int perm_k(int kt, int g, int v) {
+  return 32 * kt + (v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4));
+}
+
+__device__ inline void load_regs(const float* w, MlpRegs& R, int c, int g) {
+  const float* W1 = w + kSurW1;
+  const float* W2 = w + kSurW2;
+  const float* W3 = w + kSurW3;
+  const float* W4 = w + kSurW4;
+  for (int mt = 0; mt < 4; ++mt) {
+    s16x4 a;
+    for (int v = 0; v < 4; ++v) {
+      const int k = 4 * g + v;
+      a[v] = static_cast<short>(to_bf16(k < kSurIn ? W1[k * 64 + mt * 16 + c] : 0.0f));
+    }
+    R.a1[mt] = a;
+    for (int kt = 0; kt < 2; ++kt) {
+      bf16x8 b2, b3;
+      for (int v = 0; v < 8; ++v) {
+        const int k = perm_k(kt, g, v);
+        b2[v] = bf16_of(W2[k * 64 + mt * 16 + c]);
+        b3[v] = bf16_of(W3[k * 64 + mt * 16 + c]);
+      }
+      R.a2[mt][kt] = b2;
+      R.a3[mt][kt] = b3;
+    }
+  }
+  for (int kt = 0; kt < 2; ++kt) {
+    bf16x8 b4;
+    for (int v = 0; v < 8; ++v) {
+      const int k = perm_k(kt, g, v);
+      b4[v] = bf16_of(c < kSurOut ? W4[k * kSurOut + c] : 0.0f);
+    }
+    R.a4[kt] = b4;
+  }
+}
+
+// Biases into LDS: sb[0..191] hidden layers, sb[192..195] outputs.
+__device__ inline void load_bias(const float* w, float* sb) {
+  for (int t = threadIdx.x + blockDim.x * threadIdx.y; t < 196; t += blockDim.x * blockDim.y) {
+    if (t < 64) sb[t] = w[kSurB1 + t];
+    else if (t < 128) sb[t] = w[kSurB2 + t - 64];
+    else if (t < 192) sb[t] = w[kSurB3 + t - 128];
+    else sb[t] = w[kSurB4 + t - 192];
+  }
+}
+
+// One group of 16 cells.  xin: this lane's normalised features 4g .. 4g+3
+// (zero for g >= 2).  Returns, for lanes with g == 0, the normalised
+// outputs 0..3 of cell l % 16.
+__device__ inline f32x4 mlp_group(const MlpRegs& R, const float* sb, const float xin[4], int g) {
+  s16x4 bx;
+  for (int v = 0; v < 4; ++v) bx[v] = static_cast<short>(to_bf16(xin[v]));
+  f32x4 h[4];
+  for (int mt = 0; mt < 4; ++mt) {
+    f32x4 acc;
+    for (int r = 0; r < 4; ++r) acc[r] = sb[mt * 16 + 4 * g + r];
+    h[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.a1[mt], bx, acc, 0, 0, 0);
+  }
+  for (int layer = 0; layer < 2; ++layer) {
+    bf16x8 bk[2];
+    for (int kt = 0; kt < 2; ++kt)
+      for (int v = 0; v < 4; ++v) {
+        bk[kt][v] = bf16_of(gelu_fast(h[2 * kt][v]));
+        bk[kt][v + 4] = bf16_of(gelu_fast(h[2 * kt + 1][v]));
+      }
+    const float* b = sb + 64 * (layer + 1);
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 acc;
+      for (int r = 0; r < 4; ++r) acc[r] = b[mt * 16 + 4 * g + r];
+      const bf16x8* a = layer == 0 ? R.a2[mt] : R.a3[mt];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bk[0], acc, 0, 0, 0);
+      h[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bk[1], acc, 0, 0, 0);
+    }
+  }
+  bf16x8 bk[2];
+  for (int kt = 0; kt < 2; ++kt)
+    for (int v = 0; v < 4; ++v) {
+      bk[kt][v] = bf16_of(gelu_fast(h[2 * kt][v]));
+      bk[kt][v + 4] = bf16_of(gelu_fast(h[2 * kt + 1][v]));
+    }
+  f32x4 acc;
+  for (int r = 0; r < 4; ++r) acc[r] = g == 0 ? sb[192 + r] : 0.0f;
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a4[0], bk[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a4[1], bk[1], acc, 0, 0, 0);
+  return acc;
+}
+
+__device__ inline void norm_features(const float f[kSurIn], const SurNorm& nm, int g,
+                                     float xin[4]) {
+  for (int v = 0; v < 4; ++v) {
+    const int k = 4 * g + v;
+    xin[v] = k < kSurIn ? (f[k] - nm.xm[k]) / nm.xs[k] : 0.0f;
+  }
+}
+
+}  // namespace
+
+// Raw features x[n][7] -> raw outputs y[n][4] (tests and timing).
+__global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, const float* x,
+                                                   long n, float* y) {
+  __shared__ float sb[196];
+  load_bias(w, sb);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, g = lane >> 4;
+  MlpRegs R;
+  load_regs(w, R, c, g);
+  const long wave = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const long waves = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
+  for (long grp = wave; grp * 16 < n; grp += waves) {
+    const long cell = grp * 16 + c;
+    float f[kSurIn];
+    for (int q = 0; q < kSurIn; ++q) f[q] = cell < n ? x[cell * kSurIn + q] : nm.xm[q];
+    float xin[4];
+    norm_features(f, nm, g, xin);
+    const f32x4 o = mlp_group(R, sb, xin, g);
+    if (g == 0 && cell < n)
+      for (int q = 0; q < kSurOut; ++q) y[cell * kSurOut + q] = o[q] * nm.ys[q] + nm.ym[q];
+  }
+}
+
+// Warm-rain source terms with the surrogate (fallout runs separately):
+// cells whose features all lie in the box use the network's tendencies,
+// the others the plain scheme.  Counts the fallback cells.
+__global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
+                                                       float* qv, float* qc, float* qr,
+                                                       MicroCfg<float> mc, const float* w,
+                                                       SurNorm nm, unsigned long long* nfall) {
+  __shared__ float sb[196];
+  __shared__ unsigned cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  load_bias(w, sb);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, gg = lane >> 4;
+  MlpRegs R;
+  load_regs(w, R, c, gg);
+  const long n = static_cast<long>(g.nx) * g.ny * g.nz;
+  const long wave = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const long waves = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
+  unsigned mine = 0;
+  for (long grp = wave; grp * 16 < n; grp += waves) {
+    const long lin = grp * 16 + c;
+    const bool valid = lin < n;
+    size_t ci = 0;
+    MicroCell<float> m{};
+    float f[kSurIn];
+    if (valid) {
+      const int i = static_cast<int>(lin % g.nx);
+      const int j = static_cast<int>((lin / g.nx) % g.ny);
+      const int k = static_cast<int>(lin / (static_cast<long>(g.nx) * g.ny));
+      ci = g.cell(i, j, k);
+      m = micro_prepare(rho[ci], rt[ci], qv[ci], qc[ci], qr[ci]);
+      f[0] = m.T; f[1] = m.p; f[2] = m.rho; f[3] = m.qv; f[4] = m.qc; f[5] = m.qr; f[6] = mc.dt;
+    } else {
+      for (int q = 0; q < kSurIn; ++q) f[q] = nm.xm[q];
+    }
+    bool inbox = true;
+    for (int q = 0; q < kSurIn; ++q) inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
+    float xin[4];
+    norm_features(f, nm, gg, xin);
+    const f32x4 o = mlp_group(R, sb, xin, gg);
+    if (gg == 0 && valid) {
+      if (inbox) {
+        m.qv = fmaxf(m.qv + (o[0] * nm.ys[0] + nm.ym[0]), 0.0f);
+        m.qc = fmaxf(m.qc + (o[1] * nm.ys[1] + nm.ym[1]), 0.0f);
+        m.qr = fmaxf(m.qr + (o[2] * nm.ys[2] + nm.ym[2]), 0.0f);
+        m.th = m.th + (o[3] * nm.ys[3] + nm.ym[3]);
+      } else {
+        micro_apply(m, mc);
+        ++mine;
+      }
+      qv[ci] = m.rho * m.qv;
+      qc[ci] = m.rho * m.qc;
+      qr[ci] = m.rho * m.qr;
+      rt[ci] = m.rho * m.th;
+    }
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(nfall, static_cast<unsigned long long>(cnt));
+}
+
+// Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
+// warm_rain.cu:122-191), one thread per column, top down.
+template <bool TERRAIN>
+__global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
+                          const float* lev_idz_c, float dt, float vt, float depth, int fallout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const float dz_nom = fmaxf(depth, 1.0e-3f);
+  const float frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
+  float inc = 0.0f;
+  for (int k = g.nz - 1; k >= 0; --k) {
+    const size_t c = g.cell(i, j, k);
+    float rqr = qr[c];
+    if (fallout == 1) {
+      const float old = fmaxf(rqr, 0.0f) * dz_nom;
+      const float out = frac_nom * old;
+      rqr = (old - out + inc) / dz_nom;
+      inc = out;
+    } else {
+      const float idz = fmaxf(TERRAIN ? idz_c[g.metric(i, j, k)] : lev_idz_c[k], 1.0e-6f);
+      const float dz = 1.0f / idz;
+      const float frac = fminf(fmaxf(vt * dt * idz, 0.0f), 1.0f);
+      const float old = fmaxf(rqr, 0.0f) * dz;
+      const float out = frac * old;
+      rqr = (old - out + inc) * idz;
+      inc = out;
+    }
+    qr[c] = rqr;
+  }
+  precip[static_cast<size_t>(j) * g.nx + i] += inc;
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+namespace {
+inline unsigned nblk(long n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+inline unsigned infer_blocks(long groups) {
+  const long waves = groups < 2048 ? groups : 2048;
+  return static_cast<unsigned>((waves + 3) / 4);
+}
+}  // namespace
+
+void launch_sur_sample(hipStream_t st, const Geo& g, const float* rho, const float* rt,
+                       const float* qv, const float* qc, const float* qr,
+                       const MicroCfg<float>& mc, unsigned long long seed, long n, float* x,
+                       float* y, long long* cell) {
+  hipLaunchKernelGGL(k_sur_sample, dim3(nblk(n, 256)), dim3(256), 0, st, g, rho, rt, qv, qc, qr,
+                     mc, seed, n, x, y, cell);
+}
+
+void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const float* y, int B,
+                           const SurNorm& nm, bool update, double* loss_part, int* nloss) {
+  float* p = T.params;
+  // forward
+  hipLaunchKernelGGL(k_sur_norm_in, dim3(nblk(static_cast<long>(B) * 8, 256)), dim3(256), 0, st,
+                     x, static_cast<long>(B), nm, T.xn);
+  const float* in = T.xn;
+  int ldi = 8, K = kSurIn;
+  const int wofs[4] = {kSurW1, kSurW2, kSurW3, kSurW4};
+  const int bofs[4] = {kSurB1, kSurB2, kSurB3, kSurB4};
+  for (int l = 0; l < 4; ++l) {
+    const int N = l < 3 ? 64 : kSurOut;
+    hipLaunchKernelGGL(k_dense_fwd, dim3(nblk(static_cast<long>(B) * N, 256)), dim3(256), 0, st,
+                       in, ldi, K, p + wofs[l], p + bofs[l], N, B, l < 3 ? 1 : 0, T.z[l],
+                       l < 3 ? T.a[l] : nullptr);
+    in = l < 3 ? T.a[l] : nullptr;
+    ldi = 64;
+    K = 64;
+  }
+  // loss and output gradient
+  const unsigned lb = nblk(static_cast<long>(B) * kSurOut, 256);
+  hipLaunchKernelGGL(k_mse_grad, dim3(lb), dim3(256), 0, st, T.z[3], y, B, nm, T.dz[3],
+                     loss_part);
+  if (nloss) *nloss = static_cast<int>(lb);
+  // backward
+  const int slices = 16;
+  for (int l = 3; l >= 0; --l) {
+    const int N = l < 3 ? 64 : kSurOut;
+    const int Kl = l == 0 ? kSurIn : 64;
+    const float* inl = l == 0 ? T.xn : T.a[l - 1];
+    const int ldl = l == 0 ? 8 : 64;
+    const long parts = static_cast<long>(Kl + 1) * N * slices;
+    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nblk(parts, 256)), dim3(256), 0, st, inl, ldl,
+                       Kl, T.dz[l], N, B, slices, T.wpart);
+    hipLaunchKernelGGL(k_dense_grad_reduce, dim3(nblk(static_cast<long>(Kl + 1) * N, 256)),
+                       dim3(256), 0, st, T.wpart, Kl, N, slices, T.grad + wofs[l],
+                       T.grad + bofs[l]);
+    if (l > 0)
+      hipLaunchKernelGGL(k_dense_bwd_data, dim3(nblk(static_cast<long>(B) * 64, 256)), dim3(256),
+                         0, st, T.dz[l], N, p + wofs[l], 64, T.z[l - 1], B, T.dz[l - 1]);
+  }
+  if (update) {
+    T.step += 1;
+    const float c1 = 1.0f - powf(T.beta1, static_cast<float>(T.step));
+    const float c2 = 1.0f - powf(T.beta2, static_cast<float>(T.step));
+    hipLaunchKernelGGL(k_adam, dim3(nblk(kSurParams, 256)), dim3(256), 0, st, p, T.grad, T.m, T.v,
+                       kSurParams, T.lr, T.beta1, T.beta2, T.eps, c1, c2);
+  }
+}
+
+void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const float* x, long n,
+                      float* y) {
+  hipLaunchKernelGGL(k_sur_infer, dim3(infer_blocks((n + 15) / 16)), dim3(256), 0, st, w, nm, x,
+                     n, y);
+}
+
+void launch_zero_counter(hipStream_t st, unsigned long long* p) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, p);
+}
+
+void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
+                          float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
+                          const SurNorm& nm, unsigned long long* nfall) {
+  const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
+  hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks((cells + 15) / 16)), dim3(256), 0, st, g,
+                     rho, rt, qv, qc, qr, mc, w, nm, nfall);
+}
+
+void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
+                    const float* lev_idz_c, float dt, float vt, float depth, int fallout,
+                    bool terrain) {
+  const dim3 b(64, 4, 1), gr((g.nx + 63) / 64, (g.ny + 3) / 4);
+  if (terrain)
+    hipLaunchKernelGGL(k_fallout<true>, gr, b, 0, st, g, qr, precip, idz_c, lev_idz_c, dt, vt,
+                       depth, fallout);
+  else
+    hipLaunchKernelGGL(k_fallout<false>, gr, b, 0, st, g, qr, precip, idz_c, lev_idz_c, dt, vt,
+                       depth, fallout);
+}
+
+}  // namespace gwm

@@ -66,6 +66,15 @@ class WarmRainConfig(ctypes.Structure):
                 ("accretion_rate", ctypes.c_double)]
 
 
+class SurrogateNorm(ctypes.Structure):
+    _fields_ = [("x_mean", ctypes.c_float * 7), ("x_std", ctypes.c_float * 7),
+                ("y_mean", ctypes.c_float * 4), ("y_std", ctypes.c_float * 4),
+                ("box_lo", ctypes.c_float * 7), ("box_hi", ctypes.c_float * 7)]
+
+
+SURROGATE_IN, SURROGATE_OUT, SURROGATE_PARAMS = 7, 4, 9092
+
+
 def warm_rain_config(dt=2.0, condensation_relaxation=1.0, cloud_autoconversion_threshold=1.0e-4,
                      cloud_autoconversion_rate=1.0e-3, rain_evaporation_rate=5.0e-3,
                      rain_terminal_velocity=0.0, sedimentation_layer_depth_m=0.0,
@@ -166,6 +175,22 @@ def lib():
         L.gwm_init_mountain_wave.argtypes = [vp, dp, dp, dp, dp, dp]
         L.gwm_init_constant_n.argtypes = [vp, dp, dp, dp, dp, dp]
         L.gwm_init_convection.argtypes = [vp, dp, ctypes.c_ulonglong, dp, dp]
+        ull, lp = ctypes.c_ulonglong, ctypes.c_long
+        L.gwm_surrogate_create.argtypes = [vp, ull, pvp]
+        L.gwm_surrogate_destroy.argtypes = [vp]
+        L.gwm_surrogate_set_norm.argtypes = [vp, ctypes.POINTER(SurrogateNorm)]
+        L.gwm_surrogate_get_norm.argtypes = [vp, ctypes.POINTER(SurrogateNorm)]
+        L.gwm_surrogate_set_params.argtypes = [vp, vp]
+        L.gwm_surrogate_get_params.argtypes = [vp, vp]
+        L.gwm_surrogate_sample.argtypes = [vp, ctypes.POINTER(WarmRainConfig), ull, lp, vp, vp, vp]
+        L.gwm_surrogate_train.argtypes = [vp, vp, vp, lp, ip, ip, dp, dp, vp]
+        L.gwm_surrogate_loss_grad.argtypes = [vp, vp, vp, ip, vp, vp]
+        L.gwm_surrogate_infer.argtypes = [vp, vp, lp, vp]
+        L.gwm_warm_rain_surrogate.argtypes = [vp, ctypes.POINTER(WarmRainConfig), vp, ip,
+                                              ctypes.POINTER(ctypes.c_longlong)]
+        L.gwm_moist_step_surrogate.argtypes = [vp, ctypes.POINTER(StepConfig),
+                                               ctypes.POINTER(WarmRainConfig), vp,
+                                               ctypes.POINTER(BoundaryConfig)]
         _lib = L
     return _lib
 
@@ -372,6 +397,23 @@ class State:
         b = None if boundary is None else ctypes.byref(boundary)
         check(lib().gwm_moist_step(self.h, ctypes.byref(cfg), ctypes.byref(wr), b))
 
+    def copy_from(self, other):
+        """Device-to-device copy of every field of another state of the grid."""
+        check(lib().gwm_state_copy(self.h, other.h))
+
+    def warm_rain_surrogate(self, cfg, surrogate, metric_fallout=True):
+        """Warm rain with the surrogate source terms; returns the number of
+        cells that fell back to the plain scheme."""
+        n = ctypes.c_longlong()
+        check(lib().gwm_warm_rain_surrogate(self.h, ctypes.byref(cfg), surrogate.h,
+                                            int(metric_fallout), ctypes.byref(n)))
+        return n.value
+
+    def moist_step_surrogate(self, cfg, wr, surrogate, boundary=None):
+        b = None if boundary is None else ctypes.byref(boundary)
+        check(lib().gwm_moist_step_surrogate(self.h, ctypes.byref(cfg), ctypes.byref(wr),
+                                             surrogate.h, b))
+
     def summary(self):
         out = np.zeros(32, np.float64)
         check(lib().gwm_budget_sums(self.h, _ptr(out)))
@@ -400,3 +442,111 @@ class State:
 
     def init_convection(self, p_surface=1.0e5, seed=0, noise_amplitude=0.5, noise_top_m=1000.0):
         check(lib().gwm_init_convection(self.h, p_surface, seed, noise_amplitude, noise_top_m))
+
+
+class DeviceArray:
+    """A device allocation of the context (raw bytes), with host copies."""
+
+    def __init__(self, ctx, shape, dtype=np.float32):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        check(lib().gwm_device_alloc(ctx.h, self.nbytes, ctypes.byref(self.ptr)))
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise GwmError("DeviceArray.upload: size mismatch")
+        check(lib().gwm_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().gwm_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def download_rows(self, rows):
+        """Host copy of the first `rows` rows."""
+        rows = min(int(rows), self.shape[0])
+        out = np.empty((rows,) + self.shape[1:], self.dtype)
+        check(lib().gwm_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def offset(self, rows):
+        """Pointer to row `rows` of a 2-D array."""
+        row_bytes = self.nbytes // self.shape[0]
+        return ctypes.c_void_p(self.ptr.value + rows * row_bytes)
+
+    def free(self):
+        if self.ptr and not _shutting_down:
+            lib().gwm_device_free(self.ctx.h, self.ptr)
+        self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """The warm-rain surrogate MLP (parameters on the device)."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().gwm_surrogate_create(ctx.h, int(seed), ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and not _shutting_down:
+            lib().gwm_surrogate_destroy(self.h)
+        self.h = None
+
+    def set_norm(self, x_mean, x_std, y_mean, y_std, box_lo, box_hi):
+        n = SurrogateNorm()
+        for name, v, k in (("x_mean", x_mean, 7), ("x_std", x_std, 7), ("y_mean", y_mean, 4),
+                           ("y_std", y_std, 4), ("box_lo", box_lo, 7), ("box_hi", box_hi, 7)):
+            getattr(n, name)[:] = [float(t) for t in np.asarray(v, np.float64).reshape(k)]
+        check(lib().gwm_surrogate_set_norm(self.h, ctypes.byref(n)))
+
+    def norm(self):
+        n = SurrogateNorm()
+        check(lib().gwm_surrogate_get_norm(self.h, ctypes.byref(n)))
+        return {k: np.array(getattr(n, k)[:], np.float32) for k, _ in SurrogateNorm._fields_}
+
+    def params(self):
+        out = np.empty(SURROGATE_PARAMS, np.float32)
+        check(lib().gwm_surrogate_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        if a.size != SURROGATE_PARAMS:
+            raise GwmError("surrogate parameters: wrong size")
+        check(lib().gwm_surrogate_set_params(self.h, _ptr(a)))
+
+    def train(self, x, y, n, batch=4096, epochs=3, lr=2e-3, lr_final=1e-4, x_row=0):
+        losses = np.zeros(epochs, np.float64)
+        check(lib().gwm_surrogate_train(self.h, x.offset(x_row), y.offset(x_row), int(n),
+                                        int(batch), int(epochs), float(lr), float(lr_final),
+                                        _ptr(losses)))
+        return losses
+
+    def loss_grad(self, x, y, n):
+        loss = ctypes.c_double()
+        g = np.empty(SURROGATE_PARAMS, np.float32)
+        check(lib().gwm_surrogate_loss_grad(self.h, x.ptr, y.ptr, int(n), ctypes.byref(loss),
+                                            _ptr(g)))
+        return loss.value, g
+
+    def infer(self, x, n, y):
+        check(lib().gwm_surrogate_infer(self.h, x.ptr, int(n), y.ptr))
+
+
+def surrogate_sample(state, cfg, seed, n, x, y, cell=None, row=0):
+    """Training pairs from a state into device arrays x [.., 7], y [.., 4]
+    (starting at `row`); `cell` optionally receives the cell indices."""
+    check(lib().gwm_surrogate_sample(state.h, ctypes.byref(cfg), int(seed), int(n),
+                                     x.offset(row), y.offset(row),
+                                     None if cell is None else cell.offset(row)))

@@ -32,15 +32,18 @@ struct Options {
   bool periodic_x = true, periodic_y = true;
   std::string terrain_kind = "flat";
   double tcx = 0.5, tcy = 0.5, thx = 0.15, thy = 0.15, theight = 0.0;
-  // ThermoBubbleConfig / MountainWaveBackgroundConfig defaults
-  float rho_background = 1.0f, theta_background = 300.0f;
-  float u = 0.0f, v = 0.0f, w = 0.0f;
+  // ThermoBubbleConfig / MountainWaveBackgroundConfig defaults.  Values are
+  // kept in double and rounded to the context precision where used, so an
+  // FP32 run sees the reference's float values and an FP64 run the decimal
+  // ones.
+  double rho_background = 1.0, theta_background = 300.0;
+  double u = 0.0, v = 0.0, w = 0.0;
   bool u_set = false;
-  float theta_perturbation = 2.0f;
-  float cx = 0.5f, cy = 0.5f, cz = 0.5f, rx = 0.1f, ry = 0.1f, rz = 0.1f;
-  float rho_surface = 1.18f, theta_ref = 300.0f, density_scale_height = 8200.0f;
+  double theta_perturbation = 2.0;
+  double cx = 0.5, cy = 0.5, cz = 0.5, rx = 0.1, ry = 0.1, rz = 0.1;
+  double rho_surface = 1.18, theta_ref = 300.0, density_scale_height = 8200.0;
   // DryStepperConfig
-  float dt = 2.0f, gravity = 9.81f;
+  double dt = 2.0, gravity = 9.81;
   int fast_substeps = 1;
   int steps = 10;
   std::string summary_json_path, plan_view_json_path;
@@ -68,7 +71,6 @@ void parse(int argc, char** argv, Options& o) {
     };
     auto i32 = [&] { return std::stoi(val()); };
     auto f64 = [&] { return std::stod(val()); };
-    auto f32 = [&] { return std::stof(val()); };
     if (a == "--case") o.case_kind = val();
     else if (a == "--nx") o.nx = i32();
     else if (a == "--ny") o.ny = i32();
@@ -90,25 +92,25 @@ void parse(int argc, char** argv, Options& o) {
     else if (a == "--terrain-half-width-x") o.thx = f64();
     else if (a == "--terrain-half-width-y") o.thy = f64();
     else if (a == "--terrain-height") o.theight = f64();
-    else if (a == "--rho-background") o.rho_background = f32();
-    else if (a == "--theta-background") o.theta_background = f32();
-    else if (a == "--u-background") { o.u = f32(); o.u_set = true; }
-    else if (a == "--v-background") o.v = f32();
-    else if (a == "--w-background") o.w = f32();
-    else if (a == "--theta-perturbation") o.theta_perturbation = f32();
-    else if (a == "--center-x-fraction") o.cx = f32();
-    else if (a == "--center-y-fraction") o.cy = f32();
-    else if (a == "--center-z-fraction") o.cz = f32();
-    else if (a == "--radius-x-fraction") o.rx = f32();
-    else if (a == "--radius-y-fraction") o.ry = f32();
-    else if (a == "--radius-z-fraction") o.rz = f32();
-    else if (a == "--rho-surface") o.rho_surface = f32();
-    else if (a == "--theta-ref") o.theta_ref = f32();
-    else if (a == "--density-scale-height") o.density_scale_height = f32();
+    else if (a == "--rho-background") o.rho_background = f64();
+    else if (a == "--theta-background") o.theta_background = f64();
+    else if (a == "--u-background") { o.u = f64(); o.u_set = true; }
+    else if (a == "--v-background") o.v = f64();
+    else if (a == "--w-background") o.w = f64();
+    else if (a == "--theta-perturbation") o.theta_perturbation = f64();
+    else if (a == "--center-x-fraction") o.cx = f64();
+    else if (a == "--center-y-fraction") o.cy = f64();
+    else if (a == "--center-z-fraction") o.cz = f64();
+    else if (a == "--radius-x-fraction") o.rx = f64();
+    else if (a == "--radius-y-fraction") o.ry = f64();
+    else if (a == "--radius-z-fraction") o.rz = f64();
+    else if (a == "--rho-surface") o.rho_surface = f64();
+    else if (a == "--theta-ref") o.theta_ref = f64();
+    else if (a == "--density-scale-height") o.density_scale_height = f64();
     else if (a == "--steps") o.steps = i32();
-    else if (a == "--dt") o.dt = f32();
+    else if (a == "--dt") o.dt = f64();
     else if (a == "--fast-substeps") o.fast_substeps = i32();
-    else if (a == "--gravity") o.gravity = f32();
+    else if (a == "--gravity") o.gravity = f64();
     else if (a == "--summary-json") o.summary_json_path = val();
     else if (a == "--plan-view-json") o.plan_view_json_path = val();
     else if (a == "--plan-view-level") o.plan_view_level = i32();
@@ -190,7 +192,7 @@ int run(const Options& o) {
     else if (o.case_kind == "hydrostatic_rest")
       check(gwm_init_hydrostatic_rest(st, o.rho_surface, o.theta_ref));
     else if (o.case_kind == "mountain_wave_background")
-      check(gwm_init_mountain_wave(st, o.rho_surface, o.theta_ref, o.u_set ? o.u : 15.0f, o.v,
+      check(gwm_init_mountain_wave(st, o.rho_surface, o.theta_ref, o.u_set ? o.u : 15.0, o.v,
                                    o.w));
     else
       throw std::runtime_error("Unsupported case kind: " + o.case_kind);

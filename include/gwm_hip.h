@@ -294,6 +294,57 @@ int gwm_init_constant_n(gwm_state* state, double theta_surface, double brunt_vai
 int gwm_init_convection(gwm_state* state, double p_surface, unsigned long long seed,
                         double noise_amplitude, double noise_top_m);
 
+/* ---- warm-rain surrogate --------------------------------------------------
+ * No counterpart in the reference: an extension that replaces the source
+ * terms of apply_warm_rain_microphysics (warm_rain.cu:36-120) by a per-cell
+ * MLP, 7 features (T, p, rho_d, qv, qc, qr, dt) -> 64 -> 64 -> 64 -> 4
+ * outputs (dqv, dqc, dqr, dtheta), GELU (tanh form).  Parameters are FP32,
+ * flat, W[in][out]: W1 b1 W2 b2 W3 b3 W4 b4 (GWM_SURROGATE_PARAMS values).
+ * Training runs in FP32 on the device; inference on the matrix cores with
+ * BF16 weights and activations and FP32 accumulation.  Cells whose raw
+ * features leave the box [box_lo, box_hi] use the plain scheme.  All
+ * surrogate calls need an FP32 context. */
+enum { GWM_SURROGATE_IN = 7, GWM_SURROGATE_OUT = 4, GWM_SURROGATE_PARAMS = 9092 };
+typedef struct gwm_surrogate gwm_surrogate;
+typedef struct {
+  float x_mean[7], x_std[7]; /* feature normalisation */
+  float y_mean[4], y_std[4]; /* target normalisation  */
+  float box_lo[7], box_hi[7];/* fallback box, raw feature units */
+} gwm_surrogate_norm;
+/* Seeded Xavier-uniform weights, zero biases. */
+int gwm_surrogate_create(gwm_ctx* ctx, unsigned long long seed, gwm_surrogate** out);
+int gwm_surrogate_destroy(gwm_surrogate* m);
+int gwm_surrogate_set_norm(gwm_surrogate* m, const gwm_surrogate_norm* n);
+int gwm_surrogate_get_norm(gwm_surrogate* m, gwm_surrogate_norm* n);
+/* Host copies of the FP32 parameters (set also resets the optimiser). */
+int gwm_surrogate_set_params(gwm_surrogate* m, const float* host);
+int gwm_surrogate_get_params(gwm_surrogate* m, float* host);
+/* n training pairs from the state: cells drawn by a counter hash of (seed,
+ * sample); dev_x [n][7] raw features before the adjustment, dev_y [n][4]
+ * the plain scheme's change over one call (device FP32 arrays); dev_cell
+ * (optional, int64 [n]) the cell index (k*ny + j)*nx + i. */
+int gwm_surrogate_sample(gwm_state* state, const gwm_warm_rain_config* cfg,
+                         unsigned long long seed, long n, void* dev_x, void* dev_y,
+                         void* dev_cell);
+/* Adam on the mean squared error of the normalised outputs, minibatches of
+ * `batch` consecutive rows, learning rate cosine-decayed from lr to
+ * lr_final; epoch_loss[e] = mean squared normalised error of epoch e. */
+int gwm_surrogate_train(gwm_surrogate* m, const void* dev_x, const void* dev_y, long n, int batch,
+                        int epochs, double lr, double lr_final, double* epoch_loss);
+/* Loss and gradient (FP32 kernels, no update) over n rows. */
+int gwm_surrogate_loss_grad(gwm_surrogate* m, const void* dev_x, const void* dev_y, int n,
+                            double* loss, float* grad_host);
+/* BF16 matrix-core inference: raw features [n][7] -> raw outputs [n][4]. */
+int gwm_surrogate_infer(gwm_surrogate* m, const void* dev_x, long n, void* dev_y);
+/* gwm_warm_rain with the surrogate source terms (fallout unchanged);
+ * n_fallback (optional) receives the number of cells outside the box. */
+int gwm_warm_rain_surrogate(gwm_state* state, const gwm_warm_rain_config* cfg, gwm_surrogate* m,
+                            int metric_fallout, long long* n_fallback);
+/* gwm_moist_step with the surrogate warm rain. */
+int gwm_moist_step_surrogate(gwm_state* state, const gwm_step_config* cfg,
+                             const gwm_warm_rain_config* wr, gwm_surrogate* m,
+                             const gwm_boundary_config* boundary);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,10 +114,11 @@ def test_driver_summary_matches_oracle(orc, case, precision):
     final = w.summary()
     # The JSON carries six significant digits: a printed value is within
     # 5e-6 of its own magnitude.  FP32 adds the device/host FP32 trajectory
-    # difference; extremes of a field move by up to ~2e-3 of its range after
-    # three steps (the FP32 parity tests bound the rel-L2 at 2e-4).
+    # difference (device and host powf differ by an ulp now and then, and the
+    # scheme amplifies it): the extremes of w move by up to ~1 % of its range
+    # after three steps, while the FP32 parity tests bound the rel-L2 at 2e-4.
     digits = 6e-6
-    traj = 0.0 if precision == "fp64" else 2e-3
+    traj = 0.0 if precision == "fp64" else 1e-2
     bad = []
     for part, ref in (("initial", initial), ("final", final)):
         s = got[part]
