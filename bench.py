@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--surrogate-pairs", type=float, default=2e7,
                    help="training pairs of the surrogate leg, from seeds 0-7 (0 to skip)")
     p.add_argument("--surrogate-epochs", type=int, default=4)
+    p.add_argument("--ensemble-members", type=int, default=32,
+                   help="members of the batched-ensemble leg (0 to skip)")
+    p.add_argument("--ensemble-steps", type=int, default=20)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run one device-to-device state copy (profiler byte calibration)")
     return p.parse_args()
@@ -174,6 +177,7 @@ def surrogate_leg(G, a, cells, info):
     _, sstats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, dt, surrogate=sur,
                        ctx=ctx, info=sinfo)
     return {
+        "_model": (sur.params(), sur.norm()),
         "pairs": n, "seeds": seeds, "epochs": a.surrogate_epochs,
         "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
         "epoch_loss": [float(v) for v in losses],
@@ -181,6 +185,92 @@ def surrogate_leg(G, a, cells, info):
         "value": cells * a.steps / ss, "dt": dt,
         "state_finite": bool(np.isfinite(ssum["w_max"])),
         "kernels": sstats,
+    }
+
+
+def ensemble_members(G, n, seed=0, z_top=16000.0):
+    """Uneven members: nx, ny uniform in {96, 128, ..., 320}, nz in {40, 48,
+    60} (seeded); the convection sounding (moist) with, for half the members,
+    a warm bubble (+1 to +3 K, seeded centre, radius 2 km) and for the other
+    half a density current (-15 K cold bubble, radius 4 km at z = 3 km), as
+    a cos^2 theta perturbation at fixed rho_d.  One context (stream) each."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nx, ny = (int(v) for v in rng.choice(np.arange(96, 321, 32), 2))
+        nz = int(rng.choice([40, 48, 60]))
+        ctx = G.Context(0, G.F32)
+        zi = np.linspace(0.0, z_top, nz + 1)
+        grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+        st = G.State(grid, 3)
+        st.init_convection(1.0e5, 100 + m, 0.5, 1000.0)
+        warm = m % 2 == 0
+        if warm:
+            amp, r0 = float(rng.uniform(1.0, 3.0)), 2000.0
+            cx, cy, cz = rng.uniform(0.3, 0.7) * nx * 1e3, rng.uniform(0.3, 0.7) * ny * 1e3, 2000.0
+        else:
+            amp, r0 = -15.0, 4000.0
+            cx, cy, cz = 0.5 * nx * 1e3, 0.5 * ny * 1e3, 3000.0
+        zc = 0.5 * (zi[1:] + zi[:-1])
+        x = (np.arange(nx) + 0.5) * 1e3
+        y = (np.arange(ny) + 0.5) * 1e3
+        r = np.sqrt(((x[None, None, :] - cx) / r0) ** 2 + ((y[None, :, None] - cy) / r0) ** 2 +
+                    ((zc[:, None, None] - cz) / r0) ** 2)
+        dth = np.where(r < 1.0, amp * np.cos(0.5 * np.pi * r) ** 2, 0.0)
+        rho = st.get(0).astype(np.float64)
+        st.set(1, (st.get(1) + rho * dth).astype(np.float32))
+        out.append((ctx, grid, st, nx * ny * nz))
+    return out
+
+
+def ensemble_leg(G, a, sur_src):
+    """C5: 32 uneven members, FP32 + surrogate, dt 2 s, 4 substeps, advanced
+    concurrently from captured graphs (one stream per member), against the
+    same members stepped one after the other without graphs."""
+    import numpy as np
+
+    members = ensemble_members(G, a.ensemble_members)
+    cells = sum(m[3] for m in members)
+    cfg = G.step_config(dt=2.0, fast_substeps=4)
+    wr = G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0)
+    surs = None
+    if sur_src is not None:
+        params, norm = sur_src
+        surs = []
+        for ctx, _, _, _ in members:
+            s = G.Surrogate(ctx, 0)
+            s.set_norm(**{k: v for k, v in norm.items()})
+            s.set_params(params)
+            surs.append(s)
+    states = [m[2] for m in members]
+    # warm-up: captures every member's graphs
+    G.ensemble_steps(states, cfg, wr, 3, surs)
+    t0 = time.perf_counter()
+    G.ensemble_steps(states, cfg, wr, a.ensemble_steps, surs)
+    t_graph = time.perf_counter() - t0
+    # the same work launched member by member, eager
+    for st, (ctx, _, _, _) in zip(states, members):
+        ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.ensemble_steps):
+        for i, st in enumerate(states):
+            if surs is None:
+                st.moist_step(cfg, wr)
+            else:
+                st.moist_step_surrogate(cfg, wr, surs[i])
+    for ctx, _, _, _ in members:
+        ctx.sync()
+    t_eager = time.perf_counter() - t0
+    finite = all(np.isfinite(st.summary()["w_max"]) for st in states)
+    return {
+        "members": len(members), "total_cells": cells,
+        "shapes": [list(map(int, (m[2].grid.nx, m[2].grid.ny, m[2].grid.nz))) for m in members],
+        "steps": a.ensemble_steps, "surrogate": surs is not None,
+        "value_graph_concurrent": cells * a.ensemble_steps / t_graph,
+        "value_eager_sequential": cells * a.ensemble_steps / t_eager,
+        "state_finite": bool(finite),
     }
 
 
@@ -251,6 +341,11 @@ def main():
     sur = surrogate_leg(G, a, cells, info) if a.surrogate_pairs > 0 else None
     if sur is not None and fp64_control:
         sur["over_fp64_plain"] = sur["value"] / fp64_control
+    ens = None
+    model = sur.pop("_model", None) if sur else None
+    if a.ensemble_members > 0:
+        log(f"ensemble: {a.ensemble_members} members")
+        ens = ensemble_leg(G, a, model)
     if a.cpu_baseline:
         log("CPU baseline")
     cpu = (cpu_baseline(a.nx, a.ny, a.nz, a.cpu_threads, a.control_dt or a.dt)
@@ -277,6 +372,7 @@ def main():
         "fp64_plain_control_value": fp64_control,
         "fp32_over_fp64": (value / fp64) if fp64 else None,
         "surrogate": sur,
+        "ensemble": ens,
         "dry_mass": summary["total_dry_mass"],
         "kernels": stats,
         "kernel_roofline": kernel_roofline,

@@ -18,6 +18,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gwm_kernels.hpp"
@@ -298,6 +299,15 @@ struct gwm_state {
   void* rref3 = nullptr;      // nx*ny*nz
   void* pref3 = nullptr;
   void* theta_scr = nullptr;  // cell field
+  // Captured two-step moist-step graphs, one per buffer parity (the step
+  // rotates the buffer sets and tracer buffers with period two).
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    int cur = -1;
+    void* tr0 = nullptr;
+    std::vector<double> key;
+  };
+  Graph graphs[2];
   bool faces_synced = false;
   // view of a stage set
   void* vset[5] = {};
@@ -347,6 +357,8 @@ struct gwm_state {
   }
   ~gwm_state() {
     if (view) return;
+    for (auto& gr : graphs)
+      if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
     struct Release {
       gwm_ctx* c;
       ~Release() { ctx_release(c); }
@@ -1936,6 +1948,7 @@ struct gwm_surrogate {
   int batch = 0;  // rows the training buffers hold
   gwm::SurNorm norm{};
   bool have_norm = false;
+  long version = 0;  // bumped by set_norm (graphs bake the normalisation in)
   unsigned long long* nfall = nullptr;
   double* loss_part = nullptr;
   size_t loss_n = 0;
@@ -2105,6 +2118,7 @@ int gwm_surrogate_set_norm(gwm_surrogate* m, const gwm_surrogate_norm* n) {
     require(m && n, "null argument");
     m->norm = to_norm(*n);
     m->have_norm = true;
+    m->version += 1;
   });
 }
 
@@ -2262,6 +2276,202 @@ int gwm_moist_step_surrogate(gwm_state* s, const gwm_step_config* c,
         s, *c, b ? (b->mode == GWM_BOUNDARY_STRIPS ? GWM_BOUNDARY_STRIPS : GWM_BOUNDARY_NONE)
                  : GWM_BOUNDARY_NONE);
     warm_rain_surrogate(s, *wr, m, 1, nullptr);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Graph-replayed moist steps and ensembles (extension).
+// ---------------------------------------------------------------------------
+namespace {
+
+void moist_step_f32(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& wr,
+                    gwm_surrogate* M) {
+  ssprk3<float>(S, c, nullptr);
+  tracers_ssprk3<float>(S, c, GWM_BOUNDARY_NONE);
+  if (M) warm_rain_surrogate(S, wr, M, 1, nullptr);
+  else warm_rain<float>(S, wr, 1);
+}
+
+template <class R>
+void moist_step_any(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& wr,
+                    gwm_surrogate* M) {
+  if constexpr (std::is_same<R, float>::value) {
+    moist_step_f32(S, c, wr, M);
+  } else {
+    require(M == nullptr, "the surrogate runs on FP32 states");
+    ssprk3<R>(S, c, nullptr);
+    tracers_ssprk3<R>(S, c, GWM_BOUNDARY_NONE);
+    warm_rain<R>(S, wr, 1);
+  }
+}
+
+std::vector<double> graph_key(const gwm_step_config& c, const gwm_warm_rain_config& w,
+                              const gwm_surrogate* M) {
+  return {c.dt, static_cast<double>(c.fast_substeps), c.gravity,
+          static_cast<double>(c.fast_update_horizontal_momentum),
+          static_cast<double>(c.fast_update_density), w.dt, w.condensation_relaxation,
+          w.cloud_autoconversion_threshold, w.cloud_autoconversion_rate,
+          w.rain_evaporation_rate, w.rain_terminal_velocity, w.sedimentation_layer_depth_m,
+          static_cast<double>(w.enable_latent_heating), w.accretion_rate,
+          static_cast<double>(reinterpret_cast<uintptr_t>(M)),
+          M ? static_cast<double>(M->version) : -1.0};
+}
+
+// The captured graph of two steps from the current buffer parity, capturing
+// it on first use.  Before a capture, one eager step runs (it chooses the
+// chunk counts, allocates every scratch buffer and enters the two-step
+// buffer cycle); returns the number of eager steps taken (0 or 1).
+template <class R>
+int ensure_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& w,
+                 gwm_surrogate* M, gwm_state::Graph** out) {
+  const auto key = graph_key(c, w, M);
+  for (auto& gr : S->graphs)
+    if (gr.exec && gr.cur == S->cur && gr.tr0 == (S->tr.empty() ? nullptr : S->tr[0]) &&
+        gr.key == key) {
+      *out = &gr;
+      return 0;
+    }
+  int eager = 0;
+  if (S->graphs[0].exec == nullptr || S->graphs[0].key != key) {
+    moist_step_any<R>(S, c, w, M);
+    eager = 1;
+    for (auto& gr : S->graphs)
+      if (gr.exec) {
+        HIPCHK(hipGraphExecDestroy(gr.exec));
+        gr = gwm_state::Graph{};
+      }
+  }
+  gwm_state::Graph& gr = S->graphs[S->graphs[0].exec ? 1 : 0];
+  if (gr.exec) {
+    HIPCHK(hipGraphExecDestroy(gr.exec));
+    gr = gwm_state::Graph{};
+  }
+  const int cur0 = S->cur;
+  void* tr0 = S->tr.empty() ? nullptr : S->tr[0];
+  const bool timing = S->ctx->timing;
+  S->ctx->timing = false;
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(S->ctx->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    moist_step_any<R>(S, c, w, M);
+    moist_step_any<R>(S, c, w, M);
+  } catch (...) {
+    (void)hipStreamEndCapture(S->ctx->stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    S->ctx->timing = timing;
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(S->ctx->stream, &graph));
+  S->ctx->timing = timing;
+  require(S->cur == cur0 && (S->tr.empty() ? nullptr : S->tr[0]) == tr0,
+          "graph capture needs the two-step buffer cycle");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(e);
+  gr.exec = exec;
+  gr.cur = cur0;
+  gr.tr0 = tr0;
+  gr.key = key;
+  *out = &gr;
+  return eager;
+}
+
+template <class R>
+void moist_steps_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& w,
+                       gwm_surrogate* M, int nsteps) {
+  int done = 0;
+  ensure_synced<R>(S, S->set[S->cur]);
+  while (nsteps - done >= 2) {
+    gwm_state::Graph* gr = nullptr;
+    done += ensure_graph<R>(S, c, w, M, &gr);
+    if (nsteps - done < 2) break;
+    HIPCHK(hipGraphLaunch(gr->exec, S->ctx->stream));
+    done += 2;
+  }
+  while (done < nsteps) {
+    moist_step_any<R>(S, c, w, M);
+    ++done;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gwm_moist_steps_graph(gwm_state* s, const gwm_step_config* c, const gwm_warm_rain_config* wr,
+                          gwm_surrogate* m, int nsteps) {
+  return guard([&] {
+    S_(s);
+    require(c && wr, "null config");
+    require(nsteps >= 0, "nsteps must be >= 0");
+    require(!s->view, "not on a stage view");
+    if (m) require(m->ctx == s->ctx, "surrogate and state belong to different contexts");
+    dispatch(s->ctx, [&](auto r) {
+      moist_steps_graph<decltype(r)>(s, *c, *wr, m, nsteps);
+      return 0;
+    });
+  });
+}
+
+int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* c,
+                       const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
+                       int nsteps) {
+  return guard([&] {
+    require(states && c && wr && n >= 1, "null argument");
+    require(nsteps >= 0, "nsteps must be >= 0");
+    for (int e = 0; e < n; ++e) {
+      S_(states[e]);
+      for (int f = 0; f < e; ++f)
+        require(states[f]->ctx != states[e]->ctx,
+                "ensemble members need a context (stream) each");
+      if (surrogates && surrogates[e])
+        require(surrogates[e]->ctx == states[e]->ctx,
+                "surrogate and state belong to different contexts");
+    }
+    // Members run concurrently, each on its own stream: step pairs are
+    // replayed from each member's captured graph, interleaved across members.
+    std::vector<int> done(static_cast<size_t>(n), 0);
+    for (int e = 0; e < n; ++e)
+      dispatch(states[e]->ctx, [&](auto r) {
+        ensure_synced<decltype(r)>(states[e], states[e]->set[states[e]->cur]);
+        return 0;
+      });
+    bool more = true;
+    while (more) {
+      more = false;
+      for (int e = 0; e < n; ++e) {
+        gwm_state* S = states[e];
+        gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
+        int& d = done[static_cast<size_t>(e)];
+        if (nsteps - d < 2) continue;
+        dispatch(S->ctx, [&](auto r) {
+          using R = decltype(r);
+          gwm_state::Graph* gr = nullptr;
+          d += ensure_graph<R>(S, *c, *wr, M, &gr);
+          if (nsteps - d >= 2) {
+            HIPCHK(hipGraphLaunch(gr->exec, S->ctx->stream));
+            d += 2;
+          }
+          return 0;
+        });
+        if (nsteps - d >= 2) more = true;
+      }
+    }
+    for (int e = 0; e < n; ++e) {
+      gwm_state* S = states[e];
+      gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
+      while (done[static_cast<size_t>(e)] < nsteps) {
+        dispatch(S->ctx, [&](auto r) {
+          moist_step_any<decltype(r)>(S, *c, *wr, M);
+          return 0;
+        });
+        ++done[static_cast<size_t>(e)];
+      }
+    }
+    for (int e = 0; e < n; ++e) HIPCHK(hipStreamSynchronize(states[e]->ctx->stream));
   });
 }
 

@@ -191,6 +191,10 @@ def lib():
         L.gwm_moist_step_surrogate.argtypes = [vp, ctypes.POINTER(StepConfig),
                                                ctypes.POINTER(WarmRainConfig), vp,
                                                ctypes.POINTER(BoundaryConfig)]
+        L.gwm_moist_steps_graph.argtypes = [vp, ctypes.POINTER(StepConfig),
+                                            ctypes.POINTER(WarmRainConfig), vp, ip]
+        L.gwm_ensemble_steps.argtypes = [vp, ip, ctypes.POINTER(StepConfig),
+                                         ctypes.POINTER(WarmRainConfig), vp, ip]
         _lib = L
     return _lib
 
@@ -397,6 +401,12 @@ class State:
         b = None if boundary is None else ctypes.byref(boundary)
         check(lib().gwm_moist_step(self.h, ctypes.byref(cfg), ctypes.byref(wr), b))
 
+    def moist_steps_graph(self, cfg, wr, nsteps, surrogate=None):
+        """nsteps moist steps replayed from a captured two-step HIP graph."""
+        check(lib().gwm_moist_steps_graph(self.h, ctypes.byref(cfg), ctypes.byref(wr),
+                                          None if surrogate is None else surrogate.h,
+                                          int(nsteps)))
+
     def copy_from(self, other):
         """Device-to-device copy of every field of another state of the grid."""
         check(lib().gwm_state_copy(self.h, other.h))
@@ -550,3 +560,14 @@ def surrogate_sample(state, cfg, seed, n, x, y, cell=None, row=0):
     check(lib().gwm_surrogate_sample(state.h, ctypes.byref(cfg), int(seed), int(n),
                                      x.offset(row), y.offset(row),
                                      None if cell is None else cell.offset(row)))
+
+
+def ensemble_steps(states, cfg, wr, nsteps, surrogates=None):
+    """Advance states (one context each) nsteps moist steps concurrently."""
+    n = len(states)
+    arr = (ctypes.c_void_p * n)(*[st.h.value for st in states])
+    sur = None
+    if surrogates is not None:
+        sur = (ctypes.c_void_p * n)(*[None if m is None else m.h.value for m in surrogates])
+    check(lib().gwm_ensemble_steps(arr, n, ctypes.byref(cfg), ctypes.byref(wr), sur,
+                                   int(nsteps)))

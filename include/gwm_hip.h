@@ -345,6 +345,23 @@ int gwm_moist_step_surrogate(gwm_state* state, const gwm_step_config* cfg,
                              const gwm_warm_rain_config* wr, gwm_surrogate* m,
                              const gwm_boundary_config* boundary);
 
+/* ---- graph-replayed steps and ensembles -----------------------------------
+ * No counterpart in the reference (its stepper launches every operator from
+ * the host): the moist step (gwm_moist_step without boundaries, or
+ * gwm_moist_step_surrogate when `surrogate` is not NULL) is captured once
+ * into a HIP graph of two steps -- the buffer rotation has period two --
+ * after one eager step, and replayed.  Results equal the eager steps'.  A
+ * change of configuration or of the surrogate's normalisation re-captures. */
+int gwm_moist_steps_graph(gwm_state* state, const gwm_step_config* cfg,
+                          const gwm_warm_rain_config* wr, gwm_surrogate* surrogate, int nsteps);
+/* Ensemble of states of any sizes, each on its own context (stream):
+ * every member advances nsteps moist steps; members run concurrently,
+ * their graph replays interleaved; returns after all members finished.
+ * surrogates: NULL, or one per member (entries may be NULL). */
+int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* cfg,
+                       const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
+                       int nsteps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -498,3 +498,57 @@ def test_level_chunks_do_not_change_results(ctx32, ctx64, prec):
     report = ctx.tuning_report()
     assert any(k.startswith("fast_column:130x16x12") for k in report), report
     assert all(v in (1, 2, 4, 8) for v in report.values())
+
+
+def _moist_pair(prec, ctx, seed=21, shape=(40, 24, 12)):
+    nx, ny, nz = shape
+    zi = np.linspace(0.0, 12000.0, nz + 1)
+    grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    out = []
+    for _ in range(2):
+        gs = G.State(grid, 3)
+        gs.init_convection(1.0e5, seed, 0.5, 1000.0)
+        qv = gs.get(O.TRACER0)
+        gs.set(O.TRACER0 + 1, (0.03 * qv).astype(qv.dtype))
+        gs.set(O.TRACER0 + 2, (0.02 * qv).astype(qv.dtype))
+        out.append(gs)
+    return grid, out
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_graph_replayed_steps_equal_eager_steps(ctx32, ctx64, prec):
+    ctx = ctx_for(prec, ctx32, ctx64)
+    grid, (a, b) = _moist_pair(prec, ctx)
+    cfg = G.step_config(dt=2.0, fast_substeps=3)
+    wr = G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0)
+    for _ in range(7):
+        a.moist_step(cfg, wr)
+    b.moist_steps_graph(cfg, wr, 3)   # eager + one graph replay
+    b.moist_steps_graph(cfg, wr, 4)   # other parity: capture + replays
+    for f in FIELDS + (O.TRACER0, O.TRACER0 + 1, O.TRACER0 + 2, O.PRECIP):
+        assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_ensemble_steps_equal_member_steps():
+    """Members of different sizes, one context each, advanced concurrently
+    from their graphs, equal the same members stepped one by one."""
+    shapes = [(40, 24, 12), (64, 32, 10), (24, 40, 14)]
+    cfg = G.step_config(dt=2.0, fast_substeps=4)
+    wr = G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0)
+    ctxs = [G.Context(0, G.F32) for _ in shapes]
+    members, refs, keep = [], [], []
+    for i, (ctx, shape) in enumerate(zip(ctxs, shapes)):
+        grid, (a, b) = _moist_pair(4, ctx, seed=30 + i, shape=shape)
+        keep.append(grid)
+        members.append(a)
+        refs.append(b)
+    G.ensemble_steps(members, cfg, wr, 5)
+    for r in refs:
+        for _ in range(5):
+            r.moist_step(cfg, wr)
+    for m, r in zip(members, refs):
+        for f in FIELDS + (O.TRACER0, O.TRACER0 + 2, O.PRECIP):
+            assert np.array_equal(m.get(f), r.get(f)), f
+    del members, refs, keep
+    for c in ctxs:
+        c.close()
