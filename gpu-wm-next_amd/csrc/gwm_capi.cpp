@@ -299,6 +299,21 @@ struct gwm_state {
   void* rref3 = nullptr;      // nx*ny*nz
   void* pref3 = nullptr;
   void* theta_scr = nullptr;  // cell field
+  // Perturbation form (flat, doubly periodic, FP32): rho and rho*theta are
+  // stored as departures from per-level bases, the FP64 level means of the
+  // state they were built from; the device holds the bases in working
+  // precision (rho_b, (rho theta)_b, theta_b, p_b).
+  bool pert = false;
+  std::vector<double> base_rho64, base_rt64;
+  void* b_dev = nullptr;  // 4 * nz values: rho_b, rt_b, theta_b, p_b
+  template <class R>
+  gwm::Base<R> base() const {
+    if (!pert) return gwm::Base<R>{nullptr, nullptr, nullptr, nullptr};
+    const gwm_state* o = view ? owner : this;
+    const R* b = static_cast<const R*>(o->b_dev);
+    const int nz = grid->geo.nz;
+    return gwm::Base<R>{b, b + nz, b + 2 * nz, b + 3 * nz};
+  }
   // Captured two-step moist-step graphs, one per buffer parity (the step
   // rotates the buffer sets and tracer buffers with period two).
   struct Graph {
@@ -373,7 +388,7 @@ struct gwm_state {
     for (auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr})
       for (void* p : *v)
         if (p) (void)hipFree(p);
-    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, rref3, pref3, theta_scr})
+    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, rref3, pref3, theta_scr, b_dev})
       if (p) (void)hipFree(p);
   }
 };
@@ -416,7 +431,7 @@ void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
   int nch = 0;
   {
     Timed t(S->ctx, "level_partials");
-    gwm::launch_level_partials<R>(S->ctx->stream, g, rho, rt, 0, part, &nch);
+    gwm::launch_level_partials<R>(S->ctx->stream, g, rho, rt, 0, part, &nch, S->base<R>());
   }
   {
     Timed t(S->ctx, "level_finalize");
@@ -498,7 +513,8 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     if (!colref) {
       if (s == 0 || S->ctx->fast_point) {
         Timed t(S->ctx, "level_partials");
-        gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nparts);
+        gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nparts,
+                                      S->base<R>());
       }
       Timed t(S->ctx, "level_finalize");
       gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nparts, part, 1,
@@ -535,6 +551,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     a.horizontal = c.fast_update_horizontal_momentum;
     a.density = c.fast_update_density;
     a.col_ref = colref;
+    a.base = S->base<R>();
     if (S->ctx->fast_point) {
       Timed t(S->ctx, "fast_substep");
       gwm::launch_fast_substep<R>(S->ctx->stream, a, G->terrain_nonzero);
@@ -563,6 +580,7 @@ template <class R>
 void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* const* q0,
                 void* const* out, int mode, R wa, R wb, int out_theta) {
   const Geo& g = S->g();
+  require(!(S->pert && S->ctx->fast_point), "the perturbation form runs on the column kernels");
   level_theta_ref<R>(S, static_cast<const R*>(in[0]), static_cast<const R*>(in[1]));
   gwm::SlowArgs<R> a{};
   a.g = g;
@@ -577,6 +595,7 @@ void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* c
   a.wb = wb;
   a.mode = mode;
   a.out_theta = out_theta;
+  a.base = S->base<R>();
   if (S->ctx->fast_point) {
     Timed t(S->ctx, "slow_stage");
     gwm::launch_slow_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
@@ -677,7 +696,8 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
         // ys[1] holds theta; rebuild rho*theta = rho*theta (K10) for the view.
         Timed t(S->ctx, "theta");
         gwm::launch_rebuild_rt<R>(S->ctx->stream, g, static_cast<const R*>(ys[0]),
-                                  static_cast<const R*>(ys[1]), static_cast<R*>(zs[1]));
+                                  static_cast<const R*>(ys[1]), static_cast<R*>(zs[1]),
+                                  S->base<R>());
       }
       gwm_state view;
       view.ctx = S->ctx;
@@ -724,7 +744,8 @@ void fast_modes(gwm_state* S, const gwm_step_config& c) {
   {
     Timed t(S->ctx, "theta");
     gwm::launch_theta<R>(S->ctx->stream, g, static_cast<const R*>(S->set[X][0]),
-                         static_cast<const R*>(S->set[X][1]), static_cast<R*>(S->theta_scr));
+                         static_cast<const R*>(S->set[X][1]), static_cast<R*>(S->theta_scr),
+                         S->base<R>());
   }
   void** xs = S->set[X];
   const int r = subcycle<R>(S, c, static_cast<const R*>(xs[0]), static_cast<const R*>(xs[2]),
@@ -794,6 +815,7 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
     a.mode = mode;
     a.wa = wa;
     a.wb = wb;
+    a.base_rho = S->base<R>().rho;
     if (S->ctx->fast_point) {
       Timed tt(S->ctx, "tracer_stage");
       gwm::launch_tracer_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
@@ -841,6 +863,7 @@ void warm_rain(gwm_state* S, const gwm_warm_rain_config& c, int metric) {
   a.heat = c.enable_latent_heating;
   if (metric) a.fallout = a.vt > R(0) ? 2 : 0;
   else a.fallout = (a.vt > R(0) && a.depth > R(0)) ? 1 : 0;
+  a.base = S->base<R>();
   Timed t(S->ctx, "warm_rain");
   gwm::launch_warm_rain<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
 }
@@ -848,12 +871,14 @@ void warm_rain(gwm_state* S, const gwm_warm_rain_config& c, int metric) {
 // Compensated reduction of one field; returns {sum, min, max}.
 template <class R>
 void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, const R* rho,
-                  double out[3]) {
+                  double out[3], const R* fbase = nullptr, const R* rbase = nullptr) {
   const int nb = 512;
   double* part = S->ctx->scratch(static_cast<size_t>(nb) * 4);
   gwm::ReduceArgs<R> a{};
   a.f = f;
   a.rho = rho;
+  a.fbase = fbase;
+  a.rbase = rbase;
   a.idz3 = S->grid->terrain_nonzero ? static_cast<const R*>(S->grid->idz_c) : nullptr;
   a.idz_lev = static_cast<const R*>(S->grid->lev_idz_c);
   a.fx = fx;
@@ -892,14 +917,23 @@ void budget(gwm_state* S, double* o) {
   for (int a = 0; a < GWM_SUMMARY_N; ++a) o[a] = 0.0;
   void** ds = S->cur_set();
   const R* rho = static_cast<const R*>(ds[0]);
+  const gwm::Base<R> B = S->base<R>();
   double r[3];
-  reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r);
-  o[GWM_S_TOTAL_DRY_MASS] = r[0];
+  // Perturbation form: sums of the departures plus the FP64 base sums, so
+  // the mass budget keeps the departures' precision.
+  double base_rho = 0.0, base_rt = 0.0;
+  if (S->pert)
+    for (int k = 0; k < g.nz; ++k) {
+      base_rho += static_cast<double>(g.nx) * g.ny * S->base_rho64[static_cast<size_t>(k)];
+      base_rt += static_cast<double>(g.nx) * g.ny * S->base_rt64[static_cast<size_t>(k)];
+    }
+  reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r, B.rho);
+  o[GWM_S_TOTAL_DRY_MASS] = r[0] + base_rho;
   o[GWM_S_RHO_MIN] = r[1];
   o[GWM_S_RHO_MAX] = r[2];
-  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 0, nullptr, r);
-  o[GWM_S_TOTAL_RHO_THETA] = r[0];
-  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 2, rho, r);
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 0, nullptr, r, B.rt);
+  o[GWM_S_TOTAL_RHO_THETA] = r[0] + base_rt;
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 2, rho, r, B.rt, B.rho);
   o[GWM_S_THETA_MIN] = r[1];
   o[GWM_S_THETA_MAX] = r[2];
   reduce_field<R>(S, static_cast<const R*>(ds[2]), g.nx + 1, g.ny, g.nz, 0, nullptr, r);
@@ -918,7 +952,7 @@ void budget(gwm_state* S, double* o) {
     const R* q = static_cast<const R*>(S->field(GWM_TRACER0 + t));
     reduce_field<R>(S, q, g.nx, g.ny, g.nz, 1, nullptr, r);
     o[GWM_S_VAPOR_PATH + t] = r[0];
-    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 2, rho, r);
+    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 2, rho, r, nullptr, B.rho);
     o[GWM_S_TRACER_QMIN0 + 2 * t] = r[1];
     o[GWM_S_TRACER_QMIN0 + 2 * t + 1] = r[2];
   }
@@ -985,6 +1019,80 @@ R h_hydro_rho(R rho_s, R th, R z) {
   return h_rho_from_p_theta(p, th);
 }
 
+// ---- perturbation form: bases and conversions (host, FP64) ----------------
+double eos64(double rho, double rt) {  // dry_thermo.hpp:26-33 in FP64
+  const double Rd = 287.05, p0 = 1.0e5, kap = 0.2854;
+  const double th = rt / std::max(rho, 1.0e-6);
+  return p0 * std::pow(std::max(rho * Rd * th / p0, 1.0e-6), 1.0 / (1.0 - kap));
+}
+
+// FP64 level means of the owned cells of a full cell field (storage layout).
+template <class T>
+std::vector<double> level_means(const gwm_state* S, const T* full) {
+  const Geo& g = S->g();
+  std::vector<double> m(static_cast<size_t>(g.nz), 0.0);
+  for (int k = 0; k < g.nz; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) s += static_cast<double>(full[g.cell(i, j, k)]);
+    m[static_cast<size_t>(k)] = s / (static_cast<double>(g.nx) * g.ny);
+  }
+  return m;
+}
+
+void push_base(gwm_state* S) {
+  const int nz = S->g().nz;
+  std::vector<float> b(4 * static_cast<size_t>(nz));
+  for (int k = 0; k < nz; ++k) {
+    const double r = S->base_rho64[static_cast<size_t>(k)];
+    const double t = S->base_rt64[static_cast<size_t>(k)];
+    b[k] = static_cast<float>(r);
+    b[nz + k] = static_cast<float>(t);
+    b[2 * nz + k] = static_cast<float>(t / r);
+    b[3 * nz + k] = static_cast<float>(eos64(r, t));
+  }
+  if (!S->b_dev) S->b_dev = dalloc(b.size() * sizeof(float));
+  HIPCHK(hipMemcpy(S->b_dev, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// Full cell field -> departures from `base` (a level mean of it), FP32, device.
+template <class T>
+void upload_departure(gwm_state* S, void* dst, const T* full, const std::vector<double>& base) {
+  const Geo& g = S->g();
+  const size_t n = g.cell_total();
+  const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+  std::vector<float> d(n);
+  for (size_t a = 0; a < n; ++a)
+    d[a] = static_cast<float>(static_cast<double>(full[a]) - base[a / plane]);
+  HIPCHK(hipMemcpy(dst, d.data(), n * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// Whole state in the perturbation form: bases from the FP64 level means of
+// rho and rho*theta, departures and the other fields in FP32.
+template <class T>
+void upload_pert_fields(gwm_state* S, const T* rho, const T* rt, const T* u, const T* v,
+                        const T* w, const std::vector<std::vector<T>>& q) {
+  S->base_rho64 = level_means(S, rho);
+  S->base_rt64 = level_means(S, rt);
+  push_base(S);
+  upload_departure(S, S->cur_set()[0], rho, S->base_rho64);
+  upload_departure(S, S->cur_set()[1], rt, S->base_rt64);
+  const T* f[3] = {u, v, w};
+  for (int a = 2; a < 5; ++a) {
+    const size_t n = S->fsize(a);
+    std::vector<float> h(n);
+    for (size_t e = 0; e < n; ++e) h[e] = static_cast<float>(f[a - 2][e]);
+    HIPCHK(hipMemcpy(S->cur_set()[a], h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  for (size_t t = 0; t < q.size() && t < static_cast<size_t>(S->ntr); ++t) {
+    std::vector<float> h(q[t].size());
+    for (size_t e = 0; e < h.size(); ++e) h[e] = static_cast<float>(q[t][e]);
+    HIPCHK(hipMemcpy(S->field(GWM_TRACER0 + static_cast<int>(t)), h.data(),
+                     h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  S->faces_synced = false;
+}
+
 template <class R>
 struct HostState {
   std::vector<R> rho, rt, u, v, w;
@@ -999,6 +1107,10 @@ struct HostState {
     q.assign(S->ntr, std::vector<R>(g.cell_total(), R(0)));
   }
   void upload(gwm_state* S) {
+    if (S->pert) {
+      upload_pert_fields(S, rho.data(), rt.data(), u.data(), v.data(), w.data(), q);
+      return;
+    }
     std::vector<R>* f[5] = {&rho, &rt, &u, &v, &w};
     for (int a = 0; a < 5; ++a)
       HIPCHK(hipMemcpy(S->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R), hipMemcpyHostToDevice));
@@ -1027,7 +1139,13 @@ std::vector<R> host_metric(const gwm_state* S, int which) {
   std::vector<R> out(n);
   const void* src = which == 0 ? S->grid->zif_m : which == 1 ? S->grid->zc_m
                   : which == 2 ? S->grid->idz_c : S->grid->idz_f;
-  HIPCHK(hipMemcpy(out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost));
+  if (S->ctx->esize() == sizeof(R)) {
+    HIPCHK(hipMemcpy(out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost));
+  } else {  // FP64 host build of a perturbation-form (FP32) state
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t a = 0; a < n; ++a) out[a] = static_cast<R>(tmp[a]);
+  }
   return out;
 }
 
@@ -1127,7 +1245,13 @@ void init_mountain(gwm_state* S, double rho_s, double th, double u, double v, do
   fill_constant<R>(S, H, R(0), static_cast<R>(th), static_cast<R>(u), static_cast<R>(v),
                    static_cast<R>(w));
   std::vector<R> terr(static_cast<size_t>(g.nx) * g.ny);
-  HIPCHK(hipMemcpy(terr.data(), S->grid->terr, terr.size() * sizeof(R), hipMemcpyDeviceToHost));
+  if (S->ctx->esize() == sizeof(R)) {
+    HIPCHK(hipMemcpy(terr.data(), S->grid->terr, terr.size() * sizeof(R), hipMemcpyDeviceToHost));
+  } else {  // FP64 host build of a perturbation-form (FP32) state
+    std::vector<float> tmp(terr.size());
+    HIPCHK(hipMemcpy(tmp.data(), S->grid->terr, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t a = 0; a < tmp.size(); ++a) terr[a] = static_cast<R>(tmp[a]);
+  }
   const auto& zc = S->grid->z_c;
   const auto& wc = S->grid->w_c;
   for (int k = 0; k < g.nz; ++k)
@@ -1242,6 +1366,15 @@ auto dispatch(const gwm_ctx* c, F f) {
   if (c->prec == GWM_F64) return f(double{});
   return f(float{});
 }
+
+// Dispatch of the host-built initial states: FP64 host arithmetic for the
+// perturbation form, the context precision otherwise.
+template <class F>
+auto init_dispatch(gwm_state* S, F f) {
+  if (S->pert) return f(double{});
+  return dispatch(S->ctx, f);
+}
+
 
 }  // namespace
 
@@ -1532,6 +1665,34 @@ int gwm_state_create(gwm_ctx* ctx, const gwm_grid* grid, int ntr, gwm_state** ou
   });
 }
 
+int gwm_state_create_perturbation(gwm_ctx* ctx, const gwm_grid* grid, int ntr,
+                                  gwm_state** out) {
+  return guard([&] {
+    require(ctx && grid && out, "null argument");
+    require(ctx->prec == GWM_F32, "the perturbation form is an FP32 state form");
+    require(!grid->terrain_nonzero, "the perturbation form needs flat terrain");
+    require(grid->geo.per_x && grid->geo.per_y,
+            "the perturbation form needs a doubly periodic domain");
+    gwm_state* s = nullptr;
+    const int rc = gwm_state_create(ctx, grid, ntr, &s);
+    if (rc != GWM_OK) throw Error(rc, gwm_last_error());
+    s->pert = true;
+    // Until a state is loaded, the departures are zero against a nominal
+    // base (rho_b = 1, (rho theta)_b = 300).
+    s->base_rho64.assign(static_cast<size_t>(grid->geo.nz), 1.0);
+    s->base_rt64.assign(static_cast<size_t>(grid->geo.nz), 300.0);
+    try {
+      push_base(s);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int gwm_state_is_perturbation(const gwm_state* s) { return s && s->pert ? 1 : 0; }
+
 int gwm_state_destroy(gwm_state* s) {
   return guard([&] {
     if (s) (void)hipStreamSynchronize(s->ctx->stream);
@@ -1551,6 +1712,15 @@ int gwm_state_upload(gwm_state* s, int f, const void* host) {
     require(host, "null host pointer");
     void* d = s->field(f);
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    if (s->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
+      // full values in; the base of the field becomes their level means
+      gwm_state* o = s->view ? s->owner : s;
+      const float* h = static_cast<const float*>(host);
+      if (!s->view) (f == GWM_RHO ? o->base_rho64 : o->base_rt64) = level_means(s, h);
+      push_base(o);
+      upload_departure(s, d, h, f == GWM_RHO ? o->base_rho64 : o->base_rt64);
+      return;
+    }
     HIPCHK(hipMemcpy(d, host, s->fsize(f) * s->es(), hipMemcpyHostToDevice));
     if (f == GWM_MOM_U || f == GWM_MOM_V) s->faces_synced = false;
   });
@@ -1563,6 +1733,39 @@ int gwm_state_download(gwm_state* s, int f, void* host) {
     void* d = s->field(f);
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     HIPCHK(hipMemcpy(host, d, s->fsize(f) * s->es(), hipMemcpyDeviceToHost));
+    if (s->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
+      // departures -> full values (base added in FP64, rounded once)
+      const gwm_state* o = s->view ? s->owner : s;
+      const auto& b = f == GWM_RHO ? o->base_rho64 : o->base_rt64;
+      const Geo& g = s->g();
+      const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+      float* h = static_cast<float*>(host);
+      for (size_t a = 0; a < s->fsize(f); ++a)
+        h[a] = static_cast<float>(static_cast<double>(h[a]) + b[a / plane]);
+    }
+  });
+}
+
+int gwm_state_download_f64(gwm_state* s, int f, double* host) {
+  return guard([&] {
+    S_(s);
+    require(host, "null host pointer");
+    void* d = s->field(f);
+    const size_t n = s->fsize(f);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    if (s->es() == 8) {
+      HIPCHK(hipMemcpy(host, d, n * sizeof(double), hipMemcpyDeviceToHost));
+      return;
+    }
+    std::vector<float> h(n);
+    HIPCHK(hipMemcpy(h.data(), d, n * sizeof(float), hipMemcpyDeviceToHost));
+    const bool dep = s->pert && (f == GWM_RHO || f == GWM_RHO_THETA);
+    const gwm_state* o = s->view ? s->owner : s;
+    const Geo& g = s->g();
+    const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+    for (size_t a = 0; a < n; ++a)
+      host[a] = static_cast<double>(h[a]) +
+                (dep ? (f == GWM_RHO ? o->base_rho64 : o->base_rt64)[a / plane] : 0.0);
   });
 }
 
@@ -1570,6 +1773,12 @@ int gwm_state_fill(gwm_state* s, int f, double v) {
   return guard([&] {
     S_(s);
     void* d = s->field(f);
+    if (s->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
+      const std::vector<float> h(s->fsize(f), static_cast<float>(v));
+      const int rc = gwm_state_upload(s, f, h.data());
+      if (rc != GWM_OK) throw Error(rc, gwm_last_error());
+      return;
+    }
     dispatch(s->ctx, [&](auto r) {
       using R = decltype(r);
       Timed t(s->ctx, "fill");
@@ -1583,7 +1792,13 @@ int gwm_state_fill(gwm_state* s, int f, double v) {
 int gwm_state_copy(gwm_state* dst, const gwm_state* src) {
   return guard([&] {
     require(dst && src && dst->grid == src->grid && dst->ntr == src->ntr, "state mismatch");
+    require(dst->pert == src->pert, "state form mismatch (full / perturbation)");
     gwm_state* s = const_cast<gwm_state*>(src);
+    if (src->pert) {
+      dst->base_rho64 = src->base_rho64;
+      dst->base_rt64 = src->base_rt64;
+      push_base(dst);
+    }
     for (int f = 0; f < 5; ++f)
       HIPCHK(hipMemcpyAsync(dst->field(f), s->field(f), s->fsize(f) * s->es(),
                             hipMemcpyDeviceToDevice, dst->ctx->stream));
@@ -1723,6 +1938,7 @@ int gwm_memcpy_d2h(gwm_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int gwm_pressure(gwm_state* s) {
   return guard([&] {
     S_(s);
+    require(!s->pert, "gwm_pressure: full-form states only (use gwm_state_download)");
     void* p = s->field(GWM_PRESSURE);
     void** fs = s->cur_set();
     dispatch(s->ctx, [&](auto r) {
@@ -1865,11 +2081,16 @@ int gwm_budget_sums(gwm_state* s, double* out) {
 int gwm_init_constant(gwm_state* s, double rho, double th, double u, double v, double w) {
   return guard([&] {
     S_(s);
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       using R = decltype(r);
       HostState<R> H(s);
       fill_constant<R>(s, H, static_cast<R>(rho), static_cast<R>(th), static_cast<R>(u),
                        static_cast<R>(v), static_cast<R>(w));
+      if (s->pert) {  // tracers are left as they are
+        upload_pert_fields(s, H.rho.data(), H.rt.data(), H.u.data(), H.v.data(), H.w.data(),
+                           std::vector<std::vector<R>>{});
+        return 0;
+      }
       H.q.clear();
       std::vector<R>* f[5] = {&H.rho, &H.rt, &H.u, &H.v, &H.w};
       for (int a = 0; a < 5; ++a)
@@ -1883,7 +2104,7 @@ int gwm_init_constant(gwm_state* s, double rho, double th, double u, double v, d
 int gwm_init_hydrostatic_rest(gwm_state* s, double rho_s, double th) {
   return guard([&] {
     S_(s);
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       using R = decltype(r);
       HostState<R> H(s);
       init_hydro<R>(s, rho_s, th, H);
@@ -1897,7 +2118,7 @@ int gwm_init_bubble(gwm_state* s, const double* p, int sign) {
   return guard([&] {
     S_(s);
     require(p, "null parameters");
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       init_bubble<decltype(r)>(s, p, sign);
       return 0;
     });
@@ -1907,7 +2128,7 @@ int gwm_init_bubble(gwm_state* s, const double* p, int sign) {
 int gwm_init_mountain_wave(gwm_state* s, double rho_s, double th, double u, double v, double w) {
   return guard([&] {
     S_(s);
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       init_mountain<decltype(r)>(s, rho_s, th, u, v, w);
       return 0;
     });
@@ -1918,7 +2139,7 @@ int gwm_init_constant_n(gwm_state* s, double th_s, double N, double p_s, double 
   return guard([&] {
     S_(s);
     require(N > 0 && th_s > 0 && p_s > 0, "bad sounding parameters");
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       init_constant_n<decltype(r)>(s, th_s, N, p_s, u, v);
       return 0;
     });
@@ -1928,7 +2149,7 @@ int gwm_init_constant_n(gwm_state* s, double th_s, double N, double p_s, double 
 int gwm_init_convection(gwm_state* s, double p_s, unsigned long long seed, double amp, double ztop) {
   return guard([&] {
     S_(s);
-    dispatch(s->ctx, [&](auto r) {
+    init_dispatch(s, [&](auto r) {
       init_convection<decltype(r)>(s, p_s, seed, amp, ztop);
       return 0;
     });
@@ -2046,7 +2267,7 @@ void warm_rain_surrogate(gwm_state* S, const gwm_warm_rain_config& c, gwm_surrog
   {
     Timed t(S->ctx, "warm_rain_surrogate");
     gwm::launch_warm_rain_sur(S->ctx->stream, g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params,
-                              M->norm, M->nfall);
+                              M->norm, M->nfall, S->base<float>());
   }
   const float vt = static_cast<float>(c.rain_terminal_velocity);
   const float depth = static_cast<float>(c.sedimentation_layer_depth_m);
@@ -2175,7 +2396,7 @@ int gwm_surrogate_sample(gwm_state* s, const gwm_warm_rain_config* c, unsigned l
                            static_cast<const float*>(s->field(GWM_TRACER0 + 1)),
                            static_cast<const float*>(s->field(GWM_TRACER0 + 2)), micro_cfg(*c),
                            seed, n, static_cast<float*>(dev_x), static_cast<float*>(dev_y),
-                           static_cast<long long*>(dev_cell));
+                           static_cast<long long*>(dev_cell), s->base<float>());
     HIPCHK(hipGetLastError());
   });
 }

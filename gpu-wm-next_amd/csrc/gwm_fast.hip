@@ -90,9 +90,17 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       rt_h = L.cell(a.theta, ti0 + hx, tj0 + hy, k);
     }
   };
+  // Perturbation form: rho' and theta' in, p' out (K10 rho*theta = rho*theta_fixed
+  // written for the departures: rho_b theta' + rho' (theta_b + theta')).
+  const bool pert = a.base.rho != nullptr;
+  auto press = [&](R r, R th, int k) -> R {
+    if (!pert) return d_pressure(r, r * th);
+    const R rb = a.base.rho[k], tb = a.base.th[k];
+    return p_pert(rb * th + r * (tb + th), a.base.rt[k], a.base.p[k]);
+  };
   auto stage_plane = [&](int k) {
-    sp[k & 3][ci] = d_pressure(rr_c, rr_c * rt_c);
-    if (t < NHALO) sp[k & 3][hci] = d_pressure(rr_h, rr_h * rt_h);
+    sp[k & 3][ci] = press(rr_c, rt_c, k);
+    if (t < NHALO) sp[k & 3][hci] = press(rr_h, rt_h, k);
   };
 
   // Face momenta of one level for the own cell (prefetched a level ahead).
@@ -254,12 +262,15 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
         const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : a.m.lev_idz_c[k];
         const R div = ((u1 - ub1) - (u0 - ub0)) / dx + ((v1 - vb1) - (v0 - vb0)) / dy +
                       ((w1 - wb1) - (w0 - wb0)) * idz;
-        rho_new = rmax(R(1.0e-6), rho_new - dtf * div);
+        const R floor = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
+        rho_new = rmax(floor, rho_new - dtf * div);
       }
       const size_t c0 = g.cell(i, j, k);
       const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k);
       a.rho_out[c0] = rho_new;
-      if (a.rt_out) a.rt_out[c0] = rho_new * th_k;
+      if (a.rt_out)
+        a.rt_out[c0] = pert ? a.base.rho[k] * th_k + rho_new * (a.base.th[k] + th_k)
+                            : rho_new * th_k;
       a.u_out[xf0] = u0;
       a.v_out[yf0] = v0;
       a.w_out[c0] = w0;
@@ -274,7 +285,7 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       double sr = 0.0, spp = 0.0;
       if (active) {
         sr = static_cast<double>(rho_new);
-        spp = static_cast<double>(d_pressure(rho_new, rho_new * th_k));
+        spp = static_cast<double>(press(rho_new, th_k, k));
       }
       sr = wave_sum(sr);
       spp = wave_sum(spp);

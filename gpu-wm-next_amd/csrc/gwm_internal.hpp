@@ -75,6 +75,10 @@ __device__ inline float rpow(float a, float b) { return powf(a, b); }
 __device__ inline double rpow(double a, double b) { return pow(a, b); }
 __device__ inline float rexp(float a) { return expf(a); }
 __device__ inline double rexp(double a) { return exp(a); }
+__device__ inline float rexpm1(float a) { return expm1f(a); }
+__device__ inline double rexpm1(double a) { return expm1(a); }
+__device__ inline float rlog1p(float a) { return log1pf(a); }
+__device__ inline double rlog1p(double a) { return log1p(a); }
 
 // Physical constants in the working precision (dry_thermo.hpp:17-19,
 // moist_thermo.hpp:18-21).  FP32 uses the reference's float literals.
@@ -118,6 +122,35 @@ __device__ inline R d_qsat(R T, R p) {  // moist_thermo.hpp:23-36
   const R es = R(6.112) * rexp((R(17.67) * tc) / rmax(tc + R(243.5), R(1.0e-6))) * R(100);
   const R denom = rmax(p - (R(1) - Phys<R>::eps) * es, R(1000));
   return rmin(rmax(Phys<R>::eps * es / denom, R(0)), R(0.05));
+}
+
+// ---------------------------------------------------------------------------
+// Perturbation form (flat, doubly periodic grids).  The state holds
+// rho' = rho - rho_b(k) and (rho theta)' = rho theta - (rho theta)_b(k)
+// against a per-level base (the FP64 level means of the initial state,
+// rounded to working precision here); rho == nullptr means the full form.
+//   theta' = theta - theta_b = ((rho theta)' - theta_b rho') / rho,
+//   p'     = p - p_b = p_b expm1(gamma log1p((rho theta)' / (rho theta)_b)),
+// gamma = 1 / (1 - kappa), which is the EOS of dry_thermo.hpp:26-33 written
+// for the departure, so that pressure differences keep their relative
+// precision.
+// ---------------------------------------------------------------------------
+template <class R>
+struct Base {
+  const R* rho;  // rho_b (nz)
+  const R* rt;   // (rho theta)_b
+  const R* th;   // theta_b = (rho theta)_b / rho_b
+  const R* p;    // p_b = EOS(rho_b, (rho theta)_b)
+};
+
+template <class R>
+__device__ inline R p_pert(R rtp, R rtb, R pb) {
+  const R gamma = R(1) / (R(1) - Phys<R>::kappa);
+  return pb * rexpm1(gamma * rlog1p(rtp / rtb));
+}
+template <class R>
+__device__ inline R theta_pert(R rhop, R rtp, R rhob, R thb) {
+  return (rtp - thb * rhop) / rmax(rhob + rhop, R(1.0e-6));
 }
 
 // ---------------------------------------------------------------------------

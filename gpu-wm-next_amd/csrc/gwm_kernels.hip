@@ -63,7 +63,7 @@ __device__ inline double block_sum(double v, double* sh) {
 template <class R>
 __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, const R* aux,
                                                          int mode, int rows_per_chunk,
-                                                         double* partials) {
+                                                         double* partials, Base<R> base) {
   __shared__ double sh[256];
   const int k = blockIdx.y;
   const int c = blockIdx.x;
@@ -72,11 +72,24 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
   const int j1 = min(g.ny, j0 + rows_per_chunk);
   const int t = threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
+  const bool pert = base.rho != nullptr;
+  const R rb = pert ? base.rho[k] : R(0), thb = pert ? base.th[k] : R(0);
+  const R rtb = pert ? base.rt[k] : R(0), pb = pert ? base.p[k] : R(0);
   for (int j = j0; j < j1; ++j)
     for (int i = t; i < g.nx; i += 256) {
       const size_t c0 = g.cell(i, j, k);
       const R r = rho[c0];
-      if (mode == 0) {
+      if (pert) {
+        // departures: theta' of (rho', (rho theta)'), or rho' and p' of
+        // (rho', theta' fixed)
+        if (mode == 0) {
+          s0 += static_cast<double>(theta_pert(r, aux[c0], rb, thb));
+        } else {
+          const R tp = aux[c0];
+          s0 += static_cast<double>(r);
+          s1 += static_cast<double>(p_pert(rb * tp + r * (thb + tp), rtb, pb));
+        }
+      } else if (mode == 0) {
         s0 += static_cast<double>(aux[c0] / rmax(r, R(1.0e-6)));
       } else {
         s0 += static_cast<double>(r);
@@ -476,24 +489,23 @@ __global__ __launch_bounds__(256) void k_fast_substep(FastArgs<R> a) {
 
 // theta = rho_theta / max(rho, 1e-6) on owned cells (K9, dry_fast_modes.cu:86-95)
 template <class R>
-__global__ void k_theta(Geo g, const R* rho, const R* rt, R* th) {
+__global__ void k_theta(Geo g, const R* rho, const R* rt, R* th, Base<R> base) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny) return;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const size_t c = g.cell(i, j, k);
-  th[c] = d_theta(rho[c], rt[c]);
+  th[c] = base.rho ? theta_pert(rho[c], rt[c], base.rho[k], base.th[k]) : d_theta(rho[c], rt[c]);
 }
 
-// rho_theta = rho * theta over owned cells (K10, dry_fast_modes.cu:97-106)
 template <class R>
-__global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt) {
+__global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt, Base<R> base) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
   const size_t c = g.cell(i, j, k);
-  rt[c] = rho[c] * th[c];
+  rt[c] = base.rho ? base.rho[k] * th[c] + rho[c] * (base.th[k] + th[c]) : rho[c] * th[c];
 }
 
 // ===========================================================================
@@ -563,9 +575,13 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
   const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
   size_t c = g.cell(i, j, g.nz - 1);
   R n_rho = a.rho[c], n_rt = a.rt[c], n_qv = a.qv[c], n_qc = a.qc[c], n_qr = a.qr[c];
+  const bool pert = a.base.rho != nullptr;
   for (int k = g.nz - 1; k >= 0; --k) {
-    const R rho = rmax(n_rho, R(1.0e-6));
-    const R rt0 = n_rt;
+    // Perturbation form: full rho and rho*theta for the microphysics; the
+    // stored departure of rho*theta then changes by rho * dtheta only.
+    const R rtp = n_rt;
+    const R rho = rmax(pert ? n_rho + a.base.rho[k] : n_rho, R(1.0e-6));
+    const R rt0 = pert ? n_rt + a.base.rt[k] : n_rt;
     const R in_qv = n_qv, in_qc = n_qc, in_qr = n_qr;
     if (k > 0) {
       const size_t cn = c - plane;
@@ -576,12 +592,13 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
       n_qr = a.qr[cn];
     }
     MicroCell<R> m = micro_prepare(rho, rt0, in_qv, in_qc, in_qr);
+    const R th0 = m.th;
     micro_apply(m, mc);
     const R th = m.th, qv = m.qv, qc = m.qc, qr = m.qr;
     a.qv[c] = rho * qv;
     a.qc[c] = rho * qc;
     R rqr = rho * qr;
-    a.rt[c] = rho * th;
+    a.rt[c] = pert ? rtp + rho * (th - th0) : rho * th;
     if (nominal) {
       const R old = rmax(rqr, R(0)) * dz_nom;
       const R out = frac_nom * old;
@@ -763,17 +780,24 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
     const int k = static_cast<int>(e / (static_cast<size_t>(a.fx) * a.fy));
     const size_t idx = (static_cast<size_t>(k) * (a.fy + 2 * a.h) + (j + a.h)) * (a.fx + 2 * a.h) + (i + a.h);
     double v = static_cast<double>(a.f[idx]);
+    // perturbation form: extremes of the full value, sums of the departure
+    const double full = a.fbase ? v + static_cast<double>(a.fbase[k]) : v;
+    double ext = full;
     if (a.kind == 1) {  // column path
       const R idz = a.idz3 ? a.idz3[(static_cast<size_t>(k) * a.fy + j) * a.fx + i] : a.idz_lev[k];
       v = v * (1.0 / static_cast<double>(idz));
+      ext = v;
     } else if (a.kind == 2) {  // theta_m / mixing ratio: f / max(rho, 1e-12)
-      v = v / fmax(static_cast<double>(a.rho[idx]), 1.0e-12);
+      const double r = static_cast<double>(a.rho[idx]) +
+                       (a.rbase ? static_cast<double>(a.rbase[k]) : 0.0);
+      v = full / fmax(r, 1.0e-12);
+      ext = v;
     }
     const double s2 = sum + v;
     comp += fabs(sum) >= fabs(v) ? (sum - s2) + v : (v - s2) + sum;
     sum = s2;
-    mn = fmin(mn, v);
-    mx = fmax(mx, v);
+    mn = fmin(mn, ext);
+    mx = fmax(mx, ext);
   }
   // Block combine in a fixed tree order; every node is a TwoSum whose
   // rounding error joins the compensation, so sum + comp stays exact to
@@ -890,11 +914,11 @@ inline unsigned blocks_for(size_t n, unsigned b) { return static_cast<unsigned>(
 
 template <class R>
 void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
-                           double* partials, int* nchunk_out) {
+                           double* partials, int* nchunk_out, Base<R> base) {
   const int rows = 8;
   const int nchunk = (g.ny + rows - 1) / rows;
   hipLaunchKernelGGL(k_level_partials<R>, dim3(nchunk, g.nz), dim3(256), 0, st, g, rho, aux, mode,
-                     rows, partials);
+                     rows, partials, base);
   *nchunk_out = nchunk;
 }
 template <class R>
@@ -932,14 +956,15 @@ void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain) {
     hipLaunchKernelGGL((k_fast_substep<R, false>), cell_grid(a.g, b), b, 0, st, a);
 }
 template <class R>
-void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th) {
+void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th, Base<R> base) {
   const dim3 b(64, 4, 1);
-  hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th);
+  hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th, base);
 }
 template <class R>
-void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt) {
+void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt,
+                       Base<R> base) {
   const dim3 b(64, 4, 1);
-  hipLaunchKernelGGL(k_rebuild_rt<R>, cell_grid(g, b), b, 0, st, g, rho, th, rt);
+  hipLaunchKernelGGL(k_rebuild_rt<R>, cell_grid(g, b), b, 0, st, g, rho, th, rt, base);
 }
 template <class R>
 void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain) {
@@ -1005,7 +1030,7 @@ void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
 
 #define GWM_INSTANTIATE(R)                                                                          \
   template void launch_level_partials<R>(hipStream_t, const Geo&, const R*, const R*, int,        \
-                                         double*, int*);                                          \
+                                         double*, int*, Base<R>);                                 \
   template void launch_level_finalize<R>(hipStream_t, int, int, const double*, int, double, R*,   \
                                          R*);                                                     \
   template void launch_pressure<R>(hipStream_t, const Geo&, const R*, const R*, R*);             \
@@ -1013,9 +1038,9 @@ void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
   template void launch_column_reference<R>(hipStream_t, const Geo&, const R*, const R*, const R*, \
                                            R, R*, R*);                                            \
   template void launch_fast_substep<R>(hipStream_t, const FastArgs<R>&, bool);                   \
-  template void launch_theta<R>(hipStream_t, const Geo&, const R*, const R*, R*);                \
+  template void launch_theta<R>(hipStream_t, const Geo&, const R*, const R*, R*, Base<R>);       \
   template void launch_tracer_stage<R>(hipStream_t, const TracerArgs<R>&, bool);                 \
-  template void launch_rebuild_rt<R>(hipStream_t, const Geo&, const R*, const R*, R*);           \
+  template void launch_rebuild_rt<R>(hipStream_t, const Geo&, const R*, const R*, R*, Base<R>);  \
   template void launch_warm_rain<R>(hipStream_t, const WarmRainArgs<R>&, bool);                  \
   template void launch_halo_fill<R>(hipStream_t, const Geo&, int, R*);                           \
   template void launch_sync_interfaces<R>(hipStream_t, const Geo&, int, R*);                     \

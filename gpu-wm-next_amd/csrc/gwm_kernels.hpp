@@ -17,7 +17,8 @@ struct SlowArgs {
   R dt;
   R wa, wb;
   int mode;       // 0 tendencies, 1 update, 2 update + blend
-  int out_theta;  // write theta instead of rho*theta in the cell output
+  int out_theta;  // write theta (theta' in the perturbation form) instead of rho*theta
+  Base<R> base;   // perturbation form (base.rho != nullptr)
 };
 
 template <class R>
@@ -34,6 +35,7 @@ struct FastArgs {
   DMetrics<R> m;
   R dtf, gravity;
   int horizontal, density, col_ref;
+  Base<R> base;  // perturbation form: rho, rho_out, rt_out are departures, theta is theta'
 };
 
 template <class R>
@@ -49,6 +51,7 @@ struct TracerArgs {
   int ntr;
   R dt, wa, wb;
   int mode;
+  const R* base_rho;  // perturbation form: rho is rho' against base_rho (nz)
 };
 
 template <class R>
@@ -63,6 +66,7 @@ struct WarmRainArgs {
   R dt, relax, thr, krate, erate, vt, depth, kacc;
   int heat;
   int fallout;  // 0 none, 1 nominal, 2 metric
+  Base<R> base;  // perturbation form
 };
 
 template <class R>
@@ -74,11 +78,15 @@ struct ReduceArgs {
   int fx, fy, fz, h;
   int kind;  // 0 value, 1 value*dz, 2 value/rho
   double* partials;
+  // Perturbation form: per-level bases added for min / max (and for rho in
+  // kind 2); sums stay over the stored departures.
+  const R* fbase;
+  const R* rbase;
 };
 
 template <class R>
 void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
-                           double* partials, int* nchunk_out);
+                           double* partials, int* nchunk_out, Base<R> base = Base<R>{});
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1);
@@ -103,7 +111,8 @@ void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int 
                         double* part_out, int* nparts);
 int fast_column_parts(int nx, int ny);
 template <class R>
-void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th);
+void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th,
+                  Base<R> base = Base<R>{});
 template <class R>
 void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
 // Column-marching tracer stage (gwm_slow.hip), same arguments as
@@ -111,7 +120,8 @@ void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
 template <class R>
 void launch_tracer_column(hipStream_t st, const TracerArgs<R>& a, bool terrain, int nkc);
 template <class R>
-void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt);
+void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt,
+                       Base<R> base = Base<R>{});
 template <class R>
 void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain);
 template <class R>

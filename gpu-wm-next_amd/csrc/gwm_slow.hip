@@ -76,14 +76,23 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
   }
 
   const int nz = g.nz;
+  const bool pert = a.base.rho != nullptr;
   auto stage_one = [&](int ex, int ey, int kk) {
     const int ii = ti0 + ex, jj = tj0 + ey;
     const int e = (ex + 1) + PX * (ey + 1);
     const R r = L.cell(s.rho, ii, jj, kk);
     const R rt = L.cell(s.rt, ii, jj, kk);
-    const R rr = rmax(r, R(1.0e-6));
-    sTH[kk & 3][e] = d_theta(r, rt);
-    sP[kk & 3][e] = d_pressure(r, rt);
+    R rr;
+    if (pert) {  // theta' and p' of the departures
+      const R rb = a.base.rho[kk];
+      rr = rmax(rb + r, R(1.0e-6));
+      sTH[kk & 3][e] = theta_pert(r, rt, rb, a.base.th[kk]);
+      sP[kk & 3][e] = p_pert(rt, a.base.rt[kk], a.base.p[kk]);
+    } else {
+      rr = rmax(r, R(1.0e-6));
+      sTH[kk & 3][e] = d_theta(r, rt);
+      sP[kk & 3][e] = d_pressure(r, rt);
+    }
     sUC[kk & 3][e] = R(0.5) * (L.xf(s.u, ii, jj, kk) + L.xf(s.u, ii + 1, jj, kk)) / rr;
     sVC[kk & 3][e] = R(0.5) * (L.yf(s.v, ii, jj, kk) + L.yf(s.v, ii, jj + 1, kk)) / rr;
     sWC[kk & 3][e] = R(0.5) * (L.zf(s.w, ii, jj, kk) + L.zf(s.w, ii, jj, kk + 1)) / rr;
@@ -162,7 +171,17 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
       const R fn = mn * (mn >= R(0) ? thc : thn);
       const R fb = mb * (mb >= R(0) ? thb : thc);
       const R ft = mt * (mt >= R(0) ? thc : tht);
-      t_rt = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+      if (pert) {
+        // theta = theta_b(level) + theta': the base parts of the horizontal
+        // fluxes share theta_b(k); the vertical ones take the upwind level's.
+        const R tb = a.base.th[k];
+        const R tbb = (mb >= R(0) && k > 0) ? a.base.th[k - 1] : tb;
+        const R tbt = (mt < R(0) && k + 1 < nz) ? a.base.th[k + 1] : tb;
+        t_rt = -((tb * (me - mw) + (fe - fw)) / dx + (tb * (mn - ms) + (fn - fs)) / dy +
+                 ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
+      } else {
+        t_rt = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+      }
     }
 
     // ---- x-face (K17 + K21)
@@ -237,6 +256,18 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
       const R ft = upwind(WC(i, j, ku), qc, qt);
       const R idz = IDZF(i, j, k);
       const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
+      if (pert) {
+        // theta_ref holds the level means of theta'; the bases cancel in
+        // theta_f - theta_ref,f
+        const R rl = L.cell(s.rho, i, j, kl) + a.base.rho[kl];
+        const R ru = L.cell(s.rho, i, j, ku) + a.base.rho[ku];
+        const R tf = R(0.5) * (TH(i, j, kl) + TH(i, j, ku));
+        const R trp = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
+        const R trf = R(0.5) * (a.base.th[kl] + a.base.th[ku]) + trp;
+        const R rf = R(0.5) * (rl + ru);
+        const R bb = a.gravity * (tf - trp) / rmax(trf, R(1.0e-6));
+        return A + rf * bb;
+      }
       const R rl = L.cell(s.rho, i, j, kl), ru = L.cell(s.rho, i, j, ku);
       const R tl = L.cell(s.rt, i, j, kl) / rmax(rl, R(1.0e-6));
       const R tu = L.cell(s.rt, i, j, ku) / rmax(ru, R(1.0e-6));
@@ -264,11 +295,18 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
       if (k == nz - 1) a.out.w[g.zf(i, j, nz)] = R(0);
       continue;
     }
-    const R fl = R(1.0e-6), mf = Phys<R>::mom_floor;
-    const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(s.rho[ci], t_rho, fl));
-    const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(s.rt[ci], t_rt, fl));
+    const R mf = Phys<R>::mom_floor;
+    // floors on the full values (departures: 1e-6 - base)
+    const R flr = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
+    const R flt = pert ? R(1.0e-6) - a.base.rt[k] : R(1.0e-6);
+    const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(s.rho[ci], t_rho, flr));
+    const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(s.rt[ci], t_rt, flt));
     a.out.rho[ci] = rho_new;
-    a.out.rt[ci] = a.out_theta ? rt_new / rmax(rho_new, R(1.0e-6)) : rt_new;
+    if (pert)
+      a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, a.base.rho[k], a.base.th[k])
+                                 : rt_new;
+    else
+      a.out.rt[ci] = a.out_theta ? rt_new / rmax(rho_new, R(1.0e-6)) : rt_new;
     {
       const size_t f = g.xf(i, j, k);
       a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tu0, mf));
@@ -346,7 +384,8 @@ __global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tile
   auto stage_one = [&](int ex, int ey, int kk) {
     const int ii = ti0 + ex, jj = tj0 + ey;
     const int e = (ex + 1) + PX * (ey + 1);
-    const R r = rmax(L.cell(a.rho, ii, jj, kk), R(1.0e-6));
+    const R r0 = L.cell(a.rho, ii, jj, kk);
+    const R r = rmax(a.base_rho ? r0 + a.base_rho[kk] : r0, R(1.0e-6));
 #pragma unroll
     for (int n = 0; n < NT; ++n) sQ[n][kk & 3][e] = L.cell(a.in[n], ii, jj, kk) / r;
   };

@@ -64,7 +64,7 @@ __device__ inline unsigned long long mix64(unsigned long long z) {  // splitmix6
 __global__ void k_sur_sample(Geo g, const float* rho, const float* rt, const float* qv,
                              const float* qc, const float* qr, MicroCfg<float> mc,
                              unsigned long long seed, long n, float* x, float* y,
-                             long long* cell_out) {
+                             long long* cell_out, Base<float> base) {
   const long s = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const unsigned long long cells = static_cast<unsigned long long>(g.nx) * g.ny * g.nz;
@@ -75,7 +75,9 @@ __global__ void k_sur_sample(Geo g, const float* rho, const float* rt, const flo
   const int k = static_cast<int>(lin / (static_cast<unsigned long long>(g.nx) * g.ny));
   const size_t c = g.cell(i, j, k);
   if (cell_out) cell_out[s] = static_cast<long long>(lin);
-  MicroCell<float> m = micro_prepare(rho[c], rt[c], qv[c], qc[c], qr[c]);
+  const bool pert = base.rho != nullptr;
+  MicroCell<float> m = micro_prepare(pert ? rho[c] + base.rho[k] : rho[c],
+                                     pert ? rt[c] + base.rt[k] : rt[c], qv[c], qc[c], qr[c]);
   float* xs = x + s * kSurIn;
   xs[0] = m.T;
   xs[1] = m.p;
@@ -351,7 +353,8 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
                                                        float* qv, float* qc, float* qr,
                                                        MicroCfg<float> mc, const float* w,
-                                                       SurNorm nm, unsigned long long* nfall) {
+                                                       SurNorm nm, unsigned long long* nfall,
+                                                       Base<float> base) {
   __shared__ float sb[196];
   __shared__ unsigned cnt;
   if (threadIdx.x == 0) cnt = 0;
@@ -365,10 +368,12 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
   const long wave = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const long waves = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
   unsigned mine = 0;
+  const bool pert = base.rho != nullptr;
   for (long grp = wave; grp * 16 < n; grp += waves) {
     const long lin = grp * 16 + c;
     const bool valid = lin < n;
     size_t ci = 0;
+    int kk = 0;
     MicroCell<float> m{};
     float f[kSurIn];
     if (valid) {
@@ -376,7 +381,9 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       const int j = static_cast<int>((lin / g.nx) % g.ny);
       const int k = static_cast<int>(lin / (static_cast<long>(g.nx) * g.ny));
       ci = g.cell(i, j, k);
-      m = micro_prepare(rho[ci], rt[ci], qv[ci], qc[ci], qr[ci]);
+      kk = k;
+      m = micro_prepare(pert ? rho[ci] + base.rho[k] : rho[ci],
+                        pert ? rt[ci] + base.rt[k] : rt[ci], qv[ci], qc[ci], qr[ci]);
       f[0] = m.T; f[1] = m.p; f[2] = m.rho; f[3] = m.qv; f[4] = m.qc; f[5] = m.qr; f[6] = mc.dt;
     } else {
       for (int q = 0; q < kSurIn; ++q) f[q] = nm.xm[q];
@@ -387,6 +394,7 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
     norm_features(f, nm, gg, xin);
     const f32x4 o = mlp_group(R, sb, xin, gg);
     if (gg == 0 && valid) {
+      const float th0 = m.th, rtp = rt[ci];
       if (inbox) {
         m.qv = fmaxf(m.qv + (o[0] * nm.ys[0] + nm.ym[0]), 0.0f);
         m.qc = fmaxf(m.qc + (o[1] * nm.ys[1] + nm.ym[1]), 0.0f);
@@ -399,7 +407,9 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       qv[ci] = m.rho * m.qv;
       qc[ci] = m.rho * m.qc;
       qr[ci] = m.rho * m.qr;
-      rt[ci] = m.rho * m.th;
+      // perturbation form: the departure changes by rho * dtheta only
+      rt[ci] = pert ? rtp + m.rho * (m.th - th0) : m.rho * m.th;
+      (void)kk;
     }
   }
   if (mine) atomicAdd(&cnt, mine);
@@ -456,9 +466,9 @@ inline unsigned infer_blocks(long groups) {
 void launch_sur_sample(hipStream_t st, const Geo& g, const float* rho, const float* rt,
                        const float* qv, const float* qc, const float* qr,
                        const MicroCfg<float>& mc, unsigned long long seed, long n, float* x,
-                       float* y, long long* cell) {
+                       float* y, long long* cell, Base<float> base) {
   hipLaunchKernelGGL(k_sur_sample, dim3(nblk(n, 256)), dim3(256), 0, st, g, rho, rt, qv, qc, qr,
-                     mc, seed, n, x, y, cell);
+                     mc, seed, n, x, y, cell, base);
 }
 
 void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const float* y, int B,
@@ -523,10 +533,10 @@ void launch_zero_counter(hipStream_t st, unsigned long long* p) {
 
 void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* nfall) {
+                          const SurNorm& nm, unsigned long long* nfall, Base<float> base) {
   const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
   hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks((cells + 15) / 16)), dim3(256), 0, st, g,
-                     rho, rt, qv, qc, qr, mc, w, nm, nfall);
+                     rho, rt, qv, qc, qr, mc, w, nm, nfall, base);
 }
 
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,

@@ -44,7 +44,7 @@ struct SurTrain {
 void launch_sur_sample(hipStream_t st, const Geo& g, const float* rho, const float* rt,
                        const float* qv, const float* qc, const float* qr,
                        const MicroCfg<float>& mc, unsigned long long seed, long n, float* x,
-                       float* y, long long* cell);
+                       float* y, long long* cell, Base<float> base);
 // Forward + loss + backward on B rows (x raw [B][7], y raw [B][4]); with
 // `update`, one Adam step.  Per-block loss partials (sum of squared
 // normalised errors) go to loss_part, their count to *nloss.
@@ -54,7 +54,7 @@ void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const f
                       float* y);
 void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* nfall);
+                          const SurNorm& nm, unsigned long long* nfall, Base<float> base);
 // Clears a device counter with a kernel (no memset node in a captured graph).
 void launch_zero_counter(hipStream_t st, unsigned long long* p);
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,

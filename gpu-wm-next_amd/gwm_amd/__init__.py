@@ -135,6 +135,9 @@ def lib():
                                            ctypes.POINTER(ctypes.c_double)]
         L.gwm_ctx_reset_kernel_stats.argtypes = [vp]
         L.gwm_ctx_set_chunks.argtypes = [vp, ip]
+        L.gwm_state_create_perturbation.argtypes = [vp, vp, ip, pvp]
+        L.gwm_state_is_perturbation.argtypes = [vp]
+        L.gwm_state_download_f64.argtypes = [vp, ip, vp]
         L.gwm_ctx_tuning_report.argtypes = [vp, cp, ctypes.c_size_t]
         L.gwm_grid_create.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, pvp]
         L.gwm_grid_destroy.argtypes = [vp]
@@ -312,12 +315,16 @@ class Grid:
 class State:
     """Device state; storage() returns the halo-padded arrays (nz, py, px)."""
 
-    def __init__(self, grid, n_tracers=0):
+    def __init__(self, grid, n_tracers=0, perturbation=False):
+        """perturbation=True: the FP32 perturbation form (rho and rho*theta
+        stored as departures from FP64 level-mean bases)."""
         self.grid = grid
         self.ctx = grid.ctx
         self.n_tracers = n_tracers
         self.h = ctypes.c_void_p()
-        check(lib().gwm_state_create(self.ctx.h, grid.h, n_tracers, ctypes.byref(self.h)))
+        create = lib().gwm_state_create_perturbation if perturbation else lib().gwm_state_create
+        check(create(self.ctx.h, grid.h, n_tracers, ctypes.byref(self.h)))
+        self.perturbation = bool(lib().gwm_state_is_perturbation(self.h))
 
     def __del__(self):
         if getattr(self, "h", None) and not _shutting_down:
@@ -351,6 +358,16 @@ class State:
             return s[None]
         h = self.grid.halo
         return np.ascontiguousarray(s[:, h:s.shape[1] - h, h:s.shape[2] - h])
+
+    def get_f64(self, field):
+        """Owned values in FP64 (perturbation form: departure + FP64 base)."""
+        shape = self.storage_shape(field)
+        out = np.empty(shape, np.float64)
+        check(lib().gwm_state_download_f64(self.h, field, _ptr(out)))
+        if field == PRECIP:
+            return out[None]
+        h = self.grid.halo
+        return np.ascontiguousarray(out[:, h:out.shape[1] - h, h:out.shape[2] - h])
 
     def set(self, field, arr):
         if field == PRECIP:

@@ -210,10 +210,25 @@ int gwm_stretched_levels(int nz, double dz_bottom, double dz_top, double z_top, 
  * (include/gwm/physics/warm_rain.hpp:25-46), all on the device. */
 int gwm_state_create(gwm_ctx* ctx, const gwm_grid* grid, int n_tracers, gwm_state** out);
 int gwm_state_destroy(gwm_state* state);
+/* FP32 perturbation form (extension, no reference counterpart): rho_d and
+ * rho_d*theta_m are stored as FP32 departures from per-level bases, the FP64
+ * level means of the state they were loaded from; pressure enters through
+ * p' = p_b expm1(gamma log1p((rho theta)'/(rho theta)_b)), so pressure
+ * differences keep their relative precision and the dry-mass budget is the
+ * FP64 base sum plus a compensated sum of departures.  Needs an FP32
+ * context, flat terrain and a doubly periodic grid.  Uploads, downloads and
+ * fills of GWM_RHO / GWM_RHO_THETA take and return full values (an upload
+ * re-bases that field); every operator accepts the state. */
+int gwm_state_create_perturbation(gwm_ctx* ctx, const gwm_grid* grid, int n_tracers,
+                                  gwm_state** out);
+int gwm_state_is_perturbation(const gwm_state* state);
 /* Element count of a field's storage (halo included). */
 long gwm_state_field_size(const gwm_state* state, int field);
 int gwm_state_upload(gwm_state* state, int field, const void* host);
 int gwm_state_download(gwm_state* state, int field, void* host);
+/* Download as FP64 (a perturbation-form departure plus its FP64 base,
+ * without rounding to FP32). */
+int gwm_state_download_f64(gwm_state* state, int field, double* host);
 int gwm_state_fill(gwm_state* state, int field, double value);
 int gwm_state_copy(gwm_state* dst, const gwm_state* src);
 /* Keep a copy of the current state as the lateral-boundary reference. */

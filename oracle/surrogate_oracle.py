@@ -46,8 +46,20 @@ def unpack(p):
     return out
 
 
-def normalise_x(x, norm):
+def normalise_x(x, norm, fp32=False):
+    """(x - mean) / std; fp32=True rounds like the device, which normalises
+    in FP32 before the FP32 layers."""
+    if fp32:
+        return ((np.asarray(x, np.float32) - np.asarray(norm["x_mean"], np.float32)) /
+                np.asarray(norm["x_std"], np.float32)).astype(np.float64)
     return (np.asarray(x, np.float64) - norm["x_mean"]) / norm["x_std"]
+
+
+def normalise_y(y, norm, fp32=False):
+    if fp32:
+        return ((np.asarray(y, np.float32) - np.asarray(norm["y_mean"], np.float32)) /
+                np.asarray(norm["y_std"], np.float32)).astype(np.float64)
+    return (np.asarray(y, np.float64) - norm["y_mean"]) / norm["y_std"]
 
 
 def forward(p, xn):
@@ -63,10 +75,12 @@ def forward(p, xn):
     return zs, acts, h
 
 
-def loss_grad(p, x_raw, y_raw, norm):
-    """Mean squared error of the normalised outputs and its gradient."""
-    xn = normalise_x(x_raw, norm)
-    yt = (np.asarray(y_raw, np.float64) - norm["y_mean"]) / norm["y_std"]
+def loss_grad(p, x_raw, y_raw, norm, fp32_normalise=False):
+    """Mean squared error of the normalised outputs and its gradient
+    (fp32_normalise: features and targets normalised in FP32, as on the
+    device; the network itself stays FP64)."""
+    xn = normalise_x(x_raw, norm, fp32_normalise)
+    yt = normalise_y(y_raw, norm, fp32_normalise)
     zs, acts, y = forward(p, xn)
     B = xn.shape[0]
     d = y - yt

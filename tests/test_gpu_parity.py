@@ -524,9 +524,13 @@ def test_graph_replayed_steps_equal_eager_steps(ctx32, ctx64, prec):
     for _ in range(7):
         a.moist_step(cfg, wr)
     b.moist_steps_graph(cfg, wr, 3)   # eager + one graph replay
-    b.moist_steps_graph(cfg, wr, 4)   # other parity: capture + replays
+    b.moist_steps_graph(cfg, wr, 4)   # replays
     for f in FIELDS + (O.TRACER0, O.TRACER0 + 1, O.TRACER0 + 2, O.PRECIP):
-        assert np.array_equal(a.get(f), b.get(f)), f
+        x, y = a.get(f), b.get(f)
+        d = np.abs(x.astype(np.float64) - y)
+        print(f, "nonfinite", (~np.isfinite(x)).sum(), (~np.isfinite(y)).sum(),
+              "max diff", np.nanmax(d), "cells differing", int((x != y).sum()))
+        assert np.array_equal(x, y, equal_nan=True), f
 
 
 def test_ensemble_steps_equal_member_steps():

@@ -1,0 +1,129 @@
+"""FP32 perturbation form against the FP64 oracle (GPU).
+
+The perturbation-form state (rho' and (rho theta)' in FP32 against FP64
+level-mean bases) starts from exactly the values of the FP64 oracle's world
+(downloaded in FP64), steps the same configuration, and must stay within
+relative L2 1e-4 of the FP64 oracle on every field -- the bar the reference's
+own FP32 formulation misses by an order of magnitude -- while the dry-mass
+budget (FP64 base sums plus compensated sums of the departures) changes by
+at most 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import gwm_amd as G
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (O.RHO, O.RHO_THETA, O.MOM_U, O.MOM_V, O.MOM_W)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = G.Context(0, G.F32)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b, floor):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), floor * np.sqrt(b.size))
+
+
+def bubble_pair(ctx, moist=False):
+    nx, ny, nz = 48, 40, 20
+    zi = np.linspace(0.0, 10000.0, nz + 1)
+    grid = G.Grid(ctx, nx, ny, nz, dx=500.0, dy=500.0, z_interfaces=zi)
+    ntr = 3 if moist else 0
+    st = G.State(grid, ntr, perturbation=True)
+    full = G.State(grid, ntr)
+    w = O.World(nx, ny, nz, dx=500.0, dy=500.0, z_interfaces=zi, n_tracers=ntr, precision=8)
+    if moist:
+        st.init_convection(1.0e5, 5, 0.5, 3000.0)
+    else:
+        st.init_bubble(-1.16, 300.0, u=3.0, center=(0.45, 0.55, 0.3), radius=(0.2, 0.25, 0.25),
+                       theta_perturbation=2.0)
+    assert st.perturbation
+    fields = FIELDS + ((16, 17, 18) if moist else ())
+    if moist:  # some cloud and rain to exercise the warm-rain terms
+        qv = st.get_f64(16)
+        rng = np.random.default_rng(1)
+        st.set(16, (qv * rng.uniform(0.95, 1.2, qv.shape)).astype(np.float32))
+        st.set(17, (0.02 * qv).astype(np.float32))
+        st.set(18, (0.01 * qv).astype(np.float32))
+    for f in fields:
+        v = st.get_f64(f)
+        w.set(f, v)
+        full.set(f, v.astype(np.float32))
+    return grid, st, full, w, fields
+
+
+def test_perturbation_initial_state_matches_fp64(ctx):
+    nx, ny, nz = 32, 24, 16
+    zi = np.linspace(0.0, 12000.0, nz + 1)
+    g32 = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    st = G.State(g32, 3, perturbation=True)
+    st.init_convection(1.0e5, 9, 0.5, 1000.0)
+    c64 = G.Context(0, G.F64)
+    g64 = G.Grid(c64, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    ref = G.State(g64, 3)
+    ref.init_convection(1.0e5, 9, 0.5, 1000.0)
+    for f in (O.RHO, O.RHO_THETA):
+        a, b = st.get_f64(f), ref.get(f)
+        # FP64 base + FP32 departure: within an FP32 ulp of the departure
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-9, f
+        # the FP32 download rounds the full value once
+        assert np.max(np.abs(st.get(f) - b) / np.abs(b)) < 1.2e-7, f
+    s, r = st.summary(), ref.summary()
+    assert abs(s["total_dry_mass"] - r["total_dry_mass"]) <= 1e-12 * r["total_dry_mass"]
+    assert abs(s["total_rho_theta_m"] - r["total_rho_theta_m"]) <= 1e-12 * r["total_rho_theta_m"]
+    del ref, g64
+    c64.close()
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_perturbation_form_within_1e4_of_fp64_oracle(ctx, moist):
+    grid, st, full, w, fields = bubble_pair(ctx, moist)
+    cfg = dict(dt=2.0, fast_substeps=3)
+    wr = dict(dt=2.0, rain_terminal_velocity=12.0)
+    m0 = st.summary()["total_dry_mass"]
+    for _ in range(5):
+        if moist:
+            st.moist_step(G.step_config(**cfg), G.warm_rain_config(**wr))
+            full.moist_step(G.step_config(**cfg), G.warm_rain_config(**wr))
+            w.ssprk3_step(O.step_cfg(**cfg))
+            w.tracers_ssprk3(O.step_cfg(**cfg))
+            w.warm_rain(O.warm_rain_cfg(**wr), metric_fallout=True)
+        else:
+            st.ssprk3_step(G.step_config(**cfg))
+            full.ssprk3_step(G.step_config(**cfg))
+            w.ssprk3_step(O.step_cfg(**cfg))
+    floors = {O.RHO: 1.0, O.RHO_THETA: 300.0, O.MOM_U: 1.0, O.MOM_V: 1.0, O.MOM_W: 1.0,
+              16: 1e-3, 17: 1e-4, 18: 1e-4}
+    worst_p, worst_f = 0.0, 0.0
+    for f in fields:
+        ref = w.get(f)
+        ep = rel_l2(st.get_f64(f), ref, floors[f] * 1e-3)
+        ef = rel_l2(full.get(f), ref, floors[f] * 1e-3)
+        print(f"field {f}: perturbation form {ep:.2e}, full FP32 form {ef:.2e}")
+        worst_p, worst_f = max(worst_p, ep), max(worst_f, ef)
+        assert ep <= 1e-4, (f, ep)
+    print(f"worst: perturbation {worst_p:.2e}, full FP32 {worst_f:.2e}")
+    # dry mass: the flux form conserves it; departures keep 1e-12
+    m1 = st.summary()["total_dry_mass"]
+    assert abs(m1 - m0) <= 1e-12 * m0, (m0, m1)
+    assert abs(m1 - w.summary()["total_dry_mass"]) <= 1e-10 * m0
+
+
+def test_perturbation_graph_replay_equals_eager(ctx):
+    _, a, _, _, _ = bubble_pair(ctx, moist=True)
+    _, b, _, _, _ = bubble_pair(ctx, moist=True)
+    cfg = G.step_config(dt=2.0, fast_substeps=3)
+    wr = G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0)
+    for _ in range(5):
+        a.moist_step(cfg, wr)
+    b.moist_steps_graph(cfg, wr, 5)
+    for f in FIELDS + (16, 17, 18):
+        assert np.array_equal(a.get_f64(f), b.get_f64(f), equal_nan=True), f
