@@ -35,7 +35,7 @@ __device__ inline float gelu_grad_f(float x) {
 // Same function as gelu_f written as x * sigmoid(2u), for inference.
 __device__ inline float gelu_fast(float x) {
   const float u = kGeluA * (x + kGeluB * x * x * x);
-  return x / (1.0f + __expf(-2.0f * u));
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
 // FP32 -> BF16 with round-to-nearest-even (NaN kept quiet).
@@ -313,37 +313,50 @@ __device__ inline f32x4 mlp_group(const MlpRegs& R, const float* sb, const float
   return acc;
 }
 
-__device__ inline void norm_features(const float f[kSurIn], const SurNorm& nm, int g,
-                                     float xin[4]) {
-  for (int v = 0; v < 4; ++v) {
-    const int k = 4 * g + v;
-    xin[v] = k < kSurIn ? (f[k] - nm.xm[k]) / nm.xs[k] : 0.0f;
+}  // namespace
+
+// Both inference kernels run in tiles of 256 cells per work-group (64 per
+// wave, one per lane): each lane prepares its own cell's features and writes
+// them, normalised, to LDS; the wave then evaluates its 64 cells as four
+// 16-cell MFMA groups, leaving the outputs in LDS; finally each lane applies
+// its own cell's outputs.  The trip count is uniform over the work-group, so
+// the barriers are too.
+constexpr int kTile = 256;
+
+__device__ inline void tile_mlp(const MlpRegs& R, const float* sb, const float (*feat)[8],
+                                float (*out)[kSurOut], int lane) {
+  const int c = lane & 15, g = lane >> 4;
+  for (int q = 0; q < 4; ++q) {
+    float xin[4];
+    for (int v = 0; v < 4; ++v) xin[v] = g < 2 ? feat[16 * q + c][4 * g + v] : 0.0f;
+    const f32x4 o = mlp_group(R, sb, xin, g);
+    if (g == 0)
+      for (int r = 0; r < kSurOut; ++r) out[16 * q + c][r] = o[r];
   }
 }
-
-}  // namespace
 
 // Raw features x[n][7] -> raw outputs y[n][4] (tests and timing).
 __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, const float* x,
                                                    long n, float* y) {
   __shared__ float sb[196];
+  __shared__ float sfeat[4][64][8];
+  __shared__ float sout[4][64][kSurOut];
   load_bias(w, sb);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int c = lane & 15, g = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   MlpRegs R;
-  load_regs(w, R, c, g);
-  const long wave = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const long waves = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
-  for (long grp = wave; grp * 16 < n; grp += waves) {
-    const long cell = grp * 16 + c;
-    float f[kSurIn];
-    for (int q = 0; q < kSurIn; ++q) f[q] = cell < n ? x[cell * kSurIn + q] : nm.xm[q];
-    float xin[4];
-    norm_features(f, nm, g, xin);
-    const f32x4 o = mlp_group(R, sb, xin, g);
-    if (g == 0 && cell < n)
-      for (int q = 0; q < kSurOut; ++q) y[cell * kSurOut + q] = o[q] * nm.ys[q] + nm.ym[q];
+  load_regs(w, R, lane & 15, lane >> 4);
+  for (long tile = blockIdx.x; tile * kTile < n; tile += gridDim.x) {
+    const long cell = tile * kTile + wv * 64 + lane;
+    const bool valid = cell < n;
+    for (int q = 0; q < 8; ++q)
+      sfeat[wv][lane][q] = (valid && q < kSurIn) ? (x[cell * kSurIn + q] - nm.xm[q]) / nm.xs[q]
+                                                 : 0.0f;
+    __syncthreads();
+    tile_mlp(R, sb, sfeat[wv], sout[wv], lane);
+    __syncthreads();
+    if (valid)
+      for (int q = 0; q < kSurOut; ++q) y[cell * kSurOut + q] = sout[wv][lane][q] * nm.ys[q] + nm.ym[q];
   }
 }
 
@@ -356,46 +369,50 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
                                                        SurNorm nm, unsigned long long* nfall,
                                                        Base<float> base) {
   __shared__ float sb[196];
+  __shared__ float sfeat[4][64][8];
+  __shared__ float sout[4][64][kSurOut];
   __shared__ unsigned cnt;
   if (threadIdx.x == 0) cnt = 0;
   load_bias(w, sb);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int c = lane & 15, gg = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   MlpRegs R;
-  load_regs(w, R, c, gg);
-  const long n = static_cast<long>(g.nx) * g.ny * g.nz;
-  const long wave = (static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const long waves = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
-  unsigned mine = 0;
+  load_regs(w, R, lane & 15, lane >> 4);
+  const int nxy = g.nx * g.ny;
+  const int n = nxy * g.nz;
   const bool pert = base.rho != nullptr;
-  for (long grp = wave; grp * 16 < n; grp += waves) {
-    const long lin = grp * 16 + c;
-    const bool valid = lin < n;
+  unsigned mine = 0;
+  for (int tile = blockIdx.x; tile * kTile < n; tile += gridDim.x) {
+    const int cell = tile * kTile + wv * 64 + lane;
+    const bool valid = cell < n;
     size_t ci = 0;
-    int kk = 0;
     MicroCell<float> m{};
     float f[kSurIn];
+    float rtp = 0.0f;
     if (valid) {
-      const int i = static_cast<int>(lin % g.nx);
-      const int j = static_cast<int>((lin / g.nx) % g.ny);
-      const int k = static_cast<int>(lin / (static_cast<long>(g.nx) * g.ny));
+      const int k = cell / nxy, rem = cell - k * nxy;
+      const int j = rem / g.nx, i = rem - j * g.nx;
       ci = g.cell(i, j, k);
-      kk = k;
-      m = micro_prepare(pert ? rho[ci] + base.rho[k] : rho[ci],
-                        pert ? rt[ci] + base.rt[k] : rt[ci], qv[ci], qc[ci], qr[ci]);
+      rtp = rt[ci];
+      m = micro_prepare(pert ? rho[ci] + base.rho[k] : rho[ci], pert ? rtp + base.rt[k] : rtp,
+                        qv[ci], qc[ci], qr[ci]);
       f[0] = m.T; f[1] = m.p; f[2] = m.rho; f[3] = m.qv; f[4] = m.qc; f[5] = m.qr; f[6] = mc.dt;
     } else {
       for (int q = 0; q < kSurIn; ++q) f[q] = nm.xm[q];
     }
     bool inbox = true;
-    for (int q = 0; q < kSurIn; ++q) inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
-    float xin[4];
-    norm_features(f, nm, gg, xin);
-    const f32x4 o = mlp_group(R, sb, xin, gg);
-    if (gg == 0 && valid) {
-      const float th0 = m.th, rtp = rt[ci];
+    for (int q = 0; q < kSurIn; ++q) {
+      inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
+      sfeat[wv][lane][q] = (f[q] - nm.xm[q]) / nm.xs[q];
+    }
+    sfeat[wv][lane][7] = 0.0f;
+    __syncthreads();
+    tile_mlp(R, sb, sfeat[wv], sout[wv], lane);
+    __syncthreads();
+    if (valid) {
+      const float th0 = m.th;
       if (inbox) {
+        const float* o = sout[wv][lane];
         m.qv = fmaxf(m.qv + (o[0] * nm.ys[0] + nm.ym[0]), 0.0f);
         m.qc = fmaxf(m.qc + (o[1] * nm.ys[1] + nm.ym[1]), 0.0f);
         m.qr = fmaxf(m.qr + (o[2] * nm.ys[2] + nm.ym[2]), 0.0f);
@@ -409,7 +426,6 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       qr[ci] = m.rho * m.qr;
       // perturbation form: the departure changes by rho * dtheta only
       rt[ci] = pert ? rtp + m.rho * (m.th - th0) : m.rho * m.th;
-      (void)kk;
     }
   }
   if (mine) atomicAdd(&cnt, mine);
@@ -457,9 +473,9 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 // ---------------------------------------------------------------------------
 namespace {
 inline unsigned nblk(long n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
-inline unsigned infer_blocks(long groups) {
-  const long waves = groups < 2048 ? groups : 2048;
-  return static_cast<unsigned>((waves + 3) / 4);
+inline unsigned infer_blocks(long cells) {
+  const long tiles = (cells + kTile - 1) / kTile;
+  return static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
 }
 }  // namespace
 
@@ -523,8 +539,7 @@ void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const fl
 
 void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const float* x, long n,
                       float* y) {
-  hipLaunchKernelGGL(k_sur_infer, dim3(infer_blocks((n + 15) / 16)), dim3(256), 0, st, w, nm, x,
-                     n, y);
+  hipLaunchKernelGGL(k_sur_infer, dim3(infer_blocks(n)), dim3(256), 0, st, w, nm, x, n, y);
 }
 
 void launch_zero_counter(hipStream_t st, unsigned long long* p) {
@@ -535,8 +550,8 @@ void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float*
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
                           const SurNorm& nm, unsigned long long* nfall, Base<float> base) {
   const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
-  hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks((cells + 15) / 16)), dim3(256), 0, st, g,
-                     rho, rt, qv, qc, qr, mc, w, nm, nfall, base);
+  hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks(cells)), dim3(256), 0, st, g, rho, rt,
+                     qv, qc, qr, mc, w, nm, nfall, base);
 }
 
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
