@@ -89,9 +89,6 @@ struct gwm_ctx {
   size_t partials_n = 0;
   int refs = 0;            // live grids and states
   bool closing = false;    // gwm_ctx_destroy called while refs > 0
-  // Stencil kernels: column march (default) or the point kernels
-  // (GWM_KERNELS=point, kept for comparison).
-  bool fast_point = false;
   // Level chunks per column of the column kernels: chosen per (kernel,
   // shape, precision) by timing the candidates at first use (autotuner), or
   // fixed by gwm_ctx_set_chunks / GWM_CHUNKS.
@@ -296,8 +293,6 @@ struct gwm_state {
   void* theta_ref = nullptr;  // nz
   void* rref_lev = nullptr;   // nz
   void* pref_lev = nullptr;   // nz
-  void* rref3 = nullptr;      // nx*ny*nz
-  void* pref3 = nullptr;
   void* theta_scr = nullptr;  // cell field
   // Perturbation form (flat, doubly periodic, FP32): rho and rho*theta are
   // stored as departures from per-level bases, the FP64 level means of the
@@ -388,7 +383,7 @@ struct gwm_state {
     for (auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr})
       for (void* p : *v)
         if (p) (void)hipFree(p);
-    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, rref3, pref3, theta_scr, b_dev})
+    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev})
       if (p) (void)hipFree(p);
   }
 };
@@ -492,9 +487,6 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   if (!colref) {
     if (!S->rref_lev) S->rref_lev = dalloc(g.nz * sizeof(R));
     if (!S->pref_lev) S->pref_lev = dalloc(g.nz * sizeof(R));
-  } else if (S->ctx->fast_point) {
-    if (!S->rref3) S->rref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
-    if (!S->pref3) S->pref3 = dalloc(static_cast<size_t>(g.nx) * g.ny * g.nz * sizeof(R));
   }
   const int outs[2] = {z0, z1};
   const R* rho = rho_in;
@@ -511,7 +503,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   for (int s = 0; s < ns; ++s) {
     const int o = outs[s % 2];
     if (!colref) {
-      if (s == 0 || S->ctx->fast_point) {
+      if (s == 0) {
         Timed t(S->ctx, "level_partials");
         gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nparts,
                                       S->base<R>());
@@ -520,11 +512,6 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
       gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nparts, part, 1,
                                     static_cast<double>(g.nx) * g.ny,
                                     static_cast<R*>(S->rref_lev), static_cast<R*>(S->pref_lev));
-    } else if (S->ctx->fast_point) {
-      Timed t(S->ctx, "column_reference");
-      gwm::launch_column_reference<R>(S->ctx->stream, g, rho, theta,
-                                      static_cast<const R*>(G->zc_m), static_cast<R>(c.gravity),
-                                      static_cast<R*>(S->rref3), static_cast<R*>(S->pref3));
     }
     gwm::FastArgs<R> a{};
     a.g = g;
@@ -543,8 +530,6 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     a.w_out = static_cast<R*>(S->set[o][4]);
     a.rref_lev = static_cast<const R*>(S->rref_lev);
     a.pref_lev = static_cast<const R*>(S->pref_lev);
-    a.rref3 = static_cast<const R*>(S->rref3);
-    a.pref3 = static_cast<const R*>(S->pref3);
     a.m = G->dm<R>();
     a.dtf = dtf;
     a.gravity = static_cast<R>(c.gravity);
@@ -552,21 +537,15 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     a.density = c.fast_update_density;
     a.col_ref = colref;
     a.base = S->base<R>();
-    if (S->ctx->fast_point) {
-      Timed t(S->ctx, "fast_substep");
-      gwm::launch_fast_substep<R>(S->ctx->stream, a, G->terrain_nonzero);
-    } else {
-      const bool emit = !colref && s + 1 < ns;
-      auto run = [&](int n) {
-        gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
-                                   emit ? part : nullptr, emit ? &nparts : nullptr);
-      };
-      const int nkc = colref ? 1
-                             : pick_chunks(S->ctx, "fast_column", g,
-                                           G->terrain_nonzero ? 1 : 0, run);
-      Timed t(S->ctx, "fast_column");
-      run(nkc);
-    }
+    const bool emit = !colref && s + 1 < ns;
+    auto run = [&](int n) {
+      gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
+                                 emit ? part : nullptr, emit ? &nparts : nullptr);
+    };
+    const int nkc =
+        colref ? 1 : pick_chunks(S->ctx, "fast_column", g, G->terrain_nonzero ? 1 : 0, run);
+    Timed t(S->ctx, "fast_column");
+    run(nkc);
     rho = a.rho_out;
     u = a.u_out;
     v = a.v_out;
@@ -580,7 +559,6 @@ template <class R>
 void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* const* q0,
                 void* const* out, int mode, R wa, R wb, int out_theta) {
   const Geo& g = S->g();
-  require(!(S->pert && S->ctx->fast_point), "the perturbation form runs on the column kernels");
   level_theta_ref<R>(S, static_cast<const R*>(in[0]), static_cast<const R*>(in[1]));
   gwm::SlowArgs<R> a{};
   a.g = g;
@@ -596,11 +574,6 @@ void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* c
   a.mode = mode;
   a.out_theta = out_theta;
   a.base = S->base<R>();
-  if (S->ctx->fast_point) {
-    Timed t(S->ctx, "slow_stage");
-    gwm::launch_slow_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
-    return;
-  }
   auto run = [&](int n) {
     gwm::launch_slow_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
   };
@@ -816,11 +789,6 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
     a.wa = wa;
     a.wb = wb;
     a.base_rho = S->base<R>().rho;
-    if (S->ctx->fast_point) {
-      Timed tt(S->ctx, "tracer_stage");
-      gwm::launch_tracer_stage<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
-      return;
-    }
     auto run = [&](int n) {
       gwm::launch_tracer_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
     };
@@ -1396,8 +1364,6 @@ int gwm_ctx_create(int device, int precision, gwm_ctx** out) {
     auto c = std::make_unique<gwm_ctx>();
     c->device = device;
     c->prec = precision;
-    const char* fk = std::getenv("GWM_KERNELS");
-    c->fast_point = fk != nullptr && std::string(fk) == "point";
     const char* ch = std::getenv("GWM_CHUNKS");
     if (ch != nullptr) c->force_chunks = std::max(0, std::atoi(ch));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1460,6 +1426,20 @@ int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks) {
     require(ctx, "null context");
     require(chunks >= 0, "chunks must be >= 0");
     ctx->force_chunks = chunks;
+  });
+}
+
+int gwm_fp32_math(gwm_ctx* ctx, int fn, const void* dev_x, void* dev_y, long n) {
+  return guard([&] {
+    require(ctx && dev_x && dev_y, "null argument");
+    require(fn >= GWM_MATH_POW_EOS && fn <= GWM_MATH_RCP, "unknown function");
+    require(n >= 0, "n must be >= 0");
+    if (n == 0) return;
+    HIPCHK(hipSetDevice(ctx->device));
+    gwm::launch_fp32_math(ctx->stream, fn, static_cast<const float*>(dev_x),
+                          static_cast<float*>(dev_y), static_cast<size_t>(n));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 

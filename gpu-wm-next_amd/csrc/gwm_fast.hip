@@ -5,8 +5,8 @@
 // fixed theta, K11 column hydrostatic reference (terrain), K12/K13
 // horizontal pressure gradient, K14 vertical perturbation branch, K15
 // density from the divergence of (m - m_base), and K10 rho*theta rebuild on
-// the last substep.  Arithmetic per value is the same as the point kernel in
-// gwm_kernels.hip (k_fast_substep + k_column_reference).
+// the last substep.  Arithmetic is the step kernels' q* forms
+// (gwm_internal.hpp).
 //
 // Mapping.  A work-group of 64 x 4 threads owns a 64 x 4 column tile and
 // marches it upwards over a chunk of levels [k0, k1), one level per
@@ -44,7 +44,7 @@ __device__ inline double wave_sum(double v) {
 
 }  // namespace
 
-template <class R, bool TERRAIN, bool COLREF>
+template <class R, bool TERRAIN, bool COLREF, bool PERT>
 __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
                                                      int nkc, double* part_out) {
   __shared__ R sp[4][PN];
@@ -78,23 +78,26 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   const int hci = (hx + 1) + PX * (hy + 1);
 
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R rdx = R(1) / dx, rdy = R(1) / dy;
   const R dtf = a.dtf, grav = a.gravity;
 
   // Raw loads of one plane: own cell and (t < NHALO) one ring cell.
   R rr_c, rt_c, rr_h = R(0), rt_h = R(0);
+  const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;
+  const int oc = stage_pos(g, i, j).c, hc = stage_pos(g, ti0 + hx, tj0 + hy).c;
   auto load_plane = [&](int k) {
-    rr_c = L.cell(a.rho, i, j, k);
-    rt_c = L.cell(a.theta, i, j, k);
+    rr_c = ld_at(a.rho, oc, pc, k);
+    rt_c = ld_at(a.theta, oc, pc, k);
     if (t < NHALO) {
-      rr_h = L.cell(a.rho, ti0 + hx, tj0 + hy, k);
-      rt_h = L.cell(a.theta, ti0 + hx, tj0 + hy, k);
+      rr_h = ld_at(a.rho, hc, pc, k);
+      rt_h = ld_at(a.theta, hc, pc, k);
     }
   };
   // Perturbation form: rho' and theta' in, p' out (K10 rho*theta = rho*theta_fixed
   // written for the departures: rho_b theta' + rho' (theta_b + theta')).
-  const bool pert = a.base.rho != nullptr;
+  constexpr bool pert = PERT;
   auto press = [&](R r, R th, int k) -> R {
-    if (!pert) return d_pressure(r, r * th);
+    if (!pert) return q_pressure(r, r * th);
     const R rb = a.base.rho[k], tb = a.base.th[k];
     return p_pert(rb * th + r * (tb + th), a.base.rt[k], a.base.p[k]);
   };
@@ -153,8 +156,8 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   R pref_k = R(0), rref_k = R(0);
   if (COLREF) {
     const R rt0 = rho_k * th_k;
-    pref_k = d_pressure(rho_k, rt0);
-    rref_k = d_rho_from_p_theta(pref_k, d_theta(rho_k, rt0));
+    pref_k = q_pressure(rho_k, rt0);
+    rref_k = q_rho_from_p_theta(pref_k, q_theta(rho_k, rt0));
   }
 
   const bool ow = !g.per_x, os = !g.per_y;
@@ -195,16 +198,16 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
     // Column reference at level k+1 (K11 recurrence, own column).
     R pref_k1 = R(0), rref_k1 = R(0);
     if (COLREF && k + 1 < nz && active) {
-      const R thk = d_theta(rho_k1, rho_k1 * th_k1);
+      const R thk = q_theta(rho_k1, rho_k1 * th_k1);
       const R dz = a.m.zc[own_m + (k + 1) * plane_m] - a.m.zc[own_m + k * plane_m];
       R pn = rmax(R(1), pref_k - grav * rref_k * dz);
       for (int it = 0; it < 4; ++it) {
-        const R rn = d_rho_from_p_theta(pn, thk);
+        const R rn = q_rho_from_p_theta(pn, thk);
         pn = pref_k - grav * R(0.5) * (rref_k + rn) * dz;
         pn = rmax(pn, R(1));
       }
       pref_k1 = pn;
-      rref_k1 = d_rho_from_p_theta(pn, thk);
+      rref_k1 = q_rho_from_p_theta(pn, thk);
     }
 
     R rho_new = R(0);
@@ -215,30 +218,30 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
         const R* zc = a.m.zc;
         if (k == 0) {
           const R dz = zc[m + plane_m] - zc[m];
-          return (pup[off] - pk[off]) / rmax(dz, R(1.0e-6));
+          return qdiv(pup[off] - pk[off], rmax(dz, R(1.0e-6)));
         }
         if (k + 1 == nz) {
           const R dz = zc[m + k * plane_m] - zc[m + (k - 1) * plane_m];
-          return (pk[off] - pdn[off]) / rmax(dz, R(1.0e-6));
+          return qdiv(pk[off] - pdn[off], rmax(dz, R(1.0e-6)));
         }
         const R dz = zc[m + (k + 1) * plane_m] - zc[m + (k - 1) * plane_m];
-        return (pup[off] - pdn[off]) / rmax(dz, R(1.0e-6));
+        return qdiv(pup[off] - pdn[off], rmax(dz, R(1.0e-6)));
       };
-      auto pgf = [&](int offl, int offr, int il, int jl, int ir, int jr, R d) -> R {
+      auto pgf = [&](int offl, int offr, int il, int jl, int ir, int jr, R d, R rd) -> R {
         const R pl = pk[offl], pr = pk[offr];
-        if (!TERRAIN) return (pr - pl) / d;
+        if (!TERRAIN) return qdivc(pr - pl, d, rd);
         const size_t ml = mcol(il, jl), mr = mcol(ir, jr);
-        const R dzd = (a.m.zc[mr + k * plane_m] - a.m.zc[ml + k * plane_m]) / d;
+        const R dzd = qdivc(a.m.zc[mr + k * plane_m] - a.m.zc[ml + k * plane_m], d, rd);
         const R gl = vgrad(offl, ml), gr = vgrad(offr, mr);
-        return (pr - pl) / d - dzd * (R(0.5) * (gl + gr));
+        return qdivc(pr - pl, d, rd) - dzd * (R(0.5) * (gl + gr));
       };
       // K12 / K13: faces i, i+1 and j, j+1 of the cell.
       R u0 = uc0, u1 = uc1, v0 = vc0, v1 = vc1;
       if (a.horizontal) {
-        if (!(i == 0 && ow)) u0 = uc0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx));
-        if (!(i + 1 == g.nx && ow)) u1 = uc1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx));
-        if (!(j == 0 && os)) v0 = vc0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy));
-        if (!(j + 1 == g.ny && os)) v1 = vc1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy));
+        if (!(i == 0 && ow)) u0 = uc0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx, rdx));
+        if (!(i + 1 == g.nx && ow)) u1 = uc1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx, rdx));
+        if (!(j == 0 && os)) v0 = vc0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy));
+        if (!(j + 1 == g.ny && os)) v1 = vc1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy));
       }
       // K14: w face k+1 (faces 0 and nz are left as they are).
       R w1 = wc1;
@@ -260,7 +263,8 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       rho_new = rho_k;
       if (a.density) {
         const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : a.m.lev_idz_c[k];
-        const R div = ((u1 - ub1) - (u0 - ub0)) / dx + ((v1 - vb1) - (v0 - vb0)) / dy +
+        const R div = qdivc((u1 - ub1) - (u0 - ub0), dx, rdx) +
+                      qdivc((v1 - vb1) - (v0 - vb0), dy, rdy) +
                       ((w1 - wb1) - (w0 - wb0)) * idz;
         const R floor = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
@@ -318,17 +322,20 @@ void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int 
   const dim3 grid(tiles * nkc), blk(TX, TY, 1);
   if (a.col_ref) {
     if (terrain)
-      hipLaunchKernelGGL((k_fast_column<R, true, true>), grid, blk, 0, st, a, tx, tiles, 1,
+      hipLaunchKernelGGL((k_fast_column<R, true, true, false>), grid, blk, 0, st, a, tx, tiles, 1,
                          nullptr);
     else
-      hipLaunchKernelGGL((k_fast_column<R, false, true>), grid, blk, 0, st, a, tx, tiles, 1,
+      hipLaunchKernelGGL((k_fast_column<R, false, true, false>), grid, blk, 0, st, a, tx, tiles, 1,
                          nullptr);
   } else if (terrain) {
-    hipLaunchKernelGGL((k_fast_column<R, true, false>), grid, blk, 0, st, a, tx, tiles, nkc,
+    hipLaunchKernelGGL((k_fast_column<R, true, false, false>), grid, blk, 0, st, a, tx, tiles, nkc,
                        part_out);
+  } else if (a.base.rho) {  // perturbation form (flat, doubly periodic)
+    hipLaunchKernelGGL((k_fast_column<R, false, false, true>), grid, blk, 0, st, a, tx, tiles,
+                       nkc, part_out);
   } else {
-    hipLaunchKernelGGL((k_fast_column<R, false, false>), grid, blk, 0, st, a, tx, tiles, nkc,
-                       part_out);
+    hipLaunchKernelGGL((k_fast_column<R, false, false, false>), grid, blk, 0, st, a, tx, tiles,
+                       nkc, part_out);
   }
 }
 

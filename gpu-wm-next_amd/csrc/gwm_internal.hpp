@@ -58,10 +58,13 @@ __host__ __device__ inline int map_cell(int i, int n, int periodic) {
 }
 // Along the face-normal axis of a face field (n+1 owned faces, faces 0 and n
 // coincide under periodicity): ghost -1 <- face n-1, ghost n+1 <- face 1.
+// Indices further out (work-items past the end of a partial tile) wrap
+// modulo n, so every periodic index maps to an owned face.
 __host__ __device__ inline int map_face(int i, int n, int periodic) {
   if (i >= 0 && i <= n) return i;
   if (!periodic) return -1;
-  return i < 0 ? i + n : i - n;
+  const int m = i % n;
+  return m < 0 ? m + n : m;
 }
 
 // ---------------------------------------------------------------------------
@@ -79,6 +82,56 @@ __device__ inline float rexpm1(float a) { return expm1f(a); }
 __device__ inline double rexpm1(double a) { return expm1(a); }
 __device__ inline float rlog1p(float a) { return log1pf(a); }
 __device__ inline double rlog1p(double a) { return log1p(a); }
+
+// ---------------------------------------------------------------------------
+// Arithmetic of the time-step kernels (q*).  FP64 keeps correctly rounded
+// division and the library pow/exp/log1p/expm1, so the FP64 path matches the
+// FP64 oracle to ~1e-12.  FP32 uses the hardware reciprocal, base-2
+// logarithm and base-2 exponential (v_rcp_f32, v_log_f32, v_exp_f32, each
+// within 1 ulp): a power costs ~6 instructions instead of ~170 and a division
+// 2 instead of ~12, which is what the step's FP32 kernels are bound by.  Each
+// result stays within a few ulp of the correctly rounded value (bounds
+// measured on the device by tests/test_fp32_math.py).  Arguments of qpow and
+// qlog are positive normal numbers (the callers floor them).
+// ---------------------------------------------------------------------------
+__device__ inline float qrcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ inline double qrcp(double b) { return 1.0 / b; }
+__device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double qdiv(double a, double b) { return a / b; }
+__device__ inline float qpow(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+__device__ inline double qpow(double x, double y) { return pow(x, y); }
+__device__ inline float qexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ inline double qexp(double x) { return exp(x); }
+// log1p by Goldberg's correction: log(u) * x / (u - 1) with u = 1 + x
+// rounded, exact when u == 1 (accurate for |x| small, where 1 + x loses bits).
+__device__ inline float qlog1p(float x) {
+  const float u = 1.0f + x;
+  if (u == 1.0f) return x;
+  return (__builtin_amdgcn_logf(u) * 0.693147180559945309f) *
+         (x * __builtin_amdgcn_rcpf(u - 1.0f));
+}
+__device__ inline double qlog1p(double x) { return log1p(x); }
+// expm1 by Kahan's correction: (u - 1) * y / log(u) with u = exp(y) rounded.
+__device__ inline float qexpm1(float y) {
+  const float u = __builtin_amdgcn_exp2f(y * 1.44269504088896341f);
+  if (u == 1.0f) return y;
+  const float um1 = u - 1.0f;
+  if (um1 == -1.0f) return -1.0f;
+  return um1 * (y * __builtin_amdgcn_rcpf(__builtin_amdgcn_logf(u) * 0.693147180559945309f));
+}
+__device__ inline double qexpm1(double y) { return expm1(y); }
+// Division by a uniform grid step d with its reciprocal rd = 1 / d (correctly
+// rounded): FP32 multiplies, FP64 divides.
+__device__ inline float qdivc(float x, float d, float rd) {
+  (void)d;
+  return x * rd;
+}
+__device__ inline double qdivc(double x, double d, double rd) {
+  (void)rd;
+  return x / d;
+}
 
 // Physical constants in the working precision (dry_thermo.hpp:17-19,
 // moist_thermo.hpp:18-21).  FP32 uses the reference's float literals.
@@ -111,6 +164,30 @@ template <class R>
 __device__ inline R d_rho_from_p_theta(R p, R th) {  // dry_thermo.hpp:35-40
   const R ex = rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
   return p / rmax(Phys<R>::Rd * th * ex, R(1.0e-6));
+}
+// The same three functions in the step kernels' arithmetic (q* above); the
+// operator gwm_pressure keeps the library forms.
+template <class R>
+__device__ inline R q_theta(R rho, R rt) {
+  return qdiv(rt, rmax(rho, R(1.0e-6)));
+}
+template <class R>
+__device__ inline R q_pressure(R rho, R rt) {
+  const R th = q_theta(rho, rt);
+  const R ratio = rmax(qdiv(rho * Phys<R>::Rd * th, Phys<R>::p0), R(1.0e-6));
+  return Phys<R>::p0 * qpow(ratio, R(1) / (R(1) - Phys<R>::kappa));
+}
+template <class R>
+__device__ inline R q_rho_from_p_theta(R p, R th) {
+  const R ex = qpow(rmax(qdiv(p, Phys<R>::p0), R(1.0e-6)), Phys<R>::kappa);
+  return qdiv(p, rmax(Phys<R>::Rd * th * ex, R(1.0e-6)));
+}
+template <class R>
+__device__ inline R q_qsat(R T, R p) {
+  const R tc = T - R(273.15);
+  const R es = R(6.112) * qexp(qdiv(R(17.67) * tc, rmax(tc + R(243.5), R(1.0e-6)))) * R(100);
+  const R denom = rmax(p - (R(1) - Phys<R>::eps) * es, R(1000));
+  return rmin(rmax(qdiv(Phys<R>::eps * es, denom), R(0)), R(0.05));
 }
 template <class R>
 __device__ inline R d_cp() {
@@ -146,11 +223,11 @@ struct Base {
 template <class R>
 __device__ inline R p_pert(R rtp, R rtb, R pb) {
   const R gamma = R(1) / (R(1) - Phys<R>::kappa);
-  return pb * rexpm1(gamma * rlog1p(rtp / rtb));
+  return pb * qexpm1(gamma * qlog1p(qdiv(rtp, rtb)));
 }
 template <class R>
 __device__ inline R theta_pert(R rhop, R rtp, R rhob, R thb) {
-  return (rtp - thb * rhop) / rmax(rhob + rhop, R(1.0e-6));
+  return qdiv(rtp - thb * rhop, rmax(rhob + rhop, R(1.0e-6)));
 }
 
 // ---------------------------------------------------------------------------
@@ -207,6 +284,30 @@ struct Loader {
   __device__ int mi(int i) const { return g.per_x ? map_cell(i, g.nx, 1) : i; }
   __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
 };
+
+// Element offsets (level 0) of one staged position (ii, jj) of a tile: its
+// cell (and z-faces), x-faces ii and ii+1, y-faces jj and jj+1, with the 1x1
+// halo map applied once; -1 marks a value that is zero (open side).
+struct StagePos {
+  int c, u0, u1, v0, v1;
+};
+__device__ inline StagePos stage_pos(const Geo& g, int ii, int jj) {
+  const int ci = map_cell(ii, g.nx, g.per_x), cj = map_cell(jj, g.ny, g.per_y);
+  const int fi0 = map_face(ii, g.nx, g.per_x), fi1 = map_face(ii + 1, g.nx, g.per_x);
+  const int fj0 = map_face(jj, g.ny, g.per_y), fj1 = map_face(jj + 1, g.ny, g.per_y);
+  StagePos p;
+  p.c = (ci < 0 || cj < 0) ? -1 : (cj + g.h) * g.pxc + ci + g.h;
+  p.u0 = (fi0 < 0 || cj < 0) ? -1 : (cj + g.h) * g.pxu + fi0 + g.h;
+  p.u1 = (fi1 < 0 || cj < 0) ? -1 : (cj + g.h) * g.pxu + fi1 + g.h;
+  p.v0 = (ci < 0 || fj0 < 0) ? -1 : (fj0 + g.h) * g.pxc + ci + g.h;
+  p.v1 = (ci < 0 || fj1 < 0) ? -1 : (fj1 + g.h) * g.pxc + ci + g.h;
+  return p;
+}
+// Value at level-0 offset off of level k (level stride `plane`), 0 for off < 0.
+template <class R>
+__device__ inline R ld_at(const R* f, int off, unsigned plane, int k) {
+  return off < 0 ? R(0) : f[static_cast<unsigned>(off) + static_cast<unsigned>(k) * plane];
+}
 
 // Kernel launch record for optional per-kernel event timing.
 struct LaunchTimer;

@@ -14,28 +14,6 @@ namespace gwm {
 
 namespace {
 
-template <class R>
-__device__ inline R upw(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
-  return a >= R(0) ? a * ql : a * qr;
-}
-
-// dry_pressure_gradient.cu:37-73, with p supplied by a functor.
-template <class R, class PF>
-__device__ inline R vgrad(const PF& P, const R* zc, const Geo& g, int gi, int gj, int k, int li,
-                          int lj) {
-  if (g.nz == 1) return R(0);
-  if (k == 0) {
-    const R dz = zc[g.metric(gi, gj, 1)] - zc[g.metric(gi, gj, 0)];
-    return (P(li, lj, 1) - P(li, lj, 0)) / rmax(dz, R(1.0e-6));
-  }
-  if (k + 1 == g.nz) {
-    const R dz = zc[g.metric(gi, gj, k)] - zc[g.metric(gi, gj, k - 1)];
-    return (P(li, lj, k) - P(li, lj, k - 1)) / rmax(dz, R(1.0e-6));
-  }
-  const R dz = zc[g.metric(gi, gj, k + 1)] - zc[g.metric(gi, gj, k - 1)];
-  return (P(li, lj, k + 1) - P(li, lj, k - 1)) / rmax(dz, R(1.0e-6));
-}
-
 // Deterministic block sum of a double (256 threads).
 __device__ inline double block_sum(double v, double* sh) {
   const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -93,7 +71,7 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
         s0 += static_cast<double>(aux[c0] / rmax(r, R(1.0e-6)));
       } else {
         s0 += static_cast<double>(r);
-        s1 += static_cast<double>(d_pressure(r, r * aux[c0]));
+        s1 += static_cast<double>(q_pressure(r, r * aux[c0]));
       }
     }
   s0 = block_sum(s0, sh);
@@ -149,345 +127,6 @@ __global__ void k_pressure(Geo g, const R* rho, const R* rt, R* p) {
   p[a] = d_pressure(r, t);
 }
 
-// ===========================================================================
-// Slow stage: compute_slow_tendencies (dry_core.cu:400-539) = K2 + K16-K19 +
-// K21-K22 + K3, fused with the stage update / SSP blend (dry_core.cu:572-646,
-// K4/K5).  One thread per cell (i,j,k); it also owns x-face i, y-face j and
-// z-face k, plus face nx / ny / nz on the last index.
-//   mode 0: write tendencies;  mode 1: out = max(floor, S + dt*T);
-//   mode 2: out = wa*Q0 + wb*max(floor, S + dt*T).
-// With out_theta, the cell output is theta = (rho theta)_new / max(rho_new,
-// 1e-6) (the fast-mode entry value, dry_fast_modes.cu:86-95) instead of
-// rho theta.
-// ===========================================================================
-template <class R, bool TERRAIN>
-__global__ __launch_bounds__(256) void k_slow_stage(SlowArgs<R> a) {
-  const Geo g = a.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const Loader<R> L{g};
-  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
-  const DState<R> s = a.s;
-  const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
-
-  auto RHO = [&](int ii, int jj, int kk) { return L.cell(s.rho, ii, jj, kk); };
-  auto RT = [&](int ii, int jj, int kk) { return L.cell(s.rt, ii, jj, kk); };
-  auto TH = [&](int ii, int jj, int kk) { return d_theta(RHO(ii, jj, kk), RT(ii, jj, kk)); };
-  auto P = [&](int ii, int jj, int kk) { return d_pressure(RHO(ii, jj, kk), RT(ii, jj, kk)); };
-  auto U = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
-  auto V = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
-  auto W = [&](int ii, int jj, int kk) { return L.zf(s.w, ii, jj, kk); };
-  // K16 cell velocities (dry_momentum_flux.cu:36-73)
-  auto UC = [&](int ii, int jj, int kk) {
-    return R(0.5) * (U(ii, jj, kk) + U(ii + 1, jj, kk)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
-  };
-  auto VC = [&](int ii, int jj, int kk) {
-    return R(0.5) * (V(ii, jj, kk) + V(ii, jj + 1, kk)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
-  };
-  auto WC = [&](int ii, int jj, int kk) {
-    return R(0.5) * (W(ii, jj, kk) + W(ii, jj, kk + 1)) / rmax(RHO(ii, jj, kk), R(1.0e-6));
-  };
-  auto IDZC = [&](int ii, int jj, int kk) -> R {
-    if (TERRAIN) return a.m.idz_c[g.metric(L.mi(ii), L.mj(jj), kk)];
-    return a.m.lev_idz_c[kk];
-  };
-  auto IDZF = [&](int ii, int jj, int kk) -> R {
-    if (TERRAIN) return a.m.idz_f[g.metric(L.mi(ii), L.mj(jj), kk)];
-    return a.m.lev_idz_f[kk];
-  };
-
-  // ---- K2: continuity and upwind theta flux at cell (i,j,k) (dry_core.cu:62-144)
-  R t_rho, t_rt;
-  {
-    const R mw = U(i, j, k), me = U(i + 1, j, k);
-    const R ms = V(i, j, k), mn = V(i, j + 1, k);
-    const R mb = W(i, j, k), mt = W(i, j, k + 1);
-    const R idz = IDZC(i, j, k);
-    const R div = (me - mw) / dx + (mn - ms) / dy + (mt - mb) * idz;
-    t_rho = -div;
-    const R thc = TH(i, j, k);
-    const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
-    const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
-    const R thb = k > 0 ? TH(i, j, k - 1) : thc;
-    const R tht = k + 1 < g.nz ? TH(i, j, k + 1) : thc;
-    const R fw = mw * (mw >= R(0) ? thw : thc);
-    const R fe = me * (me >= R(0) ? thc : the);
-    const R fs = ms * (ms >= R(0) ? ths : thc);
-    const R fn = mn * (mn >= R(0) ? thc : thn);
-    const R fb = mb * (mb >= R(0) ? thb : thc);
-    const R ft = mt * (mt >= R(0) ? thc : tht);
-    t_rt = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-  }
-
-  // ---- x-face tendency at face fi (K17 + K21) ----
-  auto tend_u = [&](int fi) -> R {
-    if ((fi == 0 && ow) || (fi == g.nx && oe)) return R(0);
-    const R qw = U(fi - 1, j, k), qc = U(fi, j, k), qe = U(fi + 1, j, k);
-    const R qs = U(fi, j - 1, k), qn = U(fi, j + 1, k);
-    const R qb = k > 0 ? U(fi, j, k - 1) : qc;
-    const R qt = k + 1 < g.nz ? U(fi, j, k + 1) : qc;
-    const R fw = upw(UC(fi - 1, j, k), qw, qc);
-    const R fe = upw(UC(fi, j, k), qc, qe);
-    R fs = 0, fn = 0, fb = 0, ft = 0;
-    if (!(j == 0 && os)) fs = upw(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
-    if (!(j + 1 == g.ny && on)) fn = upw(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
-    if (k > 0) fb = upw(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
-    if (k + 1 < g.nz) ft = upw(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
-    const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
-    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-    // K21 (dry_pressure_gradient.cu:75-123)
-    const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
-    R term;
-    if (TERRAIN) {
-      const int gj = L.mj(j), gl = L.mi(fi - 1), gr = L.mi(fi);
-      const R zl = a.m.zc[g.metric(gl, gj, k)], zr = a.m.zc[g.metric(gr, gj, k)];
-      const R dzdx = (zr - zl) / dx;
-      const R dl = vgrad<R>(P, a.m.zc, g, gl, gj, k, fi - 1, j);
-      const R drr = vgrad<R>(P, a.m.zc, g, gr, gj, k, fi, j);
-      term = (pr - pl) / dx - dzdx * (R(0.5) * (dl + drr));
-    } else {
-      term = (pr - pl) / dx;  // dzdx == 0 exactly on flat terrain
-    }
-    return A + (-term);
-  };
-  // ---- y-face tendency at face fj (K18 + K22) ----
-  auto tend_v = [&](int fj) -> R {
-    if ((fj == 0 && os) || (fj == g.ny && on)) return R(0);
-    const R qw = V(i - 1, fj, k), qc = V(i, fj, k), qe = V(i + 1, fj, k);
-    const R qs = V(i, fj - 1, k), qn = V(i, fj + 1, k);
-    const R qb = k > 0 ? V(i, fj, k - 1) : qc;
-    const R qt = k + 1 < g.nz ? V(i, fj, k + 1) : qc;
-    R fw = 0, fe = 0, fb = 0, ft = 0;
-    if (!(i == 0 && ow)) fw = upw(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
-    if (!(i + 1 == g.nx && oe)) fe = upw(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
-    const R fs = upw(VC(i, fj - 1, k), qs, qc);
-    const R fn = upw(VC(i, fj, k), qc, qn);
-    if (k > 0) fb = upw(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
-    if (k + 1 < g.nz) ft = upw(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
-    const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
-    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-    const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
-    R term;
-    if (TERRAIN) {
-      const int gi = L.mi(i), gs = L.mj(fj - 1), gn = L.mj(fj);
-      const R zs = a.m.zc[g.metric(gi, gs, k)], zn = a.m.zc[g.metric(gi, gn, k)];
-      const R dzdy = (zn - zs) / dy;
-      const R ds = vgrad<R>(P, a.m.zc, g, gi, gs, k, i, fj - 1);
-      const R dn = vgrad<R>(P, a.m.zc, g, gi, gn, k, i, fj);
-      term = (pn - ps) / dy - dzdy * (R(0.5) * (ds + dn));
-    } else {
-      term = (pn - ps) / dy;
-    }
-    return A + (-term);
-  };
-  // ---- z-face tendency at interior face kf (K19 + K3) ----
-  auto tend_w = [&](int kf) -> R {
-    if (kf == 0 || kf == g.nz) return R(0);
-    const int kl = kf - 1, ku = kf;
-    const R qw = W(i - 1, j, kf), qc = W(i, j, kf), qe = W(i + 1, j, kf);
-    const R qs = W(i, j - 1, kf), qn = W(i, j + 1, kf);
-    const R qb = W(i, j, kf - 1), qt = W(i, j, kf + 1);
-    R fw = 0, fe = 0, fs = 0, fn = 0;
-    if (!(i == 0 && ow)) fw = upw(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
-    if (!(i + 1 == g.nx && oe)) fe = upw(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
-    if (!(j == 0 && os)) fs = upw(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
-    if (!(j + 1 == g.ny && on)) fn = upw(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
-    const R fb = upw(WC(i, j, kl), qb, qc);
-    const R ft = upw(WC(i, j, ku), qc, qt);
-    const R idz = IDZF(i, j, kf);
-    const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-    // K3 buoyancy (dry_core.cu:146-187)
-    const R rl = RHO(i, j, kl), ru = RHO(i, j, ku);
-    const R tl = RT(i, j, kl) / rmax(rl, R(1.0e-6));
-    const R tu = RT(i, j, ku) / rmax(ru, R(1.0e-6));
-    const R tf = R(0.5) * (tl + tu);
-    const R trf = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
-    const R rf = R(0.5) * (rl + ru);
-    const R b = a.gravity * (tf - trf) / rmax(trf, R(1.0e-6));
-    return A + rf * b;
-  };
-
-  const R dt = a.dt;
-  auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
-  auto fin = [&](R q0v, R v) { return a.mode == 2 ? a.wa * q0v + a.wb * v : v; };
-
-  const size_t ci = g.cell(i, j, k);
-  const R tu0 = tend_u(i), tv0 = tend_v(j), tw0 = tend_w(k);
-  if (a.mode == 0) {
-    a.out.rho[ci] = t_rho;
-    a.out.rt[ci] = t_rt;
-    a.out.u[g.xf(i, j, k)] = tu0;
-    a.out.v[g.yf(i, j, k)] = tv0;
-    a.out.w[g.zf(i, j, k)] = tw0;
-    if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx);
-    if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny);
-    if (k == g.nz - 1) a.out.w[g.zf(i, j, g.nz)] = R(0);
-    return;
-  }
-  const R fl = R(1.0e-6), mf = Phys<R>::mom_floor;
-  const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(s.rho[ci], t_rho, fl));
-  const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(s.rt[ci], t_rt, fl));
-  a.out.rho[ci] = rho_new;
-  a.out.rt[ci] = a.out_theta ? rt_new / rmax(rho_new, R(1.0e-6)) : rt_new;
-  {
-    const size_t f = g.xf(i, j, k);
-    a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tu0, mf));
-  }
-  {
-    const size_t f = g.yf(i, j, k);
-    a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tv0, mf));
-  }
-  {
-    const size_t f = g.zf(i, j, k);
-    a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], tw0, mf));
-  }
-  if (i == g.nx - 1) {
-    const size_t f = g.xf(g.nx, j, k);
-    a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx), mf));
-  }
-  if (j == g.ny - 1) {
-    const size_t f = g.yf(i, g.ny, k);
-    a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny), mf));
-  }
-  if (k == g.nz - 1) {
-    const size_t f = g.zf(i, j, g.nz);
-    a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], R(0), mf));
-  }
-}
-
-// ===========================================================================
-// Column hydrostatic reference (K11, dry_fast_modes.cu:108-158), terrain
-// path.  One thread per column; writes rho_ref / p_ref as (k*ny+j)*nx+i.
-// ===========================================================================
-template <class R>
-__global__ void k_column_reference(Geo g, const R* rho, const R* theta, const R* zc, R gravity,
-                                   R* rref, R* pref) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= g.nx || j >= g.ny) return;
-  // K10 then K20 at level 0: rt_fast = rho*theta
-  R r0 = rho[g.cell(i, j, 0)];
-  R rt0 = r0 * theta[g.cell(i, j, 0)];
-  const R th0 = d_theta(r0, rt0);
-  R pk = d_pressure(r0, rt0);
-  R rk = d_rho_from_p_theta(pk, th0);
-  pref[g.metric(i, j, 0)] = pk;
-  rref[g.metric(i, j, 0)] = rk;
-  for (int k = 1; k < g.nz; ++k) {
-    const R r = rho[g.cell(i, j, k)];
-    const R thk = d_theta(r, r * theta[g.cell(i, j, k)]);
-    const R dz = zc[g.metric(i, j, k)] - zc[g.metric(i, j, k - 1)];
-    R pn = rmax(R(1), pk - gravity * rk * dz);
-    for (int it = 0; it < 4; ++it) {
-      const R rn = d_rho_from_p_theta(pn, thk);
-      pn = pk - gravity * R(0.5) * (rk + rn) * dz;
-      pn = rmax(pn, R(1));
-    }
-    pk = pn;
-    rk = d_rho_from_p_theta(pn, thk);
-    pref[g.metric(i, j, k)] = pk;
-    rref[g.metric(i, j, k)] = rk;
-  }
-}
-
-// ===========================================================================
-// Fast substep: one substep of apply_local_split_explicit_fast_modes
-// (dry_fast_modes.cu:495-652): K10+K20 pressure from rho and the fixed theta,
-// K12/K13 horizontal pressure gradient, K14 vertical perturbation branch,
-// face synchronisation (identity under the wrap-face invariant), K15 density
-// from the divergence of (m - m_base), K10 rho*theta rebuild (last substep).
-// One thread per cell; u/v/w of the cell's upper faces are recomputed
-// redundantly so that rho needs no second pass.  In -> out ping-pong.
-// ===========================================================================
-template <class R, bool TERRAIN>
-__global__ __launch_bounds__(256) void k_fast_substep(FastArgs<R> a) {
-  const Geo g = a.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const Loader<R> L{g};
-  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
-  const R dtf = a.dtf, grav = a.gravity;
-  const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
-
-  auto P = [&](int ii, int jj, int kk) -> R {
-    const R r = L.cell(a.rho, ii, jj, kk);
-    return d_pressure(r, r * L.cell(a.theta, ii, jj, kk));
-  };
-  const R pc = P(i, j, k);
-  auto pgf_x = [&](int fi, R pl, R pr) -> R {
-    if (!TERRAIN) return (pr - pl) / dx;
-    const int gj = L.mj(j), gl = L.mi(fi - 1), gr = L.mi(fi);
-    const R zl = a.m.zc[g.metric(gl, gj, k)], zr = a.m.zc[g.metric(gr, gj, k)];
-    const R dzdx = (zr - zl) / dx;
-    const R dl = vgrad<R>(P, a.m.zc, g, gl, gj, k, fi - 1, j);
-    const R drr = vgrad<R>(P, a.m.zc, g, gr, gj, k, fi, j);
-    return (pr - pl) / dx - dzdx * (R(0.5) * (dl + drr));
-  };
-  auto pgf_y = [&](int fj, R ps, R pn) -> R {
-    if (!TERRAIN) return (pn - ps) / dy;
-    const int gi = L.mi(i), gs = L.mj(fj - 1), gn = L.mj(fj);
-    const R zs = a.m.zc[g.metric(gi, gs, k)], zn = a.m.zc[g.metric(gi, gn, k)];
-    const R dzdy = (zn - zs) / dy;
-    const R ds = vgrad<R>(P, a.m.zc, g, gi, gs, k, i, fj - 1);
-    const R dn = vgrad<R>(P, a.m.zc, g, gi, gn, k, i, fj);
-    return (pn - ps) / dy - dzdy * (R(0.5) * (ds + dn));
-  };
-  auto new_u = [&](int fi, R pl, R pr) -> R {
-    const R cur = a.u[g.xf(fi, j, k)];
-    if (!a.horizontal || (fi == 0 && ow) || (fi == g.nx && oe)) return cur;
-    return cur + (-dtf * pgf_x(fi, pl, pr));
-  };
-  auto new_v = [&](int fj, R ps, R pn) -> R {
-    const R cur = a.v[g.yf(i, fj, k)];
-    if (!a.horizontal || (fj == 0 && os) || (fj == g.ny && on)) return cur;
-    return cur + (-dtf * pgf_y(fj, ps, pn));
-  };
-  auto RREF = [&](int kk) -> R { return a.col_ref ? a.rref3[g.metric(i, j, kk)] : a.rref_lev[kk]; };
-  auto PREF = [&](int kk) -> R { return a.col_ref ? a.pref3[g.metric(i, j, kk)] : a.pref_lev[kk]; };
-  auto new_w = [&](int kf, R plo, R pup) -> R {
-    const R cur = a.w[g.zf(i, j, kf)];
-    if (kf == 0 || kf == g.nz) return cur;
-    const R ppl = plo - PREF(kf - 1);
-    const R ppu = pup - PREF(kf);
-    const R rpf = R(0.5) * ((a.rho[g.cell(i, j, kf - 1)] - RREF(kf - 1)) +
-                            (a.rho[g.cell(i, j, kf)] - RREF(kf)));
-    const R idz = TERRAIN ? a.m.idz_f[g.metric(i, j, kf)] : a.m.lev_idz_f[kf];
-    const R tend = -(ppu - ppl) * idz - grav * rpf;
-    return cur + dtf * tend;
-  };
-
-  const R pw = P(i - 1, j, k), pe = P(i + 1, j, k);
-  const R ps = P(i, j - 1, k), pn = P(i, j + 1, k);
-  const R u0 = new_u(i, pw, pc), u1 = new_u(i + 1, pc, pe);
-  const R v0 = new_v(j, ps, pc), v1 = new_v(j + 1, pc, pn);
-  const R pb = k > 0 ? P(i, j, k - 1) : R(0);
-  const R pt = k + 1 < g.nz ? P(i, j, k + 1) : R(0);
-  const R w0 = new_w(k, pb, pc), w1 = new_w(k + 1, pc, pt);
-
-  const size_t ci = g.cell(i, j, k);
-  R rho_new = a.rho[ci];
-  if (a.density) {
-    const R idz = TERRAIN ? a.m.idz_c[g.metric(i, j, k)] : a.m.lev_idz_c[k];
-    const R div = ((u1 - a.ub[g.xf(i + 1, j, k)]) - (u0 - a.ub[g.xf(i, j, k)])) / dx +
-                  ((v1 - a.vb[g.yf(i, j + 1, k)]) - (v0 - a.vb[g.yf(i, j, k)])) / dy +
-                  ((w1 - a.wb[g.zf(i, j, k + 1)]) - (w0 - a.wb[g.zf(i, j, k)])) * idz;
-    rho_new = rmax(R(1.0e-6), rho_new - dtf * div);
-  }
-  a.rho_out[ci] = rho_new;
-  if (a.rt_out) a.rt_out[ci] = rho_new * a.theta[ci];
-  a.u_out[g.xf(i, j, k)] = u0;
-  a.v_out[g.yf(i, j, k)] = v0;
-  a.w_out[g.zf(i, j, k)] = w0;
-  if (i == g.nx - 1) a.u_out[g.xf(g.nx, j, k)] = u1;
-  if (j == g.ny - 1) a.v_out[g.yf(i, g.ny, k)] = v1;
-  if (k == g.nz - 1) a.w_out[g.zf(i, j, g.nz)] = w1;
-}
-
-// theta = rho_theta / max(rho, 1e-6) on owned cells (K9, dry_fast_modes.cu:86-95)
 template <class R>
 __global__ void k_theta(Geo g, const R* rho, const R* rt, R* th, Base<R> base) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,7 +134,7 @@ __global__ void k_theta(Geo g, const R* rho, const R* rt, R* th, Base<R> base) {
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const size_t c = g.cell(i, j, k);
-  th[c] = base.rho ? theta_pert(rho[c], rt[c], base.rho[k], base.th[k]) : d_theta(rho[c], rt[c]);
+  th[c] = base.rho ? theta_pert(rho[c], rt[c], base.rho[k], base.th[k]) : q_theta(rho[c], rt[c]);
 }
 
 template <class R>
@@ -506,50 +145,6 @@ __global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt, Base<R> ba
   if (i >= g.nx || j >= g.ny) return;
   const size_t c = g.cell(i, j, k);
   rt[c] = base.rho ? base.rho[k] * th[c] + rho[c] * (base.th[k] + th[c]) : rho[c] * th[c];
-}
-
-// ===========================================================================
-// Passive tracers, one SSPRK3 stage for all tracers (K23,
-// dry_transport.cu:49-125 + blend 40-47) with frozen rho and face momenta.
-// mode 1: out = max(0, in - dt div);  mode 2: out = wa*q0 + wb*(...).
-// ===========================================================================
-template <class R, bool TERRAIN>
-__global__ __launch_bounds__(256) void k_tracer_stage(TracerArgs<R> a) {
-  const Geo g = a.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const Loader<R> L{g};
-  const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
-  const R rc = a.rho[g.cell(i, j, k)];
-  const R rw = L.cell(a.rho, i - 1, j, k), re = L.cell(a.rho, i + 1, j, k);
-  const R rs = L.cell(a.rho, i, j - 1, k), rn = L.cell(a.rho, i, j + 1, k);
-  const R rb = k > 0 ? a.rho[g.cell(i, j, k - 1)] : R(1);
-  const R rt_ = k + 1 < g.nz ? a.rho[g.cell(i, j, k + 1)] : R(1);
-  const R mw = a.u[g.xf(i, j, k)], me = a.u[g.xf(i + 1, j, k)];
-  const R ms = a.v[g.yf(i, j, k)], mn = a.v[g.yf(i, j + 1, k)];
-  const R mb = a.w[g.zf(i, j, k)], mt = a.w[g.zf(i, j, k + 1)];
-  const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k];
-  const size_t ci = g.cell(i, j, k);
-  for (int t = 0; t < a.ntr; ++t) {
-    const R* qi = a.in[t];
-    auto q = [&](R rq, R r) { return rq / rmax(r, R(1.0e-6)); };
-    const R qc = q(qi[ci], rc);
-    const R qw = q(L.cell(qi, i - 1, j, k), rw), qe = q(L.cell(qi, i + 1, j, k), re);
-    const R qs = q(L.cell(qi, i, j - 1, k), rs), qn = q(L.cell(qi, i, j + 1, k), rn);
-    const R qb = k > 0 ? q(qi[g.cell(i, j, k - 1)], rb) : qc;
-    const R qt = k + 1 < g.nz ? q(qi[g.cell(i, j, k + 1)], rt_) : qc;
-    const R fw = mw * (mw >= R(0) ? qw : qc);
-    const R fe = me * (me >= R(0) ? qc : qe);
-    const R fs = ms * (ms >= R(0) ? qs : qc);
-    const R fn = mn * (mn >= R(0) ? qc : qn);
-    const R fb = mb * (mb >= R(0) ? qb : qc);
-    const R ft = mt * (mt >= R(0) ? qc : qt);
-    const R div = (fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz;
-    const R v = rmax(R(0), qi[ci] - a.dt * div);
-    a.out[t][ci] = a.mode == 2 ? a.wa * a.q0[t][ci] + a.wb * v : v;
-  }
 }
 
 // ===========================================================================
@@ -891,6 +486,22 @@ __global__ void k_metrics(Geo g, const double* zif, const double* zc, const doub
 }
 
 // Copy of owned + halo storage (whole buffer).
+__global__ void k_fp32_math(int fn, const float* x, float* y, size_t n) {
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float v = x[t];
+  float r;
+  switch (fn) {
+    case 0: r = qpow(v, 1.0f / (1.0f - Phys<float>::kappa)); break;
+    case 1: r = qpow(v, Phys<float>::kappa); break;
+    case 2: r = qexp(v); break;
+    case 3: r = qlog1p(v); break;
+    case 4: r = qexpm1(v); break;
+    default: r = qrcp(v); break;
+  }
+  y[t] = r;
+}
+
 template <class R>
 __global__ void k_copy(const R* src, R* dst, size_t n) {
   const size_t a = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -933,29 +544,6 @@ void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R*
                      rho, rt, p);
 }
 template <class R>
-void launch_slow_stage(hipStream_t st, const SlowArgs<R>& a, bool terrain) {
-  const dim3 b(64, 4, 1);
-  if (terrain)
-    hipLaunchKernelGGL((k_slow_stage<R, true>), cell_grid(a.g, b), b, 0, st, a);
-  else
-    hipLaunchKernelGGL((k_slow_stage<R, false>), cell_grid(a.g, b), b, 0, st, a);
-}
-template <class R>
-void launch_column_reference(hipStream_t st, const Geo& g, const R* rho, const R* theta,
-                             const R* zc, R gravity, R* rref, R* pref) {
-  const dim3 b(64, 4, 1);
-  hipLaunchKernelGGL(k_column_reference<R>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), b, 0, st, g,
-                     rho, theta, zc, gravity, rref, pref);
-}
-template <class R>
-void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain) {
-  const dim3 b(64, 4, 1);
-  if (terrain)
-    hipLaunchKernelGGL((k_fast_substep<R, true>), cell_grid(a.g, b), b, 0, st, a);
-  else
-    hipLaunchKernelGGL((k_fast_substep<R, false>), cell_grid(a.g, b), b, 0, st, a);
-}
-template <class R>
 void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th, Base<R> base) {
   const dim3 b(64, 4, 1);
   hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th, base);
@@ -965,14 +553,6 @@ void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, 
                        Base<R> base) {
   const dim3 b(64, 4, 1);
   hipLaunchKernelGGL(k_rebuild_rt<R>, cell_grid(g, b), b, 0, st, g, rho, th, rt, base);
-}
-template <class R>
-void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain) {
-  const dim3 b(64, 4, 1);
-  if (terrain)
-    hipLaunchKernelGGL((k_tracer_stage<R, true>), cell_grid(a.g, b), b, 0, st, a);
-  else
-    hipLaunchKernelGGL((k_tracer_stage<R, false>), cell_grid(a.g, b), b, 0, st, a);
 }
 template <class R>
 void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain) {
@@ -1019,6 +599,9 @@ void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const doubl
   hipLaunchKernelGGL(k_metrics<R>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), b, 0, st, g, zif, zc,
                      wif, wc, terrain, zif_m, zc_m, idz_c, idz_f, sx, sy, terr);
 }
+void launch_fp32_math(hipStream_t st, int fn, const float* x, float* y, size_t n) {
+  hipLaunchKernelGGL(k_fp32_math, dim3(blocks_for(n, 256)), dim3(256), 0, st, fn, x, y, n);
+}
 template <class R>
 void launch_copy(hipStream_t st, const R* src, R* dst, size_t n) {
   hipLaunchKernelGGL(k_copy<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, n);
@@ -1034,12 +617,7 @@ void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
   template void launch_level_finalize<R>(hipStream_t, int, int, const double*, int, double, R*,   \
                                          R*);                                                     \
   template void launch_pressure<R>(hipStream_t, const Geo&, const R*, const R*, R*);             \
-  template void launch_slow_stage<R>(hipStream_t, const SlowArgs<R>&, bool);                     \
-  template void launch_column_reference<R>(hipStream_t, const Geo&, const R*, const R*, const R*, \
-                                           R, R*, R*);                                            \
-  template void launch_fast_substep<R>(hipStream_t, const FastArgs<R>&, bool);                   \
   template void launch_theta<R>(hipStream_t, const Geo&, const R*, const R*, R*, Base<R>);       \
-  template void launch_tracer_stage<R>(hipStream_t, const TracerArgs<R>&, bool);                 \
   template void launch_rebuild_rt<R>(hipStream_t, const Geo&, const R*, const R*, R*, Base<R>);  \
   template void launch_warm_rain<R>(hipStream_t, const WarmRainArgs<R>&, bool);                  \
   template void launch_halo_fill<R>(hipStream_t, const Geo&, int, R*);                           \

@@ -31,7 +31,6 @@ struct FastArgs {
   R *rho_out, *u_out, *v_out, *w_out;
   R* rt_out;  // rho*theta rebuild (last substep), may be null
   const R *rref_lev, *pref_lev;  // flat reference (nz)
-  const R *rref3, *pref3;        // column reference ((k*ny+j)*nx+i)
   DMetrics<R> m;
   R dtf, gravity;
   int horizontal, density, col_ref;
@@ -92,16 +91,13 @@ void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* par
                            double count, R* o0, R* o1);
 template <class R>
 void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p);
-template <class R>
-void launch_slow_stage(hipStream_t st, const SlowArgs<R>& a, bool terrain);
-// Column-marching slow stage (gwm_slow.hip), same modes as launch_slow_stage.
+// The step kernels' FP32 helpers elementwise (fn: GWM_MATH_* of gwm_hip.h).
+void launch_fp32_math(hipStream_t st, int fn, const float* x, float* y, size_t n);
+// Column-marching slow stage (gwm_slow.hip): compute_slow_tendencies fused
+// with the stage update / SSP blend; mode 0 writes tendencies, 1 updates,
+// 2 updates and blends (SlowArgs).
 template <class R>
 void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int nkc);
-template <class R>
-void launch_column_reference(hipStream_t st, const Geo& g, const R* rho, const R* theta,
-                             const R* zc, R gravity, R* rref, R* pref);
-template <class R>
-void launch_fast_substep(hipStream_t st, const FastArgs<R>& a, bool terrain);
 // Column-marching fast substep (gwm_fast.hip).  With part_out (flat refs),
 // also writes the next substep's per-wave level partials [k][wave][2] and
 // returns their count per level in *nparts.
@@ -113,10 +109,8 @@ int fast_column_parts(int nx, int ny);
 template <class R>
 void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th,
                   Base<R> base = Base<R>{});
-template <class R>
-void launch_tracer_stage(hipStream_t st, const TracerArgs<R>& a, bool terrain);
-// Column-marching tracer stage (gwm_slow.hip), same arguments as
-// launch_tracer_stage.
+// Column-marching tracer stage (gwm_slow.hip): one SSPRK3 stage of every
+// tracer with frozen rho and face momenta (mode 1 update, 2 update + blend).
 template <class R>
 void launch_tracer_column(hipStream_t st, const TracerArgs<R>& a, bool terrain, int nkc);
 template <class R>

@@ -8,6 +8,7 @@
 // function, warm_rain.cu:57-58, 71-72), autoconversion, the accretion
 // extension (rate 0 reproduces the reference), then rain evaporation
 // against the re-derived saturation ratio, and clips the mixing ratios at 0.
+// Arithmetic is the step kernels' (q* in gwm_internal.hpp).
 #pragma once
 
 #include "gwm_internal.hpp"
@@ -33,14 +34,14 @@ template <class R>
 __device__ inline MicroCell<R> micro_prepare(R rho_raw, R rt0, R rqv, R rqc, R rqr) {
   MicroCell<R> m;
   m.rho = rmax(rho_raw, R(1.0e-6));
-  m.th = rt0 / m.rho;
-  m.p = d_pressure(m.rho, rt0);
-  m.T = d_theta(m.rho, rt0) * rpow(rmax(m.p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
-  const R ex = rmax(m.T / rmax(m.th, R(1.0e-6)), R(1.0e-6));
-  m.qv = rmax(rqv / m.rho, R(0));
-  m.qc = rmax(rqc / m.rho, R(0));
-  m.qr = rmax(rqr / m.rho, R(0));
-  m.lfac = Phys<R>::Lv / (d_cp<R>() * rmax(ex, R(1.0e-6)));
+  m.th = qdiv(rt0, m.rho);
+  m.p = q_pressure(m.rho, rt0);
+  m.T = q_theta(m.rho, rt0) * qpow(rmax(qdiv(m.p, Phys<R>::p0), R(1.0e-6)), Phys<R>::kappa);
+  const R ex = rmax(qdiv(m.T, rmax(m.th, R(1.0e-6))), R(1.0e-6));
+  m.qv = rmax(qdiv(rqv, m.rho), R(0));
+  m.qc = rmax(qdiv(rqc, m.rho), R(0));
+  m.qr = rmax(qdiv(rqr, m.rho), R(0));
+  m.lfac = qdiv(Phys<R>::Lv, d_cp<R>() * rmax(ex, R(1.0e-6)));
   return m;
 }
 
@@ -49,7 +50,7 @@ __device__ inline void micro_apply(MicroCell<R>& m, const MicroCfg<R>& c) {
   const R rho = m.rho;
   R th = m.th, qv = m.qv, qc = m.qc, qr = m.qr;
   const R lfac = m.lfac;
-  R qs = d_qsat(m.T, m.p);
+  R qs = q_qsat(m.T, m.p);
   if (qv > qs) {
     const R cnd = rmin((qv - qs) * rmax(c.relax, R(0)), qv);
     qv -= cnd;
@@ -72,9 +73,9 @@ __device__ inline void micro_apply(MicroCell<R>& m, const MicroCfg<R>& c) {
     qc -= ac;
     qr += ac;
   }
-  const R p = d_pressure(rho, rho * th);
-  const R T = th * rpow(rmax(p / Phys<R>::p0, R(1.0e-6)), Phys<R>::kappa);
-  qs = d_qsat(T, p);
+  const R p = q_pressure(rho, rho * th);
+  const R T = th * qpow(rmax(qdiv(p, Phys<R>::p0), R(1.0e-6)), Phys<R>::kappa);
+  qs = q_qsat(T, p);
   if (qr > R(0) && qv < qs && c.erate > R(0)) {
     const R e = rmin(qr, c.dt * c.erate * rmin(qs - qv, qr));
     qr -= e;

@@ -3,9 +3,13 @@
 // compute_slow_tendencies (src/dycore/dry_core.cu:400-539) = K2 continuity and
 // upwind theta flux + K16-K19 momentum flux divergence + K21-K22 horizontal
 // pressure gradient + K3 buoyancy, fused with the stage update / SSP blend of
-// advance_dry_state_ssprk3 (dry_core.cu:572-646).  Modes as in the point
-// kernel k_slow_stage (gwm_kernels.hip): 0 tendencies, 1 update, 2 update +
-// blend; out_theta writes theta instead of rho*theta.
+// advance_dry_state_ssprk3 (dry_core.cu:572-646, K4/K5).  Modes: 0 writes
+// tendencies; 1 out = max(floor, S + dt*T); 2 out = wa*Q0 + wb*max(floor,
+// S + dt*T).  With out_theta the cell output is theta = (rho theta)_new /
+// max(rho_new, 1e-6), the fast-mode entry value (dry_fast_modes.cu:86-95).
+// A thread owns cell (i,j,k), x-face i, y-face j and z-face k, plus face
+// nx / ny / nz on the last index.  Arithmetic is the step kernels' q* forms
+// (gwm_internal.hpp).
 //
 // A work-group of 64 x 4 threads marches a 64 x 4 column tile upwards.  For
 // each level it stages, over the tile plus a one-cell ring, the derived cell
@@ -17,7 +21,10 @@
 // The column is split into level chunks (count chosen by the autotuner) so
 // the chip holds more independent work-groups; a chunk stages the level
 // below its first one.
-// Raw face momenta for the upwind values are read from global memory.
+// FP32 stages the raw face momenta of each level in LDS as well, so the
+// upwind values of every flux come from LDS; FP64 reads them from global
+// memory.  Global offsets of the staged positions are computed once per
+// work-group (stage_pos), so the halo map leaves the level loop.
 //
 // The tracer stage below follows the same pattern.
 #include "gwm_kernels.hpp"
@@ -46,12 +53,19 @@ __device__ inline R upwind(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
   return a >= R(0) ? a * ql : a * qr;
 }
 
+
 }  // namespace
 
-template <class R, bool TERRAIN>
+template <class R, bool TERRAIN, bool PERT>
 __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
                                                      int nkc) {
+  // FP32 also stages the raw face momenta of each level (tile plus ring), so
+  // the upwind values of every flux come from LDS; FP64 reads them from
+  // global memory (its LDS budget would halve the resident work-groups).
+  constexpr bool RAW = sizeof(R) == 4;
+  constexpr int RS = RAW ? 4 : 1, RN = RAW ? PN : 1;
   __shared__ R sTH[4][PN], sP[4][PN], sUC[4][PN], sVC[4][PN], sWC[4][PN];
+  __shared__ R sU[RS][RN], sV[RS][RN], sW[RS][RN];
   const Geo g = a.g;
   const Loader<R> L{g};
   const DState<R> s = a.s;
@@ -76,12 +90,23 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
   }
 
   const int nz = g.nz;
-  const bool pert = a.base.rho != nullptr;
-  auto stage_one = [&](int ex, int ey, int kk) {
-    const int ii = ti0 + ex, jj = tj0 + ey;
-    const int e = (ex + 1) + PX * (ey + 1);
-    const R r = L.cell(s.rho, ii, jj, kk);
-    const R rt = L.cell(s.rt, ii, jj, kk);
+  constexpr bool pert = PERT;
+  const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;  // cell / z-face level stride
+  const unsigned pu = static_cast<unsigned>(g.pxu) * g.pyc;  // x-face level stride
+  const unsigned pv = static_cast<unsigned>(g.pxc) * g.pyv;  // y-face level stride
+  const StagePos own = stage_pos(g, i, j);
+  const StagePos ring = stage_pos(g, ti0 + hx, tj0 + hy);
+  const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
+
+  // Stages level kk at LDS entry e; returns the raw rho, rho*theta and the
+  // w face above the cell.
+  auto stage_one = [&](const StagePos& o, int e, int kk, R& r, R& rt, R& w1) {
+    r = ld_at(s.rho, o.c, pc, kk);
+    rt = ld_at(s.rt, o.c, pc, kk);
+    const R u0 = ld_at(s.u, o.u0, pu, kk), u1 = ld_at(s.u, o.u1, pu, kk);
+    const R v0 = ld_at(s.v, o.v0, pv, kk), v1 = ld_at(s.v, o.v1, pv, kk);
+    const R w0 = ld_at(s.w, o.c, pc, kk);
+    w1 = ld_at(s.w, o.c, pc, kk + 1);
     R rr;
     if (pert) {  // theta' and p' of the departures
       const R rb = a.base.rho[kk];
@@ -90,247 +115,270 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
       sP[kk & 3][e] = p_pert(rt, a.base.rt[kk], a.base.p[kk]);
     } else {
       rr = rmax(r, R(1.0e-6));
-      sTH[kk & 3][e] = d_theta(r, rt);
-      sP[kk & 3][e] = d_pressure(r, rt);
+      sTH[kk & 3][e] = q_theta(r, rt);
+      sP[kk & 3][e] = q_pressure(r, rt);
     }
-    sUC[kk & 3][e] = R(0.5) * (L.xf(s.u, ii, jj, kk) + L.xf(s.u, ii + 1, jj, kk)) / rr;
-    sVC[kk & 3][e] = R(0.5) * (L.yf(s.v, ii, jj, kk) + L.yf(s.v, ii, jj + 1, kk)) / rr;
-    sWC[kk & 3][e] = R(0.5) * (L.zf(s.w, ii, jj, kk) + L.zf(s.w, ii, jj, kk + 1)) / rr;
+    sUC[kk & 3][e] = qdiv(R(0.5) * (u0 + u1), rr);
+    sVC[kk & 3][e] = qdiv(R(0.5) * (v0 + v1), rr);
+    sWC[kk & 3][e] = qdiv(R(0.5) * (w0 + w1), rr);
+    if constexpr (RAW) {
+      sU[kk & 3][e] = u0;
+      sV[kk & 3][e] = v0;
+      sW[kk & 3][e] = w0;
+    }
   };
+  // Own-column rho (levels k-1, k, k+1), rho*theta (k, k+1) and the top w face.
+  R r_km1 = R(0), r_k = R(0), r_k1 = R(0), rt_k = R(0), rt_k1 = R(0), w_top = R(0);
   auto stage = [&](int kk) {
-    stage_one(lx, ly, kk);
-    if (t < NHALO) stage_one(hx, hy, kk);
+    R r, rt, w1;
+    stage_one(own, eo, kk, r, rt, w1);
+    r_k1 = r;
+    rt_k1 = rt;
+    if (kk + 1 == nz) w_top = w1;
+    if (t < NHALO) stage_one(ring, eh, kk, r, rt, w1);
   };
 
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R rdx = R(1) / dx, rdy = R(1) / dy;
   const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
   const size_t plane_m = static_cast<size_t>(g.nx) * g.ny;
   auto mcol = [&](int ii, int jj) -> size_t {
     return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);
   };
 
-  if (k0 > 0) stage(k0 - 1);
+  if (k0 > 0) {
+    stage(k0 - 1);
+    r_km1 = r_k1;
+  }
   stage(k0);
+  r_k = r_k1;
+  rt_k = rt_k1;
   for (int k = k0; k < k1; ++k) {
     if (k + 1 < nz) stage(k + 1);
     __syncthreads();
-    if (!active) continue;
+    if (active) {
+      // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
+      auto at = [&](int ii, int jj) { return (ii - ti0 + 1) + PX * (jj - tj0 + 1); };
+      auto TH = [&](int ii, int jj, int kk) { return sTH[kk & 3][at(ii, jj)]; };
+      auto P = [&](int ii, int jj, int kk) { return sP[kk & 3][at(ii, jj)]; };
+      auto UC = [&](int ii, int jj, int kk) { return sUC[kk & 3][at(ii, jj)]; };
+      auto VC = [&](int ii, int jj, int kk) { return sVC[kk & 3][at(ii, jj)]; };
+      auto WC = [&](int ii, int jj, int kk) { return sWC[kk & 3][at(ii, jj)]; };
+      // Raw face momenta: staged (FP32) or global; Ug / Vg for the domain-edge
+      // faces nx / ny, whose outer neighbour lies beyond the ring.
+      auto Ug = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
+      auto Vg = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
+      auto U = [&](int ii, int jj, int kk) -> R {
+        if constexpr (RAW) return sU[kk & 3][at(ii, jj)];
+        else return L.xf(s.u, ii, jj, kk);
+      };
+      auto V = [&](int ii, int jj, int kk) -> R {
+        if constexpr (RAW) return sV[kk & 3][at(ii, jj)];
+        else return L.yf(s.v, ii, jj, kk);
+      };
+      auto W = [&](int ii, int jj, int kk) -> R {  // z-face kk < nz
+        if constexpr (RAW) return sW[kk & 3][at(ii, jj)];
+        else return L.zf(s.w, ii, jj, kk);
+      };
+      const R w_up = k + 1 < nz ? W(i, j, k + 1) : w_top;  // z-face k+1 of the own cell
+      auto IDZC = [&](int ii, int jj, int kk) -> R {
+        if (TERRAIN) return a.m.idz_c[mcol(ii, jj) + kk * plane_m];
+        return a.m.lev_idz_c[kk];
+      };
+      auto IDZF = [&](int ii, int jj, int kk) -> R {
+        if (TERRAIN) return a.m.idz_f[mcol(ii, jj) + kk * plane_m];
+        return a.m.lev_idz_f[kk];
+      };
+      // dp/dz at column (ii, jj) (dry_pressure_gradient.cu:37-73).
+      auto vgrad = [&](int ii, int jj) -> R {
+        if (nz == 1) return R(0);
+        const R* zc = a.m.zc;
+        const size_t m = mcol(ii, jj);
+        if (k == 0) {
+          const R dz = zc[m + plane_m] - zc[m];
+          return qdiv(P(ii, jj, 1) - P(ii, jj, 0), rmax(dz, R(1.0e-6)));
+        }
+        if (k + 1 == nz) {
+          const R dz = zc[m + k * plane_m] - zc[m + (k - 1) * plane_m];
+          return qdiv(P(ii, jj, k) - P(ii, jj, k - 1), rmax(dz, R(1.0e-6)));
+        }
+        const R dz = zc[m + (k + 1) * plane_m] - zc[m + (k - 1) * plane_m];
+        return qdiv(P(ii, jj, k + 1) - P(ii, jj, k - 1), rmax(dz, R(1.0e-6)));
+      };
 
-    // LDS views of the staged quantities at (ii, jj, kk), kk in k-1 .. k+1.
-    auto at = [&](int ii, int jj) { return (ii - ti0 + 1) + PX * (jj - tj0 + 1); };
-    auto TH = [&](int ii, int jj, int kk) { return sTH[kk & 3][at(ii, jj)]; };
-    auto P = [&](int ii, int jj, int kk) { return sP[kk & 3][at(ii, jj)]; };
-    auto UC = [&](int ii, int jj, int kk) { return sUC[kk & 3][at(ii, jj)]; };
-    auto VC = [&](int ii, int jj, int kk) { return sVC[kk & 3][at(ii, jj)]; };
-    auto WC = [&](int ii, int jj, int kk) { return sWC[kk & 3][at(ii, jj)]; };
-    auto U = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
-    auto V = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
-    auto W = [&](int ii, int jj, int kk) { return L.zf(s.w, ii, jj, kk); };
-    auto IDZC = [&](int ii, int jj, int kk) -> R {
-      if (TERRAIN) return a.m.idz_c[mcol(ii, jj) + kk * plane_m];
-      return a.m.lev_idz_c[kk];
-    };
-    auto IDZF = [&](int ii, int jj, int kk) -> R {
-      if (TERRAIN) return a.m.idz_f[mcol(ii, jj) + kk * plane_m];
-      return a.m.lev_idz_f[kk];
-    };
-    // dp/dz at column (ii, jj) (dry_pressure_gradient.cu:37-73).
-    auto vgrad = [&](int ii, int jj) -> R {
-      if (nz == 1) return R(0);
-      const R* zc = a.m.zc;
-      const size_t m = mcol(ii, jj);
-      if (k == 0) {
-        const R dz = zc[m + plane_m] - zc[m];
-        return (P(ii, jj, 1) - P(ii, jj, 0)) / rmax(dz, R(1.0e-6));
+      // ---- K2 (dry_core.cu:62-144)
+      R t_rho, t_rt;
+      {
+        const R mw = U(i, j, k), me = U(i + 1, j, k);
+        const R ms = V(i, j, k), mn = V(i, j + 1, k);
+        const R mb = W(i, j, k), mt = w_up;
+        const R idz = IDZC(i, j, k);
+        const R div = qdivc(me - mw, dx, rdx) + qdivc(mn - ms, dy, rdy) + (mt - mb) * idz;
+        t_rho = -div;
+        const R thc = TH(i, j, k);
+        const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
+        const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
+        const R thb = k > 0 ? TH(i, j, k - 1) : thc;
+        const R tht = k + 1 < nz ? TH(i, j, k + 1) : thc;
+        const R fw = mw * (mw >= R(0) ? thw : thc);
+        const R fe = me * (me >= R(0) ? thc : the);
+        const R fs = ms * (ms >= R(0) ? ths : thc);
+        const R fn = mn * (mn >= R(0) ? thc : thn);
+        const R fb = mb * (mb >= R(0) ? thb : thc);
+        const R ft = mt * (mt >= R(0) ? thc : tht);
+        if (pert) {
+          // theta = theta_b(level) + theta': the base parts of the horizontal
+          // fluxes share theta_b(k); the vertical ones take the upwind level's.
+          const R tb = a.base.th[k];
+          const R tbb = (mb >= R(0) && k > 0) ? a.base.th[k - 1] : tb;
+          const R tbt = (mt < R(0) && k + 1 < nz) ? a.base.th[k + 1] : tb;
+          t_rt = -(qdivc(tb * (me - mw) + (fe - fw), dx, rdx) +
+                   qdivc(tb * (mn - ms) + (fn - fs), dy, rdy) +
+                   ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
+        } else {
+          t_rt = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
+        }
       }
-      if (k + 1 == nz) {
-        const R dz = zc[m + k * plane_m] - zc[m + (k - 1) * plane_m];
-        return (P(ii, jj, k) - P(ii, jj, k - 1)) / rmax(dz, R(1.0e-6));
-      }
-      const R dz = zc[m + (k + 1) * plane_m] - zc[m + (k - 1) * plane_m];
-      return (P(ii, jj, k + 1) - P(ii, jj, k - 1)) / rmax(dz, R(1.0e-6));
-    };
 
-    // ---- K2 (dry_core.cu:62-144)
-    R t_rho, t_rt;
-    {
-      const R mw = U(i, j, k), me = U(i + 1, j, k);
-      const R ms = V(i, j, k), mn = V(i, j + 1, k);
-      const R mb = W(i, j, k), mt = W(i, j, k + 1);
-      const R idz = IDZC(i, j, k);
-      const R div = (me - mw) / dx + (mn - ms) / dy + (mt - mb) * idz;
-      t_rho = -div;
-      const R thc = TH(i, j, k);
-      const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
-      const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
-      const R thb = k > 0 ? TH(i, j, k - 1) : thc;
-      const R tht = k + 1 < nz ? TH(i, j, k + 1) : thc;
-      const R fw = mw * (mw >= R(0) ? thw : thc);
-      const R fe = me * (me >= R(0) ? thc : the);
-      const R fs = ms * (ms >= R(0) ? ths : thc);
-      const R fn = mn * (mn >= R(0) ? thc : thn);
-      const R fb = mb * (mb >= R(0) ? thb : thc);
-      const R ft = mt * (mt >= R(0) ? thc : tht);
-      if (pert) {
-        // theta = theta_b(level) + theta': the base parts of the horizontal
-        // fluxes share theta_b(k); the vertical ones take the upwind level's.
-        const R tb = a.base.th[k];
-        const R tbb = (mb >= R(0) && k > 0) ? a.base.th[k - 1] : tb;
-        const R tbt = (mt < R(0) && k + 1 < nz) ? a.base.th[k + 1] : tb;
-        t_rt = -((tb * (me - mw) + (fe - fw)) / dx + (tb * (mn - ms) + (fn - fs)) / dy +
-                 ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
-      } else {
-        t_rt = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-      }
-    }
-
-    // ---- x-face (K17 + K21)
-    auto tend_u = [&](int fi) -> R {
-      if ((fi == 0 && ow) || (fi == g.nx && oe)) return R(0);
-      const R qw = U(fi - 1, j, k), qc = U(fi, j, k), qe = U(fi + 1, j, k);
-      const R qs = U(fi, j - 1, k), qn = U(fi, j + 1, k);
-      const R qb = k > 0 ? U(fi, j, k - 1) : qc;
-      const R qt = k + 1 < nz ? U(fi, j, k + 1) : qc;
-      const R fw = upwind(UC(fi - 1, j, k), qw, qc);
-      const R fe = upwind(UC(fi, j, k), qc, qe);
-      R fs = 0, fn = 0, fb = 0, ft = 0;
-      if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
-      if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
-      if (k > 0) fb = upwind(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
-      if (k + 1 < nz) ft = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
-      const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
-      const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-      const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
-      R term;
-      if (TERRAIN) {
-        const R zl = a.m.zc[mcol(fi - 1, j) + k * plane_m];
-        const R zr = a.m.zc[mcol(fi, j) + k * plane_m];
-        const R dzdx = (zr - zl) / dx;
-        term = (pr - pl) / dx - dzdx * (R(0.5) * (vgrad(fi - 1, j) + vgrad(fi, j)));
-      } else {
-        term = (pr - pl) / dx;
-      }
-      return A + (-term);
-    };
-    // ---- y-face (K18 + K22)
-    auto tend_v = [&](int fj) -> R {
-      if ((fj == 0 && os) || (fj == g.ny && on)) return R(0);
-      const R qw = V(i - 1, fj, k), qc = V(i, fj, k), qe = V(i + 1, fj, k);
-      const R qs = V(i, fj - 1, k), qn = V(i, fj + 1, k);
-      const R qb = k > 0 ? V(i, fj, k - 1) : qc;
-      const R qt = k + 1 < nz ? V(i, fj, k + 1) : qc;
-      R fw = 0, fe = 0, fb = 0, ft = 0;
-      if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
-      if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
-      const R fs = upwind(VC(i, fj - 1, k), qs, qc);
-      const R fn = upwind(VC(i, fj, k), qc, qn);
-      if (k > 0) fb = upwind(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
-      if (k + 1 < nz) ft = upwind(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
-      const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
-      const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-      const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
-      R term;
-      if (TERRAIN) {
-        const R zs = a.m.zc[mcol(i, fj - 1) + k * plane_m];
-        const R zn = a.m.zc[mcol(i, fj) + k * plane_m];
-        const R dzdy = (zn - zs) / dy;
-        term = (pn - ps) / dy - dzdy * (R(0.5) * (vgrad(i, fj - 1) + vgrad(i, fj)));
-      } else {
-        term = (pn - ps) / dy;
-      }
-      return A + (-term);
-    };
-    // ---- z-face at interior face k (K19 + K3)
-    auto tend_w = [&]() -> R {
-      if (k == 0) return R(0);
-      const int kl = k - 1, ku = k;
-      const R qw = W(i - 1, j, k), qc = W(i, j, k), qe = W(i + 1, j, k);
-      const R qs = W(i, j - 1, k), qn = W(i, j + 1, k);
-      const R qb = W(i, j, k - 1), qt = W(i, j, k + 1);
-      R fw = 0, fe = 0, fs = 0, fn = 0;
-      if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
-      if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
-      if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
-      if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
-      const R fb = upwind(WC(i, j, kl), qb, qc);
-      const R ft = upwind(WC(i, j, ku), qc, qt);
-      const R idz = IDZF(i, j, k);
-      const R A = -((fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz);
-      if (pert) {
-        // theta_ref holds the level means of theta'; the bases cancel in
-        // theta_f - theta_ref,f
-        const R rl = L.cell(s.rho, i, j, kl) + a.base.rho[kl];
-        const R ru = L.cell(s.rho, i, j, ku) + a.base.rho[ku];
+      // ---- x-face (K17 + K21)
+      auto tend_u = [&](int fi, const auto& Uf) -> R {
+        if ((fi == 0 && ow) || (fi == g.nx && oe)) return R(0);
+        const R qw = Uf(fi - 1, j, k), qc = Uf(fi, j, k), qe = Uf(fi + 1, j, k);
+        const R qs = Uf(fi, j - 1, k), qn = Uf(fi, j + 1, k);
+        const R qb = k > 0 ? Uf(fi, j, k - 1) : qc;
+        const R qt = k + 1 < nz ? Uf(fi, j, k + 1) : qc;
+        const R fw = upwind(UC(fi - 1, j, k), qw, qc);
+        const R fe = upwind(UC(fi, j, k), qc, qe);
+        R fs = 0, fn = 0, fb = 0, ft = 0;
+        if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
+        if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
+        if (k > 0) fb = upwind(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
+        if (k + 1 < nz) ft = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
+        const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
+        const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
+        const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
+        R term;
+        if (TERRAIN) {
+          const R zl = a.m.zc[mcol(fi - 1, j) + k * plane_m];
+          const R zr = a.m.zc[mcol(fi, j) + k * plane_m];
+          const R dzdx = qdivc(zr - zl, dx, rdx);
+          term = qdivc(pr - pl, dx, rdx) - dzdx * (R(0.5) * (vgrad(fi - 1, j) + vgrad(fi, j)));
+        } else {
+          term = qdivc(pr - pl, dx, rdx);
+        }
+        return A + (-term);
+      };
+      // ---- y-face (K18 + K22)
+      auto tend_v = [&](int fj, const auto& Vf) -> R {
+        if ((fj == 0 && os) || (fj == g.ny && on)) return R(0);
+        const R qw = Vf(i - 1, fj, k), qc = Vf(i, fj, k), qe = Vf(i + 1, fj, k);
+        const R qs = Vf(i, fj - 1, k), qn = Vf(i, fj + 1, k);
+        const R qb = k > 0 ? Vf(i, fj, k - 1) : qc;
+        const R qt = k + 1 < nz ? Vf(i, fj, k + 1) : qc;
+        R fw = 0, fe = 0, fb = 0, ft = 0;
+        if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
+        if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
+        const R fs = upwind(VC(i, fj - 1, k), qs, qc);
+        const R fn = upwind(VC(i, fj, k), qc, qn);
+        if (k > 0) fb = upwind(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
+        if (k + 1 < nz) ft = upwind(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
+        const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
+        const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
+        const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
+        R term;
+        if (TERRAIN) {
+          const R zs = a.m.zc[mcol(i, fj - 1) + k * plane_m];
+          const R zn = a.m.zc[mcol(i, fj) + k * plane_m];
+          const R dzdy = qdivc(zn - zs, dy, rdy);
+          term = qdivc(pn - ps, dy, rdy) - dzdy * (R(0.5) * (vgrad(i, fj - 1) + vgrad(i, fj)));
+        } else {
+          term = qdivc(pn - ps, dy, rdy);
+        }
+        return A + (-term);
+      };
+      // ---- z-face at interior face k (K19 + K3)
+      auto tend_w = [&]() -> R {
+        if (k == 0) return R(0);
+        const int kl = k - 1, ku = k;
+        const R qw = W(i - 1, j, k), qc = W(i, j, k), qe = W(i + 1, j, k);
+        const R qs = W(i, j - 1, k), qn = W(i, j + 1, k);
+        const R qb = W(i, j, k - 1), qt = w_up;
+        R fw = 0, fe = 0, fs = 0, fn = 0;
+        if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
+        if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
+        if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
+        if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
+        const R fb = upwind(WC(i, j, kl), qb, qc);
+        const R ft = upwind(WC(i, j, ku), qc, qt);
+        const R idz = IDZF(i, j, k);
+        const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
         const R tf = R(0.5) * (TH(i, j, kl) + TH(i, j, ku));
-        const R trp = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
-        const R trf = R(0.5) * (a.base.th[kl] + a.base.th[ku]) + trp;
-        const R rf = R(0.5) * (rl + ru);
-        const R bb = a.gravity * (tf - trp) / rmax(trf, R(1.0e-6));
+        if (pert) {
+          // theta_ref holds the level means of theta'; the bases cancel in
+          // theta_f - theta_ref,f
+          const R rl = r_km1 + a.base.rho[kl];
+          const R ru = r_k + a.base.rho[ku];
+          const R trp = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
+          const R trf = R(0.5) * (a.base.th[kl] + a.base.th[ku]) + trp;
+          const R rf = R(0.5) * (rl + ru);
+          const R bb = qdiv(a.gravity * (tf - trp), rmax(trf, R(1.0e-6)));
+          return A + rf * bb;
+        }
+        const R trf = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
+        const R rf = R(0.5) * (r_km1 + r_k);
+        const R bb = qdiv(a.gravity * (tf - trf), rmax(trf, R(1.0e-6)));
         return A + rf * bb;
+      };
+
+      const R dt = a.dt;
+      auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
+      auto fin = [&](R q0v, R v) { return a.mode == 2 ? a.wa * q0v + a.wb * v : v; };
+
+      const size_t ci = g.cell(i, j, k);
+      const size_t xf = g.xf(i, j, k), yf = g.yf(i, j, k);
+      const R tu0 = tend_u(i, U), tv0 = tend_v(j, V), tw0 = tend_w();
+      if (a.mode == 0) {
+        a.out.rho[ci] = t_rho;
+        a.out.rt[ci] = t_rt;
+        a.out.u[xf] = tu0;
+        a.out.v[yf] = tv0;
+        a.out.w[ci] = tw0;
+        if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx, Ug);
+        if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny, Vg);
+        if (k == nz - 1) a.out.w[g.zf(i, j, nz)] = R(0);
+      } else {
+        const R mf = Phys<R>::mom_floor;
+        // floors on the full values (departures: 1e-6 - base)
+        const R flr = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
+        const R flt = pert ? R(1.0e-6) - a.base.rt[k] : R(1.0e-6);
+        const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(r_k, t_rho, flr));
+        const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(rt_k, t_rt, flt));
+        a.out.rho[ci] = rho_new;
+        if (pert)
+          a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, a.base.rho[k], a.base.th[k])
+                                     : rt_new;
+        else
+          a.out.rt[ci] = a.out_theta ? qdiv(rt_new, rmax(rho_new, R(1.0e-6))) : rt_new;
+        a.out.u[xf] = fin(a.mode == 2 ? a.q0.u[xf] : R(0), upd(U(i, j, k), tu0, mf));
+        a.out.v[yf] = fin(a.mode == 2 ? a.q0.v[yf] : R(0), upd(V(i, j, k), tv0, mf));
+        a.out.w[ci] = fin(a.mode == 2 ? a.q0.w[ci] : R(0), upd(W(i, j, k), tw0, mf));
+        if (i == g.nx - 1) {
+          const size_t f = g.xf(g.nx, j, k);
+          a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx, Ug), mf));
+        }
+        if (j == g.ny - 1) {
+          const size_t f = g.yf(i, g.ny, k);
+          a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny, Vg), mf));
+        }
+        if (k == nz - 1) {
+          const size_t f = g.zf(i, j, nz);
+          a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(w_top, R(0), mf));
+        }
       }
-      const R rl = L.cell(s.rho, i, j, kl), ru = L.cell(s.rho, i, j, ku);
-      const R tl = L.cell(s.rt, i, j, kl) / rmax(rl, R(1.0e-6));
-      const R tu = L.cell(s.rt, i, j, ku) / rmax(ru, R(1.0e-6));
-      const R tf = R(0.5) * (tl + tu);
-      const R trf = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
-      const R rf = R(0.5) * (rl + ru);
-      const R bb = a.gravity * (tf - trf) / rmax(trf, R(1.0e-6));
-      return A + rf * bb;
-    };
-
-    const R dt = a.dt;
-    auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
-    auto fin = [&](R q0v, R v) { return a.mode == 2 ? a.wa * q0v + a.wb * v : v; };
-
-    const size_t ci = g.cell(i, j, k);
-    const R tu0 = tend_u(i), tv0 = tend_v(j), tw0 = tend_w();
-    if (a.mode == 0) {
-      a.out.rho[ci] = t_rho;
-      a.out.rt[ci] = t_rt;
-      a.out.u[g.xf(i, j, k)] = tu0;
-      a.out.v[g.yf(i, j, k)] = tv0;
-      a.out.w[g.zf(i, j, k)] = tw0;
-      if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx);
-      if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny);
-      if (k == nz - 1) a.out.w[g.zf(i, j, nz)] = R(0);
-      continue;
     }
-    const R mf = Phys<R>::mom_floor;
-    // floors on the full values (departures: 1e-6 - base)
-    const R flr = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
-    const R flt = pert ? R(1.0e-6) - a.base.rt[k] : R(1.0e-6);
-    const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(s.rho[ci], t_rho, flr));
-    const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(s.rt[ci], t_rt, flt));
-    a.out.rho[ci] = rho_new;
-    if (pert)
-      a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, a.base.rho[k], a.base.th[k])
-                                 : rt_new;
-    else
-      a.out.rt[ci] = a.out_theta ? rt_new / rmax(rho_new, R(1.0e-6)) : rt_new;
-    {
-      const size_t f = g.xf(i, j, k);
-      a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tu0, mf));
-    }
-    {
-      const size_t f = g.yf(i, j, k);
-      a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tv0, mf));
-    }
-    {
-      const size_t f = g.zf(i, j, k);
-      a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], tw0, mf));
-    }
-    if (i == g.nx - 1) {
-      const size_t f = g.xf(g.nx, j, k);
-      a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx), mf));
-    }
-    if (j == g.ny - 1) {
-      const size_t f = g.yf(i, g.ny, k);
-      a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny), mf));
-    }
-    if (k == nz - 1) {
-      const size_t f = g.zf(i, j, nz);
-      a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(s.w[f], R(0), mf));
-    }
+    r_km1 = r_k;
+    r_k = r_k1;
+    rt_k = rt_k1;
   }
 }
 
@@ -341,9 +389,11 @@ void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int 
   nkc = nkc < 1 ? 1 : (nkc > a.g.nz ? a.g.nz : nkc);
   const dim3 blk(TX, TY, 1), grid(tiles * nkc);
   if (terrain)
-    hipLaunchKernelGGL((k_slow_column<R, true>), grid, blk, 0, st, a, tx, tiles, nkc);
+    hipLaunchKernelGGL((k_slow_column<R, true, false>), grid, blk, 0, st, a, tx, tiles, nkc);
+  else if (a.base.rho)  // perturbation form (flat, doubly periodic)
+    hipLaunchKernelGGL((k_slow_column<R, false, true>), grid, blk, 0, st, a, tx, tiles, nkc);
   else
-    hipLaunchKernelGGL((k_slow_column<R, false>), grid, blk, 0, st, a, tx, tiles, nkc);
+    hipLaunchKernelGGL((k_slow_column<R, false, false>), grid, blk, 0, st, a, tx, tiles, nkc);
 }
 
 template void launch_slow_column<float>(hipStream_t, const SlowArgs<float>&, bool, int);
@@ -355,14 +405,14 @@ template void launch_slow_column<double>(hipStream_t, const SlowArgs<double>&, b
 // upwind flux form with the positivity clip).  The mixing ratios
 // q = (rho q) / max(rho, 1e-6) of every tracer are staged per level over
 // the tile plus ring into LDS once, instead of being divided out again at
-// each of the seven uses.  Same arithmetic per value as k_tracer_stage.
+// each of the seven uses.  mode 1: out = max(0, in - dt div); mode 2:
+// out = wa*q0 + wb*(...) (dry_transport.cu:40-47).
 // ===========================================================================
 template <class R, bool TERRAIN, int NT>
 __global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
                                                        int nkc) {
   __shared__ R sQ[NT][4][PN];
   const Geo g = a.g;
-  const Loader<R> L{g};
   int b, k0, k1;
   const int tile = tile_of(tiles, nkc, g.nz, b, k0, k1);
   (void)b;
@@ -381,19 +431,21 @@ __global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tile
     hy = q >> 1;
   }
   const int nz = g.nz;
-  auto stage_one = [&](int ex, int ey, int kk) {
-    const int ii = ti0 + ex, jj = tj0 + ey;
-    const int e = (ex + 1) + PX * (ey + 1);
-    const R r0 = L.cell(a.rho, ii, jj, kk);
+  const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;
+  const int oc = stage_pos(g, i, j).c, hc = stage_pos(g, ti0 + hx, tj0 + hy).c;
+  const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
+  auto stage_one = [&](int off, int e, int kk) {
+    const R r0 = ld_at(a.rho, off, pc, kk);
     const R r = rmax(a.base_rho ? r0 + a.base_rho[kk] : r0, R(1.0e-6));
 #pragma unroll
-    for (int n = 0; n < NT; ++n) sQ[n][kk & 3][e] = L.cell(a.in[n], ii, jj, kk) / r;
+    for (int n = 0; n < NT; ++n) sQ[n][kk & 3][e] = qdiv(ld_at(a.in[n], off, pc, kk), r);
   };
   auto stage = [&](int kk) {
-    stage_one(lx, ly, kk);
-    if (t < NHALO) stage_one(hx, hy, kk);
+    stage_one(oc, eo, kk);
+    if (t < NHALO) stage_one(hc, eh, kk);
   };
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
+  const R rdx = R(1) / dx, rdy = R(1) / dy;
   const int c0 = (lx + 1) + PX * (ly + 1);
   if (k0 > 0) stage(k0 - 1);
   stage(k0);
@@ -419,7 +471,7 @@ __global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tile
       const R fn = mn * (mn >= R(0) ? qc : qn);
       const R fb = mb * (mb >= R(0) ? qb : qc);
       const R ft = mt * (mt >= R(0) ? qc : qt);
-      const R div = (fe - fw) / dx + (fn - fs) / dy + (ft - fb) * idz;
+      const R div = qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz;
       const R v = rmax(R(0), a.in[n][ci] - a.dt * div);
       a.out[n][ci] = a.mode == 2 ? a.wa * a.q0[n][ci] + a.wb * v : v;
     }

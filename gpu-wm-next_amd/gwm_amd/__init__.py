@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libgwm_hip.so")
 HEADER = os.path.join(REPO, "include", "gwm_hip.h")
 
 F32, F64 = 0, 1
+# gwm_fp32_math functions (GWM_MATH_*)
+MATH_POW_EOS, MATH_POW_EXNER, MATH_EXP, MATH_LOG1P, MATH_EXPM1, MATH_RCP = range(6)
 RHO, RHO_THETA, MOM_U, MOM_V, MOM_W, PRESSURE = 0, 1, 2, 3, 4, 5
 TEND_RHO, TEND_RHO_THETA, TEND_U, TEND_V, TEND_W = 6, 7, 8, 9, 10
 PRECIP, TRACER0 = 11, 16
@@ -139,6 +141,7 @@ def lib():
         L.gwm_state_is_perturbation.argtypes = [vp]
         L.gwm_state_download_f64.argtypes = [vp, ip, vp]
         L.gwm_ctx_tuning_report.argtypes = [vp, cp, ctypes.c_size_t]
+        L.gwm_fp32_math.argtypes = [vp, ip, vp, vp, ctypes.c_long]
         L.gwm_grid_create.argtypes = [vp, ctypes.POINTER(GridDesc), vp, vp, pvp]
         L.gwm_grid_destroy.argtypes = [vp]
         L.gwm_grid_download_metric.argtypes = [vp, ip, vp]
@@ -265,6 +268,15 @@ class Context:
     def set_chunks(self, chunks):
         """Fix the level-chunk count of the column kernels (0: autotune)."""
         check(lib().gwm_ctx_set_chunks(self.h, int(chunks)))
+
+    def fp32_math(self, fn, x):
+        """The step kernels' FP32 helper `fn` (MATH_*) applied to x on the device."""
+        x = np.ascontiguousarray(x, np.float32)
+        dx = DeviceArray(self, x.shape)
+        dy = DeviceArray(self, x.shape)
+        dx.upload(x)
+        check(lib().gwm_fp32_math(self.h, int(fn), dx.ptr, dy.ptr, x.size))
+        return dy.download()
 
     def tuning_report(self):
         """{launch shape key: chunk count} chosen by the autotuner."""

@@ -178,6 +178,25 @@ int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks);
 /* "kernel:NXxNYxNZ:prec:variant=chunks;..." for every tuned launch shape. */
 int gwm_ctx_tuning_report(gwm_ctx* ctx, char* buf, size_t size);
 
+/* Diagnostic, no counterpart in the reference: evaluates one of the FP32
+ * arithmetic helpers of the step kernels elementwise on the device, so their
+ * accuracy can be measured (dev_y[i] = f(dev_x[i]), n floats each):
+ *   GWM_MATH_POW_EOS    x^(1/(1-kappa))  (equation of state)
+ *   GWM_MATH_POW_EXNER  x^kappa          (Exner function)
+ *   GWM_MATH_EXP        exp(x)
+ *   GWM_MATH_LOG1P      log(1+x)
+ *   GWM_MATH_EXPM1      exp(x)-1
+ *   GWM_MATH_RCP        1/x                                              */
+enum {
+  GWM_MATH_POW_EOS = 0,
+  GWM_MATH_POW_EXNER = 1,
+  GWM_MATH_EXP = 2,
+  GWM_MATH_LOG1P = 3,
+  GWM_MATH_EXPM1 = 4,
+  GWM_MATH_RCP = 5
+};
+int gwm_fp32_math(gwm_ctx* ctx, int fn, const void* dev_x, void* dev_y, long n);
+
 /* ---- grid / metrics ------------------------------------------------------
  * Replaces GridMetrics::make_hybrid_height (grid_metrics.hpp:87-239) and
  * build_rectilinear_domain (src/domain/idealized_domain_builder.cpp:39-64).
