@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
           s1 += static_cast<double>(p_pert(rb * tp + r * (thb + tp), rtb, pb));
         }
       } else if (mode == 0) {
-        s0 += static_cast<double>(aux[c0] / rmax(r, R(1.0e-6)));
+        s0 += static_cast<double>(q_theta(r, aux[c0]));
       } else {
         s0 += static_cast<double>(r);
         s1 += static_cast<double>(q_pressure(r, r * aux[c0]));
@@ -164,28 +164,32 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
   const bool nominal = a.fallout == 1, metric = a.fallout == 2;
   const R dz_nom = rmax(a.depth, R(1.0e-3));
   const R frac_nom = rmin(rmax(a.vt * dt / dz_nom, R(0)), R(1));
+  const R rdz_nom = R(1) / dz_nom;
   const MicroCfg<R> mc{dt, a.relax, a.thr, a.krate, a.erate, a.kacc, a.heat};
-  // The loads of the level below are issued before the current level's
-  // arithmetic, so the march overlaps memory latency with compute.
+  // The loads of the level three below are issued before the current
+  // level's arithmetic: three levels of loads in flight per column keep
+  // enough requests outstanding at one column per thread.
   const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
   size_t c = g.cell(i, j, g.nz - 1);
-  R n_rho = a.rho[c], n_rt = a.rt[c], n_qv = a.qv[c], n_qc = a.qc[c], n_qr = a.qr[c];
+  struct Lv {
+    R rho, rt, qv, qc, qr;
+  };
+  auto load = [&](size_t cc) -> Lv { return {a.rho[cc], a.rt[cc], a.qv[cc], a.qc[cc], a.qr[cc]}; };
+  Lv l1 = load(c), l2{}, l3{};
+  if (g.nz > 1) l2 = load(c - plane);
+  if (g.nz > 2) l3 = load(c - 2 * plane);
   const bool pert = a.base.rho != nullptr;
   for (int k = g.nz - 1; k >= 0; --k) {
+    const Lv cur = l1;
+    l1 = l2;
+    l2 = l3;
+    if (k >= 3) l3 = load(c - 3 * plane);
     // Perturbation form: full rho and rho*theta for the microphysics; the
     // stored departure of rho*theta then changes by rho * dtheta only.
-    const R rtp = n_rt;
-    const R rho = rmax(pert ? n_rho + a.base.rho[k] : n_rho, R(1.0e-6));
-    const R rt0 = pert ? n_rt + a.base.rt[k] : n_rt;
-    const R in_qv = n_qv, in_qc = n_qc, in_qr = n_qr;
-    if (k > 0) {
-      const size_t cn = c - plane;
-      n_rho = a.rho[cn];
-      n_rt = a.rt[cn];
-      n_qv = a.qv[cn];
-      n_qc = a.qc[cn];
-      n_qr = a.qr[cn];
-    }
+    const R rtp = cur.rt;
+    const R rho = rmax(pert ? cur.rho + a.base.rho[k] : cur.rho, R(1.0e-6));
+    const R rt0 = pert ? cur.rt + a.base.rt[k] : cur.rt;
+    const R in_qv = cur.qv, in_qc = cur.qc, in_qr = cur.qr;
     MicroCell<R> m = micro_prepare(rho, rt0, in_qv, in_qc, in_qr);
     const R th0 = m.th;
     micro_apply(m, mc);
@@ -197,11 +201,11 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
     if (nominal) {
       const R old = rmax(rqr, R(0)) * dz_nom;
       const R out = frac_nom * old;
-      rqr = (old - out + inc) / dz_nom;
+      rqr = qdivc(old - out + inc, dz_nom, rdz_nom);
       inc = out;
     } else if (metric) {
       const R idz = rmax(TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k], R(1.0e-6));
-      const R dz = R(1) / idz;
+      const R dz = qrcp(idz);
       const R frac = rmin(rmax(a.vt * dt * idz, R(0)), R(1));
       const R old = rmax(rqr, R(0)) * dz;
       const R out = frac * old;

@@ -434,47 +434,83 @@ __global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tile
   const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;
   const int oc = stage_pos(g, i, j).c, hc = stage_pos(g, ti0 + hx, tj0 + hy).c;
   const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
-  auto stage_one = [&](int off, int e, int kk) {
+  // Own raw rho*q of the staged level (kept for the update) and the ring.
+  R own_n[NT];
+  auto stage_one = [&](int off, int e, int kk, R* raw) {
     const R r0 = ld_at(a.rho, off, pc, kk);
     const R r = rmax(a.base_rho ? r0 + a.base_rho[kk] : r0, R(1.0e-6));
 #pragma unroll
-    for (int n = 0; n < NT; ++n) sQ[n][kk & 3][e] = qdiv(ld_at(a.in[n], off, pc, kk), r);
+    for (int n = 0; n < NT; ++n) {
+      raw[n] = ld_at(a.in[n], off, pc, kk);
+      sQ[n][kk & 3][e] = qdiv(raw[n], r);
+    }
   };
   auto stage = [&](int kk) {
-    stage_one(oc, eo, kk);
-    if (t < NHALO) stage_one(hc, eh, kk);
+    stage_one(oc, eo, kk, own_n);
+    if (t < NHALO) {
+      R ring_raw[NT];
+      stage_one(hc, eh, kk, ring_raw);
+    }
   };
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
   const R rdx = R(1) / dx, rdy = R(1) / dy;
   const int c0 = (lx + 1) + PX * (ly + 1);
+  // Face momenta (and SSP bases) of one level for the own cell, loaded a
+  // level ahead; the w face above a cell is the next level's lower face.
+  R f_mw = 0, f_me = 0, f_ms = 0, f_mn = 0, f_mt = 0, f_q0[NT] = {};
+  auto load_faces = [&](int k) {
+    f_mw = a.u[g.xf(i, j, k)];
+    f_me = a.u[g.xf(i + 1, j, k)];
+    f_ms = a.v[g.yf(i, j, k)];
+    f_mn = a.v[g.yf(i, j + 1, k)];
+    f_mt = a.w[g.zf(i, j, k + 1)];
+    if (a.mode == 2) {
+#pragma unroll
+      for (int n = 0; n < NT; ++n) f_q0[n] = a.q0[n][g.cell(i, j, k)];
+    }
+  };
+  R own[NT];
   if (k0 > 0) stage(k0 - 1);
   stage(k0);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) own[n] = own_n[n];
+  R mb = R(0);
+  if (active) {
+    mb = a.w[g.zf(i, j, k0)];
+    load_faces(k0);
+  }
   for (int k = k0; k < k1; ++k) {
     if (k + 1 < nz) stage(k + 1);
-    __syncthreads();
-    if (!active) continue;
-    const R mw = a.u[g.xf(i, j, k)], me = a.u[g.xf(i + 1, j, k)];
-    const R ms = a.v[g.yf(i, j, k)], mn = a.v[g.yf(i, j + 1, k)];
-    const R mb = a.w[g.zf(i, j, k)], mt = a.w[g.zf(i, j, k + 1)];
-    const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k];
-    const size_t ci = g.cell(i, j, k);
+    const R mw = f_mw, me = f_me, ms = f_ms, mn = f_mn, mt = f_mt;
+    R q0v[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-      const R* q = sQ[n][k & 3];
-      const R qc = q[c0];
-      const R qw = q[c0 - 1], qe = q[c0 + 1], qs = q[c0 - PX], qn = q[c0 + PX];
-      const R qb = k > 0 ? sQ[n][(k - 1) & 3][c0] : qc;
-      const R qt = k + 1 < nz ? sQ[n][(k + 1) & 3][c0] : qc;
-      const R fw = mw * (mw >= R(0) ? qw : qc);
-      const R fe = me * (me >= R(0) ? qc : qe);
-      const R fs = ms * (ms >= R(0) ? qs : qc);
-      const R fn = mn * (mn >= R(0) ? qc : qn);
-      const R fb = mb * (mb >= R(0) ? qb : qc);
-      const R ft = mt * (mt >= R(0) ? qc : qt);
-      const R div = qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz;
-      const R v = rmax(R(0), a.in[n][ci] - a.dt * div);
-      a.out[n][ci] = a.mode == 2 ? a.wa * a.q0[n][ci] + a.wb * v : v;
+    for (int n = 0; n < NT; ++n) q0v[n] = f_q0[n];
+    if (active && k + 1 < k1) load_faces(k + 1);
+    __syncthreads();
+    if (active) {
+      const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k];
+      const size_t ci = g.cell(i, j, k);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const R* q = sQ[n][k & 3];
+        const R qc = q[c0];
+        const R qw = q[c0 - 1], qe = q[c0 + 1], qs = q[c0 - PX], qn = q[c0 + PX];
+        const R qb = k > 0 ? sQ[n][(k - 1) & 3][c0] : qc;
+        const R qt = k + 1 < nz ? sQ[n][(k + 1) & 3][c0] : qc;
+        const R fw = mw * (mw >= R(0) ? qw : qc);
+        const R fe = me * (me >= R(0) ? qc : qe);
+        const R fs = ms * (ms >= R(0) ? qs : qc);
+        const R fn = mn * (mn >= R(0) ? qc : qn);
+        const R fb = mb * (mb >= R(0) ? qb : qc);
+        const R ft = mt * (mt >= R(0) ? qc : qt);
+        const R div = qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz;
+        const R v = rmax(R(0), own[n] - a.dt * div);
+        a.out[n][ci] = a.mode == 2 ? a.wa * q0v[n] + a.wb * v : v;
+      }
     }
+    mb = mt;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) own[n] = own_n[n];
   }
 }
 
