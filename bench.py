@@ -48,29 +48,26 @@ def parse():
     p.add_argument("--ensemble-members", type=int, default=32,
                    help="members of the batched-ensemble leg (0 to skip)")
     p.add_argument("--ensemble-steps", type=int, default=20)
+    p.add_argument("--perturbation", type=int, default=1,
+                   help="time the FP32 perturbation form, plain and with the surrogate (0 to skip)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run one device-to-device state copy (profiler byte calibration)")
     return p.parse_args()
 
 
-def make_case(G, precision, nx, ny, nz):
-    ctx = G.Context(0, precision)
+def make_case(G, precision, nx, ny, nz, ctx=None, perturbation=False):
+    if ctx is None:
+        ctx = G.Context(0, precision)
     zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
     grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
-    st = G.State(grid, 3)
+    st = G.State(grid, 3, perturbation=perturbation)
     st.init_convection(1.0e5, 42, 0.5, 1000.0)
     return ctx, grid, st, zi
 
 
 def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None, ctx=None,
-        calibrate_copy=False, info=None):
-    if ctx is None:
-        ctx, grid, st, zi = make_case(G, precision, nx, ny, nz)
-    else:
-        zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
-        grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
-        st = G.State(grid, 3)
-        st.init_convection(1.0e5, 42, 0.5, 1000.0)
+        calibrate_copy=False, info=None, perturbation=False):
+    ctx, grid, st, zi = make_case(G, precision, nx, ny, nz, ctx, perturbation)
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
 
@@ -186,6 +183,34 @@ def surrogate_leg(G, a, cells, info):
         "state_finite": bool(np.isfinite(ssum["w_max"])),
         "kernels": sstats,
     }
+
+
+def perturbation_leg(G, a, cells, sur_src):
+    """The FP32 perturbation form (rho' and (rho theta)' against FP64
+    level-mean bases; within 1e-4 of the FP64 oracle in the tests) on the
+    same case at the finite-state dt: plain warm rain, and with the trained
+    surrogate."""
+    import numpy as np
+
+    dt = a.control_dt or a.dt
+    ctx = G.Context(0, G.F32)
+    out = {"dt": dt}
+    log("perturbation form: plain")
+    s, _, summ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt, ctx=ctx,
+                     perturbation=True)
+    out["value_plain"] = cells * a.steps / s
+    out["state_finite"] = bool(np.isfinite(summ["w_max"]))
+    if sur_src is not None:
+        params, norm = sur_src
+        sur = G.Surrogate(ctx, 0)
+        sur.set_norm(**{k: v for k, v in norm.items()})
+        sur.set_params(params)
+        log("perturbation form: surrogate")
+        s, _, summ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt, surrogate=sur,
+                         ctx=ctx, perturbation=True)
+        out["value_surrogate"] = cells * a.steps / s
+        out["state_finite_surrogate"] = bool(np.isfinite(summ["w_max"]))
+    return out
 
 
 def ensemble_members(G, n, seed=0, z_top=16000.0):
@@ -343,6 +368,11 @@ def main():
         sur["over_fp64_plain"] = sur["value"] / fp64_control
     ens = None
     model = sur.pop("_model", None) if sur else None
+    pert = perturbation_leg(G, a, cells, model) if a.perturbation else None
+    if pert is not None and fp64_control:
+        for k in ("value_plain", "value_surrogate"):
+            if k in pert:
+                pert[k.replace("value", "over_fp64_plain")] = pert[k] / fp64_control
     if a.ensemble_members > 0:
         log(f"ensemble: {a.ensemble_members} members")
         ens = ensemble_leg(G, a, model)
@@ -372,6 +402,7 @@ def main():
         "fp64_plain_control_value": fp64_control,
         "fp32_over_fp64": (value / fp64) if fp64 else None,
         "surrogate": sur,
+        "perturbation_form": pert,
         "ensemble": ens,
         "dry_mass": summary["total_dry_mass"],
         "kernels": stats,
