@@ -32,18 +32,21 @@ __device__ inline float gelu_grad_f(float x) {
   const float t = tanhf(u);
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluA * (1.0f + 3.0f * kGeluB * x * x);
 }
-// Same function as gelu_f written as x * sigmoid(2u), for inference.
+// Same function as gelu_f written as x * sigmoid(2u) = x / (1 + 2^(x (c0 +
+// c1 x^2))), c0 = -2 A log2(e), c1 = c0 B, for inference: seven instructions
+// (hardware exp2 and reciprocal), within a few FP32 ulp of gelu_f, far below
+// the BF16 rounding that follows.
 __device__ inline float gelu_fast(float x) {
-  const float u = kGeluA * (x + kGeluB * x * x * x);
-  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  constexpr float c0 = -2.0f * kGeluA * 1.44269504088896341f;
+  constexpr float c1 = c0 * kGeluB;
+  const float a = x * __builtin_fmaf(c1, x * x, c0);
+  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(a));
 }
 
-// FP32 -> BF16 with round-to-nearest-even (NaN kept quiet).
+// FP32 -> BF16, round to nearest even, NaN kept quiet (v_cvt_pk_bf16_f32).
+__device__ inline __bf16 bf16_of(float f) { return static_cast<__bf16>(f); }
 __device__ inline unsigned short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return 0x7fc0;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
+  return __builtin_bit_cast(unsigned short, bf16_of(f));
 }
 
 __device__ inline unsigned long long mix64(unsigned long long z) {  // splitmix64 finaliser
@@ -214,7 +217,10 @@ namespace {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 struct MlpRegs {
   s16x4 a1[4];      // layer 1, per output tile
@@ -223,7 +229,6 @@ struct MlpRegs {
   bf16x8 a4[2];     // layer 4, [k tile]
 };
 
-__device__ inline __bf16 bf16_of(float f) { return __builtin_bit_cast(__bf16, to_bf16(f)); }
 
 __device__ inline int perm_k(int kt, int g, int v) {
   return 32 * kt + (v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4));
@@ -272,6 +277,21 @@ __device__ inline void load_bias(const float* w, float* sb) {
   }
 }
 
+// GELU of two accumulator quads, rounded to BF16 in pairs (one
+// v_cvt_pk_bf16_f32 per two values) and packed as the next layer's k-slice:
+// slots 0-3 from lo, 4-7 from hi.
+__device__ inline bf16x8 gelu_pack(const f32x4& lo, const f32x4& hi) {
+  auto cvt2 = [](float a, float b) {
+    const f32x2 v = {gelu_fast(a), gelu_fast(b)};
+    return __builtin_convertvector(v, bf16x2);
+  };
+  const bf16x2 p0 = cvt2(lo[0], lo[1]), p1 = cvt2(lo[2], lo[3]);
+  const bf16x2 p2 = cvt2(hi[0], hi[1]), p3 = cvt2(hi[2], hi[3]);
+  const bf16x4 q0 = __builtin_shufflevector(p0, p1, 0, 1, 2, 3);
+  const bf16x4 q1 = __builtin_shufflevector(p2, p3, 0, 1, 2, 3);
+  return __builtin_shufflevector(q0, q1, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 // One group of 16 cells.  xin: this lane's normalised features 4g .. 4g+3
 // (zero for g >= 2).  Returns, for lanes with g == 0, the normalised
 // outputs 0..3 of cell l % 16.
@@ -286,11 +306,7 @@ __device__ inline f32x4 mlp_group(const MlpRegs& R, const float* sb, const float
   }
   for (int layer = 0; layer < 2; ++layer) {
     bf16x8 bk[2];
-    for (int kt = 0; kt < 2; ++kt)
-      for (int v = 0; v < 4; ++v) {
-        bk[kt][v] = bf16_of(gelu_fast(h[2 * kt][v]));
-        bk[kt][v + 4] = bf16_of(gelu_fast(h[2 * kt + 1][v]));
-      }
+    for (int kt = 0; kt < 2; ++kt) bk[kt] = gelu_pack(h[2 * kt], h[2 * kt + 1]);
     const float* b = sb + 64 * (layer + 1);
     for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc;
@@ -301,11 +317,7 @@ __device__ inline f32x4 mlp_group(const MlpRegs& R, const float* sb, const float
     }
   }
   bf16x8 bk[2];
-  for (int kt = 0; kt < 2; ++kt)
-    for (int v = 0; v < 4; ++v) {
-      bk[kt][v] = bf16_of(gelu_fast(h[2 * kt][v]));
-      bk[kt][v + 4] = bf16_of(gelu_fast(h[2 * kt + 1][v]));
-    }
+  for (int kt = 0; kt < 2; ++kt) bk[kt] = gelu_pack(h[2 * kt], h[2 * kt + 1]);
   f32x4 acc;
   for (int r = 0; r < 4; ++r) acc[r] = g == 0 ? sb[192 + r] : 0.0f;
   acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a4[0], bk[0], acc, 0, 0, 0);
