@@ -294,6 +294,10 @@ struct gwm_state {
   void* rref_lev = nullptr;   // nz
   void* pref_lev = nullptr;   // nz
   void* theta_scr = nullptr;  // cell field
+  // FP64 level partial sums of the step (theta_ref, fast-mode level means):
+  // owned by the state and sized once for its grid, so the step graphs
+  // captured from it keep a valid address.
+  double* step_part = nullptr;
   // Perturbation form (flat, doubly periodic, FP32): rho and rho*theta are
   // stored as departures from per-level bases, the FP64 level means of the
   // state they were built from; the device holds the bases in working
@@ -383,8 +387,18 @@ struct gwm_state {
     for (auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr})
       for (void* p : *v)
         if (p) (void)hipFree(p);
-    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev})
+    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
+                    static_cast<void*>(step_part)})
       if (p) (void)hipFree(p);
+  }
+  // Partial-sum buffer of the step's level means (both uses fit).
+  double* level_part() {
+    if (!step_part) {
+      const Geo& gg = g();
+      const size_t per = std::max<size_t>(gwm::fast_column_parts(gg.nx, gg.ny), (gg.ny + 7) / 8);
+      step_part = static_cast<double*>(dalloc(static_cast<size_t>(gg.nz) * per * 2 * sizeof(double)));
+    }
+    return step_part;
   }
 };
 
@@ -422,7 +436,7 @@ template <class R>
 void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
   const Geo& g = S->g();
   if (!S->theta_ref) S->theta_ref = dalloc(g.nz * sizeof(R));
-  double* part = S->ctx->scratch(static_cast<size_t>(g.nz) * ((g.ny + 7) / 8) * 2);
+  double* part = S->level_part();
   int nch = 0;
   {
     Timed t(S->ctx, "level_partials");
@@ -494,11 +508,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   int last = z0;
   // Level partials: substep 0 from a pass over the entry state, later
   // substeps from the previous substep's column kernel.
-  const int nwave = gwm::fast_column_parts(g.nx, g.ny);
-  const int nrow = (g.ny + 7) / 8;
-  double* part = nullptr;
-  if (!colref)
-    part = S->ctx->scratch(static_cast<size_t>(g.nz) * std::max(nwave, nrow) * 2);
+  double* part = colref ? nullptr : S->level_part();
   int nparts = 0;
   for (int s = 0; s < ns; ++s) {
     const int o = outs[s % 2];
