@@ -395,7 +395,8 @@ struct gwm_state {
   double* level_part() {
     if (!step_part) {
       const Geo& gg = g();
-      const size_t per = std::max<size_t>(gwm::fast_column_parts(gg.nx, gg.ny), (gg.ny + 7) / 8);
+      const size_t per = std::max<size_t>(gwm::fast_column_parts(gg.nx, gg.ny),
+                                          gwm::level_partials_chunks(gg.ny));
       step_part = static_cast<double*>(dalloc(static_cast<size_t>(gg.nz) * per * 2 * sizeof(double)));
     }
     return step_part;

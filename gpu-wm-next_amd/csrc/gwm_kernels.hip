@@ -38,42 +38,58 @@ __device__ inline double block_sum(double v, double* sh) {
 // partials in fixed order).  Block = 256 threads over a chunk of rows of one
 // level; partials[(k*nchunk + c)*2 + {0,1}].
 // ===========================================================================
+constexpr int kPartialRows = 8;  // rows of one level per work-group
+
 template <class R>
 __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, const R* aux,
-                                                         int mode, int rows_per_chunk,
-                                                         double* partials, Base<R> base) {
+                                                         int mode, double* partials,
+                                                         Base<R> base) {
   __shared__ double sh[256];
   const int k = blockIdx.y;
   const int c = blockIdx.x;
   const int nchunk = gridDim.x;
-  const int j0 = c * rows_per_chunk;
-  const int j1 = min(g.ny, j0 + rows_per_chunk);
+  const int j0 = c * kPartialRows;
   const int t = threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
   const bool pert = base.rho != nullptr;
   const R rb = pert ? base.rho[k] : R(0), thb = pert ? base.th[k] : R(0);
   const R rtb = pert ? base.rt[k] : R(0), pb = pert ? base.p[k] : R(0);
-  for (int j = j0; j < j1; ++j)
-    for (int i = t; i < g.nx; i += 256) {
-      const size_t c0 = g.cell(i, j, k);
-      const R r = rho[c0];
+  // Each thread loads a column segment of the chunk's rows at once (all
+  // loads in flight), then accumulates in row order.
+  for (int i = t; i < g.nx; i += 256) {
+    R rv[kPartialRows], av[kPartialRows];
+#pragma unroll
+    for (int q = 0; q < kPartialRows; ++q) {
+      rv[q] = R(0);
+      av[q] = R(0);
+      if (j0 + q < g.ny) {
+        const size_t c0 = g.cell(i, j0 + q, k);
+        rv[q] = rho[c0];
+        av[q] = aux[c0];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPartialRows; ++q) {
+      if (j0 + q >= g.ny) break;
+      const R r = rv[q];
       if (pert) {
         // departures: theta' of (rho', (rho theta)'), or rho' and p' of
         // (rho', theta' fixed)
         if (mode == 0) {
-          s0 += static_cast<double>(theta_pert(r, aux[c0], rb, thb));
+          s0 += static_cast<double>(theta_pert(r, av[q], rb, thb));
         } else {
-          const R tp = aux[c0];
+          const R tp = av[q];
           s0 += static_cast<double>(r);
           s1 += static_cast<double>(p_pert(rb * tp + r * (thb + tp), rtb, pb));
         }
       } else if (mode == 0) {
-        s0 += static_cast<double>(q_theta(r, aux[c0]));
+        s0 += static_cast<double>(q_theta(r, av[q]));
       } else {
         s0 += static_cast<double>(r);
-        s1 += static_cast<double>(q_pressure(r, r * aux[c0]));
+        s1 += static_cast<double>(q_pressure(r, r * av[q]));
       }
     }
+  }
   s0 = block_sum(s0, sh);
   s1 = block_sum(s1, sh);
   if (t == 0) {
@@ -530,12 +546,12 @@ inline unsigned blocks_for(size_t n, unsigned b) { return static_cast<unsigned>(
 template <class R>
 void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
                            double* partials, int* nchunk_out, Base<R> base) {
-  const int rows = 8;
-  const int nchunk = (g.ny + rows - 1) / rows;
+  const int nchunk = level_partials_chunks(g.ny);
   hipLaunchKernelGGL(k_level_partials<R>, dim3(nchunk, g.nz), dim3(256), 0, st, g, rho, aux, mode,
-                     rows, partials, base);
+                     partials, base);
   *nchunk_out = nchunk;
 }
+int level_partials_chunks(int ny) { return (ny + kPartialRows - 1) / kPartialRows; }
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1) {

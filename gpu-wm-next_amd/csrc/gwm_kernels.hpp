@@ -86,6 +86,8 @@ struct ReduceArgs {
 template <class R>
 void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
                            double* partials, int* nchunk_out, Base<R> base = Base<R>{});
+// Partials per level written by launch_level_partials for ny rows.
+int level_partials_chunks(int ny);
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1);
