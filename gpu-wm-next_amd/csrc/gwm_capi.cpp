@@ -487,13 +487,14 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
   return best;
 }
 
-// One fast subcycle (apply_local_split_explicit_fast_modes) from
-// (rho_in, faces_in) with fixed theta and base faces; returns the set index
-// holding the result (rho, rho*theta, faces).  out sets are a[0], a[1].
+// One fast subcycle (apply_local_split_explicit_fast_modes) from rho_in and
+// the entry faces (ub, vb, wb), which are also the base faces of K15, with
+// fixed theta; returns the set index holding the result (rho, rho*theta,
+// faces).  Between substeps the out sets z0 / z1 hold face departures from
+// the entry faces.
 template <class R>
-int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u_in,
-             const R* v_in, const R* w_in, const R* theta, const R* ub, const R* vb, const R* wb,
-             int z0, int z1) {
+int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* theta,
+             const R* ub, const R* vb, const R* wb, int z0, int z1) {
   const Geo& g = S->g();
   const gwm_grid* G = S->grid;
   const int ns = std::max(1, c.fast_substeps);
@@ -505,7 +506,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
   }
   const int outs[2] = {z0, z1};
   const R* rho = rho_in;
-  const R *u = u_in, *v = v_in, *w = w_in;
+  const R *u = ub, *v = vb, *w = wb;  // departures after the first substep
   int last = z0;
   // Level partials: substep 0 from a pass over the entry state, later
   // substeps from the previous substep's column kernel.
@@ -547,6 +548,8 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* u
     a.horizontal = c.fast_update_horizontal_momentum;
     a.density = c.fast_update_density;
     a.col_ref = colref;
+    a.first = s == 0;
+    a.last = s == ns - 1;
     a.base = S->base<R>();
     const bool emit = !colref && s + 1 < ns;
     auto run = [&](int n) {
@@ -695,10 +698,9 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
       require(fast_fn(&view, 0.0, fast_user) == 0, "fast-mode callback failed");
       return Z0;
     }
-    return subcycle<R>(S, c, static_cast<const R*>(ys[0]), static_cast<const R*>(ys[2]),
-                       static_cast<const R*>(ys[3]), static_cast<const R*>(ys[4]),
-                       static_cast<const R*>(ys[1]), static_cast<const R*>(ys[2]),
-                       static_cast<const R*>(ys[3]), static_cast<const R*>(ys[4]), Z0, Z1);
+    return subcycle<R>(S, c, static_cast<const R*>(ys[0]), static_cast<const R*>(ys[1]),
+                       static_cast<const R*>(ys[2]), static_cast<const R*>(ys[3]),
+                       static_cast<const R*>(ys[4]), Z0, Z1);
   };
   // stage 1
   slow_stage<R>(S, c, in, nullptr, S->set[Y], 1, R(0), R(1), 1);
@@ -732,8 +734,7 @@ void fast_modes(gwm_state* S, const gwm_step_config& c) {
                          S->base<R>());
   }
   void** xs = S->set[X];
-  const int r = subcycle<R>(S, c, static_cast<const R*>(xs[0]), static_cast<const R*>(xs[2]),
-                            static_cast<const R*>(xs[3]), static_cast<const R*>(xs[4]),
+  const int r = subcycle<R>(S, c, static_cast<const R*>(xs[0]),
                             static_cast<const R*>(S->theta_scr), static_cast<const R*>(xs[2]),
                             static_cast<const R*>(xs[3]), static_cast<const R*>(xs[4]),
                             others[0], others[1]);

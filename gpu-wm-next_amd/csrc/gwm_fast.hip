@@ -24,6 +24,14 @@
 // contiguous band of tiles and the ring rows shared by neighbouring tiles hit
 // the same L2.
 //
+// Faces as departures.  Inside a subcycle the kernel carries the face
+// momenta as departures m - m_base from the subcycle's entry faces (K15
+// needs only the divergence of m - m_base, and m - m_base starts at zero), so
+// a middle substep reads rho, theta and three departure fields and writes
+// rho and three departures -- 9 fields instead of the 12 of the full faces
+// and their bases.  The first substep reads no faces; the last reads the
+// base faces and writes full faces (m_base + departure) and rho*theta.
+//
 // Flat terrain: the level means of rho and p for the next substep are
 // emitted as per-wave FP64 partial sums of the new rho (fixed order: a wave
 // butterfly, lane 0 writes), replacing a separate pass over the state.
@@ -106,21 +114,28 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
     if (t < NHALO) sp[k & 3][hci] = press(rr_h, rt_h, k);
   };
 
-  // Face momenta of one level for the own cell (prefetched a level ahead).
+  // Face departures m - m_base of one level for the own cell (prefetched a
+  // level ahead): zero on the first substep (the faces are the base faces),
+  // read from the previous substep's output otherwise.  The base faces are
+  // read on the last substep only, which writes full faces.
   R f_u0 = 0, f_u1 = 0, f_v0 = 0, f_v1 = 0, f_w1 = 0, f_ub0 = 0, f_ub1 = 0, f_vb0 = 0,
     f_vb1 = 0, f_wb1 = 0;
   auto load_faces = [&](int k) {
     const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k), zf1 = g.zf(i, j, k + 1);
-    f_u0 = a.u[xf0];
-    f_u1 = a.u[xf0 + 1];
-    f_v0 = a.v[yf0];
-    f_v1 = a.v[yf0 + g.pxc];
-    f_w1 = a.w[zf1];
-    f_ub0 = a.ub[xf0];
-    f_ub1 = a.ub[xf0 + 1];
-    f_vb0 = a.vb[yf0];
-    f_vb1 = a.vb[yf0 + g.pxc];
-    f_wb1 = a.wb[zf1];
+    if (!a.first) {
+      f_u0 = a.u[xf0];
+      f_u1 = a.u[xf0 + 1];
+      f_v0 = a.v[yf0];
+      f_v1 = a.v[yf0 + g.pxc];
+      f_w1 = a.w[zf1];
+    }
+    if (a.last) {
+      f_ub0 = a.ub[xf0];
+      f_ub1 = a.ub[xf0 + 1];
+      f_vb0 = a.vb[yf0];
+      f_vb1 = a.vb[yf0 + g.pxc];
+      f_wb1 = a.wb[zf1];
+    }
   };
 
   // Metric column index (periodic wrap of global i, j; dry_fast_modes.cu:24-30).
@@ -164,8 +179,8 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   R w0 = R(0), wb0 = R(0);
   if (active) {
     const size_t zf0 = g.zf(i, j, k0);
-    w0 = a.w[zf0];
-    wb0 = a.wb[zf0];
+    w0 = a.first ? R(0) : a.w[zf0];
+    wb0 = a.last ? a.wb[zf0] : R(0);
     if (k0 > 0) {
       // K14 at face k0 from levels k0-1 and k0 (flat references).
       const R ppl = sp[(k0 - 1) & 3][ci] - a.pref_lev[k0 - 1];
@@ -263,9 +278,8 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       rho_new = rho_k;
       if (a.density) {
         const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : a.m.lev_idz_c[k];
-        const R div = qdivc((u1 - ub1) - (u0 - ub0), dx, rdx) +
-                      qdivc((v1 - vb1) - (v0 - vb0), dy, rdy) +
-                      ((w1 - wb1) - (w0 - wb0)) * idz;
+        // divergence of (m - m_base): the faces are departures already
+        const R div = qdivc(u1 - u0, dx, rdx) + qdivc(v1 - v0, dy, rdy) + (w1 - w0) * idz;
         const R floor = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
@@ -275,12 +289,13 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       if (a.rt_out)
         a.rt_out[c0] = pert ? a.base.rho[k] * th_k + rho_new * (a.base.th[k] + th_k)
                             : rho_new * th_k;
-      a.u_out[xf0] = u0;
-      a.v_out[yf0] = v0;
-      a.w_out[c0] = w0;
-      if (i == g.nx - 1) a.u_out[xf0 + 1] = u1;
-      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = v1;
-      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = w1;
+      // departures, or full faces base + departure on the last substep
+      a.u_out[xf0] = a.last ? ub0 + u0 : u0;
+      a.v_out[yf0] = a.last ? vb0 + v0 : v0;
+      a.w_out[c0] = a.last ? wb0 + w0 : w0;
+      if (i == g.nx - 1) a.u_out[xf0 + 1] = a.last ? ub1 + u1 : u1;
+      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = a.last ? vb1 + v1 : v1;
+      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = a.last ? wb1 + w1 : w1;
       w0 = w1;
       wb0 = wb1;
     }

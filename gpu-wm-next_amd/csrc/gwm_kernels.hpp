@@ -26,9 +26,10 @@ struct FastArgs {
   Geo g;
   const R* rho;    // current rho
   const R* theta;  // fixed theta of the subcycle
-  const R *u, *v, *w;     // current faces
-  const R *ub, *vb, *wb;  // faces at subcycle entry
-  R *rho_out, *u_out, *v_out, *w_out;
+  const R *u, *v, *w;     // face departures m - m_base (read unless first)
+  const R *ub, *vb, *wb;  // faces at subcycle entry (read on the last substep)
+  int first, last;        // first / last substep of the subcycle
+  R *rho_out, *u_out, *v_out, *w_out;  // face departures, full faces if last
   R* rt_out;  // rho*theta rebuild (last substep), may be null
   const R *rref_lev, *pref_lev;  // flat reference (nz)
   DMetrics<R> m;
