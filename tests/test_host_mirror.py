@@ -26,7 +26,7 @@ def _need(path):
 
 
 def test_binaries_link_against_the_hip_library():
-    for name in ["gwm_idealized_driver"] + CPP_TESTS:
+    for name in ["gwm_idealized_driver", "gwm_prepared_case_driver"] + CPP_TESTS:
         path = os.path.join(BIN, name)
         _need(path)
         out = subprocess.run(["ldd", path], capture_output=True, text=True, check=True).stdout
