@@ -449,6 +449,7 @@ struct WorldBase {
   virtual ~WorldBase() = default;
   int prec = 4;
   int n_tracers = 0;
+  unsigned tracer_strip_mask = ~0u;  // tracers the strip boundary overwrites
   std::vector<Desc> L;
   virtual long storage_size(int rank, int field, int* dims) = 0;
   virtual void get_rank(int rank, int field, void* out) = 0;
@@ -1253,6 +1254,10 @@ struct World : WorldBase {
   }
   void apply_tracer_boundary(std::vector<Field<R>>& f, int t, int mode) {
     if (mode == ORC_BOUNDARY_NONE) return;
+    // Tracers outside the mask keep their strip values: the prepared-case
+    // tracer updater overwrites only the tracers its boundary snapshot
+    // carries (prepared_case_init.cpp:567-638).
+    if (t < 32 && !((tracer_strip_mask >> t) & 1u)) return;
     require(have_boundary, "boundary state not set");
     for (size_t n = 0; n < L.size(); ++n) {
       const Desc& d = L[n];
@@ -1746,6 +1751,9 @@ int orc_fast_modes(void* w, const orc_step_cfg* c) {
 }
 int orc_set_boundary_from_current(void* w) {
   return guard([&] { W(w)->set_boundary(); });
+}
+int orc_set_tracer_strip_mask(void* w, unsigned mask) {
+  return guard([&] { W(w)->tracer_strip_mask = mask; });
 }
 int orc_apply_boundary(void* w, int mode, double width) {
   return guard([&] { W(w)->boundary_now(mode, width); });

@@ -146,6 +146,8 @@ int orc_momentum_flux_only(void* world);
 int orc_pressure_gradient_only(void* world);
 int orc_fast_modes(void* world, const orc_step_cfg* cfg);
 int orc_set_boundary_from_current(void* world);
+/* Bit t set: tracer t takes the strip boundary (default all). */
+int orc_set_tracer_strip_mask(void* world, unsigned mask);
 /* apply_reference_boundaries (strips) or the relaxation extension, now. */
 int orc_apply_boundary(void* world, int mode, double width);
 /* Scalar thermodynamics: fn 0 theta(rho, rho_theta), 1 pressure(rho,

@@ -110,6 +110,7 @@ def lib():
         L.orc_thermo.restype = dp
         L.orc_thermo.argtypes = [ip, ip, dp, dp, dp]
         L.orc_apply_boundary.argtypes = [vp, ip, dp]
+        L.orc_set_tracer_strip_mask.argtypes = [vp, ctypes.c_uint]
         L.orc_time_steps.restype = dp
         L.orc_time_steps.argtypes = [vp, ctypes.POINTER(StepCfg), ctypes.POINTER(WarmRainCfg),
                                      ip, ip, ip]
@@ -241,6 +242,10 @@ class World:
 
     def set_boundary_from_current(self):
         _check(lib().orc_set_boundary_from_current(self.h))
+
+    def set_tracer_strip_mask(self, mask):
+        """Bit t set: tracer t takes the strip boundary (default all)."""
+        _check(lib().orc_set_tracer_strip_mask(self.h, ctypes.c_uint(mask)))
 
     def apply_boundary(self, mode, width=0.0):
         _check(lib().orc_apply_boundary(self.h, mode, float(width)))
