@@ -161,6 +161,13 @@ void* dalloc(size_t bytes) {
   return p;
 }
 
+// Host <-> device copy ordered on a context stream (non-blocking, so a
+// null-stream hipMemcpy would not wait for its kernels), complete on return.
+void scopy(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -323,8 +330,10 @@ struct gwm_state {
   };
   Graph graphs[2];
   bool faces_synced = false;
-  // view of a stage set
+  // view of a stage set; a tracer stage view also points at the stage's
+  // tracer buffers (empty: the owner's canonical tracers)
   void* vset[5] = {};
+  std::vector<void*> vtr;
 
   const Geo& g() const { return grid->geo; }
   size_t es() const { return ctx->esize(); }
@@ -367,7 +376,8 @@ struct gwm_state {
     if (f == GWM_PRECIP) return precip;
     const int t = f - GWM_TRACER0;
     require(t >= 0 && t < ntr, "bad field id");
-    return view ? owner->tr[t] : tr[t];
+    if (view) return vtr.empty() ? owner->tr[t] : vtr[t];
+    return tr[t];
   }
   ~gwm_state() {
     if (view) return;
@@ -749,9 +759,13 @@ void slow_tendencies(gwm_state* S, const gwm_step_config& c) {
   slow_stage<R>(S, c, S->cur_set(), nullptr, out, 0, R(0), R(1), 0);
 }
 
-// advance_passive_tracers_ssprk3 (dry_transport.cu:440-518)
+// advance_passive_tracers_ssprk3 (dry_transport.cu:440-518).  The lateral
+// boundary runs on each stage input (strips from the captured reference,
+// then the host callback `fn` on a stage view at the step-relative times 0,
+// dt, dt/2, as apply_tracer_boundaries_to_fields, dry_transport.cu:254-276).
 template <class R>
-void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
+void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode,
+                    gwm_boundary_fn fn = nullptr, void* user = nullptr) {
   if (S->ntr == 0) return;
   const Geo& g = S->g();
   ensure_synced<R>(S, S->set[S->cur]);
@@ -769,26 +783,41 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
   a.lev_idz_c = static_cast<const R*>(S->grid->lev_idz_c);
   a.ntr = S->ntr;
   a.dt = static_cast<R>(c.dt);
-  auto strips = [&](std::vector<void*>& f) {
-    if (bmode == GWM_BOUNDARY_NONE) return;
-    require(S->have_boundary, "boundary reference not captured");
-    for (int t = 0; t < S->ntr; ++t) {
-      Timed tt(S->ctx, "boundary");
-      gwm::launch_boundary<R>(S->ctx->stream, g, 0, static_cast<R*>(f[t]),
-                              static_cast<const R*>(S->bref_tr[t]), GWM_BOUNDARY_STRIPS, 0.0);
+  auto strips = [&](std::vector<void*>& f, double sim_time) {
+    if (bmode != GWM_BOUNDARY_NONE) {
+      require(S->have_boundary, "boundary reference not captured");
+      for (int t = 0; t < S->ntr; ++t) {
+        Timed tt(S->ctx, "boundary");
+        gwm::launch_boundary<R>(S->ctx->stream, g, 0, static_cast<R*>(f[t]),
+                                static_cast<const R*>(S->bref_tr[t]), GWM_BOUNDARY_STRIPS, 0.0);
+      }
+    }
+    if (fn) {
+      gwm_state view;
+      view.ctx = S->ctx;
+      view.grid = S->grid;
+      view.ntr = S->ntr;
+      view.view = true;
+      view.owner = S;
+      view.faces_synced = true;
+      for (int q = 0; q < 5; ++q) view.vset[q] = ds[q];
+      view.vtr = f;
+      HIPCHK(hipStreamSynchronize(S->ctx->stream));
+      require(fn(&view, sim_time, user) == 0, "tracer boundary callback failed");
     }
   };
+  const bool bnd = bmode != GWM_BOUNDARY_NONE || fn;
   std::vector<void*>& A = S->tr_scr[0];
   std::vector<void*>& B = S->tr_scr[1];
   std::vector<void*>& C = S->tr_scr[2];
   std::vector<void*> in1 = S->tr;
-  if (bmode != GWM_BOUNDARY_NONE) {
+  if (bnd) {
     for (int t = 0; t < S->ntr; ++t) {
       Timed tt(S->ctx, "copy");
       gwm::launch_copy<R>(S->ctx->stream, static_cast<const R*>(S->tr[t]), static_cast<R*>(C[t]),
                           g.cell_total());
     }
-    strips(C);
+    strips(C, 0.0);
     in1 = C;
   }
   auto stage = [&](const std::vector<void*>& in, std::vector<void*>& out, int mode, R wa, R wb) {
@@ -810,9 +839,9 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode) {
     run(nkc);
   };
   stage(in1, A, 1, R(0), R(1));
-  strips(A);
+  if (bnd) strips(A, c.dt);
   stage(A, B, 2, R(0.75), R(0.25));
-  strips(B);
+  if (bnd) strips(B, 0.5 * c.dt);
   stage(B, A, 2, R(1) / R(3), R(2) / R(3));
   std::swap(S->tr, S->tr_scr[0]);
 }
@@ -1032,7 +1061,7 @@ void push_base(gwm_state* S) {
     b[3 * nz + k] = static_cast<float>(eos64(r, t));
   }
   if (!S->b_dev) S->b_dev = dalloc(b.size() * sizeof(float));
-  HIPCHK(hipMemcpy(S->b_dev, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+  scopy(S->ctx->stream, S->b_dev, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice);
 }
 
 // Full cell field -> departures from `base` (a level mean of it), FP32, device.
@@ -1044,7 +1073,7 @@ void upload_departure(gwm_state* S, void* dst, const T* full, const std::vector<
   std::vector<float> d(n);
   for (size_t a = 0; a < n; ++a)
     d[a] = static_cast<float>(static_cast<double>(full[a]) - base[a / plane]);
-  HIPCHK(hipMemcpy(dst, d.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  scopy(S->ctx->stream, dst, d.data(), n * sizeof(float), hipMemcpyHostToDevice);
 }
 
 // Whole state in the perturbation form: bases from the FP64 level means of
@@ -1062,13 +1091,13 @@ void upload_pert_fields(gwm_state* S, const T* rho, const T* rt, const T* u, con
     const size_t n = S->fsize(a);
     std::vector<float> h(n);
     for (size_t e = 0; e < n; ++e) h[e] = static_cast<float>(f[a - 2][e]);
-    HIPCHK(hipMemcpy(S->cur_set()[a], h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    scopy(S->ctx->stream, S->cur_set()[a], h.data(), n * sizeof(float), hipMemcpyHostToDevice);
   }
   for (size_t t = 0; t < q.size() && t < static_cast<size_t>(S->ntr); ++t) {
     std::vector<float> h(q[t].size());
     for (size_t e = 0; e < h.size(); ++e) h[e] = static_cast<float>(q[t][e]);
-    HIPCHK(hipMemcpy(S->field(GWM_TRACER0 + static_cast<int>(t)), h.data(),
-                     h.size() * sizeof(float), hipMemcpyHostToDevice));
+    scopy(S->ctx->stream, S->field(GWM_TRACER0 + static_cast<int>(t)), h.data(),
+                     h.size() * sizeof(float), hipMemcpyHostToDevice);
   }
   S->faces_synced = false;
 }
@@ -1093,10 +1122,11 @@ struct HostState {
     }
     std::vector<R>* f[5] = {&rho, &rt, &u, &v, &w};
     for (int a = 0; a < 5; ++a)
-      HIPCHK(hipMemcpy(S->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R), hipMemcpyHostToDevice));
+      scopy(S->ctx->stream, S->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R),
+            hipMemcpyHostToDevice);
     for (int t = 0; t < S->ntr; ++t)
-      HIPCHK(hipMemcpy(S->field(GWM_TRACER0 + t), q[t].data(), q[t].size() * sizeof(R),
-                       hipMemcpyHostToDevice));
+      scopy(S->ctx->stream, S->field(GWM_TRACER0 + t), q[t].data(), q[t].size() * sizeof(R),
+            hipMemcpyHostToDevice);
     S->faces_synced = false;
   }
 };
@@ -1120,10 +1150,10 @@ std::vector<R> host_metric(const gwm_state* S, int which) {
   const void* src = which == 0 ? S->grid->zif_m : which == 1 ? S->grid->zc_m
                   : which == 2 ? S->grid->idz_c : S->grid->idz_f;
   if (S->ctx->esize() == sizeof(R)) {
-    HIPCHK(hipMemcpy(out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost));
+    scopy(S->ctx->stream, out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost);
   } else {  // FP64 host build of a perturbation-form (FP32) state
     std::vector<float> tmp(n);
-    HIPCHK(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    scopy(S->ctx->stream, tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
     for (size_t a = 0; a < n; ++a) out[a] = static_cast<R>(tmp[a]);
   }
   return out;
@@ -1226,10 +1256,10 @@ void init_mountain(gwm_state* S, double rho_s, double th, double u, double v, do
                    static_cast<R>(w));
   std::vector<R> terr(static_cast<size_t>(g.nx) * g.ny);
   if (S->ctx->esize() == sizeof(R)) {
-    HIPCHK(hipMemcpy(terr.data(), S->grid->terr, terr.size() * sizeof(R), hipMemcpyDeviceToHost));
+    scopy(S->ctx->stream, terr.data(), S->grid->terr, terr.size() * sizeof(R), hipMemcpyDeviceToHost);
   } else {  // FP64 host build of a perturbation-form (FP32) state
     std::vector<float> tmp(terr.size());
-    HIPCHK(hipMemcpy(tmp.data(), S->grid->terr, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    scopy(S->ctx->stream, tmp.data(), S->grid->terr, tmp.size() * sizeof(float), hipMemcpyDeviceToHost);
     for (size_t a = 0; a < tmp.size(); ++a) terr[a] = static_cast<R>(tmp[a]);
   }
   const auto& zc = S->grid->z_c;
@@ -1556,13 +1586,13 @@ int gwm_grid_download_metric(gwm_grid* G, int which, void* host) {
     const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
     const size_t es = G->ctx->esize();
     switch (which) {
-      case 0: HIPCHK(hipMemcpy(host, G->zif_m, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost)); break;
-      case 1: HIPCHK(hipMemcpy(host, G->zc_m, n2 * g.nz * es, hipMemcpyDeviceToHost)); break;
-      case 2: HIPCHK(hipMemcpy(host, G->idz_c, n2 * g.nz * es, hipMemcpyDeviceToHost)); break;
-      case 3: HIPCHK(hipMemcpy(host, G->idz_f, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost)); break;
-      case 4: HIPCHK(hipMemcpy(host, G->terr, n2 * es, hipMemcpyDeviceToHost)); break;
-      case 5: HIPCHK(hipMemcpy(host, G->sx, n2 * es, hipMemcpyDeviceToHost)); break;
-      case 6: HIPCHK(hipMemcpy(host, G->sy, n2 * es, hipMemcpyDeviceToHost)); break;
+      case 0: scopy(G->ctx->stream, host, G->zif_m, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost); break;
+      case 1: scopy(G->ctx->stream, host, G->zc_m, n2 * g.nz * es, hipMemcpyDeviceToHost); break;
+      case 2: scopy(G->ctx->stream, host, G->idz_c, n2 * g.nz * es, hipMemcpyDeviceToHost); break;
+      case 3: scopy(G->ctx->stream, host, G->idz_f, n2 * (g.nz + 1) * es, hipMemcpyDeviceToHost); break;
+      case 4: scopy(G->ctx->stream, host, G->terr, n2 * es, hipMemcpyDeviceToHost); break;
+      case 5: scopy(G->ctx->stream, host, G->sx, n2 * es, hipMemcpyDeviceToHost); break;
+      case 6: scopy(G->ctx->stream, host, G->sy, n2 * es, hipMemcpyDeviceToHost); break;
       case 7: std::memcpy(host, G->z_if.data(), G->z_if.size() * sizeof(double)); break;
       case 8: std::memcpy(host, G->z_c.data(), G->z_c.size() * sizeof(double)); break;
       case 9: std::memcpy(host, G->w_if.data(), G->w_if.size() * sizeof(double)); break;
@@ -1649,9 +1679,13 @@ int gwm_state_create(gwm_ctx* ctx, const gwm_grid* grid, int ntr, gwm_state** ou
     S->ensure_sets(1);
     for (int t = 0; t < ntr; ++t) S->tr.push_back(dalloc(S->g().cell_total() * ctx->esize()));
     S->precip = dalloc(static_cast<size_t>(S->g().nx) * S->g().ny * ctx->esize());
-    for (int f = 0; f < 5; ++f) HIPCHK(hipMemset(S->set[0][f], 0, S->fsize(f) * ctx->esize()));
-    for (void* p : S->tr) HIPCHK(hipMemset(p, 0, S->g().cell_total() * ctx->esize()));
-    HIPCHK(hipMemset(S->precip, 0, static_cast<size_t>(S->g().nx) * S->g().ny * ctx->esize()));
+    for (int f = 0; f < 5; ++f)
+      HIPCHK(hipMemsetAsync(S->set[0][f], 0, S->fsize(f) * ctx->esize(), ctx->stream));
+    for (void* p : S->tr)
+      HIPCHK(hipMemsetAsync(p, 0, S->g().cell_total() * ctx->esize(), ctx->stream));
+    HIPCHK(hipMemsetAsync(S->precip, 0, static_cast<size_t>(S->g().nx) * S->g().ny * ctx->esize(),
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     S->faces_synced = false;
     *out = S.release();
   });
@@ -1713,7 +1747,7 @@ int gwm_state_upload(gwm_state* s, int f, const void* host) {
       upload_departure(s, d, h, f == GWM_RHO ? o->base_rho64 : o->base_rt64);
       return;
     }
-    HIPCHK(hipMemcpy(d, host, s->fsize(f) * s->es(), hipMemcpyHostToDevice));
+    scopy(s->ctx->stream, d, host, s->fsize(f) * s->es(), hipMemcpyHostToDevice);
     if (f == GWM_MOM_U || f == GWM_MOM_V) s->faces_synced = false;
   });
 }
@@ -1724,7 +1758,7 @@ int gwm_state_download(gwm_state* s, int f, void* host) {
     require(host, "null host pointer");
     void* d = s->field(f);
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
-    HIPCHK(hipMemcpy(host, d, s->fsize(f) * s->es(), hipMemcpyDeviceToHost));
+    scopy(s->ctx->stream, host, d, s->fsize(f) * s->es(), hipMemcpyDeviceToHost);
     if (s->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
       // departures -> full values (base added in FP64, rounded once)
       const gwm_state* o = s->view ? s->owner : s;
@@ -1746,11 +1780,11 @@ int gwm_state_download_f64(gwm_state* s, int f, double* host) {
     const size_t n = s->fsize(f);
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     if (s->es() == 8) {
-      HIPCHK(hipMemcpy(host, d, n * sizeof(double), hipMemcpyDeviceToHost));
+      scopy(s->ctx->stream, host, d, n * sizeof(double), hipMemcpyDeviceToHost);
       return;
     }
     std::vector<float> h(n);
-    HIPCHK(hipMemcpy(h.data(), d, n * sizeof(float), hipMemcpyDeviceToHost));
+    scopy(s->ctx->stream, h.data(), d, n * sizeof(float), hipMemcpyDeviceToHost);
     const bool dep = s->pert && (f == GWM_RHO || f == GWM_RHO_THETA);
     const gwm_state* o = s->view ? s->owner : s;
     const Geo& g = s->g();
@@ -1914,16 +1948,21 @@ int gwm_device_free(gwm_ctx* ctx, void* p) {
     if (p) HIPCHK(hipFree(p));
   });
 }
+// The context stream is non-blocking (it does not order with the null
+// stream), so the copies run on it: after every kernel the context queued
+// before them, complete when the call returns.
 int gwm_memcpy_h2d(gwm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return guard([&] {
     require(ctx, "null context");
-    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 int gwm_memcpy_d2h(gwm_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return guard([&] {
     require(ctx, "null context");
-    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 
@@ -2000,6 +2039,21 @@ int gwm_tracers_ssprk3(gwm_state* s, const gwm_step_config* c, int bmode) {
     require(!s->view, "operator not available on a stage view");
     dispatch(s->ctx, [&](auto r) {
       tracers_ssprk3<decltype(r)>(s, *c, bmode);
+      return 0;
+    });
+  });
+}
+
+int gwm_tracers_ssprk3_callback(gwm_state* s, const gwm_step_config* c, int bmode,
+                                gwm_boundary_fn fn, void* user) {
+  return guard([&] {
+    S_(s);
+    require(c, "null config");
+    require(!s->view, "operator not available on a stage view");
+    require(bmode == GWM_BOUNDARY_NONE || bmode == GWM_BOUNDARY_STRIPS,
+            "tracer boundary mode must be GWM_BOUNDARY_NONE or GWM_BOUNDARY_STRIPS");
+    dispatch(s->ctx, [&](auto r) {
+      tracers_ssprk3<decltype(r)>(s, *c, bmode, fn, user);
       return 0;
     });
   });
@@ -2086,7 +2140,8 @@ int gwm_init_constant(gwm_state* s, double rho, double th, double u, double v, d
       H.q.clear();
       std::vector<R>* f[5] = {&H.rho, &H.rt, &H.u, &H.v, &H.w};
       for (int a = 0; a < 5; ++a)
-        HIPCHK(hipMemcpy(s->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R), hipMemcpyHostToDevice));
+        scopy(s->ctx->stream, s->cur_set()[a], f[a]->data(), f[a]->size() * sizeof(R),
+              hipMemcpyHostToDevice);
       s->faces_synced = false;
       return 0;
     });
@@ -2314,9 +2369,10 @@ int gwm_surrogate_create(gwm_ctx* ctx, unsigned long long seed, gwm_surrogate** 
     M->T.grad = fa(gwm::kSurParams);
     M->T.m = fa(gwm::kSurParams);
     M->T.v = fa(gwm::kSurParams);
-    HIPCHK(hipMemcpy(M->T.params, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(M->T.m, 0, gwm::kSurParams * sizeof(float)));
-    HIPCHK(hipMemset(M->T.v, 0, gwm::kSurParams * sizeof(float)));
+    scopy(M->ctx->stream, M->T.params, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice);
+    HIPCHK(hipMemsetAsync(M->T.m, 0, gwm::kSurParams * sizeof(float), ctx->stream));
+    HIPCHK(hipMemsetAsync(M->T.v, 0, gwm::kSurParams * sizeof(float), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     M->nfall = static_cast<unsigned long long*>(dalloc(sizeof(unsigned long long)));
     *out = M.release();
   });

@@ -88,8 +88,9 @@ __device__ inline double rlog1p(double a) { return log1p(a); }
 // division and the library pow/exp/log1p/expm1, so the FP64 path matches the
 // FP64 oracle to ~1e-12.  FP32 uses the hardware reciprocal, base-2
 // logarithm and base-2 exponential (v_rcp_f32, v_log_f32, v_exp_f32, each
-// within 1 ulp): a power costs ~6 instructions instead of ~170 and a division
-// 2 instead of ~12, which is what the step's FP32 kernels are bound by.  Each
+// within 1 ulp): a power costs ~16 instructions instead of ~170 and a
+// division 2 instead of ~12, which is what the step's FP32 kernels are bound
+// by.  Each
 // result stays within a few ulp of the correctly rounded value (bounds
 // measured on the device by tests/test_fp32_math.py).  Arguments of qpow and
 // qlog are positive normal numbers (the callers floor them).
@@ -98,11 +99,41 @@ __device__ inline float qrcp(float b) { return __builtin_amdgcn_rcpf(b); }
 __device__ inline double qrcp(double b) { return 1.0 / b; }
 __device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ inline double qdiv(double a, double b) { return a / b; }
+// 2^t for t = hi + lo (|lo| << |hi|): the integer part of hi goes to the
+// exponent exactly (v_ldexp_f32), the hardware exp2 sees only the fraction,
+// so the rounding of a large t costs no accuracy.
+__device__ inline float qexp2_split(float hi, float lo) {
+  const float n = __builtin_rintf(hi);
+  const float f = (hi - n) + lo;  // hi - n is exact
+  return __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), static_cast<int>(n));
+}
+// x^y = 2^(y log2 x), x > 0 normal: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// so the hardware log2 works near 1 where its error is small in absolute
+// terms; y*e is carried with its exact rounding error (FMA) into the
+// fraction.  About 16 instructions, within ~2 ulp.
 __device__ inline float qpow(float x, float y) {
-  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  float m = __builtin_amdgcn_frexp_mantf(x);  // [0.5, 1)
+  int e = __builtin_amdgcn_frexp_expf(x);
+  if (m < 0.70710678118654752f) {
+    m *= 2.0f;
+    e -= 1;
+  }
+  const float l = __builtin_amdgcn_logf(m);  // log2 m in [-0.5, 0.5]
+  const float fe = static_cast<float>(e);
+  const float ye = y * fe;
+  const float ye_err = __builtin_fmaf(y, fe, -ye);
+  return qexp2_split(ye, __builtin_fmaf(y, l, ye_err));
 }
 __device__ inline double qpow(double x, double y) { return pow(x, y); }
-__device__ inline float qexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+// e^x = 2^(x log2 e) with the product carried to double length (FMA and
+// the low part of log2 e), within ~1 ulp.
+__device__ inline float qexp(float x) {
+  constexpr float L = 1.44269504088896341f;
+  constexpr float L_lo = 1.9259629891e-8f;  // log2(e) - L
+  const float t = x * L;
+  const float t_err = __builtin_fmaf(x, L, -t);
+  return qexp2_split(t, __builtin_fmaf(x, L_lo, t_err));
+}
 __device__ inline double qexp(double x) { return exp(x); }
 // log1p by Goldberg's correction: log(u) * x / (u - 1) with u = 1 + x
 // rounded, exact when u == 1 (accurate for |x| small, where 1 + x loses bits).
@@ -115,7 +146,7 @@ __device__ inline float qlog1p(float x) {
 __device__ inline double qlog1p(double x) { return log1p(x); }
 // expm1 by Kahan's correction: (u - 1) * y / log(u) with u = exp(y) rounded.
 __device__ inline float qexpm1(float y) {
-  const float u = __builtin_amdgcn_exp2f(y * 1.44269504088896341f);
+  const float u = qexp(y);
   if (u == 1.0f) return y;
   const float um1 = u - 1.0f;
   if (um1 == -1.0f) return -1.0f;

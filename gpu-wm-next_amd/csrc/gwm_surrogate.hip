@@ -455,6 +455,7 @@ __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
   if (i >= g.nx || j >= g.ny) return;
   const float dz_nom = fmaxf(depth, 1.0e-3f);
   const float frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
+  const float rdz_nom = 1.0f / dz_nom;
   float inc = 0.0f;
   for (int k = g.nz - 1; k >= 0; --k) {
     const size_t c = g.cell(i, j, k);
@@ -462,11 +463,11 @@ __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
     if (fallout == 1) {
       const float old = fmaxf(rqr, 0.0f) * dz_nom;
       const float out = frac_nom * old;
-      rqr = (old - out + inc) / dz_nom;
+      rqr = qdivc(old - out + inc, dz_nom, rdz_nom);
       inc = out;
     } else {
       const float idz = fmaxf(TERRAIN ? idz_c[g.metric(i, j, k)] : lev_idz_c[k], 1.0e-6f);
-      const float dz = 1.0f / idz;
+      const float dz = qrcp(idz);
       const float frac = fminf(fmaxf(vt * dt * idz, 0.0f), 1.0f);
       const float old = fmaxf(rqr, 0.0f) * dz;
       const float out = frac * old;

@@ -615,6 +615,50 @@ inline void download_tracers(gwm_state* s, std::vector<state::TracerState>& tr,
     gwm::detail::scatter(buf, L, G, 0, 0, 0, loc);
   }
 }
+
+// A host TracerBoundaryUpdater called from inside the device tracer stepper.
+struct TracerPluginCall {
+  const std::vector<state::TracerState>* shape = nullptr;
+  const std::vector<domain::SubdomainDescriptor>* layout = nullptr;
+  Global G;
+  TracerBoundaryUpdater* boundary = nullptr;
+  std::exception_ptr error;
+};
+
+// apply_tracer_boundaries_to_fields (src/dycore/dry_transport.cu:254-276):
+// each tracer of the stage goes to the updater on its own, as one-tracer
+// states whose registry holds only that tracer's spec.
+inline int tracer_boundary_trampoline(gwm_state* stage, double t, void* user) {
+  auto* c = static_cast<TracerPluginCall*>(user);
+  try {
+    const auto& shape = *c->shape;
+    std::vector<real> buf(c->G.size(0, 0, 0));
+    for (size_t q = 0; q < shape.front().size(); ++q) {
+      const int f = GWM_TRACER0 + static_cast<int>(q);
+      check(gwm_halo_fill(stage, f));
+      check(gwm_state_download(stage, f, buf.data()));
+      std::vector<state::TracerState> staged;
+      staged.reserve(shape.size());
+      for (const auto& x : shape) {
+        state::TracerRegistry reg;
+        reg.add(x.registry().at(static_cast<int>(q)));
+        staged.emplace_back(std::move(reg), x.nx(), x.ny(), x.nz(), x.halo(), "stage_tracer");
+      }
+      std::vector<state::Field3D<real>*> loc;
+      for (auto& x : staged) loc.push_back(&x.mass(0));
+      gwm::detail::scatter(buf, *c->layout, c->G, 0, 0, 0, loc);
+      c->boundary->apply(staged, *c->layout, static_cast<real>(t));
+      std::vector<const state::Field3D<real>*> cl;
+      for (const auto& x : staged) cl.push_back(&x.mass(0));
+      gwm::detail::gather(cl, *c->layout, c->G, 0, 0, 0, buf);
+      check(gwm_state_upload(stage, f, buf.data()));
+    }
+    return GWM_OK;
+  } catch (...) {
+    c->error = std::current_exception();
+    return GWM_E_INVALID;
+  }
+}
 }  // namespace detail
 
 inline std::vector<state::TracerState> make_specific_humidity_tracers_from_global_field(
@@ -637,9 +681,9 @@ inline std::vector<state::TracerState> make_warm_rain_tracers_from_global_fields
 }
 
 // advance_passive_tracers_ssprk3 (src/dycore/dry_transport.cu:440-518) with
-// the post-dry density and mass fluxes frozen over the step.  Only the null
-// tracer boundary is available through this overload; strip boundaries run
-// through gwm_tracers_ssprk3's boundary mode.
+// the post-dry density and mass fluxes frozen over the step.  The null
+// tracer boundary stays on the device; any other TracerBoundaryUpdater runs
+// on host copies of each stage input (gwm_tracers_ssprk3_callback).
 inline void advance_passive_tracers_ssprk3(std::vector<state::TracerState>& tracers,
                                            const std::vector<DryState>& dry_states,
                                            const std::vector<domain::SubdomainDescriptor>& layout,
@@ -648,16 +692,25 @@ inline void advance_passive_tracers_ssprk3(std::vector<state::TracerState>& trac
                                            TracerBoundaryUpdater& boundary_updater) {
   require(tracers.size() == layout.size(), "Tracer/layout size mismatch");
   if (tracers.empty() || tracers.front().size() == 0) return;
-  require(dynamic_cast<NullTracerBoundaryUpdater*>(&boundary_updater) != nullptr,
-          "advance_passive_tracers_ssprk3: only NullTracerBoundaryUpdater runs on the host "
-          "mirror; use gwm_tracers_ssprk3 with GWM_BOUNDARY_STRIPS for lateral strips");
   const auto G = gwm::detail::global_of(layout, metrics);
   const int ntr = static_cast<int>(tracers.front().size());
   gwm::detail::StateLease s(metrics.device.get(), ntr);
   detail::upload_dry(s, dry_states, layout, G);
   detail::upload_tracers(s, tracers, layout, G);
   const auto c = detail::cfg_of(config);
-  detail::check(gwm_tracers_ssprk3(s, &c, GWM_BOUNDARY_NONE));
+  if (dynamic_cast<NullTracerBoundaryUpdater*>(&boundary_updater) != nullptr) {
+    detail::check(gwm_tracers_ssprk3(s, &c, GWM_BOUNDARY_NONE));
+  } else {
+    detail::TracerPluginCall call;
+    call.shape = &tracers;
+    call.layout = &layout;
+    call.G = G;
+    call.boundary = &boundary_updater;
+    const int rc = gwm_tracers_ssprk3_callback(s, &c, GWM_BOUNDARY_NONE,
+                                               detail::tracer_boundary_trampoline, &call);
+    if (call.error) std::rethrow_exception(call.error);
+    detail::check(rc);
+  }
   detail::download_tracers(s, tracers, layout, G);
 }
 

@@ -293,6 +293,15 @@ int gwm_ssprk3_step_custom_fast(gwm_state* state, const gwm_step_config* cfg,
                                 void* fast_user);
 /* advance_passive_tracers_ssprk3 (src/dycore/dry_transport.cu:440-518). */
 int gwm_tracers_ssprk3(gwm_state* state, const gwm_step_config* cfg, int boundary_mode);
+/* advance_passive_tracers_ssprk3 with a host TracerBoundaryUpdater plug-in
+ * (include/gwm/dycore/passive_tracer.hpp:13-19): fn runs on a stage view
+ * whose tracer fields are the stage input of every tracer, at the
+ * step-relative times 0, dt and dt/2 (dry_transport.cu:485-508), after the
+ * strips of boundary_mode (GWM_BOUNDARY_NONE or GWM_BOUNDARY_STRIPS).  The
+ * tracers are independent, so one call per stage for all tracers stands for
+ * the reference's one call per stage and tracer. */
+int gwm_tracers_ssprk3_callback(gwm_state* state, const gwm_step_config* cfg,
+                                int boundary_mode, gwm_boundary_fn fn, void* user);
 /* apply_warm_rain_microphysics, both overloads (include/gwm/physics/warm_rain.hpp:57-68):
  * metric_fallout = 0 -> nominal fallout (warm_rain.cu:293-350),
  * metric_fallout = 1 -> layout/metric overload (warm_rain.cu:352-432). */

@@ -42,11 +42,11 @@ def signed_log_uniform(rng, n, lo, hi):
 CASES = [
     # fn, input sampler, float64 reference, max ulp, mean ulp
     (G.MATH_POW_EOS, lambda r, n: r.uniform(0.05, 2.0, n).astype(np.float32),
-     lambda x: np.power(x, np.float64(GAMMA)), 4.0, 1.0),
+     lambda x: np.power(x, np.float64(GAMMA)), 3.0, 1.0),
     (G.MATH_POW_EXNER, lambda r, n: r.uniform(0.05, 1.2, n).astype(np.float32),
      lambda x: np.power(x, np.float64(KAPPA)), 3.0, 1.0),
-    (G.MATH_EXP, lambda r, n: r.uniform(-20.0, 10.0, n).astype(np.float32), np.exp, 8.0, 2.0),
-    (G.MATH_LOG1P, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.5), np.log1p, 4.0, 1.0),
+    (G.MATH_EXP, lambda r, n: r.uniform(-20.0, 10.0, n).astype(np.float32), np.exp, 3.0, 1.0),
+    (G.MATH_LOG1P, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.5), np.log1p, 5.0, 1.0),
     (G.MATH_EXPM1, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.7), np.expm1, 4.0, 1.0),
     (G.MATH_RCP, lambda r, n: np.exp(r.uniform(np.log(1e-6), np.log(1e6), n)).astype(np.float32),
      lambda x: 1.0 / x, 1.0, 0.5),
