@@ -68,6 +68,8 @@ def make_case(G, precision, nx, ny, nz, ctx=None, perturbation=False):
 def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None, ctx=None,
         calibrate_copy=False, info=None, perturbation=False):
     ctx, grid, st, zi = make_case(G, precision, nx, ny, nz, ctx, perturbation)
+    if info is not None:
+        info["dry_mass_initial"] = st.summary()["total_dry_mass"]
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
 
@@ -195,10 +197,14 @@ def perturbation_leg(G, a, cells, sur_src):
     ctx = G.Context(0, G.F32)
     out = {"dt": dt}
     log("perturbation form: plain")
+    info = {}
     s, _, summ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt, ctx=ctx,
-                     perturbation=True)
+                     perturbation=True, info=info)
     out["value_plain"] = cells * a.steps / s
     out["state_finite"] = bool(np.isfinite(summ["w_max"]))
+    m0 = info["dry_mass_initial"]
+    out["dry_mass_rel_change"] = abs(summ["total_dry_mass"] - m0) / m0
+    out["dry_mass_steps"] = a.steps + a.warmup
     if sur_src is not None:
         params, norm = sur_src
         sur = G.Surrogate(ctx, 0)

@@ -89,15 +89,26 @@ __device__ inline double rlog1p(double a) { return log1p(a); }
 // FP64 oracle to ~1e-12.  FP32 uses the hardware reciprocal, base-2
 // logarithm and base-2 exponential (v_rcp_f32, v_log_f32, v_exp_f32, each
 // within 1 ulp): a power costs ~16 instructions instead of ~170 and a
-// division 2 instead of ~12, which is what the step's FP32 kernels are bound
+// division 4 instead of ~12, which is what the step's FP32 kernels are bound
 // by.  Each
 // result stays within a few ulp of the correctly rounded value (bounds
 // measured on the device by tests/test_fp32_math.py).  Arguments of qpow and
 // qlog are positive normal numbers (the callers floor them).
 // ---------------------------------------------------------------------------
-__device__ inline float qrcp(float b) { return __builtin_amdgcn_rcpf(b); }
+// Quotients take the hardware reciprocal r and one correction with the exact
+// residual e = a - q b (FMA): q + e r is the correctly rounded a / b except
+// in rare near-tie cases, 4 instructions instead of ~12 (normal operands; the
+// callers floor the divisors).
+__device__ inline float qrcp(float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  return __builtin_fmaf(__builtin_fmaf(-r, b, 1.0f), r, r);
+}
 __device__ inline double qrcp(double b) { return 1.0 / b; }
-__device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline float qdiv(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, b, a), r, q);
+}
 __device__ inline double qdiv(double a, double b) { return a / b; }
 // 2^t for t = hi + lo (|lo| << |hi|): the integer part of hi goes to the
 // exponent exactly (v_ldexp_f32), the hardware exp2 sees only the fraction,
@@ -154,10 +165,10 @@ __device__ inline float qexpm1(float y) {
 }
 __device__ inline double qexpm1(double y) { return expm1(y); }
 // Division by a uniform grid step d with its reciprocal rd = 1 / d (correctly
-// rounded): FP32 multiplies, FP64 divides.
+// rounded): FP32 multiplies and corrects with the residual, FP64 divides.
 __device__ inline float qdivc(float x, float d, float rd) {
-  (void)d;
-  return x * rd;
+  const float q = x * rd;
+  return __builtin_fmaf(__builtin_fmaf(-q, d, x), rd, q);
 }
 __device__ inline double qdivc(double x, double d, double rd) {
   (void)rd;

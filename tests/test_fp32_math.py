@@ -49,7 +49,7 @@ CASES = [
     (G.MATH_LOG1P, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.5), np.log1p, 5.0, 1.0),
     (G.MATH_EXPM1, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.7), np.expm1, 4.0, 1.0),
     (G.MATH_RCP, lambda r, n: np.exp(r.uniform(np.log(1e-6), np.log(1e6), n)).astype(np.float32),
-     lambda x: 1.0 / x, 1.0, 0.5),
+     lambda x: 1.0 / x, 0.501, 0.26),
 ]
 
 
