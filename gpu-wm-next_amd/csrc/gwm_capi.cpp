@@ -1510,6 +1510,8 @@ int gwm_grid_create(gwm_ctx* ctx, const gwm_grid_desc* d, const double* z_interf
   return guard([&] {
     require(ctx && d && out, "null argument");
     require(d->nx > 0 && d->ny > 0 && d->nz > 0, "grid dimensions must be positive");
+    // the column kernels stage per-level values of a whole column in LDS
+    require(d->nz <= 512, "at most 512 levels");
     require(d->halo >= 1, "halo must be at least 1");
     require(d->dx > 0 && d->dy > 0, "grid spacing must be positive");
     require(d->halo <= d->nx && d->halo <= d->ny, "halo wider than the domain");

@@ -52,10 +52,19 @@ __device__ inline double wave_sum(double v) {
 
 }  // namespace
 
-template <class R, bool TERRAIN, bool COLREF, bool PERT>
-__global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
-                                                     int nkc, double* part_out) {
+// Per-level values of a chunk (levels k0-1 .. k1+1), staged in LDS once per
+// work-group so the level loop issues no global loads besides the pipelined
+// planes and faces.
+enum { LV_PREF = 0, LV_RREF, LV_IDZF, LV_IDZC, LV_BRHO, LV_BTH, LV_BRT, LV_BP, NLV };
+
+// KIND: bit 0 first substep of the subcycle, bit 1 last substep.
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
+__global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
+                                                         int nkc, double* part_out) {
+  constexpr bool FIRST = (KIND & 1) != 0, LAST = (KIND & 2) != 0;
   __shared__ R sp[4][PN];
+  extern __shared__ __align__(16) unsigned char lev_raw[];
+  R* const lev = reinterpret_cast<R*>(lev_raw);
   const Geo g = a.g;
   const Loader<R> L{g};
   const int nz = g.nz;
@@ -85,57 +94,89 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   }
   const int hci = (hx + 1) + PX * (hy + 1);
 
+  // Per-level values of levels k0-1 .. k1+1 (indices clamped to the arrays).
+  for (int q = t; q < k1 - k0 + 3; q += TX * TY) {
+    const int kk = k0 - 1 + q;
+    const int kc = kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk), kf = kk < 0 ? 0 : (kk > nz ? nz : kk);
+    R* e = lev + q * NLV;
+    if (!COLREF) {
+      e[LV_PREF] = a.pref_lev[kc];
+      e[LV_RREF] = a.rref_lev[kc];
+    }
+    if (!TERRAIN) {
+      e[LV_IDZF] = a.m.lev_idz_f[kf];
+      e[LV_IDZC] = a.m.lev_idz_c[kc];
+    }
+    if (PERT) {
+      e[LV_BRHO] = a.base.rho[kc];
+      e[LV_BTH] = a.base.th[kc];
+      e[LV_BRT] = a.base.rt[kc];
+      e[LV_BP] = a.base.p[kc];
+    }
+  }
+  auto LV = [&](int kk, int q) -> R { return lev[(kk - k0 + 1) * NLV + q]; };
+  __syncthreads();
+
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
   const R rdx = R(1) / dx, rdy = R(1) / dy;
   const R dtf = a.dtf, grav = a.gravity;
 
-  // Raw loads of one plane: own cell and (t < NHALO) one ring cell.
-  R rr_c, rt_c, rr_h = R(0), rt_h = R(0);
+  // Loads are unconditional (threads without a ring cell, or outside the
+  // domain, read a valid stand-in and discard it), so the level loop is
+  // straight-line code and each wait covers exactly the loads it needs:
+  // the loads of level k+3 planes and k+1 faces stay in flight while level
+  // k is computed.
   const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;
-  const int oc = stage_pos(g, i, j).c, hc = stage_pos(g, ti0 + hx, tj0 + hy).c;
-  auto load_plane = [&](int k) {
-    rr_c = ld_at(a.rho, oc, pc, k);
-    rt_c = ld_at(a.theta, oc, pc, k);
-    if (t < NHALO) {
-      rr_h = ld_at(a.rho, hc, pc, k);
-      rt_h = ld_at(a.theta, hc, pc, k);
-    }
+  const int oc = stage_pos(g, i, j).c;
+  const int hc = t < NHALO ? stage_pos(g, ti0 + hx, tj0 + hy).c : oc;
+  auto ldc = [&](const R* f, int off, int k) -> R {
+    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(k) * pc];
+    return off < 0 ? R(0) : v;
+  };
+  struct Plane {
+    R rc, tc, rh, th;
+  };
+  auto load_plane = [&](int k) -> Plane {
+    return Plane{ldc(a.rho, oc, k), ldc(a.theta, oc, k), ldc(a.rho, hc, k), ldc(a.theta, hc, k)};
   };
   // Perturbation form: rho' and theta' in, p' out (K10 rho*theta = rho*theta_fixed
   // written for the departures: rho_b theta' + rho' (theta_b + theta')).
-  constexpr bool pert = PERT;
   auto press = [&](R r, R th, int k) -> R {
-    if (!pert) return q_pressure(r, r * th);
-    const R rb = a.base.rho[k], tb = a.base.th[k];
-    return p_pert(rb * th + r * (tb + th), a.base.rt[k], a.base.p[k]);
+    if (!PERT) return q_pressure(r, r * th);
+    const R rb = LV(k, LV_BRHO), tb = LV(k, LV_BTH);
+    return p_pert(rb * th + r * (tb + th), LV(k, LV_BRT), LV(k, LV_BP));
   };
-  auto stage_plane = [&](int k) {
-    sp[k & 3][ci] = press(rr_c, rt_c, k);
-    if (t < NHALO) sp[k & 3][hci] = press(rr_h, rt_h, k);
+  auto stage_plane = [&](const Plane& P, int k) {
+    sp[k & 3][ci] = press(P.rc, P.tc, k);
+    if (t < NHALO) sp[k & 3][hci] = press(P.rh, P.th, k);
   };
 
-  // Face departures m - m_base of one level for the own cell (prefetched a
-  // level ahead): zero on the first substep (the faces are the base faces),
-  // read from the previous substep's output otherwise.  The base faces are
-  // read on the last substep only, which writes full faces.
-  R f_u0 = 0, f_u1 = 0, f_v0 = 0, f_v1 = 0, f_w1 = 0, f_ub0 = 0, f_ub1 = 0, f_vb0 = 0,
-    f_vb1 = 0, f_wb1 = 0;
-  auto load_faces = [&](int k) {
-    const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k), zf1 = g.zf(i, j, k + 1);
-    if (!a.first) {
-      f_u0 = a.u[xf0];
-      f_u1 = a.u[xf0 + 1];
-      f_v0 = a.v[yf0];
-      f_v1 = a.v[yf0 + g.pxc];
-      f_w1 = a.w[zf1];
+  // Face departures m - m_base of one level of the own cell: none on the
+  // first substep (the faces are the base faces), the previous substep's
+  // output otherwise; the base faces on the last substep only, which writes
+  // full faces.  Indices clamped into the domain for threads outside it.
+  const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  struct Faces {
+    R u0, u1, v0, v1, w1, ub0, ub1, vb0, vb1, wb1;
+  };
+  auto load_faces = [&](int k) -> Faces {
+    Faces F{};
+    const size_t xf0 = g.xf(ic, jc, k), yf0 = g.yf(ic, jc, k), zf1 = g.zf(ic, jc, k + 1);
+    if constexpr (!FIRST) {
+      F.u0 = a.u[xf0];
+      F.u1 = a.u[xf0 + 1];
+      F.v0 = a.v[yf0];
+      F.v1 = a.v[yf0 + g.pxc];
+      F.w1 = a.w[zf1];
     }
-    if (a.last) {
-      f_ub0 = a.ub[xf0];
-      f_ub1 = a.ub[xf0 + 1];
-      f_vb0 = a.vb[yf0];
-      f_vb1 = a.vb[yf0 + g.pxc];
-      f_wb1 = a.wb[zf1];
+    if constexpr (LAST) {
+      F.ub0 = a.ub[xf0];
+      F.ub1 = a.ub[xf0 + 1];
+      F.vb0 = a.vb[yf0];
+      F.vb1 = a.vb[yf0 + g.pxc];
+      F.wb1 = a.wb[zf1];
     }
+    return F;
   };
 
   // Metric column index (periodic wrap of global i, j; dry_fast_modes.cu:24-30).
@@ -145,26 +186,36 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   const size_t plane_m = static_cast<size_t>(g.nx) * g.ny;
   const size_t own_m = active ? mcol(i, j) : 0;
 
-  // ---- prologue: planes k0-1 .. k0+1, own column values, bottom w face.
+  // ---- prologue: planes k0-1 .. k0+1 staged, plane k0+2 and faces of k0
+  // in flight, bottom w face.
+  const int kmax = nz - 1 < k1 ? nz - 1 : k1;  // highest plane a chunk stages
   R rho_km1 = R(0);
   if (k0 > 0) {
-    load_plane(k0 - 1);
-    stage_plane(k0 - 1);
-    rho_km1 = rr_c;
+    const Plane P = load_plane(k0 - 1);
+    stage_plane(P, k0 - 1);
+    rho_km1 = P.rc;
   }
   R rho_k, th_k, rho_k1 = R(0), th_k1 = R(0);
-  load_plane(k0);
-  stage_plane(k0);
-  rho_k = rr_c;
-  th_k = rt_c;
-  if (k0 + 1 < nz) {
-    load_plane(k0 + 1);
-    stage_plane(k0 + 1);
-    rho_k1 = rr_c;
-    th_k1 = rt_c;
+  {
+    const Plane P = load_plane(k0);
+    stage_plane(P, k0);
+    rho_k = P.rc;
+    th_k = P.tc;
   }
-  if (k0 + 2 < nz && k0 + 2 <= k1) load_plane(k0 + 2);
-  if (active) load_faces(k0);
+  if (k0 + 1 < nz) {
+    const Plane P = load_plane(k0 + 1);
+    stage_plane(P, k0 + 1);
+    rho_k1 = P.rc;
+    th_k1 = P.tc;
+  }
+  Plane P2 = load_plane(k0 + 2 < kmax ? k0 + 2 : kmax);
+  Faces F = load_faces(k0);
+  R w0, wb0;
+  {
+    const size_t zf0 = g.zf(ic, jc, k0);
+    w0 = FIRST ? R(0) : a.w[zf0];
+    wb0 = LAST ? a.wb[zf0] : R(0);
+  }
   __syncthreads();
 
   // Column reference (K11) at level k0 (whole columns only: k0 == 0).
@@ -176,35 +227,26 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
   }
 
   const bool ow = !g.per_x, os = !g.per_y;
-  R w0 = R(0), wb0 = R(0);
-  if (active) {
-    const size_t zf0 = g.zf(i, j, k0);
-    w0 = a.first ? R(0) : a.w[zf0];
-    wb0 = a.last ? a.wb[zf0] : R(0);
-    if (k0 > 0) {
-      // K14 at face k0 from levels k0-1 and k0 (flat references).
-      const R ppl = sp[(k0 - 1) & 3][ci] - a.pref_lev[k0 - 1];
-      const R ppu = sp[k0 & 3][ci] - a.pref_lev[k0];
-      const R rpf = R(0.5) * ((rho_km1 - a.rref_lev[k0 - 1]) + (rho_k - a.rref_lev[k0]));
-      const R idz = TERRAIN ? a.m.idz_f[own_m + k0 * plane_m] : a.m.lev_idz_f[k0];
-      const R tend = -(ppu - ppl) * idz - grav * rpf;
-      w0 = w0 + dtf * tend;
-    }
+  if (active && k0 > 0) {
+    // K14 at face k0 from levels k0-1 and k0 (flat references).
+    const R ppl = sp[(k0 - 1) & 3][ci] - LV(k0 - 1, LV_PREF);
+    const R ppu = sp[k0 & 3][ci] - LV(k0, LV_PREF);
+    const R rpf = R(0.5) * ((rho_km1 - LV(k0 - 1, LV_RREF)) + (rho_k - LV(k0, LV_RREF)));
+    const R idz = TERRAIN ? a.m.idz_f[own_m + k0 * plane_m] : LV(k0, LV_IDZF);
+    const R tend = -(ppu - ppl) * idz - grav * rpf;
+    w0 = w0 + dtf * tend;
   }
 
-  for (int k = k0; k < k1; ++k) {
-    // Stage level k+2 (loaded last iteration) and start loading level k+3.
-    R rho_k2 = R(0), th_k2 = R(0);
-    if (k + 2 < nz && k + 2 <= k1) {
-      stage_plane(k + 2);
-      rho_k2 = rr_c;
-      th_k2 = rt_c;
-      if (k + 3 < nz && k + 3 <= k1) load_plane(k + 3);
-    }
-    // Faces of this level; start loading the next level's.
-    const R uc0 = f_u0, uc1 = f_u1, vc0 = f_v0, vc1 = f_v1, wc1 = f_w1;
-    const R ub0 = f_ub0, ub1 = f_ub1, vb0 = f_vb0, vb1 = f_vb1, wb1 = f_wb1;
-    if (active && k + 1 < k1) load_faces(k + 1);
+  // Level k: P2 / F hold plane k+2 and the faces of level k (loaded an
+  // iteration ago); the loads of plane k+3 and of the faces of level k+1 go
+  // into P3 / Fn first.  The loop is unrolled by two with the buffers
+  // swapped, so no register copy waits for loads in flight.
+  auto level = [&](int k, const Plane& P2, const Faces& F, Plane& P3, Faces& Fn) {
+    P3 = load_plane(k + 3 < kmax ? k + 3 : kmax);
+    Fn = load_faces(k + 1 < k1 ? k + 1 : k1 - 1);
+    // Stage plane k+2.
+    if (k + 2 <= kmax) stage_plane(P2, k + 2);
+    const R rho_k2 = P2.rc, th_k2 = P2.tc;
 
     const R* pk = sp[k & 3];
     const R* pdn = sp[(k + 3) & 3];  // level k-1
@@ -251,54 +293,54 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
         return qdivc(pr - pl, d, rd) - dzd * (R(0.5) * (gl + gr));
       };
       // K12 / K13: faces i, i+1 and j, j+1 of the cell.
-      R u0 = uc0, u1 = uc1, v0 = vc0, v1 = vc1;
+      R u0 = F.u0, u1 = F.u1, v0 = F.v0, v1 = F.v1;
       if (a.horizontal) {
-        if (!(i == 0 && ow)) u0 = uc0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx, rdx));
-        if (!(i + 1 == g.nx && ow)) u1 = uc1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx, rdx));
-        if (!(j == 0 && os)) v0 = vc0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy));
-        if (!(j + 1 == g.ny && os)) v1 = vc1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy));
+        if (!(i == 0 && ow)) u0 = F.u0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx, rdx));
+        if (!(i + 1 == g.nx && ow)) u1 = F.u1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx, rdx));
+        if (!(j == 0 && os)) v0 = F.v0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy));
+        if (!(j + 1 == g.ny && os)) v1 = F.v1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy));
       }
       // K14: w face k+1 (faces 0 and nz are left as they are).
-      R w1 = wc1;
+      R w1 = F.w1;
       if (k + 1 < nz) {
         R rl, pl, ru, pu;
         if (COLREF) {
           rl = rref_k; pl = pref_k; ru = rref_k1; pu = pref_k1;
         } else {
-          rl = a.rref_lev[k]; pl = a.pref_lev[k]; ru = a.rref_lev[k + 1]; pu = a.pref_lev[k + 1];
+          rl = LV(k, LV_RREF); pl = LV(k, LV_PREF); ru = LV(k + 1, LV_RREF); pu = LV(k + 1, LV_PREF);
         }
         const R ppl = pk[ci] - pl;
         const R ppu = pup[ci] - pu;
         const R rpf = R(0.5) * ((rho_k - rl) + (rho_k1 - ru));
-        const R idz = TERRAIN ? a.m.idz_f[own_m + (k + 1) * plane_m] : a.m.lev_idz_f[k + 1];
+        const R idz = TERRAIN ? a.m.idz_f[own_m + (k + 1) * plane_m] : LV(k + 1, LV_IDZF);
         const R tend = -(ppu - ppl) * idz - grav * rpf;
-        w1 = wc1 + dtf * tend;
+        w1 = F.w1 + dtf * tend;
       }
       // K15
       rho_new = rho_k;
       if (a.density) {
-        const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : a.m.lev_idz_c[k];
+        const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : LV(k, LV_IDZC);
         // divergence of (m - m_base): the faces are departures already
         const R div = qdivc(u1 - u0, dx, rdx) + qdivc(v1 - v0, dy, rdy) + (w1 - w0) * idz;
-        const R floor = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
+        const R floor = PERT ? R(1.0e-6) - LV(k, LV_BRHO) : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
       const size_t c0 = g.cell(i, j, k);
       const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k);
       a.rho_out[c0] = rho_new;
       if (a.rt_out)
-        a.rt_out[c0] = pert ? a.base.rho[k] * th_k + rho_new * (a.base.th[k] + th_k)
+        a.rt_out[c0] = PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k)
                             : rho_new * th_k;
       // departures, or full faces base + departure on the last substep
-      a.u_out[xf0] = a.last ? ub0 + u0 : u0;
-      a.v_out[yf0] = a.last ? vb0 + v0 : v0;
-      a.w_out[c0] = a.last ? wb0 + w0 : w0;
-      if (i == g.nx - 1) a.u_out[xf0 + 1] = a.last ? ub1 + u1 : u1;
-      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = a.last ? vb1 + v1 : v1;
-      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = a.last ? wb1 + w1 : w1;
+      a.u_out[xf0] = LAST ? F.ub0 + u0 : u0;
+      a.v_out[yf0] = LAST ? F.vb0 + v0 : v0;
+      a.w_out[c0] = LAST ? wb0 + w0 : w0;
+      if (i == g.nx - 1) a.u_out[xf0 + 1] = LAST ? F.ub1 + u1 : u1;
+      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = LAST ? F.vb1 + v1 : v1;
+      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = LAST ? F.wb1 + w1 : w1;
       w0 = w1;
-      wb0 = wb1;
     }
+    wb0 = F.wb1;
 
     if (part_out) {  // level sums of the next substep (flat refs, K6/K7)
       double sr = 0.0, spp = 0.0;
@@ -309,8 +351,8 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
       sr = wave_sum(sr);
       spp = wave_sum(spp);
       if (lx == 0) {
-        const size_t nw = static_cast<size_t>(tiles) * 4;
-        const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * 4 + ly) * 2;
+        const size_t nw = static_cast<size_t>(tiles) * TY;  // one wave per tile row
+        const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * TY + ly) * 2;
         part_out[slot] = sr;
         part_out[slot + 1] = spp;
       }
@@ -323,6 +365,36 @@ __global__ __launch_bounds__(256) void k_fast_column(FastArgs<R> a, int tiles_x,
     pref_k = pref_k1;
     rref_k = rref_k1;
     __syncthreads();
+  };
+  Plane P3;
+  Faces Fn;
+  for (int k = k0; k < k1; k += 2) {
+    level(k, P2, F, P3, Fn);
+    if (k + 1 < k1) level(k + 1, P3, Fn, P2, F);
+  }
+}
+
+template <class R, bool TERRAIN, bool COLREF, bool PERT>
+void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, dim3 grid, dim3 blk, size_t lds,
+                      int tx, int tiles, int nkc, double* part_out) {
+  const int kind = (a.first ? 1 : 0) | (a.last ? 2 : 0);
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0>), grid, blk, lds, st, a, tx,
+                         tiles, nkc, part_out);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 1>), grid, blk, lds, st, a, tx,
+                         tiles, nkc, part_out);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 2>), grid, blk, lds, st, a, tx,
+                         tiles, nkc, part_out);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 3>), grid, blk, lds, st, a, tx,
+                         tiles, nkc, part_out);
+      break;
   }
 }
 
@@ -331,31 +403,28 @@ void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int 
                         double* part_out, int* nparts_out) {
   const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TY - 1) / TY;
   const int tiles = tx * ty;
-  if (nparts_out) *nparts_out = tiles * 4;
+  if (nparts_out) *nparts_out = tiles * TY;
   if (a.col_ref || nkc < 1) nkc = 1;
   if (nkc > a.g.nz) nkc = a.g.nz;
   const dim3 grid(tiles * nkc), blk(TX, TY, 1);
+  // per-level LDS: the longest chunk plus the levels below and above it
+  const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * NLV * sizeof(R);
   if (a.col_ref) {
     if (terrain)
-      hipLaunchKernelGGL((k_fast_column<R, true, true, false>), grid, blk, 0, st, a, tx, tiles, 1,
-                         nullptr);
+      launch_fast_kind<R, true, true, false>(st, a, grid, blk, lds, tx, tiles, 1, nullptr);
     else
-      hipLaunchKernelGGL((k_fast_column<R, false, true, false>), grid, blk, 0, st, a, tx, tiles, 1,
-                         nullptr);
+      launch_fast_kind<R, false, true, false>(st, a, grid, blk, lds, tx, tiles, 1, nullptr);
   } else if (terrain) {
-    hipLaunchKernelGGL((k_fast_column<R, true, false, false>), grid, blk, 0, st, a, tx, tiles, nkc,
-                       part_out);
+    launch_fast_kind<R, true, false, false>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
   } else if (a.base.rho) {  // perturbation form (flat, doubly periodic)
-    hipLaunchKernelGGL((k_fast_column<R, false, false, true>), grid, blk, 0, st, a, tx, tiles,
-                       nkc, part_out);
+    launch_fast_kind<R, false, false, true>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
   } else {
-    hipLaunchKernelGGL((k_fast_column<R, false, false, false>), grid, blk, 0, st, a, tx, tiles,
-                       nkc, part_out);
+    launch_fast_kind<R, false, false, false>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
   }
 }
 
 int fast_column_parts(int nx, int ny) {
-  return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * 4;
+  return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * TY;
 }
 
 template void launch_fast_column<float>(hipStream_t, const FastArgs<float>&, bool, int, double*,

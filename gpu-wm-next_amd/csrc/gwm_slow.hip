@@ -16,8 +16,8 @@
 // quantities every flux needs -- theta, p (dry_thermo.hpp:21-33) and the cell
 // velocities (dry_momentum_flux.cu:36-73) -- into LDS once, instead of
 // recomputing each of them at every use (a cell's p and velocities enter up
-// to a dozen face fluxes).  Four LDS slots per quantity hold levels k-1 ..
-// k+1 while level k+1 is staged one iteration ahead: one barrier per level.
+// to a dozen face fluxes).  Three LDS slots per quantity hold levels k-1 ..
+// k+1 (level k+1 staged at the top of iteration k); two barriers per level.
 // The column is split into level chunks (count chosen by the autotuner) so
 // the chip holds more independent work-groups; a chunk stages the level
 // below its first one.
@@ -57,14 +57,20 @@ __device__ inline R upwind(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
 }  // namespace
 
 template <class R, bool TERRAIN, bool PERT>
-__global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
+__global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
                                                      int nkc) {
   // FP32 also stages the raw face momenta of each level (tile plus ring), so
   // the upwind values of every flux come from LDS; FP64 reads them from
   // global memory (its LDS budget would halve the resident work-groups).
+  //
+  // LDS holds three level slots (k-1, k, k+1; slot kk % 3) and each level
+  // takes two barriers: one before level k+1 overwrites the slot of k-2, one
+  // before the reads.  A fourth slot would save the first barrier but cost a
+  // quarter more LDS, and with it a resident work-group per CU.
   constexpr bool RAW = sizeof(R) == 4;
-  constexpr int RS = RAW ? 4 : 1, RN = RAW ? PN : 1;
-  __shared__ R sTH[4][PN], sP[4][PN], sUC[4][PN], sVC[4][PN], sWC[4][PN];
+  constexpr int NSL = 3;
+  constexpr int RS = RAW ? NSL : 1, RN = RAW ? PN : 1;
+  __shared__ R sTH[NSL][PN], sP[NSL][PN], sUC[NSL][PN], sVC[NSL][PN], sWC[NSL][PN];
   __shared__ R sU[RS][RN], sV[RS][RN], sW[RS][RN];
   const Geo g = a.g;
   const Loader<R> L{g};
@@ -111,20 +117,20 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
     if (pert) {  // theta' and p' of the departures
       const R rb = a.base.rho[kk];
       rr = rmax(rb + r, R(1.0e-6));
-      sTH[kk & 3][e] = theta_pert(r, rt, rb, a.base.th[kk]);
-      sP[kk & 3][e] = p_pert(rt, a.base.rt[kk], a.base.p[kk]);
+      sTH[kk % NSL][e] = theta_pert(r, rt, rb, a.base.th[kk]);
+      sP[kk % NSL][e] = p_pert(rt, a.base.rt[kk], a.base.p[kk]);
     } else {
       rr = rmax(r, R(1.0e-6));
-      sTH[kk & 3][e] = q_theta(r, rt);
-      sP[kk & 3][e] = q_pressure(r, rt);
+      sTH[kk % NSL][e] = q_theta(r, rt);
+      sP[kk % NSL][e] = q_pressure(r, rt);
     }
-    sUC[kk & 3][e] = qdiv(R(0.5) * (u0 + u1), rr);
-    sVC[kk & 3][e] = qdiv(R(0.5) * (v0 + v1), rr);
-    sWC[kk & 3][e] = qdiv(R(0.5) * (w0 + w1), rr);
+    sUC[kk % NSL][e] = qdiv(R(0.5) * (u0 + u1), rr);
+    sVC[kk % NSL][e] = qdiv(R(0.5) * (v0 + v1), rr);
+    sWC[kk % NSL][e] = qdiv(R(0.5) * (w0 + w1), rr);
     if constexpr (RAW) {
-      sU[kk & 3][e] = u0;
-      sV[kk & 3][e] = v0;
-      sW[kk & 3][e] = w0;
+      sU[kk % NSL][e] = u0;
+      sV[kk % NSL][e] = v0;
+      sW[kk % NSL][e] = w0;
     }
   };
   // Own-column rho (levels k-1, k, k+1), rho*theta (k, k+1) and the top w face.
@@ -154,30 +160,31 @@ __global__ __launch_bounds__(256) void k_slow_column(SlowArgs<R> a, int tiles_x,
   r_k = r_k1;
   rt_k = rt_k1;
   for (int k = k0; k < k1; ++k) {
+    __syncthreads();
     if (k + 1 < nz) stage(k + 1);
     __syncthreads();
     if (active) {
       // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
       auto at = [&](int ii, int jj) { return (ii - ti0 + 1) + PX * (jj - tj0 + 1); };
-      auto TH = [&](int ii, int jj, int kk) { return sTH[kk & 3][at(ii, jj)]; };
-      auto P = [&](int ii, int jj, int kk) { return sP[kk & 3][at(ii, jj)]; };
-      auto UC = [&](int ii, int jj, int kk) { return sUC[kk & 3][at(ii, jj)]; };
-      auto VC = [&](int ii, int jj, int kk) { return sVC[kk & 3][at(ii, jj)]; };
-      auto WC = [&](int ii, int jj, int kk) { return sWC[kk & 3][at(ii, jj)]; };
+      auto TH = [&](int ii, int jj, int kk) { return sTH[kk % NSL][at(ii, jj)]; };
+      auto P = [&](int ii, int jj, int kk) { return sP[kk % NSL][at(ii, jj)]; };
+      auto UC = [&](int ii, int jj, int kk) { return sUC[kk % NSL][at(ii, jj)]; };
+      auto VC = [&](int ii, int jj, int kk) { return sVC[kk % NSL][at(ii, jj)]; };
+      auto WC = [&](int ii, int jj, int kk) { return sWC[kk % NSL][at(ii, jj)]; };
       // Raw face momenta: staged (FP32) or global; Ug / Vg for the domain-edge
       // faces nx / ny, whose outer neighbour lies beyond the ring.
       auto Ug = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
       auto Vg = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
       auto U = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sU[kk & 3][at(ii, jj)];
+        if constexpr (RAW) return sU[kk % NSL][at(ii, jj)];
         else return L.xf(s.u, ii, jj, kk);
       };
       auto V = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sV[kk & 3][at(ii, jj)];
+        if constexpr (RAW) return sV[kk % NSL][at(ii, jj)];
         else return L.yf(s.v, ii, jj, kk);
       };
       auto W = [&](int ii, int jj, int kk) -> R {  // z-face kk < nz
-        if constexpr (RAW) return sW[kk & 3][at(ii, jj)];
+        if constexpr (RAW) return sW[kk % NSL][at(ii, jj)];
         else return L.zf(s.w, ii, jj, kk);
       };
       const R w_up = k + 1 < nz ? W(i, j, k + 1) : w_top;  // z-face k+1 of the own cell
@@ -409,7 +416,7 @@ template void launch_slow_column<double>(hipStream_t, const SlowArgs<double>&, b
 // out = wa*q0 + wb*(...) (dry_transport.cu:40-47).
 // ===========================================================================
 template <class R, bool TERRAIN, int NT>
-__global__ __launch_bounds__(256) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
+__global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
                                                        int nkc) {
   __shared__ R sQ[NT][4][PN];
   const Geo g = a.g;
