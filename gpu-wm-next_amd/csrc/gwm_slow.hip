@@ -56,7 +56,11 @@ __device__ inline R upwind(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
 
 }  // namespace
 
-template <class R, bool TERRAIN, bool PERT>
+// Per-level values of a chunk (levels k0-1 .. k1+1), staged in LDS once per
+// work-group.
+enum { SL_TREF = 0, SL_IDZF, SL_IDZC, SL_BRHO, SL_BTH, SL_BRT, SL_BP, SL_N };
+
+template <class R, bool TERRAIN, bool PERT, int MODE>
 __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
                                                      int nkc) {
   // FP32 also stages the raw face momenta of each level (tile plus ring), so
@@ -104,44 +108,93 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   const StagePos ring = stage_pos(g, ti0 + hx, tj0 + hy);
   const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
 
-  // Stages level kk at LDS entry e; returns the raw rho, rho*theta and the
-  // w face above the cell.
-  auto stage_one = [&](const StagePos& o, int e, int kk, R& r, R& rt, R& w1) {
-    r = ld_at(s.rho, o.c, pc, kk);
-    rt = ld_at(s.rt, o.c, pc, kk);
-    const R u0 = ld_at(s.u, o.u0, pu, kk), u1 = ld_at(s.u, o.u1, pu, kk);
-    const R v0 = ld_at(s.v, o.v0, pv, kk), v1 = ld_at(s.v, o.v1, pv, kk);
-    const R w0 = ld_at(s.w, o.c, pc, kk);
-    w1 = ld_at(s.w, o.c, pc, kk + 1);
+  // Per-level values of levels k0-1 .. k1+1 (indices clamped to the arrays).
+  extern __shared__ __align__(16) unsigned char lev_raw[];
+  R* const lev = reinterpret_cast<R*>(lev_raw);
+  for (int q = t; q < k1 - k0 + 3; q += TX * TY) {
+    const int kk = k0 - 1 + q;
+    const int kc = kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk), kf = kk < 0 ? 0 : (kk > nz ? nz : kk);
+    R* e = lev + q * SL_N;
+    e[SL_TREF] = a.theta_ref[kc];
+    if (!TERRAIN) {
+      e[SL_IDZF] = a.m.lev_idz_f[kf];
+      e[SL_IDZC] = a.m.lev_idz_c[kc];
+    }
+    if (PERT) {
+      e[SL_BRHO] = a.base.rho[kc];
+      e[SL_BTH] = a.base.th[kc];
+      e[SL_BRT] = a.base.rt[kc];
+      e[SL_BP] = a.base.p[kc];
+    }
+  }
+  auto LV = [&](int kk, int q) -> R { return lev[(kk - k0 + 1) * SL_N + q]; };
+  __syncthreads();
+
+  // Raw values of one level at a staged position.  Loads are unconditional
+  // (threads without a ring cell, or outside the domain, read a valid
+  // stand-in and discard it), so the level loop is straight-line code and
+  // each wait covers exactly the loads it needs: the loads of level k+2 (and
+  // of the SSP base of level k+1) stay in flight while level k is computed.
+  struct Raw {
+    R r, rt, u0, u1, v0, v1, w0, w1;
+  };
+  struct Lev {
+    Raw o, h;  // own cell, ring cell
+  };
+  auto ldo = [&](const R* f, int off, unsigned plane, int kk) -> R {
+    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * plane];
+    return off < 0 ? R(0) : v;
+  };
+  const StagePos hpos = t < NHALO ? ring : own;
+  auto load_raw = [&](const StagePos& o, int kk) -> Raw {
+    return Raw{ldo(s.rho, o.c, pc, kk),  ldo(s.rt, o.c, pc, kk),  ldo(s.u, o.u0, pu, kk),
+               ldo(s.u, o.u1, pu, kk),   ldo(s.v, o.v0, pv, kk),  ldo(s.v, o.v1, pv, kk),
+               ldo(s.w, o.c, pc, kk),    ldo(s.w, o.c, pc, kk + 1)};
+  };
+  auto load_lev = [&](int kk) -> Lev { return Lev{load_raw(own, kk), load_raw(hpos, kk)}; };
+  // Stages level kk at LDS entry e.
+  auto stage_raw = [&](const Raw& q, int e, int kk) {
     R rr;
     if (pert) {  // theta' and p' of the departures
-      const R rb = a.base.rho[kk];
-      rr = rmax(rb + r, R(1.0e-6));
-      sTH[kk % NSL][e] = theta_pert(r, rt, rb, a.base.th[kk]);
-      sP[kk % NSL][e] = p_pert(rt, a.base.rt[kk], a.base.p[kk]);
+      const R rb = LV(kk, SL_BRHO);
+      rr = rmax(rb + q.r, R(1.0e-6));
+      sTH[kk % NSL][e] = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
+      sP[kk % NSL][e] = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
     } else {
-      rr = rmax(r, R(1.0e-6));
-      sTH[kk % NSL][e] = q_theta(r, rt);
-      sP[kk % NSL][e] = q_pressure(r, rt);
+      rr = rmax(q.r, R(1.0e-6));
+      sTH[kk % NSL][e] = q_theta(q.r, q.rt);
+      sP[kk % NSL][e] = q_pressure(q.r, q.rt);
     }
-    sUC[kk % NSL][e] = qdiv(R(0.5) * (u0 + u1), rr);
-    sVC[kk % NSL][e] = qdiv(R(0.5) * (v0 + v1), rr);
-    sWC[kk % NSL][e] = qdiv(R(0.5) * (w0 + w1), rr);
+    sUC[kk % NSL][e] = qdiv(R(0.5) * (q.u0 + q.u1), rr);
+    sVC[kk % NSL][e] = qdiv(R(0.5) * (q.v0 + q.v1), rr);
+    sWC[kk % NSL][e] = qdiv(R(0.5) * (q.w0 + q.w1), rr);
     if constexpr (RAW) {
-      sU[kk % NSL][e] = u0;
-      sV[kk % NSL][e] = v0;
-      sW[kk % NSL][e] = w0;
+      sU[kk % NSL][e] = q.u0;
+      sV[kk % NSL][e] = q.v0;
+      sW[kk % NSL][e] = q.w0;
     }
   };
   // Own-column rho (levels k-1, k, k+1), rho*theta (k, k+1) and the top w face.
   R r_km1 = R(0), r_k = R(0), r_k1 = R(0), rt_k = R(0), rt_k1 = R(0), w_top = R(0);
-  auto stage = [&](int kk) {
-    R r, rt, w1;
-    stage_one(own, eo, kk, r, rt, w1);
-    r_k1 = r;
-    rt_k1 = rt;
-    if (kk + 1 == nz) w_top = w1;
-    if (t < NHALO) stage_one(ring, eh, kk, r, rt, w1);
+  auto stage_lev = [&](const Lev& q, int kk) {
+    stage_raw(q.o, eo, kk);
+    r_k1 = q.o.r;
+    rt_k1 = q.o.rt;
+    if (kk + 1 == nz) w_top = q.o.w1;
+    if (t < NHALO) stage_raw(q.h, eh, kk);
+  };
+  // SSP base of the own cell and faces (mode 2), clamped for threads outside.
+  const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  struct Q0 {
+    R rho, rt, u, v, w;
+  };
+  auto load_q0 = [&](int kk) -> Q0 {
+    Q0 q{};
+    if constexpr (MODE == 2) {
+      const size_t cc = g.cell(ic, jc, kk), xx = g.xf(ic, jc, kk), yy = g.yf(ic, jc, kk);
+      q = Q0{a.q0.rho[cc], a.q0.rt[cc], a.q0.u[xx], a.q0.v[yy], a.q0.w[cc]};
+    }
+    return q;
   };
 
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
@@ -152,16 +205,37 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);
   };
 
+  const int kcap = k1 < nz - 1 ? k1 : nz - 1;  // highest level a chunk stages
   if (k0 > 0) {
-    stage(k0 - 1);
+    stage_lev(load_lev(k0 - 1), k0 - 1);
     r_km1 = r_k1;
   }
-  stage(k0);
+  stage_lev(load_lev(k0), k0);
   r_k = r_k1;
   rt_k = rt_k1;
-  for (int k = k0; k < k1; ++k) {
+  // FP32 pipelines the raw loads a level ahead; FP64 (twice the registers per
+  // value) loads each level where it stages it.
+  constexpr bool PIPE = sizeof(R) == 4;
+  Lev LA{};
+  if constexpr (PIPE) LA = load_lev(k0 + 1 < kcap ? k0 + 1 : kcap);
+
+  // Level k: Ln holds the raw level k+1 (loaded an iteration ago).  After
+  // staging it, the loads of the SSP base of level k and then those of level
+  // k+2 (into Lnn) are issued; both stay in flight while level k is computed,
+  // and since loads complete in order, waiting for the base (needed first)
+  // leaves level k+2 in flight.  Unrolled by two with the buffers swapped, so
+  // no register copy waits for loads in flight.
+  auto level = [&](int k, const Lev& Ln, Lev& Lnn) {
     __syncthreads();
-    if (k + 1 < nz) stage(k + 1);
+    Q0 Qc;
+    if constexpr (PIPE) {
+      if (k + 1 < nz) stage_lev(Ln, k + 1);
+      Qc = load_q0(k);
+      Lnn = load_lev(k + 2 < kcap ? k + 2 : kcap);
+    } else {
+      if (k + 1 < nz) stage_lev(load_lev(k + 1), k + 1);
+      Qc = load_q0(k);
+    }
     __syncthreads();
     if (active) {
       // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
@@ -190,11 +264,11 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       const R w_up = k + 1 < nz ? W(i, j, k + 1) : w_top;  // z-face k+1 of the own cell
       auto IDZC = [&](int ii, int jj, int kk) -> R {
         if (TERRAIN) return a.m.idz_c[mcol(ii, jj) + kk * plane_m];
-        return a.m.lev_idz_c[kk];
+        return LV(kk, SL_IDZC);
       };
       auto IDZF = [&](int ii, int jj, int kk) -> R {
         if (TERRAIN) return a.m.idz_f[mcol(ii, jj) + kk * plane_m];
-        return a.m.lev_idz_f[kk];
+        return LV(kk, SL_IDZF);
       };
       // dp/dz at column (ii, jj) (dry_pressure_gradient.cu:37-73).
       auto vgrad = [&](int ii, int jj) -> R {
@@ -236,9 +310,9 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         if (pert) {
           // theta = theta_b(level) + theta': the base parts of the horizontal
           // fluxes share theta_b(k); the vertical ones take the upwind level's.
-          const R tb = a.base.th[k];
-          const R tbb = (mb >= R(0) && k > 0) ? a.base.th[k - 1] : tb;
-          const R tbt = (mt < R(0) && k + 1 < nz) ? a.base.th[k + 1] : tb;
+          const R tb = LV(k, SL_BTH);
+          const R tbb = (mb >= R(0) && k > 0) ? LV(k - 1, SL_BTH) : tb;
+          const R tbt = (mt < R(0) && k + 1 < nz) ? LV(k + 1, SL_BTH) : tb;
           t_rt = -(qdivc(tb * (me - mw) + (fe - fw), dx, rdx) +
                    qdivc(tb * (mn - ms) + (fn - fs), dy, rdy) +
                    ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
@@ -323,15 +397,15 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         if (pert) {
           // theta_ref holds the level means of theta'; the bases cancel in
           // theta_f - theta_ref,f
-          const R rl = r_km1 + a.base.rho[kl];
-          const R ru = r_k + a.base.rho[ku];
-          const R trp = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
-          const R trf = R(0.5) * (a.base.th[kl] + a.base.th[ku]) + trp;
+          const R rl = r_km1 + LV(kl, SL_BRHO);
+          const R ru = r_k + LV(ku, SL_BRHO);
+          const R trp = R(0.5) * (LV(kl, SL_TREF) + LV(ku, SL_TREF));
+          const R trf = R(0.5) * (LV(kl, SL_BTH) + LV(ku, SL_BTH)) + trp;
           const R rf = R(0.5) * (rl + ru);
           const R bb = qdiv(a.gravity * (tf - trp), rmax(trf, R(1.0e-6)));
           return A + rf * bb;
         }
-        const R trf = R(0.5) * (a.theta_ref[kl] + a.theta_ref[ku]);
+        const R trf = R(0.5) * (LV(kl, SL_TREF) + LV(ku, SL_TREF));
         const R rf = R(0.5) * (r_km1 + r_k);
         const R bb = qdiv(a.gravity * (tf - trf), rmax(trf, R(1.0e-6)));
         return A + rf * bb;
@@ -339,12 +413,12 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
 
       const R dt = a.dt;
       auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
-      auto fin = [&](R q0v, R v) { return a.mode == 2 ? a.wa * q0v + a.wb * v : v; };
+      auto fin = [&](R q0v, R v) { return MODE == 2 ? a.wa * q0v + a.wb * v : v; };
 
       const size_t ci = g.cell(i, j, k);
       const size_t xf = g.xf(i, j, k), yf = g.yf(i, j, k);
       const R tu0 = tend_u(i, U), tv0 = tend_v(j, V), tw0 = tend_w();
-      if (a.mode == 0) {
+      if constexpr (MODE == 0) {
         a.out.rho[ci] = t_rho;
         a.out.rt[ci] = t_rt;
         a.out.u[xf] = tu0;
@@ -356,36 +430,60 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       } else {
         const R mf = Phys<R>::mom_floor;
         // floors on the full values (departures: 1e-6 - base)
-        const R flr = pert ? R(1.0e-6) - a.base.rho[k] : R(1.0e-6);
-        const R flt = pert ? R(1.0e-6) - a.base.rt[k] : R(1.0e-6);
-        const R rho_new = fin(a.mode == 2 ? a.q0.rho[ci] : R(0), upd(r_k, t_rho, flr));
-        const R rt_new = fin(a.mode == 2 ? a.q0.rt[ci] : R(0), upd(rt_k, t_rt, flt));
+        const R flr = pert ? R(1.0e-6) - LV(k, SL_BRHO) : R(1.0e-6);
+        const R flt = pert ? R(1.0e-6) - LV(k, SL_BRT) : R(1.0e-6);
+        const R rho_new = fin(Qc.rho, upd(r_k, t_rho, flr));
+        const R rt_new = fin(Qc.rt, upd(rt_k, t_rt, flt));
         a.out.rho[ci] = rho_new;
         if (pert)
-          a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, a.base.rho[k], a.base.th[k])
+          a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, LV(k, SL_BRHO), LV(k, SL_BTH))
                                      : rt_new;
         else
           a.out.rt[ci] = a.out_theta ? qdiv(rt_new, rmax(rho_new, R(1.0e-6))) : rt_new;
-        a.out.u[xf] = fin(a.mode == 2 ? a.q0.u[xf] : R(0), upd(U(i, j, k), tu0, mf));
-        a.out.v[yf] = fin(a.mode == 2 ? a.q0.v[yf] : R(0), upd(V(i, j, k), tv0, mf));
-        a.out.w[ci] = fin(a.mode == 2 ? a.q0.w[ci] : R(0), upd(W(i, j, k), tw0, mf));
+        a.out.u[xf] = fin(Qc.u, upd(U(i, j, k), tu0, mf));
+        a.out.v[yf] = fin(Qc.v, upd(V(i, j, k), tv0, mf));
+        a.out.w[ci] = fin(Qc.w, upd(W(i, j, k), tw0, mf));
         if (i == g.nx - 1) {
           const size_t f = g.xf(g.nx, j, k);
-          a.out.u[f] = fin(a.mode == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx, Ug), mf));
+          a.out.u[f] = fin(MODE == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx, Ug), mf));
         }
         if (j == g.ny - 1) {
           const size_t f = g.yf(i, g.ny, k);
-          a.out.v[f] = fin(a.mode == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny, Vg), mf));
+          a.out.v[f] = fin(MODE == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny, Vg), mf));
         }
         if (k == nz - 1) {
           const size_t f = g.zf(i, j, nz);
-          a.out.w[f] = fin(a.mode == 2 ? a.q0.w[f] : R(0), upd(w_top, R(0), mf));
+          a.out.w[f] = fin(MODE == 2 ? a.q0.w[f] : R(0), upd(w_top, R(0), mf));
         }
       }
     }
     r_km1 = r_k;
     r_k = r_k1;
     rt_k = rt_k1;
+  };
+  Lev LB{};
+  for (int k = k0; k < k1; k += 2) {
+    level(k, LA, LB);
+    if (k + 1 < k1) level(k + 1, LB, LA);
+  }
+}
+
+template <class R, bool TERRAIN, bool PERT>
+void launch_slow_mode(hipStream_t st, const SlowArgs<R>& a, dim3 grid, dim3 blk, size_t lds,
+                      int tx, int tiles, int nkc) {
+  switch (a.mode) {
+    case 0:
+      hipLaunchKernelGGL((k_slow_column<R, TERRAIN, PERT, 0>), grid, blk, lds, st, a, tx, tiles,
+                         nkc);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_slow_column<R, TERRAIN, PERT, 1>), grid, blk, lds, st, a, tx, tiles,
+                         nkc);
+      break;
+    default:
+      hipLaunchKernelGGL((k_slow_column<R, TERRAIN, PERT, 2>), grid, blk, lds, st, a, tx, tiles,
+                         nkc);
+      break;
   }
 }
 
@@ -395,12 +493,14 @@ void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int 
   const int tiles = tx * ty;
   nkc = nkc < 1 ? 1 : (nkc > a.g.nz ? a.g.nz : nkc);
   const dim3 blk(TX, TY, 1), grid(tiles * nkc);
+  // per-level LDS: the longest chunk plus the levels below and above it
+  const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * SL_N * sizeof(R);
   if (terrain)
-    hipLaunchKernelGGL((k_slow_column<R, true, false>), grid, blk, 0, st, a, tx, tiles, nkc);
+    launch_slow_mode<R, true, false>(st, a, grid, blk, lds, tx, tiles, nkc);
   else if (a.base.rho)  // perturbation form (flat, doubly periodic)
-    hipLaunchKernelGGL((k_slow_column<R, false, true>), grid, blk, 0, st, a, tx, tiles, nkc);
+    launch_slow_mode<R, false, true>(st, a, grid, blk, lds, tx, tiles, nkc);
   else
-    hipLaunchKernelGGL((k_slow_column<R, false, false>), grid, blk, 0, st, a, tx, tiles, nkc);
+    launch_slow_mode<R, false, false>(st, a, grid, blk, lds, tx, tiles, nkc);
 }
 
 template void launch_slow_column<float>(hipStream_t, const SlowArgs<float>&, bool, int);
@@ -415,10 +515,12 @@ template void launch_slow_column<double>(hipStream_t, const SlowArgs<double>&, b
 // each of the seven uses.  mode 1: out = max(0, in - dt div); mode 2:
 // out = wa*q0 + wb*(...) (dry_transport.cu:40-47).
 // ===========================================================================
-template <class R, bool TERRAIN, int NT>
+template <class R, bool TERRAIN, int NT, int MODE>
 __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
                                                        int nkc) {
   __shared__ R sQ[NT][4][PN];
+  extern __shared__ __align__(16) unsigned char lev_raw[];
+  R* const lev = reinterpret_cast<R*>(lev_raw);
   const Geo g = a.g;
   int b, k0, k1;
   const int tile = tile_of(tiles, nkc, g.nz, b, k0, k1);
@@ -438,64 +540,101 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
     hy = q >> 1;
   }
   const int nz = g.nz;
+  const bool pert = a.base_rho != nullptr;
+
+  // Per-level values of levels k0-1 .. k1+1: 1/dz (flat) and the density base.
+  for (int q = t; q < k1 - k0 + 3; q += TX * TY) {
+    const int kk = k0 - 1 + q;
+    const int kc = kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk);
+    lev[2 * q] = TERRAIN ? R(0) : a.lev_idz_c[kc];
+    lev[2 * q + 1] = pert ? a.base_rho[kc] : R(0);
+  }
+  auto LV = [&](int kk, int q) -> R { return lev[2 * (kk - k0 + 1) + q]; };
+  __syncthreads();
+
+  // Loads are unconditional (stand-ins for threads without a ring cell or
+  // outside the domain), so waits cover exactly the loads they need: the
+  // raw level k+2 and the faces of level k+1 stay in flight while level k is
+  // computed.
   const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;
-  const int oc = stage_pos(g, i, j).c, hc = stage_pos(g, ti0 + hx, tj0 + hy).c;
+  const int oc = stage_pos(g, i, j).c;
+  const int hc = t < NHALO ? stage_pos(g, ti0 + hx, tj0 + hy).c : oc;
   const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
-  // Own raw rho*q of the staged level (kept for the update) and the ring.
-  R own_n[NT];
-  auto stage_one = [&](int off, int e, int kk, R* raw) {
-    const R r0 = ld_at(a.rho, off, pc, kk);
-    const R r = rmax(a.base_rho ? r0 + a.base_rho[kk] : r0, R(1.0e-6));
-#pragma unroll
-    for (int n = 0; n < NT; ++n) {
-      raw[n] = ld_at(a.in[n], off, pc, kk);
-      sQ[n][kk & 3][e] = qdiv(raw[n], r);
-    }
+  auto ldo = [&](const R* f, int off, int kk) -> R {
+    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * pc];
+    return off < 0 ? R(0) : v;
   };
-  auto stage = [&](int kk) {
-    stage_one(oc, eo, kk, own_n);
-    if (t < NHALO) {
-      R ring_raw[NT];
-      stage_one(hc, eh, kk, ring_raw);
+  struct TRaw {
+    R r, q[NT];
+  };
+  struct TLev {
+    TRaw o, h;  // own cell, ring cell
+  };
+  auto load_traw = [&](int off, int kk) -> TRaw {
+    TRaw x;
+    x.r = ldo(a.rho, off, kk);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) x.q[n] = ldo(a.in[n], off, kk);
+    return x;
+  };
+  auto load_tlev = [&](int kk) -> TLev { return TLev{load_traw(oc, kk), load_traw(hc, kk)}; };
+  auto stage_traw = [&](const TRaw& x, int e, int kk) {
+    const R r = rmax(pert ? x.r + LV(kk, 1) : x.r, R(1.0e-6));
+#pragma unroll
+    for (int n = 0; n < NT; ++n) sQ[n][kk & 3][e] = qdiv(x.q[n], r);
+  };
+  auto stage_tlev = [&](const TLev& x, int kk) {
+    stage_traw(x.o, eo, kk);
+    if (t < NHALO) stage_traw(x.h, eh, kk);
+  };
+  // Face momenta (and SSP bases) of one level for the own cell; the w face
+  // above a cell is the next level's lower face.  Clamped for threads outside.
+  const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  struct TF {
+    R mw, me, ms, mn, mt, q0[NT];
+  };
+  auto load_tf = [&](int k) -> TF {
+    TF f{};
+    f.mw = a.u[g.xf(ic, jc, k)];
+    f.me = a.u[g.xf(ic + 1, jc, k)];
+    f.ms = a.v[g.yf(ic, jc, k)];
+    f.mn = a.v[g.yf(ic, jc + 1, k)];
+    f.mt = a.w[g.zf(ic, jc, k + 1)];
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int n = 0; n < NT; ++n) f.q0[n] = a.q0[n][g.cell(ic, jc, k)];
     }
+    return f;
   };
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
   const R rdx = R(1) / dx, rdy = R(1) / dy;
   const int c0 = (lx + 1) + PX * (ly + 1);
-  // Face momenta (and SSP bases) of one level for the own cell, loaded a
-  // level ahead; the w face above a cell is the next level's lower face.
-  R f_mw = 0, f_me = 0, f_ms = 0, f_mn = 0, f_mt = 0, f_q0[NT] = {};
-  auto load_faces = [&](int k) {
-    f_mw = a.u[g.xf(i, j, k)];
-    f_me = a.u[g.xf(i + 1, j, k)];
-    f_ms = a.v[g.yf(i, j, k)];
-    f_mn = a.v[g.yf(i, j + 1, k)];
-    f_mt = a.w[g.zf(i, j, k + 1)];
-    if (a.mode == 2) {
-#pragma unroll
-      for (int n = 0; n < NT; ++n) f_q0[n] = a.q0[n][g.cell(i, j, k)];
-    }
-  };
+
+  const int kcap = k1 < nz - 1 ? k1 : nz - 1;  // highest level a chunk stages
+  // own raw rho*q of level k (the update's base)
   R own[NT];
-  if (k0 > 0) stage(k0 - 1);
-  stage(k0);
+  if (k0 > 0) stage_tlev(load_tlev(k0 - 1), k0 - 1);
+  {
+    const TLev x = load_tlev(k0);
+    stage_tlev(x, k0);
 #pragma unroll
-  for (int n = 0; n < NT; ++n) own[n] = own_n[n];
-  R mb = R(0);
-  if (active) {
-    mb = a.w[g.zf(i, j, k0)];
-    load_faces(k0);
+    for (int n = 0; n < NT; ++n) own[n] = x.o.q[n];
   }
-  for (int k = k0; k < k1; ++k) {
-    if (k + 1 < nz) stage(k + 1);
-    const R mw = f_mw, me = f_me, ms = f_ms, mn = f_mn, mt = f_mt;
-    R q0v[NT];
-#pragma unroll
-    for (int n = 0; n < NT; ++n) q0v[n] = f_q0[n];
-    if (active && k + 1 < k1) load_faces(k + 1);
+  TLev LA = load_tlev(k0 + 1 < kcap ? k0 + 1 : kcap);
+  TF FA = load_tf(k0);
+  R mb = a.w[g.zf(ic, jc, k0)];
+
+  // Level k: Ln holds the raw level k+1 and Fc the faces of level k (loaded
+  // an iteration ago); raw level k+2 and the faces of k+1 go into Lnn / Fn.
+  // Unrolled by two with the buffers swapped, so no register copy waits for
+  // loads in flight.
+  auto level = [&](int k, const TLev& Ln, const TF& Fc, TLev& Lnn, TF& Fn) {
+    if (k + 1 < nz) stage_tlev(Ln, k + 1);
+    Lnn = load_tlev(k + 2 < kcap ? k + 2 : kcap);
+    Fn = load_tf(k + 1 < k1 ? k + 1 : k1 - 1);
     __syncthreads();
     if (active) {
-      const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : a.lev_idz_c[k];
+      const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : LV(k, 0);
       const size_t ci = g.cell(i, j, k);
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
@@ -504,20 +643,26 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
         const R qw = q[c0 - 1], qe = q[c0 + 1], qs = q[c0 - PX], qn = q[c0 + PX];
         const R qb = k > 0 ? sQ[n][(k - 1) & 3][c0] : qc;
         const R qt = k + 1 < nz ? sQ[n][(k + 1) & 3][c0] : qc;
-        const R fw = mw * (mw >= R(0) ? qw : qc);
-        const R fe = me * (me >= R(0) ? qc : qe);
-        const R fs = ms * (ms >= R(0) ? qs : qc);
-        const R fn = mn * (mn >= R(0) ? qc : qn);
+        const R fw = Fc.mw * (Fc.mw >= R(0) ? qw : qc);
+        const R fe = Fc.me * (Fc.me >= R(0) ? qc : qe);
+        const R fs = Fc.ms * (Fc.ms >= R(0) ? qs : qc);
+        const R fn = Fc.mn * (Fc.mn >= R(0) ? qc : qn);
         const R fb = mb * (mb >= R(0) ? qb : qc);
-        const R ft = mt * (mt >= R(0) ? qc : qt);
+        const R ft = Fc.mt * (Fc.mt >= R(0) ? qc : qt);
         const R div = qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz;
         const R v = rmax(R(0), own[n] - a.dt * div);
-        a.out[n][ci] = a.mode == 2 ? a.wa * q0v[n] + a.wb * v : v;
+        a.out[n][ci] = MODE == 2 ? a.wa * Fc.q0[n] + a.wb * v : v;
       }
     }
-    mb = mt;
+    mb = Fc.mt;
 #pragma unroll
-    for (int n = 0; n < NT; ++n) own[n] = own_n[n];
+    for (int n = 0; n < NT; ++n) own[n] = Ln.o.q[n];
+  };
+  TLev LB{};
+  TF FB{};
+  for (int k = k0; k < k1; k += 2) {
+    level(k, LA, FA, LB, FB);
+    if (k + 1 < k1) level(k + 1, LB, FB, LA, FA);
   }
 }
 
@@ -525,10 +670,20 @@ template <class R, int NT>
 void launch_tracer_column_n(hipStream_t st, const TracerArgs<R>& a, bool terrain, int tx,
                             int tiles, int nkc) {
   const dim3 blk(TX, TY, 1), grid(tiles * nkc);
-  if (terrain)
-    hipLaunchKernelGGL((k_tracer_column<R, true, NT>), grid, blk, 0, st, a, tx, tiles, nkc);
-  else
-    hipLaunchKernelGGL((k_tracer_column<R, false, NT>), grid, blk, 0, st, a, tx, tiles, nkc);
+  // per-level LDS: the longest chunk plus the levels below and above it
+  const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * 2 * sizeof(R);
+  const bool m2 = a.mode == 2;
+  if (terrain) {
+    if (m2)
+      hipLaunchKernelGGL((k_tracer_column<R, true, NT, 2>), grid, blk, lds, st, a, tx, tiles, nkc);
+    else
+      hipLaunchKernelGGL((k_tracer_column<R, true, NT, 1>), grid, blk, lds, st, a, tx, tiles, nkc);
+  } else {
+    if (m2)
+      hipLaunchKernelGGL((k_tracer_column<R, false, NT, 2>), grid, blk, lds, st, a, tx, tiles, nkc);
+    else
+      hipLaunchKernelGGL((k_tracer_column<R, false, NT, 1>), grid, blk, lds, st, a, tx, tiles, nkc);
+  }
 }
 
 // Tracers are advanced in groups of at most three per launch.
