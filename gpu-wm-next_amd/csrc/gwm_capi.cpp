@@ -1519,6 +1519,9 @@ int gwm_grid_create(gwm_ctx* ctx, const gwm_grid_desc* d, const double* z_interf
       const double elems = static_cast<double>(d->nx + 1 + 2 * d->halo) *
                            (d->ny + 1 + 2 * d->halo) * (d->nz + 1);
       require(elems < 2147483647.0, "grid too large: a field must hold fewer than 2^31 elements");
+      // kernels address arrays by 32-bit byte offsets
+      require(elems * static_cast<double>(ctx->esize()) < 4294967296.0,
+              "grid too large: each field must stay below 4 GiB");
     }
     auto G = std::make_unique<gwm_grid>();
     require(!ctx->closing, "context already destroyed");

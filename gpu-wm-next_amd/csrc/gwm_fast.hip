@@ -130,7 +130,7 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
   const int oc = stage_pos(g, i, j).c;
   const int hc = t < NHALO ? stage_pos(g, ti0 + hx, tj0 + hy).c : oc;
   auto ldc = [&](const R* f, int off, int k) -> R {
-    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(k) * pc];
+    const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(k) * pc);
     return off < 0 ? R(0) : v;
   };
   struct Plane {
@@ -161,20 +161,23 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
   };
   auto load_faces = [&](int k) -> Faces {
     Faces F{};
-    const size_t xf0 = g.xf(ic, jc, k), yf0 = g.yf(ic, jc, k), zf1 = g.zf(ic, jc, k + 1);
+    const unsigned xf0 = static_cast<unsigned>(g.xf(ic, jc, k));
+    const unsigned yf0 = static_cast<unsigned>(g.yf(ic, jc, k));
+    const unsigned yf1 = yf0 + static_cast<unsigned>(g.pxc);
+    const unsigned zf1 = static_cast<unsigned>(g.zf(ic, jc, k + 1));
     if constexpr (!FIRST) {
-      F.u0 = a.u[xf0];
-      F.u1 = a.u[xf0 + 1];
-      F.v0 = a.v[yf0];
-      F.v1 = a.v[yf0 + g.pxc];
-      F.w1 = a.w[zf1];
+      F.u0 = ld32(a.u, xf0);
+      F.u1 = ld32(a.u, xf0 + 1);
+      F.v0 = ld32(a.v, yf0);
+      F.v1 = ld32(a.v, yf1);
+      F.w1 = ld32(a.w, zf1);
     }
     if constexpr (LAST) {
-      F.ub0 = a.ub[xf0];
-      F.ub1 = a.ub[xf0 + 1];
-      F.vb0 = a.vb[yf0];
-      F.vb1 = a.vb[yf0 + g.pxc];
-      F.wb1 = a.wb[zf1];
+      F.ub0 = ld32(a.ub, xf0);
+      F.ub1 = ld32(a.ub, xf0 + 1);
+      F.vb0 = ld32(a.vb, yf0);
+      F.vb1 = ld32(a.vb, yf1);
+      F.wb1 = ld32(a.wb, zf1);
     }
     return F;
   };
@@ -325,19 +328,20 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
         const R floor = PERT ? R(1.0e-6) - LV(k, LV_BRHO) : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
-      const size_t c0 = g.cell(i, j, k);
-      const size_t xf0 = g.xf(i, j, k), yf0 = g.yf(i, j, k);
-      a.rho_out[c0] = rho_new;
+      const unsigned c0 = static_cast<unsigned>(g.cell(i, j, k));
+      const unsigned xf0 = static_cast<unsigned>(g.xf(i, j, k));
+      const unsigned yf0 = static_cast<unsigned>(g.yf(i, j, k));
+      st32(a.rho_out, c0, rho_new);
       if (a.rt_out)
-        a.rt_out[c0] = PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k)
-                            : rho_new * th_k;
+        st32(a.rt_out, c0,
+             PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k) : rho_new * th_k);
       // departures, or full faces base + departure on the last substep
-      a.u_out[xf0] = LAST ? F.ub0 + u0 : u0;
-      a.v_out[yf0] = LAST ? F.vb0 + v0 : v0;
-      a.w_out[c0] = LAST ? wb0 + w0 : w0;
-      if (i == g.nx - 1) a.u_out[xf0 + 1] = LAST ? F.ub1 + u1 : u1;
-      if (j == g.ny - 1) a.v_out[yf0 + g.pxc] = LAST ? F.vb1 + v1 : v1;
-      if (k == nz - 1) a.w_out[g.zf(i, j, k + 1)] = LAST ? F.wb1 + w1 : w1;
+      st32(a.u_out, xf0, LAST ? F.ub0 + u0 : u0);
+      st32(a.v_out, yf0, LAST ? F.vb0 + v0 : v0);
+      st32(a.w_out, c0, LAST ? wb0 + w0 : w0);
+      if (i == g.nx - 1) st32(a.u_out, xf0 + 1, LAST ? F.ub1 + u1 : u1);
+      if (j == g.ny - 1) st32(a.v_out, yf0 + static_cast<unsigned>(g.pxc), LAST ? F.vb1 + v1 : v1);
+      if (k == nz - 1) st32(a.w_out, c0 + pc, LAST ? F.wb1 + w1 : w1);
       w0 = w1;
     }
     wb0 = F.wb1;

@@ -345,6 +345,20 @@ __device__ inline StagePos stage_pos(const Geo& g, int ii, int jj) {
   p.v1 = (ci < 0 || fj1 < 0) ? -1 : (fj1 + g.h) * g.pxc + ci + g.h;
   return p;
 }
+// Element idx of an array addressed by a 32-bit byte offset from its base
+// (grids are limited so that every array stays below 4 GiB): the compiler
+// then uses the scalar-base + 32-bit vector-offset form of global loads and
+// stores instead of 64-bit address arithmetic per access.
+template <class T>
+__device__ inline T ld32(const T* f, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) +
+                                     idx * static_cast<unsigned>(sizeof(T)));
+}
+template <class T>
+__device__ inline void st32(T* f, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(f) + idx * static_cast<unsigned>(sizeof(T))) = v;
+}
+
 // Value at level-0 offset off of level k (level stride `plane`), 0 for off < 0.
 template <class R>
 __device__ inline R ld_at(const R* f, int off, unsigned plane, int k) {

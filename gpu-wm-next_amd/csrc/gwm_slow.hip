@@ -142,7 +142,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     Raw o, h;  // own cell, ring cell
   };
   auto ldo = [&](const R* f, int off, unsigned plane, int kk) -> R {
-    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * plane];
+    const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * plane);
     return off < 0 ? R(0) : v;
   };
   const StagePos hpos = t < NHALO ? ring : own;
@@ -191,8 +191,11 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   auto load_q0 = [&](int kk) -> Q0 {
     Q0 q{};
     if constexpr (MODE == 2) {
-      const size_t cc = g.cell(ic, jc, kk), xx = g.xf(ic, jc, kk), yy = g.yf(ic, jc, kk);
-      q = Q0{a.q0.rho[cc], a.q0.rt[cc], a.q0.u[xx], a.q0.v[yy], a.q0.w[cc]};
+      const unsigned cc = static_cast<unsigned>(g.cell(ic, jc, kk));
+      const unsigned xx = static_cast<unsigned>(g.xf(ic, jc, kk));
+      const unsigned yy = static_cast<unsigned>(g.yf(ic, jc, kk));
+      q = Q0{ld32(a.q0.rho, cc), ld32(a.q0.rt, cc), ld32(a.q0.u, xx), ld32(a.q0.v, yy),
+             ld32(a.q0.w, cc)};
     }
     return q;
   };
@@ -323,16 +326,19 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
 
       // ---- x-face (K17 + K21)
       auto tend_u = [&](int fi, const auto& Uf) -> R {
-        if ((fi == 0 && ow) || (fi == g.nx && oe)) return R(0);
+        // closed outer faces and open-side fluxes are selected away, not
+        // branched around: every read stays inside the staged tile and ring
+        const bool closed = (fi == 0 && ow) || (fi == g.nx && oe);
         const R qw = Uf(fi - 1, j, k), qc = Uf(fi, j, k), qe = Uf(fi + 1, j, k);
         const R qs = Uf(fi, j - 1, k), qn = Uf(fi, j + 1, k);
         const R qb = k > 0 ? Uf(fi, j, k - 1) : qc;
         const R qt = k + 1 < nz ? Uf(fi, j, k + 1) : qc;
         const R fw = upwind(UC(fi - 1, j, k), qw, qc);
         const R fe = upwind(UC(fi, j, k), qc, qe);
-        R fs = 0, fn = 0, fb = 0, ft = 0;
-        if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
-        if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
+        const R fs0 = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
+        const R fn0 = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
+        const R fs = (j == 0 && os) ? R(0) : fs0, fn = (j + 1 == g.ny && on) ? R(0) : fn0;
+        R fb = 0, ft = 0;
         if (k > 0) fb = upwind(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
         if (k + 1 < nz) ft = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
         const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
@@ -347,18 +353,19 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         } else {
           term = qdivc(pr - pl, dx, rdx);
         }
-        return A + (-term);
+        return closed ? R(0) : A + (-term);
       };
       // ---- y-face (K18 + K22)
       auto tend_v = [&](int fj, const auto& Vf) -> R {
-        if ((fj == 0 && os) || (fj == g.ny && on)) return R(0);
+        const bool closed = (fj == 0 && os) || (fj == g.ny && on);
         const R qw = Vf(i - 1, fj, k), qc = Vf(i, fj, k), qe = Vf(i + 1, fj, k);
         const R qs = Vf(i, fj - 1, k), qn = Vf(i, fj + 1, k);
         const R qb = k > 0 ? Vf(i, fj, k - 1) : qc;
         const R qt = k + 1 < nz ? Vf(i, fj, k + 1) : qc;
-        R fw = 0, fe = 0, fb = 0, ft = 0;
-        if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
-        if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
+        const R fw0 = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
+        const R fe0 = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
+        const R fw = (i == 0 && ow) ? R(0) : fw0, fe = (i + 1 == g.nx && oe) ? R(0) : fe0;
+        R fb = 0, ft = 0;
         const R fs = upwind(VC(i, fj - 1, k), qs, qc);
         const R fn = upwind(VC(i, fj, k), qc, qn);
         if (k > 0) fb = upwind(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
@@ -375,7 +382,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         } else {
           term = qdivc(pn - ps, dy, rdy);
         }
-        return A + (-term);
+        return closed ? R(0) : A + (-term);
       };
       // ---- z-face at interior face k (K19 + K3)
       auto tend_w = [&]() -> R {
@@ -384,11 +391,12 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         const R qw = W(i - 1, j, k), qc = W(i, j, k), qe = W(i + 1, j, k);
         const R qs = W(i, j - 1, k), qn = W(i, j + 1, k);
         const R qb = W(i, j, k - 1), qt = w_up;
-        R fw = 0, fe = 0, fs = 0, fn = 0;
-        if (!(i == 0 && ow)) fw = upwind(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
-        if (!(i + 1 == g.nx && oe)) fe = upwind(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
-        if (!(j == 0 && os)) fs = upwind(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
-        if (!(j + 1 == g.ny && on)) fn = upwind(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
+        const R fw0 = upwind(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
+        const R fe0 = upwind(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
+        const R fs0 = upwind(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
+        const R fn0 = upwind(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
+        const R fw = (i == 0 && ow) ? R(0) : fw0, fe = (i + 1 == g.nx && oe) ? R(0) : fe0;
+        const R fs = (j == 0 && os) ? R(0) : fs0, fn = (j + 1 == g.ny && on) ? R(0) : fn0;
         const R fb = upwind(WC(i, j, kl), qb, qc);
         const R ft = upwind(WC(i, j, ku), qc, qt);
         const R idz = IDZF(i, j, k);
@@ -415,15 +423,16 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
       auto fin = [&](R q0v, R v) { return MODE == 2 ? a.wa * q0v + a.wb * v : v; };
 
-      const size_t ci = g.cell(i, j, k);
-      const size_t xf = g.xf(i, j, k), yf = g.yf(i, j, k);
+      const unsigned ci = static_cast<unsigned>(g.cell(i, j, k));
+      const unsigned xf = static_cast<unsigned>(g.xf(i, j, k));
+      const unsigned yf = static_cast<unsigned>(g.yf(i, j, k));
       const R tu0 = tend_u(i, U), tv0 = tend_v(j, V), tw0 = tend_w();
       if constexpr (MODE == 0) {
-        a.out.rho[ci] = t_rho;
-        a.out.rt[ci] = t_rt;
-        a.out.u[xf] = tu0;
-        a.out.v[yf] = tv0;
-        a.out.w[ci] = tw0;
+        st32(a.out.rho, ci, t_rho);
+        st32(a.out.rt, ci, t_rt);
+        st32(a.out.u, xf, tu0);
+        st32(a.out.v, yf, tv0);
+        st32(a.out.w, ci, tw0);
         if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx, Ug);
         if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny, Vg);
         if (k == nz - 1) a.out.w[g.zf(i, j, nz)] = R(0);
@@ -434,15 +443,15 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         const R flt = pert ? R(1.0e-6) - LV(k, SL_BRT) : R(1.0e-6);
         const R rho_new = fin(Qc.rho, upd(r_k, t_rho, flr));
         const R rt_new = fin(Qc.rt, upd(rt_k, t_rt, flt));
-        a.out.rho[ci] = rho_new;
+        st32(a.out.rho, ci, rho_new);
         if (pert)
-          a.out.rt[ci] = a.out_theta ? theta_pert(rho_new, rt_new, LV(k, SL_BRHO), LV(k, SL_BTH))
-                                     : rt_new;
+          st32(a.out.rt, ci,
+               a.out_theta ? theta_pert(rho_new, rt_new, LV(k, SL_BRHO), LV(k, SL_BTH)) : rt_new);
         else
-          a.out.rt[ci] = a.out_theta ? qdiv(rt_new, rmax(rho_new, R(1.0e-6))) : rt_new;
-        a.out.u[xf] = fin(Qc.u, upd(U(i, j, k), tu0, mf));
-        a.out.v[yf] = fin(Qc.v, upd(V(i, j, k), tv0, mf));
-        a.out.w[ci] = fin(Qc.w, upd(W(i, j, k), tw0, mf));
+          st32(a.out.rt, ci, a.out_theta ? qdiv(rt_new, rmax(rho_new, R(1.0e-6))) : rt_new);
+        st32(a.out.u, xf, fin(Qc.u, upd(U(i, j, k), tu0, mf)));
+        st32(a.out.v, yf, fin(Qc.v, upd(V(i, j, k), tv0, mf)));
+        st32(a.out.w, ci, fin(Qc.w, upd(W(i, j, k), tw0, mf)));
         if (i == g.nx - 1) {
           const size_t f = g.xf(g.nx, j, k);
           a.out.u[f] = fin(MODE == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx, Ug), mf));
@@ -561,7 +570,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
   const int hc = t < NHALO ? stage_pos(g, ti0 + hx, tj0 + hy).c : oc;
   const int eo = (lx + 1) + PX * (ly + 1), eh = (hx + 1) + PX * (hy + 1);
   auto ldo = [&](const R* f, int off, int kk) -> R {
-    const R v = f[static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * pc];
+    const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * pc);
     return off < 0 ? R(0) : v;
   };
   struct TRaw {
@@ -595,14 +604,17 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
   };
   auto load_tf = [&](int k) -> TF {
     TF f{};
-    f.mw = a.u[g.xf(ic, jc, k)];
-    f.me = a.u[g.xf(ic + 1, jc, k)];
-    f.ms = a.v[g.yf(ic, jc, k)];
-    f.mn = a.v[g.yf(ic, jc + 1, k)];
-    f.mt = a.w[g.zf(ic, jc, k + 1)];
+    const unsigned xf = static_cast<unsigned>(g.xf(ic, jc, k));
+    const unsigned yf = static_cast<unsigned>(g.yf(ic, jc, k));
+    const unsigned cc = static_cast<unsigned>(g.cell(ic, jc, k));
+    f.mw = ld32(a.u, xf);
+    f.me = ld32(a.u, xf + 1);
+    f.ms = ld32(a.v, yf);
+    f.mn = ld32(a.v, yf + static_cast<unsigned>(g.pxc));
+    f.mt = ld32(a.w, cc + pc);
     if constexpr (MODE == 2) {
 #pragma unroll
-      for (int n = 0; n < NT; ++n) f.q0[n] = a.q0[n][g.cell(ic, jc, k)];
+      for (int n = 0; n < NT; ++n) f.q0[n] = ld32(a.q0[n], cc);
     }
     return f;
   };
@@ -635,7 +647,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
     __syncthreads();
     if (active) {
       const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : LV(k, 0);
-      const size_t ci = g.cell(i, j, k);
+      const unsigned ci = static_cast<unsigned>(g.cell(i, j, k));
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
         const R* q = sQ[n][k & 3];
@@ -651,7 +663,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
         const R ft = Fc.mt * (Fc.mt >= R(0) ? qc : qt);
         const R div = qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz;
         const R v = rmax(R(0), own[n] - a.dt * div);
-        a.out[n][ci] = MODE == 2 ? a.wa * Fc.q0[n] + a.wb * v : v;
+        st32(a.out[n], ci, MODE == 2 ? a.wa * Fc.q0[n] + a.wb * v : v);
       }
     }
     mb = Fc.mt;
