@@ -241,30 +241,38 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     }
     __syncthreads();
     if (active) {
+      // Levels below and above k, clamped to k at the column ends: a value
+      // read at the clamped level equals the level-k one, and the fluxes
+      // through the column ends are selected away, so the level loop has no
+      // per-level branches and its LDS reads issue together.
+      const int kb = k > 0 ? k - 1 : k, kt = k + 1 < nz ? k + 1 : k;
+      const int s_b = kb % NSL, s_c = k % NSL, s_t = kt % NSL;
+      auto SL = [&](int kk) { return kk == k ? s_c : (kk < k ? s_b : s_t); };
       // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
       auto at = [&](int ii, int jj) { return (ii - ti0 + 1) + PX * (jj - tj0 + 1); };
-      auto TH = [&](int ii, int jj, int kk) { return sTH[kk % NSL][at(ii, jj)]; };
-      auto P = [&](int ii, int jj, int kk) { return sP[kk % NSL][at(ii, jj)]; };
-      auto UC = [&](int ii, int jj, int kk) { return sUC[kk % NSL][at(ii, jj)]; };
-      auto VC = [&](int ii, int jj, int kk) { return sVC[kk % NSL][at(ii, jj)]; };
-      auto WC = [&](int ii, int jj, int kk) { return sWC[kk % NSL][at(ii, jj)]; };
+      auto TH = [&](int ii, int jj, int kk) { return sTH[SL(kk)][at(ii, jj)]; };
+      auto P = [&](int ii, int jj, int kk) { return sP[SL(kk)][at(ii, jj)]; };
+      auto UC = [&](int ii, int jj, int kk) { return sUC[SL(kk)][at(ii, jj)]; };
+      auto VC = [&](int ii, int jj, int kk) { return sVC[SL(kk)][at(ii, jj)]; };
+      auto WC = [&](int ii, int jj, int kk) { return sWC[SL(kk)][at(ii, jj)]; };
       // Raw face momenta: staged (FP32) or global; Ug / Vg for the domain-edge
       // faces nx / ny, whose outer neighbour lies beyond the ring.
       auto Ug = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
       auto Vg = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
       auto U = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sU[kk % NSL][at(ii, jj)];
+        if constexpr (RAW) return sU[SL(kk)][at(ii, jj)];
         else return L.xf(s.u, ii, jj, kk);
       };
       auto V = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sV[kk % NSL][at(ii, jj)];
+        if constexpr (RAW) return sV[SL(kk)][at(ii, jj)];
         else return L.yf(s.v, ii, jj, kk);
       };
       auto W = [&](int ii, int jj, int kk) -> R {  // z-face kk < nz
-        if constexpr (RAW) return sW[kk % NSL][at(ii, jj)];
+        if constexpr (RAW) return sW[SL(kk)][at(ii, jj)];
         else return L.zf(s.w, ii, jj, kk);
       };
-      const R w_up = k + 1 < nz ? W(i, j, k + 1) : w_top;  // z-face k+1 of the own cell
+      const R w_kt = W(i, j, kt);
+      const R w_up = k + 1 < nz ? w_kt : w_top;  // z-face k+1 of the own cell
       auto IDZC = [&](int ii, int jj, int kk) -> R {
         if (TERRAIN) return a.m.idz_c[mcol(ii, jj) + kk * plane_m];
         return LV(kk, SL_IDZC);
@@ -302,8 +310,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         const R thc = TH(i, j, k);
         const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
         const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
-        const R thb = k > 0 ? TH(i, j, k - 1) : thc;
-        const R tht = k + 1 < nz ? TH(i, j, k + 1) : thc;
+        const R thb = TH(i, j, kb), tht = TH(i, j, kt);
         const R fw = mw * (mw >= R(0) ? thw : thc);
         const R fe = me * (me >= R(0) ? thc : the);
         const R fs = ms * (ms >= R(0) ? ths : thc);
@@ -314,8 +321,8 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
           // theta = theta_b(level) + theta': the base parts of the horizontal
           // fluxes share theta_b(k); the vertical ones take the upwind level's.
           const R tb = LV(k, SL_BTH);
-          const R tbb = (mb >= R(0) && k > 0) ? LV(k - 1, SL_BTH) : tb;
-          const R tbt = (mt < R(0) && k + 1 < nz) ? LV(k + 1, SL_BTH) : tb;
+          const R tbb = mb >= R(0) ? LV(kb, SL_BTH) : tb;
+          const R tbt = mt < R(0) ? LV(kt, SL_BTH) : tb;
           t_rt = -(qdivc(tb * (me - mw) + (fe - fw), dx, rdx) +
                    qdivc(tb * (mn - ms) + (fn - fs), dy, rdy) +
                    ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
@@ -331,16 +338,15 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         const bool closed = (fi == 0 && ow) || (fi == g.nx && oe);
         const R qw = Uf(fi - 1, j, k), qc = Uf(fi, j, k), qe = Uf(fi + 1, j, k);
         const R qs = Uf(fi, j - 1, k), qn = Uf(fi, j + 1, k);
-        const R qb = k > 0 ? Uf(fi, j, k - 1) : qc;
-        const R qt = k + 1 < nz ? Uf(fi, j, k + 1) : qc;
+        const R qb = Uf(fi, j, kb), qt = Uf(fi, j, kt);
         const R fw = upwind(UC(fi - 1, j, k), qw, qc);
         const R fe = upwind(UC(fi, j, k), qc, qe);
         const R fs0 = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
         const R fn0 = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
         const R fs = (j == 0 && os) ? R(0) : fs0, fn = (j + 1 == g.ny && on) ? R(0) : fn0;
-        R fb = 0, ft = 0;
-        if (k > 0) fb = upwind(R(0.5) * (WC(fi - 1, j, k - 1) + WC(fi, j, k - 1)), qb, qc);
-        if (k + 1 < nz) ft = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
+        const R fb0 = upwind(R(0.5) * (WC(fi - 1, j, kb) + WC(fi, j, kb)), qb, qc);
+        const R ft0 = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
+        const R fb = k > 0 ? fb0 : R(0), ft = k + 1 < nz ? ft0 : R(0);
         const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
         const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
         const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
@@ -360,16 +366,15 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         const bool closed = (fj == 0 && os) || (fj == g.ny && on);
         const R qw = Vf(i - 1, fj, k), qc = Vf(i, fj, k), qe = Vf(i + 1, fj, k);
         const R qs = Vf(i, fj - 1, k), qn = Vf(i, fj + 1, k);
-        const R qb = k > 0 ? Vf(i, fj, k - 1) : qc;
-        const R qt = k + 1 < nz ? Vf(i, fj, k + 1) : qc;
+        const R qb = Vf(i, fj, kb), qt = Vf(i, fj, kt);
         const R fw0 = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
         const R fe0 = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
         const R fw = (i == 0 && ow) ? R(0) : fw0, fe = (i + 1 == g.nx && oe) ? R(0) : fe0;
-        R fb = 0, ft = 0;
         const R fs = upwind(VC(i, fj - 1, k), qs, qc);
         const R fn = upwind(VC(i, fj, k), qc, qn);
-        if (k > 0) fb = upwind(R(0.5) * (WC(i, fj - 1, k - 1) + WC(i, fj, k - 1)), qb, qc);
-        if (k + 1 < nz) ft = upwind(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
+        const R fb0 = upwind(R(0.5) * (WC(i, fj - 1, kb) + WC(i, fj, kb)), qb, qc);
+        const R ft0 = upwind(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
+        const R fb = k > 0 ? fb0 : R(0), ft = k + 1 < nz ? ft0 : R(0);
         const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
         const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
         const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
@@ -646,6 +651,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
     Fn = load_tf(k + 1 < k1 ? k + 1 : k1 - 1);
     __syncthreads();
     if (active) {
+      const int kb = k > 0 ? k - 1 : k, kt = k + 1 < nz ? k + 1 : k;
       const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : LV(k, 0);
       const unsigned ci = static_cast<unsigned>(g.cell(i, j, k));
 #pragma unroll
@@ -653,8 +659,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
         const R* q = sQ[n][k & 3];
         const R qc = q[c0];
         const R qw = q[c0 - 1], qe = q[c0 + 1], qs = q[c0 - PX], qn = q[c0 + PX];
-        const R qb = k > 0 ? sQ[n][(k - 1) & 3][c0] : qc;
-        const R qt = k + 1 < nz ? sQ[n][(k + 1) & 3][c0] : qc;
+        const R qb = sQ[n][kb & 3][c0], qt = sQ[n][kt & 3][c0];  // qc at the column ends
         const R fw = Fc.mw * (Fc.mw >= R(0) ? qw : qc);
         const R fe = Fc.me * (Fc.me >= R(0) ? qc : qe);
         const R fs = Fc.ms * (Fc.ms >= R(0) ? qs : qc);
