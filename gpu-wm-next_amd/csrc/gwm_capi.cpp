@@ -474,23 +474,32 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
   auto it = ctx->tuned.find(key);
   if (it != ctx->tuned.end()) return it->second;
   hipEvent_t e0 = ctx->ev(), e1 = ctx->ev();
+  // Three rounds over the candidates, each timing two launches after a warm
+  // one; a candidate's time is its fastest round, so clock and cache drift
+  // between rounds does not decide the choice.
+  const int cands[4] = {1, 2, 4, 8};
+  float t_min[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  for (int round = 0; round < 3; ++round)
+    for (int c = 0; c < 4; ++c) {
+      const int n = cands[c];
+      if (n > 1 && g.nz / n < 4) continue;
+      launch(n);
+      HIPCHK(hipEventRecord(e0, ctx->stream));
+      launch(n);
+      launch(n);
+      HIPCHK(hipEventRecord(e1, ctx->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      t_min[c] = std::min(t_min[c], ms);
+    }
   int best = 1;
   float best_ms = 1e30f;
-  for (int n : {1, 2, 4, 8}) {
-    if (n > 1 && g.nz / n < 4) continue;
-    launch(n);
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    launch(n);
-    launch(n);
-    HIPCHK(hipEventRecord(e1, ctx->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = n;
+  for (int c = 0; c < 4; ++c)
+    if (t_min[c] < best_ms) {
+      best_ms = t_min[c];
+      best = cands[c];
     }
-  }
   ctx->pool.push_back(e0);
   ctx->pool.push_back(e1);
   ctx->tuned[key] = best;

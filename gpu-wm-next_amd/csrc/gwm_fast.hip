@@ -41,8 +41,8 @@ namespace gwm {
 
 namespace {
 
-constexpr int TX = 64, TY = 4, PX = TX + 2, PY = TY + 2, PN = PX * PY;
-constexpr int NHALO = PN - TX * TY;  // 140 ring cells
+constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, PN = PX * PY;
+constexpr int NHALO = PN - TX * TY;  // 148 ring cells
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll

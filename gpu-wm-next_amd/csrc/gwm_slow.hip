@@ -71,11 +71,23 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   // takes two barriers: one before level k+1 overwrites the slot of k-2, one
   // before the reads.  A fourth slot would save the first barrier but cost a
   // quarter more LDS, and with it a resident work-group per CU.
+  //
+  // FP32 keeps the staged quantities of one position together, (quantity,
+  // slot) innermost, with an odd stride between positions (no LDS bank
+  // conflicts between neighbouring lanes): a thread's reads are its own entry
+  // plus compile-time offsets, so the per-level address arithmetic is one add
+  // per slot.  FP64 keeps one plane per quantity and slot (its register
+  // budget is the tighter one, and the interleaved form costs it registers).
   constexpr bool RAW = sizeof(R) == 4;
   constexpr int NSL = 3;
-  constexpr int RS = RAW ? NSL : 1, RN = RAW ? PN : 1;
-  __shared__ R sTH[NSL][PN], sP[NSL][PN], sUC[NSL][PN], sVC[NSL][PN], sWC[NSL][PN];
-  __shared__ R sU[RS][RN], sV[RS][RN], sW[RS][RN];
+  constexpr int NQ = RAW ? 8 : 5;  // theta, p, uc, vc, wc (+ raw u, v, w)
+  constexpr int QS = (NQ * NSL) | 1;
+  enum { Q_TH = 0, Q_P, Q_UC, Q_VC, Q_WC, Q_U, Q_V, Q_W };
+  __shared__ R sA[RAW ? PN * QS : NQ * NSL * PN];
+  auto lds = [&](int q, int slot, int e) -> R& {
+    if constexpr (RAW) return sA[e * QS + slot * NQ + q];
+    else return sA[(q * NSL + slot) * PN + e];
+  };
   const Geo g = a.g;
   const Loader<R> L{g};
   const DState<R> s = a.s;
@@ -158,20 +170,20 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     if (pert) {  // theta' and p' of the departures
       const R rb = LV(kk, SL_BRHO);
       rr = rmax(rb + q.r, R(1.0e-6));
-      sTH[kk % NSL][e] = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
-      sP[kk % NSL][e] = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
+      lds(Q_TH, kk % NSL, e) = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
+      lds(Q_P, kk % NSL, e) = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
     } else {
       rr = rmax(q.r, R(1.0e-6));
-      sTH[kk % NSL][e] = q_theta(q.r, q.rt);
-      sP[kk % NSL][e] = q_pressure(q.r, q.rt);
+      lds(Q_TH, kk % NSL, e) = q_theta(q.r, q.rt);
+      lds(Q_P, kk % NSL, e) = q_pressure(q.r, q.rt);
     }
-    sUC[kk % NSL][e] = qdiv(R(0.5) * (q.u0 + q.u1), rr);
-    sVC[kk % NSL][e] = qdiv(R(0.5) * (q.v0 + q.v1), rr);
-    sWC[kk % NSL][e] = qdiv(R(0.5) * (q.w0 + q.w1), rr);
+    lds(Q_UC, kk % NSL, e) = qdiv(R(0.5) * (q.u0 + q.u1), rr);
+    lds(Q_VC, kk % NSL, e) = qdiv(R(0.5) * (q.v0 + q.v1), rr);
+    lds(Q_WC, kk % NSL, e) = qdiv(R(0.5) * (q.w0 + q.w1), rr);
     if constexpr (RAW) {
-      sU[kk % NSL][e] = q.u0;
-      sV[kk % NSL][e] = q.v0;
-      sW[kk % NSL][e] = q.w0;
+      lds(Q_U, kk % NSL, e) = q.u0;
+      lds(Q_V, kk % NSL, e) = q.v0;
+      lds(Q_W, kk % NSL, e) = q.w0;
     }
   };
   // Own-column rho (levels k-1, k, k+1), rho*theta (k, k+1) and the top w face.
@@ -249,26 +261,28 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       const int s_b = kb % NSL, s_c = k % NSL, s_t = kt % NSL;
       auto SL = [&](int kk) { return kk == k ? s_c : (kk < k ? s_b : s_t); };
       // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
-      auto at = [&](int ii, int jj) { return (ii - ti0 + 1) + PX * (jj - tj0 + 1); };
-      auto TH = [&](int ii, int jj, int kk) { return sTH[SL(kk)][at(ii, jj)]; };
-      auto P = [&](int ii, int jj, int kk) { return sP[SL(kk)][at(ii, jj)]; };
-      auto UC = [&](int ii, int jj, int kk) { return sUC[SL(kk)][at(ii, jj)]; };
-      auto VC = [&](int ii, int jj, int kk) { return sVC[SL(kk)][at(ii, jj)]; };
-      auto WC = [&](int ii, int jj, int kk) { return sWC[SL(kk)][at(ii, jj)]; };
+      // (relative to the own entry eo, so neighbour offsets fold into the
+      // LDS instructions' immediate offsets)
+      auto at = [&](int ii, int jj) { return eo + (ii - i) + PX * (jj - j); };
+      auto TH = [&](int ii, int jj, int kk) { return lds(Q_TH, SL(kk), at(ii, jj)); };
+      auto P = [&](int ii, int jj, int kk) { return lds(Q_P, SL(kk), at(ii, jj)); };
+      auto UC = [&](int ii, int jj, int kk) { return lds(Q_UC, SL(kk), at(ii, jj)); };
+      auto VC = [&](int ii, int jj, int kk) { return lds(Q_VC, SL(kk), at(ii, jj)); };
+      auto WC = [&](int ii, int jj, int kk) { return lds(Q_WC, SL(kk), at(ii, jj)); };
       // Raw face momenta: staged (FP32) or global; Ug / Vg for the domain-edge
       // faces nx / ny, whose outer neighbour lies beyond the ring.
       auto Ug = [&](int ii, int jj, int kk) { return L.xf(s.u, ii, jj, kk); };
       auto Vg = [&](int ii, int jj, int kk) { return L.yf(s.v, ii, jj, kk); };
       auto U = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sU[SL(kk)][at(ii, jj)];
+        if constexpr (RAW) return lds(Q_U, SL(kk), at(ii, jj));
         else return L.xf(s.u, ii, jj, kk);
       };
       auto V = [&](int ii, int jj, int kk) -> R {
-        if constexpr (RAW) return sV[SL(kk)][at(ii, jj)];
+        if constexpr (RAW) return lds(Q_V, SL(kk), at(ii, jj));
         else return L.yf(s.v, ii, jj, kk);
       };
       auto W = [&](int ii, int jj, int kk) -> R {  // z-face kk < nz
-        if constexpr (RAW) return sW[SL(kk)][at(ii, jj)];
+        if constexpr (RAW) return lds(Q_W, SL(kk), at(ii, jj));
         else return L.zf(s.w, ii, jj, kk);
       };
       const R w_kt = W(i, j, kt);
