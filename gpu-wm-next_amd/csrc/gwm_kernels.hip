@@ -98,23 +98,39 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
   }
 }
 
-// One work-group per level: each thread sums a strided subset of the
-// partials in order, then a fixed-shape tree adds the 256 thread sums, so the
-// result does not depend on scheduling.
+// One wave per level: lane l sums partials l, l+64, ... in order, with four
+// independent accumulators so the loads stay in flight, then a fixed
+// butterfly adds the lanes, so the result does not depend on scheduling.  A
+// single wave keeps the launch short: the kernel runs between every two
+// fast substeps.
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <class R>
-__global__ __launch_bounds__(256) void k_level_finalize(int nz, int nchunk,
-                                                         const double* partials, int mode,
-                                                         double count, R* out0, R* out1) {
-  __shared__ double sh[256];
+__global__ __launch_bounds__(64) void k_level_finalize(int nz, int nchunk,
+                                                        const double* partials, int mode,
+                                                        double count, R* out0, R* out1) {
   const int k = blockIdx.x;
   const int t = threadIdx.x;
-  double s0 = 0.0, s1 = 0.0;
-  for (int c = t; c < nchunk; c += 256) {
-    s0 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 0];
-    s1 += partials[(static_cast<size_t>(k) * nchunk + c) * 2 + 1];
+  const double* p = partials + static_cast<size_t>(k) * nchunk * 2;
+  double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = t;
+  for (; c + 192 < nchunk; c += 256) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a0[q] += p[(c + 64 * q) * 2 + 0];
+      a1[q] += p[(c + 64 * q) * 2 + 1];
+    }
   }
-  s0 = block_sum(s0, sh);
-  s1 = block_sum(s1, sh);
+  for (int q = 0; c < nchunk; c += 64, ++q) {
+    a0[q] += p[c * 2 + 0];
+    a1[q] += p[c * 2 + 1];
+  }
+  const double s0 = wave_sum64((a0[0] + a0[1]) + (a0[2] + a0[3]));
+  const double s1 = wave_sum64((a1[0] + a1[1]) + (a1[2] + a1[3]));
   if (t != 0) return;
   if (mode == 0) {
     out0[k] = static_cast<R>(s0 / count);  // dry_core.cu:485-490
@@ -555,7 +571,7 @@ int level_partials_chunks(int ny) { return (ny + kPartialRows - 1) / kPartialRow
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1) {
-  hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(256), 0, st, nz, nchunk, partials,
+  hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(64), 0, st, nz, nchunk, partials,
                      mode, count, o0, o1);
 }
 template <class R>
