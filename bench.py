@@ -38,6 +38,24 @@ def measured_traffic(nx, ny, nz):
     return None
 
 
+def copy_ceiling(G, nx, ny, nz, reps=10):
+    """Device-to-device copy rate of a moist state (read + write bytes over
+    wall time, 8 fields and the surface bucket per copy): the box's
+    STREAM-copy ceiling beside the 8 TB/s peak."""
+    ctx, grid, a, _ = make_case(G, G.F32, nx, ny, nz)
+    b = G.State(grid, 3)
+    b.copy_from(a)
+    import numpy as np
+    elems = sum(int(np.prod(a.storage_shape(f))) for f in range(5))
+    elems += 3 * int(np.prod(a.storage_shape(0))) + nx * ny
+    nbytes = elems * 4 * 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_from(a)
+    secs = time.perf_counter() - t0
+    return nbytes * reps / secs / 1e9
+
+
 def log(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -396,6 +414,11 @@ def main():
             gbs = b / (stats[name]["avg_ms"] * 1e-3) / 1e9
             kernel_roofline[name] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / 8000.0, 4),
                                      "algorithmic_bytes": int(b)}
+    log("copy ceiling")
+    copy_gbs = copy_ceiling(G, a.nx, a.ny, a.nz)
+    if roofline:
+        roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
+        roofline["frac_of_copy_ceiling"] = round(roofline["achieved"] / copy_gbs, 4)
     fp64 = None
     if a.fp64_steps > 0:
         log("FP64 plain step")
