@@ -170,6 +170,8 @@ def lib():
         L.gwm_fast_modes.argtypes = [vp, ctypes.POINTER(StepConfig)]
         L.gwm_ssprk3_step.argtypes = [vp, ctypes.POINTER(StepConfig), ctypes.POINTER(BoundaryConfig)]
         L.gwm_tracers_ssprk3.argtypes = [vp, ctypes.POINTER(StepConfig), ip]
+        L.gwm_tracers_ssprk3_callback.argtypes = [vp, ctypes.POINTER(StepConfig), ip,
+                                                  BOUNDARY_FN, vp]
         L.gwm_warm_rain.argtypes = [vp, ctypes.POINTER(WarmRainConfig), ip]
         L.gwm_apply_boundary.argtypes = [vp, ip, dp]
         L.gwm_moist_step.argtypes = [vp, ctypes.POINTER(StepConfig),
@@ -420,6 +422,26 @@ class State:
     def tracers_ssprk3(self, cfg, boundary_mode=BOUNDARY_NONE):
         check(lib().gwm_tracers_ssprk3(self.h, ctypes.byref(cfg), boundary_mode))
 
+    def tracers_ssprk3_callback(self, cfg, fn, boundary_mode=BOUNDARY_NONE):
+        """Tracer SSPRK3 with a host boundary plug-in: fn(stage, sim_time) is
+        called on each stage input with a StageView of the stage's tracer
+        fields and returns nothing (an exception fails the step)."""
+        errors = []
+
+        def tramp(view, t, _user):
+            try:
+                fn(StageView(self, view), t)
+                return 0
+            except Exception as e:  # reported after the call returns
+                errors.append(e)
+                return 1
+
+        cb = BOUNDARY_FN(tramp)
+        rc = lib().gwm_tracers_ssprk3_callback(self.h, ctypes.byref(cfg), boundary_mode, cb, None)
+        if errors:
+            raise errors[0]
+        check(rc)
+
     def warm_rain(self, cfg, metric_fallout=True):
         check(lib().gwm_warm_rain(self.h, ctypes.byref(cfg), int(metric_fallout)))
 
@@ -527,6 +549,31 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class StageView:
+    """A stage state handed to a boundary plug-in (valid during the call):
+    the owner's grid and tracer count, the stage's fields."""
+
+    def __init__(self, owner, handle):
+        self.owner = owner
+        self.h = ctypes.c_void_p(handle)
+
+    def get(self, field):
+        out = np.empty(self.owner.storage_shape(field), self.owner.ctx.dtype)
+        check(lib().gwm_halo_fill(self.h, field))
+        check(lib().gwm_state_download(self.h, field, _ptr(out)))
+        h = self.owner.grid.halo
+        return out[:, h:out.shape[1] - h, h:out.shape[2] - h]
+
+    def set_storage(self, field, arr):
+        a = np.ascontiguousarray(arr, self.owner.ctx.dtype)
+        check(lib().gwm_state_upload(self.h, field, _ptr(a)))
+
+    def storage(self, field):
+        out = np.empty(self.owner.storage_shape(field), self.owner.ctx.dtype)
+        check(lib().gwm_state_download(self.h, field, _ptr(out)))
+        return out
 
 
 class Surrogate:
