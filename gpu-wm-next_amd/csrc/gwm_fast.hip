@@ -156,15 +156,25 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
   // output otherwise; the base faces on the last substep only, which writes
   // full faces.  Indices clamped into the domain for threads outside it.
   const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  // Offsets of level k are the level-0 offset plus k times the field's level
+  // stride (k is uniform, so the product is scalar work).
+  const unsigned sxk = static_cast<unsigned>(g.pyc) * g.pxu;  // x faces
+  const unsigned syk = static_cast<unsigned>(g.pyv) * g.pxc;  // y faces
+  const unsigned xf_c = static_cast<unsigned>(g.xf(ic, jc, 0));
+  const unsigned yf_c = static_cast<unsigned>(g.yf(ic, jc, 0));
+  const unsigned zf_c = static_cast<unsigned>(g.zf(ic, jc, 0));
+  const unsigned c_o = static_cast<unsigned>(g.cell(i, j, 0));
+  const unsigned xf_o = static_cast<unsigned>(g.xf(i, j, 0));
+  const unsigned yf_o = static_cast<unsigned>(g.yf(i, j, 0));
   struct Faces {
     R u0, u1, v0, v1, w1, ub0, ub1, vb0, vb1, wb1;
   };
   auto load_faces = [&](int k) -> Faces {
     Faces F{};
-    const unsigned xf0 = static_cast<unsigned>(g.xf(ic, jc, k));
-    const unsigned yf0 = static_cast<unsigned>(g.yf(ic, jc, k));
+    const unsigned xf0 = xf_c + static_cast<unsigned>(k) * sxk;
+    const unsigned yf0 = yf_c + static_cast<unsigned>(k) * syk;
     const unsigned yf1 = yf0 + static_cast<unsigned>(g.pxc);
-    const unsigned zf1 = static_cast<unsigned>(g.zf(ic, jc, k + 1));
+    const unsigned zf1 = zf_c + static_cast<unsigned>(k + 1) * pc;
     if constexpr (!FIRST) {
       F.u0 = ld32(a.u, xf0);
       F.u1 = ld32(a.u, xf0 + 1);
@@ -328,9 +338,9 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
         const R floor = PERT ? R(1.0e-6) - LV(k, LV_BRHO) : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
-      const unsigned c0 = static_cast<unsigned>(g.cell(i, j, k));
-      const unsigned xf0 = static_cast<unsigned>(g.xf(i, j, k));
-      const unsigned yf0 = static_cast<unsigned>(g.yf(i, j, k));
+      const unsigned c0 = c_o + static_cast<unsigned>(k) * pc;
+      const unsigned xf0 = xf_o + static_cast<unsigned>(k) * sxk;
+      const unsigned yf0 = yf_o + static_cast<unsigned>(k) * syk;
       st32(a.rho_out, c0, rho_new);
       if (a.rt_out)
         st32(a.rt_out, c0,

@@ -98,46 +98,69 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
   }
 }
 
-// One wave per level: lane l sums partials l, l+64, ... in order, with four
-// independent accumulators so the loads stay in flight, then a fixed
-// butterfly adds the lanes, so the result does not depend on scheduling.  A
-// single wave keeps the launch short: the kernel runs between every two
-// fast substeps.
+// One 256-thread block per level: thread t sums partials t, t+256, ... in
+// order over eight interleaved accumulators (eight loads in flight per
+// round: the kernel is bound by load latency, not bytes), a fixed butterfly
+// adds each wave's lanes and thread 0 adds the four waves in order, so the
+// result does not depend on scheduling.  The kernel runs between every two
+// fast substeps, so its length is the round trips it waits for.
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
+constexpr int kFinalizeThreads = 256;
+
 template <class R>
-__global__ __launch_bounds__(64) void k_level_finalize(int nz, int nchunk,
-                                                        const double* partials, int mode,
-                                                        double count, R* out0, R* out1) {
+__global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize(int nz, int nchunk,
+                                                                     const double* partials,
+                                                                     int mode, double count,
+                                                                     R* out0, R* out1) {
+  __shared__ double sh[2][kFinalizeThreads / 64];
   const int k = blockIdx.x;
   const int t = threadIdx.x;
-  const double* p = partials + static_cast<size_t>(k) * nchunk * 2;
-  double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-  int c = t;
-  for (; c + 192 < nchunk; c += 256) {
+  const double2* p = reinterpret_cast<const double2*>(partials) + static_cast<size_t>(k) * nchunk;
+  double a0[8], a1[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a0[q] += p[(c + 64 * q) * 2 + 0];
-      a1[q] += p[(c + 64 * q) * 2 + 1];
+  for (int q = 0; q < 8; ++q) a0[q] = a1[q] = 0.0;
+  int c = t;
+  for (; c + 7 * kFinalizeThreads < nchunk; c += 8 * kFinalizeThreads) {
+    double2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = p[c + q * kFinalizeThreads];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      a0[q] += v[q].x;
+      a1[q] += v[q].y;
     }
   }
-  for (int q = 0; c < nchunk; c += 64, ++q) {
-    a0[q] += p[c * 2 + 0];
-    a1[q] += p[c * 2 + 1];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (c + q * kFinalizeThreads < nchunk) {
+      const double2 v = p[c + q * kFinalizeThreads];
+      a0[q] += v.x;
+      a1[q] += v.y;
+    }
   }
-  const double s0 = wave_sum64((a0[0] + a0[1]) + (a0[2] + a0[3]));
-  const double s1 = wave_sum64((a1[0] + a1[1]) + (a1[2] + a1[3]));
+  const double s0 = wave_sum64(((a0[0] + a0[1]) + (a0[2] + a0[3])) +
+                               ((a0[4] + a0[5]) + (a0[6] + a0[7])));
+  const double s1 = wave_sum64(((a1[0] + a1[1]) + (a1[2] + a1[3])) +
+                               ((a1[4] + a1[5]) + (a1[6] + a1[7])));
+  if ((t & 63) == 0) {
+    sh[0][t >> 6] = s0;
+    sh[1][t >> 6] = s1;
+  }
+  __syncthreads();
   if (t != 0) return;
+  const double S0 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+  const double S1 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
   if (mode == 0) {
-    out0[k] = static_cast<R>(s0 / count);  // dry_core.cu:485-490
+    out0[k] = static_cast<R>(S0 / count);  // dry_core.cu:485-490
   } else {
     const R inv = R(1) / static_cast<R>(count);  // dry_fast_modes.cu:481-482
-    out0[k] = static_cast<R>(s0) * inv;
-    out1[k] = static_cast<R>(s1) * inv;
+    out0[k] = static_cast<R>(S0) * inv;
+    out1[k] = static_cast<R>(S1) * inv;
   }
 }
 
@@ -571,7 +594,7 @@ int level_partials_chunks(int ny) { return (ny + kPartialRows - 1) / kPartialRow
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1) {
-  hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(64), 0, st, nz, nchunk, partials,
+  hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(kFinalizeThreads), 0, st, nz, nchunk, partials,
                      mode, count, o0, o1);
 }
 template <class R>

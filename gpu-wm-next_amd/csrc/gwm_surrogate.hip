@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
     const long cell = tile * kTile + wv * 64 + lane;
     const bool valid = cell < n;
     for (int q = 0; q < 8; ++q)
-      sfeat[wv][lane][q] = (valid && q < kSurIn) ? (x[cell * kSurIn + q] - nm.xm[q]) / nm.xs[q]
+      sfeat[wv][lane][q] = (valid && q < kSurIn) ? qdiv(x[cell * kSurIn + q] - nm.xm[q], nm.xs[q])
                                                  : 0.0f;
     __syncthreads();
     tile_mlp(R, sb, sfeat[wv], sout[wv], lane);
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
     bool inbox = true;
     for (int q = 0; q < kSurIn; ++q) {
       inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
-      sfeat[wv][lane][q] = (f[q] - nm.xm[q]) / nm.xs[q];
+      sfeat[wv][lane][q] = qdiv(f[q] - nm.xm[q], nm.xs[q]);
     }
     sfeat[wv][lane][7] = 0.0f;
     __syncthreads();
