@@ -33,8 +33,8 @@
 // base faces and writes full faces (m_base + departure) and rho*theta.
 //
 // Flat terrain: the level means of rho and p for the next substep are
-// emitted as per-wave FP64 partial sums of the new rho (fixed order: a wave
-// butterfly, lane 0 writes), replacing a separate pass over the state.
+// emitted as per-wave FP64 partial sums of the new rho (fixed order: a DPP
+// reduction, lane 63 writes), replacing a separate pass over the state.
 #include "gwm_kernels.hpp"
 
 namespace gwm {
@@ -44,9 +44,30 @@ namespace {
 constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, PN = PX * PY;
 constexpr int NHALO = PN - TX * TY;  // 148 ring cells
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// One DPP move of a double (both halves); lanes of rows outside ROWS hold
+// unspecified values.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(u & 0xffffffffull), CTRL, ROWS, 0xF,
+                                          false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(u >> 32), CTRL, ROWS, 0xF, false);
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                        static_cast<unsigned>(lo));
+}
+// Sum over the 64 lanes of a wave (all lanes active), complete in lane 63
+// only: lane pairs and quads (quad_perm), half rows (row_half_mirror), rows
+// (row_mirror), then rows 0+1 into row 1 and 2+3 into row 3 (row_bcast:15)
+// and rows 0-3 into row 3 (row_bcast:31; rows outside a broadcast end with
+// values lane 63 never reads).  DPP moves instead of LDS permutes, in a
+// fixed order.
+__device__ inline double wave_sum_to63(double v) {
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xF>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
   return v;
 }
 
@@ -362,9 +383,9 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
         sr = static_cast<double>(rho_new);
         spp = static_cast<double>(press(rho_new, th_k, k));
       }
-      sr = wave_sum(sr);
-      spp = wave_sum(spp);
-      if (lx == 0) {
+      sr = wave_sum_to63(sr);
+      spp = wave_sum_to63(spp);
+      if (lx == TX - 1) {
         const size_t nw = static_cast<size_t>(tiles) * TY;  // one wave per tile row
         const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * TY + ly) * 2;
         part_out[slot] = sr;
