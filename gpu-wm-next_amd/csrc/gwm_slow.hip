@@ -197,15 +197,23 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   };
   // SSP base of the own cell and faces (mode 2), clamped for threads outside.
   const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  // Offsets of level kk are the level-0 offset plus kk times the field's
+  // level stride (kk is uniform, so the product is scalar work).
+  const unsigned q0c = static_cast<unsigned>(g.cell(ic, jc, 0));
+  const unsigned q0x = static_cast<unsigned>(g.xf(ic, jc, 0));
+  const unsigned q0y = static_cast<unsigned>(g.yf(ic, jc, 0));
+  const unsigned oc0 = static_cast<unsigned>(g.cell(i, j, 0));
+  const unsigned ox0 = static_cast<unsigned>(g.xf(i, j, 0));
+  const unsigned oy0 = static_cast<unsigned>(g.yf(i, j, 0));
   struct Q0 {
     R rho, rt, u, v, w;
   };
   auto load_q0 = [&](int kk) -> Q0 {
     Q0 q{};
     if constexpr (MODE == 2) {
-      const unsigned cc = static_cast<unsigned>(g.cell(ic, jc, kk));
-      const unsigned xx = static_cast<unsigned>(g.xf(ic, jc, kk));
-      const unsigned yy = static_cast<unsigned>(g.yf(ic, jc, kk));
+      const unsigned cc = q0c + static_cast<unsigned>(kk) * pc;
+      const unsigned xx = q0x + static_cast<unsigned>(kk) * pu;
+      const unsigned yy = q0y + static_cast<unsigned>(kk) * pv;
       q = Q0{ld32(a.q0.rho, cc), ld32(a.q0.rt, cc), ld32(a.q0.u, xx), ld32(a.q0.v, yy),
              ld32(a.q0.w, cc)};
     }
@@ -220,7 +228,9 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);
   };
 
-  const int kcap = k1 < nz - 1 ? k1 : nz - 1;  // highest level a chunk stages
+  // highest level a chunk stages (uniform; kept in a scalar register so the
+  // clamped level indices and their offsets stay scalar work)
+  const int kcap = __builtin_amdgcn_readfirstlane(k1 < nz - 1 ? k1 : nz - 1);
   if (k0 > 0) {
     stage_lev(load_lev(k0 - 1), k0 - 1);
     r_km1 = r_k1;
@@ -442,9 +452,11 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       auto upd = [&](R base, R tend, R floor) { return rmax(floor, base + dt * tend); };
       auto fin = [&](R q0v, R v) { return MODE == 2 ? a.wa * q0v + a.wb * v : v; };
 
-      const unsigned ci = static_cast<unsigned>(g.cell(i, j, k));
-      const unsigned xf = static_cast<unsigned>(g.xf(i, j, k));
-      const unsigned yf = static_cast<unsigned>(g.yf(i, j, k));
+      const unsigned ci = oc0 + static_cast<unsigned>(k) * pc;
+      const unsigned xf = ox0 + static_cast<unsigned>(k) * pu;
+      const unsigned yf = oy0 + static_cast<unsigned>(k) * pv;
+      // the extra faces: x face nx (i == nx-1), y face ny (j == ny-1), z face nz
+      const unsigned xfe = xf + 1u, yfe = yf + static_cast<unsigned>(g.pxc), zft = ci + pc;
       const R tu0 = tend_u(i, U), tv0 = tend_v(j, V), tw0 = tend_w();
       if constexpr (MODE == 0) {
         st32(a.out.rho, ci, t_rho);
@@ -452,9 +464,9 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         st32(a.out.u, xf, tu0);
         st32(a.out.v, yf, tv0);
         st32(a.out.w, ci, tw0);
-        if (i == g.nx - 1) a.out.u[g.xf(g.nx, j, k)] = tend_u(g.nx, Ug);
-        if (j == g.ny - 1) a.out.v[g.yf(i, g.ny, k)] = tend_v(g.ny, Vg);
-        if (k == nz - 1) a.out.w[g.zf(i, j, nz)] = R(0);
+        if (i == g.nx - 1) st32(a.out.u, xfe, tend_u(g.nx, Ug));
+        if (j == g.ny - 1) st32(a.out.v, yfe, tend_v(g.ny, Vg));
+        if (k == nz - 1) st32(a.out.w, zft, R(0));
       } else {
         const R mf = Phys<R>::mom_floor;
         // floors on the full values (departures: 1e-6 - base)
@@ -471,18 +483,14 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
         st32(a.out.u, xf, fin(Qc.u, upd(U(i, j, k), tu0, mf)));
         st32(a.out.v, yf, fin(Qc.v, upd(V(i, j, k), tv0, mf)));
         st32(a.out.w, ci, fin(Qc.w, upd(W(i, j, k), tw0, mf)));
-        if (i == g.nx - 1) {
-          const size_t f = g.xf(g.nx, j, k);
-          a.out.u[f] = fin(MODE == 2 ? a.q0.u[f] : R(0), upd(s.u[f], tend_u(g.nx, Ug), mf));
-        }
-        if (j == g.ny - 1) {
-          const size_t f = g.yf(i, g.ny, k);
-          a.out.v[f] = fin(MODE == 2 ? a.q0.v[f] : R(0), upd(s.v[f], tend_v(g.ny, Vg), mf));
-        }
-        if (k == nz - 1) {
-          const size_t f = g.zf(i, j, nz);
-          a.out.w[f] = fin(MODE == 2 ? a.q0.w[f] : R(0), upd(w_top, R(0), mf));
-        }
+        if (i == g.nx - 1)
+          st32(a.out.u, xfe,
+               fin(MODE == 2 ? ld32(a.q0.u, xfe) : R(0), upd(ld32(s.u, xfe), tend_u(g.nx, Ug), mf)));
+        if (j == g.ny - 1)
+          st32(a.out.v, yfe,
+               fin(MODE == 2 ? ld32(a.q0.v, yfe) : R(0), upd(ld32(s.v, yfe), tend_v(g.ny, Vg), mf)));
+        if (k == nz - 1)
+          st32(a.out.w, zft, fin(MODE == 2 ? ld32(a.q0.w, zft) : R(0), upd(w_top, R(0), mf)));
       }
     }
     r_km1 = r_k;
@@ -621,11 +629,18 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
   struct TF {
     R mw, me, ms, mn, mt, q0[NT];
   };
+  // Offsets of level k: level-0 offset plus k times the level stride.
+  const unsigned tpu = static_cast<unsigned>(g.pxu) * g.pyc;  // x faces
+  const unsigned tpv = static_cast<unsigned>(g.pxc) * g.pyv;  // y faces
+  const unsigned txf0 = static_cast<unsigned>(g.xf(ic, jc, 0));
+  const unsigned tyf0 = static_cast<unsigned>(g.yf(ic, jc, 0));
+  const unsigned tcc0 = static_cast<unsigned>(g.cell(ic, jc, 0));
+  const unsigned tci0 = static_cast<unsigned>(g.cell(i, j, 0));
   auto load_tf = [&](int k) -> TF {
     TF f{};
-    const unsigned xf = static_cast<unsigned>(g.xf(ic, jc, k));
-    const unsigned yf = static_cast<unsigned>(g.yf(ic, jc, k));
-    const unsigned cc = static_cast<unsigned>(g.cell(ic, jc, k));
+    const unsigned xf = txf0 + static_cast<unsigned>(k) * tpu;
+    const unsigned yf = tyf0 + static_cast<unsigned>(k) * tpv;
+    const unsigned cc = tcc0 + static_cast<unsigned>(k) * pc;
     f.mw = ld32(a.u, xf);
     f.me = ld32(a.u, xf + 1);
     f.ms = ld32(a.v, yf);
@@ -641,7 +656,9 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
   const R rdx = R(1) / dx, rdy = R(1) / dy;
   const int c0 = (lx + 1) + PX * (ly + 1);
 
-  const int kcap = k1 < nz - 1 ? k1 : nz - 1;  // highest level a chunk stages
+  // highest level a chunk stages (uniform; kept in a scalar register so the
+  // clamped level indices and their offsets stay scalar work)
+  const int kcap = __builtin_amdgcn_readfirstlane(k1 < nz - 1 ? k1 : nz - 1);
   // own raw rho*q of level k (the update's base)
   R own[NT];
   if (k0 > 0) stage_tlev(load_tlev(k0 - 1), k0 - 1);
@@ -667,7 +684,7 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
     if (active) {
       const int kb = k > 0 ? k - 1 : k, kt = k + 1 < nz ? k + 1 : k;
       const R idz = TERRAIN ? a.idz_c[g.metric(i, j, k)] : LV(k, 0);
-      const unsigned ci = static_cast<unsigned>(g.cell(i, j, k));
+      const unsigned ci = tci0 + static_cast<unsigned>(k) * pc;
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
         const R* q = sQ[n][k & 3];
