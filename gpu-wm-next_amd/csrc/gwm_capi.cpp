@@ -330,6 +330,10 @@ struct gwm_state {
   };
   Graph graphs[2];
   bool faces_synced = false;
+  // theta_ref partials of the state (rho_set) that a subcycle's last substep
+  // wrote, ready for the next stage's theta_ref (no pass over the state)
+  const void* theta_parts_rho = nullptr;
+  int theta_nparts = 0;
   // view of a stage set; a tracer stage view also points at the stage's
   // tracer buffers (empty: the owner's canonical tracers)
   void* vset[5] = {};
@@ -449,10 +453,13 @@ void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
   if (!S->theta_ref) S->theta_ref = dalloc(g.nz * sizeof(R));
   double* part = S->level_part();
   int nch = 0;
-  {
+  if (S->theta_parts_rho == rho) {  // emitted by the subcycle that wrote (rho, rt)
+    nch = S->theta_nparts;
+  } else {
     Timed t(S->ctx, "level_partials");
     gwm::launch_level_partials<R>(S->ctx->stream, g, rho, rt, 0, part, &nch, S->base<R>());
   }
+  S->theta_parts_rho = nullptr;
   {
     Timed t(S->ctx, "level_finalize");
     gwm::launch_level_finalize<R>(S->ctx->stream, g.nz, nch, part, 0,
@@ -513,7 +520,7 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
 // the entry faces.
 template <class R>
 int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* theta,
-             const R* ub, const R* vb, const R* wb, int z0, int z1) {
+             const R* ub, const R* vb, const R* wb, int z0, int z1, bool theta_parts = false) {
   const Geo& g = S->g();
   const gwm_grid* G = S->grid;
   const int ns = std::max(1, c.fast_substeps);
@@ -570,7 +577,11 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
     a.first = s == 0;
     a.last = s == ns - 1;
     a.base = S->base<R>();
-    const bool emit = !colref && s + 1 < ns;
+    // level sums for the next substep, or (last substep, on request) the
+    // theta_ref sums of the written state for the next stage
+    const bool emit_th = !colref && theta_parts && s + 1 == ns;
+    const bool emit = (!colref && s + 1 < ns) || emit_th;
+    a.part_theta = emit_th ? 1 : 0;
     auto run = [&](int n) {
       gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
                                  emit ? part : nullptr, emit ? &nparts : nullptr);
@@ -584,6 +595,10 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
     v = a.v_out;
     w = a.w_out;
     last = o;
+    if (emit_th) {
+      S->theta_parts_rho = a.rho_out;
+      S->theta_nparts = nparts;
+    }
   }
   return last;
 }
@@ -670,6 +685,7 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
             gwm_boundary_fn fast_fn = nullptr, void* fast_user = nullptr) {
   const bool bnd = b && (b->mode != GWM_BOUNDARY_NONE || b->callback);
   S->ensure_sets(bnd ? 5 : 4);
+  S->theta_parts_rho = nullptr;
   const int X = S->cur;
   int others[4], n = 0;
   for (int s = 0; s < S->nsets && n < 4; ++s)
@@ -687,7 +703,9 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
     in = S->set[Wb];
     run_boundary<R>(S, in, b, 0.0);
   }
-  auto sub = [&](int y) {
+  // theta_parts: the subcycle's last substep also emits the next stage's
+  // theta_ref sums; only without boundaries, which change the state between.
+  auto sub = [&](int y, bool theta_parts) {
     void** ys = S->set[y];
     if (fast_fn) {
       // Host plug-in: hand it the stage state (rho, rho*theta, faces) in Z0.
@@ -719,19 +737,19 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
     }
     return subcycle<R>(S, c, static_cast<const R*>(ys[0]), static_cast<const R*>(ys[1]),
                        static_cast<const R*>(ys[2]), static_cast<const R*>(ys[3]),
-                       static_cast<const R*>(ys[4]), Z0, Z1);
+                       static_cast<const R*>(ys[4]), Z0, Z1, theta_parts);
   };
   // stage 1
   slow_stage<R>(S, c, in, nullptr, S->set[Y], 1, R(0), R(1), 1);
-  int r = sub(Y);
+  int r = sub(Y, !bnd);
   run_boundary<R>(S, S->set[r], b, c.dt);
   // stage 2
   slow_stage<R>(S, c, S->set[r], S->set[X], S->set[Y], 2, R(0.75), R(0.25), 1);
-  r = sub(Y);
+  r = sub(Y, !bnd);
   run_boundary<R>(S, S->set[r], b, 0.5 * c.dt);
   // stage 3
   slow_stage<R>(S, c, S->set[r], S->set[X], S->set[Y], 2, R(1) / R(3), R(2) / R(3), 1);
-  r = sub(Y);
+  r = sub(Y, false);
   S->cur = r;
 }
 

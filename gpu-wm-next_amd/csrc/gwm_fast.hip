@@ -301,7 +301,7 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
       rref_k1 = q_rho_from_p_theta(pn, thk);
     }
 
-    R rho_new = R(0);
+    R rho_new = R(0), rt_new = R(0);
     if (active) {
       // dp/dz at a neighbour column, from the LDS planes (dry_pressure_gradient.cu:37-73).
       auto vgrad = [&](int off, size_t m) -> R {
@@ -363,9 +363,8 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
       const unsigned xf0 = xf_o + static_cast<unsigned>(k) * sxk;
       const unsigned yf0 = yf_o + static_cast<unsigned>(k) * syk;
       st32(a.rho_out, c0, rho_new);
-      if (a.rt_out)
-        st32(a.rt_out, c0,
-             PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k) : rho_new * th_k);
+      rt_new = PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k) : rho_new * th_k;
+      if (a.rt_out) st32(a.rt_out, c0, rt_new);
       // departures, or full faces base + departure on the last substep
       st32(a.u_out, xf0, LAST ? F.ub0 + u0 : u0);
       st32(a.v_out, yf0, LAST ? F.vb0 + v0 : v0);
@@ -380,8 +379,14 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
     if (part_out) {  // level sums of the next substep (flat refs, K6/K7)
       double sr = 0.0, spp = 0.0;
       if (active) {
-        sr = static_cast<double>(rho_new);
-        spp = static_cast<double>(press(rho_new, th_k, k));
+        if (a.part_theta) {  // theta_ref of the next stage (dry_core.cu:462-490)
+          sr = static_cast<double>(PERT ? theta_pert(rho_new, rt_new, LV(k, LV_BRHO),
+                                                     LV(k, LV_BTH))
+                                        : q_theta(rho_new, rt_new));
+        } else {
+          sr = static_cast<double>(rho_new);
+          spp = static_cast<double>(press(rho_new, th_k, k));
+        }
       }
       sr = wave_sum_to63(sr);
       spp = wave_sum_to63(spp);

@@ -36,6 +36,8 @@ struct FastArgs {
   R dtf, gravity;
   int horizontal, density, col_ref;
   Base<R> base;  // perturbation form: rho, rho_out, rt_out are departures, theta is theta'
+  int part_theta;  // partials (if emitted) are the level sums of theta of the written
+                   // (rho, rho*theta), for theta_ref, instead of rho and p
 };
 
 template <class R>
