@@ -558,16 +558,18 @@ def test_ensemble_steps_equal_member_steps():
         c.close()
 
 
-def test_theta_ref_from_last_substep_matches_separate_pass(ctx32):
+@pytest.mark.parametrize("prec", [4, 8])
+def test_theta_ref_from_last_substep_matches_separate_pass(ctx32, ctx64, prec):
     """Without boundaries the last fast substep of SSPRK3 stages 1-2 emits the
     next stage's theta_ref level sums; with a boundary callback (one that
     changes nothing) a separate pass over the state computes them.  Same
-    per-cell theta summed in another order, so the states agree to FP32
-    rounding; a wrong fused sum (another quantity, level or state) would move
+    per-cell theta summed in another order, so the states agree to the
+    working precision's rounding; a wrong fused sum (another quantity, level or state) would move
     theta_ref by O(1) K and the buoyancy with it."""
     nx, ny, nz = 96, 64, 24
     zi = np.linspace(0.0, 12000.0, nz + 1)
-    grid = G.Grid(ctx32, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    grid = G.Grid(ctx_for(prec, ctx32, ctx64), nx, ny, nz, dx=1000.0, dy=1000.0,
+                  z_interfaces=zi)
     a, b = G.State(grid, 3), G.State(grid, 3)
     for s in (a, b):
         s.init_convection(1.0e5, 7, 0.5, 1000.0)
@@ -588,4 +590,5 @@ def test_theta_ref_from_last_substep_matches_separate_pass(ctx32):
         x, y = a.get(f).astype(np.float64), b.get(f).astype(np.float64)
         scale = np.max(np.abs(y))
         assert np.all(np.isfinite(x)), f
-        assert np.max(np.abs(x - y)) <= 1e-5 * scale, (f, np.max(np.abs(x - y)) / scale)
+        tol = 1e-5 if prec == 4 else 1e-11
+        assert np.max(np.abs(x - y)) <= tol * scale, (f, np.max(np.abs(x - y)) / scale)
