@@ -44,33 +44,6 @@ namespace {
 constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, PN = PX * PY;
 constexpr int NHALO = PN - TX * TY;  // 148 ring cells
 
-// One DPP move of a double (both halves); lanes of rows outside ROWS hold
-// unspecified values.
-template <int CTRL, int ROWS>
-__device__ inline double dpp_f64(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(u & 0xffffffffull), CTRL, ROWS, 0xF,
-                                          false);
-  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(u >> 32), CTRL, ROWS, 0xF, false);
-  return __builtin_bit_cast(double, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
-                                        static_cast<unsigned>(lo));
-}
-// Sum over the 64 lanes of a wave (all lanes active), complete in lane 63
-// only: lane pairs and quads (quad_perm), half rows (row_half_mirror), rows
-// (row_mirror), then rows 0+1 into row 1 and 2+3 into row 3 (row_bcast:15)
-// and rows 0-3 into row 3 (row_bcast:31; rows outside a broadcast end with
-// values lane 63 never reads).  DPP moves instead of LDS permutes, in a
-// fixed order.
-__device__ inline double wave_sum_to63(double v) {
-  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f64<0x141, 0xF>(v);  // row_half_mirror
-  v += dpp_f64<0x140, 0xF>(v);  // row_mirror
-  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  return v;
-}
-
 }  // namespace
 
 // Per-level values of a chunk (levels k0-1 .. k1+1), staged in LDS once per

@@ -100,16 +100,11 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
 
 // One 256-thread block per level: thread t sums partials t, t+256, ... in
 // order over eight interleaved accumulators (eight loads in flight per
-// round: the kernel is bound by load latency, not bytes), a fixed butterfly
-// adds each wave's lanes and thread 0 adds the four waves in order, so the
-// result does not depend on scheduling.  The kernel runs between every two
-// fast substeps, so its length is the round trips it waits for.
-__device__ inline double wave_sum64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
+// round: the kernel is bound by load latency, not bytes), a fixed DPP
+// reduction adds each wave's lanes (total in lane 63) and thread 0 adds the
+// four waves in order, so the result does not depend on scheduling.  The
+// kernel runs between every two fast substeps, so its length is the round
+// trips it waits for.
 constexpr int kFinalizeThreads = 256;
 
 template <class R>
@@ -143,11 +138,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize(int nz, int
       a1[q] += v.y;
     }
   }
-  const double s0 = wave_sum64(((a0[0] + a0[1]) + (a0[2] + a0[3])) +
-                               ((a0[4] + a0[5]) + (a0[6] + a0[7])));
-  const double s1 = wave_sum64(((a1[0] + a1[1]) + (a1[2] + a1[3])) +
-                               ((a1[4] + a1[5]) + (a1[6] + a1[7])));
-  if ((t & 63) == 0) {
+  const double s0 = wave_sum_to63(((a0[0] + a0[1]) + (a0[2] + a0[3])) +
+                                  ((a0[4] + a0[5]) + (a0[6] + a0[7])));
+  const double s1 = wave_sum_to63(((a1[0] + a1[1]) + (a1[2] + a1[3])) +
+                                  ((a1[4] + a1[5]) + (a1[6] + a1[7])));
+  if ((t & 63) == 63) {
     sh[0][t >> 6] = s0;
     sh[1][t >> 6] = s1;
   }
