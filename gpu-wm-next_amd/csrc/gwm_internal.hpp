@@ -327,6 +327,22 @@ struct Loader {
   __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
 };
 
+// Work-group (bx, by) of a 2-D grid whose work-groups (x fastest) run on XCD
+// b mod 8, remapped so that each XCD holds a contiguous band of them.  Rows
+// start 4 B past a cache line, so work-groups that are neighbours in x share
+// the lines at their ends; on one XCD a shared line is fetched into one L2.
+__device__ inline void xcd_block2d(int& bx, int& by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  const int gx = gridDim.x, nb = gx * gridDim.y;
+  if ((nb & 7) == 0) {
+    const int b = bx + gx * by;
+    const int r = (b & 7) * (nb >> 3) + (b >> 3);
+    bx = r % gx;
+    by = r / gx;
+  }
+}
+
 // One DPP move of a double (both halves); lanes of rows outside ROWS hold
 // unspecified values.
 template <int CTRL, int ROWS>

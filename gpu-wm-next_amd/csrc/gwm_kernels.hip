@@ -206,20 +206,8 @@ __global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt, Base<R> ba
 template <class R, bool TERRAIN>
 __global__ void k_warm_rain(WarmRainArgs<R> a) {
   const Geo g = a.g;
-  // Work-group b (x fastest) runs on XCD b mod 8.  A row starts 4 B past a
-  // cache line, so neighbouring work-groups in x share the lines at their
-  // ends; the remap gives each XCD a contiguous band of work-groups, so a
-  // shared line is fetched into one L2 instead of two.
-  int bx = blockIdx.x, by = blockIdx.y;
-  {
-    const int gx = gridDim.x, nb = gx * gridDim.y;
-    if ((nb & 7) == 0) {
-      const int b = bx + gx * by;
-      const int r = (b & 7) * (nb >> 3) + (b >> 3);
-      bx = r % gx;
-      by = r / gx;
-    }
-  }
+  int bx, by;
+  xcd_block2d(bx, by);
   const int i = bx * blockDim.x + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;

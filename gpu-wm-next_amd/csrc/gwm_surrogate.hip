@@ -450,13 +450,17 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
 template <bool TERRAIN>
 __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
                           const float* lev_idz_c, float dt, float vt, float depth, int fallout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int bx, by;
+  xcd_block2d(bx, by);
+  const int i = bx * blockDim.x + threadIdx.x;
+  const int j = by * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
   const float dz_nom = fmaxf(depth, 1.0e-3f);
   const float frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
   const float rdz_nom = 1.0f / dz_nom;
   float inc = 0.0f;
+  // unrolled so that the independent loads of several levels are in flight
+#pragma unroll 4
   for (int k = g.nz - 1; k >= 0; --k) {
     const size_t c = g.cell(i, j, k);
     float rqr = qr[c];
