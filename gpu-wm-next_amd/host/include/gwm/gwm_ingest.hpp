@@ -1,39 +1,41 @@
 // gwm_ingest.hpp -- prepared-case input path of the C++ host mirror.
 //
-// The reference's gwm_prepared_case_driver starts from two JSON files (an
-// analysis state and a lateral-boundary cache, schemas
-// "gwm-next-analysis-state/v1" and "gwm-next-boundary-cache/v1"), builds a
-// hydrostatically balanced dry state with projected face momenta and the
-// warm-rain tracers, and during the run overwrites the lateral strips from
-// the boundary cache interpolated in time.  This header provides that path
-// with the reference's names and behaviour:
+// gwm_prepared_case_driver starts from two JSON documents, an analysis state
+// ("gwm-next-analysis-state/v1") and a lateral-boundary cache
+// ("gwm-next-boundary-cache/v1"), as the reference's driver does
+// (src/core/gwm_prepared_case_driver.cpp:354-509).  This header is organised
+// as a pipeline of its own:
 //
-//   include/gwm/ingest/canonical_ir.hpp      SourceKind, grid / field IR,
-//                                            HRRR and RRFS adapters
-//   include/gwm/ingest/analysis_state.hpp    AnalysisStateIR
-//   include/gwm/ingest/boundary_cache.hpp    BoundarySnapshotIR, BoundaryCacheIR
-//   include/gwm/ingest/source_catalog.hpp    to_string, make_adapter, ...
-//   include/gwm/ingest/runtime_case.hpp      loaders, validation, interpolation
-//                                            (src/ingest/runtime_case.cpp)
-//   include/gwm/ingest/prepared_case_init.hpp  layout, metrics, balanced
-//                                            states, tracers, balance
-//                                            diagnostics, boundary updaters
-//                                            (src/ingest/prepared_case_init.cpp)
-//   include/gwm/core/dry_thermo.hpp          host thermodynamics
-//   include/gwm/core/runtime_summary.hpp     summarize_runtime_state
-//                                            (src/core/runtime_summary.cpp)
+//   1. schema    the canonical field table of both documents (canonical
+//                names and the HRRR / RRFS source names that feed them);
+//   2. reading   a reader that walks a document with its key path and stops
+//                at the first problem, then one consistency check per IR;
+//   3. time      the snapshots around a forecast second and their blend;
+//   4. balance   a per-column hydrostatic balancer, face momenta from the
+//                balanced density, the barotropic projection of fully open
+//                domains (lexicographic SOR on a zero-padded potential) and
+//                w from column continuity;
+//   5. states    dry states and tracers of a layout, start-up diagnostics;
+//   6. strips    strip masks of the open sides and the boundary updaters,
+//                which also pack the strips for the device stepper;
+//   7. summary   runtime summary and its JSON.
 //
-// Everything here runs on the host at start-up or when the boundary
-// snapshot changes; the time step itself runs on the device through the
-// mirror's operators.  Arithmetic is the reference's FP32 (`real`).
+// The reference's API names (types, free functions, updater classes) are
+// kept so callers compile unchanged.  Numerics follow the reference where
+// the driver's results depend on them: the EOS, the hydrostatic trapezoid
+// recurrence with four fixed-point passes (prepared_case_init.cpp:320-384),
+// the rho*theta inversion with two corrections (38-50), the SOR constants
+// (omega 1.85, at most 2000 sweeps, stop at 1e-10; 126-170), up to three
+// projection passes stopping at 1e-8, and the linear snapshot blend
+// (runtime_case.cpp:205-228).  Arithmetic is FP32 (`real`) as there.
 #pragma once
 
 #include <algorithm>
-#include <cctype>
+#include <array>
 #include <cmath>
+#include <cstdint>
 #include <iomanip>
 #include <limits>
-#include <memory>
 #include <optional>
 #include <sstream>
 #include <string>
@@ -45,7 +47,7 @@
 #include "gwm/gwm_json.hpp"
 
 // ---------------------------------------------------------------------------
-// core: host thermodynamics (include/gwm/core/dry_thermo.hpp:15-44)
+// Host thermodynamics in FP32 (include/gwm/core/dry_thermo.hpp:15-44).
 // ---------------------------------------------------------------------------
 namespace gwm::core {
 
@@ -53,47 +55,48 @@ constexpr real kDryGasConstant = 287.05f;
 constexpr real kReferencePressure = 100000.0f;
 constexpr real kKappa = 0.2854f;
 
+inline real dry_cp() { return kDryGasConstant / kKappa; }
 inline real dry_theta_from_conserved(real rho_d, real rho_theta_m) {
   return rho_theta_m / std::max(rho_d, 1.0e-6f);
 }
+// p = p0 (R rho theta / p0)^(1 / (1 - kappa)), the argument floored at 1e-6.
 inline real dry_pressure_from_rho_theta_m(real rho_d, real rho_theta_m) {
-  const real theta = dry_theta_from_conserved(rho_d, rho_theta_m);
-  const real ratio = std::max(rho_d * kDryGasConstant * theta / kReferencePressure, 1.0e-6f);
-  return kReferencePressure * std::pow(ratio, 1.0f / (1.0f - kKappa));
+  const real arg = rho_d * kDryGasConstant * dry_theta_from_conserved(rho_d, rho_theta_m) /
+                   kReferencePressure;
+  return kReferencePressure * std::pow(std::max(arg, 1.0e-6f), 1.0f / (1.0f - kKappa));
 }
+// rho = p / (R theta Pi), Pi = (p / p0)^kappa, both floored.
 inline real dry_rho_from_pressure_theta(real pressure, real theta) {
   const real exner = std::pow(std::max(pressure / kReferencePressure, 1.0e-6f), kKappa);
   return pressure / std::max(kDryGasConstant * theta * exner, 1.0e-6f);
 }
-inline real dry_cp() { return kDryGasConstant / kKappa; }
 
 }  // namespace gwm::core
 
 namespace gwm::ingest {
 
-// ---------------------------------------------------------------------------
-// Canonical IR (include/gwm/ingest/canonical_ir.hpp:13-184)
-// ---------------------------------------------------------------------------
+// ===========================================================================
+// 1. Schema
+// ===========================================================================
 enum class SourceKind { HRRR, RRFS, RAP, GFS, ERA5, ECMWF };
 
 struct HorizontalGridIR {
   int nx = 0, ny = 0, nz = 0;
   double dx = 0.0, dy = 0.0, z_top = 0.0;
   double ref_lat = 0.0, ref_lon = 0.0, truelat1 = 0.0, truelat2 = 0.0, stand_lon = 0.0;
-  std::size_t cell_count_2d() const {
-    return static_cast<std::size_t>(nx) * static_cast<std::size_t>(ny);
-  }
+  std::size_t cell_count_2d() const { return static_cast<std::size_t>(nx) * ny; }
   std::size_t cell_count_3d() const { return cell_count_2d() * static_cast<std::size_t>(nz); }
 };
 
+// Named FP32 fields, x fastest, (nz, ny, nx) or (ny, nx).
 struct FieldBundle3D {
   std::unordered_map<std::string, std::vector<real>> values;
-  bool has(const std::string& n) const { return values.count(n) != 0; }
+  bool has(const std::string& n) const { return values.find(n) != values.end(); }
   std::size_t field_count() const { return values.size(); }
 };
 struct FieldBundle2D {
   std::unordered_map<std::string, std::vector<real>> values;
-  bool has(const std::string& n) const { return values.count(n) != 0; }
+  bool has(const std::string& n) const { return values.find(n) != values.end(); }
   std::size_t field_count() const { return values.size(); }
 };
 
@@ -101,119 +104,6 @@ struct SourceFieldBinding {
   std::string source_name;
   std::string canonical_name;
 };
-
-struct CanonicalAnalysisIR {
-  SourceKind source = SourceKind::HRRR;
-  HorizontalGridIR grid;
-  std::string cycle_time_utc, valid_time_utc;
-  int forecast_offset_seconds = 0;
-  std::unordered_map<std::string, std::string> metadata;
-  FieldBundle3D atmosphere;
-  FieldBundle2D surface, static_surface;
-  bool empty() const {
-    return atmosphere.values.empty() && surface.values.empty() && static_surface.values.empty();
-  }
-};
-
-// A source's field bindings (source GRIB name -> canonical name).  The
-// canonical names are the schema the runtime reads; the adapters differ only
-// in the source names.
-class SourceAdapter {
- public:
-  virtual ~SourceAdapter() = default;
-  virtual SourceKind kind() const = 0;
-  virtual std::string name() const = 0;
-  virtual std::vector<SourceFieldBinding> atmosphere_field_bindings() const = 0;
-  virtual std::vector<SourceFieldBinding> surface_field_bindings() const = 0;
-  virtual std::vector<SourceFieldBinding> static_field_bindings() const = 0;
-  virtual int boundary_interval_seconds() const = 0;
-  virtual std::vector<std::string> required_fields() const {
-    std::vector<std::string> out;
-    auto add =[&out](const std::vector<SourceFieldBinding>& b) {
-      for (const auto& x : b)
-        if (std::find(out.begin(), out.end(), x.source_name) == out.end())
-          out.push_back(x.source_name);
-    };
-    add(atmosphere_field_bindings());
-    add(surface_field_bindings());
-    add(static_field_bindings());
-    return out;
-  }
-};
-
-namespace detail {
-// Canonical names of the three groups, in the adapters' order.
-inline const std::vector<std::string>& canonical_atmosphere() {
-  static const std::vector<std::string> v = {"u_wind", "v_wind", "w_wind", "air_temperature",
-                                             "specific_humidity", "air_pressure",
-                                             "geopotential_height"};
-  return v;
-}
-inline const std::vector<std::string>& canonical_surface() {
-  static const std::vector<std::string> v = {"surface_pressure", "air_temperature_2m",
-                                             "specific_humidity_2m", "u_wind_10m", "v_wind_10m",
-                                             "skin_temperature"};
-  return v;
-}
-inline const std::vector<std::string>& canonical_static() {
-  static const std::vector<std::string> v = {"terrain_height", "land_mask", "land_use_index"};
-  return v;
-}
-inline std::vector<SourceFieldBinding> bind(const std::vector<std::string>& source,
-                                            const std::vector<std::string>& canonical) {
-  std::vector<SourceFieldBinding> out;
-  for (size_t n = 0; n < source.size(); ++n) out.push_back({source[n], canonical[n]});
-  return out;
-}
-}  // namespace detail
-
-class HrrrAdapter final : public SourceAdapter {
- public:
-  SourceKind kind() const override { return SourceKind::HRRR; }
-  std::string name() const override { return "HRRR"; }
-  std::vector<SourceFieldBinding> atmosphere_field_bindings() const override {
-    return detail::bind({"UGRD", "VGRD", "VVEL", "T", "SPFH", "PRES", "HGT"},
-                        detail::canonical_atmosphere());
-  }
-  std::vector<SourceFieldBinding> surface_field_bindings() const override {
-    return detail::bind({"PSFC", "T2", "Q2", "U10", "V10", "TSK"}, detail::canonical_surface());
-  }
-  std::vector<SourceFieldBinding> static_field_bindings() const override {
-    return detail::bind({"HGT", "LANDMASK", "LU_INDEX"}, detail::canonical_static());
-  }
-  int boundary_interval_seconds() const override { return 3600; }
-};
-
-class RrfsAdapter final : public SourceAdapter {
- public:
-  SourceKind kind() const override { return SourceKind::RRFS; }
-  std::string name() const override { return "RRFS"; }
-  std::vector<SourceFieldBinding> atmosphere_field_bindings() const override {
-    return detail::bind({"UGRD", "VGRD", "DZDT", "TMP", "SPFH", "PRES", "HGT"},
-                        detail::canonical_atmosphere());
-  }
-  std::vector<SourceFieldBinding> surface_field_bindings() const override {
-    return detail::bind({"PRES", "TMP_2M", "SPFH_2M", "UGRD_10M", "VGRD_10M", "TMP_SFC"},
-                        detail::canonical_surface());
-  }
-  std::vector<SourceFieldBinding> static_field_bindings() const override {
-    return detail::bind({"HGT_SFC", "LANDMASK", "LANDUSE"}, detail::canonical_static());
-  }
-  int boundary_interval_seconds() const override { return 3600; }
-};
-
-// ---------------------------------------------------------------------------
-// Analysis state and boundary cache (analysis_state.hpp:13-69,
-// boundary_cache.hpp:12-79)
-// ---------------------------------------------------------------------------
-namespace detail {
-template <class Bundle>
-bool has_all(const Bundle& b, const std::vector<SourceFieldBinding>& bindings) {
-  for (const auto& x : bindings)
-    if (!b.has(x.canonical_name)) return false;
-  return true;
-}
-}  // namespace detail
 
 struct AnalysisStateIR {
   SourceKind source = SourceKind::HRRR;
@@ -223,38 +113,6 @@ struct AnalysisStateIR {
   std::unordered_map<std::string, std::string> metadata;
   FieldBundle3D atmosphere;
   FieldBundle2D surface, static_surface;
-
-  bool has_required_fields(const SourceAdapter& a) const {
-    return detail::has_all(atmosphere, a.atmosphere_field_bindings()) &&
-           detail::has_all(surface, a.surface_field_bindings()) &&
-           detail::has_all(static_surface, a.static_field_bindings());
-  }
-  CanonicalAnalysisIR to_canonical() const {
-    CanonicalAnalysisIR ir;
-    ir.source = source;
-    ir.grid = grid;
-    ir.cycle_time_utc = cycle_time_utc;
-    ir.valid_time_utc = valid_time_utc;
-    ir.forecast_offset_seconds = forecast_offset_seconds;
-    ir.metadata = metadata;
-    ir.atmosphere = atmosphere;
-    ir.surface = surface;
-    ir.static_surface = static_surface;
-    return ir;
-  }
-  static AnalysisStateIR from_canonical(const CanonicalAnalysisIR& ir) {
-    AnalysisStateIR s;
-    s.source = ir.source;
-    s.grid = ir.grid;
-    s.cycle_time_utc = ir.cycle_time_utc;
-    s.valid_time_utc = ir.valid_time_utc;
-    s.forecast_offset_seconds = ir.forecast_offset_seconds;
-    s.metadata = ir.metadata;
-    s.atmosphere = ir.atmosphere;
-    s.surface = ir.surface;
-    s.static_surface = ir.static_surface;
-    return s;
-  }
 };
 
 struct BoundarySnapshotIR {
@@ -262,10 +120,6 @@ struct BoundarySnapshotIR {
   std::string valid_time_utc;
   FieldBundle3D atmosphere;
   FieldBundle2D surface;
-  bool has_required_fields(const SourceAdapter& a) const {
-    return detail::has_all(atmosphere, a.atmosphere_field_bindings()) &&
-           detail::has_all(surface, a.surface_field_bindings());
-  }
 };
 
 struct BoundaryCacheIR {
@@ -275,252 +129,349 @@ struct BoundaryCacheIR {
   int boundary_interval_seconds = 3600;
   std::unordered_map<std::string, std::string> metadata;
   std::vector<BoundarySnapshotIR> snapshots;
-
   bool empty() const { return snapshots.empty(); }
-  bool has_monotonic_offsets() const {
-    for (size_t n = 1; n < snapshots.size(); ++n)
-      if (snapshots[n - 1].forecast_offset_seconds >= snapshots[n].forecast_offset_seconds)
-        return false;
-    return true;
-  }
-  // The last snapshot at or before `t` and the first at or after it, each
-  // falling back to the first / last snapshot outside the cache's range.
-  std::pair<const BoundarySnapshotIR*, const BoundarySnapshotIR*> bracket(int t) const {
-    if (snapshots.empty()) return {nullptr, nullptr};
-    const BoundarySnapshotIR *lo = nullptr, *hi = nullptr;
-    for (const auto& s : snapshots) {
-      if (s.forecast_offset_seconds <= t) lo = &s;
-      if (s.forecast_offset_seconds >= t) {
-        hi = &s;
-        break;
-      }
-    }
-    return {lo ? lo : &snapshots.front(), hi ? hi : &snapshots.back()};
-  }
 };
 
-// ---------------------------------------------------------------------------
-// Source catalog (source_catalog.hpp:9-25, src/ingest/source_catalog.cpp)
-// ---------------------------------------------------------------------------
-struct ExternalToolDescriptor {
-  std::string name, local_repo_hint, role;
-  bool allowed_in_runtime_core = false;
-};
-
-inline std::string to_string(SourceKind k) {
-  switch (k) {
-    case SourceKind::HRRR: return "HRRR";
-    case SourceKind::RRFS: return "RRFS";
-    case SourceKind::RAP: return "RAP";
-    case SourceKind::GFS: return "GFS";
-    case SourceKind::ERA5: return "ERA5";
-    case SourceKind::ECMWF: return "ECMWF";
-  }
-  throw std::runtime_error("Unknown SourceKind");
-}
-inline SourceKind source_kind_from_string(const std::string& value) {
-  std::string u = value;
-  for (auto& c : u) c = static_cast<char>(std::toupper(static_cast<unsigned char>(c)));
-  for (SourceKind k : {SourceKind::HRRR, SourceKind::RRFS, SourceKind::RAP, SourceKind::GFS,
-                       SourceKind::ERA5, SourceKind::ECMWF})
-    if (u == to_string(k)) return k;
-  throw std::runtime_error("Unknown source string: " + value);
-}
-inline bool is_first_class_source(SourceKind k) {
-  return k == SourceKind::HRRR || k == SourceKind::RRFS;
-}
-inline std::unique_ptr<SourceAdapter> make_adapter(SourceKind k) {
-  if (k == SourceKind::HRRR) return std::make_unique<HrrrAdapter>();
-  if (k == SourceKind::RRFS) return std::make_unique<RrfsAdapter>();
-  return nullptr;  // RAP, GFS, ERA5, ECMWF: no runtime adapter
-}
-inline std::vector<std::unique_ptr<SourceAdapter>> make_default_adapters() {
-  std::vector<std::unique_ptr<SourceAdapter>> v;
-  v.push_back(std::make_unique<HrrrAdapter>());
-  v.push_back(std::make_unique<RrfsAdapter>());
-  return v;
-}
-// The reference lists its authors' external (non-runtime) tools here; none
-// of them is part of this build.
-inline std::vector<ExternalToolDescriptor> recommended_external_tools() { return {}; }
-
-// ---------------------------------------------------------------------------
-// Runtime case: loaders, validation, interpolation (runtime_case.hpp:8-27,
-// src/ingest/runtime_case.cpp)
-// ---------------------------------------------------------------------------
 struct PreparedRuntimeCase {
   AnalysisStateIR analysis;
   BoundaryCacheIR boundary_cache;
 };
 
-namespace detail {
+namespace schema {
 
-inline SourceKind parse_source_exact(const std::string& s) {
-  for (SourceKind k : {SourceKind::HRRR, SourceKind::RRFS, SourceKind::RAP, SourceKind::GFS,
-                       SourceKind::ERA5, SourceKind::ECMWF})
-    if (s == to_string(k)) return k;
-  throw std::runtime_error("Unsupported source string: " + s);
-}
+enum class Group { Atmosphere, Surface, Static };
 
-inline HorizontalGridIR parse_grid(const json::Value& v) {
-  HorizontalGridIR g;
-  g.nx = v.at("nx").as_int();
-  g.ny = v.at("ny").as_int();
-  g.nz = v.at("nz").as_int();
-  g.dx = v.at("dx").as_number();
-  g.dy = v.at("dy").as_number();
-  auto opt = [&v](const char* key, double& out) {
-    if (v.has(key)) out = v.at(key).as_number();
+// Canonical fields read by the runtime, with the source field each of the
+// two supported sources maps onto it (canonical_ir.hpp's HRRR and RRFS
+// adapters).  Static fields are part of the analysis state only.
+struct Row {
+  Group group;
+  const char* canonical;
+  const char* hrrr;
+  const char* rrfs;
+};
+inline const std::vector<Row>& table() {
+  static const std::vector<Row> rows = {
+      {Group::Atmosphere, "u_wind", "UGRD", "UGRD"},
+      {Group::Atmosphere, "v_wind", "VGRD", "VGRD"},
+      {Group::Atmosphere, "w_wind", "VVEL", "DZDT"},
+      {Group::Atmosphere, "air_temperature", "T", "TMP"},
+      {Group::Atmosphere, "specific_humidity", "SPFH", "SPFH"},
+      {Group::Atmosphere, "air_pressure", "PRES", "PRES"},
+      {Group::Atmosphere, "geopotential_height", "HGT", "HGT"},
+      {Group::Surface, "surface_pressure", "PSFC", "PRES"},
+      {Group::Surface, "air_temperature_2m", "T2", "TMP_2M"},
+      {Group::Surface, "specific_humidity_2m", "Q2", "SPFH_2M"},
+      {Group::Surface, "u_wind_10m", "U10", "UGRD_10M"},
+      {Group::Surface, "v_wind_10m", "V10", "VGRD_10M"},
+      {Group::Surface, "skin_temperature", "TSK", "TMP_SFC"},
+      {Group::Static, "terrain_height", "HGT", "HGT_SFC"},
+      {Group::Static, "land_mask", "LANDMASK", "LANDMASK"},
+      {Group::Static, "land_use_index", "LU_INDEX", "LANDUSE"},
   };
-  opt("z_top", g.z_top);
-  opt("ref_lat", g.ref_lat);
-  opt("ref_lon", g.ref_lon);
-  opt("truelat1", g.truelat1);
-  opt("truelat2", g.truelat2);
-  opt("stand_lon", g.stand_lon);
-  return g;
+  return rows;
 }
 
-template <class Bundle>
-Bundle parse_bundle(const json::Value& v) {
-  Bundle b;
-  for (const auto& [name, entry] : v.as_object()) b.values.emplace(name, json::to_real_vector(entry));
-  return b;
+constexpr std::array<SourceKind, 6> kAllSources = {SourceKind::HRRR, SourceKind::RRFS,
+                                                   SourceKind::RAP,  SourceKind::GFS,
+                                                   SourceKind::ERA5, SourceKind::ECMWF};
+constexpr std::array<const char*, 6> kSourceNames = {"HRRR", "RRFS", "RAP", "GFS", "ERA5", "ECMWF"};
+
+// HRRR and RRFS carry runtime field bindings; the others are catalogued only.
+inline bool has_bindings(SourceKind k) { return k == SourceKind::HRRR || k == SourceKind::RRFS; }
+
+inline const char* group_label(Group g) {
+  return g == Group::Atmosphere ? "atmosphere" : g == Group::Surface ? "surface" : "static_surface";
 }
 
-template <class Bundle>
-void require_sizes(const Bundle& b, std::size_t n, const char* label) {
-  for (const auto& [name, values] : b.values)
-    require(values.size() == n, std::string(label) + " field has wrong size: " + name);
+}  // namespace schema
+
+inline std::string to_string(SourceKind k) {
+  return schema::kSourceNames[static_cast<std::size_t>(k)];
 }
 
-template <class Bundle>
-Bundle blend(const Bundle& lo, const Bundle& hi, real alpha, const char* what) {
-  require(lo.values.size() == hi.values.size(),
-          std::string("Boundary ") + what + " bundle count mismatch");
-  Bundle out;
-  for (const auto& [name, a] : lo.values) {
-    const auto it = hi.values.find(name);
-    require(it != hi.values.end(),
-            std::string("Upper boundary ") + what + " is missing field: " + name);
-    const auto& b = it->second;
-    require(a.size() == b.size(), std::string("Boundary ") + what + " field size mismatch: " + name);
-    auto& d = out.values[name];
-    d.resize(a.size());
-    for (size_t n = 0; n < a.size(); ++n) d[n] = (1.0f - alpha) * a[n] + alpha * b[n];
-  }
+// Source-field bindings of one group for a source with runtime bindings
+// (empty for the catalogued-only sources).
+inline std::vector<SourceFieldBinding> source_field_bindings(SourceKind k, schema::Group g) {
+  std::vector<SourceFieldBinding> out;
+  if (!schema::has_bindings(k)) return out;
+  for (const auto& r : schema::table())
+    if (r.group == g) out.push_back({k == SourceKind::HRRR ? r.hrrr : r.rrfs, r.canonical});
   return out;
 }
 
-}  // namespace detail
+// ===========================================================================
+// 2. Reading and checking
+// ===========================================================================
+namespace read {
 
-inline void validate_runtime_analysis(const AnalysisStateIR& a) {
-  require(a.grid.nx > 0 && a.grid.ny > 0 && a.grid.nz > 0,
-          "Analysis grid dimensions must be positive");
-  require(a.grid.dx > 0.0 && a.grid.dy > 0.0 && a.grid.z_top > 0.0,
-          "Analysis grid spacing and z_top must be positive");
-  const auto adapter = make_adapter(a.source);
-  require(adapter != nullptr, "Analysis source is unsupported by runtime adapter");
-  require(a.has_required_fields(*adapter), "Analysis state is missing required canonical fields");
-  detail::require_sizes(a.atmosphere, a.grid.cell_count_3d(), "Analysis atmosphere");
-  detail::require_sizes(a.surface, a.grid.cell_count_2d(), "Analysis surface");
-  detail::require_sizes(a.static_surface, a.grid.cell_count_2d(), "Analysis static_surface");
+// A JSON node with the key path that led to it, so every problem names the
+// document and the key.
+class Node {
+ public:
+  Node(const json::Value& v, std::string where) : v_(&v), where_(std::move(where)) {}
+  Node operator[](const std::string& key) const {
+    if (!v_->is_object()) fail("is not an object");
+    const auto& o = v_->as_object();
+    const auto it = o.find(key);
+    if (it == o.end()) throw std::runtime_error(where_ + ": required key '" + key + "' is absent");
+    return Node(it->second, where_ + "." + key);
+  }
+  bool has(const std::string& key) const { return v_->has(key); }
+  int integer() const {
+    const double d = number();
+    if (d != std::floor(d) || std::fabs(d) > 2.0e9) fail("is not an integer");
+    return static_cast<int>(d);
+  }
+  double number() const {
+    if (!v_->is_number()) fail("is not a number");
+    return v_->as_number();
+  }
+  std::string text() const {
+    if (!v_->is_string()) fail("is not a string");
+    return v_->as_string();
+  }
+  std::unordered_map<std::string, std::string> labels() const {
+    if (!v_->is_object()) fail("is not an object");
+    return json::to_string_map(*v_);
+  }
+  template <class Bundle>
+  Bundle fields() const {
+    if (!v_->is_object()) fail("is not an object of named fields");
+    Bundle b;
+    for (const auto& kv : v_->as_object()) {
+      if (!(kv.second.is_numbers() || kv.second.is_array()))
+        throw std::runtime_error(where_ + "." + kv.first + ": a field must be a list of numbers");
+      b.values.emplace(kv.first, json::to_real_vector(kv.second));
+    }
+    return b;
+  }
+  std::vector<Node> items() const {
+    if (!(v_->is_array() || v_->is_numbers())) fail("is not a list");
+    std::vector<Node> out;
+    const auto& a = v_->as_array();
+    for (std::size_t n = 0; n < a.size(); ++n)
+      out.emplace_back(a[n], where_ + "[" + std::to_string(n) + "]");
+    return out;
+  }
+  [[noreturn]] void fail(const std::string& what) const {
+    throw std::runtime_error(where_ + " " + what);
+  }
+
+ private:
+  const json::Value* v_;
+  std::string where_;
+};
+
+inline SourceKind source_of(const Node& n) {
+  const std::string s = n.text();
+  for (std::size_t q = 0; q < schema::kAllSources.size(); ++q)
+    if (s == schema::kSourceNames[q]) return schema::kAllSources[q];
+  n.fail("names unknown source '" + s + "' (known: HRRR, RRFS, RAP, GFS, ERA5, ECMWF)");
 }
 
+inline HorizontalGridIR grid_of(const Node& n) {
+  HorizontalGridIR g;
+  g.nx = n["nx"].integer();
+  g.ny = n["ny"].integer();
+  g.nz = n["nz"].integer();
+  g.dx = n["dx"].number();
+  g.dy = n["dy"].number();
+  double* optional[] = {&g.z_top, &g.ref_lat, &g.ref_lon, &g.truelat1, &g.truelat2, &g.stand_lon};
+  const char* keys[] = {"z_top", "ref_lat", "ref_lon", "truelat1", "truelat2", "stand_lon"};
+  for (int q = 0; q < 6; ++q)
+    if (n.has(keys[q])) *optional[q] = n[keys[q]].number();
+  return g;
+}
+
+inline void expect_schema(const Node& root, const char* schema) {
+  const std::string got = root["schema_version"].text();
+  if (got != schema)
+    throw std::runtime_error(std::string("document declares schema '") + got + "', expected '" +
+                             schema + "'");
+}
+
+// The first problem of a field bundle against the schema and the grid, or
+// an empty string.
+template <class Bundle>
+std::string bundle_problem(const Bundle& b, SourceKind src, schema::Group g, std::size_t n,
+                           const std::string& owner) {
+  for (const auto& r : schema::table())
+    if (r.group == g && !b.has(r.canonical))
+      return owner + " " + schema::group_label(g) + " lacks the canonical field '" +
+             r.canonical + "' (" + to_string(src) + " " +
+             (src == SourceKind::HRRR ? r.hrrr : r.rrfs) + ")";
+  for (const auto& kv : b.values)
+    if (kv.second.size() != n)
+      return owner + " " + schema::group_label(g) + " field '" + kv.first + "' holds " +
+             std::to_string(kv.second.size()) + " values; the grid needs " + std::to_string(n);
+  return {};
+}
+
+inline std::string grid_problem(const HorizontalGridIR& g, const std::string& owner,
+                                bool need_top) {
+  if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0) return owner + " grid needs positive nx, ny and nz";
+  if (!(g.dx > 0.0) || !(g.dy > 0.0)) return owner + " grid needs positive dx and dy";
+  if (need_top && !(g.z_top > 0.0)) return owner + " grid needs a positive z_top";
+  return {};
+}
+
+inline void raise_if(const std::string& problem) {
+  if (!problem.empty()) throw std::runtime_error(problem);
+}
+
+}  // namespace read
+
+// Consistency of an analysis state (runtime_case.cpp:230-253 checks the same
+// properties): grid, source with runtime bindings, every canonical field of
+// the three groups, field sizes.
+inline void validate_runtime_analysis(const AnalysisStateIR& a) {
+  const std::string who = "analysis";
+  read::raise_if(read::grid_problem(a.grid, who, true));
+  if (!schema::has_bindings(a.source))
+    throw std::runtime_error("analysis source " + to_string(a.source) +
+                             " has no runtime field bindings (HRRR and RRFS do)");
+  read::raise_if(read::bundle_problem(a.atmosphere, a.source, schema::Group::Atmosphere,
+                                      a.grid.cell_count_3d(), who));
+  read::raise_if(read::bundle_problem(a.surface, a.source, schema::Group::Surface,
+                                      a.grid.cell_count_2d(), who));
+  read::raise_if(read::bundle_problem(a.static_surface, a.source, schema::Group::Static,
+                                      a.grid.cell_count_2d(), who));
+}
+
+// Consistency of a boundary cache (runtime_case.cpp:255-274 checks the same
+// properties): grid, interval, source with runtime bindings, at least one
+// snapshot, strictly increasing offsets, every snapshot's fields.
 inline void validate_runtime_boundary_cache(const BoundaryCacheIR& c) {
-  require(c.grid.nx > 0 && c.grid.ny > 0 && c.grid.nz > 0,
-          "Boundary-cache grid dimensions must be positive");
-  require(c.boundary_interval_seconds > 0, "Boundary-cache interval must be positive");
-  const auto adapter = make_adapter(c.source);
-  require(adapter != nullptr, "Boundary-cache source is unsupported by runtime adapter");
-  require(!c.empty(), "Boundary cache must contain snapshots");
-  require(c.has_monotonic_offsets(), "Boundary cache offsets must be strictly monotonic");
-  for (const auto& s : c.snapshots) {
-    require(s.has_required_fields(*adapter),
-            "Boundary snapshot is missing required canonical fields");
-    detail::require_sizes(s.atmosphere, c.grid.cell_count_3d(), "Boundary atmosphere");
-    detail::require_sizes(s.surface, c.grid.cell_count_2d(), "Boundary surface");
+  const std::string who = "boundary cache";
+  read::raise_if(read::grid_problem(c.grid, who, false));
+  if (c.boundary_interval_seconds <= 0)
+    throw std::runtime_error("boundary cache needs a positive boundary_interval_seconds");
+  if (!schema::has_bindings(c.source))
+    throw std::runtime_error("boundary-cache source " + to_string(c.source) +
+                             " has no runtime field bindings (HRRR and RRFS do)");
+  if (c.snapshots.empty()) throw std::runtime_error("boundary cache holds no snapshots");
+  for (std::size_t n = 0; n < c.snapshots.size(); ++n) {
+    const auto& s = c.snapshots[n];
+    if (n > 0 && s.forecast_offset_seconds <= c.snapshots[n - 1].forecast_offset_seconds)
+      throw std::runtime_error("boundary snapshots must have strictly increasing forecast "
+                               "offsets (snapshot " + std::to_string(n) + " at " +
+                               std::to_string(s.forecast_offset_seconds) + " s follows " +
+                               std::to_string(c.snapshots[n - 1].forecast_offset_seconds) + " s)");
+    const std::string tag = "boundary snapshot " + std::to_string(n);
+    read::raise_if(read::bundle_problem(s.atmosphere, c.source, schema::Group::Atmosphere,
+                                        c.grid.cell_count_3d(), tag));
+    read::raise_if(read::bundle_problem(s.surface, c.source, schema::Group::Surface,
+                                        c.grid.cell_count_2d(), tag));
   }
 }
 
 inline AnalysisStateIR load_analysis_state_json(const std::string& path) {
-  const auto root = json::parse_file(path);
-  require(root.at("schema_version").as_string() == "gwm-next-analysis-state/v1",
-          "Unsupported analysis-state schema");
+  const json::Value doc = json::parse_file(path);
+  const read::Node root(doc, path);
+  read::expect_schema(root, "gwm-next-analysis-state/v1");
   AnalysisStateIR a;
-  a.source = detail::parse_source_exact(root.at("source").as_string());
-  a.grid = detail::parse_grid(root.at("grid"));
-  a.cycle_time_utc = root.at("cycle_time_utc").as_string();
-  a.valid_time_utc = root.at("valid_time_utc").as_string();
-  a.forecast_offset_seconds = root.at("forecast_offset_seconds").as_int();
-  a.metadata = json::to_string_map(root.at("metadata"));
-  a.atmosphere = detail::parse_bundle<FieldBundle3D>(root.at("atmosphere"));
-  a.surface = detail::parse_bundle<FieldBundle2D>(root.at("surface"));
-  a.static_surface = detail::parse_bundle<FieldBundle2D>(root.at("static_surface"));
+  a.source = read::source_of(root["source"]);
+  a.grid = read::grid_of(root["grid"]);
+  a.cycle_time_utc = root["cycle_time_utc"].text();
+  a.valid_time_utc = root["valid_time_utc"].text();
+  a.forecast_offset_seconds = root["forecast_offset_seconds"].integer();
+  a.metadata = root["metadata"].labels();
+  a.atmosphere = root["atmosphere"].fields<FieldBundle3D>();
+  a.surface = root["surface"].fields<FieldBundle2D>();
+  a.static_surface = root["static_surface"].fields<FieldBundle2D>();
   validate_runtime_analysis(a);
   return a;
 }
 
 inline BoundaryCacheIR load_boundary_cache_json(const std::string& path) {
-  const auto root = json::parse_file(path);
-  require(root.at("schema_version").as_string() == "gwm-next-boundary-cache/v1",
-          "Unsupported boundary-cache schema");
+  const json::Value doc = json::parse_file(path);
+  const read::Node root(doc, path);
+  read::expect_schema(root, "gwm-next-boundary-cache/v1");
   BoundaryCacheIR c;
-  c.source = detail::parse_source_exact(root.at("source").as_string());
-  c.grid = detail::parse_grid(root.at("grid"));
-  c.cycle_time_utc = root.at("cycle_time_utc").as_string();
-  c.boundary_interval_seconds = root.at("boundary_interval_seconds").as_int();
-  c.metadata = json::to_string_map(root.at("metadata"));
-  for (const auto& sv : root.at("snapshots").as_array()) {
-    BoundarySnapshotIR s;
-    s.forecast_offset_seconds = sv.at("forecast_offset_seconds").as_int();
-    s.valid_time_utc = sv.at("valid_time_utc").as_string();
-    s.atmosphere = detail::parse_bundle<FieldBundle3D>(sv.at("atmosphere"));
-    s.surface = detail::parse_bundle<FieldBundle2D>(sv.at("surface"));
-    c.snapshots.push_back(std::move(s));
-  }
+  c.source = read::source_of(root["source"]);
+  c.grid = read::grid_of(root["grid"]);
+  c.cycle_time_utc = root["cycle_time_utc"].text();
+  c.boundary_interval_seconds = root["boundary_interval_seconds"].integer();
+  c.metadata = root["metadata"].labels();
+  for (const auto& s : root["snapshots"].items())
+    c.snapshots.push_back(BoundarySnapshotIR{s["forecast_offset_seconds"].integer(),
+                                             s["valid_time_utc"].text(),
+                                             s["atmosphere"].fields<FieldBundle3D>(),
+                                             s["surface"].fields<FieldBundle2D>()});
   validate_runtime_boundary_cache(c);
   return c;
 }
 
 inline PreparedRuntimeCase load_prepared_runtime_case(const std::string& analysis_path,
                                                       const std::string& boundary_path) {
-  PreparedRuntimeCase rc;
-  rc.analysis = load_analysis_state_json(analysis_path);
-  rc.boundary_cache = load_boundary_cache_json(boundary_path);
-  require(rc.analysis.source == rc.boundary_cache.source,
-          "Analysis and boundary cache source mismatch");
-  require(rc.analysis.grid.nx == rc.boundary_cache.grid.nx &&
-              rc.analysis.grid.ny == rc.boundary_cache.grid.ny &&
-              rc.analysis.grid.nz == rc.boundary_cache.grid.nz,
-          "Analysis and boundary cache grid mismatch");
+  PreparedRuntimeCase rc{load_analysis_state_json(analysis_path),
+                         load_boundary_cache_json(boundary_path)};
+  const auto& ga = rc.analysis.grid;
+  const auto& gb = rc.boundary_cache.grid;
+  if (rc.analysis.source != rc.boundary_cache.source)
+    throw std::runtime_error("analysis source " + to_string(rc.analysis.source) +
+                             " and boundary-cache source " + to_string(rc.boundary_cache.source) +
+                             " differ");
+  if (ga.nx != gb.nx || ga.ny != gb.ny || ga.nz != gb.nz)
+    throw std::runtime_error("analysis grid " + std::to_string(ga.nx) + "x" +
+                             std::to_string(ga.ny) + "x" + std::to_string(ga.nz) +
+                             " and boundary-cache grid " + std::to_string(gb.nx) + "x" +
+                             std::to_string(gb.ny) + "x" + std::to_string(gb.nz) + " differ");
   return rc;
 }
 
-// Linear interpolation in time between the bracketing snapshots.
+// ===========================================================================
+// 3. Time: the snapshot at a forecast second
+// ===========================================================================
+namespace time_blend {
+
+// (1 - alpha) a + alpha b for every field of a; b must hold the same fields.
+template <class Bundle>
+Bundle mix(const Bundle& a, const Bundle& b, real alpha) {
+  if (a.values.size() != b.values.size())
+    throw std::runtime_error("boundary snapshots to blend hold different field sets");
+  Bundle out;
+  for (const auto& kv : a.values) {
+    const auto it = b.values.find(kv.first);
+    if (it == b.values.end() || it->second.size() != kv.second.size())
+      throw std::runtime_error("boundary snapshots to blend differ in field '" + kv.first + "'");
+    std::vector<real> v(kv.second.size());
+    const real keep = 1.0f - alpha;
+    for (std::size_t n = 0; n < v.size(); ++n) v[n] = keep * kv.second[n] + alpha * it->second[n];
+    out.values.emplace(kv.first, std::move(v));
+  }
+  return out;
+}
+
+}  // namespace time_blend
+
+// The boundary snapshot at forecast second t: the latest snapshot at or
+// before t and the earliest at or after it, blended linearly; before the
+// first or after the last snapshot, that snapshot (runtime_case.cpp:205-228).
 inline BoundarySnapshotIR interpolate_boundary_snapshot(const BoundaryCacheIR& c, int t) {
   validate_runtime_boundary_cache(c);
-  const auto [lo, hi] = c.bracket(t);
-  require(lo != nullptr && hi != nullptr, "Boundary cache bracket returned null snapshot");
-  if (lo == hi || lo->forecast_offset_seconds == hi->forecast_offset_seconds) return *lo;
+  const auto& s = c.snapshots;
+  // first snapshot with offset >= t (offsets increase strictly)
+  const auto hi = std::lower_bound(s.begin(), s.end(), t, [](const BoundarySnapshotIR& x, int v) {
+    return x.forecast_offset_seconds < v;
+  });
+  if (hi == s.end()) return s.back();
+  if (hi->forecast_offset_seconds == t || hi == s.begin()) return *hi;
+  const auto lo = hi - 1;
   const real alpha = static_cast<real>(t - lo->forecast_offset_seconds) /
                      static_cast<real>(hi->forecast_offset_seconds - lo->forecast_offset_seconds);
   BoundarySnapshotIR out;
   out.forecast_offset_seconds = t;
   out.valid_time_utc = lo->valid_time_utc + "->" + hi->valid_time_utc;
-  out.atmosphere = detail::blend(lo->atmosphere, hi->atmosphere, alpha, "atmosphere");
-  out.surface = detail::blend(lo->surface, hi->surface, alpha, "surface");
+  out.atmosphere = time_blend::mix(lo->atmosphere, hi->atmosphere, alpha);
+  out.surface = time_blend::mix(lo->surface, hi->surface, alpha);
   return out;
 }
 
-}  // namespace gwm::ingest
-
-// ---------------------------------------------------------------------------
-// Prepared-case initialisation (prepared_case_init.hpp:16-134,
-// src/ingest/prepared_case_init.cpp)
-// ---------------------------------------------------------------------------
-namespace gwm::ingest {
-
+// ===========================================================================
+// 4. Balance
+// ===========================================================================
 struct PreparedCaseInitConfig {
   int halo = 1;
   int ranks_x = 1;
@@ -530,6 +481,272 @@ struct PreparedCaseInitConfig {
   bool periodic_y = false;
 };
 
+namespace balance {
+
+inline constexpr real kGravity = 9.81f;
+
+// Global (haloless) fields of a grid: cells (nz, ny, nx), x-faces
+// (nz, ny, nx+1), y-faces (nz, ny+1, nx), z-faces (nz+1, ny, nx).
+struct Dims {
+  int nx, ny, nz;
+  std::size_t cell(int i, int j, int k) const {
+    return (static_cast<std::size_t>(k) * ny + j) * nx + i;
+  }
+  std::size_t xface(int fi, int j, int k) const {
+    return (static_cast<std::size_t>(k) * ny + j) * (nx + 1) + fi;
+  }
+  std::size_t yface(int i, int fj, int k) const {
+    return (static_cast<std::size_t>(k) * (ny + 1) + fj) * nx + i;
+  }
+  std::size_t zface(int i, int j, int fk) const { return cell(i, j, fk); }
+  std::size_t col(int i, int j) const { return static_cast<std::size_t>(j) * nx + i; }
+  std::size_t cells() const { return static_cast<std::size_t>(nx) * ny * nz; }
+};
+
+// The balanced dry fields of one analysis (or boundary snapshot).
+struct Fields {
+  std::vector<real> pressure, theta, rho, rho_theta;  // cells
+  std::vector<real> mom_u, mom_v, mom_w;              // faces
+};
+
+// rho*theta whose EOS pressure at rho = 1 is p: the closed-form inverse of
+// the EOS without its floors, then two multiplicative corrections through
+// the floored EOS (prepared_case_init.cpp:38-50).
+inline real rho_theta_for_pressure(real p) {
+  using namespace gwm::core;
+  const real expo = 1.0f - kKappa;
+  real guess = std::pow(std::max(p / kReferencePressure, 1.0e-6f), expo) *
+               (kReferencePressure / kDryGasConstant);
+  for (int pass = 0; pass < 2; ++pass) {
+    const real implied = std::max(dry_pressure_from_rho_theta_m(1.0f, guess), 1.0f);
+    guess *= std::pow(std::max(p / implied, 1.0e-6f), expo);
+  }
+  return guess;
+}
+
+// One column: potential temperature from the analysis (T, p), the analysis
+// pressure at the lowest level, and above it the trapezoid hydrostatic
+// recurrence p_k = p_{k-1} - g (rho_{k-1} + rho_k) dz / 2 solved by four
+// fixed-point passes from an explicit first guess, all floored at 1 Pa
+// (prepared_case_init.cpp:320-384).  In: p_src, t_src, z (nz each).
+struct ColumnBalance {
+  std::vector<real> p, theta, rho, rho_theta;
+  void solve(const real* p_src, const real* t_src, const real* z, int nz) {
+    using namespace gwm::core;
+    p.resize(nz);
+    theta.resize(nz);
+    rho.resize(nz);
+    rho_theta.resize(nz);
+    for (int k = 0; k < nz; ++k) {
+      const real pk = std::max(p_src[k], 1.0f);
+      theta[k] = t_src[k] * std::pow(kReferencePressure / pk, kKappa);
+    }
+    p[0] = std::max(p_src[0], 1.0f);
+    rho[0] = dry_rho_from_pressure_theta(p[0], theta[0]);
+    for (int k = 1; k < nz; ++k) {
+      const real dz = z[k] - z[k - 1];
+      const real below = p[k - 1], rho_below = rho[k - 1];
+      real pk = std::max(1.0f, below - kGravity * rho_below * dz);
+      for (int pass = 0; pass < 4; ++pass) {
+        const real rho_here = dry_rho_from_pressure_theta(pk, theta[k]);
+        pk = std::max(below - kGravity * 0.5f * (rho_below + rho_here) * dz, 1.0f);
+      }
+      p[k] = pk;
+      rho[k] = dry_rho_from_pressure_theta(pk, theta[k]);
+    }
+    for (int k = 0; k < nz; ++k) rho_theta[k] = rho_theta_for_pressure(p[k]);
+  }
+};
+
+// Successive over-relaxation for lap(phi) = rhs on the cell centres of an
+// nx x ny grid with phi = 0 outside (five-point Laplacian), lexicographic
+// order, omega 1.85, at most 2000 sweeps, stopping once no update moves phi
+// by more than 1e-10 (prepared_case_init.cpp:126-170).  phi is held with a
+// ring of zeros so every point reads four neighbours.
+inline std::vector<real> sor_potential(const std::vector<real>& rhs, int nx, int ny, real dx,
+                                       real dy) {
+  const int px = nx + 2;
+  std::vector<real> pad(static_cast<std::size_t>(px) * (ny + 2), 0.0f);
+  std::vector<real> phi(static_cast<std::size_t>(nx) * ny, 0.0f);
+  if (nx <= 0 || ny <= 0) return phi;
+  const real cx = 1.0f / std::max(dx * dx, 1.0e-6f);
+  const real cy = 1.0f / std::max(dy * dy, 1.0e-6f);
+  const real diag = 2.0f * (cx + cy);
+  if (!(diag > 0.0f)) return phi;
+  constexpr real omega = 1.85f;
+  for (int sweep = 0; sweep < 2000; ++sweep) {
+    real largest = 0.0f;
+    for (int j = 0; j < ny; ++j) {
+      real* row = pad.data() + static_cast<std::size_t>(j + 1) * px + 1;
+      const real* rs = rhs.data() + static_cast<std::size_t>(j) * nx;
+      for (int i = 0; i < nx; ++i) {
+        const real target =
+            ((row[i - 1] + row[i + 1]) * cx + (row[i - px] + row[i + px]) * cy - rs[i]) / diag;
+        const real next = row[i] + omega * (target - row[i]);
+        largest = std::max(largest, std::fabs(next - row[i]));
+        row[i] = next;
+      }
+    }
+    if (largest <= 1.0e-10f) break;
+  }
+  for (int j = 0; j < ny; ++j)
+    std::copy_n(pad.data() + static_cast<std::size_t>(j + 1) * px + 1, nx,
+                phi.data() + static_cast<std::size_t>(j) * nx);
+  return phi;
+}
+
+// Layer thicknesses dz = 1 / max(inv_dz_cell, 1e-6) of every cell.
+inline std::vector<real> thicknesses(const domain::GridMetrics& m, const Dims& d) {
+  std::vector<real> dz(d.cells());
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i)
+        dz[d.cell(i, j, k)] = 1.0f / std::max(m.inv_dz_cell(i, j, k), 1.0e-6f);
+  return dz;
+}
+
+// Horizontal mass-flux divergence of every cell.
+inline std::vector<real> horizontal_divergence(const std::vector<real>& mu,
+                                               const std::vector<real>& mv, const Dims& d,
+                                               real dx, real dy) {
+  std::vector<real> out(d.cells());
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i)
+        out[d.cell(i, j, k)] = (mu[d.xface(i + 1, j, k)] - mu[d.xface(i, j, k)]) / dx +
+                               (mv[d.yface(i, j + 1, k)] - mv[d.yface(i, j, k)]) / dy;
+  return out;
+}
+
+// Column means of a cell field weighted by the layer thickness.
+inline std::vector<real> column_means(const std::vector<real>& f, const std::vector<real>& dz,
+                                      const Dims& d) {
+  std::vector<real> acc(static_cast<std::size_t>(d.nx) * d.ny, 0.0f), depth(acc.size(), 0.0f);
+  for (int k = 0; k < d.nz; ++k)
+    for (std::size_t c = 0; c < acc.size(); ++c) {
+      const std::size_t e = static_cast<std::size_t>(k) * acc.size() + c;
+      acc[c] += f[e] * dz[e];
+      depth[c] += dz[e];
+    }
+  for (std::size_t c = 0; c < acc.size(); ++c) acc[c] /= std::max(depth[c], 1.0e-6f);
+  return acc;
+}
+
+// The balanced dry state of an analysis: hydrostatic columns, face momenta
+// rho_f * u_f with the density and wind averaged to each face (wrapped on
+// periodic axes, clamped on open ones), on fully open domains up to three
+// barotropic projections that remove the column-mean horizontal divergence
+// (stopping once it is at most 1e-8), and w from column continuity with the
+// column mean removed, zero at the ground and the lid
+// (prepared_case_init.cpp:172-318, 320-384).
+inline Fields build(const AnalysisStateIR& a, const domain::GridMetrics& m) {
+  const Dims d{a.grid.nx, a.grid.ny, a.grid.nz};
+  const real dx = static_cast<real>(a.grid.dx), dy = static_cast<real>(a.grid.dy);
+  const auto& p_src = a.atmosphere.values.at("air_pressure");
+  const auto& t_src = a.atmosphere.values.at("air_temperature");
+  const auto& u_src = a.atmosphere.values.at("u_wind");
+  const auto& v_src = a.atmosphere.values.at("v_wind");
+  Fields f;
+  f.pressure.resize(d.cells());
+  f.theta.resize(d.cells());
+  f.rho.resize(d.cells());
+  f.rho_theta.resize(d.cells());
+  {
+    std::vector<real> cp(d.nz), ct(d.nz), cz(d.nz);
+    ColumnBalance col;
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        for (int k = 0; k < d.nz; ++k) {
+          cp[k] = p_src[d.cell(i, j, k)];
+          ct[k] = t_src[d.cell(i, j, k)];
+          cz[k] = m.z_center(i, j, k);
+        }
+        col.solve(cp.data(), ct.data(), cz.data(), d.nz);
+        for (int k = 0; k < d.nz; ++k) {
+          const std::size_t c = d.cell(i, j, k);
+          f.pressure[c] = col.p[k];
+          f.theta[c] = col.theta[k];
+          f.rho[c] = col.rho[k];
+          f.rho_theta[c] = col.rho_theta[k];
+        }
+      }
+  }
+  auto side = [](int q, int n, bool periodic) {
+    if (periodic) return ((q % n) + n) % n;
+    return std::min(std::max(q, 0), n - 1);
+  };
+  f.mom_u.assign(static_cast<std::size_t>(d.nx + 1) * d.ny * d.nz, 0.0f);
+  f.mom_v.assign(static_cast<std::size_t>(d.nx) * (d.ny + 1) * d.nz, 0.0f);
+  f.mom_w.assign(static_cast<std::size_t>(d.nx) * d.ny * (d.nz + 1), 0.0f);
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j) {
+      for (int fi = 0; fi <= d.nx; ++fi) {
+        const std::size_t l = d.cell(side(fi - 1, d.nx, m.periodic_x), j, k);
+        const std::size_t r = d.cell(side(fi, d.nx, m.periodic_x), j, k);
+        f.mom_u[d.xface(fi, j, k)] = 0.5f * (f.rho[l] + f.rho[r]) * (0.5f * (u_src[l] + u_src[r]));
+      }
+    }
+  for (int k = 0; k < d.nz; ++k)
+    for (int fj = 0; fj <= d.ny; ++fj)
+      for (int i = 0; i < d.nx; ++i) {
+        const std::size_t s = d.cell(i, side(fj - 1, d.ny, m.periodic_y), k);
+        const std::size_t n = d.cell(i, side(fj, d.ny, m.periodic_y), k);
+        f.mom_v[d.yface(i, fj, k)] = 0.5f * (f.rho[s] + f.rho[n]) * (0.5f * (v_src[s] + v_src[n]));
+      }
+  const std::vector<real> dz = thicknesses(m, d);
+  if (!m.periodic_x && !m.periodic_y) {
+    for (int pass = 0; pass < 3; ++pass) {
+      const auto mean =
+          column_means(horizontal_divergence(f.mom_u, f.mom_v, d, dx, dy), dz, d);
+      real worst = 0.0f;
+      for (real v : mean) worst = std::max(worst, std::fabs(v));
+      if (worst <= 1.0e-8f) break;
+      const auto phi = sor_potential(mean, d.nx, d.ny, dx, dy);
+      // phi is 0 outside the domain, so the outer faces see one side only
+      auto at = [&](int i, int j) { return (i < 0 || i >= d.nx || j < 0 || j >= d.ny) ? 0.0f : phi[d.col(i, j)]; };
+      for (int j = 0; j < d.ny; ++j)
+        for (int fi = 0; fi <= d.nx; ++fi) {
+          const real grad = (at(fi, j) - at(fi - 1, j)) / dx;
+          for (int k = 0; k < d.nz; ++k) f.mom_u[d.xface(fi, j, k)] -= grad;
+        }
+      for (int fj = 0; fj <= d.ny; ++fj)
+        for (int i = 0; i < d.nx; ++i) {
+          const real grad = (at(i, fj) - at(i, fj - 1)) / dy;
+          for (int k = 0; k < d.nz; ++k) f.mom_v[d.yface(i, fj, k)] -= grad;
+        }
+    }
+  }
+  const auto hdiv = horizontal_divergence(f.mom_u, f.mom_v, d, dx, dy);
+  const auto mean = column_means(hdiv, dz, d);
+  for (int j = 0; j < d.ny; ++j)
+    for (int i = 0; i < d.nx; ++i) {
+      real w = 0.0f;
+      f.mom_w[d.zface(i, j, 0)] = 0.0f;
+      for (int k = 0; k < d.nz; ++k) {
+        const std::size_t c = d.cell(i, j, k);
+        w = w - (hdiv[c] - mean[d.col(i, j)]) * dz[c];
+        f.mom_w[d.zface(i, j, k + 1)] = w;
+      }
+      f.mom_w[d.zface(i, j, d.nz)] = 0.0f;
+    }
+  return f;
+}
+
+// An analysis whose atmosphere and surface are a boundary snapshot's.
+inline AnalysisStateIR with_snapshot(const AnalysisStateIR& a, const BoundarySnapshotIR& s) {
+  AnalysisStateIR out = a;
+  out.valid_time_utc = s.valid_time_utc;
+  out.forecast_offset_seconds = s.forecast_offset_seconds;
+  out.atmosphere = s.atmosphere;
+  out.surface = s.surface;
+  return out;
+}
+
+}  // namespace balance
+
+// ===========================================================================
+// 5. States of a layout and the start-up diagnostics
+// ===========================================================================
 struct PreparedCaseBalanceDiagnostics {
   real max_rel_eos = 0.0f;
   real max_rel_hydrostatic = 0.0f;
@@ -543,282 +760,23 @@ struct PreparedCaseBalanceDiagnostics {
 };
 
 namespace detail {
-
-// Global (no halo) index maps of the analysis fields: x fastest.
-struct GlobalIndex {
-  int nx, ny;
-  size_t c(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * ny + j) * nx + i;
-  }
-  size_t xf(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-  }
-  size_t yf(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-  }
-  size_t zf(int i, int j, int k) const {
-    return (static_cast<size_t>(k) * ny + j) * nx + i;
-  }
-  size_t c2(int i, int j) const { return static_cast<size_t>(j) * nx + i; }
+inline void same_shape(const AnalysisStateIR& a, const domain::GridMetrics& m) {
+  if (m.nx != a.grid.nx || m.ny != a.grid.ny || m.nz != a.grid.nz)
+    throw std::runtime_error("grid metrics and analysis differ in shape");
+}
+// Running maximum.
+struct Peak {
+  real v = 0.0f;
+  void see(real x) { v = std::max(v, x); }
 };
-
-inline constexpr real kPreparedGravity = 9.81f;
-
-inline int wrap_or_clamp(int i, int n, bool periodic) {
-  if (!periodic) return std::clamp(i, 0, n - 1);
-  const int m = i % n;
-  return m < 0 ? m + n : m;
-}
-
-inline void require_metrics_compatible(const AnalysisStateIR& a, const domain::GridMetrics& m) {
-  require(m.nx == a.grid.nx && m.ny == a.grid.ny && m.nz == a.grid.nz,
-          "Prepared-case metrics must match analysis grid dimensions");
-}
-
-// rho*theta that reproduces pressure p through the EOS at rho = 1: a power
-// law start and two corrections (prepared_case_init.cpp:38-50).
-inline real rho_theta_m_from_pressure(real p) {
-  using namespace gwm::core;
-  real rt = (kReferencePressure / kDryGasConstant) *
-            std::pow(std::max(p / kReferencePressure, 1.0e-6f), 1.0f - kKappa);
-  for (int it = 0; it < 2; ++it) {
-    const real pe = dry_pressure_from_rho_theta_m(1.0f, rt);
-    const real ratio = std::max(p / std::max(pe, 1.0f), 1.0e-6f);
-    rt *= std::pow(ratio, 1.0f - kKappa);
-  }
-  return rt;
-}
-
-struct BalancedColumns {
-  std::vector<real> pressure, theta, rho_d, rho_theta_m;
-};
-
-// Hydrostatic columns (prepared_case_init.cpp:320-384): theta from the
-// analysis T and p; the bottom level keeps the analysis pressure, each level
-// above integrates dp = -g rho dz with a trapezoid, four fixed-point passes.
-inline BalancedColumns balanced_columns(const AnalysisStateIR& a, const domain::GridMetrics& m) {
-  using namespace gwm::core;
-  require_metrics_compatible(a, m);
-  const int nx = a.grid.nx, ny = a.grid.ny, nz = a.grid.nz;
-  const GlobalIndex G{nx, ny};
-  const auto& ps = a.atmosphere.values.at("air_pressure");
-  const auto& ts = a.atmosphere.values.at("air_temperature");
-  BalancedColumns b;
-  const size_t n = a.grid.cell_count_3d();
-  b.pressure.assign(n, 0.0f);
-  b.theta.assign(n, 0.0f);
-  b.rho_d.assign(n, 0.0f);
-  b.rho_theta_m.assign(n, 0.0f);
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      const size_t c0 = G.c(i, j, 0);
-      const real p0 = std::max(ps[c0], 1.0f);
-      const real th0 = ts[c0] * std::pow(kReferencePressure / p0, kKappa);
-      b.theta[c0] = th0;
-      b.pressure[c0] = p0;
-      b.rho_d[c0] = dry_rho_from_pressure_theta(p0, th0);
-      b.rho_theta_m[c0] = rho_theta_m_from_pressure(p0);
-      for (int k = 1; k < nz; ++k) {
-        const size_t c = G.c(i, j, k), cb = G.c(i, j, k - 1);
-        const real psrc = std::max(ps[c], 1.0f);
-        const real thk = ts[c] * std::pow(kReferencePressure / psrc, kKappa);
-        b.theta[c] = thk;
-        const real dz = m.z_center(i, j, k) - m.z_center(i, j, k - 1);
-        real pn = std::max(1.0f, b.pressure[cb] - kPreparedGravity * b.rho_d[cb] * dz);
-        for (int it = 0; it < 4; ++it) {
-          const real rn = dry_rho_from_pressure_theta(pn, thk);
-          pn = b.pressure[cb] - kPreparedGravity * 0.5f * (b.rho_d[cb] + rn) * dz;
-          pn = std::max(pn, 1.0f);
-        }
-        b.pressure[c] = pn;
-        b.rho_d[c] = dry_rho_from_pressure_theta(pn, thk);
-        b.rho_theta_m[c] = rho_theta_m_from_pressure(pn);
-      }
-    }
-  return b;
-}
-
-struct FaceMomenta {
-  std::vector<real> mom_u, mom_v, mom_w;
-};
-
-// Column mean of the horizontal mass-flux divergence weighted by dz.
-inline std::vector<real> column_mean_divergence(const std::vector<real>& mu,
-                                                const std::vector<real>& mv,
-                                                const AnalysisStateIR& a,
-                                                const domain::GridMetrics& m) {
-  const int nx = a.grid.nx, ny = a.grid.ny, nz = a.grid.nz;
-  const GlobalIndex G{nx, ny};
-  const real dx = static_cast<real>(a.grid.dx), dy = static_cast<real>(a.grid.dy);
-  std::vector<real> out(static_cast<size_t>(nx) * ny, 0.0f);
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      real bias = 0.0f, height = 0.0f;
-      for (int k = 0; k < nz; ++k) {
-        const real hdiv = (mu[G.xf(i + 1, j, k)] - mu[G.xf(i, j, k)]) / dx +
-                          (mv[G.yf(i, j + 1, k)] - mv[G.yf(i, j, k)]) / dy;
-        const real dz = 1.0f / std::max(m.inv_dz_cell(i, j, k), 1.0e-6f);
-        bias += hdiv * dz;
-        height += dz;
-      }
-      out[G.c2(i, j)] = bias / std::max(height, 1.0e-6f);
-    }
-  return out;
-}
-
-// Potential phi with lap(phi) = rhs, phi = 0 outside the domain, by
-// successive over-relaxation (omega 1.85, at most 2000 sweeps, stop when no
-// update exceeds 1e-10) (prepared_case_init.cpp:126-170).
-inline std::vector<real> barotropic_potential(const std::vector<real>& rhs, int nx, int ny,
-                                              real dx, real dy) {
-  std::vector<real> phi(static_cast<size_t>(nx) * ny, 0.0f);
-  if (nx == 0 || ny == 0) return phi;
-  const real ix2 = 1.0f / std::max(dx * dx, 1.0e-6f), iy2 = 1.0f / std::max(dy * dy, 1.0e-6f);
-  const real denom = 2.0f * (ix2 + iy2);
-  if (denom <= 0.0f) return phi;
-  const GlobalIndex G{nx, ny};
-  for (int it = 0; it < 2000; ++it) {
-    real biggest = 0.0f;
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        const real w = i > 0 ? phi[G.c2(i - 1, j)] : 0.0f;
-        const real e = i + 1 < nx ? phi[G.c2(i + 1, j)] : 0.0f;
-        const real s = j > 0 ? phi[G.c2(i, j - 1)] : 0.0f;
-        const real n = j + 1 < ny ? phi[G.c2(i, j + 1)] : 0.0f;
-        const size_t c = G.c2(i, j);
-        const real star = ((w + e) * ix2 + (s + n) * iy2 - rhs[c]) / denom;
-        const real upd = phi[c] + 1.85f * (star - phi[c]);
-        biggest = std::max(biggest, std::fabs(upd - phi[c]));
-        phi[c] = upd;
-      }
-    if (biggest <= 1.0e-10f) break;
-  }
-  return phi;
-}
-
-// Face momenta from the balanced density and the analysis winds
-// (prepared_case_init.cpp:172-318): rho and the wind averaged to each face
-// (neighbours wrapped on periodic axes, clamped on open ones); on fully open
-// domains up to three barotropic projections remove the column-mean
-// horizontal divergence; w from the column continuity equation with the
-// column-mean divergence removed, zero at the ground and the lid.
-inline FaceMomenta projected_face_momenta(const AnalysisStateIR& a, const domain::GridMetrics& m,
-                                          const BalancedColumns& b) {
-  const int nx = a.grid.nx, ny = a.grid.ny, nz = a.grid.nz;
-  const GlobalIndex G{nx, ny};
-  const auto& uw = a.atmosphere.values.at("u_wind");
-  const auto& vw = a.atmosphere.values.at("v_wind");
-  const real dx = static_cast<real>(a.grid.dx), dy = static_cast<real>(a.grid.dy);
-  FaceMomenta f;
-  f.mom_u.assign(static_cast<size_t>(nx + 1) * ny * nz, 0.0f);
-  f.mom_v.assign(static_cast<size_t>(nx) * (ny + 1) * nz, 0.0f);
-  f.mom_w.assign(static_cast<size_t>(nx) * ny * (nz + 1), 0.0f);
-  for (int j = 0; j < ny; ++j)
-    for (int fi = 0; fi <= nx; ++fi) {
-      const int ir = wrap_or_clamp(fi, nx, m.periodic_x), il = wrap_or_clamp(fi - 1, nx, m.periodic_x);
-      for (int k = 0; k < nz; ++k) {
-        const size_t l = G.c(il, j, k), r = G.c(ir, j, k);
-        f.mom_u[G.xf(fi, j, k)] = 0.5f * (b.rho_d[l] + b.rho_d[r]) * (0.5f * (uw[l] + uw[r]));
-      }
-    }
-  for (int fj = 0; fj <= ny; ++fj) {
-    const int jn = wrap_or_clamp(fj, ny, m.periodic_y), js = wrap_or_clamp(fj - 1, ny, m.periodic_y);
-    for (int i = 0; i < nx; ++i)
-      for (int k = 0; k < nz; ++k) {
-        const size_t s = G.c(i, js, k), n = G.c(i, jn, k);
-        f.mom_v[G.yf(i, fj, k)] = 0.5f * (b.rho_d[s] + b.rho_d[n]) * (0.5f * (vw[s] + vw[n]));
-      }
-  }
-  if (!m.periodic_x && !m.periodic_y) {
-    for (int pass = 0; pass < 3; ++pass) {
-      const auto mean = column_mean_divergence(f.mom_u, f.mom_v, a, m);
-      real worst = 0.0f;
-      for (real v : mean) worst = std::max(worst, std::fabs(v));
-      if (worst <= 1.0e-8f) break;
-      const auto phi = barotropic_potential(mean, nx, ny, dx, dy);
-      for (int j = 0; j < ny; ++j)
-        for (int fi = 0; fi <= nx; ++fi) {
-          const real pl = fi > 0 ? phi[G.c2(fi - 1, j)] : 0.0f;
-          const real pr = fi < nx ? phi[G.c2(fi, j)] : 0.0f;
-          const real corr = (pr - pl) / dx;
-          for (int k = 0; k < nz; ++k) f.mom_u[G.xf(fi, j, k)] -= corr;
-        }
-      for (int fj = 0; fj <= ny; ++fj)
-        for (int i = 0; i < nx; ++i) {
-          const real ps = fj > 0 ? phi[G.c2(i, fj - 1)] : 0.0f;
-          const real pn = fj < ny ? phi[G.c2(i, fj)] : 0.0f;
-          const real corr = (pn - ps) / dy;
-          for (int k = 0; k < nz; ++k) f.mom_v[G.yf(i, fj, k)] -= corr;
-        }
-    }
-  }
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      auto hdiv = [&](int k) {
-        return (f.mom_u[G.xf(i + 1, j, k)] - f.mom_u[G.xf(i, j, k)]) / dx +
-               (f.mom_v[G.yf(i, j + 1, k)] - f.mom_v[G.yf(i, j, k)]) / dy;
-      };
-      auto dz_of = [&](int k) { return 1.0f / std::max(m.inv_dz_cell(i, j, k), 1.0e-6f); };
-      f.mom_w[G.zf(i, j, 0)] = 0.0f;
-      real bias = 0.0f, height = 0.0f;
-      for (int k = 0; k < nz; ++k) {
-        const real dz = dz_of(k);
-        bias += hdiv(k) * dz;
-        height += dz;
-      }
-      const real mean = bias / std::max(height, 1.0e-6f);
-      for (int k = 0; k < nz; ++k)
-        f.mom_w[G.zf(i, j, k + 1)] = f.mom_w[G.zf(i, j, k)] - (hdiv(k) - mean) * dz_of(k);
-      f.mom_w[G.zf(i, j, nz)] = 0.0f;
-    }
-  return f;
-}
-
-inline AnalysisStateIR boundary_analysis(const AnalysisStateIR& a, const BoundarySnapshotIR& s) {
-  AnalysisStateIR out = a;
-  out.valid_time_utc = s.valid_time_utc;
-  out.forecast_offset_seconds = s.forecast_offset_seconds;
-  out.atmosphere = s.atmosphere;
-  out.surface = s.surface;
-  return out;
-}
-
-// Balanced fields of the boundary snapshot at the current forecast offset,
-// rebuilt only when the offset (rounded to a second) changes
-// (prepared_case_init.cpp:396-422).
-struct BoundaryProjectionCache {
-  int offset = std::numeric_limits<int>::min();
-  BoundarySnapshotIR snapshot;
-  BalancedColumns columns;
-  FaceMomenta faces;
-
-  void refresh(const AnalysisStateIR& a, const BoundaryCacheIR& c, const domain::GridMetrics& m,
-               real step_start, real sim_time) {
-    const int t = static_cast<int>(std::lround(step_start + sim_time));
-    if (t == offset) return;
-    snapshot = interpolate_boundary_snapshot(c, t);
-    const auto ba = boundary_analysis(a, snapshot);
-    columns = balanced_columns(ba, m);
-    faces = projected_face_momenta(ba, m, columns);
-    offset = t;
-  }
-};
-
-// Host metrics (terrain-following heights and layer thicknesses) without a
-// device grid, for the boundary projections of the updaters.
-inline domain::GridMetrics host_metrics_like(const domain::GridMetrics& m) {
-  domain::GridMetrics h = m;
-  h.device.reset();
-  return h;
-}
-
 }  // namespace detail
 
 inline std::vector<domain::SubdomainDescriptor> make_prepared_case_layout(
     const AnalysisStateIR& a, const PreparedCaseInitConfig& c) {
   validate_runtime_analysis(a);
-  require(c.halo > 0, "Prepared-case halo must be positive");
-  require(c.ranks_x > 0 && c.ranks_y > 0, "Prepared-case rank counts must be positive");
+  if (c.halo <= 0) throw std::runtime_error("prepared-case halo must be at least 1");
+  if (c.ranks_x <= 0 || c.ranks_y <= 0)
+    throw std::runtime_error("prepared-case rank counts must be at least 1");
   return comm::VirtualRankLayout::build(a.grid.nx, a.grid.ny, a.grid.nz, c.halo, c.ranks_x,
                                         c.ranks_y, c.periodic_x, c.periodic_y);
 }
@@ -826,101 +784,105 @@ inline std::vector<domain::SubdomainDescriptor> make_prepared_case_layout(
 inline domain::GridMetrics make_prepared_case_metrics(const AnalysisStateIR& a,
                                                       const PreparedCaseInitConfig& c) {
   validate_runtime_analysis(a);
-  const auto it = a.static_surface.values.find("terrain_height");
-  require(it != a.static_surface.values.end(), "Prepared analysis state requires terrain_height");
-  return domain::GridMetrics::make_hybrid_height(a.grid.nx, a.grid.ny, a.grid.nz, a.grid.dx,
-                                                 a.grid.dy, a.grid.z_top, it->second,
-                                                 c.terrain_taper_eta, c.periodic_x, c.periodic_y,
-                                                 {}, c.halo);
+  return domain::GridMetrics::make_hybrid_height(
+      a.grid.nx, a.grid.ny, a.grid.nz, a.grid.dx, a.grid.dy, a.grid.z_top,
+      a.static_surface.values.at("terrain_height"), c.terrain_taper_eta, c.periodic_x,
+      c.periodic_y, {}, c.halo);
 }
 
+namespace detail {
+// Copies balanced global fields into the ranks of a layout.
+inline void fill_dry(std::vector<dycore::DryState>& states,
+                     const std::vector<domain::SubdomainDescriptor>& layout,
+                     const balance::Fields& f, const balance::Dims& d) {
+  for (std::size_t r = 0; r < layout.size(); ++r) {
+    const auto& L = layout[r];
+    auto& s = states[r];
+    const int nxl = L.nx_local(), nyl = L.ny_local();
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < nyl; ++j) {
+        for (int i = 0; i < nxl; ++i) {
+          const std::size_t c = d.cell(L.i_begin + i, L.j_begin + j, k);
+          s.rho_d(i, j, k) = f.rho[c];
+          s.rho_theta_m(i, j, k) = f.rho_theta[c];
+        }
+        for (int fi = 0; fi <= nxl; ++fi)
+          s.mom_u.storage()(fi, j, k) = f.mom_u[d.xface(L.i_begin + fi, L.j_begin + j, k)];
+      }
+    for (int k = 0; k < L.nz; ++k)
+      for (int fj = 0; fj <= nyl; ++fj)
+        for (int i = 0; i < nxl; ++i)
+          s.mom_v.storage()(i, fj, k) = f.mom_v[d.yface(L.i_begin + i, L.j_begin + fj, k)];
+    for (int fk = 0; fk <= L.nz; ++fk)
+      for (int j = 0; j < nyl; ++j)
+        for (int i = 0; i < nxl; ++i)
+          s.mom_w.storage()(i, j, fk) = f.mom_w[d.zface(L.i_begin + i, L.j_begin + j, fk)];
+  }
+}
+}  // namespace detail
+
+// The balanced dry state of the analysis on each rank of the layout
+// (make_dry_states_from_analysis, prepared_case_init.cpp:644-744).
 inline std::vector<dycore::DryState> make_dry_states_from_analysis(
     const AnalysisStateIR& a, const domain::GridMetrics& m,
     const std::vector<domain::SubdomainDescriptor>& layout,
     const std::string& label_prefix = "prepared_case") {
   validate_runtime_analysis(a);
-  require(!layout.empty(), "Prepared-case layout must not be empty");
-  detail::require_metrics_compatible(a, m);
-  const auto b = detail::balanced_columns(a, m);
-  const auto f = detail::projected_face_momenta(a, m, b);
-  const detail::GlobalIndex G{a.grid.nx, a.grid.ny};
+  if (layout.empty()) throw std::runtime_error("prepared-case layout has no ranks");
+  detail::same_shape(a, m);
+  const balance::Dims d{a.grid.nx, a.grid.ny, a.grid.nz};
+  const balance::Fields f = balance::build(a, m);
   std::vector<dycore::DryState> states;
   states.reserve(layout.size());
-  for (const auto& d : layout) {
-    dycore::DryState s(d.nx_local(), d.ny_local(), d.nz, d.halo,
-                       label_prefix + "_rank_" + std::to_string(d.rank));
-    s.fill_constant(0.0f, 300.0f, 0.0f, 0.0f, 0.0f);
-    for (int k = 0; k < d.nz; ++k)
-      for (int j = 0; j < d.ny_local(); ++j)
-        for (int i = 0; i < d.nx_local(); ++i) {
-          const size_t c = G.c(d.i_begin + i, d.j_begin + j, k);
-          s.rho_d(i, j, k) = b.rho_d[c];
-          s.rho_theta_m(i, j, k) = b.rho_theta_m[c];
-        }
-    for (int k = 0; k < d.nz; ++k)
-      for (int j = 0; j < d.ny_local(); ++j)
-        for (int fi = 0; fi <= d.nx_local(); ++fi)
-          s.mom_u.storage()(fi, j, k) = f.mom_u[G.xf(d.i_begin + fi, d.j_begin + j, k)];
-    for (int k = 0; k < d.nz; ++k)
-      for (int fj = 0; fj <= d.ny_local(); ++fj)
-        for (int i = 0; i < d.nx_local(); ++i)
-          s.mom_v.storage()(i, fj, k) = f.mom_v[G.yf(d.i_begin + i, d.j_begin + fj, k)];
-    for (int fk = 0; fk <= d.nz; ++fk)
-      for (int j = 0; j < d.ny_local(); ++j)
-        for (int i = 0; i < d.nx_local(); ++i)
-          s.mom_w.storage()(i, j, fk) = f.mom_w[G.zf(d.i_begin + i, d.j_begin + j, fk)];
-    states.push_back(std::move(s));
+  for (const auto& L : layout) {
+    states.emplace_back(L.nx_local(), L.ny_local(), L.nz, L.halo,
+                        label_prefix + "_rank_" + std::to_string(L.rank));
+    states.back().fill_constant(0.0f, 300.0f, 0.0f, 0.0f, 0.0f);
   }
+  detail::fill_dry(states, layout, f, d);
   return states;
 }
 
-// Tracer masses rho_d * q from the analysis mixing ratios of the registry's
-// tracers (specific humidity required; other tracers when present)
-// (prepared_case_init.cpp:923-1007).
+// Tracer masses rho_d * q of every registry tracer the analysis carries
+// (specific humidity is required), zero (or the baseline's) otherwise
+// (make_tracers_from_analysis, prepared_case_init.cpp:923-1007).
 inline std::vector<state::TracerState> make_tracers_from_analysis(
     const AnalysisStateIR& a, const std::vector<dycore::DryState>& dry,
     const std::vector<domain::SubdomainDescriptor>& layout, const state::TracerRegistry& registry,
     const std::string& label_prefix = "prepared_case_tracer",
     const std::vector<state::TracerState>* baseline = nullptr) {
   validate_runtime_analysis(a);
-  require(dry.size() == layout.size(), "Dry-state/layout size mismatch in make_tracers_from_analysis");
-  if (baseline)
-    require(baseline->size() == layout.size(),
-            "Baseline tracer/layout size mismatch in make_tracers_from_analysis");
-  const detail::GlobalIndex G{a.grid.nx, a.grid.ny};
+  if (dry.size() != layout.size() || (baseline && baseline->size() != layout.size()))
+    throw std::runtime_error("tracer start-up: dry states, baseline and layout disagree in rank "
+                             "count");
+  const balance::Dims d{a.grid.nx, a.grid.ny, a.grid.nz};
   std::vector<state::TracerState> out;
   out.reserve(layout.size());
-  for (size_t r = 0; r < layout.size(); ++r) {
-    const auto& d = layout[r];
-    state::TracerState t(registry, d.nx_local(), d.ny_local(), d.nz, d.halo,
-                         label_prefix + "_rank_" + std::to_string(d.rank));
-    t.fill_zero();
+  for (std::size_t r = 0; r < layout.size(); ++r) {
+    const auto& L = layout[r];
+    out.emplace_back(registry, L.nx_local(), L.ny_local(), L.nz, L.halo,
+                     label_prefix + "_rank_" + std::to_string(L.rank));
+    out.back().fill_zero();
     if (baseline)
-      for (size_t q = 0; q < t.size(); ++q)
-        t.mass(static_cast<int>(q)).copy_all_from((*baseline)[r].mass(static_cast<int>(q)));
-    out.push_back(std::move(t));
+      for (std::size_t q = 0; q < registry.size(); ++q)
+        out.back().mass(static_cast<int>(q)).copy_all_from((*baseline)[r].mass(static_cast<int>(q)));
   }
-  for (size_t q = 0; q < registry.size(); ++q) {
-    const auto& name = registry.at(static_cast<int>(q)).name;
-    const std::vector<real>* ratio = nullptr;
-    if (name == state::kSpecificHumidityTracerName) {
-      ratio = &a.atmosphere.values.at(state::kSpecificHumidityTracerName);
-      require(ratio->size() == a.grid.cell_count_3d(),
-              "specific_humidity analysis field size mismatch");
-    } else {
-      const auto it = a.atmosphere.values.find(name);
-      if (it == a.atmosphere.values.end()) continue;
-      ratio = &it->second;
-      require(ratio->size() == a.grid.cell_count_3d(),
-              "Tracer analysis field size mismatch for " + name);
-    }
-    for (size_t r = 0; r < layout.size(); ++r) {
-      const auto& d = layout[r];
-      auto& rq = out[r].mass(name);
-      for (int k = 0; k < d.nz; ++k)
-        for (int j = 0; j < d.ny_local(); ++j)
-          for (int i = 0; i < d.nx_local(); ++i)
-            rq(i, j, k) = dry[r].rho_d(i, j, k) * (*ratio)[G.c(d.i_begin + i, d.j_begin + j, k)];
+  if (!a.atmosphere.has(state::kSpecificHumidityTracerName))
+    throw std::runtime_error("tracer start-up needs the analysis specific_humidity");
+  for (std::size_t q = 0; q < registry.size(); ++q) {
+    const std::string& name = registry.at(static_cast<int>(q)).name;
+    const auto it = a.atmosphere.values.find(name);
+    if (it == a.atmosphere.values.end()) continue;
+    const std::vector<real>& ratio = it->second;
+    if (ratio.size() != d.cells())
+      throw std::runtime_error("analysis tracer field '" + name + "' does not match the grid");
+    for (std::size_t r = 0; r < layout.size(); ++r) {
+      const auto& L = layout[r];
+      auto& mass = out[r].mass(static_cast<int>(q));
+      for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < L.ny_local(); ++j)
+          for (int i = 0; i < L.nx_local(); ++i)
+            mass(i, j, k) = dry[r].rho_d(i, j, k) * ratio[d.cell(L.i_begin + i, L.j_begin + j, k)];
     }
   }
   return out;
@@ -942,11 +904,19 @@ inline std::vector<state::TracerState> make_warm_rain_tracers_from_analysis(
                                     label_prefix);
 }
 
-// Start-up balance of a state against the balanced columns of its analysis
-// (prepared_case_init.cpp:746-908): EOS and hydrostatic residuals, the
-// fast-mode vertical perturbations, the discrete mass divergence, w at the
-// ground and lid, the tracer closure, and the source heights against the
-// metric heights when the analysis carries geopotential_height.
+// How well a state holds the balance of its analysis
+// (diagnose_prepared_case_balance, prepared_case_init.cpp:746-908 defines
+// the quantities):
+//  - max_rel_eos: |rho - rho(p_bal, theta)| / max(|rho(p_bal, theta)|, 1);
+//  - max_rel_hydrostatic: |p - p_bal| / max(|p_bal|, 1);
+//  - max_rel_fast_vertical: per interior w face, the largest of the two
+//    cells' |p - p_bal| / max(|p_bal|, 1) and the face's |rho' averaged| /
+//    max(|rho_bal averaged|, 1);
+//  - mass divergence: |div m| and |div m| / max(|d/dx|, |d/dy|, |d/dz|, 1e-12);
+//  - |m_w| at the ground and the lid;
+//  - tracer closure |rho q - rho_d q_analysis| for the tracers the analysis has;
+//  - source heights against the metric level heights, when the analysis
+//    carries geopotential_height.
 inline PreparedCaseBalanceDiagnostics diagnose_prepared_case_balance(
     const AnalysisStateIR& a, const domain::GridMetrics& m,
     const std::vector<dycore::DryState>& dry,
@@ -954,239 +924,281 @@ inline PreparedCaseBalanceDiagnostics diagnose_prepared_case_balance(
     const std::vector<state::TracerState>* tracers = nullptr) {
   using namespace gwm::core;
   validate_runtime_analysis(a);
-  detail::require_metrics_compatible(a, m);
-  require(dry.size() == layout.size(),
-          "Dry-state/layout size mismatch in diagnose_prepared_case_balance");
-  if (tracers)
-    require(tracers->size() == layout.size(),
-            "Tracer/layout size mismatch in diagnose_prepared_case_balance");
-  const auto b = detail::balanced_columns(a, m);
-  const detail::GlobalIndex G{a.grid.nx, a.grid.ny};
-  PreparedCaseBalanceDiagnostics out;
-  const size_t n = a.grid.cell_count_3d();
-  std::vector<real> rho(n, 0.0f), theta(n, 0.0f), p(n, 0.0f);
+  detail::same_shape(a, m);
+  if (dry.size() != layout.size() || (tracers && tracers->size() != layout.size()))
+    throw std::runtime_error("balance diagnostics: states, tracers and layout disagree in rank "
+                             "count");
+  const balance::Dims d{a.grid.nx, a.grid.ny, a.grid.nz};
+  const balance::Fields bal = balance::build(a, m);
+  // the state's rho, theta and p on the global grid
+  std::vector<real> rho(d.cells()), theta(d.cells()), p(d.cells());
+  detail::Peak eos, hydro, fastv, mdiv_abs, mdiv_rel, w_bot, w_top, closure;
   const real dx = static_cast<real>(a.grid.dx), dy = static_cast<real>(a.grid.dy);
-  for (size_t r = 0; r < layout.size(); ++r) {
-    const auto& d = layout[r];
+  for (std::size_t r = 0; r < layout.size(); ++r) {
+    const auto& L = layout[r];
     const auto& s = dry[r];
-    for (int j = 0; j < d.ny_local(); ++j)
-      for (int i = 0; i < d.nx_local(); ++i) {
-        const int gi = d.i_begin + i, gj = d.j_begin + j;
-        for (int k = 0; k < d.nz; ++k) {
-          const size_t c = G.c(gi, gj, k);
-          rho[c] = s.rho_d(i, j, k);
-          theta[c] = dry_theta_from_conserved(s.rho_d(i, j, k), s.rho_theta_m(i, j, k));
-          p[c] = dry_pressure_from_rho_theta_m(s.rho_d(i, j, k), s.rho_theta_m(i, j, k));
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < L.ny_local(); ++j)
+        for (int i = 0; i < L.nx_local(); ++i) {
+          const int gi = L.i_begin + i, gj = L.j_begin + j;
+          const std::size_t c = d.cell(gi, gj, k);
+          const real rr = s.rho_d(i, j, k), rt = s.rho_theta_m(i, j, k);
+          rho[c] = rr;
+          theta[c] = dry_theta_from_conserved(rr, rt);
+          p[c] = dry_pressure_from_rho_theta_m(rr, rt);
           const real ddx = (s.mom_u.storage()(i + 1, j, k) - s.mom_u.storage()(i, j, k)) / dx;
           const real ddy = (s.mom_v.storage()(i, j + 1, k) - s.mom_v.storage()(i, j, k)) / dy;
           const real ddz = (s.mom_w.storage()(i, j, k + 1) - s.mom_w.storage()(i, j, k)) *
                            m.inv_dz_cell(gi, gj, k);
-          const real div = ddx + ddy + ddz;
-          const real scale =
-              std::max({std::fabs(ddx), std::fabs(ddy), std::fabs(ddz), 1.0e-12f});
-          out.max_abs_mass_divergence = std::max(out.max_abs_mass_divergence, std::fabs(div));
-          out.max_rel_mass_divergence =
-              std::max(out.max_rel_mass_divergence, std::fabs(div) / scale);
+          const real div = std::fabs(ddx + ddy + ddz);
+          mdiv_abs.see(div);
+          mdiv_rel.see(div / std::max({std::fabs(ddx), std::fabs(ddy), std::fabs(ddz), 1.0e-12f}));
         }
-        out.max_abs_mom_w_bottom =
-            std::max(out.max_abs_mom_w_bottom, std::fabs(s.mom_w.storage()(i, j, 0)));
-        out.max_abs_mom_w_top =
-            std::max(out.max_abs_mom_w_top, std::fabs(s.mom_w.storage()(i, j, d.nz)));
+    for (int j = 0; j < L.ny_local(); ++j)
+      for (int i = 0; i < L.nx_local(); ++i) {
+        w_bot.see(std::fabs(s.mom_w.storage()(i, j, 0)));
+        w_top.see(std::fabs(s.mom_w.storage()(i, j, L.nz)));
       }
   }
-  const auto zsrc = a.atmosphere.values.find("geopotential_height");
-  for (int j = 0; j < a.grid.ny; ++j)
-    for (int i = 0; i < a.grid.nx; ++i) {
-      for (int k = 0; k < a.grid.nz; ++k) {
-        const size_t c = G.c(i, j, k);
-        const real rexp = dry_rho_from_pressure_theta(b.pressure[c], theta[c]);
-        out.max_rel_eos =
-            std::max(out.max_rel_eos, std::fabs(rho[c] - rexp) / std::max(std::fabs(rexp), 1.0f));
-        out.max_rel_hydrostatic =
-            std::max(out.max_rel_hydrostatic,
-                     std::fabs(p[c] - b.pressure[c]) / std::max(std::fabs(b.pressure[c]), 1.0f));
-        if (zsrc != a.atmosphere.values.end()) {
-          const real mis = std::fabs(zsrc->second[c] - m.z_center(i, j, k));
-          out.max_abs_z_src_minus_metric =
-              out.max_abs_z_src_minus_metric ? std::max(*out.max_abs_z_src_minus_metric, mis) : mis;
-        }
+  auto rel = [](real x, real ref) { return std::fabs(x - ref) / std::max(std::fabs(ref), 1.0f); };
+  for (std::size_t c = 0; c < d.cells(); ++c) {
+    eos.see(rel(rho[c], dry_rho_from_pressure_theta(bal.pressure[c], theta[c])));
+    hydro.see(rel(p[c], bal.pressure[c]));
+  }
+  for (int k = 1; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        const std::size_t lo = d.cell(i, j, k - 1), hi = d.cell(i, j, k);
+        const real rho_dep = 0.5f * ((rho[lo] - bal.rho[lo]) + (rho[hi] - bal.rho[hi]));
+        const real rho_ref = 0.5f * (bal.rho[lo] + bal.rho[hi]);
+        fastv.see(std::max({rel(p[lo], bal.pressure[lo]), rel(p[hi], bal.pressure[hi]),
+                            std::fabs(rho_dep) / std::max(std::fabs(rho_ref), 1.0f), 0.0f}));
       }
-      for (int fk = 1; fk < a.grid.nz; ++fk) {
-        const size_t lo = G.c(i, j, fk - 1), hi = G.c(i, j, fk);
-        const real ppl = p[lo] - b.pressure[lo], ppu = p[hi] - b.pressure[hi];
-        const real rpf = 0.5f * ((rho[lo] - b.rho_d[lo]) + (rho[hi] - b.rho_d[hi]));
-        const real rrf = 0.5f * (b.rho_d[lo] + b.rho_d[hi]);
-        const real scale = std::max({std::fabs(ppl) / std::max(std::fabs(b.pressure[lo]), 1.0f),
-                                     std::fabs(ppu) / std::max(std::fabs(b.pressure[hi]), 1.0f),
-                                     std::fabs(rpf) / std::max(std::fabs(rrf), 1.0f), 0.0f});
-        out.max_rel_fast_vertical = std::max(out.max_rel_fast_vertical, scale);
-      }
-    }
-  if (tracers) {
-    for (size_t r = 0; r < layout.size(); ++r) {
-      const auto& d = layout[r];
+  PreparedCaseBalanceDiagnostics out;
+  const auto zs = a.atmosphere.values.find("geopotential_height");
+  if (zs != a.atmosphere.values.end()) {
+    detail::Peak miss;
+    for (int k = 0; k < d.nz; ++k)
+      for (int j = 0; j < d.ny; ++j)
+        for (int i = 0; i < d.nx; ++i)
+          miss.see(std::fabs(zs->second[d.cell(i, j, k)] - m.z_center(i, j, k)));
+    out.max_abs_z_src_minus_metric = miss.v;
+  }
+  if (tracers)
+    for (std::size_t r = 0; r < layout.size(); ++r) {
+      const auto& L = layout[r];
       const auto& t = (*tracers)[r];
-      for (size_t q = 0; q < t.size(); ++q) {
+      for (std::size_t q = 0; q < t.size(); ++q) {
         const auto it = a.atmosphere.values.find(t.registry().at(static_cast<int>(q)).name);
         if (it == a.atmosphere.values.end()) continue;
-        const auto& rq = t.mass(static_cast<int>(q));
-        for (int j = 0; j < d.ny_local(); ++j)
-          for (int i = 0; i < d.nx_local(); ++i)
-            for (int k = 0; k < d.nz; ++k) {
-              const real expect =
-                  dry[r].rho_d(i, j, k) * it->second[G.c(d.i_begin + i, d.j_begin + j, k)];
-              out.max_abs_tracer_closure =
-                  std::max(out.max_abs_tracer_closure, std::fabs(rq(i, j, k) - expect));
-            }
+        const auto& mass = t.mass(static_cast<int>(q));
+        for (int k = 0; k < L.nz; ++k)
+          for (int j = 0; j < L.ny_local(); ++j)
+            for (int i = 0; i < L.nx_local(); ++i)
+              closure.see(std::fabs(mass(i, j, k) -
+                                    dry[r].rho_d(i, j, k) *
+                                        it->second[d.cell(L.i_begin + i, L.j_begin + j, k)]));
       }
     }
-  }
+  out.max_rel_eos = eos.v;
+  out.max_rel_hydrostatic = hydro.v;
+  out.max_rel_fast_vertical = fastv.v;
+  out.max_abs_mass_divergence = mdiv_abs.v;
+  out.max_rel_mass_divergence = mdiv_rel.v;
+  out.max_abs_mom_w_bottom = w_bot.v;
+  out.max_abs_mom_w_top = w_top.v;
+  out.max_abs_tracer_closure = closure.v;
   return out;
 }
 
-// Lateral-strip overwrite from the balanced boundary snapshot: on each open
-// side touched by a rank, the outermost owned cells (rho, rho*theta), the
-// outermost normal faces and every tangential and vertical face of that
-// strip (prepared_case_init.cpp:424-565).
-inline void apply_prepared_dry_strips(std::vector<dycore::DryState>& states,
-                                      const std::vector<domain::SubdomainDescriptor>& layout,
-                                      int gnx, int gny, const detail::BalancedColumns& b,
-                                      const detail::FaceMomenta& f) {
-  require(states.size() == layout.size(),
-          "State/layout size mismatch in apply_cached_dry_boundary_strips");
-  const detail::GlobalIndex G{gnx, gny};
-  for (size_t r = 0; r < layout.size(); ++r) {
-    auto& s = states[r];
-    const auto& d = layout[r];
-    // Column strip at local i (global gi): cells, the x-face xi, y-faces, z-faces.
-    auto column_strip = [&](int i, int xi) {
-      const int gi = d.i_begin + i;
-      for (int j = 0; j < d.ny_local(); ++j)
-        for (int k = 0; k < d.nz; ++k) {
-          const size_t c = G.c(gi, d.j_begin + j, k);
-          s.rho_d(i, j, k) = b.rho_d[c];
-          s.rho_theta_m(i, j, k) = b.rho_theta_m[c];
-          s.mom_u.storage()(xi, j, k) = f.mom_u[G.xf(d.i_begin + xi, d.j_begin + j, k)];
-        }
-      for (int fj = 0; fj <= d.ny_local(); ++fj)
-        for (int k = 0; k < d.nz; ++k)
-          s.mom_v.storage()(i, fj, k) = f.mom_v[G.yf(gi, d.j_begin + fj, k)];
-      for (int j = 0; j < d.ny_local(); ++j)
-        for (int fk = 0; fk <= d.nz; ++fk)
-          s.mom_w.storage()(i, j, fk) = f.mom_w[G.zf(gi, d.j_begin + j, fk)];
-    };
-    auto row_strip = [&](int j, int yj) {
-      const int gj = d.j_begin + j;
-      for (int i = 0; i < d.nx_local(); ++i)
-        for (int k = 0; k < d.nz; ++k) {
-          const size_t c = G.c(d.i_begin + i, gj, k);
-          s.rho_d(i, j, k) = b.rho_d[c];
-          s.rho_theta_m(i, j, k) = b.rho_theta_m[c];
-          s.mom_v.storage()(i, yj, k) = f.mom_v[G.yf(d.i_begin + i, d.j_begin + yj, k)];
-        }
-      for (int fi = 0; fi <= d.nx_local(); ++fi)
-        for (int k = 0; k < d.nz; ++k)
-          s.mom_u.storage()(fi, j, k) = f.mom_u[G.xf(d.i_begin + fi, gj, k)];
-      for (int i = 0; i < d.nx_local(); ++i)
-        for (int fk = 0; fk <= d.nz; ++fk)
-          s.mom_w.storage()(i, j, fk) = f.mom_w[G.zf(d.i_begin + i, gj, fk)];
-    };
-    if (dycore::touches_west_boundary(d)) column_strip(0, 0);
-    if (dycore::touches_east_boundary(d)) column_strip(d.nx_local() - 1, d.nx_local());
-    if (dycore::touches_south_boundary(d)) row_strip(0, 0);
-    if (dycore::touches_north_boundary(d)) row_strip(d.ny_local() - 1, d.ny_local());
+// ===========================================================================
+// 6. Lateral strips
+// ===========================================================================
+namespace strips {
+
+// Which open sides a rank touches, and membership of its cells and faces in
+// the strips that the prepared boundary overwrites (the reference's column
+// and row strips, prepared_case_init.cpp:424-565, described as masks):
+//  - a cell (and each of its z-faces) is in a strip when it is an outermost
+//    owned cell on a touched side;
+//  - an x-face is overwritten when it is the outer face of a touched west /
+//    east side, or lies on a row of a touched south / north strip;
+//  - a y-face likewise with the axes exchanged.
+struct Sides {
+  bool w, e, s, n;
+  int nx, ny;  // local owned cells
+  explicit Sides(const domain::SubdomainDescriptor& L)
+      : w(dycore::touches_west_boundary(L)), e(dycore::touches_east_boundary(L)),
+        s(dycore::touches_south_boundary(L)), n(dycore::touches_north_boundary(L)),
+        nx(L.nx_local()), ny(L.ny_local()) {}
+  bool any() const { return w || e || s || n; }
+  bool cell(int i, int j) const {
+    return (w && i == 0) || (e && i == nx - 1) || (s && j == 0) || (n && j == ny - 1);
+  }
+  bool xface(int fi, int j) const {
+    return (w && fi == 0) || (e && fi == nx) || (s && j == 0) || (n && j == ny - 1);
+  }
+  bool yface(int i, int fj) const {
+    return (s && fj == 0) || (n && fj == ny) || (w && i == 0) || (e && i == nx - 1);
+  }
+};
+
+// Overwrites the strips of every rank with balanced global fields.
+inline void dry(std::vector<dycore::DryState>& states,
+                const std::vector<domain::SubdomainDescriptor>& layout, const balance::Dims& d,
+                const balance::Fields& f) {
+  if (states.size() != layout.size())
+    throw std::runtime_error("prepared boundary: states and layout disagree in rank count");
+  for (std::size_t r = 0; r < layout.size(); ++r) {
+    const auto& L = layout[r];
+    const Sides S(L);
+    if (!S.any()) continue;
+    auto& st = states[r];
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < S.ny; ++j) {
+        const int gj = L.j_begin + j;
+        for (int i = 0; i < S.nx; ++i)
+          if (S.cell(i, j)) {
+            const std::size_t c = d.cell(L.i_begin + i, gj, k);
+            st.rho_d(i, j, k) = f.rho[c];
+            st.rho_theta_m(i, j, k) = f.rho_theta[c];
+          }
+        for (int fi = 0; fi <= S.nx; ++fi)
+          if (S.xface(fi, j)) st.mom_u.storage()(fi, j, k) = f.mom_u[d.xface(L.i_begin + fi, gj, k)];
+      }
+    for (int k = 0; k < L.nz; ++k)
+      for (int fj = 0; fj <= S.ny; ++fj)
+        for (int i = 0; i < S.nx; ++i)
+          if (S.yface(i, fj))
+            st.mom_v.storage()(i, fj, k) = f.mom_v[d.yface(L.i_begin + i, L.j_begin + fj, k)];
+    for (int j = 0; j < S.ny; ++j)
+      for (int i = 0; i < S.nx; ++i)
+        if (S.cell(i, j))
+          for (int fk = 0; fk <= L.nz; ++fk)
+            st.mom_w.storage()(i, j, fk) = f.mom_w[d.zface(L.i_begin + i, L.j_begin + j, fk)];
   }
 }
 
-// Tracer strips: rho_d (balanced boundary) times the snapshot's mixing ratio,
-// for the tracers the snapshot carries (prepared_case_init.cpp:567-638).
-inline void apply_prepared_tracer_strips(std::vector<state::TracerState>& states,
-                                         const std::vector<domain::SubdomainDescriptor>& layout,
-                                         const BoundarySnapshotIR& snap, int gnx, int gny,
-                                         const std::vector<real>& rho_d) {
-  require(states.size() == layout.size(),
-          "Tracer/layout size mismatch in apply_cached_tracer_boundary_strips");
-  const detail::GlobalIndex G{gnx, gny};
-  for (size_t r = 0; r < layout.size(); ++r) {
-    auto& s = states[r];
-    const auto& d = layout[r];
-    for (size_t q = 0; q < s.size(); ++q) {
-      const auto it = snap.atmosphere.values.find(s.registry().at(static_cast<int>(q)).name);
+// Tracer strips: balanced rho_d times the snapshot's mixing ratio, for the
+// tracers the snapshot carries (prepared_case_init.cpp:567-638).
+inline void tracers(std::vector<state::TracerState>& states,
+                    const std::vector<domain::SubdomainDescriptor>& layout, const balance::Dims& d,
+                    const BoundarySnapshotIR& snap, const std::vector<real>& rho) {
+  if (states.size() != layout.size())
+    throw std::runtime_error("prepared tracer boundary: tracers and layout disagree in rank count");
+  for (std::size_t r = 0; r < layout.size(); ++r) {
+    const auto& L = layout[r];
+    const Sides S(L);
+    if (!S.any()) continue;
+    for (std::size_t q = 0; q < states[r].size(); ++q) {
+      const auto it = snap.atmosphere.values.find(states[r].registry().at(static_cast<int>(q)).name);
       if (it == snap.atmosphere.values.end()) continue;
-      const auto& ratio = it->second;
-      auto& rq = s.mass(static_cast<int>(q));
-      auto cell = [&](int i, int j) {
-        for (int k = 0; k < d.nz; ++k) {
-          const size_t c = G.c(d.i_begin + i, d.j_begin + j, k);
-          rq(i, j, k) = rho_d[c] * ratio[c];
-        }
-      };
-      if (dycore::touches_west_boundary(d))
-        for (int j = 0; j < d.ny_local(); ++j) cell(0, j);
-      if (dycore::touches_east_boundary(d))
-        for (int j = 0; j < d.ny_local(); ++j) cell(d.nx_local() - 1, j);
-      if (dycore::touches_south_boundary(d))
-        for (int i = 0; i < d.nx_local(); ++i) cell(i, 0);
-      if (dycore::touches_north_boundary(d))
-        for (int i = 0; i < d.nx_local(); ++i) cell(i, d.ny_local() - 1);
+      auto& mass = states[r].mass(static_cast<int>(q));
+      for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < S.ny; ++j)
+          for (int i = 0; i < S.nx; ++i)
+            if (S.cell(i, j)) {
+              const std::size_t c = d.cell(L.i_begin + i, L.j_begin + j, k);
+              mass(i, j, k) = rho[c] * it->second[c];
+            }
     }
   }
 }
 
+// Balanced boundary fields for a forecast second, rebuilt only when the
+// second changes (the reference re-projects at the same points,
+// prepared_case_init.cpp:396-422).
+class Source {
+ public:
+  Source(AnalysisStateIR analysis, BoundaryCacheIR cache, const PreparedCaseInitConfig& cfg)
+      : analysis_(std::move(analysis)), cache_(std::move(cache)),
+        metrics_(make_prepared_case_metrics(analysis_, cfg)),
+        dims_{analysis_.grid.nx, analysis_.grid.ny, analysis_.grid.nz} {}
+  bool empty() const { return cache_.empty(); }
+  // Brings the balanced fields to forecast second round(step_start + t).
+  void at(real step_start, real t) {
+    const long second = std::lround(step_start + t);
+    if (have_ && second == second_) return;
+    snapshot_ = interpolate_boundary_snapshot(cache_, static_cast<int>(second));
+    fields_ = balance::build(balance::with_snapshot(analysis_, snapshot_), metrics_);
+    second_ = second;
+    have_ = true;
+  }
+  const balance::Fields& fields() const { return fields_; }
+  const BoundarySnapshotIR& snapshot() const { return snapshot_; }
+  const balance::Dims& dims() const { return dims_; }
+
+ private:
+  AnalysisStateIR analysis_;
+  BoundaryCacheIR cache_;
+  domain::GridMetrics metrics_;
+  balance::Dims dims_;
+  balance::Fields fields_;
+  BoundarySnapshotIR snapshot_;
+  long second_ = 0;
+  bool have_ = false;
+};
+
+}  // namespace strips
+
+// BoundaryUpdater of the prepared driver: overwrites the lateral strips with
+// the balanced state of the boundary snapshot at forecast second
+// round(step start + stage time) (PreparedCaseBoundaryUpdater,
+// prepared_case_init.cpp:1019-1047).
 class PreparedCaseBoundaryUpdater final : public dycore::BoundaryUpdater {
  public:
   PreparedCaseBoundaryUpdater(AnalysisStateIR analysis, BoundaryCacheIR cache,
                               PreparedCaseInitConfig config)
-      : analysis_(std::move(analysis)), cache_(std::move(cache)), config_(config),
-        metrics_(make_prepared_case_metrics(analysis_, config_)) {}
+      : src_(std::move(analysis), std::move(cache), config) {}
   void set_step_start_time(real t) { step_start_ = std::max(t, 0.0f); }
+  real step_start_time() const { return step_start_; }
   void apply(std::vector<dycore::DryState>& states,
              const std::vector<domain::SubdomainDescriptor>& layout, real sim_time) override {
-    if (cache_.empty()) return;
-    proj_.refresh(analysis_, cache_, metrics_, step_start_, sim_time);
-    apply_prepared_dry_strips(states, layout, analysis_.grid.nx, analysis_.grid.ny, proj_.columns,
-                              proj_.faces);
+    if (src_.empty()) return;
+    src_.at(step_start_, sim_time);
+    strips::dry(states, layout, src_.dims(), src_.fields());
+  }
+  // The balanced fields at stage time t of the current step (for callers
+  // that apply the strips themselves, such as the device stepper).
+  const balance::Fields* fields_at(real sim_time) {
+    if (src_.empty()) return nullptr;
+    src_.at(step_start_, sim_time);
+    return &src_.fields();
   }
 
  private:
-  AnalysisStateIR analysis_;
-  BoundaryCacheIR cache_;
-  PreparedCaseInitConfig config_{};
-  domain::GridMetrics metrics_{};
+  strips::Source src_;
   real step_start_ = 0.0f;
-  detail::BoundaryProjectionCache proj_;
 };
 
+// TracerBoundaryUpdater of the prepared driver: vapour (and any tracer the
+// snapshot carries) on the strips, from the balanced density of the same
+// snapshot (PreparedCaseTracerBoundaryUpdater, prepared_case_init.cpp:1049-1083).
 class PreparedCaseTracerBoundaryUpdater final : public dycore::TracerBoundaryUpdater {
  public:
   PreparedCaseTracerBoundaryUpdater(AnalysisStateIR analysis, BoundaryCacheIR cache,
                                     PreparedCaseInitConfig config)
-      : analysis_(std::move(analysis)), cache_(std::move(cache)), config_(config),
-        metrics_(make_prepared_case_metrics(analysis_, config_)) {}
+      : src_(std::move(analysis), std::move(cache), config) {}
   void set_step_start_time(real t) { step_start_ = std::max(t, 0.0f); }
   void apply(std::vector<state::TracerState>& states,
              const std::vector<domain::SubdomainDescriptor>& layout, real sim_time) override {
-    if (cache_.empty() || states.empty()) return;
-    proj_.refresh(analysis_, cache_, metrics_, step_start_, sim_time);
-    apply_prepared_tracer_strips(states, layout, proj_.snapshot, analysis_.grid.nx,
-                                 analysis_.grid.ny, proj_.columns.rho_d);
+    if (src_.empty() || states.empty()) return;
+    src_.at(step_start_, sim_time);
+    strips::tracers(states, layout, src_.dims(), src_.snapshot(), src_.fields().rho);
   }
 
  private:
-  AnalysisStateIR analysis_;
-  BoundaryCacheIR cache_;
-  PreparedCaseInitConfig config_{};
-  domain::GridMetrics metrics_{};
+  strips::Source src_;
   real step_start_ = 0.0f;
-  detail::BoundaryProjectionCache proj_;
 };
 
 }  // namespace gwm::ingest
 
-// ---------------------------------------------------------------------------
-// Runtime summary (include/gwm/core/runtime_summary.hpp:12-55,
-// src/core/runtime_summary.cpp)
-// ---------------------------------------------------------------------------
+// ===========================================================================
+// 7. Runtime summary (include/gwm/core/runtime_summary.hpp:12-55; the
+// quantities of src/core/runtime_summary.cpp:27-178, the JSON keys of
+// runtime_summary.cpp:180-251)
+// ===========================================================================
 namespace gwm::core {
 
 struct TracerSummary {
@@ -1217,67 +1229,62 @@ struct RuntimeStateSummary {
   MoistureBudgetSummary moisture;
 };
 
+// Dry summary, per tracer the FP64 column path sum of rho q dz and the
+// range of q = rho q / max(rho, 1e-12), the water paths, and (with
+// accumulations) the surface precipitation statistics.
 inline RuntimeStateSummary summarize_runtime_state(
     const std::vector<dycore::DryState>& states, const std::vector<state::TracerState>& tracers,
     const std::vector<domain::SubdomainDescriptor>& layout, const domain::GridMetrics& metrics,
     const std::vector<physics::WarmRainSurfaceAccumulation>* accumulations = nullptr) {
-  require(states.size() == tracers.size(), "State/tracer size mismatch in summarize_runtime_state");
-  require(states.size() == layout.size(), "State/layout size mismatch in summarize_runtime_state");
+  if (states.size() != tracers.size() || states.size() != layout.size())
+    throw std::runtime_error("runtime summary: states, tracers and layout disagree in rank count");
   RuntimeStateSummary out;
   out.dry = dycore::summarize_dry_states(states);
   if (states.empty() || tracers.empty()) return out;
   const auto& reg = tracers.front().registry();
-  for (size_t q = 0; q < reg.size(); ++q) {
+  const std::size_t nq = reg.size();
+  std::vector<double> path(nq, 0.0), qmin(nq, std::numeric_limits<double>::max()),
+      qmax(nq, -std::numeric_limits<double>::max());
+  bool cells = false;
+  for (std::size_t r = 0; r < states.size(); ++r) {
+    const auto& L = layout[r];
+    const auto& tr = tracers[r];
+    if (tr.size() != nq) throw std::runtime_error("runtime summary: tracer sets differ by rank");
+    for (std::size_t q = 0; q < nq; ++q)
+      if (tr.registry().at(static_cast<int>(q)).name != reg.at(static_cast<int>(q)).name)
+        throw std::runtime_error("runtime summary: tracer order differs by rank");
+    const auto& rho = states[r].rho_d;
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < L.ny_local(); ++j)
+        for (int i = 0; i < L.nx_local(); ++i) {
+          cells = true;
+          const double dz =
+              1.0 / static_cast<double>(metrics.inv_dz_cell(L.i_begin + i, L.j_begin + j, k));
+          const double rr = std::max(static_cast<double>(rho(i, j, k)), 1.0e-12);
+          for (std::size_t q = 0; q < nq; ++q) {
+            const double m = static_cast<double>(tr.mass(static_cast<int>(q))(i, j, k));
+            path[q] += m * dz;
+            qmin[q] = std::min(qmin[q], m / rr);
+            qmax[q] = std::max(qmax[q], m / rr);
+          }
+        }
+  }
+  for (std::size_t q = 0; q < nq; ++q) {
     const auto& spec = reg.at(static_cast<int>(q));
     TracerSummary t;
     t.name = spec.name;
     t.units = spec.units;
     t.positive = spec.positive;
-    t.mixing_ratio = {std::numeric_limits<double>::max(), -std::numeric_limits<double>::max()};
+    t.column_integrated_sum_kg_m2 = path[q];
+    t.mixing_ratio = cells ? dycore::ScalarRange{qmin[q], qmax[q]} : dycore::ScalarRange{0.0, 0.0};
     out.tracers.push_back(std::move(t));
   }
-  bool any = false;
-  for (size_t r = 0; r < states.size(); ++r) {
-    require(tracers[r].size() == out.tracers.size(),
-            "Tracer registry mismatch in summarize_runtime_state");
-    const auto& d = layout[r];
-    for (size_t q = 0; q < out.tracers.size(); ++q) {
-      require(reg.at(static_cast<int>(q)).name == tracers[r].registry().at(static_cast<int>(q)).name,
-              "Tracer ordering mismatch in summarize_runtime_state");
-      const auto& rq = tracers[r].mass(static_cast<int>(q));
-      for (int k = 0; k < d.nz; ++k)
-        for (int j = 0; j < d.ny_local(); ++j)
-          for (int i = 0; i < d.nx_local(); ++i)
-            out.tracers[q].column_integrated_sum_kg_m2 +=
-                static_cast<double>(rq(i, j, k)) *
-                (1.0 / static_cast<double>(metrics.inv_dz_cell(d.i_begin + i, d.j_begin + j, k)));
-    }
-    const auto& rho = states[r].rho_d;
-    for (int k = 0; k < rho.nz(); ++k)
-      for (int j = 0; j < rho.ny(); ++j)
-        for (int i = 0; i < rho.nx(); ++i) {
-          any = true;
-          const double rr = std::max(static_cast<double>(rho(i, j, k)), 1.0e-12);
-          for (size_t q = 0; q < out.tracers.size(); ++q) {
-            const double v =
-                static_cast<double>(tracers[r].mass(static_cast<int>(q))(i, j, k)) / rr;
-            auto& mr = out.tracers[q].mixing_ratio;
-            mr.min = std::min(mr.min, v);
-            mr.max = std::max(mr.max, v);
-          }
-        }
-  }
-  if (!any)
-    for (auto& t : out.tracers) t.mixing_ratio = {0.0, 0.0};
-  auto path_of = [&out](const char* name) {
-    for (const auto& t : out.tracers)
-      if (t.name == name) return t.column_integrated_sum_kg_m2;
-    return 0.0;
-  };
   auto& mo = out.moisture;
-  mo.vapor_water_path_sum_kg_m2 = path_of(state::kSpecificHumidityTracerName);
-  mo.cloud_water_path_sum_kg_m2 = path_of(state::kCloudWaterTracerName);
-  mo.rain_water_path_sum_kg_m2 = path_of(state::kRainWaterTracerName);
+  for (const auto& t : out.tracers) {
+    if (t.name == state::kSpecificHumidityTracerName) mo.vapor_water_path_sum_kg_m2 = t.column_integrated_sum_kg_m2;
+    if (t.name == state::kCloudWaterTracerName) mo.cloud_water_path_sum_kg_m2 = t.column_integrated_sum_kg_m2;
+    if (t.name == state::kRainWaterTracerName) mo.rain_water_path_sum_kg_m2 = t.column_integrated_sum_kg_m2;
+  }
   mo.condensed_water_path_sum_kg_m2 = mo.cloud_water_path_sum_kg_m2 + mo.rain_water_path_sum_kg_m2;
   mo.total_water_path_sum_kg_m2 = mo.vapor_water_path_sum_kg_m2 + mo.condensed_water_path_sum_kg_m2;
   if (accumulations) {
@@ -1285,77 +1292,120 @@ inline RuntimeStateSummary summarize_runtime_state(
     mo.accumulated_surface_precipitation_sum_mm = w.accumulated_surface_precipitation_sum_mm;
     mo.mean_surface_precipitation_mm = w.mean_surface_precipitation_mm;
     mo.max_surface_precipitation_mm = w.max_surface_precipitation_mm;
-    double wet = 0.0;
-    for (const auto& acc : *accumulations)
+    double wet_sum = 0.0;
+    for (const auto& acc : *accumulations) {
+      mo.total_surface_cell_count += acc.nx * acc.ny;
       for (int j = 0; j < acc.ny; ++j)
         for (int i = 0; i < acc.nx; ++i) {
-          ++mo.total_surface_cell_count;
           const double v = acc.liquid_precipitation_kg_m2(i, j, 0);
           if (v > 0.0) {
-            ++mo.precipitating_surface_cell_count;
-            wet += v;
+            mo.precipitating_surface_cell_count += 1;
+            wet_sum += v;
           }
         }
+    }
     if (mo.total_surface_cell_count > 0)
       mo.precipitating_surface_fraction = static_cast<double>(mo.precipitating_surface_cell_count) /
-                                          static_cast<double>(mo.total_surface_cell_count);
+                                          mo.total_surface_cell_count;
     if (mo.precipitating_surface_cell_count > 0)
-      mo.mean_precipitating_surface_precipitation_mm =
-          wet / static_cast<double>(mo.precipitating_surface_cell_count);
+      mo.mean_precipitating_surface_precipitation_mm = wet_sum / mo.precipitating_surface_cell_count;
   }
   return out;
 }
+
+namespace detail {
+// A small JSON object writer with a fixed indent.
+class JsonObject {
+ public:
+  JsonObject(std::ostringstream& o, std::string indent) : o_(o), in_(std::move(indent)) {
+    o_ << "{\n";
+  }
+  template <class V>
+  JsonObject& num(const char* key, V v) {
+    sep();
+    o_ << in_ << "\"" << key << "\": " << v;
+    return *this;
+  }
+  JsonObject& flag(const char* key, bool v) {
+    sep();
+    o_ << in_ << "\"" << key << "\": " << (v ? "true" : "false");
+    return *this;
+  }
+  JsonObject& text(const char* key, const std::string& v) {
+    sep();
+    o_ << in_ << "\"" << key << "\": \"" << v << "\"";
+    return *this;
+  }
+  JsonObject& range(const char* key, const dycore::ScalarRange& r) {
+    sep();
+    o_ << in_ << "\"" << key << "\": {\"min\": " << r.min << ", \"max\": " << r.max << "}";
+    return *this;
+  }
+  // Opens a nested object under `key`; the caller closes it with close().
+  JsonObject open(const char* key) {
+    sep();
+    o_ << in_ << "\"" << key << "\": ";
+    return JsonObject(o_, in_ + "  ");
+  }
+  void close(const std::string& outer_indent) {
+    o_ << "\n" << outer_indent << "}";
+  }
+
+ private:
+  void sep() {
+    if (!first_) o_ << ",\n";
+    first_ = false;
+  }
+  std::ostringstream& o_;
+  std::string in_;
+  bool first_ = true;
+};
+}  // namespace detail
 
 inline std::string runtime_state_summary_to_json(const RuntimeStateSummary& s,
                                                  const std::string& in = "  ") {
   std::ostringstream o;
   o << std::setprecision(17);
+  const std::string outer = in.size() >= 2 ? in.substr(0, in.size() - 2) : std::string();
+  detail::JsonObject top(o, in);
   const auto& d = s.dry;
-  auto range = [&](const char* key, const dycore::ScalarRange& r) {
-    o << in << "\"" << key << "\": {\"min\": " << r.min << ", \"max\": " << r.max << "},\n";
-  };
-  o << "{\n";
-  o << in << "\"total_dry_mass\": " << d.total_dry_mass << ",\n";
-  o << in << "\"total_rho_theta_m\": " << d.total_rho_theta_m << ",\n";
-  o << in << "\"total_horizontal_momentum_x\": " << d.total_horizontal_momentum_x << ",\n";
-  o << in << "\"total_horizontal_momentum_y\": " << d.total_horizontal_momentum_y << ",\n";
-  o << in << "\"total_vertical_momentum\": " << d.total_vertical_momentum << ",\n";
-  range("rho_d", d.rho_d);
-  range("theta_m", d.theta_m);
-  range("u_face", d.u_face);
-  range("v_face", d.v_face);
-  range("w_face", d.w_face);
+  top.num("total_dry_mass", d.total_dry_mass)
+      .num("total_rho_theta_m", d.total_rho_theta_m)
+      .num("total_horizontal_momentum_x", d.total_horizontal_momentum_x)
+      .num("total_horizontal_momentum_y", d.total_horizontal_momentum_y)
+      .num("total_vertical_momentum", d.total_vertical_momentum)
+      .range("rho_d", d.rho_d)
+      .range("theta_m", d.theta_m)
+      .range("u_face", d.u_face)
+      .range("v_face", d.v_face)
+      .range("w_face", d.w_face);
   const auto& m = s.moisture;
-  o << in << "\"moisture\": {\n";
-  o << in << "  \"vapor_water_path_sum_kg_m2\": " << m.vapor_water_path_sum_kg_m2 << ",\n";
-  o << in << "  \"cloud_water_path_sum_kg_m2\": " << m.cloud_water_path_sum_kg_m2 << ",\n";
-  o << in << "  \"rain_water_path_sum_kg_m2\": " << m.rain_water_path_sum_kg_m2 << ",\n";
-  o << in << "  \"condensed_water_path_sum_kg_m2\": " << m.condensed_water_path_sum_kg_m2 << ",\n";
-  o << in << "  \"total_water_path_sum_kg_m2\": " << m.total_water_path_sum_kg_m2 << ",\n";
-  o << in << "  \"accumulated_surface_precipitation_sum_mm\": "
-    << m.accumulated_surface_precipitation_sum_mm << ",\n";
-  o << in << "  \"mean_surface_precipitation_mm\": " << m.mean_surface_precipitation_mm << ",\n";
-  o << in << "  \"max_surface_precipitation_mm\": " << m.max_surface_precipitation_mm << ",\n";
-  o << in << "  \"total_surface_cell_count\": " << m.total_surface_cell_count << ",\n";
-  o << in << "  \"precipitating_surface_cell_count\": " << m.precipitating_surface_cell_count
-    << ",\n";
-  o << in << "  \"precipitating_surface_fraction\": " << m.precipitating_surface_fraction << ",\n";
-  o << in << "  \"mean_precipitating_surface_precipitation_mm\": "
-    << m.mean_precipitating_surface_precipitation_mm << "\n";
-  o << in << "},\n";
-  o << in << "\"tracers\": {\n";
-  for (size_t q = 0; q < s.tracers.size(); ++q) {
-    const auto& t = s.tracers[q];
-    o << in << "  \"" << t.name << "\": {\n";
-    o << in << "    \"units\": \"" << t.units << "\",\n";
-    o << in << "    \"positive\": " << (t.positive ? "true" : "false") << ",\n";
-    o << in << "    \"column_integrated_sum_kg_m2\": " << t.column_integrated_sum_kg_m2 << ",\n";
-    o << in << "    \"mixing_ratio\": {\"min\": " << t.mixing_ratio.min
-      << ", \"max\": " << t.mixing_ratio.max << "}\n";
-    o << in << "  }" << (q + 1 != s.tracers.size() ? "," : "") << "\n";
+  auto mo = top.open("moisture");
+  mo.num("vapor_water_path_sum_kg_m2", m.vapor_water_path_sum_kg_m2)
+      .num("cloud_water_path_sum_kg_m2", m.cloud_water_path_sum_kg_m2)
+      .num("rain_water_path_sum_kg_m2", m.rain_water_path_sum_kg_m2)
+      .num("condensed_water_path_sum_kg_m2", m.condensed_water_path_sum_kg_m2)
+      .num("total_water_path_sum_kg_m2", m.total_water_path_sum_kg_m2)
+      .num("accumulated_surface_precipitation_sum_mm", m.accumulated_surface_precipitation_sum_mm)
+      .num("mean_surface_precipitation_mm", m.mean_surface_precipitation_mm)
+      .num("max_surface_precipitation_mm", m.max_surface_precipitation_mm)
+      .num("total_surface_cell_count", m.total_surface_cell_count)
+      .num("precipitating_surface_cell_count", m.precipitating_surface_cell_count)
+      .num("precipitating_surface_fraction", m.precipitating_surface_fraction)
+      .num("mean_precipitating_surface_precipitation_mm",
+           m.mean_precipitating_surface_precipitation_mm);
+  mo.close(in);
+  auto tr = top.open("tracers");
+  for (const auto& t : s.tracers) {
+    auto one = tr.open(t.name.c_str());
+    one.text("units", t.units)
+        .flag("positive", t.positive)
+        .num("column_integrated_sum_kg_m2", t.column_integrated_sum_kg_m2)
+        .range("mixing_ratio", t.mixing_ratio);
+    one.close(in + "  ");
   }
-  o << in << "}\n";
-  o << "}";
+  tr.close(in);
+  top.close(outer);
   return o.str();
 }
 

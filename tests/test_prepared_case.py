@@ -1,9 +1,9 @@
 """The prepared-case path: gwm_prepared_case_driver and its start-up.
 
 CPU:
-- the driver's flag and input errors (raised before any device work) follow
-  the reference's messages (gwm_prepared_case_driver.cpp:169-250,
-  runtime_case.cpp:151-274);
+- the driver's flag errors follow the reference's flag contract
+  (gwm_prepared_case_driver.cpp:169-250) and its input errors name the
+  document, key and problem (the checks of runtime_case.cpp:151-274);
 - the prepared-case oracle (oracle/prepared_case_oracle.py) is pinned by
   closed forms: the EOS at rho = 1 returns the pressure rho*theta was built
   from; every level of a balanced column satisfies the trapezoid
@@ -76,13 +76,15 @@ def _mutate(path, fn):
 
 @pytest.mark.parametrize("mutation,msg", [
     ("truncate", "JSON parse error"),
-    ("schema", "Unsupported analysis-state schema"),
-    ("source", "Unsupported source string: NOAA"),
-    ("missing", "Analysis state is missing required canonical fields"),
-    ("size", "Analysis atmosphere field has wrong size: u_wind"),
-    ("gfs", "Analysis source is unsupported by runtime adapter"),
-    ("order", "Boundary cache offsets must be strictly monotonic"),
-    ("mismatch", "Analysis and boundary cache grid mismatch"),
+    ("schema", "declares schema 'gwm-next-analysis-state/v0', expected "
+               "'gwm-next-analysis-state/v1'"),
+    ("source", ".source names unknown source 'NOAA'"),
+    ("missing", "analysis atmosphere lacks the canonical field 'v_wind' (HRRR VGRD)"),
+    ("size", "analysis atmosphere field 'u_wind' holds 119 values; the grid needs 120"),
+    ("gfs", "analysis source GFS has no runtime field bindings"),
+    ("order", "boundary snapshots must have strictly increasing forecast offsets "
+              "(snapshot 1 at 0 s follows 3600 s)"),
+    ("mismatch", "analysis grid 6x5x4 and boundary-cache grid 5x5x4 differ"),
 ])
 def test_driver_rejects_bad_inputs(tmp_path, mutation, msg):
     snaps = ((0, 1.0, 1.5, 0.001), (3600, 2.0, 3.0, 0.002))
