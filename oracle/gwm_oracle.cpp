@@ -337,6 +337,28 @@ struct Metrics {
   }
 };
 
+// K11 column hydrostatic reference of one column (dry_fast_modes.cu:108-158):
+// p_ref(0) = p_bottom, rho_ref from (p_ref, theta) at every level, and above
+// the bottom the trapezoid recurrence p_k = p_{k-1} - g (rho_{k-1} + rho_k)
+// dz / 2 solved by four fixed-point passes from the explicit guess, floored
+// at 1 Pa.  th[k], zc[k]: theta and centre height of level k.
+template <class R>
+void column_reference(int nz, const R* th, const R* zc, R g, R p_bottom, R* pref, R* rref) {
+  pref[0] = p_bottom;
+  rref[0] = rho_from_p_theta(pref[0], th[0]);
+  for (int k = 1; k < nz; ++k) {
+    const R dz = zc[k] - zc[k - 1];
+    R pn = std::max(R(1), pref[k - 1] - g * rref[k - 1] * dz);
+    for (int it = 0; it < 4; ++it) {
+      const R rn = rho_from_p_theta(pn, th[k]);
+      pn = pref[k - 1] - g * R(0.5) * (rref[k - 1] + rn) * dz;
+      pn = std::max(pn, R(1));
+    }
+    pref[k] = pn;
+    rref[k] = rho_from_p_theta(pn, th[k]);
+  }
+}
+
 double taper_weight(double eta, double taper) {  // grid_metrics.hpp:126-144
   if (eta <= 0.0) return 1.0;
   if (eta >= 1.0) return 0.0;
@@ -1006,25 +1028,22 @@ struct World : WorldBase {
         if (M.terrain_active) {
           // K11 column hydrostatic reference (dry_fast_modes.cu:108-158)
 #pragma omp parallel for schedule(static)
-          for (int j = 0; j < ny; ++j)
+          for (int j = 0; j < ny; ++j) {
+            std::vector<R> thc(nz), zcc(nz), pc(nz), rc(nz);
             for (int i = 0; i < nx; ++i) {
               const int gi = wi(d.i0 + i), gj = wj(d.j0 + j);
-              const R th0 = theta_of(rh[n](i, j, 0), rtf[n](i, j, 0));
-              pref[n](i, j, 0) = pf[n](i, j, 0);
-              rref[n](i, j, 0) = rho_from_p_theta(pref[n](i, j, 0), th0);
-              for (int k = 1; k < nz; ++k) {
-                const R thk = theta_of(rh[n](i, j, k), rtf[n](i, j, k));
-                const R dz = M.zc_m[M.mi(gi, gj, k)] - M.zc_m[M.mi(gi, gj, k - 1)];
-                R pn = std::max(R(1), pref[n](i, j, k - 1) - g * rref[n](i, j, k - 1) * dz);
-                for (int it = 0; it < 4; ++it) {
-                  const R rn = rho_from_p_theta(pn, thk);
-                  pn = pref[n](i, j, k - 1) - g * R(0.5) * (rref[n](i, j, k - 1) + rn) * dz;
-                  pn = std::max(pn, R(1));
-                }
-                pref[n](i, j, k) = pn;
-                rref[n](i, j, k) = rho_from_p_theta(pn, thk);
+              for (int k = 0; k < nz; ++k) {
+                thc[k] = theta_of(rh[n](i, j, k), rtf[n](i, j, k));
+                zcc[k] = M.zc_m[M.mi(gi, gj, k)];
+              }
+              column_reference<R>(nz, thc.data(), zcc.data(), g, pf[n](i, j, 0), pc.data(),
+                                  rc.data());
+              for (int k = 0; k < nz; ++k) {
+                pref[n](i, j, k) = pc[k];
+                rref[n](i, j, k) = rc[k];
               }
             }
+          }
         } else {
           // K8
           for (int k = 0; k < nz; ++k)
@@ -1757,6 +1776,27 @@ int orc_set_tracer_strip_mask(void* w, unsigned mask) {
 }
 int orc_apply_boundary(void* w, int mode, double width) {
   return guard([&] { W(w)->boundary_now(mode, width); });
+}
+
+// K11 of one column: theta and centre heights in, p_ref and rho_ref out
+// (working precision `precision`, FP64 arrays at the boundary).
+int orc_column_reference(int precision, int nz, const double* th, const double* zc, double g,
+                         double p_bottom, double* pref, double* rref) {
+  return guard([&] {
+    require(nz >= 1 && th && zc && pref && rref, "bad column arguments");
+    auto run = [&](auto r) {
+      using R = decltype(r);
+      std::vector<R> t(th, th + nz), z(zc, zc + nz), p(nz), q(nz);
+      column_reference<R>(nz, t.data(), z.data(), static_cast<R>(g), static_cast<R>(p_bottom),
+                          p.data(), q.data());
+      for (int k = 0; k < nz; ++k) {
+        pref[k] = static_cast<double>(p[k]);
+        rref[k] = static_cast<double>(q[k]);
+      }
+    };
+    if (precision == 4) run(float{});
+    else run(double{});
+  });
 }
 
 double orc_thermo(int precision, int fn, double a, double b, double c) {

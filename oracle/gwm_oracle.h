@@ -169,6 +169,11 @@ int orc_init_bubble(void* world, const double* p12, int sign);
 int orc_init_mountain_wave(void* world, double rho_surface, double theta_ref,
                            double u, double v, double w);
 
+/* K11 column hydrostatic reference of one column (dry_fast_modes.cu:108-158)
+ * in the given precision: th, zc (nz) in; pref, rref (nz) out. */
+int orc_column_reference(int precision, int nz, const double* th, const double* zc,
+                         double g, double p_bottom, double* pref, double* rref);
+
 /* Timing helper for the CPU baseline: runs n steps of the moist or dry step
  * and returns wall seconds. */
 double orc_time_steps(void* world, const orc_step_cfg* cfg,

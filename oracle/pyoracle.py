@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# GWM_ORACLE_LIB points the checker at another build of this source (the
+# mutation tests load deliberately broken copies of the oracle that way).
+LIB_PATH = os.environ.get("GWM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 RHO, RHO_THETA, MOM_U, MOM_V, MOM_W, PRESSURE = 0, 1, 2, 3, 4, 5
 TEND_RHO, TEND_RHO_THETA, TEND_U, TEND_V, TEND_W = 6, 7, 8, 9, 10
@@ -107,6 +109,7 @@ def lib():
         L.orc_init_hydrostatic_rest.argtypes = [vp, dp, dp]
         L.orc_init_bubble.argtypes = [vp, vp, ip]
         L.orc_init_mountain_wave.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.orc_column_reference.argtypes = [ip, ip, vp, vp, dp, dp, vp, vp]
         L.orc_thermo.restype = dp
         L.orc_thermo.argtypes = [ip, ip, dp, dp, dp]
         L.orc_apply_boundary.argtypes = [vp, ip, dp]
@@ -288,3 +291,14 @@ class World:
 
     def init_mountain_wave(self, rho_surface=1.18, theta_ref=300.0, u=15.0, v=0.0, w=0.0):
         _check(lib().orc_init_mountain_wave(self.h, rho_surface, theta_ref, u, v, w))
+
+
+def column_reference(theta, zc, p_bottom, gravity=9.81, precision=8):
+    """K11 column hydrostatic reference of one column: (p_ref, rho_ref)."""
+    th = np.ascontiguousarray(theta, np.float64)
+    z = np.ascontiguousarray(zc, np.float64)
+    p = np.zeros_like(th)
+    r = np.zeros_like(th)
+    _check(lib().orc_column_reference(precision, th.size, _ptr(th), _ptr(z), float(gravity),
+                                      float(p_bottom), _ptr(p), _ptr(r)))
+    return p, r
