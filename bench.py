@@ -4,21 +4,39 @@ Workload (the 512x512x64 warm-rain convection case): dx = dy = 1 km, 64
 levels stretched from 100 m to 500 m up to 16 km, doubly periodic, flat;
 theta = 300 K + 43 K (z/12 km)^1.25 below 12 km and isothermal above, RH =
 1 - 0.75 (z/12 km)^1.25 capped at qv = 14 g/kg, seeded +/-0.5 K theta noise
-below 1 km; dt = 5 s, 6 acoustic substeps, warm rain with 12 m/s fallout.
-A step is one full moist step: dry SSPRK3 (3 slow stages, 18 fast
-substeps), tracer SSPRK3 for qv/qc/qr, warm rain + fallout.  Inputs are
-resident in HBM before timing starts.
+below 1 km; 6 acoustic substeps, warm rain with 12 m/s fallout.  A step is
+one full moist step: dry SSPRK3 (3 slow stages, 18 fast substeps), tracer
+SSPRK3 for qv/qc/qr, warm rain + fallout.  Inputs are resident in HBM
+before timing starts.
 
-Prints one JSON line.  `value` is cell-steps/s of the FP32 step on one GPU.
+The configuration's dt of 5 s overflows the reference's explicit acoustic
+scheme within the first step on the 100 m levels, so the timed runs use dt
+0.25 s, where the state stays finite; per-step work does not depend on dt,
+and the 5 s run is reported beside it, with no throughput credited to a run
+whose state holds a NaN or infinity (counted over every prognostic field).
+
+`value` is cell-steps/s of the FP32 perturbation form (rho and rho*theta
+stored as FP32 departures from FP64 level bases), the FP32 path that stays
+within 1e-4 relative L2 of FP64; its error against the FP64 step (which the
+GPU tests hold to the FP64 oracle: 1e-11 on operators and short runs, 1e-8
+over the configurations' ill-conditioned trajectories) is measured here at
+full size.
+Prints one JSON line.
 """
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "gpu-wm-next_amd"))
+
+FIELD_NAMES = {0: "rho", 1: "rho_theta", 2: "u", 3: "v", 4: "w", 16: "qv", 17: "qc", 18: "qr"}
+# physical scale of each field: relative L2 floors at 1e-3 of it
+FIELD_SCALE = {0: 1.0, 1: 300.0, 2: 1.0, 3: 1.0, 4: 1.0, 16: 1e-3, 17: 1e-3, 18: 1e-3}
 
 
 def measured_traffic(nx, ny, nz):
@@ -27,8 +45,7 @@ def measured_traffic(nx, ny, nz):
     size); only for the 512x512x64 workload it was measured on."""
     if (nx, ny, nz) != (512, 512, 64):
         return None
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "fast_column_traffic_512x512x64.md")
+    path = os.path.join(REPO, "profiles", "fast_column_traffic_512x512x64.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -36,24 +53,6 @@ def measured_traffic(nx, ny, nz):
     except OSError:
         return None
     return None
-
-
-def copy_ceiling(G, nx, ny, nz, reps=10):
-    """Device-to-device copy rate of a moist state (read + write bytes over
-    wall time, 8 fields and the surface bucket per copy): the box's
-    STREAM-copy ceiling beside the 8 TB/s peak."""
-    ctx, grid, a, _ = make_case(G, G.F32, nx, ny, nz)
-    b = G.State(grid, 3)
-    b.copy_from(a)
-    import numpy as np
-    elems = sum(int(np.prod(a.storage_shape(f))) for f in range(5))
-    elems += 3 * int(np.prod(a.storage_shape(0))) + nx * ny
-    nbytes = elems * 4 * 2
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        b.copy_from(a)
-    secs = time.perf_counter() - t0
-    return nbytes * reps / secs / 1e9
 
 
 def log(msg):
@@ -71,9 +70,12 @@ def parse():
     p.add_argument("--nz", type=int, default=64)
     p.add_argument("--fp64-steps", type=int, default=5,
                    help="timed steps of the FP64 plain path (0 to skip)")
+    p.add_argument("--error-steps", type=int, default=10,
+                   help="steps of the error leg against the FP64 step (0 to skip)")
     p.add_argument("--cpu-baseline", type=int, default=1,
                    help="time the FP64 CPU oracle on a bounded sample (0 to skip)")
-    p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="OpenMP threads of the CPU baseline (0: the host's share)")
     p.add_argument("--dt", type=float, default=5.0)
     p.add_argument("--control-dt", type=float, default=0.25,
                    help="dt of the timed runs, one the reference scheme integrates without "
@@ -84,10 +86,14 @@ def parse():
     p.add_argument("--ensemble-members", type=int, default=32,
                    help="members of the batched-ensemble leg (0 to skip)")
     p.add_argument("--ensemble-steps", type=int, default=20)
-    p.add_argument("--perturbation", type=int, default=1,
-                   help="time the FP32 perturbation form, plain and with the surrogate (0 to skip)")
+    p.add_argument("--ensemble-dt", type=float, default=0.5,
+                   help="dt of the timed ensemble run (the configuration's 2 s diverges, and "
+                        "1 s on the 60-level members within twenty steps)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run one device-to-device state copy (profiler byte calibration)")
+    p.add_argument("--legs", default="all",
+                   help="comma list of legs beyond the headline: full,config_dt,copy,fp64,"
+                        "surrogate,error,ensemble (all: every leg)")
     return p.parse_args()
 
 
@@ -111,11 +117,16 @@ def column_mass(st, zi):
     return float(np.sum(dz * rho.reshape(rho.shape[0], -1).sum(axis=1)))
 
 
-def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None, ctx=None,
-        calibrate_copy=False, info=None, perturbation=False, mass=False):
-    ctx, grid, st, zi = make_case(G, precision, nx, ny, nz, ctx, perturbation)
-    if mass:
-        info["column_mass_initial"] = column_mass(st, zi)
+KERNELS = ("fast_column", "slow_stage", "level_partials", "level_finalize", "tracer_stage",
+           "warm_rain", "warm_rain_surrogate", "fallout")
+
+
+def run(G, precision, a, dt, timing=False, surrogate=None, ctx=None, perturbation=False,
+        calibrate_copy=False, info=None, steps=None):
+    """Warm-up, then `steps` timed moist steps; returns a leg record with
+    the throughput, or value None when the state ended non-finite."""
+    steps = a.steps if steps is None else steps
+    ctx, grid, st, zi = make_case(G, precision, a.nx, a.ny, a.nz, ctx, perturbation)
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
 
@@ -130,7 +141,7 @@ def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None,
         other.copy_from(st)
         ctx.sync()
         del other
-    for _ in range(warmup):
+    for _ in range(a.warmup):
         step()
     ctx.sync()
     if timing:
@@ -144,64 +155,152 @@ def run(G, precision, nx, ny, nz, steps, warmup, timing, dt=5.0, surrogate=None,
     secs = time.perf_counter() - t0
     stats = {}
     if timing:
-        for name in ("fast_column", "slow_stage", "level_partials", "level_finalize",
-                     "tracer_stage", "warm_rain", "warm_rain_surrogate", "fallout"):
+        for name in KERNELS:
             n, ms = ctx.kernel_stats(name)
             if n:
                 stats[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n}
         ctx.set_kernel_timing(False)
     summary = st.summary()
-    if mass:
-        info["column_mass_final"] = column_mass(st, zi)
+    finite = summary["nonfinite_values"] == 0
     if info is not None:
         info["tuning"] = ctx.tuning_report()
-    return secs, stats, summary
+    cells = a.nx * a.ny * a.nz
+    rec = {"dt": dt, "steps": steps, "state_finite": bool(finite),
+           "nonfinite_values": int(summary["nonfinite_values"]),
+           "value": cells * steps / secs if finite else None,
+           "ms_per_step": secs / steps * 1e3}
+    if not finite:
+        rec["note"] = "state non-finite after the timed steps: no throughput credited"
+    return rec, stats, summary
 
 
-def cpu_baseline(nx, ny, nz, threads, dt):
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    return model, os.cpu_count() or share, share
+
+
+def cpu_baseline(a, dt):
+    """FP64 CPU oracle on a bounded sample of the workload, as the baseline
+    plan states: OpenMP threads = the host's CPU share (OMP_NUM_THREADS when
+    the host sets it), one warm-up step, then the median of three timed
+    single steps."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import numpy as np
     import pyoracle as O
     import gwm_amd as G
 
-    # Bounded sample of the same workload: a quarter of the horizontal domain
-    # (identical columns, levels and physics), one moist step, FP64 oracle.
-    nx, ny = max(nx // 2, 8), max(ny // 2, 8)
+    model, nproc, share = host_cpu()
+    threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or share
+    # A quarter of the horizontal domain (identical columns, levels and
+    # physics), FP64 oracle, plain warm rain.
+    nx, ny, nz = max(a.nx // 2, 8), max(a.ny // 2, 8), a.nz
     zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
     ctx, grid, st, _ = make_case(G, G.F64, nx, ny, nz)
     w = O.World(nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi, n_tracers=3, precision=8)
     for f in (O.RHO, O.RHO_THETA, O.MOM_U, O.MOM_V, O.MOM_W, O.TRACER0):
         w.set(f, st.get(f))
     del st, grid, ctx
-    steps = 1
-    secs = w.time_steps(O.step_cfg(dt=dt, fast_substeps=6),
-                        O.warm_rain_cfg(dt=dt, rain_terminal_velocity=12.0), moist=True,
-                        steps=steps, threads=threads)
-    _ = np.float64(secs)
+    cfg = O.step_cfg(dt=dt, fast_substeps=6)
+    wr = O.warm_rain_cfg(dt=dt, rain_terminal_velocity=12.0)
+    warm = w.time_steps(cfg, wr, moist=True, steps=1, threads=threads)
+    runs = [w.time_steps(cfg, wr, moist=True, steps=1, threads=threads) for _ in range(3)]
+    secs = statistics.median(runs)
     return {
-        "value": nx * ny * nz * steps / secs,
+        "value": nx * ny * nz / secs,
         "unit": "cell-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} full moist step of a {nx}x{ny}x{nz} quarter of the case at dt {dt} s "
-                  f"(finite state; FP64 CPU oracle, {threads} OpenMP threads, {secs:.1f} s)",
+        "sample": f"{nx}x{ny}x{nz} quarter of the case at dt {dt} s (finite state), FP64 CPU "
+                  f"oracle, {threads} OpenMP threads; one warm-up step ({warm:.2f} s), then the "
+                  f"median of 3 single steps ({', '.join(f'{r:.2f}' for r in runs)} s)",
+        "host_cpu": model, "host_nproc": nproc, "host_cpu_share": share,
     }
 
 
-def surrogate_leg(G, a, cells, info):
-    """Train the warm-rain surrogate on plain-scheme pairs (seeds 0-7 of the
-    case), report its error against the plain scheme at seed 42, and time the
-    FP32+surrogate step."""
+def rel_errors(G, ref, states, fields):
+    """Relative L2 (floor 1e-3 of each field's physical scale) of each
+    state's fields against the reference state, downloaded in FP64."""
     import numpy as np
+    out = {name: {} for name in states}
+    for f in fields:
+        r = ref.get_f64(f) if ref.perturbation else ref.get(f).astype(np.float64)
+        den = max(np.linalg.norm(r), 1e-3 * FIELD_SCALE[f] * np.sqrt(r.size))
+        for name, st in states.items():
+            x = st.get_f64(f)
+            out[name][FIELD_NAMES[f]] = float(np.linalg.norm(x - r) / den)
+        del r
+    for name in out:
+        out[name]["worst"] = max(out[name].values())
+    return out
+
+
+def error_leg(G, a, dt, sur):
+    """From the same initial state: `error_steps` moist steps of the FP64
+    plain step, the FP32 full form, the FP32 perturbation form, and both
+    with the surrogate; relative L2 of each against the FP64 plain step,
+    and the dry-mass change of the perturbation form."""
+    ctx64 = G.Context(0, G.F64)
+    ctx32 = G.Context(0, G.F32)
+    _, _, ref, zi = make_case(G, G.F64, a.nx, a.ny, a.nz, ctx64)
+    states = {}
+    _, _, states["fp32_full"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32)
+    _, _, states["fp32_perturbation"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32,
+                                                     perturbation=True)
+    surs = {}
+    if sur is not None:
+        s = G.Surrogate(ctx32, 0)
+        s.set_norm(**{k: v for k, v in sur[1].items()})
+        s.set_params(sur[0])
+        for form, pert in (("fp32_full_surrogate", False), ("fp32_perturbation_surrogate", True)):
+            _, _, states[form], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32, perturbation=pert)
+            surs[form] = s
+    cfg = G.step_config(dt=dt, fast_substeps=6)
+    wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
+    m0 = column_mass(states["fp32_perturbation"], zi)
+    masses = [m0]
+    for _ in range(a.error_steps):
+        ref.moist_step(cfg, wr)
+        for name, st in states.items():
+            if name in surs:
+                st.moist_step_surrogate(cfg, wr, surs[name])
+            else:
+                st.moist_step(cfg, wr)
+        masses.append(column_mass(states["fp32_perturbation"], zi))
+    finite = {name: st.summary()["nonfinite_values"] == 0 for name, st in states.items()}
+    finite["fp64_plain"] = ref.summary()["nonfinite_values"] == 0
+    errs = rel_errors(G, ref, states, (0, 1, 2, 3, 4, 16, 17, 18))
+    drift = [abs(m1 - m0) / m0 for m0, m1 in zip(masses[:-1], masses[1:])]
+    return {"dt": dt, "steps": a.error_steps, "reference": "FP64 plain step on the device",
+            "rel_l2_floor": "1e-3 of each field's physical scale",
+            "rel_l2": errs, "state_finite": finite,
+            "perturbation_dry_mass_drift_per_step_max": max(drift) if drift else None,
+            "perturbation_dry_mass_rel_change_total": abs(masses[-1] - m0) / m0}
+
+
+def surrogate_train(G, a):
+    """Train the warm-rain surrogate on plain-scheme pairs (seeds 0-7 of the
+    case) and report its tendency error against the plain scheme at seed
+    42, next to the untrained network."""
     import gwm_amd.surrogate as SG
 
     ctx = G.Context(0, G.F32)
     seeds = list(range(8))
     per = int(a.surrogate_pairs) // len(seeds)
+    dt = a.control_dt or a.dt
     log(f"surrogate: sampling {a.surrogate_pairs:.0f} pairs")
     t0 = time.perf_counter()
     X, Y, n = SG.generate(ctx, seeds, per, shape=(a.nx, a.ny, a.nz),
-                          wr=G.warm_rain_config(dt=a.dt, rain_terminal_velocity=12.0))
+                          wr=G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0))
     ctx.sync()
     t_gen = time.perf_counter() - t0
     log("surrogate: training")
@@ -212,58 +311,21 @@ def surrogate_leg(G, a, cells, info):
     t_train = time.perf_counter() - t0
     del X, Y
     log(f"surrogate: epoch losses {list(losses)}; error at seed 42")
-    errs = SG.tendency_errors(ctx, sur, seed=42, shape=(min(a.nx, 256), min(a.ny, 256), a.nz),
-                              wr=G.warm_rain_config(dt=a.dt, rain_terminal_velocity=0.0))
-    # Throughput of the FP32+surrogate step at the finite-state dt.
-    dt = a.control_dt or a.dt
-    sinfo = {}
-    log("surrogate: FP32+surrogate step")
-    ss, sstats, ssum = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt,
-                           surrogate=sur, ctx=ctx)
-    _, sstats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, dt, surrogate=sur,
-                       ctx=ctx, info=sinfo)
-    return {
-        "_model": (sur.params(), sur.norm()),
-        "pairs": n, "seeds": seeds, "epochs": a.surrogate_epochs,
-        "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
-        "epoch_loss": [float(v) for v in losses],
-        "error_vs_plain_seed42": errs,
-        "value": cells * a.steps / ss, "dt": dt,
-        "state_finite": bool(np.isfinite(ssum["w_max"])),
-        "kernels": sstats,
-    }
-
-
-def perturbation_leg(G, a, cells, sur_src):
-    """The FP32 perturbation form (rho' and (rho theta)' against FP64
-    level-mean bases; within 1e-4 of the FP64 oracle in the tests) on the
-    same case at the finite-state dt: plain warm rain, and with the trained
-    surrogate."""
-    import numpy as np
-
-    dt = a.control_dt or a.dt
-    ctx = G.Context(0, G.F32)
-    out = {"dt": dt}
-    log("perturbation form: plain")
-    info = {}
-    s, _, summ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt, ctx=ctx,
-                     perturbation=True, info=info, mass=True)
-    out["value_plain"] = cells * a.steps / s
-    out["state_finite"] = bool(np.isfinite(summ["w_max"]))
-    m0, m1 = info["column_mass_initial"], info["column_mass_final"]
-    out["dry_mass_rel_change"] = abs(m1 - m0) / m0  # sum of rho dz, over all steps run
-    out["dry_mass_steps"] = a.steps + a.warmup
-    if sur_src is not None:
-        params, norm = sur_src
-        sur = G.Surrogate(ctx, 0)
-        sur.set_norm(**{k: v for k, v in norm.items()})
-        sur.set_params(params)
-        log("perturbation form: surrogate")
-        s, _, summ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, dt, surrogate=sur,
-                         ctx=ctx, perturbation=True)
-        out["value_surrogate"] = cells * a.steps / s
-        out["state_finite_surrogate"] = bool(np.isfinite(summ["w_max"]))
-    return out
+    shape = (min(a.nx, 256), min(a.ny, 256), a.nz)
+    wr0 = G.warm_rain_config(dt=dt, rain_terminal_velocity=0.0)
+    errs = SG.tendency_errors(ctx, sur, seed=42, shape=shape, wr=wr0)
+    raw = G.Surrogate(ctx, 0)
+    raw.set_norm(**{k: v for k, v in norm.items()})
+    errs_untrained = SG.tendency_errors(ctx, raw, seed=42, shape=shape, wr=wr0)
+    rec = {"pairs": n, "seeds": seeds, "epochs": a.surrogate_epochs, "dt": dt,
+           "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+           "epoch_loss": [float(v) for v in losses],
+           "tendency_error_vs_plain_seed42": {
+               k: (v if not isinstance(v, dict) else
+                   {"trained": v["surrogate"], "untrained": errs_untrained[k]["surrogate"],
+                    "no_source_terms": v["no_source_terms"]})
+               for k, v in errs.items()}}
+    return ctx, sur, (sur.params(), sur.norm()), rec
 
 
 def ensemble_members(G, n, seed=0, z_top=16000.0):
@@ -304,15 +366,16 @@ def ensemble_members(G, n, seed=0, z_top=16000.0):
 
 
 def ensemble_leg(G, a, sur_src):
-    """C5: 32 uneven members, FP32 + surrogate, dt 2 s, 4 substeps, advanced
+    """32 uneven members, FP32 + surrogate, 4 substeps, advanced
     concurrently from captured graphs (one stream per member), against the
-    same members stepped one after the other without graphs."""
-    import numpy as np
-
+    same members stepped one after the other without graphs; dt 0.5 s (the
+    configuration's 2 s overflows within three steps, 1 s within twenty on
+    the 60-level members)."""
     members = ensemble_members(G, a.ensemble_members)
     cells = sum(m[3] for m in members)
-    cfg = G.step_config(dt=2.0, fast_substeps=4)
-    wr = G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0)
+    dt = a.ensemble_dt
+    cfg = G.step_config(dt=dt, fast_substeps=4)
+    wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
     surs = None
     if sur_src is not None:
         params, norm = sur_src
@@ -328,8 +391,7 @@ def ensemble_leg(G, a, sur_src):
     t0 = time.perf_counter()
     G.ensemble_steps(states, cfg, wr, a.ensemble_steps, surs)
     t_graph = time.perf_counter() - t0
-    # the same work launched member by member, eager
-    for st, (ctx, _, _, _) in zip(states, members):
+    for ctx, _, _, _ in members:
         ctx.sync()
     t0 = time.perf_counter()
     for _ in range(a.ensemble_steps):
@@ -341,15 +403,36 @@ def ensemble_leg(G, a, sur_src):
     for ctx, _, _, _ in members:
         ctx.sync()
     t_eager = time.perf_counter() - t0
-    finite = all(np.isfinite(st.summary()["w_max"]) for st in states)
+    bad = sum(int(st.summary()["nonfinite_values"]) for st in states)
+    finite = bad == 0
     return {
-        "members": len(members), "total_cells": cells,
+        "members": len(members), "total_cells": cells, "dt": dt,
         "shapes": [list(map(int, (m[2].grid.nx, m[2].grid.ny, m[2].grid.nz))) for m in members],
         "steps": a.ensemble_steps, "surrogate": surs is not None,
-        "value_graph_concurrent": cells * a.ensemble_steps / t_graph,
-        "value_eager_sequential": cells * a.ensemble_steps / t_eager,
-        "state_finite": bool(finite),
+        "value_graph_concurrent": cells * a.ensemble_steps / t_graph if finite else None,
+        "value_eager_sequential": cells * a.ensemble_steps / t_eager if finite else None,
+        "state_finite": finite, "nonfinite_values": bad,
     }
+
+
+def copy_ceiling(G, a, reps=10):
+    """Device-to-device copy rate of a moist state (read + write bytes over
+    wall time, 8 fields and the surface bucket per copy): the box's
+    STREAM-copy ceiling beside the 8 TB/s peak."""
+    import numpy as np
+    ctx, grid, src, _ = make_case(G, G.F32, a.nx, a.ny, a.nz)
+    dst = G.State(grid, 3)
+    dst.copy_from(src)
+    elems = sum(int(np.prod(src.storage_shape(f))) for f in range(5))
+    elems += 3 * int(np.prod(src.storage_shape(0))) + a.nx * a.ny
+    nbytes = elems * 4 * 2
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_from(src)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    return nbytes * reps / secs / 1e9
 
 
 def main():
@@ -360,40 +443,28 @@ def main():
         sys.exit(2)
     import gwm_amd as G
 
+    legs = set(("full,config_dt,copy,fp64,surrogate,error,ensemble" if a.legs == "all"
+                else a.legs).split(","))
     cells = a.nx * a.ny * a.nz
-    import math
+    dt = a.control_dt if a.control_dt > 0 else a.dt
+    ns = 6
 
-    # The timed step runs at a dt the reference scheme integrates without
-    # diverging (the configuration's 5 s blows up in the reference's
-    # split-explicit scheme, and in this one, within a few steps); the per-step
-    # work does not depend on dt, and the configuration's dt is timed too.
-    main_dt = a.control_dt if a.control_dt > 0 else a.dt
-    log(f"FP32 plain step, dt {main_dt}")
-    secs, _, summary = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, main_dt,
-                           calibrate_copy=a.calibrate_copy)
-    value = cells * a.steps / secs
+    log(f"FP32 perturbation form, plain warm rain, dt {dt}")
+    head, _, summary = run(G, G.F32, a, dt, perturbation=True, calibrate_copy=a.calibrate_copy)
     # Per-kernel HIP-event timings come from a second pass, so the events do
     # not sit inside the timed region above.
     info = {}
-    log("FP32 plain step, per-kernel timing")
-    _, stats, _ = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, True, main_dt, info=info)
-    finite = all(math.isfinite(summary[k]) for k in ("w_min", "w_max", "total_dry_mass"))
-    config_dt = None
-    if main_dt != a.dt:
-        log(f"FP32 plain step at the configuration's dt {a.dt}")
-        cs, _, csum = run(G, G.F32, a.nx, a.ny, a.nz, a.steps, a.warmup, False, a.dt)
-        config_dt = {"dt": a.dt, "value": cells * a.steps / cs,
-                     "state_finite": all(math.isfinite(csum[k]) for k in ("w_min", "w_max"))}
-    ns = 6
+    log("FP32 perturbation form, per-kernel timing")
+    _, stats, _ = run(G, G.F32, a, dt, timing=True, perturbation=True, info=info)
+
     # Dominant kernel: the fast substep, which carries face departures from
     # the subcycle's entry faces.  Algorithmic elements per cell: the first
     # substep reads rho, theta and writes rho and three departures (6); a
     # middle substep reads rho, theta, three departures and writes rho and
     # three departures (9); the last reads rho, theta, three departures and
     # the three entry faces and writes rho, rho*theta and three faces (13).
-    kname = "fast_column"
-    fs = stats.get(kname)
     roofline = None
+    fs = stats.get("fast_column")
     if fs:
         elems = (6 + 9 * (ns - 2) + 13) / ns
         bytes_per_launch = elems * 4 * cells
@@ -401,11 +472,12 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
                     "traffic": measured_traffic(a.nx, a.ny, a.nz),
-                    "kernel": kname, "avg_ms": round(fs["avg_ms"], 5),
+                    "kernel": "fast_column", "avg_ms": round(fs["avg_ms"], 5),
                     "algorithmic_bytes": int(bytes_per_launch)}
     # Other stencil kernels against their own algorithmic bytes: slow stage
     # (mean of modes: stage 1 reads 5 writes 5, stages 2-3 read 10 write 5),
-    # tracer stage (3 tracers: rho, 3 faces, 3 in, 3 q0 except stage 1, 3 out).
+    # tracer stage (3 tracers: rho, 3 faces, 3 in, 3 q0 except stage 1, 3
+    # out), warm rain (read 5, write 4).
     kernel_roofline = {}
     for name, elems in (("slow_stage", (10 + 15 + 15) / 3), ("tracer_stage", (10 + 13 + 13) / 3),
                         ("warm_rain", 9)):
@@ -414,55 +486,77 @@ def main():
             gbs = b / (stats[name]["avg_ms"] * 1e-3) / 1e9
             kernel_roofline[name] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / 8000.0, 4),
                                      "algorithmic_bytes": int(b)}
-    log("copy ceiling")
-    copy_gbs = copy_ceiling(G, a.nx, a.ny, a.nz)
-    if roofline:
-        roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
-        roofline["frac_of_copy_ceiling"] = round(roofline["achieved"] / copy_gbs, 4)
-    fp64 = None
-    if a.fp64_steps > 0:
+
+    full = config_dt = fp64 = sur_rec = err = ens = None
+    if "full" in legs:
+        log("FP32 full form (the reference's formulation), dt %g" % dt)
+        full, _, _ = run(G, G.F32, a, dt)
+    if "config_dt" in legs and dt != a.dt:
+        log(f"FP32 full form at the configuration's dt {a.dt}")
+        config_dt, _, _ = run(G, G.F32, a, a.dt)
+    if "copy" in legs:
+        log("copy ceiling")
+        copy_gbs = copy_ceiling(G, a)
+        if roofline:
+            roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
+            roofline["frac_of_copy_ceiling"] = round(roofline["achieved"] / copy_gbs, 4)
+    if "fp64" in legs and a.fp64_steps > 0:
         log("FP64 plain step")
-        s64, _, _ = run(G, G.F64, a.nx, a.ny, a.nz, a.fp64_steps, 1, False, main_dt)
-        fp64 = cells * a.fp64_steps / s64
-    sur = surrogate_leg(G, a, cells, info) if a.surrogate_pairs > 0 else None
-    if sur is not None and fp64:
-        sur["over_fp64_plain"] = sur["value"] / fp64
-    ens = None
-    model = sur.pop("_model", None) if sur else None
-    pert = perturbation_leg(G, a, cells, model) if a.perturbation else None
-    if pert is not None and fp64:
-        for k in ("value_plain", "value_surrogate"):
-            if k in pert:
-                pert[k.replace("value", "over_fp64_plain")] = pert[k] / fp64
-    if a.ensemble_members > 0:
+        fp64, _, _ = run(G, G.F64, a, dt, steps=a.fp64_steps)
+    model = None
+    if "surrogate" in legs and a.surrogate_pairs > 0:
+        sctx, sur, model, sur_rec = surrogate_train(G, a)
+        log("FP32 + surrogate: full form, then perturbation form")
+        sfull, sstats, _ = run(G, G.F32, a, dt, surrogate=sur, ctx=sctx)
+        _, sstats, _ = run(G, G.F32, a, dt, surrogate=sur, ctx=sctx, timing=True)
+        spert, _, _ = run(G, G.F32, a, dt, surrogate=sur, ctx=sctx, perturbation=True)
+        sur_rec["full_form"] = sfull
+        sur_rec["perturbation_form"] = spert
+        sur_rec["kernels"] = sstats
+        del sur, sctx
+    if "error" in legs and a.error_steps > 0:
+        log(f"error leg: {a.error_steps} steps against the FP64 step")
+        err = error_leg(G, a, dt, model)
+
+    ratios = {}
+    if fp64 and fp64["value"]:
+        for name, rec in (("fp32_perturbation", head), ("fp32_full", full),
+                          ("fp32_full_surrogate", sur_rec and sur_rec["full_form"]),
+                          ("fp32_perturbation_surrogate", sur_rec and sur_rec["perturbation_form"])):
+            if rec and rec.get("value"):
+                ratios[name] = rec["value"] / fp64["value"]
+    if "ensemble" in legs and a.ensemble_members > 0:
         log(f"ensemble: {a.ensemble_members} members")
         ens = ensemble_leg(G, a, model)
+    cpu = None
     if a.cpu_baseline:
         log("CPU baseline")
-    cpu = (cpu_baseline(a.nx, a.ny, a.nz, a.cpu_threads, a.control_dt or a.dt)
-           if a.cpu_baseline else None)
+        cpu = cpu_baseline(a, dt)
+
     out = {
         "metric": "cell-steps/s per GPU, % HBM roofline; FP32+surrogate vs FP64 plain at set error",
-        "value": value,
+        "value": head["value"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": secs / a.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"warm-rain convection {a.nx}x{a.ny}x{a.nz}, dx 1 km, stretched "
-                               f"100-500 m to 16 km, dt {main_dt} s, 6 substeps, plain warm rain",
+                               f"100-500 m to 16 km, dt {dt} s, 6 substeps, plain warm rain, "
+                               f"FP32 perturbation form",
                    "global_batch": 1, "parallelism": "single GPU"},
-        "state_finite": finite,
+        "state_finite": head["state_finite"],
+        "fp32_full_form": full,
         "at_configuration_dt": config_dt,
-        "fp64_plain_value": fp64,
-        "fp32_over_fp64": (value / fp64) if fp64 else None,
-        "surrogate": sur,
-        "perturbation_form": pert,
+        "fp64_plain": fp64,
+        "over_fp64_plain": ratios,
+        "surrogate": sur_rec,
+        "error_vs_fp64": err,
         "ensemble": ens,
         "dry_mass": summary["total_dry_mass"],
         "kernels": stats,

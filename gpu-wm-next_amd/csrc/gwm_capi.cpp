@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -297,6 +298,7 @@ struct gwm_state {
   void* bref[5] = {};
   std::vector<void*> bref_tr;
   bool have_boundary = false;
+  unsigned tracer_strip_mask = ~0u;  // tracers the strip boundary overwrites
   void* theta_ref = nullptr;  // nz
   void* rref_lev = nullptr;   // nz
   void* pref_lev = nullptr;   // nz
@@ -327,7 +329,26 @@ struct gwm_state {
     int cur = -1;
     void* tr0 = nullptr;
     std::vector<double> key;
+    std::vector<const void*> bufs;  // device buffers at capture (buffers())
   };
+  // Every device buffer of the state, in a fixed order.  A captured step
+  // graph bakes these addresses in; a replay requires each buffer that
+  // existed at capture to be the same allocation still.
+  std::vector<const void*> buffers() const {
+    std::vector<const void*> b;
+    for (const auto& st : set)
+      for (const void* q : st) b.push_back(q);
+    for (const void* q : tend) b.push_back(q);
+    for (const void* q : bref) b.push_back(q);
+    for (const auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr}) {
+      b.push_back(reinterpret_cast<const void*>(v->size()));
+      for (const void* q : *v) b.push_back(q);
+    }
+    for (const void* q : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
+                          static_cast<void*>(step_part)})
+      b.push_back(q);
+    return b;
+  }
   Graph graphs[2];
   bool faces_synced = false;
   // theta_ref partials of the state (rho_set) that a subcycle's last substep
@@ -659,10 +680,12 @@ void apply_boundary_set(gwm_state* S, void* const* fs, int mode, double width) {
   }
 }
 
+// The stage's lateral boundary: first the host callback (which may refresh
+// the boundary reference with gwm_state_set_boundary, or edit the stage),
+// then `mode` against the boundary reference.
 template <class R>
 void run_boundary(gwm_state* S, void** fs, const gwm_boundary_config* b, double sim_time) {
   if (!b) return;
-  apply_boundary_set<R>(S, fs, b->mode, b->relax_width);
   if (b->callback) {
     gwm_state view;
     view.ctx = S->ctx;
@@ -676,6 +699,7 @@ void run_boundary(gwm_state* S, void** fs, const gwm_boundary_config* b, double 
     const int rc = b->callback(&view, sim_time, b->user);
     require(rc == 0, "boundary callback failed");
   }
+  apply_boundary_set<R>(S, fs, b->mode, b->relax_width);
 }
 
 // advance_dry_state_ssprk3 (dry_core.cu:541-651).  fast_fn != null replaces
@@ -810,15 +834,9 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode,
   a.lev_idz_c = static_cast<const R*>(S->grid->lev_idz_c);
   a.ntr = S->ntr;
   a.dt = static_cast<R>(c.dt);
+  // callback first (it may refresh the tracer boundary reference), then
+  // the strips of the tracers in the strip mask
   auto strips = [&](std::vector<void*>& f, double sim_time) {
-    if (bmode != GWM_BOUNDARY_NONE) {
-      require(S->have_boundary, "boundary reference not captured");
-      for (int t = 0; t < S->ntr; ++t) {
-        Timed tt(S->ctx, "boundary");
-        gwm::launch_boundary<R>(S->ctx->stream, g, 0, static_cast<R*>(f[t]),
-                                static_cast<const R*>(S->bref_tr[t]), GWM_BOUNDARY_STRIPS, 0.0);
-      }
-    }
     if (fn) {
       gwm_state view;
       view.ctx = S->ctx;
@@ -831,6 +849,15 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode,
       view.vtr = f;
       HIPCHK(hipStreamSynchronize(S->ctx->stream));
       require(fn(&view, sim_time, user) == 0, "tracer boundary callback failed");
+    }
+    if (bmode != GWM_BOUNDARY_NONE) {
+      require(S->have_boundary, "boundary reference not captured");
+      for (int t = 0; t < S->ntr; ++t) {
+        if (t < 32 && !((S->tracer_strip_mask >> t) & 1u)) continue;
+        Timed tt(S->ctx, "boundary");
+        gwm::launch_boundary<R>(S->ctx->stream, g, 0, static_cast<R*>(f[t]),
+                                static_cast<const R*>(S->bref_tr[t]), GWM_BOUNDARY_STRIPS, 0.0);
+      }
     }
   };
   const bool bnd = bmode != GWM_BOUNDARY_NONE || fn;
@@ -904,12 +931,24 @@ void warm_rain(gwm_state* S, const gwm_warm_rain_config& c, int metric) {
   gwm::launch_warm_rain<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
 }
 
-// Compensated reduction of one field; returns {sum, min, max}.
+// Where a reduction keeps its block partials: by default the context's
+// scratch (grown on demand) and a host vector; a caller may pass both.
+struct ReduceBufs {
+  double* dev = nullptr;   // nb * kReduceParts doubles, or null: context scratch
+  double* host = nullptr;  // the same count, or null: a host vector
+};
+
+constexpr int kReduceBlocks = 512;
+
+// Compensated reduction of one field; returns {sum, min, max, non-finite
+// count} (sum and extremes over the finite values).
 template <class R>
 void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, const R* rho,
-                  double out[3], const R* fbase = nullptr, const R* rbase = nullptr) {
-  const int nb = 512;
-  double* part = S->ctx->scratch(static_cast<size_t>(nb) * 4);
+                  double out[4], const R* fbase = nullptr, const R* rbase = nullptr,
+                  const ReduceBufs& rb = ReduceBufs{}, int halo = -1) {
+  const int nb = kReduceBlocks;
+  const int np = gwm::kReduceParts;
+  double* part = rb.dev ? rb.dev : S->ctx->scratch(static_cast<size_t>(nb) * np);
   gwm::ReduceArgs<R> a{};
   a.f = f;
   a.rho = rho;
@@ -920,41 +959,66 @@ void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, co
   a.fx = fx;
   a.fy = fy;
   a.fz = fz;
-  a.h = S->g().h;
+  a.h = halo >= 0 ? halo : S->g().h;
   a.kind = kind;
   a.partials = part;
-  {
+  if (rb.dev) {
+    gwm::launch_reduce<R>(S->ctx->stream, a, nb);
+    HIPCHK(hipGetLastError());
+  } else {
     Timed t(S->ctx, "reduce");
     gwm::launch_reduce<R>(S->ctx->stream, a, nb);
   }
-  std::vector<double> h(static_cast<size_t>(nb) * 4);
-  HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        S->ctx->stream));
+  std::vector<double> hv;
+  double* h = rb.host;
+  if (!h) {
+    hv.resize(static_cast<size_t>(nb) * np);
+    h = hv.data();
+  }
+  HIPCHK(hipMemcpyAsync(h, part, static_cast<size_t>(nb) * np * sizeof(double),
+                        hipMemcpyDeviceToHost, S->ctx->stream));
   HIPCHK(hipStreamSynchronize(S->ctx->stream));
-  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300;
+  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300, bad = 0.0;
   for (int b = 0; b < nb; ++b) {  // Neumaier over block partials, fixed order
     for (int q = 0; q < 2; ++q) {
-      const double v = h[b * 4 + q];
+      const double v = h[b * np + q];
       const double s2 = sum + v;
       comp += std::fabs(sum) >= std::fabs(v) ? (sum - s2) + v : (v - s2) + sum;
       sum = s2;
     }
-    mn = std::min(mn, h[b * 4 + 2]);
-    mx = std::max(mx, h[b * 4 + 3]);
+    mn = std::min(mn, h[b * np + 2]);
+    mx = std::max(mx, h[b * np + 3]);
+    bad += h[b * np + 4];
   }
   out[0] = sum + comp;
   out[1] = mn;
   out[2] = mx;
+  out[3] = bad;
 }
 
+// rb: where the reductions keep their partials (default: the context's).
 template <class R>
-void budget(gwm_state* S, double* o) {
+void budget(gwm_state* S, double* o, const ReduceBufs& rb = ReduceBufs{}) {
   const Geo& g = S->g();
   for (int a = 0; a < GWM_SUMMARY_N; ++a) o[a] = 0.0;
   void** ds = S->cur_set();
   const R* rho = static_cast<const R*>(ds[0]);
   const gwm::Base<R> B = S->base<R>();
-  double r[3];
+  double r[4];
+  // Non-finite values over every prognostic field (the extremes and sums
+  // above are over the finite values only): a field with any NaN or
+  // infinity reports NaN extremes, so no check of them passes by accident.
+  double nonfinite = 0.0;
+  auto extremes = [&](int lo, int hi) {
+    nonfinite += r[3];
+    if (r[3] > 0.0) {
+      o[lo] = std::numeric_limits<double>::quiet_NaN();
+      o[hi] = std::numeric_limits<double>::quiet_NaN();
+    } else {
+      o[lo] = r[1];
+      o[hi] = r[2];
+    }
+  };
   // Perturbation form: sums of the departures plus the FP64 base sums, so
   // the mass budget keeps the departures' precision.
   double base_rho = 0.0, base_rt = 0.0;
@@ -963,44 +1027,54 @@ void budget(gwm_state* S, double* o) {
       base_rho += static_cast<double>(g.nx) * g.ny * S->base_rho64[static_cast<size_t>(k)];
       base_rt += static_cast<double>(g.nx) * g.ny * S->base_rt64[static_cast<size_t>(k)];
     }
-  reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r, B.rho);
+  reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r, B.rho, nullptr, rb);
   o[GWM_S_TOTAL_DRY_MASS] = r[0] + base_rho;
-  o[GWM_S_RHO_MIN] = r[1];
-  o[GWM_S_RHO_MAX] = r[2];
-  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 0, nullptr, r, B.rt);
+  extremes(GWM_S_RHO_MIN, GWM_S_RHO_MAX);
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 0, nullptr, r, B.rt,
+                  nullptr, rb);
   o[GWM_S_TOTAL_RHO_THETA] = r[0] + base_rt;
-  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 2, rho, r, B.rt, B.rho);
-  o[GWM_S_THETA_MIN] = r[1];
-  o[GWM_S_THETA_MAX] = r[2];
-  reduce_field<R>(S, static_cast<const R*>(ds[2]), g.nx + 1, g.ny, g.nz, 0, nullptr, r);
+  nonfinite += r[3];
+  reduce_field<R>(S, static_cast<const R*>(ds[1]), g.nx, g.ny, g.nz, 2, rho, r, B.rt, B.rho,
+                  rb);
+  const double rt_bad = nonfinite;
+  extremes(GWM_S_THETA_MIN, GWM_S_THETA_MAX);
+  nonfinite = rt_bad;  // theta's non-finite cells are rho's and rho*theta's
+  reduce_field<R>(S, static_cast<const R*>(ds[2]), g.nx + 1, g.ny, g.nz, 0, nullptr, r, nullptr,
+                  nullptr, rb);
   o[GWM_S_TOTAL_MOM_X] = r[0];
-  o[GWM_S_U_MIN] = r[1];
-  o[GWM_S_U_MAX] = r[2];
-  reduce_field<R>(S, static_cast<const R*>(ds[3]), g.nx, g.ny + 1, g.nz, 0, nullptr, r);
+  extremes(GWM_S_U_MIN, GWM_S_U_MAX);
+  reduce_field<R>(S, static_cast<const R*>(ds[3]), g.nx, g.ny + 1, g.nz, 0, nullptr, r, nullptr,
+                  nullptr, rb);
   o[GWM_S_TOTAL_MOM_Y] = r[0];
-  o[GWM_S_V_MIN] = r[1];
-  o[GWM_S_V_MAX] = r[2];
-  reduce_field<R>(S, static_cast<const R*>(ds[4]), g.nx, g.ny, g.nz + 1, 0, nullptr, r);
+  extremes(GWM_S_V_MIN, GWM_S_V_MAX);
+  reduce_field<R>(S, static_cast<const R*>(ds[4]), g.nx, g.ny, g.nz + 1, 0, nullptr, r, nullptr,
+                  nullptr, rb);
   o[GWM_S_TOTAL_MOM_Z] = r[0];
-  o[GWM_S_W_MIN] = r[1];
-  o[GWM_S_W_MAX] = r[2];
+  extremes(GWM_S_W_MIN, GWM_S_W_MAX);
   for (int t = 0; t < std::min(S->ntr, 3); ++t) {
     const R* q = static_cast<const R*>(S->field(GWM_TRACER0 + t));
-    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 1, nullptr, r);
+    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 1, nullptr, r, nullptr, nullptr, rb);
     o[GWM_S_VAPOR_PATH + t] = r[0];
-    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 2, rho, r, nullptr, B.rho);
-    o[GWM_S_TRACER_QMIN0 + 2 * t] = r[1];
-    o[GWM_S_TRACER_QMIN0 + 2 * t + 1] = r[2];
+    nonfinite += r[3];
+    const double q_bad = nonfinite;
+    reduce_field<R>(S, q, g.nx, g.ny, g.nz, 2, rho, r, nullptr, B.rho, rb);
+    extremes(GWM_S_TRACER_QMIN0 + 2 * t, GWM_S_TRACER_QMIN0 + 2 * t + 1);
+    nonfinite = q_bad;
   }
   const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+  const R* bucket = static_cast<const R*>(S->view ? S->owner->precip : S->precip);
   std::vector<R> p(n2);
-  HIPCHK(hipMemcpyAsync(p.data(), S->view ? S->owner->precip : S->precip, n2 * sizeof(R),
+  HIPCHK(hipMemcpyAsync(p.data(), bucket, n2 * sizeof(R),
                         hipMemcpyDeviceToHost, S->ctx->stream));
   HIPCHK(hipStreamSynchronize(S->ctx->stream));
   double ps = 0.0, pc = 0.0, pm = 0.0;
   long wet = 0;
   for (size_t a = 0; a < n2; ++a) {
     const double v = static_cast<double>(p[a]);
+    if (!std::isfinite(v)) {
+      nonfinite += 1.0;
+      continue;
+    }
     const double s2 = ps + v;
     pc += std::fabs(ps) >= std::fabs(v) ? (ps - s2) + v : (v - s2) + ps;
     ps = s2;
@@ -1012,6 +1086,7 @@ void budget(gwm_state* S, double* o) {
   o[GWM_S_PRECIP_MAX] = pm;
   o[GWM_S_PRECIP_CELLS] = static_cast<double>(n2);
   o[GWM_S_PRECIPITATING_CELLS] = static_cast<double>(wet);
+  o[GWM_S_NONFINITE] = nonfinite;
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,6 +1471,52 @@ void init_convection(gwm_state* S, double p_s, unsigned long long seed, double a
         H.q[0][g.cell(i, j, k)] = r * static_cast<R>(qv[k]);
       }
   H.upload(S);
+}
+
+// extract_dry_plan_view / extract_runtime_plan_view
+// (src/io/plan_view_output.cpp:601-839) on the device: one column pass
+// writes every product, the host downloads the [field][j][i] planes.
+template <class R>
+int plan_view(gwm_state* S, int slice_k, int steps, double dt, int with_precip, double* host) {
+  const Geo& g = S->g();
+  const int k = std::clamp(slice_k, 0, g.nz - 1);
+  const int nf = S->ntr == 0 ? 10 : (S->ntr < 3 ? 13 : (with_precip ? 23 : 21));
+  const size_t plane = static_cast<size_t>(g.nx) * g.ny;
+  struct Buf {
+    double* p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } buf;
+  buf.p = static_cast<double*>(dalloc(plane * gwm::kPvCount * sizeof(double)));
+  void** ds = S->cur_set();
+  gwm::PlanViewArgs<R> a{};
+  a.g = g;
+  a.rho = static_cast<const R*>(ds[0]);
+  a.rt = static_cast<const R*>(ds[1]);
+  a.u = static_cast<const R*>(ds[2]);
+  a.v = static_cast<const R*>(ds[3]);
+  a.w = static_cast<const R*>(ds[4]);
+  if (S->ntr >= 1) a.qv = static_cast<const R*>(S->field(GWM_TRACER0));
+  if (S->ntr >= 3) {
+    a.qc = static_cast<const R*>(S->field(GWM_TRACER0 + 1));
+    a.qr = static_cast<const R*>(S->field(GWM_TRACER0 + 2));
+    if (with_precip) a.precip = static_cast<const R*>(S->view ? S->owner->precip : S->precip);
+  }
+  a.idz_c = static_cast<const R*>(S->grid->idz_c);
+  a.lev_idz_c = static_cast<const R*>(S->grid->lev_idz_c);
+  a.zc = static_cast<const R*>(S->grid->zc_m);
+  a.terrain = static_cast<const R*>(S->grid->terr);
+  a.base = S->base<R>();
+  a.k = k;
+  a.elapsed_hours = std::max(static_cast<double>(steps) * dt / 3600.0, 1.0e-9);
+  a.out = buf.p;
+  {
+    Timed t(S->ctx, "plan_view");
+    gwm::launch_plan_view<R>(S->ctx->stream, a, S->grid->terrain_nonzero);
+  }
+  scopy(S->ctx->stream, host, buf.p, plane * nf * sizeof(double), hipMemcpyDeviceToHost);
+  return nf;
 }
 
 template <class F>
@@ -1889,6 +2010,304 @@ int gwm_state_capture_boundary(gwm_state* s) {
   });
 }
 
+int gwm_state_set_boundary(gwm_state* s, int f, const void* host) {
+  return guard([&] {
+    S_(s);
+    require(host, "null host pointer");
+    gwm_state* o = s->view ? s->owner : s;
+    require(o->have_boundary, "boundary reference not captured (gwm_state_capture_boundary)");
+    void* d = nullptr;
+    if (f >= GWM_RHO && f <= GWM_MOM_W) {
+      d = o->bref[f];
+    } else {
+      const int t = f - GWM_TRACER0;
+      require(t >= 0 && t < o->ntr && t < static_cast<int>(o->bref_tr.size()), "bad field id");
+      d = o->bref_tr[static_cast<size_t>(t)];
+    }
+    HIPCHK(hipStreamSynchronize(o->ctx->stream));
+    if (o->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
+      // full values in, stored as departures from the state's bases
+      upload_departure(o, d, static_cast<const float*>(host),
+                       f == GWM_RHO ? o->base_rho64 : o->base_rt64);
+      return;
+    }
+    scopy(o->ctx->stream, d, host, o->fsize(f) * o->es(), hipMemcpyHostToDevice);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Restart dump.  The reference fixes only the artifact contract of a native
+// checkpoint (tools/casebuilder/prepare_case.py:376-392: schema
+// "gwm-next-checkpoint-stub/v1", restart family "gwm-native-checkpoint",
+// cold-start groups); this is that checkpoint.  Layout: a fixed header
+// (magic, version, shape, precision, form, tracer count, section count),
+// a table of sections (name, byte offset, byte count, FNV-1a 64 checksum),
+// then the sections: the canonical fields in storage layout (halos
+// included), the tracers, the surface bucket, the perturbation form's FP64
+// bases and the captured boundary reference.  A restore is bit-exact.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kCkptMagic[8] = {'G', 'W', 'M', 'C', 'K', 'P', 'T', '1'};
+constexpr std::uint32_t kCkptVersion = 1;
+
+struct CkptHeader {
+  char magic[8];
+  std::uint32_t version, precision, nx, ny, nz, halo, per_x, per_y, pert, ntr, boundary, nsec;
+  double dx, dy;
+};
+struct CkptSection {
+  char name[24];
+  std::uint64_t offset, bytes, fnv;
+};
+
+std::uint64_t fnv1a(const unsigned char* p, size_t n, std::uint64_t h = 1469598103934665603ull) {
+  for (size_t a = 0; a < n; ++a) {
+    h ^= p[a];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// The sections of a state, in file order: name, device pointer or host
+// bytes, byte count.
+struct CkptItem {
+  std::string name;
+  void* dev = nullptr;
+  std::vector<unsigned char>* host = nullptr;
+  size_t bytes = 0;
+};
+
+std::vector<CkptItem> ckpt_items(gwm_state* S, std::vector<unsigned char>& bases) {
+  std::vector<CkptItem> v;
+  const char* fname[5] = {"rho", "rho_theta", "mom_u", "mom_v", "mom_w"};
+  for (int f = 0; f < 5; ++f) v.push_back({fname[f], S->field(f), nullptr, S->fsize(f) * S->es()});
+  for (int t = 0; t < S->ntr; ++t)
+    v.push_back({"tracer" + std::to_string(t), S->tr[static_cast<size_t>(t)], nullptr,
+                 S->g().cell_total() * S->es()});
+  if (S->ntr > 0)
+    v.push_back({"precip", S->precip, nullptr, static_cast<size_t>(S->g().nx) * S->g().ny * S->es()});
+  if (S->pert) {
+    const size_t nz = static_cast<size_t>(S->g().nz);
+    bases.resize(2 * nz * sizeof(double));
+    v.push_back({"bases_fp64", nullptr, &bases, bases.size()});
+  }
+  if (S->have_boundary) {
+    for (int f = 0; f < 5; ++f)
+      v.push_back({std::string("boundary_") + fname[f], S->bref[f], nullptr, S->fsize(f) * S->es()});
+    for (int t = 0; t < S->ntr && t < static_cast<int>(S->bref_tr.size()); ++t)
+      v.push_back({"boundary_tracer" + std::to_string(t), S->bref_tr[static_cast<size_t>(t)], nullptr,
+                   S->g().cell_total() * S->es()});
+  }
+  return v;
+}
+
+std::string json_escape(const std::string& in) {
+  std::string o;
+  for (char c : in) {
+    if (c == '"' || c == '\\') o += '\\';
+    if (static_cast<unsigned char>(c) < 0x20) continue;
+    o += c;
+  }
+  return o;
+}
+
+void checkpoint_write(gwm_state* S, const std::string& path, const char* meta) {
+  require(!S->view, "checkpoint of a stage view");
+  const Geo& g = S->g();
+  std::vector<unsigned char> bases;
+  auto items = ckpt_items(S, bases);
+  if (S->pert) {
+    const size_t nz = static_cast<size_t>(g.nz);
+    std::memcpy(bases.data(), S->base_rho64.data(), nz * sizeof(double));
+    std::memcpy(bases.data() + nz * sizeof(double), S->base_rt64.data(), nz * sizeof(double));
+  }
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = kCkptVersion;
+  h.precision = static_cast<std::uint32_t>(S->ctx->prec);
+  h.nx = g.nx;
+  h.ny = g.ny;
+  h.nz = g.nz;
+  h.halo = g.h;
+  h.per_x = g.per_x;
+  h.per_y = g.per_y;
+  h.pert = S->pert ? 1 : 0;
+  h.ntr = static_cast<std::uint32_t>(S->ntr);
+  h.boundary = S->have_boundary ? 1 : 0;
+  h.nsec = static_cast<std::uint32_t>(items.size());
+  h.dx = g.dx;
+  h.dy = g.dy;
+  std::vector<CkptSection> sec(items.size());
+  std::uint64_t off = sizeof(CkptHeader) + sec.size() * sizeof(CkptSection);
+  for (size_t n = 0; n < items.size(); ++n) {
+    std::memset(sec[n].name, 0, sizeof sec[n].name);
+    std::strncpy(sec[n].name, items[n].name.c_str(), sizeof sec[n].name - 1);
+    off = (off + 255) & ~static_cast<std::uint64_t>(255);
+    sec[n].offset = off;
+    sec[n].bytes = items[n].bytes;
+    off += items[n].bytes;
+  }
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  const std::string tmp = path + ".partial";
+  std::FILE* fp = std::fopen(tmp.c_str(), "wb");
+  require(fp != nullptr, "checkpoint: cannot open " + tmp + " for writing");
+  struct Closer {
+    std::FILE*& f;
+    ~Closer() {
+      if (f) std::fclose(f);
+    }
+  } closer{fp};
+  auto put = [&](const void* p, size_t n, std::uint64_t at) {
+    require(std::fseek(fp, static_cast<long>(at), SEEK_SET) == 0 && std::fwrite(p, 1, n, fp) == n,
+            "checkpoint: write to " + tmp + " failed");
+  };
+  std::vector<unsigned char> buf;
+  for (size_t n = 0; n < items.size(); ++n) {
+    const auto& it = items[n];
+    const unsigned char* src;
+    if (it.host) {
+      src = it.host->data();
+    } else {
+      buf.resize(it.bytes);
+      scopy(S->ctx->stream, buf.data(), it.dev, it.bytes, hipMemcpyDeviceToHost);
+      src = buf.data();
+    }
+    sec[n].fnv = fnv1a(src, it.bytes);
+    put(src, it.bytes, sec[n].offset);
+  }
+  put(&h, sizeof h, 0);
+  put(sec.data(), sec.size() * sizeof(CkptSection), sizeof h);
+  require(std::fclose(fp) == 0, "checkpoint: closing " + tmp + " failed");
+  fp = nullptr;
+  require(std::rename(tmp.c_str(), path.c_str()) == 0, "checkpoint: cannot move " + tmp + " to " + path);
+  // Manifest beside the dump: the reference's checkpoint-stub contract
+  // (read by tools/verify) plus what the dump holds.
+  std::FILE* mf = std::fopen((path + ".json").c_str(), "wb");
+  require(mf != nullptr, "checkpoint: cannot write the manifest " + path + ".json");
+  std::string m = "{\n  \"schema_version\": \"gwm-next-checkpoint-stub/v1\",\n";
+  m += "  \"restart_family\": \"gwm-native-checkpoint\",\n";
+  m += "  \"cold_start_contract\": {\n    \"analysis_state_schema\": \"gwm-next-analysis-state/v1\",\n";
+  m += "    \"boundary_cache_schema\": \"gwm-next-boundary-cache/v1\",\n";
+  m += "    \"required_groups\": [\"atmosphere\", \"surface\", \"static_surface\"]\n  },\n";
+  const std::string base = path.substr(path.find_last_of('/') == std::string::npos ? 0 : path.find_last_of('/') + 1);
+  char line[512];
+  std::snprintf(line, sizeof line,
+                "  \"checkpoint\": {\n    \"file\": \"%s\",\n    \"format\": \"gwm-next-checkpoint/v1\",\n"
+                "    \"grid\": {\"nx\": %d, \"ny\": %d, \"nz\": %d, \"halo\": %d, \"dx\": %.17g, "
+                "\"dy\": %.17g, \"periodic_x\": %s, \"periodic_y\": %s},\n"
+                "    \"precision\": \"%s\",\n    \"perturbation_form\": %s,\n"
+                "    \"n_tracers\": %d,\n    \"boundary_reference\": %s,\n    \"sections\": %zu\n  }",
+                json_escape(base).c_str(), g.nx, g.ny, g.nz, g.h, g.dx, g.dy, g.per_x ? "true" : "false",
+                g.per_y ? "true" : "false", S->ctx->prec == GWM_F64 ? "fp64" : "fp32",
+                S->pert ? "true" : "false", S->ntr, S->have_boundary ? "true" : "false", items.size());
+  m += line;
+  if (meta != nullptr && *meta) m += std::string(",\n  \"metadata\": ") + meta;
+  m += "\n}\n";
+  const bool ok = std::fwrite(m.data(), 1, m.size(), mf) == m.size();
+  const bool closed = std::fclose(mf) == 0;
+  require(ok && closed, "checkpoint: writing the manifest " + path + ".json failed");
+}
+
+void checkpoint_read(gwm_state* S, const std::string& path) {
+  require(!S->view, "restore into a stage view");
+  const Geo& g = S->g();
+  std::FILE* fp = std::fopen(path.c_str(), "rb");
+  require(fp != nullptr, "checkpoint: cannot open " + path);
+  struct Closer {
+    std::FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{fp};
+  auto get = [&](void* p, size_t n, std::uint64_t at) {
+    require(std::fseek(fp, static_cast<long>(at), SEEK_SET) == 0 && std::fread(p, 1, n, fp) == n,
+            "checkpoint: " + path + " is truncated");
+  };
+  CkptHeader h{};
+  get(&h, sizeof h, 0);
+  require(std::memcmp(h.magic, kCkptMagic, 8) == 0, "checkpoint: " + path + " is not a gwm checkpoint");
+  require(h.version == kCkptVersion, "checkpoint: unsupported version " + std::to_string(h.version));
+  auto same = [&](bool c, const std::string& what) {
+    require(c, "checkpoint: " + path + " holds a different " + what + " than the state");
+  };
+  same(h.nx == static_cast<std::uint32_t>(g.nx) && h.ny == static_cast<std::uint32_t>(g.ny) &&
+           h.nz == static_cast<std::uint32_t>(g.nz) && h.halo == static_cast<std::uint32_t>(g.h),
+       "grid shape (" + std::to_string(h.nx) + "x" + std::to_string(h.ny) + "x" + std::to_string(h.nz) +
+           ", halo " + std::to_string(h.halo) + ")");
+  same(h.per_x == static_cast<std::uint32_t>(g.per_x) && h.per_y == static_cast<std::uint32_t>(g.per_y),
+       "lateral periodicity");
+  same(h.precision == static_cast<std::uint32_t>(S->ctx->prec), "precision");
+  same(h.pert == (S->pert ? 1u : 0u), "storage form (perturbation or full)");
+  same(h.ntr == static_cast<std::uint32_t>(S->ntr), "tracer count");
+  if (h.boundary && !S->have_boundary) {  // the reference arrives with the dump
+    for (int f = 0; f < 5; ++f)
+      if (!S->bref[f]) S->bref[f] = S->alloc_set_field(f);
+    while (static_cast<int>(S->bref_tr.size()) < S->ntr)
+      S->bref_tr.push_back(dalloc(g.cell_total() * S->es()));
+    S->have_boundary = true;
+  }
+  std::vector<unsigned char> bases;
+  const auto items = ckpt_items(S, bases);
+  require(h.nsec == items.size() || (!h.boundary && S->have_boundary),
+          "checkpoint: " + path + " has an unexpected section count");
+  std::vector<CkptSection> sec(h.nsec);
+  get(sec.data(), sec.size() * sizeof(CkptSection), sizeof h);
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  std::vector<unsigned char> buf;
+  for (size_t n = 0; n < sec.size(); ++n) {
+    const auto& it = items[n];
+    require(std::string(sec[n].name, strnlen(sec[n].name, sizeof sec[n].name)) == it.name &&
+                sec[n].bytes == it.bytes,
+            "checkpoint: section " + std::to_string(n) + " of " + path + " does not match " + it.name);
+    buf.resize(it.bytes);
+    get(buf.data(), it.bytes, sec[n].offset);
+    require(fnv1a(buf.data(), it.bytes) == sec[n].fnv,
+            "checkpoint: section " + it.name + " of " + path + " fails its checksum");
+    if (it.host)
+      *it.host = buf;
+    else
+      scopy(S->ctx->stream, it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice);
+  }
+  if (S->pert) {
+    const size_t nz = static_cast<size_t>(g.nz);
+    S->base_rho64.assign(reinterpret_cast<const double*>(bases.data()),
+                         reinterpret_cast<const double*>(bases.data()) + nz);
+    S->base_rt64.assign(reinterpret_cast<const double*>(bases.data()) + nz,
+                        reinterpret_cast<const double*>(bases.data()) + 2 * nz);
+    push_base(S);
+  }
+  S->faces_synced = false;
+  S->theta_parts_rho = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gwm_state_checkpoint_write(gwm_state* s, const char* path, const char* metadata_json) {
+  return guard([&] {
+    S_(s);
+    require(path != nullptr && *path, "null path");
+    checkpoint_write(s, path, metadata_json);
+  });
+}
+
+int gwm_state_checkpoint_read(gwm_state* s, const char* path) {
+  return guard([&] {
+    S_(s);
+    require(path != nullptr && *path, "null path");
+    checkpoint_read(s, path);
+  });
+}
+
+int gwm_state_set_tracer_strip_mask(gwm_state* s, unsigned mask) {
+  return guard([&] {
+    S_(s);
+    (s->view ? s->owner : s)->tracer_strip_mask = mask;
+  });
+}
+
 int gwm_halo_fill(gwm_state* s, int f) {
   return guard([&] {
     S_(s);
@@ -2152,6 +2571,18 @@ int gwm_budget_sums(gwm_state* s, double* out) {
     dispatch(s->ctx, [&](auto r) {
       budget<decltype(r)>(s, out);
       return 0;
+    });
+  });
+}
+
+int gwm_plan_view(gwm_state* s, int slice_k, int steps, double dt, int with_precip, double* out,
+                  int* nfields) {
+  return guard([&] {
+    S_(s);
+    require(out && nfields, "null output");
+    require(steps >= 0, "steps must be >= 0");
+    *nfields = dispatch(s->ctx, [&](auto r) {
+      return plan_view<decltype(r)>(s, slice_k, steps, dt, with_precip, out);
     });
   });
 }
@@ -2624,6 +3055,27 @@ std::vector<double> graph_key(const gwm_step_config& c, const gwm_warm_rain_conf
 // it on first use.  Before a capture, one eager step runs (it chooses the
 // chunk counts, allocates every scratch buffer and enters the two-step
 // buffer cycle); returns the number of eager steps taken (0 or 1).
+// Device buffers a captured moist-step graph references: the state's and,
+// with the surrogate, its parameters and fallback counter.
+std::vector<const void*> graph_buffers(const gwm_state* S, const gwm_surrogate* M) {
+  auto b = S->buffers();
+  if (M) {
+    b.push_back(M->T.params);
+    b.push_back(M->nfall);
+  }
+  return b;
+}
+
+// A buffer that existed when the graph was captured must still be the same
+// allocation (one allocated later, null at capture, is not referenced).
+void require_graph_buffers(const std::vector<const void*>& captured,
+                           const std::vector<const void*>& now) {
+  bool same = captured.size() == now.size();
+  for (size_t i = 0; same && i < captured.size(); ++i)
+    same = captured[i] == nullptr || captured[i] == now[i];
+  require(same, "internal: a device buffer referenced by a captured step graph was reallocated");
+}
+
 template <class R>
 int ensure_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& w,
                  gwm_surrogate* M, gwm_state::Graph** out) {
@@ -2631,6 +3083,7 @@ int ensure_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_con
   for (auto& gr : S->graphs)
     if (gr.exec && gr.cur == S->cur && gr.tr0 == (S->tr.empty() ? nullptr : S->tr[0]) &&
         gr.key == key) {
+      require_graph_buffers(gr.bufs, graph_buffers(S, M));
       *out = &gr;
       return 0;
     }
@@ -2676,6 +3129,7 @@ int ensure_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_con
   gr.cur = cur0;
   gr.tr0 = tr0;
   gr.key = key;
+  gr.bufs = graph_buffers(S, M);
   *out = &gr;
   return eager;
 }

@@ -213,9 +213,11 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
   if (i >= g.nx || j >= g.ny) return;
   const R dt = a.dt;
   R inc = 0;
-  const bool nominal = a.fallout == 1, metric = a.fallout == 2;
   const R dz_nom = rmax(a.depth, R(1.0e-3));
   const R frac_nom = rmin(rmax(a.vt * dt / dz_nom, R(0)), R(1));
+  // The nominal sweep leaves the column alone when nothing falls out
+  // (warm_rain.cu:134-136); the metric sweep has no such exit.
+  const bool nominal = a.fallout == 1 && frac_nom > R(0), metric = a.fallout == 2;
   const R rdz_nom = R(1) / dz_nom;
   const MicroCfg<R> mc{dt, a.relax, a.thr, a.krate, a.erate, a.kacc, a.heat};
   // The loads of the level three below are issued before the current
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
   const size_t n = static_cast<size_t>(a.fx) * a.fy * a.fz;
-  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300;
+  double sum = 0.0, comp = 0.0, mn = 1e300, mx = -1e300, bad = 0.0;
   for (size_t e = static_cast<size_t>(blockIdx.x) * 256 + t; e < n;
        e += static_cast<size_t>(gridDim.x) * 256) {
     const int i = static_cast<int>(e % a.fx);
@@ -444,6 +446,13 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
       v = full / fmax(r, 1.0e-12);
       ext = v;
     }
+    // A NaN or infinity is counted, and kept out of the sums and extremes,
+    // which then describe the finite values; the count tells the caller
+    // whether those describe the whole field.
+    if (!isfinite(full) || !isfinite(v)) {
+      bad += 1.0;
+      continue;
+    }
     const double s2 = sum + v;
     comp += fabs(sum) >= fabs(v) ? (sum - s2) + v : (v - s2) + sum;
     sum = s2;
@@ -453,7 +462,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
   // Block combine in a fixed tree order; every node is a TwoSum whose
   // rounding error joins the compensation, so sum + comp stays exact to
   // working precision.
-  double* out = a.partials + static_cast<size_t>(blockIdx.x) * 4;
+  double* out = a.partials + static_cast<size_t>(blockIdx.x) * kReduceParts;
   __shared__ double se[256];
   sh[t] = sum;
   se[t] = comp;
@@ -486,11 +495,20 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
     if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
     __syncthreads();
   }
+  const double bmx = sh[0];
+  __syncthreads();
+  sh[t] = bad;  // counts below 2^53 add exactly in any order
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
   if (t == 0) {
     out[0] = bsum;
     out[1] = bcomp;
     out[2] = bmn;
-    out[3] = sh[0];
+    out[3] = bmx;
+    out[4] = sh[0];
   }
 }
 

@@ -71,6 +71,10 @@ struct WarmRainArgs {
   Base<R> base;  // perturbation form
 };
 
+// Per-block partials of k_reduce: sum, compensation, min, max and the count
+// of non-finite values.
+constexpr int kReduceParts = 5;
+
 template <class R>
 struct ReduceArgs {
   const R* f;
@@ -135,6 +139,38 @@ void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* re
                      double width);
 template <class R>
 void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks);
+
+// Plan-view products (gwm-next-plan-view/v1, src/io/plan_view_output.cpp),
+// in the reference's field order: 10 dry fields, 3 with specific humidity,
+// 8 more with cloud and rain water, 2 more with the surface bucket.
+enum PlanViewField {
+  kPvTerrain = 0, kPvZCenter, kPvRho, kPvTheta, kPvPressure, kPvTemperature, kPvU, kPvV,
+  kPvWind, kPvW,
+  kPvQ, kPvRh, kPvDewpoint,
+  kPvQc, kPvQr, kPvCondensate, kPvColumnCloud, kPvColumnRain, kPvColumnCondensate,
+  kPvRainFraction, kPvReflectivity,
+  kPvPrecip, kPvPrecipRate,
+  kPvCount
+};
+
+template <class R>
+struct PlanViewArgs {
+  Geo g;
+  const R *rho, *rt, *u, *v, *w;  // state (perturbation form: departures)
+  const R *qv, *qc, *qr;          // tracer masses; null when absent
+  const R* precip;                // surface bucket (ny, nx); null when absent
+  const R* idz_c;                 // inverse cell depth, metric index
+  const R* lev_idz_c;             // the same per level (flat terrain)
+  const R* zc;                    // cell-centre heights, metric index
+  const R* terrain;               // terrain height (ny, nx)
+  Base<R> base;
+  int k;                  // slice level
+  double elapsed_hours;   // max(steps * dt / 3600, 1e-9)
+  double* out;            // [kPvCount][ny][nx]
+};
+
+template <class R>
+void launch_plan_view(hipStream_t st, const PlanViewArgs<R>& a, bool terrain);
 template <class R>
 void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const double* zc,
                     const double* wif, const double* wc, const float* terrain, R* zif_m, R* zc_m,

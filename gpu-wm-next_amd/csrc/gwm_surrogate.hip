@@ -374,7 +374,9 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 
 // Warm-rain source terms with the surrogate (fallout runs separately):
 // cells whose features all lie in the box use the network's tendencies,
-// the others the plain scheme.  Counts the fallback cells.
+// the others the plain scheme (counted as fallback cells); cells without
+// condensate at or below saturation keep their state, the plain scheme's
+// exact answer there.
 __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
                                                        float* qv, float* qc, float* qr,
                                                        MicroCfg<float> mc, const float* w,
@@ -417,18 +419,37 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
       sfeat[wv][lane][q] = qdiv(f[q] - nm.xm[q], nm.xs[q]);
     }
+    // Without cloud or rain and at or below saturation every source term
+    // of the plain scheme is zero (no condensation, nothing to evaporate,
+    // convert or accrete): such a cell takes the plain scheme's answer, its
+    // own state written back, instead of the network's residual.  The
+    // network still runs for the whole tile, so the cost does not change.
+    const bool quiet = valid && m.qc <= 0.0f && m.qr <= 0.0f && m.qv <= q_qsat(m.T, m.p);
     sfeat[wv][lane][7] = 0.0f;
     __syncthreads();
     tile_mlp(R, sb, sfeat[wv], sout[wv], lane);
     __syncthreads();
     if (valid) {
       const float th0 = m.th;
-      if (inbox) {
+      if (quiet) {
+        if (!inbox) ++mine;
+      } else if (inbox) {
         const float* o = sout[wv][lane];
-        m.qv = fmaxf(m.qv + (o[0] * nm.ys[0] + nm.ym[0]), 0.0f);
-        m.qc = fmaxf(m.qc + (o[1] * nm.ys[1] + nm.ym[1]), 0.0f);
-        m.qr = fmaxf(m.qr + (o[2] * nm.ys[2] + nm.ym[2]), 0.0f);
-        m.th = m.th + (o[3] * nm.ys[3] + nm.ym[3]);
+        // The source terms only exchange water between vapour, cloud and
+        // rain.  Cloud and rain cannot lose more than they hold; vapour, the
+        // reservoir, takes the opposite of their change (the network's own
+        // vapour change is not applied), so total water is conserved to
+        // rounding; in the rare cell where that would exhaust the vapour,
+        // the exchange and dtheta are scaled back so that it just does.
+        const float dc = fmaxf(o[1] * nm.ys[1] + nm.ym[1], -m.qc);
+        const float dr = fmaxf(o[2] * nm.ys[2] + nm.ym[2], -m.qr);
+        const float dth = o[3] * nm.ys[3] + nm.ym[3];
+        const float dv = -(dc + dr);
+        const float lam = m.qv + dv < 0.0f ? fmaxf(qdiv(m.qv, -dv), 0.0f) : 1.0f;
+        m.qv = fmaxf(m.qv + lam * dv, 0.0f);
+        m.qc = fmaxf(m.qc + lam * dc, 0.0f);
+        m.qr = fmaxf(m.qr + lam * dr, 0.0f);
+        m.th = m.th + lam * dth;
       } else {
         micro_apply(m, mc);
         ++mine;
@@ -457,6 +478,9 @@ __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
   if (i >= g.nx || j >= g.ny) return;
   const float dz_nom = fmaxf(depth, 1.0e-3f);
   const float frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
+  // nothing falls out: the nominal sweep leaves the column alone
+  // (warm_rain.cu:134-136)
+  if (fallout == 1 && !(frac_nom > 0.0f)) return;
   const float rdz_nom = 1.0f / dz_nom;
   float inc = 0.0f;
   // unrolled so that the independent loads of several levels are in flight

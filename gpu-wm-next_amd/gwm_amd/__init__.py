@@ -27,13 +27,23 @@ TEND_RHO, TEND_RHO_THETA, TEND_U, TEND_V, TEND_W = 6, 7, 8, 9, 10
 PRECIP, TRACER0 = 11, 16
 BOUNDARY_NONE, BOUNDARY_STRIPS, BOUNDARY_RELAX = 0, 1, 2
 
+PLAN_VIEW_NAMES = [
+    "terrain_height", "z_center", "rho_d", "theta_m", "air_pressure", "air_temperature",
+    "u_velocity", "v_velocity", "wind_speed", "w_velocity", "specific_humidity",
+    "relative_humidity", "dewpoint", "cloud_water_mixing_ratio", "rain_water_mixing_ratio",
+    "total_condensate", "column_cloud_water", "column_rain_water", "column_total_condensate",
+    "column_rain_fraction", "synthetic_reflectivity", "accumulated_surface_precipitation",
+    "mean_surface_precipitation_rate",
+]
+PLAN_VIEW_FIELDS = len(PLAN_VIEW_NAMES)
+
 SUMMARY_KEYS = [
     "total_dry_mass", "total_rho_theta_m", "total_horizontal_momentum_x",
     "total_horizontal_momentum_y", "total_vertical_momentum", "rho_min", "rho_max",
     "theta_min", "theta_max", "u_min", "u_max", "v_min", "v_max", "w_min", "w_max",
     "vapor_path", "cloud_path", "rain_path", "precip_sum", "precip_mean", "precip_max",
     "precip_cells", "precipitating_cells", "qv_min", "qv_max", "qc_min", "qc_max",
-    "qr_min", "qr_max",
+    "qr_min", "qr_max", "nonfinite_values",
 ]
 
 
@@ -177,6 +187,11 @@ def lib():
         L.gwm_moist_step.argtypes = [vp, ctypes.POINTER(StepConfig),
                                      ctypes.POINTER(WarmRainConfig), ctypes.POINTER(BoundaryConfig)]
         L.gwm_budget_sums.argtypes = [vp, vp]
+        L.gwm_state_set_boundary.argtypes = [vp, ip, vp]
+        L.gwm_state_checkpoint_write.argtypes = [vp, cp, cp]
+        L.gwm_state_checkpoint_read.argtypes = [vp, cp]
+        L.gwm_state_set_tracer_strip_mask.argtypes = [vp, ctypes.c_uint]
+        L.gwm_plan_view.argtypes = [vp, ip, ip, dp, ip, vp, ctypes.POINTER(ctypes.c_int)]
         L.gwm_init_constant.argtypes = [vp, dp, dp, dp, dp, dp]
         L.gwm_init_hydrostatic_rest.argtypes = [vp, dp, dp]
         L.gwm_init_bubble.argtypes = [vp, vp, ip]
@@ -393,6 +408,27 @@ class State:
         s[:, h:s.shape[1] - h, h:s.shape[2] - h] = arr
         self.set_storage(field, s)
 
+    def set_boundary(self, field, arr):
+        """Owned values of one field of the lateral-boundary reference
+        (gwm_state_set_boundary; capture_boundary first)."""
+        s = np.zeros(self.storage_shape(field), self.ctx.dtype)
+        h = self.grid.halo
+        s[:, h:s.shape[1] - h, h:s.shape[2] - h] = arr
+        check(lib().gwm_state_set_boundary(self.h, field, _ptr(np.ascontiguousarray(s))))
+
+    def checkpoint_write(self, path, metadata=None):
+        """Restart dump (gwm_state_checkpoint_write) and its manifest
+        path + '.json'; metadata: any JSON-serialisable value."""
+        import json
+        meta = None if metadata is None else json.dumps(metadata).encode()
+        check(lib().gwm_state_checkpoint_write(self.h, os.fsencode(path), meta))
+
+    def checkpoint_read(self, path):
+        check(lib().gwm_state_checkpoint_read(self.h, os.fsencode(path)))
+
+    def set_tracer_strip_mask(self, mask):
+        check(lib().gwm_state_set_tracer_strip_mask(self.h, int(mask)))
+
     def fill(self, field, value):
         check(lib().gwm_state_fill(self.h, field, float(value)))
 
@@ -474,6 +510,15 @@ class State:
         b = None if boundary is None else ctypes.byref(boundary)
         check(lib().gwm_moist_step_surrogate(self.h, ctypes.byref(cfg), ctypes.byref(wr),
                                              surrogate.h, b))
+
+    def plan_view(self, slice_k, steps, dt, with_precip=True):
+        """Plan-view products of gwm_plan_view: (field count, [23][ny][nx]
+        array in the order of PLAN_VIEW_NAMES; planes past the count are 0)."""
+        out = np.zeros((PLAN_VIEW_FIELDS, self.grid.ny, self.grid.nx), np.float64)
+        nf = ctypes.c_int(0)
+        check(lib().gwm_plan_view(self.h, int(slice_k), int(steps), float(dt), int(with_precip),
+                                  _ptr(out), ctypes.byref(nf)))
+        return nf.value, out
 
     def summary(self):
         out = np.zeros(32, np.float64)

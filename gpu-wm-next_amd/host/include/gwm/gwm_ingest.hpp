@@ -1116,9 +1116,26 @@ class Source {
         dims_{analysis_.grid.nx, analysis_.grid.ny, analysis_.grid.nz} {}
   bool empty() const { return cache_.empty(); }
   // Brings the balanced fields to forecast second round(step_start + t).
+  // The fields are a function of the second alone, so the last few seconds
+  // are kept: the RK stages visit t0, t0 + dt, t0 + dt/2 and the next step
+  // starts at t0 + dt again, which the reference rebalances each time.
   void at(real step_start, real t) {
     const long second = std::lround(step_start + t);
     if (have_ && second == second_) return;
+    for (auto it = recent_.begin(); it != recent_.end(); ++it)
+      if (it->second == second) {
+        std::swap(fields_, it->fields);
+        std::swap(snapshot_, it->snapshot);
+        it->second = second_;  // the fields swapped out are the previous second's
+        if (!have_) recent_.erase(it);
+        second_ = second;
+        have_ = true;
+        return;
+      }
+    if (have_) {
+      recent_.push_back(Entry{second_, std::move(fields_), std::move(snapshot_)});
+      if (recent_.size() > kRecent) recent_.erase(recent_.begin());
+    }
     snapshot_ = interpolate_boundary_snapshot(cache_, static_cast<int>(second));
     fields_ = balance::build(balance::with_snapshot(analysis_, snapshot_), metrics_);
     second_ = second;
@@ -1127,8 +1144,15 @@ class Source {
   const balance::Fields& fields() const { return fields_; }
   const BoundarySnapshotIR& snapshot() const { return snapshot_; }
   const balance::Dims& dims() const { return dims_; }
+  long second() const { return second_; }
 
  private:
+  struct Entry {
+    long second;
+    balance::Fields fields;
+    BoundarySnapshotIR snapshot;
+  };
+  static constexpr std::size_t kRecent = 3;
   AnalysisStateIR analysis_;
   BoundaryCacheIR cache_;
   domain::GridMetrics metrics_;
@@ -1137,6 +1161,7 @@ class Source {
   BoundarySnapshotIR snapshot_;
   long second_ = 0;
   bool have_ = false;
+  std::vector<Entry> recent_;
 };
 
 }  // namespace strips
@@ -1159,12 +1184,15 @@ class PreparedCaseBoundaryUpdater final : public dycore::BoundaryUpdater {
     strips::dry(states, layout, src_.dims(), src_.fields());
   }
   // The balanced fields at stage time t of the current step (for callers
-  // that apply the strips themselves, such as the device stepper).
-  const balance::Fields* fields_at(real sim_time) {
+  // that apply the strips themselves, such as the device run below), and
+  // the forecast second they belong to.
+  const balance::Fields* fields_at(real sim_time, long* second = nullptr) {
     if (src_.empty()) return nullptr;
     src_.at(step_start_, sim_time);
+    if (second) *second = src_.second();
     return &src_.fields();
   }
+  const balance::Dims& dims() const { return src_.dims(); }
 
  private:
   strips::Source src_;
@@ -1186,10 +1214,191 @@ class PreparedCaseTracerBoundaryUpdater final : public dycore::TracerBoundaryUpd
     src_.at(step_start_, sim_time);
     strips::tracers(states, layout, src_.dims(), src_.snapshot(), src_.fields().rho);
   }
+  // The snapshot and balanced density at stage time t (device run below).
+  bool at(real sim_time, long* second) {
+    if (src_.empty()) return false;
+    src_.at(step_start_, sim_time);
+    *second = src_.second();
+    return true;
+  }
+  const BoundarySnapshotIR& snapshot() const { return src_.snapshot(); }
+  const std::vector<real>& balanced_rho() const { return src_.fields().rho; }
+  const balance::Dims& dims() const { return src_.dims(); }
 
  private:
   strips::Source src_;
   real step_start_ = 0.0f;
+};
+
+// ---------------------------------------------------------------------------
+// The prepared run on the device.  States, tracers and the surface bucket
+// are uploaded once into one device domain and stay there between steps;
+// each RK stage's lateral strips are applied on the device (STRIPS against
+// the boundary reference) after a host callback that refreshes the
+// reference only when the stage's forecast second changes: the balanced
+// fields of that second are host work, as in the reference, but the stage
+// state never leaves the device (the host plug-in path downloads and
+// re-uploads it at every stage).  Results equal the host updaters', which
+// overwrite the same strips with the same values.
+// ---------------------------------------------------------------------------
+class PreparedCaseDeviceRun {
+ public:
+  PreparedCaseDeviceRun(const domain::GridMetrics& metrics,
+                        const std::vector<domain::SubdomainDescriptor>& layout,
+                        const std::vector<dycore::DryState>& states,
+                        const std::vector<state::TracerState>& tracers,
+                        const std::vector<physics::WarmRainSurfaceAccumulation>& acc,
+                        PreparedCaseBoundaryUpdater* dry_bc, PreparedCaseTracerBoundaryUpdater* tr_bc)
+      : layout_(layout), G_(gwm::detail::global_of(layout, metrics)),
+        ntr_(tracers.empty() ? 0 : static_cast<int>(tracers.front().size())),
+        lease_(metrics.device.get(), ntr_), dry_bc_(dry_bc), tr_bc_(tr_bc) {
+    gwm_state* s = lease_;
+    dycore::detail::upload_dry(s, states, layout, G_);
+    if (ntr_ > 0) dycore::detail::upload_tracers(s, tracers, layout, G_);
+    std::vector<real> bucket(static_cast<std::size_t>(G_.nx) * G_.ny, 0.0f);
+    for (std::size_t n = 0; n < layout.size() && n < acc.size(); ++n)
+      for (int j = 0; j < layout[n].ny_local(); ++j)
+        for (int i = 0; i < layout[n].nx_local(); ++i)
+          bucket[static_cast<std::size_t>(layout[n].j_begin + j) * G_.nx + layout[n].i_begin + i] =
+              acc[n].liquid_precipitation_kg_m2(i, j, 0);
+    gwm::detail::check(gwm_state_upload(s, GWM_PRECIP, bucket.data()));
+    if (dry_bc_ || tr_bc_) gwm::detail::check(gwm_state_capture_boundary(s));
+    if (tr_bc_ && !tracers.empty()) names_ = tracers.front().registry();
+  }
+  PreparedCaseDeviceRun(const PreparedCaseDeviceRun&) = delete;
+  PreparedCaseDeviceRun& operator=(const PreparedCaseDeviceRun&) = delete;
+
+  // advance_dry_state_ssprk3 with the prepared boundary; fast == false runs
+  // the step without fast modes (a null FastModeIntegrator).
+  void dry_step(const dycore::DryStepperConfig& config, bool fast) {
+    const auto c = dycore::detail::cfg_of(config);
+    gwm_boundary_config b{dry_bc_ ? GWM_BOUNDARY_STRIPS : GWM_BOUNDARY_NONE, 0.0,
+                          dry_bc_ ? &PreparedCaseDeviceRun::dry_refresh : nullptr, this};
+    if (fast)
+      gwm::detail::check(gwm_ssprk3_step(lease_, &c, &b));
+    else
+      gwm::detail::check(gwm_ssprk3_step_custom_fast(lease_, &c, &b, &PreparedCaseDeviceRun::no_fast,
+                                                     nullptr));
+    rethrow();
+  }
+  // advance_passive_tracers_ssprk3 with the prepared tracer boundary.
+  void tracer_step(const dycore::DryStepperConfig& config) {
+    if (ntr_ == 0) return;
+    const auto c = dycore::detail::cfg_of(config);
+    if (tr_bc_)
+      gwm::detail::check(gwm_tracers_ssprk3_callback(lease_, &c, GWM_BOUNDARY_STRIPS,
+                                                     &PreparedCaseDeviceRun::tracer_refresh, this));
+    else
+      gwm::detail::check(gwm_tracers_ssprk3(lease_, &c, GWM_BOUNDARY_NONE));
+    rethrow();
+  }
+  // apply_warm_rain_microphysics (layout/metric overload).
+  void warm_rain(const physics::WarmRainConfig& config) {
+    const auto c = physics::detail::cfg_of(config);
+    gwm::detail::check(gwm_warm_rain(lease_, &c, 1));
+  }
+  // The device state back into the host states, tracers and buckets.
+  void download(std::vector<dycore::DryState>& states, std::vector<state::TracerState>& tracers,
+                std::vector<physics::WarmRainSurfaceAccumulation>& acc) {
+    gwm_state* s = lease_;
+    dycore::detail::download_dry(s, states, layout_, G_);
+    if (ntr_ > 0) dycore::detail::download_tracers(s, tracers, layout_, G_);
+    std::vector<real> bucket(static_cast<std::size_t>(G_.nx) * G_.ny);
+    gwm::detail::check(gwm_state_download(s, GWM_PRECIP, bucket.data()));
+    for (std::size_t n = 0; n < layout_.size() && n < acc.size(); ++n)
+      for (int j = 0; j < layout_[n].ny_local(); ++j)
+        for (int i = 0; i < layout_[n].nx_local(); ++i)
+          acc[n].liquid_precipitation_kg_m2(i, j, 0) =
+              bucket[static_cast<std::size_t>(layout_[n].j_begin + j) * G_.nx + layout_[n].i_begin + i];
+  }
+  // Boundary-reference refreshes so far (host balance work happened at
+  // most this often; the stage strips themselves run on the device).
+  int dry_refreshes() const { return dry_uploads_; }
+  int tracer_refreshes() const { return tracer_uploads_; }
+
+ private:
+  // Global haloless field -> the device storage layout of field f.
+  void to_storage(const std::vector<real>& src, int ox, int oy, int oz, std::vector<real>& out) const {
+    out.assign(G_.size(ox, oy, oz), 0.0f);
+    const int fx = G_.nx + ox, fy = G_.ny + oy, fz = G_.nz + oz;
+    for (int k = 0; k < fz; ++k)
+      for (int j = 0; j < fy; ++j)
+        for (int i = 0; i < fx; ++i)
+          out[G_.index(ox, oy, i, j, k)] = src[(static_cast<std::size_t>(k) * fy + j) * fx + i];
+  }
+  static int no_fast(gwm_state*, double, void*) { return GWM_OK; }
+  static int dry_refresh(gwm_state* stage, double t, void* user) {
+    auto* self = static_cast<PreparedCaseDeviceRun*>(user);
+    try {
+      long second = 0;
+      const balance::Fields* f = self->dry_bc_->fields_at(static_cast<real>(t), &second);
+      if (f == nullptr || (self->have_dry_ && second == self->dry_second_)) return GWM_OK;
+      const std::vector<real>* src[5] = {&f->rho, &f->rho_theta, &f->mom_u, &f->mom_v, &f->mom_w};
+      std::vector<real> buf;
+      for (int q = 0; q < 5; ++q) {
+        const auto* o = gwm::detail::kOff[q];
+        self->to_storage(*src[q], o[0], o[1], o[2], buf);
+        gwm::detail::check(gwm_state_set_boundary(stage, q, buf.data()));
+      }
+      self->dry_second_ = second;
+      self->have_dry_ = true;
+      ++self->dry_uploads_;
+      return GWM_OK;
+    } catch (...) {
+      self->error_ = std::current_exception();
+      return GWM_E_INVALID;
+    }
+  }
+  static int tracer_refresh(gwm_state* stage, double t, void* user) {
+    auto* self = static_cast<PreparedCaseDeviceRun*>(user);
+    try {
+      long second = 0;
+      unsigned mask = 0;
+      if (!self->tr_bc_->at(static_cast<real>(t), &second)) {
+        gwm::detail::check(gwm_state_set_tracer_strip_mask(stage, 0u));
+        return GWM_OK;
+      }
+      if (self->have_tr_ && second == self->tr_second_) return GWM_OK;
+      const auto& snap = self->tr_bc_->snapshot();
+      const auto& rho = self->tr_bc_->balanced_rho();
+      std::vector<real> mass(rho.size()), buf;
+      for (int q = 0; q < self->ntr_; ++q) {
+        const auto it = snap.atmosphere.values.find(self->names_.at(q).name);
+        if (it == snap.atmosphere.values.end()) continue;
+        for (std::size_t c = 0; c < rho.size(); ++c) mass[c] = rho[c] * it->second[c];
+        self->to_storage(mass, 0, 0, 0, buf);
+        gwm::detail::check(gwm_state_set_boundary(stage, GWM_TRACER0 + q, buf.data()));
+        if (q < 32) mask |= 1u << q;
+      }
+      gwm::detail::check(gwm_state_set_tracer_strip_mask(stage, mask));
+      self->tr_second_ = second;
+      self->have_tr_ = true;
+      ++self->tracer_uploads_;
+      return GWM_OK;
+    } catch (...) {
+      self->error_ = std::current_exception();
+      return GWM_E_INVALID;
+    }
+  }
+  void rethrow() {
+    if (error_) {
+      auto e = error_;
+      error_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+  std::vector<domain::SubdomainDescriptor> layout_;
+  gwm::detail::Global G_;
+  int ntr_;
+  gwm::detail::StateLease lease_;
+  PreparedCaseBoundaryUpdater* dry_bc_;
+  PreparedCaseTracerBoundaryUpdater* tr_bc_;
+  state::TracerRegistry names_;
+  long dry_second_ = 0, tr_second_ = 0;
+  bool have_dry_ = false, have_tr_ = false;
+  int dry_uploads_ = 0, tracer_uploads_ = 0;
+  std::exception_ptr error_;
 };
 
 }  // namespace gwm::ingest

@@ -9,6 +9,9 @@
 // copy of the fields.  The virtual-rank split (--ranks-x/--ranks-y) is tiling
 // only and the device holds the whole domain.
 //
+// --plan-view-json writes the reference's plan-view bundle, computed on the
+// device (gwm_plan_view) at --plan-view-level.
+//
 // Extensions: --precision fp32|fp64, --device N, --stretched-levels
 // DZ_BOTTOM,DZ_TOP (flat interfaces stretched to --z-top).
 #include <cstdio>
@@ -21,6 +24,7 @@
 #include <vector>
 
 #include "gwm_hip.h"
+#include "gwm/gwm_io.hpp"
 
 namespace {
 
@@ -159,8 +163,6 @@ int run(const Options& o) {
     throw std::runtime_error("RectilinearDomainConfig spacing/top must be positive");
   if (o.ranks_x <= 0 || o.ranks_y <= 0)
     throw std::runtime_error("RectilinearDomainConfig rank counts must be positive");
-  if (!o.plan_view_json_path.empty())
-    throw std::runtime_error("--plan-view-json: plan-view output is not provided by this driver");
 
   gwm_ctx* ctx = nullptr;
   check(gwm_ctx_create(o.device, o.precision, &ctx));
@@ -218,6 +220,13 @@ int run(const Options& o) {
       std::ofstream out(o.summary_json_path, std::ios::binary);
       if (!out) throw std::runtime_error("Failed to open summary path: " + o.summary_json_path);
       out << js.str() << "\n";
+    }
+    if (!o.plan_view_json_path.empty()) {
+      // extract_dry_plan_view (gwm_idealized_driver.cpp:265-270)
+      const auto pv = gwm::io::detail::from_device(st, o.nx, o.ny, o.nz, o.dx, o.dy, o.case_kind,
+                                                   o.steps, static_cast<gwm::real>(o.dt),
+                                                   o.plan_view_level, false);
+      gwm::io::write_plan_view_bundle_json(pv, o.plan_view_json_path);
     }
   } catch (...) {
     if (st) gwm_state_destroy(st);

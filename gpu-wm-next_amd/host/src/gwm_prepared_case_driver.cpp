@@ -5,12 +5,14 @@
 // steps of {dry SSPRK3, tracer SSPRK3, warm rain} with the prepared lateral
 // strips, printing the summary JSON with the reference's keys.
 //
-// The start-up and the boundary snapshots are host work (gwm_ingest.hpp);
-// the step runs on the device through the host mirror's operators, with the
-// boundary updaters called on each RK stage as in the reference.
+// The start-up and the balanced boundary fields of each forecast second are
+// host work (gwm_ingest.hpp), as in the reference; the steps run on one
+// device domain that stays on the device for the whole run, and each RK
+// stage's lateral strips are applied there from a boundary reference the
+// updaters refresh when the forecast second changes (PreparedCaseDeviceRun).
 // GWM_TRACE_DRIVER and GWM_PHASE_SUMMARY_DIR / _STEP / _STEP_MIN / _STEP_MAX
-// behave as in the reference.  Plan-view output is not part of this build:
-// --plan-view-json is rejected.
+// behave as in the reference, and so does --plan-view-json (the products
+// computed on the device at --plan-view-level).
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
@@ -21,7 +23,7 @@
 #include <stdexcept>
 #include <string>
 
-#include "gwm/gwm_ingest.hpp"
+#include "gwm/gwm_io.hpp"
 
 namespace {
 
@@ -43,12 +45,6 @@ struct Options {
   int plan_view_level = 0;
 };
 
-class NullFastModeIntegrator final : public gwm::dycore::FastModeIntegrator {
- public:
-  void apply(std::vector<gwm::dycore::DryState>&, const std::vector<gwm::domain::SubdomainDescriptor>&,
-             const gwm::domain::GridMetrics&, const gwm::dycore::DryStepperConfig&) override {}
-};
-
 bool env_flag(const char* name) {
   const char* v = std::getenv(name);
   return v != nullptr && std::string(v) != "0" && std::string(v) != "false";
@@ -68,6 +64,11 @@ bool phase_step_selected(int step) {
   if (const auto s = env_int("GWM_PHASE_SUMMARY_STEP")) return step == *s;
   const auto lo = env_int("GWM_PHASE_SUMMARY_STEP_MIN"), hi = env_int("GWM_PHASE_SUMMARY_STEP_MAX");
   return !(lo && step < *lo) && !(hi && step > *hi);
+}
+
+bool phase_wanted(int step) {
+  const char* dir = std::getenv("GWM_PHASE_SUMMARY_DIR");
+  return dir != nullptr && *dir != '\0' && phase_step_selected(step);
 }
 
 void phase_summary(int step, real t0, const std::string& phase,
@@ -141,8 +142,6 @@ void parse(int argc, char** argv, Options& o) {
   if (o.analysis_state_path.empty()) throw std::runtime_error("--analysis-state is required");
   if (o.boundary_cache_path.empty()) throw std::runtime_error("--boundary-cache is required");
   if (o.steps < 0) throw std::runtime_error("--steps must be nonnegative");
-  if (!o.plan_view_json_path.empty())
-    throw std::runtime_error("--plan-view-json: plan-view output is not part of this build");
 }
 
 // The surface runtime the reference builds and then discards: this build
@@ -260,20 +259,23 @@ int main(int argc, char** argv) {
     trace("initial_summary_done");
 
     ing::PreparedCaseBoundaryUpdater dry_bc(rc.analysis, rc.boundary_cache, o.init);
-    gwm::dycore::NullBoundaryUpdater null_bc;
-    gwm::dycore::BoundaryUpdater& bc =
-        o.enable_boundaries ? static_cast<gwm::dycore::BoundaryUpdater&>(dry_bc) : null_bc;
     ing::PreparedCaseTracerBoundaryUpdater tr_bc(rc.analysis, rc.boundary_cache, o.init);
-    gwm::dycore::NullTracerBoundaryUpdater null_tr_bc;
-    gwm::dycore::TracerBoundaryUpdater& tbc =
-        o.enable_boundaries ? static_cast<gwm::dycore::TracerBoundaryUpdater&>(tr_bc) : null_tr_bc;
-    gwm::dycore::LocalSplitExplicitFastMode local_fast;
-    NullFastModeIntegrator null_fast;
-    gwm::dycore::FastModeIntegrator& fast =
-        o.enable_fast_modes ? static_cast<gwm::dycore::FastModeIntegrator&>(local_fast) : null_fast;
     auto wr = o.warm_rain;
     wr.dt = o.step.dt;
 
+    // The step loop of the reference (gwm_prepared_case_driver.cpp:427-463)
+    // on one device domain: the state stays on the device from the first
+    // step to the last, the boundary updaters refresh the device boundary
+    // reference, and host copies are made only for phase summaries.
+    ing::PreparedCaseDeviceRun run(metrics, layout, states, tracers, acc,
+                                   o.enable_boundaries ? &dry_bc : nullptr,
+                                   o.enable_boundaries ? &tr_bc : nullptr);
+    trace("device_state_uploaded");
+    auto phase = [&](int step, real t0, const char* name) {
+      if (!phase_wanted(step)) return;
+      run.download(states, tracers, acc);
+      phase_summary(step, t0, name, states, tracers, layout, metrics, acc);
+    };
     real sim_time = 0.0f;
     for (int step = 0; step < o.steps; ++step) {
       const std::string tag = "step_" + std::to_string(step);
@@ -285,25 +287,26 @@ int main(int argc, char** argv) {
       } else {
         trace(tag + "_boundaries_disabled");
       }
-      gwm::dycore::advance_dry_state_ssprk3(states, layout, metrics, o.step, bc, fast);
+      run.dry_step(o.step, o.enable_fast_modes);
       trace(tag + "_dry_done");
-      phase_summary(step, sim_time, "dry", states, tracers, layout, metrics, acc);
+      phase(step, sim_time, "dry");
       if (o.enable_tracer_transport) {
-        gwm::dycore::advance_passive_tracers_ssprk3(tracers, states, layout, metrics, o.step, tbc);
+        run.tracer_step(o.step);
         trace(tag + "_tracers_done");
-        phase_summary(step, sim_time, "tracers", states, tracers, layout, metrics, acc);
+        phase(step, sim_time, "tracers");
       } else {
         trace(tag + "_tracers_skipped");
       }
       if (o.enable_warm_rain) {
-        gwm::physics::apply_warm_rain_microphysics(states, tracers, layout, metrics, wr, &acc);
+        run.warm_rain(wr);
         trace(tag + "_warm_rain_done");
-        phase_summary(step, sim_time, "warm_rain", states, tracers, layout, metrics, acc);
+        phase(step, sim_time, "warm_rain");
       } else {
         trace(tag + "_warm_rain_skipped");
       }
       sim_time += o.step.dt;
     }
+    run.download(states, tracers, acc);
     trace("post_step_sync_done");
     const auto final_ = gwm::core::summarize_runtime_state(states, tracers, layout, metrics, &acc);
     trace("final_summary_done");
@@ -313,6 +316,12 @@ int main(int argc, char** argv) {
       std::ofstream out(o.summary_json_path, std::ios::binary);
       if (!out) throw std::runtime_error("Failed to open summary path: " + o.summary_json_path);
       out << json << "\n";
+    }
+    if (!o.plan_view_json_path.empty()) {
+      const auto pv = gwm::io::extract_runtime_plan_view(states, tracers, layout, metrics,
+                                                         "prepared_case", o.steps, o.step.dt,
+                                                         o.plan_view_level, &acc);
+      gwm::io::write_plan_view_bundle_json(pv, o.plan_view_json_path);
     }
     return EXIT_SUCCESS;
   } catch (const std::exception& ex) {

@@ -121,7 +121,8 @@ typedef int (*gwm_boundary_fn)(gwm_state* stage, double sim_time, void* user);
 typedef struct {
   int mode;             /* GWM_BOUNDARY_* */
   double relax_width;   /* cells, RELAX only */
-  gwm_boundary_fn callback; /* optional, runs after `mode` */
+  gwm_boundary_fn callback; /* optional, runs before `mode` (it may refresh the
+                               boundary reference with gwm_state_set_boundary) */
   void* user;
 } gwm_boundary_config;
 
@@ -154,6 +155,10 @@ enum {
   GWM_S_PRECIP_CELLS = 21,
   GWM_S_PRECIPITATING_CELLS = 22,
   GWM_S_TRACER_QMIN0 = 23, /* min/max mixing ratio of tracers 0..2 at 23..28 */
+  /* Count of NaN / infinite values over rho, rho*theta, the momenta, the
+   * tracers and the precipitation bucket.  Sums and extremes cover the finite
+   * values; the extremes of a field holding any non-finite value are NaN. */
+  GWM_S_NONFINITE = 29,
   GWM_SUMMARY_N = 32
 };
 
@@ -252,6 +257,30 @@ int gwm_state_fill(gwm_state* state, int field, double value);
 int gwm_state_copy(gwm_state* dst, const gwm_state* src);
 /* Keep a copy of the current state as the lateral-boundary reference. */
 int gwm_state_capture_boundary(gwm_state* state);
+/* Replace one field of the lateral-boundary reference (captured before) with
+ * host values in the field's storage layout, as gwm_state_upload takes them;
+ * on a stage view it writes the owner's reference.  With STRIPS, a boundary
+ * callback that refreshes the reference this way is the device form of the
+ * prepared-case boundary updaters (PreparedCaseBoundaryUpdater::apply,
+ * src/ingest/prepared_case_init.cpp:1032-1047, and the tracer updater,
+ * 1064-1083): the stage never leaves the device. */
+int gwm_state_set_boundary(gwm_state* state, int field, const void* host);
+/* Restart dump.  The reference fixes the artifact contract of a native
+ * checkpoint and leaves the data for later (tools/casebuilder/prepare_case.py:376-392,
+ * "gwm-next-checkpoint-stub/v1", restart family "gwm-native-checkpoint");
+ * this writes that checkpoint: the canonical fields in storage layout, the
+ * tracers, the surface bucket, the perturbation form's FP64 bases and the
+ * captured boundary reference, each with an FNV-1a checksum, plus a manifest
+ * `path`.json that satisfies the stub contract (metadata_json: an optional
+ * JSON value stored in it).  Reading into a state of the same grid,
+ * precision, form and tracer count restores it bit for bit; a mismatch or a
+ * failed checksum is an error that names what differs. */
+int gwm_state_checkpoint_write(gwm_state* state, const char* path, const char* metadata_json);
+int gwm_state_checkpoint_read(gwm_state* state, const char* path);
+/* Tracers the STRIPS boundary overwrites (bit t: tracer t; all by default);
+ * the prepared tracer strips cover the tracers a snapshot carries
+ * (apply_cached_tracer_boundary_strips, prepared_case_init.cpp:567-638). */
+int gwm_state_set_tracer_strip_mask(gwm_state* state, unsigned mask);
 
 /* ---- halo semantics -------------------------------------------------------
  * HaloExchange::exchange_scalar / exchange_face and
@@ -296,7 +325,7 @@ int gwm_tracers_ssprk3(gwm_state* state, const gwm_step_config* cfg, int boundar
 /* advance_passive_tracers_ssprk3 with a host TracerBoundaryUpdater plug-in
  * (include/gwm/dycore/passive_tracer.hpp:13-19): fn runs on a stage view
  * whose tracer fields are the stage input of every tracer, at the
- * step-relative times 0, dt and dt/2 (dry_transport.cu:485-508), after the
+ * step-relative times 0, dt and dt/2 (dry_transport.cu:485-508), before the
  * strips of boundary_mode (GWM_BOUNDARY_NONE or GWM_BOUNDARY_STRIPS).  The
  * tracers are independent, so one call per stage for all tracers stands for
  * the reference's one call per stage and tracer. */
@@ -314,6 +343,22 @@ int gwm_moist_step(gwm_state* state, const gwm_step_config* cfg,
                    const gwm_warm_rain_config* wr, const gwm_boundary_config* boundary);
 /* Diagnostics with compensated FP64 device sums. */
 int gwm_budget_sums(gwm_state* state, double* out);
+/* Plan-view products of the state at level slice_k (clamped to [0, nz-1]):
+ * extract_dry_plan_view / extract_runtime_plan_view
+ * (include/gwm/io/plan_view_output.hpp:40-52, src/io/plan_view_output.cpp:601-839),
+ * computed on the device in one column pass.  out: double[GWM_PLAN_VIEW_FIELDS][ny][nx]
+ * (row-major y, x), fields in the reference's order: terrain_height, z_center,
+ * rho_d, theta_m, air_pressure, air_temperature, u_velocity, v_velocity,
+ * wind_speed, w_velocity; with tracers specific_humidity, relative_humidity,
+ * dewpoint; with cloud and rain water cloud_water_mixing_ratio,
+ * rain_water_mixing_ratio, total_condensate, column_cloud_water,
+ * column_rain_water, column_total_condensate, column_rain_fraction,
+ * synthetic_reflectivity; with with_precip accumulated_surface_precipitation
+ * and mean_surface_precipitation_rate (over steps * dt).  *nfields: 10, 13,
+ * 21 or 23; only those planes are written. */
+#define GWM_PLAN_VIEW_FIELDS 23
+int gwm_plan_view(gwm_state* state, int slice_k, int steps, double dt, int with_precip,
+                  double* out, int* nfields);
 
 /* ---- initial states (host-built, uploaded) ---------------------------------
  * make_constant_dry_state (dry_core.cu:348-361), make_hydrostatic_rest_state

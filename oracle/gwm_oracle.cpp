@@ -493,6 +493,7 @@ struct WorldBase {
   virtual void tracers(const orc_step_cfg& c, int bmode) = 0;
   virtual void warm_rain(const orc_warm_rain_cfg& c, int metric) = 0;
   virtual void summary(double* out) = 0;
+  virtual int plan_view(int slice_k, int steps, double dt, int with_precip, double* out) = 0;
   virtual void init_constant(double rho, double th, double u, double v, double w) = 0;
   virtual void init_hydro(double rho_s, double th) = 0;
   virtual void init_bubble(const double* p, int sign) = 0;
@@ -1436,6 +1437,99 @@ struct World : WorldBase {
     }
   }
 
+  // extract_dry_plan_view / extract_runtime_plan_view
+  // (src/io/plan_view_output.cpp:601-839): derived fields at the slice
+  // level (theta, p, T, cell-centred velocities, wind speed, q, RH,
+  // dewpoint, qc, qr, condensate) and column maps (cloud and rain water
+  // paths, condensate, rain fraction, maximum synthetic reflectivity,
+  // surface bucket and its mean rate), [field][j][i] in the reference's
+  // field order.  Returns the field count: 10, 13, 21 or 23.
+  int plan_view(int slice_k, int steps, double dt, int with_precip, double* out) override {
+    const int nx = M.nx, ny = M.ny;
+    const int k = std::clamp(slice_k, 0, M.nz - 1);
+    const size_t plane = static_cast<size_t>(nx) * ny;
+    const int nf = n_tracers == 0 ? 10 : (n_tracers < 3 ? 13 : (with_precip ? 23 : 21));
+    const R min_rho = R(1.0e-6), min_p = R(1);
+    auto put = [&](int f, int gi, int gj, double v) {
+      out[static_cast<size_t>(f) * plane + static_cast<size_t>(gj) * nx + gi] = v;
+    };
+    // moist_thermo.hpp:38-74
+    auto rh_of = [](R q, R T, R p) {
+      const R qs = qsat(T, p);
+      return std::min(std::max(qs > R(0) ? R(100) * q / qs : R(0), R(0)), R(100));
+    };
+    auto dewpoint_of = [](R q_in, R p) {
+      const R q = std::min(std::max(q_in, R(0)), R(1));
+      const R e = p * q / std::max(K<R>::eps + (R(1) - K<R>::eps) * q, R(1.0e-6));
+      const R e_hpa = std::max(e * R(0.01), R(1.0e-6));
+      const R lt = std::log(e_hpa / R(6.112));
+      return (R(243.5) * lt) / (R(17.67) - lt) + R(273.15);
+    };
+    auto dbz_of = [](R rho, R qr_in) {
+      const R qr = std::max(qr_in, R(0));
+      if (qr <= R(1.0e-8) || rho <= R(1.0e-8)) return R(-20);
+      const R z = R(3.63e9) * std::pow(rho * qr, R(1.75));
+      return R(10) * std::log10(std::max(z, R(1.0e-6)));
+    };
+    const double hours = std::max(static_cast<double>(steps) * dt / 3600.0, 1.0e-9);
+    for (size_t n = 0; n < L.size(); ++n) {
+      const State<R>& st = S[n];
+      const Desc& d = L[n];
+      for (int j = 0; j < d.nyl(); ++j)
+        for (int i = 0; i < d.nxl(); ++i) {
+          const int gi = d.i0 + i, gj = d.j0 + j;
+          const R rho = st.rho(i, j, k), rt = st.rt(i, j, k);
+          const R rr = std::max(rho, min_rho);
+          const R theta = rt / rr;
+          const R p = pressure_of(rho, rt);
+          const R T = theta * std::pow(std::max(p, min_p) / K<R>::p0, K<R>::kappa);
+          const R u = R(0.5) * (st.u(i, j, k) + st.u(i + 1, j, k)) / rr;
+          const R v = R(0.5) * (st.v(i, j, k) + st.v(i, j + 1, k)) / rr;
+          const R w = R(0.5) * (st.w(i, j, k) + st.w(i, j, k + 1)) / rr;
+          put(0, gi, gj, static_cast<double>(M.terrain[static_cast<size_t>(gj) * nx + gi]));
+          put(1, gi, gj, static_cast<double>(M.zc_m[M.mi(gi, gj, k)]));
+          put(2, gi, gj, static_cast<double>(rho));
+          put(3, gi, gj, static_cast<double>(theta));
+          put(4, gi, gj, static_cast<double>(p));
+          put(5, gi, gj, static_cast<double>(T));
+          put(6, gi, gj, static_cast<double>(u));
+          put(7, gi, gj, static_cast<double>(v));
+          put(8, gi, gj, static_cast<double>(std::sqrt(u * u + v * v)));
+          put(9, gi, gj, static_cast<double>(w));
+          if (n_tracers == 0) continue;
+          const R q = Q[n][0](i, j, k) / rr;
+          put(10, gi, gj, static_cast<double>(q));
+          put(11, gi, gj, static_cast<double>(rh_of(q, T, p)));
+          put(12, gi, gj, static_cast<double>(dewpoint_of(q, p)));
+          if (n_tracers < 3) continue;
+          const R qc = Q[n][1](i, j, k) / rr, qr = Q[n][2](i, j, k) / rr;
+          put(13, gi, gj, static_cast<double>(qc));
+          put(14, gi, gj, static_cast<double>(qr));
+          put(15, gi, gj, static_cast<double>(qc + qr));
+          double col_c = 0.0, col_r = 0.0;
+          R dbz = R(-20);
+          for (int kk = 0; kk < d.nz; ++kk) {
+            const double dz = static_cast<double>(R(1) / M.idz_c[M.mi(gi, gj, kk)]);
+            col_c += static_cast<double>(Q[n][1](i, j, kk)) * dz;
+            col_r += static_cast<double>(Q[n][2](i, j, kk)) * dz;
+            const R rk = st.rho(i, j, kk);
+            dbz = std::max(dbz, dbz_of(rk, Q[n][2](i, j, kk) / std::max(rk, min_rho)));
+          }
+          const double total = col_c + col_r, tpos = std::max(total, 0.0);
+          put(16, gi, gj, col_c);
+          put(17, gi, gj, col_r);
+          put(18, gi, gj, total);
+          put(19, gi, gj, tpos <= 0.0 ? 0.0 : std::clamp(col_r / tpos, 0.0, 1.0));
+          put(20, gi, gj, static_cast<double>(dbz));
+          if (!with_precip) continue;
+          const double acc = static_cast<double>(precip[n](i, j, 0));
+          put(21, gi, gj, acc);
+          put(22, gi, gj, acc / hours);
+        }
+    }
+    return nf;
+  }
+
   // summarize_dry_states (dry_diagnostics.cpp:11-96) and
   // summarize_runtime_state (runtime_summary.cpp:27-178).
   void summary(double* o) override {
@@ -1828,6 +1922,12 @@ int orc_init_bubble(void* w, const double* p, int sign) {
 }
 int orc_init_mountain_wave(void* w, double rho_s, double th, double u, double v, double wv) {
   return guard([&] { W(w)->init_mountain(rho_s, th, u, v, wv); });
+}
+
+int orc_plan_view(void* w, int slice_k, int steps, double dt, int with_precip, double* out) {
+  int nf = -1;
+  guard([&] { nf = W(w)->plan_view(slice_k, steps, dt, with_precip, out); });
+  return nf;
 }
 
 double orc_time_steps(void* w, const orc_step_cfg* c, const orc_warm_rain_cfg* wr, int moist,

@@ -176,6 +176,9 @@ int orc_column_reference(int precision, int nz, const double* th, const double* 
 
 /* Timing helper for the CPU baseline: runs n steps of the moist or dry step
  * and returns wall seconds. */
+/* extract_dry_plan_view / extract_runtime_plan_view: [field][j][i] doubles,
+ * returns the field count (10, 13, 21 or 23), -1 on error. */
+int orc_plan_view(void* w, int slice_k, int steps, double dt, int with_precip, double* out);
 double orc_time_steps(void* world, const orc_step_cfg* cfg,
                       const orc_warm_rain_cfg* wr, int moist, int steps,
                       int threads);

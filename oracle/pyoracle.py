@@ -37,6 +37,17 @@ SUMMARY_KEYS = [
 ]
 
 
+PLAN_VIEW_NAMES = [
+    "terrain_height", "z_center", "rho_d", "theta_m", "air_pressure", "air_temperature",
+    "u_velocity", "v_velocity", "wind_speed", "w_velocity", "specific_humidity",
+    "relative_humidity", "dewpoint", "cloud_water_mixing_ratio", "rain_water_mixing_ratio",
+    "total_condensate", "column_cloud_water", "column_rain_water", "column_total_condensate",
+    "column_rain_fraction", "synthetic_reflectivity", "accumulated_surface_precipitation",
+    "mean_surface_precipitation_rate",
+]
+PLAN_VIEW_FIELDS = len(PLAN_VIEW_NAMES)
+
+
 class StepCfg(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("fast_substeps", ctypes.c_int),
                 ("gravity", ctypes.c_double),
@@ -114,6 +125,7 @@ def lib():
         L.orc_thermo.argtypes = [ip, ip, dp, dp, dp]
         L.orc_apply_boundary.argtypes = [vp, ip, dp]
         L.orc_set_tracer_strip_mask.argtypes = [vp, ctypes.c_uint]
+        L.orc_plan_view.argtypes = [vp, ip, ip, dp, ip, vp]
         L.orc_time_steps.restype = dp
         L.orc_time_steps.argtypes = [vp, ctypes.POINTER(StepCfg), ctypes.POINTER(WarmRainCfg),
                                      ip, ip, ip]
@@ -266,6 +278,15 @@ class World:
         out = np.zeros(32, np.float64)
         _check(lib().orc_summary(self.h, _ptr(out)))
         return dict(zip(SUMMARY_KEYS, out.tolist()))
+
+    def plan_view(self, slice_k, steps, dt, with_precip=True):
+        """Plan-view products: (field count, [23][ny][nx] array)."""
+        out = np.zeros((PLAN_VIEW_FIELDS, self.ny, self.nx), np.float64)
+        nf = lib().orc_plan_view(self.h, int(slice_k), int(steps), float(dt), int(with_precip),
+                                 _ptr(out))
+        if nf < 0:
+            raise OracleError(lib().orc_last_error().decode())
+        return nf, out
 
     def time_steps(self, cfg, wr=None, moist=False, steps=1, threads=0):
         wr = wr or warm_rain_cfg()

@@ -10,7 +10,9 @@ the normalised outputs).  Everything here is FP64 numpy.
 
 Pinned by (tests/test_surrogate_oracle.py): central finite differences of
 the loss for the gradient, the GELU closed form and its derivative at known
-points, and BF16 round-to-nearest-even known answers.
+points, BF16 round-to-nearest-even known answers, and for the conservation
+layer known answers (clipping, vapour closure, the vapour-exhaustion
+scaling) plus exact conservation and non-negativity on random inputs.
 """
 import numpy as np
 
@@ -116,6 +118,25 @@ def forward_bf16(p, x_raw, norm):
         z = h @ bf16(W) + np.asarray(b, np.float32).astype(np.float64)
         h = bf16(gelu(z)) if li < 3 else z
     return h * norm["y_std"] + norm["y_mean"]
+
+
+def constrain(q, d, dth):
+    """The surrogate step's conservation layer (gwm_surrogate.hip,
+    k_warm_rain_sur): the changes of cloud and rain are clipped at minus
+    what they hold, vapour takes the opposite of their sum (the network's
+    vapour change is not applied), and where that would exhaust the vapour
+    the exchange and dtheta are scaled by lambda = qv / -dqv.
+    q, d: [n][3] mixing ratios (qv, qc, qr) and raw changes; returns
+    (new q, applied dtheta)."""
+    q = np.asarray(q, np.float64)
+    d = np.asarray(d, np.float64)
+    dc = np.maximum(d[:, 1], -q[:, 1])
+    dr = np.maximum(d[:, 2], -q[:, 2])
+    dv = -(dc + dr)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lam = np.where(q[:, 0] + dv < 0.0, np.maximum(q[:, 0] / -dv, 0.0), 1.0)
+    ch = np.stack([dv, dc, dr], axis=1) * lam[:, None]
+    return np.maximum(q + ch, 0.0), lam * np.asarray(dth, np.float64)
 
 
 def init_xavier(seed):

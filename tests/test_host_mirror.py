@@ -139,3 +139,43 @@ def test_driver_summary_matches_oracle(orc, case, precision):
                     bad.append((part, jk, b, s[jk][b], v))
     assert not bad, bad
     assert np.isfinite(got["final"]["total_dry_mass"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_driver_plan_view_matches_oracle(orc, tmp_path, case):
+    """--plan-view-json: the dry bundle (10 fields) of the final state, against
+    the oracle's products of its own final state (FP64; the JSON carries ten
+    significant digits)."""
+    import gwm_amd as G
+
+    _need(DRIVER)
+    spec = CASES[case]
+    pv = str(tmp_path / "plan_view.json")
+    r = subprocess.run([DRIVER, "--precision", "fp64"] + spec["args"] +
+                       ["--plan-view-json", pv, "--plan-view-level", "3"],
+                       capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    with open(pv) as fh:
+        pj = json.load(fh)
+    assert pj["schema_version"] == "gwm-next-plan-view/v1" and pj["case"] == case
+    assert pj["steps"] == 3 and pj["slice_k"] == 3
+    assert [f["name"] for f in pj["fields"]] == orc.PLAN_VIEW_NAMES[:10]
+    terrain = None
+    if "terrain" in spec:
+        t = spec["terrain"]
+        terrain = G.cosine_mountain(spec["world"]["nx"], spec["world"]["ny"], t["height"],
+                                    t["cx"], t["cy"], t["hx"], t["hy"])
+    w = orc.World(**spec["world"], terrain=terrain, precision=8)
+    spec["init"](w)
+    for _ in range(3):
+        w.ssprk3_step(orc.step_cfg(**spec["step"]))
+    nf, ref = w.plan_view(3, 3, spec["step"]["dt"], with_precip=False)
+    assert nf == 10
+    nx, ny = spec["world"]["nx"], spec["world"]["ny"]
+    assert abs(pj["grid"]["slice_mean_height_m"] - ref[1].mean()) <= 1e-9 * abs(ref[1].mean())
+    for f, fj in enumerate(pj["fields"]):
+        got = np.asarray(fj["values"]).reshape(ny, nx)
+        scale = max(np.max(np.abs(ref[f])), 1e-300)
+        # ten printed digits, plus the FP64 device/oracle difference (1e-11)
+        assert np.max(np.abs(got - ref[f])) <= 1e-9 * scale + 1e-12, fj["name"]

@@ -155,5 +155,8 @@ def test_perturbation_graph_replay_equals_eager(ctx):
     for _ in range(5):
         a.moist_step(cfg, wr)
     b.moist_steps_graph(cfg, wr, 5)
+    assert a.summary()["nonfinite_values"] == 0 and b.summary()["nonfinite_values"] == 0
     for f in FIELDS + (16, 17, 18):
-        assert np.array_equal(a.get_f64(f), b.get_f64(f), equal_nan=True), f
+        x, y = a.get_f64(f), b.get_f64(f)
+        assert np.all(np.isfinite(x)) and np.all(np.isfinite(y)), f
+        assert np.array_equal(x, y), f

@@ -21,6 +21,7 @@ GPU:
   tracer updater) and warm rain with metric fallout.
 """
 import json
+import shutil
 import os
 import subprocess
 
@@ -57,8 +58,6 @@ def _run(args, timeout=120):
      "Invalid boolean value: maybe"),
     (["--analysis-state", "a", "--boundary-cache", "b", "--steps", "-1"],
      "--steps must be nonnegative"),
-    (["--analysis-state", "a", "--boundary-cache", "b", "--plan-view-json", "p.json"],
-     "--plan-view-json: plan-view output is not part of this build"),
 ])
 def test_driver_rejects_bad_flags(args, msg):
     r = _run(args)
@@ -243,7 +242,7 @@ def _oracle_run(gen, grid, steps, cfg, wr, boundaries):
         w.ssprk3_step(cfg, mode)
         w.tracers_ssprk3(cfg, mode)
         w.warm_rain(wr, metric_fallout=True)
-    return initial, w.summary()
+    return initial, w.summary(), w
 
 
 @pytest.mark.gpu
@@ -252,11 +251,18 @@ def test_prepared_driver_matches_oracle(tmp_path, boundaries):
     grid = PF.GRID
     pa, pb, gen = PF.write_case(str(tmp_path), grid=grid, snapshots=((0, 1.5, 2.0, 0.001),))
     out = str(tmp_path / "summary.json")
+    # plan view beside a copy of the analysis named analysis_state.json, which
+    # the writer merges in (surface fields and their 2 m humidity products)
+    pv_dir = tmp_path / "pv"
+    pv_dir.mkdir()
+    shutil.copy(pa, pv_dir / "analysis_state.json")
+    pv = str(pv_dir / "plan_view.json")
     steps, dt, ns = 3, 2.0, 3
     r = _run(["--analysis-state", pa, "--boundary-cache", pb, "--steps", str(steps),
               "--dt", str(dt), "--fast-substeps", str(ns),
               "--enable-boundaries", "true" if boundaries else "false",
-              "--summary-json", out], timeout=300)
+              "--summary-json", out, "--plan-view-json", pv, "--plan-view-level", "2"],
+             timeout=300)
     assert r.returncode == 0, r.stderr
     with open(out) as fh:
         js = json.load(fh)
@@ -273,7 +279,7 @@ def test_prepared_driver_matches_oracle(tmp_path, boundaries):
 
     cfg = O.step_cfg(dt=dt, fast_substeps=ns)
     wr = O.warm_rain_cfg(dt=dt, rain_terminal_velocity=12.0)
-    initial, final = _oracle_run(gen, grid, steps, cfg, wr, boundaries)
+    initial, final, world = _oracle_run(gen, grid, steps, cfg, wr, boundaries)
     for phase, ours, ref, rtol in (("initial", js["initial"], initial, 2e-6),
                                    ("final", js["final"], final, 2e-4)):
         for okey, jkey in SUM_MAP:
@@ -284,3 +290,22 @@ def test_prepared_driver_matches_oracle(tmp_path, boundaries):
                 scale = max(abs(e), 1e-9)
             assert abs(a - e) <= rtol * scale + (1e-9 if okey.endswith("_path") else 1e-7), \
                 (phase, okey, a, e)
+    # plan view: the runtime bundle with the bucket (23 fields) plus the five
+    # surface fields of the companion analysis; values against the oracle's
+    # products of its own final state (the FP32 trajectories differ at the
+    # 2e-4 level, the JSON carries ten digits)
+    with open(pv) as fh:
+        pj = json.load(fh)
+    assert pj["schema_version"] == "gwm-next-plan-view/v1" and pj["case"] == "prepared_case"
+    assert pj["steps"] == steps and pj["slice_k"] == 2
+    names = [f["name"] for f in pj["fields"]]
+    assert names[:23] == O.PLAN_VIEW_NAMES
+    assert names[23:] == ["surface_pressure", "air_temperature_2m", "specific_humidity_2m",
+                          "relative_humidity_2m", "dewpoint_2m"]
+    nf, ref = world.plan_view(2, steps, dt, with_precip=True)
+    assert nf == 23
+    for f, fj in enumerate(pj["fields"][:23]):
+        assert fj["storage"] == "row_major_yx" and fj["nx"] == grid["nx"]
+        got = np.asarray(fj["values"]).reshape(grid["ny"], grid["nx"])
+        scale = max(np.max(np.abs(ref[f])), 1e-12)
+        assert np.max(np.abs(got - ref[f])) <= 2e-3 * scale, (fj["name"], np.max(np.abs(got - ref[f])), scale)

@@ -181,3 +181,30 @@ def test_moist_step_surrogate_keeps_dry_mass(ctx):
     s = st.summary()
     assert np.isfinite(s["w_max"])
     assert abs(s["total_dry_mass"] - m0) <= 1e-6 * m0
+
+
+def test_quiescent_cells_take_the_plain_answer(ctx):
+    """Cells without cloud or rain at or below saturation have zero source
+    terms in the plain scheme; the surrogate step gives them exactly the
+    plain answer even when the network's output there is far from zero,
+    and applies the network (inside the box) everywhere else."""
+    wr = G.warm_rain_config(dt=5.0, rain_terminal_velocity=0.0)
+    _, a = small_state(ctx, seed=12)
+    _, b = small_state(ctx, seed=12)
+    qc0, qr0 = a.get(17), a.get(18)
+    sur = G.Surrogate(ctx, seed=4)
+    big = 1e30
+    sur.set_norm(x_mean=np.zeros(7), x_std=np.ones(7) * 1e3, y_mean=np.zeros(4),
+                 y_std=np.array([1e-4, 1e-4, 1e-4, 1.0]), box_lo=-big * np.ones(7),
+                 box_hi=big * np.ones(7))
+    sur.set_params((SO.init_xavier(6) * 3.0 + 0.5).astype(np.float32))
+    a.warm_rain(wr, metric_fallout=True)
+    nfall = b.warm_rain_surrogate(wr, sur, metric_fallout=True)
+    assert nfall == 0
+    quiet = (qc0 == 0) & (qr0 == 0) & (a.get(17) == 0)
+    assert 0.1 < quiet.mean() < 0.9, quiet.mean()
+    for f in (O.RHO_THETA, 16, 17, 18):
+        x, y = a.get(f), b.get(f)
+        assert np.array_equal(x[quiet], y[quiet]), f
+    # elsewhere the network's (large) changes were applied
+    assert np.mean(a.get(17)[~quiet] != b.get(17)[~quiet]) > 0.5

@@ -82,3 +82,37 @@ def test_bf16_forward_is_close_to_fp64_forward():
     y64 = S.forward(p, S.normalise_x(x, norm))[2] * norm["y_std"] + norm["y_mean"]
     yb = S.forward_bf16(p, x, norm)
     assert np.linalg.norm(yb - y64) / np.linalg.norm(y64) < 3e-2
+
+
+def test_constrain_clips_condensate_and_closes_with_vapour():
+    # known answers, one cell each: (q, raw change) -> applied change
+    q = np.array([[10e-3, 1e-3, 0.5e-3],    # condensation and autoconversion
+                  [10e-3, 0.0, 0.0],        # no condensate: negative changes clip to 0
+                  [10e-3, 1e-3, 0.5e-3],    # cloud asked to lose more than it holds
+                  [0.2e-3, 0.0, 0.0]])      # condensation larger than the vapour
+    d = np.array([[-0.4e-3, 0.3e-3, 0.1e-3],
+                  [0.3e-3, -0.2e-3, -0.1e-3],
+                  [0.0, -3e-3, 0.2e-3],
+                  [-1e-3, 0.8e-3, 0.0]])
+    dth = np.array([1.0, 0.5, -2.0, 3.0])
+    qn, th = S.constrain(q, d, dth)
+    want = np.array([[9.6e-3, 1.3e-3, 0.6e-3],
+                     [10e-3, 0.0, 0.0],
+                     [10.8e-3, 0.0, 0.7e-3],
+                     [0.0, 0.2e-3, 0.0]])
+    assert np.allclose(qn, want, rtol=0, atol=1e-15)
+    # dtheta passes unless the vapour is exhausted (lambda = 0.2 / 0.8)
+    assert np.allclose(th, [1.0, 0.5, -2.0, 0.75], rtol=0, atol=1e-15)
+    # random cells: total water conserved, nothing negative, the applied
+    # cloud and rain changes are the raw ones wherever no clip or scaling acts
+    rng = np.random.default_rng(11)
+    n = 500
+    q = np.abs(rng.normal(size=(n, 3))) * 1e-3
+    q[rng.random((n, 3)) < 0.3] = 0.0
+    d = rng.normal(size=(n, 3)) * 1e-3
+    qn, th = S.constrain(q, d, rng.normal(size=n))
+    assert np.allclose(qn.sum(axis=1), q.sum(axis=1), rtol=0, atol=1e-15)
+    assert np.all(qn >= 0.0)
+    free = (d[:, 1] >= -q[:, 1]) & (d[:, 2] >= -q[:, 2]) & (q[:, 0] - d[:, 1] - d[:, 2] >= 0)
+    assert free.any() and (~free).any()
+    assert np.allclose(qn[free, 1:] - q[free, 1:], d[free, 1:], rtol=0, atol=1e-15)
