@@ -88,6 +88,9 @@ struct gwm_ctx {
   std::vector<hipEvent_t> pool;
   double* partials = nullptr;  // device scratch for reductions
   size_t partials_n = 0;
+  // This device's copy of the static scratch of gwm_budget_sums_noalloc
+  // (resolved at context creation, once per device).
+  double* noalloc = nullptr;
   int refs = 0;            // live grids and states
   bool closing = false;    // gwm_ctx_destroy called while refs > 0
   // Level chunks per column of the column kernels: chosen per (kernel,
@@ -939,6 +942,8 @@ struct ReduceBufs {
 };
 
 constexpr int kReduceBlocks = 512;
+static_assert(kReduceBlocks * gwm::kReduceParts <= gwm::kNoAllocScratchDoubles,
+              "the budget partials must fit the static scratch");
 
 // Compensated reduction of one field; returns {sum, min, max, non-finite
 // count} (sum and extremes over the finite values).
@@ -996,7 +1001,8 @@ void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, co
   out[3] = bad;
 }
 
-// rb: where the reductions keep their partials (default: the context's).
+// rb.dev set: the allocation-free form (static scratch, no host vectors;
+// the precipitation bucket is reduced on the device too).
 template <class R>
 void budget(gwm_state* S, double* o, const ReduceBufs& rb = ReduceBufs{}) {
   const Geo& g = S->g();
@@ -1063,6 +1069,21 @@ void budget(gwm_state* S, double* o, const ReduceBufs& rb = ReduceBufs{}) {
   }
   const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
   const R* bucket = static_cast<const R*>(S->view ? S->owner->precip : S->precip);
+  if (rb.dev) {
+    // the bucket (ny, nx, no halo) reduced on the device: sum and maximum,
+    // then the count of precipitating columns
+    reduce_field<R>(S, bucket, g.nx, g.ny, 1, 0, nullptr, r, nullptr, nullptr, rb, 0);
+    const double psum = r[0], pmax = r[2];
+    nonfinite += r[3];
+    reduce_field<R>(S, bucket, g.nx, g.ny, 1, 3, nullptr, r, nullptr, nullptr, rb, 0);
+    o[GWM_S_PRECIP_SUM] = psum;
+    o[GWM_S_PRECIP_MEAN] = n2 ? psum / static_cast<double>(n2) : 0.0;
+    o[GWM_S_PRECIP_MAX] = std::max(0.0, pmax);
+    o[GWM_S_PRECIP_CELLS] = static_cast<double>(n2);
+    o[GWM_S_PRECIPITATING_CELLS] = r[0];
+    o[GWM_S_NONFINITE] = nonfinite;
+    return;
+  }
   std::vector<R> p(n2);
   HIPCHK(hipMemcpyAsync(p.data(), bucket, n2 * sizeof(R),
                         hipMemcpyDeviceToHost, S->ctx->stream));
@@ -1557,6 +1578,12 @@ int gwm_ctx_create(int device, int precision, gwm_ctx** out) {
     const char* ch = std::getenv("GWM_CHUNKS");
     if (ch != nullptr) c->force_chunks = std::max(0, std::atoi(ch));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const hipError_t se = gwm::noalloc_scratch_setup(device, &c->noalloc);
+    if (se != hipSuccess) {
+      (void)hipStreamDestroy(c->stream);
+      require(false, std::string("hipGetSymbolAddress of the budget scratch failed: ") +
+                         hipGetErrorString(se));
+    }
     *out = c.release();
   });
 }
@@ -2583,6 +2610,23 @@ int gwm_plan_view(gwm_state* s, int slice_k, int steps, double dt, int with_prec
     require(steps >= 0, "steps must be >= 0");
     *nfields = dispatch(s->ctx, [&](auto r) {
       return plan_view<decltype(r)>(s, slice_k, steps, dt, with_precip, out);
+    });
+  });
+}
+
+int gwm_budget_sums_noalloc(gwm_state* s, double* out) {
+  return guard([&] {
+    S_(s);
+    require(out, "null output");
+    require(s->ctx->noalloc, "the context has no budget scratch");
+    // host staging on the stack: no allocation on either side
+    double host[gwm::kNoAllocScratchDoubles];
+    ReduceBufs rb;
+    rb.dev = s->ctx->noalloc;
+    rb.host = host;
+    dispatch(s->ctx, [&](auto r) {
+      budget<decltype(r)>(s, out, rb);
+      return 0;
     });
   });
 }

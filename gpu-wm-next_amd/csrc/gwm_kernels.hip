@@ -10,6 +10,8 @@
 #include "gwm_kernels.hpp"
 #include "gwm_micro.hpp"
 
+#include <mutex>
+
 namespace gwm {
 
 namespace {
@@ -445,6 +447,8 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
                        (a.rbase ? static_cast<double>(a.rbase[k]) : 0.0);
       v = full / fmax(r, 1.0e-12);
       ext = v;
+    } else if (a.kind == 3) {  // count of positive values; extremes of the value
+      v = full > 0.0 ? 1.0 : 0.0;
     }
     // A NaN or infinity is counted, and kept out of the sums and extremes,
     // which then describe the finite values; the count tells the caller
@@ -664,6 +668,28 @@ void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* re
 template <class R>
 void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks) {
   hipLaunchKernelGGL(k_reduce<R>, dim3(nblocks), dim3(256), 0, st, a);
+}
+
+// Block partials of the allocation-free budget entry point
+// (gwm_budget_sums_noalloc): a file-scope array, one copy per GPU, whose
+// device address is resolved once per device by noalloc_scratch_setup.  The
+// name is a host-side stand-in; host code uses only the resolved address.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+hipError_t noalloc_scratch_setup(int device, double** out) {
+  static std::mutex mu;
+  static double* addr[kNoAllocMaxDevices] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  *out = nullptr;
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  if (addr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
 }
 template <class R>
 void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const double* zc,

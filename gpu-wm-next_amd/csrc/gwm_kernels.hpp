@@ -82,7 +82,7 @@ struct ReduceArgs {
   const R* idz3;
   const R* idz_lev;
   int fx, fy, fz, h;
-  int kind;  // 0 value, 1 value*dz, 2 value/rho
+  int kind;  // 0 value, 1 value*dz, 2 value/rho, 3 count of values > 0
   double* partials;
   // Perturbation form: per-level bases added for min / max (and for rho in
   // kind 2); sums stay over the stored departures.
@@ -171,6 +171,14 @@ struct PlanViewArgs {
 
 template <class R>
 void launch_plan_view(hipStream_t st, const PlanViewArgs<R>& a, bool terrain);
+
+// Allocation-free budget sums: k_reduce block partials live in a static
+// device array of kNoAllocScratchDoubles doubles (32 KiB), one per GPU.
+constexpr int kNoAllocScratchDoubles = 4096;
+constexpr int kNoAllocMaxDevices = 64;
+// Resolves (once per device, with `device` current) and returns the device
+// address of that array on `device`.
+hipError_t noalloc_scratch_setup(int device, double** out);
 template <class R>
 void launch_metrics(hipStream_t st, const Geo& g, const double* zif, const double* zc,
                     const double* wif, const double* wc, const float* terrain, R* zif_m, R* zc_m,

@@ -187,6 +187,7 @@ def lib():
         L.gwm_moist_step.argtypes = [vp, ctypes.POINTER(StepConfig),
                                      ctypes.POINTER(WarmRainConfig), ctypes.POINTER(BoundaryConfig)]
         L.gwm_budget_sums.argtypes = [vp, vp]
+        L.gwm_budget_sums_noalloc.argtypes = [vp, vp]
         L.gwm_state_set_boundary.argtypes = [vp, ip, vp]
         L.gwm_state_checkpoint_write.argtypes = [vp, cp, cp]
         L.gwm_state_checkpoint_read.argtypes = [vp, cp]
@@ -520,9 +521,12 @@ class State:
                                   _ptr(out), ctypes.byref(nf)))
         return nf.value, out
 
-    def summary(self):
+    def summary(self, noalloc=False):
+        """Budget sums (gwm_budget_sums; noalloc=True: the allocation-free
+        gwm_budget_sums_noalloc)."""
         out = np.zeros(32, np.float64)
-        check(lib().gwm_budget_sums(self.h, _ptr(out)))
+        fn = lib().gwm_budget_sums_noalloc if noalloc else lib().gwm_budget_sums
+        check(fn(self.h, _ptr(out)))
         return dict(zip(SUMMARY_KEYS, out.tolist()))
 
     # initial states

@@ -359,6 +359,17 @@ int gwm_budget_sums(gwm_state* state, double* out);
 #define GWM_PLAN_VIEW_FIELDS 23
 int gwm_plan_view(gwm_state* state, int slice_k, int steps, double dt, int with_precip,
                   double* out, int* nfields);
+/* gwm_budget_sums without any allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The reduction partials live
+ * in a static device array of 4096 doubles (32 KiB per GPU, held until the
+ * process ends; the budget needs 512 x 5 of them for any grid size), host
+ * staging is on the stack, and the precipitation bucket is reduced on the
+ * device.  Same slots as gwm_budget_sums: counts, extremes and the
+ * non-finite count agree exactly, sums to rounding (the bucket is summed in
+ * another order).  Calls that overlap on one GPU share the array and race;
+ * each call synchronises the state's stream before it returns. */
+int gwm_budget_sums_noalloc(gwm_state* state, double* out);
 
 /* ---- initial states (host-built, uploaded) ---------------------------------
  * make_constant_dry_state (dry_core.cu:348-361), make_hydrostatic_rest_state
