@@ -92,19 +92,43 @@ def parse():
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run one device-to-device state copy (profiler byte calibration)")
     p.add_argument("--legs", default="all",
-                   help="comma list of legs beyond the headline: full,config_dt,copy,fp64,"
+                   help="comma list of legs beyond the headline: full,graph,config_dt,copy,fp64,"
                         "surrogate,error,ensemble (all: every leg)")
     return p.parse_args()
 
 
-def make_case(G, precision, nx, ny, nz, ctx=None, perturbation=False):
+def make_case(G, precision, nx, ny, nz, ctx=None, perturbation=False, moist=False):
+    """The warm-rain convection case (sounding, seed-42 theta noise); with
+    `moist` also the seeded moisture perturbations of the surrogate's
+    training states (qv scaled by U(0.85, 1.25), cloud and rain in 30 % of
+    the cells each), the same tracer values in every precision and form."""
     if ctx is None:
         ctx = G.Context(0, precision)
     zi = G.stretched_levels(nz, 100.0, 500.0, 16000.0)
     grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
     st = G.State(grid, 3, perturbation=perturbation)
     st.init_convection(1.0e5, 42, 0.5, 1000.0)
+    if moist:
+        for t, q in enumerate(moist_tracers(G, nx, ny, nz)):
+            st.set(16 + t, q)
     return ctx, grid, st, zi
+
+
+_MOIST = {}
+
+
+def moist_tracers(G, nx, ny, nz):
+    """Tracer masses of the moist case, computed once from an FP32
+    full-form state of the case."""
+    key = (nx, ny, nz)
+    if key not in _MOIST:
+        import gwm_amd.surrogate as SG
+        ctx = G.Context(0, G.F32)
+        _, grid, st, _ = make_case(G, G.F32, nx, ny, nz, ctx)
+        _MOIST[key] = SG.moist_tracers(st, 42)
+        del st, grid
+        ctx.close()
+    return _MOIST[key]
 
 
 def column_mass(st, zi):
@@ -122,11 +146,14 @@ KERNELS = ("fast_column", "slow_stage", "level_partials", "level_finalize", "tra
 
 
 def run(G, precision, a, dt, timing=False, surrogate=None, ctx=None, perturbation=False,
-        calibrate_copy=False, info=None, steps=None):
+        calibrate_copy=False, info=None, steps=None, moist=False, graph=False):
     """Warm-up, then `steps` timed moist steps; returns a leg record with
-    the throughput, or value None when the state ended non-finite."""
+    the throughput, or value None when the state ended non-finite.  With a
+    surrogate the record also gives the share of cells that went through
+    the network at the last step (the others took the plain scheme's
+    answer: zero source terms, or outside the training box)."""
     steps = a.steps if steps is None else steps
-    ctx, grid, st, zi = make_case(G, precision, a.nx, a.ny, a.nz, ctx, perturbation)
+    ctx, grid, st, zi = make_case(G, precision, a.nx, a.ny, a.nz, ctx, perturbation, moist)
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
 
@@ -136,21 +163,31 @@ def run(G, precision, a, dt, timing=False, surrogate=None, ctx=None, perturbatio
         else:
             st.moist_step_surrogate(cfg, wr, surrogate)
 
+    def steps_graph(n):
+        # replayed from the captured two-step graph (bit-identical steps)
+        st.moist_steps_graph(cfg, wr, n, surrogate)
+
     if calibrate_copy:
         other = G.State(grid, 3)
         other.copy_from(st)
         ctx.sync()
         del other
-    for _ in range(a.warmup):
-        step()
+    if graph:
+        steps_graph(max(a.warmup, 2))
+    else:
+        for _ in range(a.warmup):
+            step()
     ctx.sync()
     if timing:
         ctx.reset_kernel_stats()
         ctx.set_kernel_timing(True)
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
+    if graph:
+        steps_graph(steps)
+    else:
+        for _ in range(steps):
+            step()
     ctx.sync()
     secs = time.perf_counter() - t0
     stats = {}
@@ -169,6 +206,10 @@ def run(G, precision, a, dt, timing=False, surrogate=None, ctx=None, perturbatio
            "nonfinite_values": int(summary["nonfinite_values"]),
            "value": cells * steps / secs if finite else None,
            "ms_per_step": secs / steps * 1e3}
+    if surrogate is not None:
+        fb, nw = surrogate.last_counts()
+        rec["network_cell_share"] = nw / cells
+        rec["fallback_cell_share"] = fb / cells
     if not finite:
         rec["note"] = "state non-finite after the timed steps: no throughput credited"
     return rec, stats, summary
@@ -245,25 +286,32 @@ def rel_errors(G, ref, states, fields):
 
 
 def error_leg(G, a, dt, sur):
-    """From the same initial state: `error_steps` moist steps of the FP64
-    plain step, the FP32 full form, the FP32 perturbation form, and both
-    with the surrogate; relative L2 of each against the FP64 plain step,
-    and the dry-mass change of the perturbation form."""
+    """From the same initial state, the moist case (cloud, rain and
+    supersaturated cells, so the source terms are large): `error_steps`
+    moist steps of the FP64 plain step, the FP32 full form, the FP32
+    perturbation form, both with the surrogate, and the perturbation form
+    without microphysics (what the surrogate stands in for); relative L2 of
+    each against the FP64 plain step, and the dry-mass change of the
+    perturbation form."""
     ctx64 = G.Context(0, G.F64)
     ctx32 = G.Context(0, G.F32)
-    _, _, ref, zi = make_case(G, G.F64, a.nx, a.ny, a.nz, ctx64)
+    _, _, ref, zi = make_case(G, G.F64, a.nx, a.ny, a.nz, ctx64, moist=True)
     states = {}
-    _, _, states["fp32_full"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32)
+    _, _, states["fp32_full"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32, moist=True)
     _, _, states["fp32_perturbation"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32,
-                                                     perturbation=True)
+                                                     perturbation=True, moist=True)
     surs = {}
     if sur is not None:
         s = G.Surrogate(ctx32, 0)
         s.set_norm(**{k: v for k, v in sur[1].items()})
         s.set_params(sur[0])
         for form, pert in (("fp32_full_surrogate", False), ("fp32_perturbation_surrogate", True)):
-            _, _, states[form], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32, perturbation=pert)
+            _, _, states[form], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32, perturbation=pert,
+                                              moist=True)
             surs[form] = s
+        # what the surrogate stands in for: the same run without microphysics
+        _, _, states["fp32_perturbation_no_microphysics"], _ = make_case(
+            G, G.F32, a.nx, a.ny, a.nz, ctx32, perturbation=True, moist=True)
     cfg = G.step_config(dt=dt, fast_substeps=6)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
     m0 = column_mass(states["fp32_perturbation"], zi)
@@ -273,6 +321,9 @@ def error_leg(G, a, dt, sur):
         for name, st in states.items():
             if name in surs:
                 st.moist_step_surrogate(cfg, wr, surs[name])
+            elif name.endswith("no_microphysics"):
+                st.ssprk3_step(cfg)
+                st.tracers_ssprk3(cfg)
             else:
                 st.moist_step(cfg, wr)
         masses.append(column_mass(states["fp32_perturbation"], zi))
@@ -299,8 +350,12 @@ def surrogate_train(G, a):
     dt = a.control_dt or a.dt
     log(f"surrogate: sampling {a.surrogate_pairs:.0f} pairs")
     t0 = time.perf_counter()
+    # pairs from four-step plain runs of each seed (the start and the
+    # near-saturated states after the first adjustments)
+    run = (G.step_config(dt=dt, fast_substeps=6),
+           G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0), 4)
     X, Y, n = SG.generate(ctx, seeds, per, shape=(a.nx, a.ny, a.nz),
-                          wr=G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0))
+                          wr=G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0), run=run)
     ctx.sync()
     t_gen = time.perf_counter() - t0
     log("surrogate: training")
@@ -443,7 +498,7 @@ def main():
         sys.exit(2)
     import gwm_amd as G
 
-    legs = set(("full,config_dt,copy,fp64,surrogate,error,ensemble" if a.legs == "all"
+    legs = set(("full,graph,config_dt,copy,fp64,surrogate,error,ensemble" if a.legs == "all"
                 else a.legs).split(","))
     cells = a.nx * a.ny * a.nz
     dt = a.control_dt if a.control_dt > 0 else a.dt
@@ -487,7 +542,10 @@ def main():
             kernel_roofline[name] = {"achieved_GBs": round(gbs, 1), "frac": round(gbs / 8000.0, 4),
                                      "algorithmic_bytes": int(b)}
 
-    full = config_dt = fp64 = sur_rec = err = ens = None
+    full = config_dt = fp64 = sur_rec = err = ens = graph = None
+    if "graph" in legs:
+        log("FP32 perturbation form replayed from the captured two-step graph")
+        graph, _, _ = run(G, G.F32, a, dt, perturbation=True, graph=True)
     if "full" in legs:
         log("FP32 full form (the reference's formulation), dt %g" % dt)
         full, _, _ = run(G, G.F32, a, dt)
@@ -513,6 +571,26 @@ def main():
         sur_rec["full_form"] = sfull
         sur_rec["perturbation_form"] = spert
         sur_rec["kernels"] = sstats
+        # The case above has no cloud or rain and stays subsaturated over the
+        # timed steps, so the network has almost nothing to do there; the
+        # moist case (cloud, rain and supersaturation in most cells) is the
+        # one where it replaces real source terms.
+        log("moist case: FP32 perturbation form plain and with the surrogate, FP64 plain")
+        mplain, _, _ = run(G, G.F32, a, dt, perturbation=True, moist=True)
+        mpert, _, _ = run(G, G.F32, a, dt, surrogate=sur, ctx=sctx, perturbation=True, moist=True)
+        _, mstats, _ = run(G, G.F32, a, dt, surrogate=sur, ctx=sctx, perturbation=True,
+                           moist=True, timing=True)
+        m64 = None
+        if a.fp64_steps > 0:
+            m64, _, _ = run(G, G.F64, a, dt, steps=a.fp64_steps, moist=True)
+        moist = {"fp32_perturbation_plain": mplain, "fp32_perturbation_surrogate": mpert,
+                 "fp64_plain": m64, "kernels": mstats}
+        if m64 and m64["value"]:
+            moist["over_fp64_plain"] = {
+                k: moist[k]["value"] / m64["value"]
+                for k in ("fp32_perturbation_plain", "fp32_perturbation_surrogate")
+                if moist[k]["value"]}
+        sur_rec["moist_case"] = moist
         del sur, sctx
     if "error" in legs and a.error_steps > 0:
         log(f"error leg: {a.error_steps} steps against the FP64 step")
@@ -521,6 +599,7 @@ def main():
     ratios = {}
     if fp64 and fp64["value"]:
         for name, rec in (("fp32_perturbation", head), ("fp32_full", full),
+                          ("fp32_perturbation_graph_replay", graph),
                           ("fp32_full_surrogate", sur_rec and sur_rec["full_form"]),
                           ("fp32_perturbation_surrogate", sur_rec and sur_rec["perturbation_form"])):
             if rec and rec.get("value"):
@@ -552,6 +631,7 @@ def main():
                    "global_batch": 1, "parallelism": "single GPU"},
         "state_finite": head["state_finite"],
         "fp32_full_form": full,
+        "fp32_perturbation_graph_replay": graph,
         "at_configuration_dt": config_dt,
         "fp64_plain": fp64,
         "over_fp64_plain": ratios,

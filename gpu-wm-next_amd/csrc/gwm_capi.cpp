@@ -2724,6 +2724,8 @@ struct gwm_surrogate {
   gwm::SurNorm norm{};
   bool have_norm = false;
   long version = 0;  // bumped by set_norm (graphs bake the normalisation in)
+  // cells of the last surrogate warm-rain call: [0] plain-scheme fallback
+  // (outside the training box), [1] through the network
   unsigned long long* nfall = nullptr;
   double* loss_part = nullptr;
   size_t loss_n = 0;
@@ -2817,7 +2819,7 @@ void warm_rain_surrogate(gwm_state* S, const gwm_warm_rain_config& c, gwm_surrog
   float* qv = static_cast<float*>(S->field(GWM_TRACER0 + 0));
   float* qc = static_cast<float*>(S->field(GWM_TRACER0 + 1));
   float* qr = static_cast<float*>(S->field(GWM_TRACER0 + 2));
-  gwm::launch_zero_counter(S->ctx->stream, M->nfall);
+  gwm::launch_zero_counters(S->ctx->stream, M->nfall, 2);
   {
     Timed t(S->ctx, "warm_rain_surrogate");
     gwm::launch_warm_rain_sur(S->ctx->stream, g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params,
@@ -2880,7 +2882,9 @@ int gwm_surrogate_create(gwm_ctx* ctx, unsigned long long seed, gwm_surrogate** 
     HIPCHK(hipMemsetAsync(M->T.m, 0, gwm::kSurParams * sizeof(float), ctx->stream));
     HIPCHK(hipMemsetAsync(M->T.v, 0, gwm::kSurParams * sizeof(float), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    M->nfall = static_cast<unsigned long long*>(dalloc(sizeof(unsigned long long)));
+    M->nfall = static_cast<unsigned long long*>(dalloc(2 * sizeof(unsigned long long)));
+    gwm::launch_zero_counters(ctx->stream, M->nfall, 2);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     *out = M.release();
   });
 }
@@ -3037,6 +3041,17 @@ int gwm_warm_rain_surrogate(gwm_state* s, const gwm_warm_rain_config* c, gwm_sur
     S_(s);
     require(c && m, "null argument");
     warm_rain_surrogate(s, *c, m, metric_fallout, n_fallback);
+  });
+}
+
+int gwm_surrogate_last_counts(gwm_surrogate* m, long long* fallback, long long* network) {
+  return guard([&] {
+    require(m != nullptr, "null surrogate");
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, m->nfall, sizeof h, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+    if (fallback) *fallback = static_cast<long long>(h[0]);
+    if (network) *network = static_cast<long long>(h[1]);
   });
 }
 

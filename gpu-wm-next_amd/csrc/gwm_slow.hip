@@ -154,6 +154,9 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     Raw o, h;  // own cell, ring cell
   };
   auto ldo = [&](const R* f, int off, unsigned plane, int kk) -> R {
+    // The perturbation form runs on doubly periodic grids only, where every
+    // staged position maps to a stored one: no zero stand-ins to select.
+    if constexpr (PERT) return ld32(f, static_cast<unsigned>(off) + static_cast<unsigned>(kk) * plane);
     const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * plane);
     return off < 0 ? R(0) : v;
   };
@@ -222,7 +225,8 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
 
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
   const R rdx = R(1) / dx, rdy = R(1) / dy;
-  const bool ow = !g.per_x, oe = !g.per_x, os = !g.per_y, on = !g.per_y;
+  // open sides (none in the perturbation form, which is doubly periodic)
+  const bool ow = !PERT && !g.per_x, oe = ow, os = !PERT && !g.per_y, on = os;
   const size_t plane_m = static_cast<size_t>(g.nx) * g.ny;
   auto mcol = [&](int ii, int jj) -> size_t {
     return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);

@@ -335,16 +335,20 @@ __device__ inline f32x4 mlp_group(const MlpRegs& R, const float* sb, const float
 // the barriers are too.
 constexpr int kTile = 256;
 
+// One 16-cell group: rows feat[0..15] -> out[0..15].
+__device__ inline void group_mlp(const MlpRegs& R, const float* sb, const float (*feat)[8],
+                                 float (*out)[kSurOut], int lane) {
+  const int c = lane & 15, g = lane >> 4;
+  float xin[4];
+  for (int v = 0; v < 4; ++v) xin[v] = g < 2 ? feat[c][4 * g + v] : 0.0f;
+  const f32x4 o = mlp_group(R, sb, xin, g);
+  if (g == 0)
+    for (int r = 0; r < kSurOut; ++r) out[c][r] = o[r];
+}
+
 __device__ inline void tile_mlp(const MlpRegs& R, const float* sb, const float (*feat)[8],
                                 float (*out)[kSurOut], int lane) {
-  const int c = lane & 15, g = lane >> 4;
-  for (int q = 0; q < 4; ++q) {
-    float xin[4];
-    for (int v = 0; v < 4; ++v) xin[v] = g < 2 ? feat[16 * q + c][4 * g + v] : 0.0f;
-    const f32x4 o = mlp_group(R, sb, xin, g);
-    if (g == 0)
-      for (int r = 0; r < kSurOut; ++r) out[16 * q + c][r] = o[r];
-  }
+  for (int q = 0; q < 4; ++q) group_mlp(R, sb, feat + 16 * q, out + 16 * q, lane);
 }
 
 // Raw features x[n][7] -> raw outputs y[n][4] (tests and timing).
@@ -377,16 +381,29 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 // the others the plain scheme (counted as fallback cells); cells without
 // condensate at or below saturation keep their state, the plain scheme's
 // exact answer there.
+//
+// Only the cells that take the network's answer go through it: a tile's
+// network cells are packed, in lane order, into the first rows of the
+// work-group's feature block, and its waves share the 16-cell groups that
+// hold them.  A tile of quiet or out-of-box cells runs no group; a full one
+// runs all sixteen, as the dense form does.  Each packed cell's outputs
+// depend on its own row only (the matrix product keeps columns apart), so
+// the results equal those of running every cell through the network.
+// counts[0] += fallback cells, counts[1] += network cells.
 __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
                                                        float* qv, float* qc, float* qr,
                                                        MicroCfg<float> mc, const float* w,
-                                                       SurNorm nm, unsigned long long* nfall,
+                                                       SurNorm nm, unsigned long long* counts,
                                                        Base<float> base) {
   __shared__ float sb[196];
-  __shared__ float sfeat[4][64][8];
-  __shared__ float sout[4][64][kSurOut];
-  __shared__ unsigned cnt;
-  if (threadIdx.x == 0) cnt = 0;
+  __shared__ float sfeat[kTile][8];
+  __shared__ float sout[kTile][kSurOut];
+  __shared__ unsigned wnet[kTile / 64];
+  __shared__ unsigned cnt_fall, cnt_net;
+  if (threadIdx.x == 0) {
+    cnt_fall = 0;
+    cnt_net = 0;
+  }
   load_bias(w, sb);
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -395,7 +412,7 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
   const int nxy = g.nx * g.ny;
   const int n = nxy * g.nz;
   const bool pert = base.rho != nullptr;
-  unsigned mine = 0;
+  unsigned mine = 0, mine_net = 0;
   for (int tile = blockIdx.x; tile * kTile < n; tile += gridDim.x) {
     const int cell = tile * kTile + wv * 64 + lane;
     const bool valid = cell < n;
@@ -415,26 +432,46 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       for (int q = 0; q < kSurIn; ++q) f[q] = nm.xm[q];
     }
     bool inbox = true;
-    for (int q = 0; q < kSurIn; ++q) {
-      inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
-      sfeat[wv][lane][q] = qdiv(f[q] - nm.xm[q], nm.xs[q]);
-    }
+    for (int q = 0; q < kSurIn; ++q) inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
     // Without cloud or rain and at or below saturation every source term
     // of the plain scheme is zero (no condensation, nothing to evaporate,
     // convert or accrete): such a cell takes the plain scheme's answer, its
-    // own state written back, instead of the network's residual.  The
-    // network still runs for the whole tile, so the cost does not change.
+    // own state written back.
     const bool quiet = valid && m.qc <= 0.0f && m.qr <= 0.0f && m.qv <= q_qsat(m.T, m.p);
-    sfeat[wv][lane][7] = 0.0f;
+    const bool net = valid && !quiet && inbox;
+    // pack the network cells: slot = cells of the waves before + lanes below
+    const unsigned long long bal = __ballot(net);
+    const unsigned below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(bal >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(
+                                                         static_cast<unsigned>(bal), 0u));
+    if (lane == 0) wnet[wv] = static_cast<unsigned>(__popcll(bal));
     __syncthreads();
-    tile_mlp(R, sb, sfeat[wv], sout[wv], lane);
+    unsigned before = 0, total = 0;
+    for (int v = 0; v < kTile / 64; ++v) {
+      const unsigned c = wnet[v];
+      before += v < wv ? c : 0u;
+      total += c;
+    }
+    const unsigned slot = before + below;
+    if (net) {
+      for (int q = 0; q < kSurIn; ++q) sfeat[slot][q] = qdiv(f[q] - nm.xm[q], nm.xs[q]);
+      sfeat[slot][7] = 0.0f;
+    }
+    // rows past the last network cell up to the group boundary: zeros
+    const unsigned padded = (total + 15u) & ~15u;
+    if (threadIdx.x >= total && threadIdx.x < padded)
+      for (int q = 0; q < 8; ++q) sfeat[threadIdx.x][q] = 0.0f;
+    __syncthreads();
+    for (unsigned gq = wv; 16u * gq < total; gq += kTile / 64)
+      group_mlp(R, sb, sfeat + 16 * gq, sout + 16 * gq, lane);
     __syncthreads();
     if (valid) {
       const float th0 = m.th;
       if (quiet) {
         if (!inbox) ++mine;
-      } else if (inbox) {
-        const float* o = sout[wv][lane];
+      } else if (net) {
+        ++mine_net;
+        const float* o = sout[slot];
         // The source terms only exchange water between vapour, cloud and
         // rain.  Cloud and rain cannot lose more than they hold; vapour, the
         // reservoir, takes the opposite of their change (the network's own
@@ -461,9 +498,13 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
       rt[ci] = pert ? rtp + m.rho * (m.th - th0) : m.rho * m.th;
     }
   }
-  if (mine) atomicAdd(&cnt, mine);
+  if (mine) atomicAdd(&cnt_fall, mine);
+  if (mine_net) atomicAdd(&cnt_net, mine_net);
   __syncthreads();
-  if (threadIdx.x == 0 && cnt) atomicAdd(nfall, static_cast<unsigned long long>(cnt));
+  if (threadIdx.x == 0) {
+    if (cnt_fall) atomicAdd(&counts[0], static_cast<unsigned long long>(cnt_fall));
+    if (cnt_net) atomicAdd(&counts[1], static_cast<unsigned long long>(cnt_net));
+  }
 }
 
 // Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
@@ -507,7 +548,9 @@ __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
   precip[static_cast<size_t>(j) * g.nx + i] += inc;
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
 
 // ---------------------------------------------------------------------------
 // Launchers
@@ -583,16 +626,16 @@ void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const f
   hipLaunchKernelGGL(k_sur_infer, dim3(infer_blocks(n)), dim3(256), 0, st, w, nm, x, n, y);
 }
 
-void launch_zero_counter(hipStream_t st, unsigned long long* p) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, p);
+void launch_zero_counters(hipStream_t st, unsigned long long* p, int n) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
 }
 
 void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* nfall, Base<float> base) {
+                          const SurNorm& nm, unsigned long long* counts, Base<float> base) {
   const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
   hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks(cells)), dim3(256), 0, st, g, rho, rt,
-                     qv, qc, qr, mc, w, nm, nfall, base);
+                     qv, qc, qr, mc, w, nm, counts, base);
 }
 
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,

@@ -54,9 +54,10 @@ void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const f
                       float* y);
 void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* nfall, Base<float> base);
-// Clears a device counter with a kernel (no memset node in a captured graph).
-void launch_zero_counter(hipStream_t st, unsigned long long* p);
+                          const SurNorm& nm, unsigned long long* counts, Base<float> base);
+// Clears n (<= 64) device counters with a kernel (no memset node in a
+// captured graph).
+void launch_zero_counters(hipStream_t st, unsigned long long* p, int n);
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
                     const float* lev_idz_c, float dt, float vt, float depth, int fallout,
                     bool terrain);

@@ -212,6 +212,8 @@ def lib():
         L.gwm_surrogate_infer.argtypes = [vp, vp, lp, vp]
         L.gwm_warm_rain_surrogate.argtypes = [vp, ctypes.POINTER(WarmRainConfig), vp, ip,
                                               ctypes.POINTER(ctypes.c_longlong)]
+        L.gwm_surrogate_last_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
+                                                ctypes.POINTER(ctypes.c_longlong)]
         L.gwm_moist_step_surrogate.argtypes = [vp, ctypes.POINTER(StepConfig),
                                                ctypes.POINTER(WarmRainConfig), vp,
                                                ctypes.POINTER(BoundaryConfig)]
@@ -583,6 +585,16 @@ class DeviceArray:
         check(lib().gwm_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
         return out
 
+    def download_range(self, row, rows):
+        """Host copy of `rows` rows from row `row` on."""
+        row = int(row)
+        rows = int(rows)
+        if row < 0 or rows < 0 or row + rows > self.shape[0]:
+            raise ValueError("download_range: rows outside the array")
+        out = np.empty((rows,) + self.shape[1:], self.dtype)
+        check(lib().gwm_memcpy_d2h(self.ctx.h, _ptr(out), self.offset(row), out.nbytes))
+        return out
+
     def offset(self, rows):
         """Pointer to row `rows` of a 2-D array."""
         row_bytes = self.nbytes // self.shape[0]
@@ -677,6 +689,13 @@ class Surrogate:
 
     def infer(self, x, n, y):
         check(lib().gwm_surrogate_infer(self.h, x.ptr, int(n), y.ptr))
+
+    def last_counts(self):
+        """(fallback cells, network cells) of the last surrogate warm-rain
+        call on this network's context."""
+        fb, nw = ctypes.c_longlong(), ctypes.c_longlong()
+        check(lib().gwm_surrogate_last_counts(self.h, ctypes.byref(fb), ctypes.byref(nw)))
+        return fb.value, nw.value
 
 
 def surrogate_sample(state, cfg, seed, n, x, y, cell=None, row=0):

@@ -29,33 +29,56 @@ def convection_state(ctx, nx, ny, nz, seed, dz_bottom=100.0, dz_top=500.0, z_top
     st = State(grid, 3)
     st.init_convection(1.0e5, int(seed), 0.5, 1000.0)
     if perturb:
-        rng = np.random.default_rng(10_000 + int(seed))
-        rho = st.get(0)
-        qv = st.get(16)
-        shape = qv.shape
-        qv = qv * rng.uniform(0.85, 1.25, shape).astype(np.float32)
-        qc = rho * (np.abs(rng.standard_normal(shape)) * 1.5e-3 *
-                    (rng.random(shape) < 0.3)).astype(np.float32)
-        qr = rho * (np.abs(rng.standard_normal(shape)) * 0.8e-3 *
-                    (rng.random(shape) < 0.3)).astype(np.float32)
-        st.set(16, qv.astype(np.float32))
-        st.set(17, qc.astype(np.float32))
-        st.set(18, qr.astype(np.float32))
+        for t, q in enumerate(moist_tracers(st, seed)):
+            st.set(16 + t, q)
     return grid, st
 
 
-def generate(ctx, seeds, pairs_per_seed, shape=(512, 512, 64), wr=None):
+def moist_tracers(st, seed):
+    """The seeded moisture perturbations of convection_state for a full-form
+    state just initialised by gwm_init_convection: the tracer masses
+    (rho qv, rho qc, rho qr) as FP32 arrays, to be set on this state or on
+    others of the same grid (any precision or form)."""
+    rng = np.random.default_rng(10_000 + int(seed))
+    rho = st.get(0)
+    qv = st.get(16)
+    shape = qv.shape
+    qv = qv * rng.uniform(0.85, 1.25, shape).astype(np.float32)
+    qc = rho * (np.abs(rng.standard_normal(shape)) * 1.5e-3 *
+                (rng.random(shape) < 0.3)).astype(np.float32)
+    qr = rho * (np.abs(rng.standard_normal(shape)) * 0.8e-3 *
+                (rng.random(shape) < 0.3)).astype(np.float32)
+    return qv.astype(np.float32), qc.astype(np.float32), qr.astype(np.float32)
+
+
+def generate(ctx, seeds, pairs_per_seed, shape=(512, 512, 64), wr=None, run=None):
     """Training pairs from the plain scheme on convection states of the
-    given seeds: returns device arrays X [n][7], Y [n][4] and n."""
+    given seeds: returns device arrays X [n][7], Y [n][4] and n.
+
+    With `run` = (step_cfg, moist_wr, steps) the pairs come from plain-scheme
+    runs: each seed's pairs are split evenly over its state at steps 0, 1,
+    .., `steps` of moist steps (dynamics, tracers, warm rain with `moist_wr`),
+    so the network also sees the near-saturated states the scheme leaves
+    behind after its first adjustment, not only the forced start."""
     if ctx.precision != F32:
         raise ValueError("the surrogate works on FP32 contexts")
     wr = wr or warm_rain_config(dt=5.0, rain_terminal_velocity=12.0)
-    n = len(seeds) * int(pairs_per_seed)
+    per = int(pairs_per_seed)
+    n = len(seeds) * per
     X = DeviceArray(ctx, (n, SURROGATE_IN))
     Y = DeviceArray(ctx, (n, SURROGATE_OUT))
+    snaps = 1 if run is None else int(run[2]) + 1
+    if snaps < 1 or per < snaps:
+        raise ValueError("generate: need at least one pair per sampled state")
+    counts = [per // snaps + (1 if k < per % snaps else 0) for k in range(snaps)]
     for s, seed in enumerate(seeds):
         _, st = convection_state(ctx, *shape, seed)
-        surrogate_sample(st, wr, seed, pairs_per_seed, X, Y, row=s * int(pairs_per_seed))
+        row = s * per
+        for k, cnt in enumerate(counts):
+            if k:
+                st.moist_step(run[0], run[1])
+            surrogate_sample(st, wr, seed + 1009 * k, cnt, X, Y, row=row)
+            row += cnt
         del st
     return X, Y, n
 
@@ -73,26 +96,43 @@ def fit_norm(x, y, box=(1.0, 99.0)):
                 box_lo=np.percentile(x, box[0], axis=0), box_hi=np.percentile(x, box[1], axis=0))
 
 
+def spread_rows(arr, n, m, blocks=64):
+    """About m rows of the first n rows of a device array, taken as `blocks`
+    contiguous runs spread evenly over them (the pairs are stored seed by
+    seed and state by state, so a prefix would see only the first ones)."""
+    if m >= n:
+        return arr.download_rows(n)
+    blocks = max(1, min(blocks, m))
+    run = m // blocks
+    starts = np.linspace(0, n - run, blocks).astype(np.int64)
+    return np.concatenate([arr.download_range(int(s), run) for s in starts])
+
+
 def train(ctx, X, Y, n, seed=0, epochs=4, batch=4096, lr=2e-3, lr_final=5e-5, norm_rows=1_000_000):
-    """A surrogate with its normalisation fitted on the first rows (the
-    sampling order is random) and trained on all n pairs."""
+    """A surrogate with its normalisation fitted on up to `norm_rows` pairs
+    spread over all n, and trained on all n pairs."""
     sur = Surrogate(ctx, seed)
     m = min(n, norm_rows)
-    norm = fit_norm(X.download_rows(m), Y.download_rows(m))
+    norm = fit_norm(spread_rows(X, n, m), spread_rows(Y, n, m))
     sur.set_norm(**norm)
     losses = sur.train(X, Y, n, batch=batch, epochs=epochs, lr=lr, lr_final=lr_final)
     return sur, norm, losses
 
 
-def tendency_errors(ctx, sur, seed=42, shape=(128, 128, 64), wr=None):
+def tendency_errors(ctx, sur, seed=42, shape=(128, 128, 64), wr=None, run=None):
     """One warm-rain call with the surrogate and with the plain scheme from
     the same convection state: relative L2 error of the surrogate's changes
     of (qv, qc, qr, theta) against the plain changes, the same for the
     method without the network (no source terms at all), and the fallback
-    count."""
+    count.  With `run` = (step_cfg, moist_wr, steps) the state is first
+    advanced `steps` plain moist steps."""
     wr = wr or warm_rain_config(dt=5.0, rain_terminal_velocity=0.0)
     _, a = convection_state(ctx, *shape, seed)
     _, b = convection_state(ctx, *shape, seed)
+    if run is not None:
+        for _ in range(int(run[2])):
+            a.moist_step(run[0], run[1])
+            b.moist_step(run[0], run[1])
 
     def snap(st):
         rho = st.get(0).astype(np.float64)

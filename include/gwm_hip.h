@@ -439,6 +439,12 @@ int gwm_surrogate_infer(gwm_surrogate* m, const void* dev_x, long n, void* dev_y
  * n_fallback (optional) receives the number of cells outside the box. */
 int gwm_warm_rain_surrogate(gwm_state* state, const gwm_warm_rain_config* cfg, gwm_surrogate* m,
                             int metric_fallout, long long* n_fallback);
+/* Cells of the last surrogate warm-rain call on this network's context
+ * (synchronises): taken by the plain scheme outside the training box, and
+ * evaluated by the network (cells without cloud or rain at or below
+ * saturation take neither: every source term is zero there).  No
+ * counterpart in the reference. */
+int gwm_surrogate_last_counts(gwm_surrogate* m, long long* fallback, long long* network);
 /* gwm_moist_step with the surrogate warm rain. */
 int gwm_moist_step_surrogate(gwm_state* state, const gwm_step_config* cfg,
                              const gwm_warm_rain_config* wr, gwm_surrogate* m,

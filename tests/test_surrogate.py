@@ -208,3 +208,45 @@ def test_quiescent_cells_take_the_plain_answer(ctx):
         assert np.array_equal(x[quiet], y[quiet]), f
     # elsewhere the network's (large) changes were applied
     assert np.mean(a.get(17)[~quiet] != b.get(17)[~quiet]) > 0.5
+
+
+def test_packed_network_cells_match_unpacked(ctx):
+    """Only the cells that take the network's answer go through it, packed
+    into 16-cell groups per tile.  A cell's result must not depend on which
+    other cells are packed with it: quieting half the cells (no water at
+    all, so every source term is zero) changes the packing of the rest but
+    not one bit of their results; the counts report the cells that went
+    through the network: those with cloud or rain, or above saturation (the
+    plain scheme condenses there)."""
+    wr = G.warm_rain_config(dt=5.0, rain_terminal_velocity=0.0)
+    _, a = small_state(ctx, seed=14)
+    _, b = small_state(ctx, seed=14)
+    _, c = small_state(ctx, seed=14)
+    qc0, qr0 = a.get(17), a.get(18)
+    a.warm_rain(wr, metric_fallout=True)
+    quiet = (qc0 == 0) & (qr0 == 0) & (a.get(17) == 0)
+    sur = G.Surrogate(ctx, seed=5)
+    big = 1e30
+    sur.set_norm(x_mean=np.zeros(7), x_std=np.ones(7) * 1e3, y_mean=np.zeros(4),
+                 y_std=np.array([1e-4, 1e-4, 1e-4, 1.0]), box_lo=-big * np.ones(7),
+                 box_hi=big * np.ones(7))
+    sur.set_params((SO.init_xavier(7) * 3.0 + 0.5).astype(np.float32))
+    mask = np.random.default_rng(3).random(qc0.shape) < 0.5
+    for f in (16, 17, 18):
+        q = c.get(f)
+        q[mask] = 0.0
+        c.set(f, q)
+    assert b.warm_rain_surrogate(wr, sur, metric_fallout=True) == 0
+    fb_b, nw_b = sur.last_counts()
+    assert c.warm_rain_surrogate(wr, sur, metric_fallout=True) == 0
+    fb_c, nw_c = sur.last_counts()
+    print("network cells", nw_b, nw_c, "of", mask.size, "quiet", int(quiet.sum()))
+    assert fb_b == 0 and fb_c == 0
+    assert nw_b == int(np.sum(~quiet))
+    assert nw_c == int(np.sum(~quiet & ~mask))
+    assert 0.2 * mask.size < nw_c < nw_b
+    for f in (O.RHO_THETA, 16, 17, 18):
+        x, y = b.get(f), c.get(f)
+        assert np.array_equal(x[~mask], y[~mask]), f
+    # the network's (large) changes were applied in the unmasked cells
+    assert np.mean(a.get(17)[~mask & ~quiet] != c.get(17)[~mask & ~quiet]) > 0.5
