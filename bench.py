@@ -421,52 +421,71 @@ def ensemble_members(G, n, seed=0, z_top=16000.0):
 
 
 def ensemble_leg(G, a, sur_src):
-    """32 uneven members, FP32 + surrogate, 4 substeps, advanced
-    concurrently from captured graphs (one stream per member), against the
-    same members stepped one after the other without graphs; dt 0.5 s (the
+    """32 uneven members, FP32 + surrogate, 4 substeps, dt 0.5 s (the
     configuration's 2 s overflows within three steps, 1 s within twenty on
-    the 60-level members)."""
-    members = ensemble_members(G, a.ensemble_members)
-    cells = sum(m[3] for m in members)
+    the 60-level members), three ways from fresh members each: one launch per
+    kernel of the step over all members (batched, graph-replayed), one
+    context (stream) per member with its own graph replays, and the members
+    stepped one after the other without graphs."""
     dt = a.ensemble_dt
     cfg = G.step_config(dt=dt, fast_substeps=4)
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
-    surs = None
-    if sur_src is not None:
-        params, norm = sur_src
-        surs = []
+
+    def fresh():
+        members = ensemble_members(G, a.ensemble_members)
+        surs = None
+        if sur_src is not None:
+            params, norm = sur_src
+            surs = []
+            for ctx, _, _, _ in members:
+                s = G.Surrogate(ctx, 0)
+                s.set_norm(**{k: v for k, v in norm.items()})
+                s.set_params(params)
+                surs.append(s)
+        return members, surs
+
+    def finish(members):
         for ctx, _, _, _ in members:
-            s = G.Surrogate(ctx, 0)
-            s.set_norm(**{k: v for k, v in norm.items()})
-            s.set_params(params)
-            surs.append(s)
-    states = [m[2] for m in members]
-    # warm-up: captures every member's graphs
-    G.ensemble_steps(states, cfg, wr, 3, surs)
-    t0 = time.perf_counter()
-    G.ensemble_steps(states, cfg, wr, a.ensemble_steps, surs)
-    t_graph = time.perf_counter() - t0
-    for ctx, _, _, _ in members:
-        ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.ensemble_steps):
-        for i, st in enumerate(states):
-            if surs is None:
-                st.moist_step(cfg, wr)
-            else:
-                st.moist_step_surrogate(cfg, wr, surs[i])
-    for ctx, _, _, _ in members:
-        ctx.sync()
-    t_eager = time.perf_counter() - t0
-    bad = sum(int(st.summary()["nonfinite_values"]) for st in states)
-    finite = bad == 0
+            ctx.sync()
+        return sum(int(m[2].summary()["nonfinite_values"]) for m in members)
+
+    out = {}
+    for mode in ("batched", "streams", "eager"):
+        members, surs = fresh()
+        states = [m[2] for m in members]
+        cells = sum(m[3] for m in members)
+        if mode == "eager":
+            def run_steps(n):
+                for _ in range(n):
+                    for i, st in enumerate(states):
+                        if surs is None:
+                            st.moist_step(cfg, wr)
+                        else:
+                            st.moist_step_surrogate(cfg, wr, surs[i])
+        else:
+            def run_steps(n):
+                G.ensemble_steps(states, cfg, wr, n, surs, streams=(mode == "streams"))
+        # warm-up: allocation, per-member tuning, and the batched form's
+        # autotuner (one timed step pair per level-chunk candidate)
+        run_steps(9)
+        finish(members)
+        t0 = time.perf_counter()
+        run_steps(a.ensemble_steps)
+        bad = finish(members)
+        secs = time.perf_counter() - t0
+        out[mode] = {"value": cells * a.ensemble_steps / secs if bad == 0 else None,
+                     "state_finite": bad == 0, "nonfinite_values": bad}
+        shapes = [list(map(int, (m[2].grid.nx, m[2].grid.ny, m[2].grid.nz))) for m in members]
+        del states, members, surs
+    finite = all(v["state_finite"] for v in out.values())
     return {
-        "members": len(members), "total_cells": cells, "dt": dt,
-        "shapes": [list(map(int, (m[2].grid.nx, m[2].grid.ny, m[2].grid.nz))) for m in members],
-        "steps": a.ensemble_steps, "surrogate": surs is not None,
-        "value_graph_concurrent": cells * a.ensemble_steps / t_graph if finite else None,
-        "value_eager_sequential": cells * a.ensemble_steps / t_eager if finite else None,
-        "state_finite": finite, "nonfinite_values": bad,
+        "members": a.ensemble_members, "total_cells": cells, "dt": dt, "shapes": shapes,
+        "steps": a.ensemble_steps, "surrogate": sur_src is not None,
+        "batched_autotuner": G.ensemble_tuning_report().strip().splitlines(),
+        "value_batched": out["batched"]["value"],
+        "value_streams_graphs": out["streams"]["value"],
+        "value_eager_sequential": out["eager"]["value"],
+        "modes": out, "state_finite": finite,
     }
 
 

@@ -16,6 +16,7 @@
 #include <exception>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -353,6 +354,9 @@ struct gwm_state {
     return b;
   }
   Graph graphs[2];
+  // configuration (graph_key) of the last eager warm-up step of a batched
+  // ensemble: every scratch buffer allocated and every kernel tuned for it
+  std::vector<double> batch_key;
   bool faces_synced = false;
   // theta_ref partials of the state (rho_set) that a subcycle's last substep
   // wrote, ready for the next stage's theta_ref (no pass over the state)
@@ -504,6 +508,8 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
                 ctx->prec == GWM_F64 ? "f64" : "f32", variant);
   auto it = ctx->tuned.find(key);
   if (it != ctx->tuned.end()) return it->second;
+  // a recorded step launches nothing, so it cannot time candidates
+  if (gwm::refuse_record("an untuned column kernel")) return 1;
   hipEvent_t e0 = ctx->ev(), e1 = ctx->ev();
   // Three rounds over the candidates, each timing two launches after a warm
   // one; a candidate's time is its fastest round, so clock and cache drift
@@ -3193,6 +3199,324 @@ int ensure_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_con
   return eager;
 }
 
+// ---------------------------------------------------------------------------
+// Batched ensemble: one launch per kernel of the step for all members.
+// Each member's two moist steps are recorded (the launchers append their
+// batched forms, nothing runs; two steps return the buffer cycle to where it
+// started); position i of every member's sequence is the same kernel, so it
+// becomes one launch over the members' work-groups, each member's range
+// starting at a multiple of 8 (the XCD remap inside a member stays the one of
+// its own launch).  The two-step sequence is captured into a graph on the
+// first member's stream and replayed.
+// ---------------------------------------------------------------------------
+struct BatchPlan {
+  struct Op {
+    const void* fn;
+    dim3 grid, block;
+    size_t lds;
+    void* items;
+    gwm::BatchMap map;
+  };
+  std::vector<Op> ops;
+  void* mem = nullptr;
+  hipStream_t stream = nullptr;  // the group's own stream (default priority)
+  hipGraphExec_t exec = nullptr;
+  BatchPlan() = default;
+  BatchPlan(const BatchPlan&) = delete;
+  BatchPlan& operator=(const BatchPlan&) = delete;
+  ~BatchPlan() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (mem) (void)hipFree(mem);
+  }
+};
+
+// Records two moist steps of a member; false if a launcher has no batched
+// form (the recorded steps then cannot be batched).
+template <class R>
+bool record_two_steps(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& w,
+                      gwm_surrogate* M, gwm::Recorder& rec) {
+  const int cur0 = S->cur;
+  void* tr0 = S->tr.empty() ? nullptr : S->tr[0];
+  const bool timing = S->ctx->timing;
+  S->ctx->timing = false;
+  gwm::recorder() = &rec;
+  try {
+    moist_step_any<R>(S, c, w, M);
+    moist_step_any<R>(S, c, w, M);
+  } catch (...) {
+    gwm::recorder() = nullptr;
+    S->ctx->timing = timing;
+    throw;
+  }
+  gwm::recorder() = nullptr;
+  S->ctx->timing = timing;
+  require(S->cur == cur0 && (S->tr.empty() ? nullptr : S->tr[0]) == tr0,
+          "internal: two recorded steps did not return the buffer cycle to its start");
+  return rec.unsupported == nullptr;
+}
+
+// The plan for the members' recorded sequences, or null if they differ.
+std::unique_ptr<BatchPlan> build_plan(const std::vector<gwm::Recorder>& recs) {
+  const size_t nops = recs[0].ops.size();
+  for (const auto& r : recs)
+    if (r.ops.size() != nops) return nullptr;
+  const int n = static_cast<int>(recs.size());
+  require(n < (1 << (32 - gwm::kBatchBlockBits)) - 1, "a batched ensemble takes at most 4094 members");
+  auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+  // layout: per op, items | work-group entries
+  std::vector<size_t> off_items(nops), off_map(nops);
+  std::vector<int> totals(nops);
+  std::vector<std::vector<int>> starts(nops);
+  size_t bytes = 0;
+  for (size_t i = 0; i < nops; ++i) {
+    const gwm::LaunchRecord& r0 = recs[0].ops[i];
+    int at = 0;
+    for (int m = 0; m < n; ++m) {
+      const gwm::LaunchRecord& r = recs[static_cast<size_t>(m)].ops[i];
+      if (r.fn != r0.fn || r.bx != r0.bx || r.by != r0.by || r.bz != r0.bz ||
+          r.item.size() != r0.item.size())
+        return nullptr;
+      require(static_cast<long>(r.gx) * r.gy < (1L << gwm::kBatchBlockBits),
+              "a batched member's grid exceeds 2^20 work-groups");
+      at = (at + 7) & ~7;
+      starts[i].push_back(at);
+      at += r.gx * r.gy;
+    }
+    totals[i] = at;
+    off_items[i] = bytes;
+    bytes = align(bytes + r0.item.size() * static_cast<size_t>(n));
+    off_map[i] = bytes;
+    bytes = align(bytes + sizeof(unsigned) * static_cast<size_t>(at));
+  }
+  std::vector<unsigned char> host(bytes, 0);
+  auto plan = std::make_unique<BatchPlan>();
+  plan->mem = dalloc(std::max<size_t>(bytes, 256));
+  unsigned char* dev = static_cast<unsigned char*>(plan->mem);
+  for (size_t i = 0; i < nops; ++i) {
+    const gwm::LaunchRecord& r0 = recs[0].ops[i];
+    const size_t isz = r0.item.size();
+    size_t lds = 0;
+    std::vector<unsigned> map(static_cast<size_t>(totals[i]), gwm::kNoMember);
+    for (int m = 0; m < n; ++m) {
+      const gwm::LaunchRecord& r = recs[static_cast<size_t>(m)].ops[i];
+      std::memcpy(host.data() + off_items[i] + isz * static_cast<size_t>(m), r.item.data(), isz);
+      lds = std::max(lds, r.lds);
+      const int s0 = starts[i][static_cast<size_t>(m)];
+      for (int b = 0; b < r.gx * r.gy; ++b)
+        map[static_cast<size_t>(s0 + b)] =
+            (static_cast<unsigned>(m) << gwm::kBatchBlockBits) | static_cast<unsigned>(b);
+    }
+    std::memcpy(host.data() + off_map[i], map.data(), sizeof(unsigned) * map.size());
+    BatchPlan::Op op;
+    op.fn = r0.fn;
+    op.grid = dim3(static_cast<unsigned>(totals[i]));
+    op.block = dim3(r0.bx, r0.by, r0.bz);
+    op.lds = lds;
+    op.items = dev + off_items[i];
+    op.map = gwm::BatchMap{reinterpret_cast<const unsigned*>(dev + off_map[i])};
+    plan->ops.push_back(op);
+  }
+  HIPCHK(hipMemcpy(plan->mem, host.data(), bytes, hipMemcpyHostToDevice));
+  return plan;
+}
+
+void launch_plan_ops(const BatchPlan& plan, hipStream_t st) {
+  for (const auto& op : plan.ops) {
+    void* items = op.items;
+    gwm::BatchMap map = op.map;
+    void* args[] = {&items, &map};
+    HIPCHK(hipLaunchKernel(op.fn, op.grid, op.block, args, op.lds, st));
+  }
+}
+
+// Level-chunk counts the column kernels of a batched ensemble are recorded
+// with: 0 keeps each member's own tuned count (chosen for a launch of that
+// member alone), the others apply to every member.  The ensemble's own
+// autotuner times each candidate on real step pairs (the candidates give
+// identical results) and keeps the fastest per ensemble signature.
+constexpr int kEnsChunkCands[4] = {0, 2, 4, 8};
+struct EnsembleTuning {
+  float ms[4] = {-1.f, -1.f, -1.f, -1.f};  // time of a step pair, -1 untested
+};
+std::mutex g_ens_mu;
+std::map<std::string, EnsembleTuning> g_ens_tuning;
+
+template <class R>
+std::string ensemble_signature(gwm_state* const* states, int n, const gwm_step_config& c,
+                               gwm_surrogate* const* surs) {
+  std::string sig = std::to_string(sizeof(R)) + ":" + std::to_string(c.fast_substeps);
+  for (int e = 0; e < n; ++e) {
+    const gwm_state* S = states[e];
+    const Geo& g = S->g();
+    char b[96];
+    std::snprintf(b, sizeof b, "|%dx%dx%d:%d%d%d:%d", g.nx, g.ny, g.nz, S->pert ? 1 : 0,
+                  S->grid->terrain_nonzero ? 1 : 0, (surs && surs[e]) ? 1 : 0, S->ntr);
+    sig += b;
+  }
+  return sig;
+}
+
+// A plan of two batched steps recorded with level-chunk override `chunks`,
+// captured into a graph on a stream of its own; null if the steps cannot be
+// batched.
+template <class R>
+std::unique_ptr<BatchPlan> make_batch_plan(gwm_state* const* states, int n,
+                                           const gwm_step_config& c,
+                                           const gwm_warm_rain_config& w,
+                                           gwm_surrogate* const* surs, int chunks) {
+  std::vector<std::pair<gwm_ctx*, int>> saved;
+  for (int e = 0; e < n; ++e) {
+    gwm_ctx* x = states[e]->ctx;
+    bool seen = false;
+    for (auto& p : saved) seen = seen || p.first == x;
+    if (!seen) saved.push_back({x, x->force_chunks});
+  }
+  auto restore = [&] {
+    for (auto& p : saved) p.first->force_chunks = p.second;
+  };
+  if (chunks > 0)
+    for (auto& p : saved) p.first->force_chunks = chunks;
+  std::vector<gwm::Recorder> recs(static_cast<size_t>(n));
+  bool ok = true;
+  try {
+    for (int e = 0; e < n && ok; ++e)
+      ok = record_two_steps<R>(states[e], c, w, surs ? surs[e] : nullptr,
+                               recs[static_cast<size_t>(e)]);
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+  if (!ok) return nullptr;
+  auto plan = build_plan(recs);
+  if (!plan) return nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(plan->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    launch_plan_ops(*plan, plan->stream);
+  } catch (...) {
+    (void)hipStreamEndCapture(plan->stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(plan->stream, &graph));
+  const hipError_t ge = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(ge);
+  return plan;
+}
+
+// Batched steps of the members (all of precision R); returns the steps every
+// member has taken: nsteps, or fewer when their steps cannot be batched (a
+// launcher without a batched form, or sequences that differ), the caller
+// then running the rest otherwise.  When a member's last warm-up was for
+// another configuration, every member first takes one eager step
+// (allocation and tuning), counted in nsteps.
+template <class R>
+int ensemble_batched(gwm_state* const* states, int n, const gwm_step_config& c,
+                     const gwm_warm_rain_config& w, gwm_surrogate* const* surs, int nsteps) {
+  auto eager = [&](int e) { moist_step_any<R>(states[e], c, w, surs ? surs[e] : nullptr); };
+  bool need_warm = false;
+  for (int e = 0; e < n; ++e) {
+    gwm_state* S = states[e];
+    ensure_synced<R>(S, S->set[S->cur]);
+    need_warm = need_warm || S->batch_key != graph_key(c, w, surs ? surs[e] : nullptr);
+  }
+  int left = nsteps;
+  if (need_warm && left > 0) {
+    for (int e = 0; e < n; ++e) {
+      eager(e);
+      states[e]->batch_key = graph_key(c, w, surs ? surs[e] : nullptr);
+    }
+    --left;
+  }
+  if (left >= 2) {
+    const std::string sig = ensemble_signature<R>(states, n, c, surs);
+    EnsembleTuning tune;
+    {
+      std::lock_guard<std::mutex> lock(g_ens_mu);
+      tune = g_ens_tuning[sig];
+    }
+    std::unique_ptr<BatchPlan> plans[4];
+    auto plan_for = [&](int ci) -> BatchPlan* {
+      if (!plans[ci]) plans[ci] = make_batch_plan<R>(states, n, c, w, surs, kEnsChunkCands[ci]);
+      return plans[ci].get();
+    };
+    // every member's queued work first
+    for (int e = 0; e < n; ++e) HIPCHK(hipStreamSynchronize(states[e]->ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    BatchPlan* last = nullptr;
+    try {
+      while (left >= 2) {
+        int ci = -1;
+        for (int q = 0; q < 4 && ci < 0; ++q)
+          if (tune.ms[q] < 0.f) ci = q;
+        if (ci >= 0) {  // a candidate not timed yet: one timed step pair
+          BatchPlan* p = plan_for(ci);
+          if (!p) break;  // the steps cannot be batched
+          if (last && last != p) HIPCHK(hipStreamSynchronize(last->stream));
+          HIPCHK(hipEventRecord(e0, p->stream));
+          HIPCHK(hipGraphLaunch(p->exec, p->stream));
+          HIPCHK(hipEventRecord(e1, p->stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          tune.ms[ci] = ms;
+          last = p;
+        } else {
+          int best = 0;
+          for (int q = 1; q < 4; ++q)
+            if (tune.ms[q] < tune.ms[best]) best = q;
+          BatchPlan* p = plan_for(best);
+          if (!p) break;
+          if (last && last != p) HIPCHK(hipStreamSynchronize(last->stream));
+          HIPCHK(hipGraphLaunch(p->exec, p->stream));
+          last = p;
+        }
+        left -= 2;
+      }
+      if (last) HIPCHK(hipStreamSynchronize(last->stream));
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+      std::lock_guard<std::mutex> lock(g_ens_mu);
+      g_ens_tuning[sig] = tune;
+    }
+    if (left >= 2) return nsteps - left;  // not batchable: the caller runs the rest
+  }
+  for (int e = 0; e < n; ++e)
+    for (int q = 0; q < left; ++q) eager(e);
+  return nsteps;
+}
+
+// The batched ensemble autotuner's times per signature, for reports.
+std::string ensemble_tuning_report() {
+  std::lock_guard<std::mutex> lock(g_ens_mu);
+  std::string out;
+  for (const auto& kv : g_ens_tuning) {
+    char b[160];
+    std::snprintf(b, sizeof b, "%zu members:", static_cast<size_t>(std::count(
+                                                   kv.first.begin(), kv.first.end(), '|')));
+    out += b;
+    for (int q = 0; q < 4; ++q) {
+      std::snprintf(b, sizeof b, " chunks %d %.3f ms", kEnsChunkCands[q], kv.second.ms[q]);
+      out += b;
+    }
+    out += "\n";
+  }
+  return out;
+}
+
 template <class R>
 void moist_steps_graph(gwm_state* S, const gwm_step_config& c, const gwm_warm_rain_config& w,
                        gwm_surrogate* M, int nsteps) {
@@ -3230,62 +3554,106 @@ int gwm_moist_steps_graph(gwm_state* s, const gwm_step_config* c, const gwm_warm
   });
 }
 
+namespace {
+
+// Members one context (stream) each, step pairs replayed from each member's
+// captured graph, interleaved across members.
+void ensemble_streams(gwm_state* const* states, int n, const gwm_step_config* c,
+                      const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
+                      int nsteps) {
+  for (int e = 0; e < n; ++e)
+    for (int f = 0; f < e; ++f)
+      require(states[f]->ctx != states[e]->ctx,
+              "ensemble members stepped on their own streams need a context each");
+  std::vector<int> done(static_cast<size_t>(n), 0);
+  for (int e = 0; e < n; ++e)
+    dispatch(states[e]->ctx, [&](auto r) {
+      ensure_synced<decltype(r)>(states[e], states[e]->set[states[e]->cur]);
+      return 0;
+    });
+  bool more = true;
+  while (more) {
+    more = false;
+    for (int e = 0; e < n; ++e) {
+      gwm_state* S = states[e];
+      gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
+      int& d = done[static_cast<size_t>(e)];
+      if (nsteps - d < 2) continue;
+      dispatch(S->ctx, [&](auto r) {
+        using R = decltype(r);
+        gwm_state::Graph* gr = nullptr;
+        d += ensure_graph<R>(S, *c, *wr, M, &gr);
+        if (nsteps - d >= 2) {
+          HIPCHK(hipGraphLaunch(gr->exec, S->ctx->stream));
+          d += 2;
+        }
+        return 0;
+      });
+      if (nsteps - d >= 2) more = true;
+    }
+  }
+  for (int e = 0; e < n; ++e) {
+    gwm_state* S = states[e];
+    gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
+    while (done[static_cast<size_t>(e)] < nsteps) {
+      dispatch(S->ctx, [&](auto r) {
+        moist_step_any<decltype(r)>(S, *c, *wr, M);
+        return 0;
+      });
+      ++done[static_cast<size_t>(e)];
+    }
+  }
+  for (int e = 0; e < n; ++e) HIPCHK(hipStreamSynchronize(states[e]->ctx->stream));
+}
+
+void check_members(gwm_state* const* states, int n, const gwm_step_config* c,
+                   const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
+                   int nsteps) {
+  require(states && c && wr && n >= 1, "null argument");
+  require(nsteps >= 0, "nsteps must be >= 0");
+  for (int e = 0; e < n; ++e) {
+    S_(states[e]);
+    require(!states[e]->view, "not on a stage view");
+    for (int f = 0; f < e; ++f) require(states[f] != states[e], "a member appears twice");
+    if (surrogates && surrogates[e])
+      require(surrogates[e]->ctx == states[e]->ctx,
+              "surrogate and state belong to different contexts");
+  }
+}
+
+}  // namespace
+
 int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* c,
                        const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
                        int nsteps) {
   return guard([&] {
-    require(states && c && wr && n >= 1, "null argument");
-    require(nsteps >= 0, "nsteps must be >= 0");
-    for (int e = 0; e < n; ++e) {
-      S_(states[e]);
-      for (int f = 0; f < e; ++f)
-        require(states[f]->ctx != states[e]->ctx,
-                "ensemble members need a context (stream) each");
-      if (surrogates && surrogates[e])
-        require(surrogates[e]->ctx == states[e]->ctx,
-                "surrogate and state belong to different contexts");
-    }
-    // Members run concurrently, each on its own stream: step pairs are
-    // replayed from each member's captured graph, interleaved across members.
-    std::vector<int> done(static_cast<size_t>(n), 0);
-    for (int e = 0; e < n; ++e)
-      dispatch(states[e]->ctx, [&](auto r) {
-        ensure_synced<decltype(r)>(states[e], states[e]->set[states[e]->cur]);
-        return 0;
+    check_members(states, n, c, wr, surrogates, nsteps);
+    bool same = true;
+    for (int e = 1; e < n; ++e)
+      same = same && states[e]->ctx->prec == states[0]->ctx->prec &&
+             states[e]->ctx->device == states[0]->ctx->device;
+    int done = 0;
+    if (same)
+      done = dispatch(states[0]->ctx, [&](auto r) {
+        return ensemble_batched<decltype(r)>(states, n, *c, *wr, surrogates, nsteps);
       });
-    bool more = true;
-    while (more) {
-      more = false;
-      for (int e = 0; e < n; ++e) {
-        gwm_state* S = states[e];
-        gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
-        int& d = done[static_cast<size_t>(e)];
-        if (nsteps - d < 2) continue;
-        dispatch(S->ctx, [&](auto r) {
-          using R = decltype(r);
-          gwm_state::Graph* gr = nullptr;
-          d += ensure_graph<R>(S, *c, *wr, M, &gr);
-          if (nsteps - d >= 2) {
-            HIPCHK(hipGraphLaunch(gr->exec, S->ctx->stream));
-            d += 2;
-          }
-          return 0;
-        });
-        if (nsteps - d >= 2) more = true;
-      }
-    }
-    for (int e = 0; e < n; ++e) {
-      gwm_state* S = states[e];
-      gwm_surrogate* M = surrogates ? surrogates[e] : nullptr;
-      while (done[static_cast<size_t>(e)] < nsteps) {
-        dispatch(S->ctx, [&](auto r) {
-          moist_step_any<decltype(r)>(S, *c, *wr, M);
-          return 0;
-        });
-        ++done[static_cast<size_t>(e)];
-      }
-    }
-    for (int e = 0; e < n; ++e) HIPCHK(hipStreamSynchronize(states[e]->ctx->stream));
+    if (done < nsteps) ensemble_streams(states, n, c, wr, surrogates, nsteps - done);
+  });
+}
+
+int gwm_ensemble_tuning_report(char* buf, size_t size) {
+  return guard([&] {
+    require(buf && size > 0, "null argument");
+    std::snprintf(buf, size, "%s", ensemble_tuning_report().c_str());
+  });
+}
+
+int gwm_ensemble_steps_streams(gwm_state* const* states, int n, const gwm_step_config* c,
+                               const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
+                               int nsteps) {
+  return guard([&] {
+    check_members(states, n, c, wr, surrogates, nsteps);
+    ensemble_streams(states, n, c, wr, surrogates, nsteps);
   });
 }
 

@@ -53,8 +53,8 @@ enum { LV_PREF = 0, LV_RREF, LV_IDZF, LV_IDZC, LV_BRHO, LV_BTH, LV_BRT, LV_BP, N
 
 // KIND: bit 0 first substep of the subcycle, bit 1 last substep.
 template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
-__global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
-                                                         int nkc, double* part_out) {
+__device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int tiles, int nkc,
+                                          double* part_out, const VBlock& vb) {
   constexpr bool FIRST = (KIND & 1) != 0, LAST = (KIND & 2) != 0;
   __shared__ R sp[4][PN];
   extern __shared__ __align__(16) unsigned char lev_raw[];
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
 
   // XCD-aware order (work-group b runs on XCD b % 8), then (chunk, tile).
   const int nb = tiles * nkc;
-  int b = blockIdx.x;
+  int b = vb.x;
   if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
   const int chunk = b / tiles, tile = b % tiles;
   const int k0 = (nz * chunk) / nkc, k1 = (nz * (chunk + 1)) / nkc;
@@ -366,8 +366,8 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
       if (lx == TX - 1) {
         const size_t nw = static_cast<size_t>(tiles) * TY;  // one wave per tile row
         const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * TY + ly) * 2;
-        part_out[slot] = sr;
-        part_out[slot + 1] = spp;
+        gstore(part_out, slot, sr);
+        gstore(part_out, slot + 1, spp);
       }
     }
 
@@ -387,10 +387,36 @@ __global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tile
   }
 }
 
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
+__global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
+                                                         int nkc, double* part_out) {
+  fast_body<R, TERRAIN, COLREF, PERT, KIND>(a, tiles_x, tiles, nkc, part_out, hw_block());
+}
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
+__global__ __launch_bounds__(TX * TY) void k_fast_column_batch(
+    const FastItem<R>* __restrict__ items, BatchMap bm) {
+  VBlock vb;
+  const FastItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  // the member's arguments, read once before the kernel stores anything
+  const FastArgs<R> a = it->a;
+  fast_body<R, TERRAIN, COLREF, PERT, KIND>(a, it->tiles_x, it->tiles, it->nkc, it->part_out, vb);
+}
+
 template <class R, bool TERRAIN, bool COLREF, bool PERT>
 void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, dim3 grid, dim3 blk, size_t lds,
                       int tx, int tiles, int nkc, double* part_out) {
   const int kind = (a.first ? 1 : 0) | (a.last ? 2 : 0);
+  if (recorder()) {
+    const void* fns[4] = {
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 0>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 1>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 2>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 3>)};
+    record_launch(fns[kind], blk, lds, FastItem<R>{a, tx, tiles, nkc, part_out, 0, 0},
+                  static_cast<int>(grid.x), 1);
+    return;
+  }
   switch (kind) {
     case 0:
       hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0>), grid, blk, lds, st, a, tx,

@@ -304,21 +304,26 @@ struct DMetrics {
 template <class R>
 struct Loader {
   Geo g;
+  // element of a field in device (global) memory
+  __device__ static R at(const R* f, size_t idx) {
+    typedef const __attribute__((address_space(1))) R gR;
+    return ((gR*)f)[idx];
+  }
   __device__ R cell(const R* f, int i, int j, int k) const {
     const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.cell(si, sj, k)];
+    return (si < 0 || sj < 0) ? R(0) : at(f, g.cell(si, sj, k));
   }
   __device__ R xf(const R* f, int i, int j, int k) const {
     const int si = map_face(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.xf(si, sj, k)];
+    return (si < 0 || sj < 0) ? R(0) : at(f, g.xf(si, sj, k));
   }
   __device__ R yf(const R* f, int i, int j, int k) const {
     const int si = map_cell(i, g.nx, g.per_x), sj = map_face(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.yf(si, sj, k)];
+    return (si < 0 || sj < 0) ? R(0) : at(f, g.yf(si, sj, k));
   }
   __device__ R zf(const R* f, int i, int j, int k) const {
     const int si = map_cell(i, g.nx, g.per_x), sj = map_cell(j, g.ny, g.per_y);
-    return (si < 0 || sj < 0) ? R(0) : f[g.zf(si, sj, k)];
+    return (si < 0 || sj < 0) ? R(0) : at(f, g.zf(si, sj, k));
   }
   // Metric column index with the reference's periodic wrap
   // (dry_fast_modes.cu:24-30); callers only pass in-range columns when the
@@ -327,14 +332,54 @@ struct Loader {
   __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
 };
 
+// The work-group index and grid extent a kernel body works with.  A plain
+// launch passes the hardware's; a batched launch (several states of an
+// ensemble in one grid) passes each member's own, its blocks being a
+// contiguous range of the launch that starts at a multiple of 8, so that
+// block b of a member runs on XCD b mod 8 as in a launch of its own.
+struct VBlock {
+  int x, y;    // work-group index
+  int gx, gy;  // grid extent
+};
+__device__ inline VBlock hw_block() {
+  return VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y),
+                static_cast<int>(gridDim.x), static_cast<int>(gridDim.y)};
+}
+
+// Member lookup of a batched launch: entry[b] of work-group b holds its
+// member (high 12 bits) and its index within the member's own grid (low 20
+// bits), one load; kNoMember marks the padding up to the next member.
+constexpr unsigned kNoMember = 0xffffffffu;
+constexpr int kBatchBlockBits = 20;
+struct BatchMap {
+  const unsigned* entry;
+};
+// Items carry the member's grid extent in fields gx, gy.  Returns null for a
+// padding work-group (uniform over it: the whole work-group returns).
+template <class Item>
+__device__ inline const Item* batch_member(const Item* items, const BatchMap& bm, VBlock& vb) {
+  // uniform over the work-group: kept in a scalar register, so the item's
+  // fields are scalar loads (the vector-load counter then only tracks the
+  // kernel's own loads, which stay in flight across its waits)
+  const unsigned e = __builtin_amdgcn_readfirstlane(bm.entry[blockIdx.x]);
+  if (e == kNoMember) return nullptr;
+  const Item* it = items + (e >> kBatchBlockBits);
+  const int b = static_cast<int>(e & ((1u << kBatchBlockBits) - 1u));
+  vb.gx = it->gx;
+  vb.gy = it->gy;
+  vb.x = b % vb.gx;
+  vb.y = b / vb.gx;
+  return it;
+}
+
 // Work-group (bx, by) of a 2-D grid whose work-groups (x fastest) run on XCD
 // b mod 8, remapped so that each XCD holds a contiguous band of them.  Rows
 // start 4 B past a cache line, so work-groups that are neighbours in x share
 // the lines at their ends; on one XCD a shared line is fetched into one L2.
-__device__ inline void xcd_block2d(int& bx, int& by) {
-  bx = blockIdx.x;
-  by = blockIdx.y;
-  const int gx = gridDim.x, nb = gx * gridDim.y;
+__device__ inline void xcd_block2d(const VBlock& v, int& bx, int& by) {
+  bx = v.x;
+  by = v.y;
+  const int gx = v.gx, nb = gx * v.gy;
   if ((nb & 7) == 0) {
     const int b = bx + gx * by;
     const int r = (b & 7) * (nb >> 3) + (b >> 3);
@@ -342,6 +387,7 @@ __device__ inline void xcd_block2d(int& bx, int& by) {
     by = r / gx;
   }
 }
+__device__ inline void xcd_block2d(int& bx, int& by) { xcd_block2d(hw_block(), bx, by); }
 
 // One DPP move of a double (both halves); lanes of rows outside ROWS hold
 // unspecified values.
@@ -394,12 +440,24 @@ __device__ inline StagePos stage_pos(const Geo& g, int ii, int jj) {
 // stores instead of 64-bit address arithmetic per access.
 template <class T>
 __device__ inline T ld32(const T* f, unsigned idx) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) +
-                                     idx * static_cast<unsigned>(sizeof(T)));
+  // Every field lives in device (global) memory: the access is issued as a
+  // global one also where the pointer came from memory (the argument items
+  // of a batched launch), which the compiler would otherwise issue as flat.
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) T gT;
+  return *(gT*)((gchar*)f + idx * static_cast<unsigned>(sizeof(T)));
+}
+// The same with a 64-bit element index.
+template <class T>
+__device__ inline void gstore(T* f, size_t idx, T v) {
+  typedef __attribute__((address_space(1))) T gT;
+  ((gT*)f)[idx] = v;
 }
 template <class T>
 __device__ inline void st32(T* f, unsigned idx, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(f) + idx * static_cast<unsigned>(sizeof(T))) = v;
+  typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) T gT;
+  *(gT*)((gchar*)f + idx * static_cast<unsigned>(sizeof(T))) = v;
 }
 
 // Value at level-0 offset off of level k (level stride `plane`), 0 for off < 0.

@@ -43,13 +43,13 @@ __device__ inline double block_sum(double v, double* sh) {
 constexpr int kPartialRows = 8;  // rows of one level per work-group
 
 template <class R>
-__global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, const R* aux,
-                                                         int mode, double* partials,
-                                                         Base<R> base) {
+__device__ __forceinline__ void level_partials_body(const Geo& g, const R* rho, const R* aux,
+                                                    int mode, double* partials,
+                                                    const Base<R>& base, const VBlock& vb) {
   __shared__ double sh[256];
-  const int k = blockIdx.y;
-  const int c = blockIdx.x;
-  const int nchunk = gridDim.x;
+  const int k = vb.y;
+  const int c = vb.x;
+  const int nchunk = vb.gx;
   const int j0 = c * kPartialRows;
   const int t = threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
@@ -100,6 +100,22 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
   }
 }
 
+template <class R>
+__global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, const R* aux,
+                                                         int mode, double* partials,
+                                                         Base<R> base) {
+  level_partials_body(g, rho, aux, mode, partials, base, hw_block());
+}
+template <class R>
+__global__ __launch_bounds__(256) void k_level_partials_batch(const PartialsItem<R>* items,
+                                                               BatchMap bm) {
+  VBlock vb;
+  const PartialsItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const PartialsItem<R> a = *it;
+  level_partials_body(a.g, a.rho, a.aux, a.mode, a.partials, a.base, vb);
+}
+
 // One 256-thread block per level: thread t sums partials t, t+256, ... in
 // order over eight interleaved accumulators (eight loads in flight per
 // round: the kernel is bound by load latency, not bytes), a fixed DPP
@@ -110,12 +126,11 @@ __global__ __launch_bounds__(256) void k_level_partials(Geo g, const R* rho, con
 constexpr int kFinalizeThreads = 256;
 
 template <class R>
-__global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize(int nz, int nchunk,
-                                                                     const double* partials,
-                                                                     int mode, double count,
-                                                                     R* out0, R* out1) {
+__device__ __forceinline__ void level_finalize_body(int nchunk, const double* partials, int mode,
+                                                    double count, R* out0, R* out1,
+                                                    const VBlock& vb) {
   __shared__ double sh[2][kFinalizeThreads / 64];
-  const int k = blockIdx.x;
+  const int k = vb.x;
   const int t = threadIdx.x;
   const double2* p = reinterpret_cast<const double2*>(partials) + static_cast<size_t>(k) * nchunk;
   double a0[8], a1[8];
@@ -159,6 +174,24 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize(int nz, int
     out0[k] = static_cast<R>(S0) * inv;
     out1[k] = static_cast<R>(S1) * inv;
   }
+}
+
+template <class R>
+__global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize(int nz, int nchunk,
+                                                                     const double* partials,
+                                                                     int mode, double count,
+                                                                     R* out0, R* out1) {
+  (void)nz;
+  level_finalize_body(nchunk, partials, mode, count, out0, out1, hw_block());
+}
+template <class R>
+__global__ __launch_bounds__(kFinalizeThreads) void k_level_finalize_batch(
+    const FinalizeItem<R>* items, BatchMap bm) {
+  VBlock vb;
+  const FinalizeItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const FinalizeItem<R> a = *it;
+  level_finalize_body(a.nchunk, a.partials, a.mode, a.count, a.out0, a.out1, vb);
 }
 
 // ===========================================================================
@@ -206,10 +239,10 @@ __global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt, Base<R> ba
 // the post-microphysics rain exactly like the reference's qr_old copy.
 // ===========================================================================
 template <class R, bool TERRAIN>
-__global__ void k_warm_rain(WarmRainArgs<R> a) {
+__device__ __forceinline__ void warm_rain_body(const WarmRainArgs<R>& a, const VBlock& vb) {
   const Geo g = a.g;
   int bx, by;
-  xcd_block2d(bx, by);
+  xcd_block2d(vb, bx, by);
   const int i = bx * blockDim.x + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -272,6 +305,19 @@ __global__ void k_warm_rain(WarmRainArgs<R> a) {
     c -= plane;
   }
   if (nominal || metric) a.precip[static_cast<size_t>(j) * g.nx + i] += inc;
+}
+
+template <class R, bool TERRAIN>
+__global__ void k_warm_rain(WarmRainArgs<R> a) {
+  warm_rain_body<R, TERRAIN>(a, hw_block());
+}
+template <class R, bool TERRAIN>
+__global__ void k_warm_rain_batch(const WarmRainItem<R>* items, BatchMap bm) {
+  VBlock vb;
+  const WarmRainItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const WarmRainArgs<R> a = it->a;
+  warm_rain_body<R, TERRAIN>(a, vb);
 }
 
 // ===========================================================================
@@ -605,30 +651,44 @@ template <class R>
 void launch_level_partials(hipStream_t st, const Geo& g, const R* rho, const R* aux, int mode,
                            double* partials, int* nchunk_out, Base<R> base) {
   const int nchunk = level_partials_chunks(g.ny);
+  *nchunk_out = nchunk;
+  if (recorder()) {
+    record_launch(reinterpret_cast<const void*>(&k_level_partials_batch<R>), dim3(256), 0,
+                  PartialsItem<R>{g, rho, aux, mode, partials, base, 0, 0}, nchunk, g.nz);
+    return;
+  }
   hipLaunchKernelGGL(k_level_partials<R>, dim3(nchunk, g.nz), dim3(256), 0, st, g, rho, aux, mode,
                      partials, base);
-  *nchunk_out = nchunk;
 }
 int level_partials_chunks(int ny) { return (ny + kPartialRows - 1) / kPartialRows; }
 template <class R>
 void launch_level_finalize(hipStream_t st, int nz, int nchunk, const double* partials, int mode,
                            double count, R* o0, R* o1) {
+  if (recorder()) {
+    record_launch(reinterpret_cast<const void*>(&k_level_finalize_batch<R>),
+                  dim3(kFinalizeThreads), 0,
+                  FinalizeItem<R>{nz, nchunk, partials, mode, count, o0, o1, 0, 0}, nz, 1);
+    return;
+  }
   hipLaunchKernelGGL(k_level_finalize<R>, dim3(nz), dim3(kFinalizeThreads), 0, st, nz, nchunk, partials,
                      mode, count, o0, o1);
 }
 template <class R>
 void launch_pressure(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* p) {
+  if (refuse_record("pressure")) return;
   hipLaunchKernelGGL(k_pressure<R>, dim3(blocks_for(g.cell_total(), 256)), dim3(256), 0, st, g,
                      rho, rt, p);
 }
 template <class R>
 void launch_theta(hipStream_t st, const Geo& g, const R* rho, const R* rt, R* th, Base<R> base) {
+  if (refuse_record("theta")) return;
   const dim3 b(64, 4, 1);
   hipLaunchKernelGGL(k_theta<R>, cell_grid(g, b), b, 0, st, g, rho, rt, th, base);
 }
 template <class R>
 void launch_rebuild_rt(hipStream_t st, const Geo& g, const R* rho, const R* th, R* rt,
                        Base<R> base) {
+  if (refuse_record("rebuild_rt")) return;
   const dim3 b(64, 4, 1);
   hipLaunchKernelGGL(k_rebuild_rt<R>, cell_grid(g, b), b, 0, st, g, rho, th, rt, base);
 }
@@ -636,6 +696,13 @@ template <class R>
 void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain) {
   const dim3 b(64, 4, 1);
   const dim3 gr((a.g.nx + 63) / 64, (a.g.ny + 3) / 4);
+  if (recorder()) {
+    const void* fn = terrain ? reinterpret_cast<const void*>(&k_warm_rain_batch<R, true>)
+                             : reinterpret_cast<const void*>(&k_warm_rain_batch<R, false>);
+    record_launch(fn, b, 0, WarmRainItem<R>{a, 0, 0}, static_cast<int>(gr.x),
+                  static_cast<int>(gr.y));
+    return;
+  }
   if (terrain)
     hipLaunchKernelGGL((k_warm_rain<R, true>), gr, b, 0, st, a);
   else
@@ -643,12 +710,14 @@ void launch_warm_rain(hipStream_t st, const WarmRainArgs<R>& a, bool terrain) {
 }
 template <class R>
 void launch_halo_fill(hipStream_t st, const Geo& g, int orient, R* f) {
+  if (refuse_record("halo_fill")) return;
   const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
   const size_t n = static_cast<size_t>(fx + 2 * g.h) * (fy + 2 * g.h) * fz;
   hipLaunchKernelGGL(k_halo_fill<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, g, orient, f);
 }
 template <class R>
 void launch_sync_interfaces(hipStream_t st, const Geo& g, int orient, R* f) {
+  if (refuse_record("sync_interfaces")) return;
   const int n = orient == 1 ? g.ny * g.nz : g.nx * g.nz;
   hipLaunchKernelGGL(k_sync_interfaces<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, g, orient, f);
 }
@@ -661,13 +730,20 @@ void launch_exchange_layout(hipStream_t st, int phase, int orient, int nranks, c
 template <class R>
 void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* ref, int mode,
                      double width) {
+  if (refuse_record("boundary")) return;
   const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
   hipLaunchKernelGGL(k_boundary<R>, dim3(blocks_for(static_cast<size_t>(fx) * fy * fz, 256)),
                      dim3(256), 0, st, g, orient, f, ref, mode, static_cast<R>(width));
 }
 template <class R>
 void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks) {
+  if (refuse_record("reduce")) return;
   hipLaunchKernelGGL(k_reduce<R>, dim3(nblocks), dim3(256), 0, st, a);
+}
+
+Recorder*& recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
 }
 
 // Block partials of the allocation-free budget entry point
@@ -704,10 +780,12 @@ void launch_fp32_math(hipStream_t st, int fn, const float* x, float* y, size_t n
 }
 template <class R>
 void launch_copy(hipStream_t st, const R* src, R* dst, size_t n) {
+  if (refuse_record("copy")) return;
   hipLaunchKernelGGL(k_copy<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, n);
 }
 template <class R>
 void launch_fill(hipStream_t st, R* dst, size_t n, R v) {
+  if (refuse_record("fill")) return;
   hipLaunchKernelGGL(k_fill<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, n, v);
 }
 

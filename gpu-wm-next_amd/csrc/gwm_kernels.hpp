@@ -1,6 +1,9 @@
 // gwm_kernels.hpp -- kernel argument blocks and launcher declarations.
 #pragma once
 
+#include <cstring>
+#include <vector>
+
 #include "gwm_internal.hpp"
 
 namespace gwm {
@@ -187,5 +190,97 @@ template <class R>
 void launch_copy(hipStream_t st, const R* src, R* dst, size_t n);
 template <class R>
 void launch_fill(hipStream_t st, R* dst, size_t n, R v);
+
+// ---------------------------------------------------------------------------
+// Batched launches: the step of several states (ensemble members of any
+// sizes) as one launch per kernel of the step.  While a Recorder is set on
+// the calling host thread, the step kernels' launchers launch nothing: each
+// appends its launch (the batched form of its kernel, block shape, dynamic
+// LDS, the member's grid and argument item) instead.  Recording one member's
+// steps gives the same sequence of kernels as any other member's with the
+// same configuration; run_batched launches each position of the sequence
+// once over all members' work-groups.
+// ---------------------------------------------------------------------------
+struct LaunchRecord {
+  const void* fn = nullptr;  // batched kernel: (const Item* items, BatchMap map)
+  unsigned bx = 1, by = 1, bz = 1;
+  size_t lds = 0;
+  int gx = 1, gy = 1;        // the member's grid
+  std::vector<unsigned char> item;
+};
+struct Recorder {
+  std::vector<LaunchRecord> ops;
+  const char* unsupported = nullptr;  // a launcher without a batched form was called
+};
+// The calling thread's recorder (null: launchers launch).
+Recorder*& recorder();
+template <class Item>
+inline void record_launch(const void* fn, dim3 block, size_t lds, Item it, int gx, int gy) {
+  it.gx = gx;
+  it.gy = gy;
+  LaunchRecord r;
+  r.fn = fn;
+  r.bx = block.x;
+  r.by = block.y;
+  r.bz = block.z;
+  r.lds = lds;
+  r.gx = gx;
+  r.gy = gy;
+  r.item.resize(sizeof(Item));
+  std::memcpy(r.item.data(), &it, sizeof(Item));
+  recorder()->ops.push_back(std::move(r));
+}
+// Marks the recording unusable (called by launchers with no batched form).
+inline bool refuse_record(const char* what) {
+  Recorder* r = recorder();
+  if (!r) return false;
+  if (!r->unsupported) r->unsupported = what;
+  return true;
+}
+
+template <class R>
+struct SlowItem {
+  SlowArgs<R> a;
+  int tiles_x, tiles, nkc;
+  int gx, gy;
+};
+template <class R>
+struct FastItem {
+  FastArgs<R> a;
+  int tiles_x, tiles, nkc;
+  double* part_out;
+  int gx, gy;
+};
+template <class R>
+struct TracerItem {
+  TracerArgs<R> a;
+  int tiles_x, tiles, nkc;
+  int gx, gy;
+};
+template <class R>
+struct WarmRainItem {
+  WarmRainArgs<R> a;
+  int gx, gy;
+};
+template <class R>
+struct PartialsItem {
+  Geo g;
+  const R* rho;
+  const R* aux;
+  int mode;
+  double* partials;
+  Base<R> base;
+  int gx, gy;
+};
+template <class R>
+struct FinalizeItem {
+  int nz, nchunk;
+  const double* partials;
+  int mode;
+  double count;
+  R* out0;
+  R* out1;
+  int gx, gy;
+};
 
 }  // namespace gwm

@@ -16,8 +16,9 @@
 // quantities every flux needs -- theta, p (dry_thermo.hpp:21-33) and the cell
 // velocities (dry_momentum_flux.cu:36-73) -- into LDS once, instead of
 // recomputing each of them at every use (a cell's p and velocities enter up
-// to a dozen face fluxes).  Three LDS slots per quantity hold levels k-1 ..
-// k+1 (level k+1 staged at the top of iteration k); two barriers per level.
+// to a dozen face fluxes).  Four LDS slots per quantity hold levels k-1 ..
+// k+1 and the one being replaced (level k+1 is staged at the top of
+// iteration k); one barrier per level.
 // The column is split into level chunks (count chosen by the autotuner) so
 // the chip holds more independent work-groups; a chunk stages the level
 // below its first one.
@@ -38,9 +39,9 @@ constexpr int NHALO = PN - TX * TY;
 
 // Work-group -> (tile, level chunk [k0, k1)).  Work-group b runs on XCD
 // b % 8; the remap gives each XCD a contiguous band of tiles of one chunk.
-__device__ inline int tile_of(int tiles, int nkc, int nz, int& b, int& k0, int& k1) {
+__device__ inline int tile_of(int blk, int tiles, int nkc, int nz, int& b, int& k0, int& k1) {
   const int nb = tiles * nkc;
-  b = blockIdx.x;
+  b = blk;
   if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
   const int chunk = b / tiles;
   k0 = (nz * chunk) / nkc;
@@ -61,16 +62,17 @@ __device__ inline R upwind(R a, R ql, R qr) {  // dry_momentum_flux.cu:75-77
 enum { SL_TREF = 0, SL_IDZF, SL_IDZC, SL_BRHO, SL_BTH, SL_BRT, SL_BP, SL_N };
 
 template <class R, bool TERRAIN, bool PERT, int MODE>
-__global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
-                                                     int nkc) {
+__device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int tiles, int nkc,
+                                          const VBlock& vb) {
   // FP32 also stages the raw face momenta of each level (tile plus ring), so
   // the upwind values of every flux come from LDS; FP64 reads them from
   // global memory (its LDS budget would halve the resident work-groups).
   //
-  // LDS holds three level slots (k-1, k, k+1; slot kk % 3) and each level
-  // takes two barriers: one before level k+1 overwrites the slot of k-2, one
-  // before the reads.  A fourth slot would save the first barrier but cost a
-  // quarter more LDS, and with it a resident work-group per CU.
+  // LDS holds four level slots (slot kk % 4): level k+1 is staged into the
+  // slot of k-3, which every wave finished reading before the previous
+  // level's barrier, so each level takes one barrier (before the reads).
+  // The fourth slot costs a third more LDS, 52 KB per work-group in FP32;
+  // three work-groups per CU still fit, and the registers allow no more.
   //
   // FP32 keeps the staged quantities of one position together, (quantity,
   // slot) innermost, with an odd stride between positions (no LDS bank
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   // per slot.  FP64 keeps one plane per quantity and slot (its register
   // budget is the tighter one, and the interleaved form costs it registers).
   constexpr bool RAW = sizeof(R) == 4;
-  constexpr int NSL = 3;
+  constexpr int NSL = 4;
   constexpr int NQ = RAW ? 8 : 5;  // theta, p, uc, vc, wc (+ raw u, v, w)
   constexpr int QS = (NQ * NSL) | 1;
   enum { Q_TH = 0, Q_P, Q_UC, Q_VC, Q_WC, Q_U, Q_V, Q_W };
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   const DState<R> s = a.s;
 
   int b, k0, k1;
-  const int tile = tile_of(tiles, nkc, g.nz, b, k0, k1);
+  const int tile = tile_of(vb.x, tiles, nkc, g.nz, b, k0, k1);
   (void)b;
   const int ti0 = (tile % tiles_x) * TX, tj0 = (tile / tiles_x) * TY;
   const int lx = threadIdx.x, ly = threadIdx.y;
@@ -173,20 +175,20 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
     if (pert) {  // theta' and p' of the departures
       const R rb = LV(kk, SL_BRHO);
       rr = rmax(rb + q.r, R(1.0e-6));
-      lds(Q_TH, kk % NSL, e) = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
-      lds(Q_P, kk % NSL, e) = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
+      lds(Q_TH, kk & (NSL - 1), e) = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
+      lds(Q_P, kk & (NSL - 1), e) = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
     } else {
       rr = rmax(q.r, R(1.0e-6));
-      lds(Q_TH, kk % NSL, e) = q_theta(q.r, q.rt);
-      lds(Q_P, kk % NSL, e) = q_pressure(q.r, q.rt);
+      lds(Q_TH, kk & (NSL - 1), e) = q_theta(q.r, q.rt);
+      lds(Q_P, kk & (NSL - 1), e) = q_pressure(q.r, q.rt);
     }
-    lds(Q_UC, kk % NSL, e) = qdiv(R(0.5) * (q.u0 + q.u1), rr);
-    lds(Q_VC, kk % NSL, e) = qdiv(R(0.5) * (q.v0 + q.v1), rr);
-    lds(Q_WC, kk % NSL, e) = qdiv(R(0.5) * (q.w0 + q.w1), rr);
+    lds(Q_UC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.u0 + q.u1), rr);
+    lds(Q_VC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.v0 + q.v1), rr);
+    lds(Q_WC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.w0 + q.w1), rr);
     if constexpr (RAW) {
-      lds(Q_U, kk % NSL, e) = q.u0;
-      lds(Q_V, kk % NSL, e) = q.v0;
-      lds(Q_W, kk % NSL, e) = q.w0;
+      lds(Q_U, kk & (NSL - 1), e) = q.u0;
+      lds(Q_V, kk & (NSL - 1), e) = q.v0;
+      lds(Q_W, kk & (NSL - 1), e) = q.w0;
     }
   };
   // Own-column rho (levels k-1, k, k+1), rho*theta (k, k+1) and the top w face.
@@ -255,7 +257,6 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   // leaves level k+2 in flight.  Unrolled by two with the buffers swapped, so
   // no register copy waits for loads in flight.
   auto level = [&](int k, const Lev& Ln, Lev& Lnn) {
-    __syncthreads();
     Q0 Qc;
     if constexpr (PIPE) {
       if (k + 1 < nz) stage_lev(Ln, k + 1);
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
       // through the column ends are selected away, so the level loop has no
       // per-level branches and its LDS reads issue together.
       const int kb = k > 0 ? k - 1 : k, kt = k + 1 < nz ? k + 1 : k;
-      const int s_b = kb % NSL, s_c = k % NSL, s_t = kt % NSL;
+      const int s_b = kb & (NSL - 1), s_c = k & (NSL - 1), s_t = kt & (NSL - 1);
       auto SL = [&](int kk) { return kk == k ? s_c : (kk < k ? s_b : s_t); };
       // LDS views at (ii, jj, kk): kk in k-1 .. k+1, (ii, jj) in the tile or ring.
       // (relative to the own entry eo, so neighbour offsets fold into the
@@ -508,9 +509,33 @@ __global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tile
   }
 }
 
+template <class R, bool TERRAIN, bool PERT, int MODE>
+__global__ __launch_bounds__(TX * TY) void k_slow_column(SlowArgs<R> a, int tiles_x, int tiles,
+                                                         int nkc) {
+  slow_body<R, TERRAIN, PERT, MODE>(a, tiles_x, tiles, nkc, hw_block());
+}
+template <class R, bool TERRAIN, bool PERT, int MODE>
+__global__ __launch_bounds__(TX * TY) void k_slow_column_batch(
+    const SlowItem<R>* __restrict__ items, BatchMap bm) {
+  VBlock vb;
+  const SlowItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  // the member's arguments, read once before the kernel stores anything
+  const SlowArgs<R> a = it->a;
+  slow_body<R, TERRAIN, PERT, MODE>(a, it->tiles_x, it->tiles, it->nkc, vb);
+}
+
 template <class R, bool TERRAIN, bool PERT>
 void launch_slow_mode(hipStream_t st, const SlowArgs<R>& a, dim3 grid, dim3 blk, size_t lds,
                       int tx, int tiles, int nkc) {
+  if (recorder()) {
+    const void* fn =
+        a.mode == 0 ? reinterpret_cast<const void*>(&k_slow_column_batch<R, TERRAIN, PERT, 0>)
+        : a.mode == 1 ? reinterpret_cast<const void*>(&k_slow_column_batch<R, TERRAIN, PERT, 1>)
+                      : reinterpret_cast<const void*>(&k_slow_column_batch<R, TERRAIN, PERT, 2>);
+    record_launch(fn, blk, lds, SlowItem<R>{a, tx, tiles, nkc, 0, 0}, static_cast<int>(grid.x), 1);
+    return;
+  }
   switch (a.mode) {
     case 0:
       hipLaunchKernelGGL((k_slow_column<R, TERRAIN, PERT, 0>), grid, blk, lds, st, a, tx, tiles,
@@ -556,14 +581,14 @@ template void launch_slow_column<double>(hipStream_t, const SlowArgs<double>&, b
 // out = wa*q0 + wb*(...) (dry_transport.cu:40-47).
 // ===========================================================================
 template <class R, bool TERRAIN, int NT, int MODE>
-__global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
-                                                       int nkc) {
+__device__ __forceinline__ void tracer_body(const TracerArgs<R>& a, int tiles_x, int tiles, int nkc,
+                                            const VBlock& vb) {
   __shared__ R sQ[NT][4][PN];
   extern __shared__ __align__(16) unsigned char lev_raw[];
   R* const lev = reinterpret_cast<R*>(lev_raw);
   const Geo g = a.g;
   int b, k0, k1;
-  const int tile = tile_of(tiles, nkc, g.nz, b, k0, k1);
+  const int tile = tile_of(vb.x, tiles, nkc, g.nz, b, k0, k1);
   (void)b;
   const int ti0 = (tile % tiles_x) * TX, tj0 = (tile / tiles_x) * TY;
   const int lx = threadIdx.x, ly = threadIdx.y;
@@ -718,6 +743,22 @@ __global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int 
   }
 }
 
+template <class R, bool TERRAIN, int NT, int MODE>
+__global__ __launch_bounds__(TX * TY) void k_tracer_column(TracerArgs<R> a, int tiles_x, int tiles,
+                                                           int nkc) {
+  tracer_body<R, TERRAIN, NT, MODE>(a, tiles_x, tiles, nkc, hw_block());
+}
+template <class R, bool TERRAIN, int NT, int MODE>
+__global__ __launch_bounds__(TX * TY) void k_tracer_column_batch(
+    const TracerItem<R>* __restrict__ items, BatchMap bm) {
+  VBlock vb;
+  const TracerItem<R>* it = batch_member(items, bm, vb);
+  if (!it) return;
+  // the member's arguments, read once before the kernel stores anything
+  const TracerArgs<R> a = it->a;
+  tracer_body<R, TERRAIN, NT, MODE>(a, it->tiles_x, it->tiles, it->nkc, vb);
+}
+
 template <class R, int NT>
 void launch_tracer_column_n(hipStream_t st, const TracerArgs<R>& a, bool terrain, int tx,
                             int tiles, int nkc) {
@@ -725,6 +766,16 @@ void launch_tracer_column_n(hipStream_t st, const TracerArgs<R>& a, bool terrain
   // per-level LDS: the longest chunk plus the levels below and above it
   const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * 2 * sizeof(R);
   const bool m2 = a.mode == 2;
+  if (recorder()) {
+    const void* fn =
+        terrain ? (m2 ? reinterpret_cast<const void*>(&k_tracer_column_batch<R, true, NT, 2>)
+                      : reinterpret_cast<const void*>(&k_tracer_column_batch<R, true, NT, 1>))
+                : (m2 ? reinterpret_cast<const void*>(&k_tracer_column_batch<R, false, NT, 2>)
+                      : reinterpret_cast<const void*>(&k_tracer_column_batch<R, false, NT, 1>));
+    record_launch(fn, blk, lds, TracerItem<R>{a, tx, tiles, nkc, 0, 0}, static_cast<int>(grid.x),
+                  1);
+    return;
+  }
   if (terrain) {
     if (m2)
       hipLaunchKernelGGL((k_tracer_column<R, true, NT, 2>), grid, blk, lds, st, a, tx, tiles, nkc);

@@ -390,11 +390,12 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 // depend on its own row only (the matrix product keeps columns apart), so
 // the results equal those of running every cell through the network.
 // counts[0] += fallback cells, counts[1] += network cells.
-__global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
-                                                       float* qv, float* qc, float* qr,
-                                                       MicroCfg<float> mc, const float* w,
-                                                       SurNorm nm, unsigned long long* counts,
-                                                       Base<float> base) {
+__device__ __forceinline__ void warm_rain_sur_body(const Geo& g, const float* rho, float* rt,
+                                                   float* qv, float* qc, float* qr,
+                                                   const MicroCfg<float>& mc, const float* w,
+                                                   const SurNorm& nm,
+                                                   unsigned long long* counts,
+                                                   const Base<float>& base, const VBlock& vb) {
   __shared__ float sb[196];
   __shared__ float sfeat[kTile][8];
   __shared__ float sout[kTile][kSurOut];
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
   const int n = nxy * g.nz;
   const bool pert = base.rho != nullptr;
   unsigned mine = 0, mine_net = 0;
-  for (int tile = blockIdx.x; tile * kTile < n; tile += gridDim.x) {
+  for (int tile = vb.x; tile * kTile < n; tile += vb.gx) {
     const int cell = tile * kTile + wv * 64 + lane;
     const bool valid = cell < n;
     size_t ci = 0;
@@ -507,13 +508,31 @@ __global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, 
   }
 }
 
+__global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
+                                                       float* qv, float* qc, float* qr,
+                                                       MicroCfg<float> mc, const float* w,
+                                                       SurNorm nm, unsigned long long* counts,
+                                                       Base<float> base) {
+  warm_rain_sur_body(g, rho, rt, qv, qc, qr, mc, w, nm, counts, base, hw_block());
+}
+__global__ __launch_bounds__(256) void k_warm_rain_sur_batch(const SurItem* __restrict__ items,
+                                                             BatchMap bm) {
+  VBlock vb;
+  const SurItem* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const SurItem a = *it;
+  warm_rain_sur_body(a.g, a.rho, a.rt, a.qv, a.qc, a.qr, a.mc, a.w, a.nm, a.counts, a.base, vb);
+}
+
 // Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
 // warm_rain.cu:122-191), one thread per column, top down.
 template <bool TERRAIN>
-__global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
-                          const float* lev_idz_c, float dt, float vt, float depth, int fallout) {
+__device__ __forceinline__ void fallout_body(const Geo& g, float* qr, float* precip,
+                                             const float* idz_c, const float* lev_idz_c, float dt,
+                                             float vt, float depth, int fallout,
+                                             const VBlock& vb) {
   int bx, by;
-  xcd_block2d(bx, by);
+  xcd_block2d(vb, bx, by);
   const int i = bx * blockDim.x + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -548,8 +567,29 @@ __global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
   precip[static_cast<size_t>(j) * g.nx + i] += inc;
 }
 
+template <bool TERRAIN>
+__global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
+                          const float* lev_idz_c, float dt, float vt, float depth, int fallout) {
+  fallout_body<TERRAIN>(g, qr, precip, idz_c, lev_idz_c, dt, vt, depth, fallout, hw_block());
+}
+template <bool TERRAIN>
+__global__ void k_fallout_batch(const FalloutItem* __restrict__ items, BatchMap bm) {
+  VBlock vb;
+  const FalloutItem* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const FalloutItem a = *it;
+  fallout_body<TERRAIN>(a.g, a.qr, a.precip, a.idz_c, a.lev_idz_c, a.dt, a.vt, a.depth, a.fallout,
+                        vb);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+__global__ void k_zero_u64_batch(const ZeroItem* __restrict__ items, BatchMap bm) {
+  VBlock vb;
+  const ZeroItem* it = batch_member(items, bm, vb);
+  if (!it) return;
+  if (static_cast<int>(threadIdx.x) < it->n) it->p[threadIdx.x] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -567,12 +607,14 @@ void launch_sur_sample(hipStream_t st, const Geo& g, const float* rho, const flo
                        const float* qv, const float* qc, const float* qr,
                        const MicroCfg<float>& mc, unsigned long long seed, long n, float* x,
                        float* y, long long* cell, Base<float> base) {
+  if (refuse_record("sur_sample")) return;
   hipLaunchKernelGGL(k_sur_sample, dim3(nblk(n, 256)), dim3(256), 0, st, g, rho, rt, qv, qc, qr,
                      mc, seed, n, x, y, cell, base);
 }
 
 void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const float* y, int B,
                            const SurNorm& nm, bool update, double* loss_part, int* nloss) {
+  if (refuse_record("sur_train_step")) return;
   float* p = T.params;
   // forward
   hipLaunchKernelGGL(k_sur_norm_in, dim3(nblk(static_cast<long>(B) * 8, 256)), dim3(256), 0, st,
@@ -623,10 +665,16 @@ void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const fl
 
 void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const float* x, long n,
                       float* y) {
+  if (refuse_record("sur_infer")) return;
   hipLaunchKernelGGL(k_sur_infer, dim3(infer_blocks(n)), dim3(256), 0, st, w, nm, x, n, y);
 }
 
 void launch_zero_counters(hipStream_t st, unsigned long long* p, int n) {
+  if (recorder()) {
+    record_launch(reinterpret_cast<const void*>(&k_zero_u64_batch), dim3(64), 0,
+                  ZeroItem{p, n, 0, 0}, 1, 1);
+    return;
+  }
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
 }
 
@@ -634,6 +682,12 @@ void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float*
                           float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
                           const SurNorm& nm, unsigned long long* counts, Base<float> base) {
   const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
+  if (recorder()) {
+    record_launch(reinterpret_cast<const void*>(&k_warm_rain_sur_batch), dim3(256), 0,
+                  SurItem{g, rho, rt, qv, qc, qr, mc, w, nm, counts, base, 0, 0},
+                  static_cast<int>(infer_blocks(cells)), 1);
+    return;
+  }
   hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks(cells)), dim3(256), 0, st, g, rho, rt,
                      qv, qc, qr, mc, w, nm, counts, base);
 }
@@ -642,6 +696,14 @@ void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, cons
                     const float* lev_idz_c, float dt, float vt, float depth, int fallout,
                     bool terrain) {
   const dim3 b(64, 4, 1), gr((g.nx + 63) / 64, (g.ny + 3) / 4);
+  if (recorder()) {
+    const void* fn = terrain ? reinterpret_cast<const void*>(&k_fallout_batch<true>)
+                             : reinterpret_cast<const void*>(&k_fallout_batch<false>);
+    record_launch(fn, b, 0,
+                  FalloutItem{g, qr, precip, idz_c, lev_idz_c, dt, vt, depth, fallout, 0, 0},
+                  static_cast<int>(gr.x), static_cast<int>(gr.y));
+    return;
+  }
   if (terrain)
     hipLaunchKernelGGL(k_fallout<true>, gr, b, 0, st, g, qr, precip, idz_c, lev_idz_c, dt, vt,
                        depth, fallout);

@@ -2,6 +2,7 @@
 // (gwm_surrogate.hip).  No counterpart in the reference.
 #pragma once
 
+#include "gwm_kernels.hpp"
 #include "gwm_micro.hpp"
 
 namespace gwm {
@@ -61,5 +62,33 @@ void launch_zero_counters(hipStream_t st, unsigned long long* p, int n);
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
                     const float* lev_idz_c, float dt, float vt, float depth, int fallout,
                     bool terrain);
+
+// Items of the batched forms (gwm_kernels.hpp, Recorder).
+struct SurItem {
+  Geo g;
+  const float* rho;
+  float *rt, *qv, *qc, *qr;
+  MicroCfg<float> mc;
+  const float* w;
+  SurNorm nm;
+  unsigned long long* counts;
+  Base<float> base;
+  int gx, gy;
+};
+struct ZeroItem {
+  unsigned long long* p;
+  int n;
+  int gx, gy;
+};
+struct FalloutItem {
+  Geo g;
+  float* qr;
+  float* precip;
+  const float* idz_c;
+  const float* lev_idz_c;
+  float dt, vt, depth;
+  int fallout;
+  int gx, gy;
+};
 
 }  // namespace gwm

@@ -219,8 +219,10 @@ def lib():
                                                ctypes.POINTER(BoundaryConfig)]
         L.gwm_moist_steps_graph.argtypes = [vp, ctypes.POINTER(StepConfig),
                                             ctypes.POINTER(WarmRainConfig), vp, ip]
-        L.gwm_ensemble_steps.argtypes = [vp, ip, ctypes.POINTER(StepConfig),
-                                         ctypes.POINTER(WarmRainConfig), vp, ip]
+        L.gwm_ensemble_tuning_report.argtypes = [cp, ctypes.c_size_t]
+        for n in ("gwm_ensemble_steps", "gwm_ensemble_steps_streams"):
+            getattr(L, n).argtypes = [vp, ip, ctypes.POINTER(StepConfig),
+                                      ctypes.POINTER(WarmRainConfig), vp, ip]
         _lib = L
     return _lib
 
@@ -706,12 +708,22 @@ def surrogate_sample(state, cfg, seed, n, x, y, cell=None, row=0):
                                      None if cell is None else cell.offset(row)))
 
 
-def ensemble_steps(states, cfg, wr, nsteps, surrogates=None):
-    """Advance states (one context each) nsteps moist steps concurrently."""
+def ensemble_tuning_report():
+    """Step-pair times of the batched ensemble's level-chunk candidates, per
+    ensemble signature (gwm_ensemble_tuning_report)."""
+    buf = ctypes.create_string_buffer(65536)
+    check(lib().gwm_ensemble_tuning_report(buf, len(buf)))
+    return buf.value.decode()
+
+
+def ensemble_steps(states, cfg, wr, nsteps, surrogates=None, streams=False):
+    """Advance states nsteps moist steps together: one launch per kernel of
+    the step over all members (gwm_ensemble_steps), or with streams=True one
+    context (stream) per member, concurrently (gwm_ensemble_steps_streams)."""
     n = len(states)
     arr = (ctypes.c_void_p * n)(*[st.h.value for st in states])
     sur = None
     if surrogates is not None:
         sur = (ctypes.c_void_p * n)(*[None if m is None else m.h.value for m in surrogates])
-    check(lib().gwm_ensemble_steps(arr, n, ctypes.byref(cfg), ctypes.byref(wr), sur,
-                                   int(nsteps)))
+    fn = lib().gwm_ensemble_steps_streams if streams else lib().gwm_ensemble_steps
+    check(fn(arr, n, ctypes.byref(cfg), ctypes.byref(wr), sur, int(nsteps)))

@@ -459,13 +459,28 @@ int gwm_moist_step_surrogate(gwm_state* state, const gwm_step_config* cfg,
  * change of configuration or of the surrogate's normalisation re-captures. */
 int gwm_moist_steps_graph(gwm_state* state, const gwm_step_config* cfg,
                           const gwm_warm_rain_config* wr, gwm_surrogate* surrogate, int nsteps);
-/* Ensemble of states of any sizes, each on its own context (stream):
- * every member advances nsteps moist steps; members run concurrently,
- * their graph replays interleaved; returns after all members finished.
+/* Ensemble of states of any sizes: every member advances nsteps moist
+ * steps, as one launch per kernel of the step over all members' work-groups
+ * (each member's two steps recorded, the batched sequence captured into a
+ * HIP graph and replayed); members of one precision on one device, which
+ * may share a context.  Where the steps cannot be batched (mixed precision,
+ * boundaries, a configuration whose kernels differ between members), the
+ * members run as gwm_ensemble_steps_streams does.  Results equal each
+ * member's own gwm_moist_step.  Returns after all members finished.
  * surrogates: NULL, or one per member (entries may be NULL). */
 int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* cfg,
                        const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
                        int nsteps);
+/* The batched ensemble's autotuner: per ensemble signature (member shapes
+ * and forms), the time of a two-step pair for each level-chunk candidate
+ * (members' own tuned counts, then 2, 4, 8 for all), -1 where untried; the
+ * fastest is used once all are timed.  One line per signature. */
+int gwm_ensemble_tuning_report(char* buf, size_t size);
+/* The same with each member on its own context (stream): members run
+ * concurrently, their two-step graph replays interleaved. */
+int gwm_ensemble_steps_streams(gwm_state* const* states, int n, const gwm_step_config* cfg,
+                               const gwm_warm_rain_config* wr,
+                               gwm_surrogate* const* surrogates, int nsteps);
 
 #ifdef __cplusplus
 }

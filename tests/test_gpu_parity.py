@@ -586,9 +586,11 @@ def test_graph_replayed_steps_equal_eager_steps(ctx32, ctx64, prec):
         assert np.array_equal(x, y), (f, int((x != y).sum()))
 
 
-def test_ensemble_steps_equal_member_steps():
-    """Members of different sizes, one context each, advanced concurrently
-    from their graphs, equal the same members stepped one by one."""
+@pytest.mark.parametrize("streams", [False, True])
+def test_ensemble_steps_equal_member_steps(streams):
+    """Members of different sizes advanced together -- one launch per kernel
+    of the step over all members (batched), or one context (stream) each
+    from their graphs -- equal the same members stepped one by one."""
     shapes = [(40, 24, 12), (64, 32, 10), (24, 40, 14)]
     cfg = G.step_config(dt=1.0, fast_substeps=4)
     wr = G.warm_rain_config(dt=1.0, rain_terminal_velocity=12.0)
@@ -599,17 +601,88 @@ def test_ensemble_steps_equal_member_steps():
         keep.append(grid)
         members.append(a)
         refs.append(b)
-    G.ensemble_steps(members, cfg, wr, 5)
+    G.ensemble_steps(members, cfg, wr, 5, streams=streams)
+    # a second call reuses the members' warm-up and records again
+    G.ensemble_steps(members, cfg, wr, 4, streams=streams)
     for r in refs:
-        for _ in range(5):
+        for _ in range(9):
+            r.moist_step(cfg, wr)
+    assert_all_finite(*members, *refs)
+    for m, r in zip(members, refs):
+        for f in FIELDS + (O.TRACER0, O.TRACER0 + 1, O.TRACER0 + 2, O.PRECIP):
+            assert np.array_equal(m.get(f), r.get(f)), f
+    del members, refs, keep
+    for c in ctxs:
+        c.close()
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_batched_ensemble_on_one_context_and_mixed_members(ctx32, ctx64, prec):
+    """Batched members may share one context; FP32 members in the
+    perturbation form and with the surrogate (the same network per member)
+    batch as well.  A mixed-precision ensemble cannot batch and runs its
+    members on their own streams.  Every member equals its own steps."""
+    ctx = ctx_for(prec, ctx32, ctx64)
+    cfg = G.step_config(dt=1.0, fast_substeps=3)
+    wr = G.warm_rain_config(dt=1.0, rain_terminal_velocity=12.0)
+    shapes = [(40, 24, 12), (24, 16, 12), (72, 40, 12)]
+    keep, members, refs = [], [], []
+    for i, shape in enumerate(shapes):
+        grid, (a, b) = _moist_pair(prec, ctx, seed=40 + i, shape=shape)
+        keep.append(grid)
+        members.append(a)
+        refs.append(b)
+    G.ensemble_steps(members, cfg, wr, 6)
+    for r in refs:
+        for _ in range(6):
             r.moist_step(cfg, wr)
     assert_all_finite(*members, *refs)
     for m, r in zip(members, refs):
         for f in FIELDS + (O.TRACER0, O.TRACER0 + 2, O.PRECIP):
             assert np.array_equal(m.get(f), r.get(f)), f
-    del members, refs, keep
-    for c in ctxs:
-        c.close()
+    if prec == 4:
+        import gwm_amd.surrogate as SG
+        import surrogate_oracle as SO
+        sur = G.Surrogate(ctx, seed=3)
+        big = 1e30
+        sur.set_norm(x_mean=np.zeros(7), x_std=np.ones(7) * 1e3, y_mean=np.zeros(4),
+                     y_std=np.array([1e-6, 1e-6, 1e-6, 1e-2]), box_lo=-big * np.ones(7),
+                     box_hi=big * np.ones(7))
+        sur.set_params((SO.init_xavier(2) * 0.5).astype(np.float32))
+        pm, pr = [], []
+        for i, shape in enumerate(shapes[:2]):
+            zi = np.linspace(0.0, 12000.0, shape[2] + 1)
+            grid = G.Grid(ctx, *shape, dx=1000.0, dy=1000.0, z_interfaces=zi)
+            keep.append(grid)
+            for lst in (pm, pr):
+                st = G.State(grid, 3, perturbation=True)
+                st.init_convection(1.0e5, 50 + i, 0.5, 1000.0)
+                qv = st.get(O.TRACER0)
+                st.set(O.TRACER0 + 1, (0.03 * qv).astype(qv.dtype))
+                lst.append(st)
+        G.ensemble_steps(pm, cfg, wr, 4, surrogates=[sur, sur])
+        for r in pr:
+            for _ in range(4):
+                r.moist_step_surrogate(cfg, wr, sur)
+        assert_all_finite(*pm, *pr)
+        for m, r in zip(pm, pr):
+            for f in FIELDS + (O.TRACER0, O.TRACER0 + 1, O.TRACER0 + 2, O.PRECIP):
+                assert np.array_equal(m.get(f), r.get(f)), f
+        del SG
+    # mixed precision: streams, one context each
+    other = ctx64 if prec == 4 else ctx32
+    grid2, (c2, d2) = _moist_pair(8 if prec == 4 else 4, other, seed=60, shape=(32, 24, 12))
+    keep.append(grid2)
+    a0 = G.State(keep[0], 3)
+    a0.copy_from(refs[0])
+    G.ensemble_steps([a0, c2], cfg, wr, 2)
+    for _ in range(2):
+        refs[0].moist_step(cfg, wr)
+        d2.moist_step(cfg, wr)
+    assert_all_finite(a0, c2)
+    for x, y in ((a0, refs[0]), (c2, d2)):
+        for f in FIELDS + (O.TRACER0, O.PRECIP):
+            assert np.array_equal(x.get(f), y.get(f)), f
 
 
 @pytest.mark.parametrize("prec", [4, 8])
