@@ -89,11 +89,13 @@ def parse():
     p.add_argument("--ensemble-dt", type=float, default=0.5,
                    help="dt of the timed ensemble run (the configuration's 2 s diverges, and "
                         "1 s on the 60-level members within twenty steps)")
+    p.add_argument("--c2-steps", type=int, default=10,
+                   help="timed steps of the mountain-wave leg (after 2 warm-up steps)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run one device-to-device state copy (profiler byte calibration)")
     p.add_argument("--legs", default="all",
                    help="comma list of legs beyond the headline: full,graph,config_dt,copy,fp64,"
-                        "surrogate,error,ensemble (all: every leg)")
+                        "surrogate,error,ensemble,c2 (all: every leg)")
     return p.parse_args()
 
 
@@ -489,6 +491,59 @@ def ensemble_leg(G, a, sur_src):
     }
 
 
+def c2_leg(G, a):
+    """The mountain-wave configuration at its size: 256x256x60 columns,
+    dx 2 km, levels stretched 50-600 m to 20 km, an Agnesi ridge (400 m,
+    half-width 10 km, centred in x), constant N = 0.01 s-1 with u = 10 m/s,
+    relaxation at the open x sides (width 5), 6 substeps; dt 0.25 s (its
+    6 s overflows in the first step).  FP64 and FP32 (full form: the
+    perturbation form is for flat, doubly periodic grids) from the same
+    state: throughput of each over the timed steps and the FP32 state's
+    relative L2 against the FP64 state at the end."""
+    import numpy as np
+    nx, ny, nz, dx = 256, 256, 60, 2000.0
+    zi = G.stretched_levels(nz, 50.0, 600.0, 20000.0)
+    ter = G.agnesi_ridge(nx, ny, dx, 0.5 * nx * dx, 10000.0, 400.0)
+    cfg = G.step_config(dt=0.25, fast_substeps=6)
+    bc = G.BoundaryConfig(G.BOUNDARY_RELAX, 5.0, G.BOUNDARY_FN(), None)
+    steps, warm = a.c2_steps, 2
+    out, states = {}, {}
+    for name, prec in (("fp64", G.F64), ("fp32_full", G.F32)):
+        ctx = G.Context(0, prec)
+        grid = G.Grid(ctx, nx, ny, nz, dx=dx, dy=dx, z_interfaces=zi, terrain=ter,
+                      periodic_x=False)
+        st = G.State(grid, 0)
+        st.init_constant_n(theta_surface=288.0, brunt_vaisala=0.01, p_surface=1.0e5, u=10.0)
+        if prec == G.F32:  # the FP64 initial state, rounded once
+            for f in range(5):
+                st.set(f, states["fp64_init"][f].astype(np.float32))
+        else:
+            states["fp64_init"] = [st.get(f) for f in range(5)]
+        st.capture_boundary()
+        for _ in range(warm):
+            st.ssprk3_step(cfg, bc)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.ssprk3_step(cfg, bc)
+        ctx.sync()
+        secs = time.perf_counter() - t0
+        bad = int(st.summary()["nonfinite_values"])
+        out[name] = {"value": nx * ny * nz * steps / secs if bad == 0 else None,
+                     "ms_per_step": secs / steps * 1e3, "state_finite": bad == 0}
+        states[name] = (ctx, grid, st)
+    errs = rel_errors(G, states["fp64"][2], {"fp32_full": states["fp32_full"][2]},
+                      (0, 1, 2, 3, 4))["fp32_full"]
+    rec = {"workload": "C2 mountain wave 256x256x60, Agnesi ridge 400 m, constant N, "
+                       "stretched 50-600 m to 20 km, relaxation in x, dry",
+           "dt": 0.25, "steps": steps, "warmup": warm, "fp64": out["fp64"],
+           "fp32_full": out["fp32_full"], "fp32_full_vs_fp64_rel_l2": errs,
+           "w_max_fp64": states["fp64"][2].summary()["w_max"]}
+    if out["fp64"]["value"] and out["fp32_full"]["value"]:
+        rec["fp32_over_fp64"] = out["fp32_full"]["value"] / out["fp64"]["value"]
+    return rec
+
+
 def copy_ceiling(G, a, reps=10):
     """Device-to-device copy rate of a moist state (read + write bytes over
     wall time, 8 fields and the surface bucket per copy): the box's
@@ -517,7 +572,7 @@ def main():
         sys.exit(2)
     import gwm_amd as G
 
-    legs = set(("full,graph,config_dt,copy,fp64,surrogate,error,ensemble" if a.legs == "all"
+    legs = set(("full,graph,config_dt,copy,fp64,surrogate,error,ensemble,c2" if a.legs == "all"
                 else a.legs).split(","))
     cells = a.nx * a.ny * a.nz
     dt = a.control_dt if a.control_dt > 0 else a.dt
@@ -626,6 +681,10 @@ def main():
     if "ensemble" in legs and a.ensemble_members > 0:
         log(f"ensemble: {a.ensemble_members} members")
         ens = ensemble_leg(G, a, model)
+    c2 = None
+    if "c2" in legs:
+        log("mountain wave (C2) at 256x256x60: FP64 and FP32")
+        c2 = c2_leg(G, a)
     cpu = None
     if a.cpu_baseline:
         log("CPU baseline")
@@ -657,6 +716,7 @@ def main():
         "surrogate": sur_rec,
         "error_vs_fp64": err,
         "ensemble": ens,
+        "mountain_wave_c2": c2,
         "dry_mass": summary["total_dry_mass"],
         "kernels": stats,
         "kernel_roofline": kernel_roofline,

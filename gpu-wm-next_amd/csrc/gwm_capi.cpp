@@ -496,12 +496,15 @@ void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
   }
 }
 
-// Autotuner for the level-chunk count of a column kernel.  `launch(n)` runs
-// the kernel with n chunks; the kernels are out-of-place, so the candidates
-// can be run on the live arguments and every candidate writes identical
+// Autotuner for the launch shape of a column kernel: the level-chunk count
+// and, where the kernel has more than one (`shapes`), the tile height.
+// `launch(code)` runs the kernel with code = chunks | 16 * shape; the kernels
+// are out-of-place and their results do not depend on the shape, so the
+// candidates run on the live arguments and every candidate writes identical
 // values.  Each candidate is timed over two launches after a warm-up.
 template <class Launch>
-int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Launch&& launch) {
+int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Launch&& launch,
+                int shapes = 1, bool whole_columns = false) {
   if (ctx->force_chunks > 0) return ctx->force_chunks;
   char key[160];
   std::snprintf(key, sizeof key, "%s:%dx%dx%d:%s:%d", kname, g.nx, g.ny, g.nz,
@@ -514,33 +517,30 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
   // Three rounds over the candidates, each timing two launches after a warm
   // one; a candidate's time is its fastest round, so clock and cache drift
   // between rounds does not decide the choice.
-  const int cands[4] = {1, 2, 4, 8};
-  float t_min[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  std::vector<int> cands;
+  for (int sh = 0; sh < shapes; ++sh)
+    for (int n : {1, 2, 4, 8})
+      if (n == 1 || (!whole_columns && g.nz / n >= 4)) cands.push_back(n | (sh << 4));
+  std::vector<float> t_min(cands.size(), 1e30f);
   for (int round = 0; round < 3; ++round)
-    for (int c = 0; c < 4; ++c) {
-      const int n = cands[c];
-      if (n > 1 && g.nz / n < 4) continue;
-      launch(n);
+    for (size_t c = 0; c < cands.size(); ++c) {
+      launch(cands[c]);
       HIPCHK(hipEventRecord(e0, ctx->stream));
-      launch(n);
-      launch(n);
+      launch(cands[c]);
+      launch(cands[c]);
       HIPCHK(hipEventRecord(e1, ctx->stream));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       t_min[c] = std::min(t_min[c], ms);
     }
-  int best = 1;
-  float best_ms = 1e30f;
-  for (int c = 0; c < 4; ++c)
-    if (t_min[c] < best_ms) {
-      best_ms = t_min[c];
-      best = cands[c];
-    }
+  size_t best = 0;
+  for (size_t c = 1; c < cands.size(); ++c)
+    if (t_min[c] < t_min[best]) best = c;
   ctx->pool.push_back(e0);
   ctx->pool.push_back(e1);
-  ctx->tuned[key] = best;
-  return best;
+  ctx->tuned[key] = cands[best];
+  return cands[best];
 }
 
 // One fast subcycle (apply_local_split_explicit_fast_modes) from rho_in and
@@ -616,8 +616,12 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
       gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
                                  emit ? part : nullptr, emit ? &nparts : nullptr);
     };
+    // flat launches choose level chunks and the tile height (8 or 4 rows);
+    // column-reference launches keep whole columns and choose the height
     const int nkc =
-        colref ? 1 : pick_chunks(S->ctx, "fast_column", g, G->terrain_nonzero ? 1 : 0, run);
+        colref ? pick_chunks(S->ctx, "fast_column", g, 2, run, 2, /*whole_columns=*/true)
+               : pick_chunks(S->ctx, "fast_column", g, G->terrain_nonzero ? 1 : 0, run,
+                             G->terrain_nonzero ? 1 : 2);
     Timed t(S->ctx, "fast_column");
     run(nkc);
     rho = a.rho_out;
@@ -3265,57 +3269,79 @@ std::unique_ptr<BatchPlan> build_plan(const std::vector<gwm::Recorder>& recs) {
   const int n = static_cast<int>(recs.size());
   require(n < (1 << (32 - gwm::kBatchBlockBits)) - 1, "a batched ensemble takes at most 4094 members");
   auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
-  // layout: per op, items | work-group entries
-  std::vector<size_t> off_items(nops), off_map(nops);
-  std::vector<int> totals(nops);
-  std::vector<std::vector<int>> starts(nops);
+  // Position i of the sequences becomes one launch per distinct kernel form
+  // there (members tuned to different tile heights run different
+  // instantiations); usually one.
+  struct Sub {
+    size_t op;
+    std::vector<int> members;
+    std::vector<int> starts;
+    int total = 0;
+    size_t off_items = 0, off_map = 0;
+  };
+  std::vector<Sub> subs;
   size_t bytes = 0;
   for (size_t i = 0; i < nops; ++i) {
-    const gwm::LaunchRecord& r0 = recs[0].ops[i];
-    int at = 0;
+    std::vector<Sub> here;
     for (int m = 0; m < n; ++m) {
       const gwm::LaunchRecord& r = recs[static_cast<size_t>(m)].ops[i];
-      if (r.fn != r0.fn || r.bx != r0.bx || r.by != r0.by || r.bz != r0.bz ||
-          r.item.size() != r0.item.size())
-        return nullptr;
       require(static_cast<long>(r.gx) * r.gy < (1L << gwm::kBatchBlockBits),
               "a batched member's grid exceeds 2^20 work-groups");
-      at = (at + 7) & ~7;
-      starts[i].push_back(at);
-      at += r.gx * r.gy;
+      Sub* sub = nullptr;
+      for (auto& h : here) {
+        const gwm::LaunchRecord& q = recs[static_cast<size_t>(h.members[0])].ops[i];
+        if (q.fn == r.fn && q.bx == r.bx && q.by == r.by && q.bz == r.bz &&
+            q.item.size() == r.item.size())
+          sub = &h;
+      }
+      if (!sub) {
+        // different kernels at one position must at least agree in kind
+        // (the same launcher), or the sequences differ
+        if (!here.empty() &&
+            recs[static_cast<size_t>(here[0].members[0])].ops[i].item.size() != r.item.size())
+          return nullptr;
+        here.push_back(Sub{i, {}, {}, 0, 0, 0});
+        sub = &here.back();
+      }
+      sub->total = (sub->total + 7) & ~7;
+      sub->starts.push_back(sub->total);
+      sub->members.push_back(m);
+      sub->total += r.gx * r.gy;
     }
-    totals[i] = at;
-    off_items[i] = bytes;
-    bytes = align(bytes + r0.item.size() * static_cast<size_t>(n));
-    off_map[i] = bytes;
-    bytes = align(bytes + sizeof(unsigned) * static_cast<size_t>(at));
+    for (auto& h : here) {
+      h.off_items = bytes;
+      bytes = align(bytes + recs[0].ops[i].item.size() * h.members.size());
+      h.off_map = bytes;
+      bytes = align(bytes + sizeof(unsigned) * static_cast<size_t>(h.total));
+      subs.push_back(std::move(h));
+    }
   }
   std::vector<unsigned char> host(bytes, 0);
   auto plan = std::make_unique<BatchPlan>();
   plan->mem = dalloc(std::max<size_t>(bytes, 256));
   unsigned char* dev = static_cast<unsigned char*>(plan->mem);
-  for (size_t i = 0; i < nops; ++i) {
-    const gwm::LaunchRecord& r0 = recs[0].ops[i];
+  for (const Sub& h : subs) {
+    const gwm::LaunchRecord& r0 = recs[static_cast<size_t>(h.members[0])].ops[h.op];
     const size_t isz = r0.item.size();
     size_t lds = 0;
-    std::vector<unsigned> map(static_cast<size_t>(totals[i]), gwm::kNoMember);
-    for (int m = 0; m < n; ++m) {
-      const gwm::LaunchRecord& r = recs[static_cast<size_t>(m)].ops[i];
-      std::memcpy(host.data() + off_items[i] + isz * static_cast<size_t>(m), r.item.data(), isz);
+    std::vector<unsigned> map(static_cast<size_t>(h.total), gwm::kNoMember);
+    for (size_t q = 0; q < h.members.size(); ++q) {
+      const gwm::LaunchRecord& r = recs[static_cast<size_t>(h.members[q])].ops[h.op];
+      std::memcpy(host.data() + h.off_items + isz * q, r.item.data(), isz);
       lds = std::max(lds, r.lds);
-      const int s0 = starts[i][static_cast<size_t>(m)];
+      const int s0 = h.starts[q];
       for (int b = 0; b < r.gx * r.gy; ++b)
         map[static_cast<size_t>(s0 + b)] =
-            (static_cast<unsigned>(m) << gwm::kBatchBlockBits) | static_cast<unsigned>(b);
+            (static_cast<unsigned>(q) << gwm::kBatchBlockBits) | static_cast<unsigned>(b);
     }
-    std::memcpy(host.data() + off_map[i], map.data(), sizeof(unsigned) * map.size());
+    std::memcpy(host.data() + h.off_map, map.data(), sizeof(unsigned) * map.size());
     BatchPlan::Op op;
     op.fn = r0.fn;
-    op.grid = dim3(static_cast<unsigned>(totals[i]));
+    op.grid = dim3(static_cast<unsigned>(h.total));
     op.block = dim3(r0.bx, r0.by, r0.bz);
     op.lds = lds;
-    op.items = dev + off_items[i];
-    op.map = gwm::BatchMap{reinterpret_cast<const unsigned*>(dev + off_map[i])};
+    op.items = dev + h.off_items;
+    op.map = gwm::BatchMap{reinterpret_cast<const unsigned*>(dev + h.off_map)};
     plan->ops.push_back(op);
   }
   HIPCHK(hipMemcpy(plan->mem, host.data(), bytes, hipMemcpyHostToDevice));
@@ -3331,15 +3357,20 @@ void launch_plan_ops(const BatchPlan& plan, hipStream_t st) {
   }
 }
 
-// Level-chunk counts the column kernels of a batched ensemble are recorded
-// with: 0 keeps each member's own tuned count (chosen for a launch of that
-// member alone), the others apply to every member.  The ensemble's own
-// autotuner times each candidate on real step pairs (the candidates give
-// identical results) and keeps the fastest per ensemble signature.
-constexpr int kEnsChunkCands[4] = {0, 2, 4, 8};
+// Launch codes the column kernels of a batched ensemble are recorded with:
+// 0 keeps each member's own tuned code (chosen for a launch of that member
+// alone; a position whose members differ in tile height becomes one launch
+// per height), the others apply to every member: level chunks 2, 4, 8 with
+// 8-row tiles (4-row tiles for all members measured 13 % slower on the
+// 32-member ensemble and are not tried).  The ensemble's own autotuner
+// times each candidate on real step pairs (the candidates give identical
+// results) and keeps the fastest per ensemble signature.
+constexpr int kEnsCands = 4;
+constexpr int kEnsChunkCands[kEnsCands] = {0, 2, 4, 8};
 struct EnsembleTuning {
-  float ms[4] = {-1.f, -1.f, -1.f, -1.f};  // time of a step pair, -1 untested
+  float ms[kEnsCands] = {-1.f, -1.f, -1.f, -1.f};  // step-pair time, -1 untested
 };
+constexpr float kNoPlan = 1e30f;  // a candidate whose steps cannot be batched
 std::mutex g_ens_mu;
 std::map<std::string, EnsembleTuning> g_ens_tuning;
 
@@ -3440,7 +3471,7 @@ int ensemble_batched(gwm_state* const* states, int n, const gwm_step_config& c,
       std::lock_guard<std::mutex> lock(g_ens_mu);
       tune = g_ens_tuning[sig];
     }
-    std::unique_ptr<BatchPlan> plans[4];
+    std::unique_ptr<BatchPlan> plans[kEnsCands];
     auto plan_for = [&](int ci) -> BatchPlan* {
       if (!plans[ci]) plans[ci] = make_batch_plan<R>(states, n, c, w, surs, kEnsChunkCands[ci]);
       return plans[ci].get();
@@ -3454,11 +3485,16 @@ int ensemble_batched(gwm_state* const* states, int n, const gwm_step_config& c,
     try {
       while (left >= 2) {
         int ci = -1;
-        for (int q = 0; q < 4 && ci < 0; ++q)
+        for (int q = 0; q < kEnsCands && ci < 0; ++q)
           if (tune.ms[q] < 0.f) ci = q;
         if (ci >= 0) {  // a candidate not timed yet: one timed step pair
           BatchPlan* p = plan_for(ci);
-          if (!p) break;  // the steps cannot be batched
+          if (!p) {
+            // not batchable with this candidate (members tuned to different
+            // tile heights give different kernels at one position)
+            tune.ms[ci] = kNoPlan;
+            continue;
+          }
           if (last && last != p) HIPCHK(hipStreamSynchronize(last->stream));
           HIPCHK(hipEventRecord(e0, p->stream));
           HIPCHK(hipGraphLaunch(p->exec, p->stream));
@@ -3470,10 +3506,10 @@ int ensemble_batched(gwm_state* const* states, int n, const gwm_step_config& c,
           last = p;
         } else {
           int best = 0;
-          for (int q = 1; q < 4; ++q)
+          for (int q = 1; q < kEnsCands; ++q)
             if (tune.ms[q] < tune.ms[best]) best = q;
-          BatchPlan* p = plan_for(best);
-          if (!p) break;
+          BatchPlan* p = tune.ms[best] < kNoPlan ? plan_for(best) : nullptr;
+          if (!p) break;  // the steps cannot be batched
           if (last && last != p) HIPCHK(hipStreamSynchronize(last->stream));
           HIPCHK(hipGraphLaunch(p->exec, p->stream));
           last = p;
@@ -3508,8 +3544,19 @@ std::string ensemble_tuning_report() {
     std::snprintf(b, sizeof b, "%zu members:", static_cast<size_t>(std::count(
                                                    kv.first.begin(), kv.first.end(), '|')));
     out += b;
-    for (int q = 0; q < 4; ++q) {
-      std::snprintf(b, sizeof b, " chunks %d %.3f ms", kEnsChunkCands[q], kv.second.ms[q]);
+    for (int q = 0; q < kEnsCands; ++q) {
+      const int cc = kEnsChunkCands[q];
+      char what[48];
+      if (cc == 0)
+        std::snprintf(what, sizeof what, "own codes");
+      else
+        std::snprintf(what, sizeof what, "chunks %d rows %d", cc & 15, (cc >> 4) ? 4 : 8);
+      if (kv.second.ms[q] >= kNoPlan)
+        std::snprintf(b, sizeof b, " %s not batchable", what);
+      else if (kv.second.ms[q] < 0.f)
+        std::snprintf(b, sizeof b, " %s untried", what);
+      else
+        std::snprintf(b, sizeof b, " %s %.3f ms", what, kv.second.ms[q]);
       out += b;
     }
     out += "\n";
@@ -3637,7 +3684,23 @@ int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* c
       done = dispatch(states[0]->ctx, [&](auto r) {
         return ensemble_batched<decltype(r)>(states, n, *c, *wr, surrogates, nsteps);
       });
-    if (done < nsteps) ensemble_streams(states, n, c, wr, surrogates, nsteps - done);
+    if (done < nsteps) {
+      bool distinct = true;
+      for (int e = 0; e < n; ++e)
+        for (int f = 0; f < e; ++f) distinct = distinct && states[f]->ctx != states[e]->ctx;
+      if (distinct) {
+        ensemble_streams(states, n, c, wr, surrogates, nsteps - done);
+      } else {  // members sharing a context: one after the other
+        for (int e = 0; e < n; ++e)
+          for (int q = done; q < nsteps; ++q)
+            dispatch(states[e]->ctx, [&](auto r) {
+              moist_step_any<decltype(r)>(states[e], *c, *wr,
+                                          surrogates ? surrogates[e] : nullptr);
+              return 0;
+            });
+        for (int e = 0; e < n; ++e) HIPCHK(hipStreamSynchronize(states[e]->ctx->stream));
+      }
+    }
   });
 }
 

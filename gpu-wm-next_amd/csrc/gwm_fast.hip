@@ -41,8 +41,9 @@ namespace gwm {
 
 namespace {
 
-constexpr int TX = 64, TY = 8, PX = TX + 2, PY = TY + 2, PN = PX * PY;
-constexpr int NHALO = PN - TX * TY;  // 148 ring cells
+// Tile width: one wave per tile row.  The tile height TY (8 or 4) is a
+// template parameter chosen per launch shape by the autotuner.
+constexpr int TX = 64, PX = TX + 2;
 
 }  // namespace
 
@@ -52,9 +53,12 @@ constexpr int NHALO = PN - TX * TY;  // 148 ring cells
 enum { LV_PREF = 0, LV_RREF, LV_IDZF, LV_IDZC, LV_BRHO, LV_BTH, LV_BRT, LV_BP, NLV };
 
 // KIND: bit 0 first substep of the subcycle, bit 1 last substep.
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
 __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int tiles, int nkc,
                                           double* part_out, const VBlock& vb) {
+  constexpr int TY = TYV, PY = TY + 2, PN = PX * PY;
+  constexpr int NHALO = PN - TX * TY;  // ring cells (148 at TY 8, 140 at TY 4)
+  static_assert(NHALO <= TX * TY, "one thread per ring cell");
   constexpr bool FIRST = (KIND & 1) != 0, LAST = (KIND & 2) != 0;
   __shared__ R sp[4][PN];
   extern __shared__ __align__(16) unsigned char lev_raw[];
@@ -363,9 +367,14 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
       }
       sr = wave_sum_to63(sr);
       spp = wave_sum_to63(spp);
-      if (lx == TX - 1) {
-        const size_t nw = static_cast<size_t>(tiles) * TY;  // one wave per tile row
-        const size_t slot = (static_cast<size_t>(k) * nw + static_cast<size_t>(tile) * TY + ly) * 2;
+      // one partial per wave (a 64-column row segment), at (level, tile
+      // column, row): the layout and so the finalize's summation order do
+      // not depend on the tile height, and a work-group's waves write
+      // consecutive slots
+      if (lx == TX - 1 && j < g.ny) {
+        const size_t nw = static_cast<size_t>(g.ny) * tiles_x;
+        const size_t slot = (static_cast<size_t>(k) * nw +
+                             static_cast<size_t>(tile % tiles_x) * g.ny + static_cast<size_t>(j)) * 2;
         gstore(part_out, slot, sr);
         gstore(part_out, slot + 1, spp);
       }
@@ -387,84 +396,94 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   }
 }
 
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
-__global__ __launch_bounds__(TX * TY) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
-                                                         int nkc, double* part_out) {
-  fast_body<R, TERRAIN, COLREF, PERT, KIND>(a, tiles_x, tiles, nkc, part_out, hw_block());
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
+__global__ __launch_bounds__(TX * TYV) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
+                                                          int nkc, double* part_out) {
+  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV>(a, tiles_x, tiles, nkc, part_out, hw_block());
 }
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND>
-__global__ __launch_bounds__(TX * TY) void k_fast_column_batch(
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
+__global__ __launch_bounds__(TX * TYV) void k_fast_column_batch(
     const FastItem<R>* __restrict__ items, BatchMap bm) {
   VBlock vb;
   const FastItem<R>* it = batch_member(items, bm, vb);
   if (!it) return;
   // the member's arguments, read once before the kernel stores anything
   const FastArgs<R> a = it->a;
-  fast_body<R, TERRAIN, COLREF, PERT, KIND>(a, it->tiles_x, it->tiles, it->nkc, it->part_out, vb);
+  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV>(a, it->tiles_x, it->tiles, it->nkc, it->part_out,
+                                                 vb);
 }
 
-template <class R, bool TERRAIN, bool COLREF, bool PERT>
-void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, dim3 grid, dim3 blk, size_t lds,
-                      int tx, int tiles, int nkc, double* part_out) {
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int TYV>
+void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, int nkc, double* part_out) {
+  const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TYV - 1) / TYV;
+  const int tiles = tx * ty;
+  const dim3 grid(tiles * nkc), blk(TX, TYV, 1);
+  // per-level LDS: the longest chunk plus the levels below and above it
+  const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * NLV * sizeof(R);
   const int kind = (a.first ? 1 : 0) | (a.last ? 2 : 0);
   if (recorder()) {
     const void* fns[4] = {
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 0>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 1>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 2>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 3>)};
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 0, TYV>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 1, TYV>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 2, TYV>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 3, TYV>)};
     record_launch(fns[kind], blk, lds, FastItem<R>{a, tx, tiles, nkc, part_out, 0, 0},
                   static_cast<int>(grid.x), 1);
     return;
   }
   switch (kind) {
     case 0:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0>), grid, blk, lds, st, a, tx,
-                         tiles, nkc, part_out);
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0, TYV>), grid, blk, lds, st, a,
+                         tx, tiles, nkc, part_out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 1>), grid, blk, lds, st, a, tx,
-                         tiles, nkc, part_out);
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 1, TYV>), grid, blk, lds, st, a,
+                         tx, tiles, nkc, part_out);
       break;
     case 2:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 2>), grid, blk, lds, st, a, tx,
-                         tiles, nkc, part_out);
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 2, TYV>), grid, blk, lds, st, a,
+                         tx, tiles, nkc, part_out);
       break;
     default:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 3>), grid, blk, lds, st, a, tx,
-                         tiles, nkc, part_out);
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 3, TYV>), grid, blk, lds, st, a,
+                         tx, tiles, nkc, part_out);
       break;
   }
 }
 
 template <class R>
-void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int nkc,
+void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int code,
                         double* part_out, int* nparts_out) {
-  const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TY - 1) / TY;
-  const int tiles = tx * ty;
-  if (nparts_out) *nparts_out = tiles * TY;
+  // code: level chunks (low 4 bits) and tile height (bit 4 set: 4 rows,
+  // else 8; flat launches without the column reference only)
+  int nkc = code & 15;
+  // 4-row tiles: flat launches, and the column-reference launches, whose
+  // whole columns give few work-groups (one per tile)
+  const bool ty4 = ((code >> 4) & 1) != 0 && (a.col_ref || !terrain);
+  if (nparts_out) *nparts_out = fast_column_parts(a.g.nx, a.g.ny);
   if (a.col_ref || nkc < 1) nkc = 1;
   if (nkc > a.g.nz) nkc = a.g.nz;
-  const dim3 grid(tiles * nkc), blk(TX, TY, 1);
-  // per-level LDS: the longest chunk plus the levels below and above it
-  const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * NLV * sizeof(R);
   if (a.col_ref) {
-    if (terrain)
-      launch_fast_kind<R, true, true, false>(st, a, grid, blk, lds, tx, tiles, 1, nullptr);
-    else
-      launch_fast_kind<R, false, true, false>(st, a, grid, blk, lds, tx, tiles, 1, nullptr);
+    if (terrain) {
+      if (ty4) launch_fast_kind<R, true, true, false, 4>(st, a, 1, nullptr);
+      else launch_fast_kind<R, true, true, false, 8>(st, a, 1, nullptr);
+    } else {
+      if (ty4) launch_fast_kind<R, false, true, false, 4>(st, a, 1, nullptr);
+      else launch_fast_kind<R, false, true, false, 8>(st, a, 1, nullptr);
+    }
   } else if (terrain) {
-    launch_fast_kind<R, true, false, false>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
+    launch_fast_kind<R, true, false, false, 8>(st, a, nkc, part_out);
   } else if (a.base.rho) {  // perturbation form (flat, doubly periodic)
-    launch_fast_kind<R, false, false, true>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
+    if (ty4) launch_fast_kind<R, false, false, true, 4>(st, a, nkc, part_out);
+    else launch_fast_kind<R, false, false, true, 8>(st, a, nkc, part_out);
   } else {
-    launch_fast_kind<R, false, false, false>(st, a, grid, blk, lds, tx, tiles, nkc, part_out);
+    if (ty4) launch_fast_kind<R, false, false, false, 4>(st, a, nkc, part_out);
+    else launch_fast_kind<R, false, false, false, 8>(st, a, nkc, part_out);
   }
 }
 
-int fast_column_parts(int nx, int ny) {
-  return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * TY;
-}
+// Level partials per level: one per 64-column row segment.
+int fast_column_parts(int nx, int ny) { return ((nx + TX - 1) / TX) * ny; }
 
 template void launch_fast_column<float>(hipStream_t, const FastArgs<float>&, bool, int, double*,
                                         int*);

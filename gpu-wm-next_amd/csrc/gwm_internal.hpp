@@ -326,10 +326,16 @@ struct Loader {
     return (si < 0 || sj < 0) ? R(0) : at(f, g.zf(si, sj, k));
   }
   // Metric column index with the reference's periodic wrap
-  // (dry_fast_modes.cu:24-30); callers only pass in-range columns when the
-  // axis is open.
-  __device__ int mi(int i) const { return g.per_x ? map_cell(i, g.nx, 1) : i; }
-  __device__ int mj(int j) const { return g.per_y ? map_cell(j, g.ny, 1) : j; }
+  // (dry_fast_modes.cu:24-30).  On an open axis a column beyond the edge
+  // (the closed outer faces, whose fluxes are selected away after an
+  // unconditional load) is clamped to the edge column, so the load stays
+  // inside the metric arrays.
+  __device__ int mi(int i) const {
+    return g.per_x ? map_cell(i, g.nx, 1) : (i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i));
+  }
+  __device__ int mj(int j) const {
+    return g.per_y ? map_cell(j, g.ny, 1) : (j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j));
+  }
 };
 
 // The work-group index and grid extent a kernel body works with.  A plain

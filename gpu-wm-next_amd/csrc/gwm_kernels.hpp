@@ -111,11 +111,13 @@ void launch_fp32_math(hipStream_t st, int fn, const float* x, float* y, size_t n
 template <class R>
 void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int nkc);
 // Column-marching fast substep (gwm_fast.hip).  With part_out (flat refs),
-// also writes the next substep's per-wave level partials [k][wave][2] and
-// returns their count per level in *nparts.
-// nkc: level chunks per column (1 = whole columns; terrain keeps 1).
+// also writes the next substep's level partials, one per 64-column row
+// segment, [k][row][tile column][2], and returns their count per level in
+// *nparts.  code: level chunks per column in the low 4 bits (1 = whole
+// columns; the column-reference path keeps 1) and, in bit 4, a tile height
+// of 4 rows instead of 8 (flat launches without the column reference).
 template <class R>
-void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int nkc,
+void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int code,
                         double* part_out, int* nparts);
 int fast_column_parts(int nx, int ny);
 template <class R>

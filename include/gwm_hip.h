@@ -174,13 +174,17 @@ int gwm_ctx_set_kernel_timing(gwm_ctx* ctx, int enable);
  * Returns launches and total milliseconds recorded since the last reset. */
 int gwm_ctx_kernel_stats(gwm_ctx* ctx, const char* name, long* launches, double* total_ms);
 int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx);
-/* Column kernels split each column into level chunks; by default the count
- * is chosen per kernel, shape and precision by timing the candidates
- * {1, 2, 4, 8} at first use (the autotuner).  chunks > 0 fixes it; 0 returns
- * to the autotuner.  Results do not depend on the choice.  No counterpart in
- * the reference (tuning control of this implementation). */
+/* Column kernels split each column into level chunks, and the flat fast
+ * kernel runs tiles of 8 or 4 rows (64 columns wide); by default both are
+ * chosen per kernel, grid shape and precision by timing the candidates
+ * (chunks {1, 2, 4, 8}, times the tile heights) at first use (the
+ * autotuner).  chunks > 0 fixes the launch code: the chunk count in the low
+ * 4 bits, bit 4 set for 4-row tiles; 0 returns to the autotuner.  Results
+ * do not depend on the choice.  No counterpart in the reference (tuning
+ * control of this implementation). */
 int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks);
-/* "kernel:NXxNYxNZ:prec:variant=chunks;..." for every tuned launch shape. */
+/* "kernel:NXxNYxNZ:prec:variant=code;..." for every tuned launch shape
+ * (code as for gwm_ctx_set_chunks). */
 int gwm_ctx_tuning_report(gwm_ctx* ctx, char* buf, size_t size);
 
 /* Diagnostic, no counterpart in the reference: evaluates one of the FP32
@@ -472,9 +476,10 @@ int gwm_ensemble_steps(gwm_state* const* states, int n, const gwm_step_config* c
                        const gwm_warm_rain_config* wr, gwm_surrogate* const* surrogates,
                        int nsteps);
 /* The batched ensemble's autotuner: per ensemble signature (member shapes
- * and forms), the time of a two-step pair for each level-chunk candidate
- * (members' own tuned counts, then 2, 4, 8 for all), -1 where untried; the
- * fastest is used once all are timed.  One line per signature. */
+ * and forms), the time of a two-step pair for each launch-code candidate
+ * (the members' own tuned codes; then for all members 2, 4, 8 level chunks
+ * with 8-row tiles), or "untried"; the fastest is used once all are timed.
+ * One line per signature. */
 int gwm_ensemble_tuning_report(char* buf, size_t size);
 /* The same with each member on its own context (stream): members run
  * concurrently, their two-step graph replays interleaved. */

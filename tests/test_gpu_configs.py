@@ -257,6 +257,55 @@ def test_c2_mountain_wave_relaxation(ctx32, ctx64):
     c.done()
 
 
+def test_c2_at_its_size_stays_inside_the_metric_arrays(ctx32, ctx64):
+    """C2 at 256x256x60 columns, one step in FP64 and FP32.  Its FP64
+    metric arrays end exactly on a 2 MiB boundary (256*256*60*8 B = 15 *
+    2 MiB), where a load one element past the end is not mapped: the slow
+    stage's closed outer x faces on the open sides load the metric of the
+    column beyond the edge before selecting their zero flux, and that
+    column is clamped to the edge one.  Both steps stay finite.  FP32
+    against FP64: rho, rho*theta and the horizontal momenta to 1e-5; w,
+    after one step only the start of the wave, carries the full form's
+    pressure rounding (one ulp of rho*theta moves p by ~10 mPa against
+    ~1 Pa horizontal differences): 1.4 % measured, bar 5 %."""
+    nx, ny, nz, dx = 256, 256, 60, 2000.0
+    zi = G.stretched_levels(nz, 50.0, 600.0, 20000.0)
+    ter = G.agnesi_ridge(nx, ny, dx, 0.5 * nx * dx, 10000.0, 400.0)
+    cfg = G.step_config(dt=0.25, fast_substeps=6)
+    bc = G.BoundaryConfig(G.BOUNDARY_RELAX, 5.0, G.BOUNDARY_FN(), None)
+    out = {}
+    for name, ctx in (("fp64", ctx64), ("fp32", ctx32)):
+        grid = G.Grid(ctx, nx, ny, nz, dx=dx, dy=dx, z_interfaces=zi, terrain=ter,
+                      periodic_x=False)
+        st = G.State(grid, 0)
+        if name == "fp64":
+            st.init_constant_n(theta_surface=288.0, brunt_vaisala=0.01, p_surface=1.0e5,
+                               u=10.0)
+            init = [st.get(f) for f in FIELDS]
+        else:
+            for f, v in zip(FIELDS, init):
+                st.set(f, v.astype(np.float32))
+        st.capture_boundary()
+        st.ssprk3_step(cfg, bc)
+        finite(st, f"C2 {name}")
+        out[name] = (grid, st)
+    e = errors(out["fp32"][1].get, _StateAsWorld(out["fp64"][1]), FIELDS)
+    c = Checks()
+    c(e, 1e-5, "C2 256x256x60 FP32 full form vs FP64, one step", DRY + ("u", "v"))
+    c(e, 5e-2, "C2 256x256x60 FP32 full form vs FP64, one step", ("w",))
+    c.done()
+
+
+class _StateAsWorld:
+    """A device state read like an oracle world (for errors())."""
+
+    def __init__(self, st):
+        self.st = st
+
+    def get(self, f):
+        return self.st.get(f)
+
+
 # -------------------------------------------------------------------- C3 ---
 # Warm-rain convection: dx = 1 km, 64 levels stretched 100 m -> 500 m to
 # 16 km, doubly periodic, the analytic sounding with +/-0.5 K theta noise

@@ -511,16 +511,19 @@ def test_budget_sums(ctx32, ctx64, prec):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-def test_level_chunks_do_not_change_results(ctx32, ctx64, prec):
-    """The column kernels split columns into level chunks (autotuned); every
-    chunk count must give bit-identical states, tracers included."""
+@pytest.mark.parametrize("ny", [16, 20])
+def test_level_chunks_do_not_change_results(ctx32, ctx64, prec, ny):
+    """The column kernels split columns into level chunks, and the flat fast
+    kernel runs tiles of 8 or 4 rows (both autotuned); every chunk count and
+    tile height must give bit-identical states, tracers included (ny 20:
+    the 8-row tiles end in a partial one, the 4-row tiles do not)."""
     ctx = ctx_for(prec, ctx32, ctx64)
-    nx, ny, nz = 130, 16, 12
+    nx, nz = 130, 12
     zi = np.linspace(0.0, 12000.0, nz + 1)
     grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
     ref = None
     try:
-        for chunks in (1, 2, 4, 8, 0):
+        for chunks in (1, 2, 4, 8, 16 | 1, 16 | 2, 16 | 8, 0):
             ctx.set_chunks(chunks)
             gs = G.State(grid, 3)
             gs.init_convection(1.0e5, 5, 0.5, 1000.0)
@@ -539,8 +542,40 @@ def test_level_chunks_do_not_change_results(ctx32, ctx64, prec):
     finally:
         ctx.set_chunks(0)
     report = ctx.tuning_report()
-    assert any(k.startswith("fast_column:130x16x12") for k in report), report
-    assert all(v in (1, 2, 4, 8) for v in report.values())
+    assert any(k.startswith(f"fast_column:130x{ny}x12") for k in report), report
+    assert all(v & 15 in (1, 2, 4, 8) and v >> 4 in (0, 1) for v in report.values())
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_column_reference_tile_heights_do_not_change_results(ctx32, ctx64, prec):
+    """On terrain the fast kernel keeps whole columns (the column reference
+    is a recurrence from the ground) and its tile height (8 or 4 rows) is
+    autotuned; both give bit-identical states on an open-x ridge grid whose
+    8-row tiles end in a partial one."""
+    ctx = ctx_for(prec, ctx32, ctx64)
+    nx, ny, nz, dx = 96, 20, 16, 2000.0
+    zi = G.stretched_levels(nz, 50.0, 600.0, 8000.0)
+    ter = G.agnesi_ridge(nx, ny, dx, 0.5 * nx * dx, 10000.0, 400.0)
+    grid = G.Grid(ctx, nx, ny, nz, dx=dx, dy=dx, z_interfaces=zi, terrain=ter,
+                  periodic_x=False)
+    cfg = G.step_config(dt=0.25, fast_substeps=4)
+    bc = G.BoundaryConfig(G.BOUNDARY_RELAX, 4.0, G.BOUNDARY_FN(), None)
+    out = []
+    try:
+        for code in (1, 16 | 1):
+            ctx.set_chunks(code)
+            st = G.State(grid, 0)
+            st.init_constant_n(theta_surface=288.0, brunt_vaisala=0.01, p_surface=1.0e5, u=10.0)
+            st.capture_boundary()
+            for _ in range(3):
+                st.ssprk3_step(cfg, bc)
+            assert st.summary()["nonfinite_values"] == 0
+            out.append([st.get(f) for f in FIELDS])
+    finally:
+        ctx.set_chunks(0)
+    assert np.abs(out[0][O.MOM_W]).max() > 0.0
+    for a, b, f in zip(out[0], out[1], FIELDS):
+        assert np.array_equal(a, b), f
 
 
 def _moist_pair(prec, ctx, seed=21, shape=(40, 24, 12)):
