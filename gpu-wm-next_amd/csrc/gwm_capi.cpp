@@ -311,6 +311,9 @@ struct gwm_state {
   // owned by the state and sized once for its grid, so the step graphs
   // captured from it keep a valid address.
   double* step_part = nullptr;
+  // Network-cell queue and segment counts of the surrogate warm-rain call
+  // (allocated at its first call, sized for the grid).
+  unsigned* sur_queue = nullptr;
   // Perturbation form (flat, doubly periodic, FP32): rho and rho*theta are
   // stored as departures from per-level bases, the FP64 level means of the
   // state they were built from; the device holds the bases in working
@@ -349,7 +352,7 @@ struct gwm_state {
       for (const void* q : *v) b.push_back(q);
     }
     for (const void* q : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
-                          static_cast<void*>(step_part)})
+                          static_cast<void*>(step_part), static_cast<void*>(sur_queue)})
       b.push_back(q);
     return b;
   }
@@ -430,8 +433,13 @@ struct gwm_state {
       for (void* p : *v)
         if (p) (void)hipFree(p);
     for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
-                    static_cast<void*>(step_part)})
+                    static_cast<void*>(step_part), static_cast<void*>(sur_queue)})
       if (p) (void)hipFree(p);
+  }
+  unsigned* surrogate_queue() {
+    if (!sur_queue)
+      sur_queue = static_cast<unsigned*>(dalloc(gwm::sur_queue_words(g()) * sizeof(unsigned)));
+    return sur_queue;
   }
   // Partial-sum buffer of the step's level means (both uses fit).
   double* level_part() {
@@ -2832,8 +2840,12 @@ void warm_rain_surrogate(gwm_state* S, const gwm_warm_rain_config& c, gwm_surrog
   gwm::launch_zero_counters(S->ctx->stream, M->nfall, 2);
   {
     Timed t(S->ctx, "warm_rain_surrogate");
-    gwm::launch_warm_rain_sur(S->ctx->stream, g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params,
-                              M->norm, M->nfall, S->base<float>());
+    gwm_state* O = S->view ? S->owner : S;
+    unsigned* q = O->surrogate_queue();
+    gwm::launch_warm_rain_sur(
+        S->ctx->stream,
+        gwm::SurArgs{g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params, M->norm, M->nfall,
+                     S->base<float>(), q, q + static_cast<size_t>(gwm::sur_segments(g)) * 64 * g.nz});
   }
   const float vt = static_cast<float>(c.rain_terminal_velocity);
   const float depth = static_cast<float>(c.sedimentation_layer_depth_m);

@@ -380,99 +380,160 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 // cells whose features all lie in the box use the network's tendencies,
 // the others the plain scheme (counted as fallback cells); cells without
 // condensate at or below saturation keep their state, the plain scheme's
-// exact answer there.
-//
-// Only the cells that take the network's answer go through it: a tile's
-// network cells are packed, in lane order, into the first rows of the
-// work-group's feature block, and its waves share the 16-cell groups that
-// hold them.  A tile of quiet or out-of-box cells runs no group; a full one
-// runs all sixteen, as the dense form does.  Each packed cell's outputs
-// depend on its own row only (the matrix product keeps columns apart), so
-// the results equal those of running every cell through the network.
-// counts[0] += fallback cells, counts[1] += network cells.
-__device__ __forceinline__ void warm_rain_sur_body(const Geo& g, const float* rho, float* rt,
-                                                   float* qv, float* qc, float* qr,
-                                                   const MicroCfg<float>& mc, const float* w,
-                                                   const SurNorm& nm,
-                                                   unsigned long long* counts,
-                                                   const Base<float>& base, const VBlock& vb) {
+// exact answer there.  Two passes (SurArgs): classify, then network.
+
+// Features of a cell and its class.  Without cloud or rain and at or below
+// saturation every source term of the plain scheme is zero (no
+// condensation, nothing to evaporate, convert or accrete): such a cell is
+// quiet and takes the plain scheme's answer, its own state written back.
+struct SurCell {
+  MicroCell<float> m;
+  bool inbox, quiet;
+};
+__device__ inline SurCell sur_cell(float rho, float rt0, float rqv, float rqc, float rqr,
+                                   const MicroCfg<float>& mc, const SurNorm& nm) {
+  SurCell s;
+  s.m = micro_prepare(rho, rt0, rqv, rqc, rqr);
+  const float f[kSurIn] = {s.m.T, s.m.p, s.m.rho, s.m.qv, s.m.qc, s.m.qr, mc.dt};
+  bool in = true;
+  for (int q = 0; q < kSurIn; ++q) in = in && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
+  s.inbox = in;
+  s.quiet = s.m.qc <= 0.0f && s.m.qr <= 0.0f && s.m.qv <= q_qsat(s.m.T, s.m.p);
+  return s;
+}
+
+// Pass 1: a work-group of 64 x 4 threads owns a 64 x 4 column tile, a wave
+// one row of it (its queue segment), a lane one column, walked top down
+// with the loads of the next two levels in flight.  Fallback cells get the
+// plain scheme, quiet cells their state written back; network cells are
+// appended to the wave's segment in (level, lane) order (ballot + mbcnt,
+// no atomics) and left for pass 2.  Lane 0 stores the segment's network
+// and fallback counts.
+__device__ __forceinline__ void sur_classify_body(const SurArgs& a, const VBlock& vb) {
+  const Geo g = a.g;
+  int bx, by;
+  xcd_block2d(vb, bx, by);
+  const int lane = threadIdx.x;
+  const int i = bx * 64 + lane;
+  const int j = by * 4 + threadIdx.y;
+  if (j >= g.ny) return;  // the whole wave
+  const int tiles_x = (g.nx + 63) >> 6;
+  const int nseg = tiles_x * g.ny;
+  const int seg = j * tiles_x + bx;
+  const bool col = i < g.nx;
+  unsigned* const q = a.queue + static_cast<size_t>(seg) * 64u * static_cast<unsigned>(g.nz);
+  const bool pert = a.base.rho != nullptr;
+  const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+  struct Lv {
+    float rho, rt, qv, qc, qr;
+  };
+  auto load = [&](size_t cc) -> Lv {
+    return {a.rho[cc], a.rt[cc], a.qv[cc], a.qc[cc], a.qr[cc]};
+  };
+  size_t c = g.cell(col ? i : 0, j, g.nz - 1);
+  Lv l1{}, l2{};
+  if (col) {
+    l1 = load(c);
+    if (g.nz > 1) l2 = load(c - plane);
+  }
+  unsigned nnet = 0, nfall = 0;  // wave-uniform
+  for (int k = g.nz - 1; k >= 0; --k, c -= plane) {
+    const Lv cur = l1;
+    l1 = l2;
+    if (col && k >= 2) l2 = load(c - 2 * plane);
+    SurCell s{};
+    if (col)
+      s = sur_cell(pert ? cur.rho + a.base.rho[k] : cur.rho, pert ? cur.rt + a.base.rt[k] : cur.rt,
+                   cur.qv, cur.qc, cur.qr, a.mc, a.nm);
+    const bool net = col && !s.quiet && s.inbox;
+    const unsigned long long bal = __ballot(net);
+    nfall += static_cast<unsigned>(__popcll(__ballot(col && !s.inbox)));
+    if (net) {
+      const unsigned below = __builtin_amdgcn_mbcnt_hi(
+          static_cast<unsigned>(bal >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u));
+      q[nnet + below] = static_cast<unsigned>(c);
+    } else if (col) {
+      MicroCell<float>& m = s.m;
+      const float th0 = m.th;
+      if (!s.quiet) micro_apply(m, a.mc);
+      a.qv[c] = m.rho * m.qv;
+      a.qc[c] = m.rho * m.qc;
+      a.qr[c] = m.rho * m.qr;
+      // perturbation form: the departure changes by rho * dtheta only
+      a.rt[c] = pert ? cur.rt + m.rho * (m.th - th0) : m.rho * m.th;
+    }
+    nnet += static_cast<unsigned>(__popcll(bal));
+  }
+  if (lane == 0) {
+    a.segs[seg] = nnet;
+    a.segs[nseg + seg] = nfall;
+  }
+}
+
+// Pass 2: work-group b takes segments b, b + G, ... in tiles of 256 listed
+// cells (one per lane; rows past the segment's count are zero).  Each lane
+// prepares its cell's features into LDS; the waves share the tile's 16-cell
+// MFMA groups; each lane then applies its own cell's outputs.  Every row of
+// a group except the last tile's tail holds a network cell, and each cell's
+// outputs depend on its own row only (the matrix product keeps columns
+// apart), so the results equal those of running every cell through the
+// network alone.  A work-group first adds its segments' counts to
+// a.counts (one atomic per counter) and returns at once when they list no
+// network cell, before loading the weights.
+__device__ __forceinline__ void sur_network_body(const SurArgs& a, const VBlock& vb) {
   __shared__ float sb[196];
   __shared__ float sfeat[kTile][8];
   __shared__ float sout[kTile][kSurOut];
-  __shared__ unsigned wnet[kTile / 64];
-  __shared__ unsigned cnt_fall, cnt_net;
-  if (threadIdx.x == 0) {
-    cnt_fall = 0;
-    cnt_net = 0;
+  const Geo g = a.g;
+  const int tiles_x = (g.nx + 63) >> 6;
+  const int nseg = tiles_x * g.ny;
+  unsigned net = 0, fall = 0;
+  for (int s = vb.x; s < nseg; s += vb.gx) {
+    net += a.segs[s];
+    fall += a.segs[nseg + s];
   }
-  load_bias(w, sb);
+  if (threadIdx.x == 0) {
+    if (fall) atomicAdd(&a.counts[0], static_cast<unsigned long long>(fall));
+    if (net) atomicAdd(&a.counts[1], static_cast<unsigned long long>(net));
+  }
+  if (net == 0) return;  // uniform over the work-group
+  load_bias(a.w, sb);
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   MlpRegs R;
-  load_regs(w, R, lane & 15, lane >> 4);
-  const int nxy = g.nx * g.ny;
-  const int n = nxy * g.nz;
-  const bool pert = base.rho != nullptr;
-  unsigned mine = 0, mine_net = 0;
-  for (int tile = vb.x; tile * kTile < n; tile += vb.gx) {
-    const int cell = tile * kTile + wv * 64 + lane;
-    const bool valid = cell < n;
-    size_t ci = 0;
-    MicroCell<float> m{};
-    float f[kSurIn];
-    float rtp = 0.0f;
-    if (valid) {
-      const int k = cell / nxy, rem = cell - k * nxy;
-      const int j = rem / g.nx, i = rem - j * g.nx;
-      ci = g.cell(i, j, k);
-      rtp = rt[ci];
-      m = micro_prepare(pert ? rho[ci] + base.rho[k] : rho[ci], pert ? rtp + base.rt[k] : rtp,
-                        qv[ci], qc[ci], qr[ci]);
-      f[0] = m.T; f[1] = m.p; f[2] = m.rho; f[3] = m.qv; f[4] = m.qc; f[5] = m.qr; f[6] = mc.dt;
-    } else {
-      for (int q = 0; q < kSurIn; ++q) f[q] = nm.xm[q];
-    }
-    bool inbox = true;
-    for (int q = 0; q < kSurIn; ++q) inbox = inbox && f[q] >= nm.lo[q] && f[q] <= nm.hi[q];
-    // Without cloud or rain and at or below saturation every source term
-    // of the plain scheme is zero (no condensation, nothing to evaporate,
-    // convert or accrete): such a cell takes the plain scheme's answer, its
-    // own state written back.
-    const bool quiet = valid && m.qc <= 0.0f && m.qr <= 0.0f && m.qv <= q_qsat(m.T, m.p);
-    const bool net = valid && !quiet && inbox;
-    // pack the network cells: slot = cells of the waves before + lanes below
-    const unsigned long long bal = __ballot(net);
-    const unsigned below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(bal >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo(
-                                                         static_cast<unsigned>(bal), 0u));
-    if (lane == 0) wnet[wv] = static_cast<unsigned>(__popcll(bal));
-    __syncthreads();
-    unsigned before = 0, total = 0;
-    for (int v = 0; v < kTile / 64; ++v) {
-      const unsigned c = wnet[v];
-      before += v < wv ? c : 0u;
-      total += c;
-    }
-    const unsigned slot = before + below;
-    if (net) {
-      for (int q = 0; q < kSurIn; ++q) sfeat[slot][q] = qdiv(f[q] - nm.xm[q], nm.xs[q]);
-      sfeat[slot][7] = 0.0f;
-    }
-    // rows past the last network cell up to the group boundary: zeros
-    const unsigned padded = (total + 15u) & ~15u;
-    if (threadIdx.x >= total && threadIdx.x < padded)
-      for (int q = 0; q < 8; ++q) sfeat[threadIdx.x][q] = 0.0f;
-    __syncthreads();
-    for (unsigned gq = wv; 16u * gq < total; gq += kTile / 64)
-      group_mlp(R, sb, sfeat + 16 * gq, sout + 16 * gq, lane);
-    __syncthreads();
-    if (valid) {
-      const float th0 = m.th;
-      if (quiet) {
-        if (!inbox) ++mine;
-      } else if (net) {
-        ++mine_net;
-        const float* o = sout[slot];
+  load_regs(a.w, R, lane & 15, lane >> 4);
+  const SurNorm& nm = a.nm;
+  const bool pert = a.base.rho != nullptr;
+  const unsigned plane = static_cast<unsigned>(g.pxc) * g.pyc;
+  const size_t cap = 64u * static_cast<size_t>(g.nz);
+  for (int s = vb.x; s < nseg; s += vb.gx) {
+    const unsigned cnt = a.segs[s];
+    const unsigned* q = a.queue + static_cast<size_t>(s) * cap;
+    for (unsigned t0 = 0; t0 < cnt; t0 += kTile) {
+      const unsigned idx = t0 + threadIdx.x;
+      const bool valid = idx < cnt;
+      unsigned c = 0;
+      float rtp = 0.0f;
+      MicroCell<float> m{};
+      if (valid) {
+        c = q[idx];
+        const unsigned k = c / plane;
+        rtp = a.rt[c];
+        m = micro_prepare(pert ? a.rho[c] + a.base.rho[k] : a.rho[c],
+                          pert ? rtp + a.base.rt[k] : rtp, a.qv[c], a.qc[c], a.qr[c]);
+        const float f[kSurIn] = {m.T, m.p, m.rho, m.qv, m.qc, m.qr, a.mc.dt};
+        for (int v = 0; v < kSurIn; ++v) sfeat[threadIdx.x][v] = qdiv(f[v] - nm.xm[v], nm.xs[v]);
+        sfeat[threadIdx.x][7] = 0.0f;
+      } else {
+        for (int v = 0; v < 8; ++v) sfeat[threadIdx.x][v] = 0.0f;
+      }
+      __syncthreads();
+      const unsigned rows = cnt - t0 < kTile ? cnt - t0 : kTile;
+      for (unsigned gq = wv; 16u * gq < rows; gq += kTile / 64)
+        group_mlp(R, sb, sfeat + 16 * gq, sout + 16 * gq, lane);
+      __syncthreads();
+      if (valid) {
+        const float* o = sout[threadIdx.x];
         // The source terms only exchange water between vapour, cloud and
         // rain.  Cloud and rain cannot lose more than they hold; vapour, the
         // reservoir, takes the opposite of their change (the network's own
@@ -484,44 +545,41 @@ __device__ __forceinline__ void warm_rain_sur_body(const Geo& g, const float* rh
         const float dth = o[3] * nm.ys[3] + nm.ym[3];
         const float dv = -(dc + dr);
         const float lam = m.qv + dv < 0.0f ? fmaxf(qdiv(m.qv, -dv), 0.0f) : 1.0f;
+        const float th0 = m.th;
         m.qv = fmaxf(m.qv + lam * dv, 0.0f);
         m.qc = fmaxf(m.qc + lam * dc, 0.0f);
         m.qr = fmaxf(m.qr + lam * dr, 0.0f);
         m.th = m.th + lam * dth;
-      } else {
-        micro_apply(m, mc);
-        ++mine;
+        a.qv[c] = m.rho * m.qv;
+        a.qc[c] = m.rho * m.qc;
+        a.qr[c] = m.rho * m.qr;
+        a.rt[c] = pert ? rtp + m.rho * (m.th - th0) : m.rho * m.th;
       }
-      qv[ci] = m.rho * m.qv;
-      qc[ci] = m.rho * m.qc;
-      qr[ci] = m.rho * m.qr;
-      // perturbation form: the departure changes by rho * dtheta only
-      rt[ci] = pert ? rtp + m.rho * (m.th - th0) : m.rho * m.th;
     }
-  }
-  if (mine) atomicAdd(&cnt_fall, mine);
-  if (mine_net) atomicAdd(&cnt_net, mine_net);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (cnt_fall) atomicAdd(&counts[0], static_cast<unsigned long long>(cnt_fall));
-    if (cnt_net) atomicAdd(&counts[1], static_cast<unsigned long long>(cnt_net));
   }
 }
 
-__global__ __launch_bounds__(256) void k_warm_rain_sur(Geo g, const float* rho, float* rt,
-                                                       float* qv, float* qc, float* qr,
-                                                       MicroCfg<float> mc, const float* w,
-                                                       SurNorm nm, unsigned long long* counts,
-                                                       Base<float> base) {
-  warm_rain_sur_body(g, rho, rt, qv, qc, qr, mc, w, nm, counts, base, hw_block());
+__global__ __launch_bounds__(256) void k_sur_classify(SurArgs a) {
+  sur_classify_body(a, hw_block());
 }
-__global__ __launch_bounds__(256) void k_warm_rain_sur_batch(const SurItem* __restrict__ items,
-                                                             BatchMap bm) {
+__global__ __launch_bounds__(256) void k_sur_classify_batch(const SurItem* __restrict__ items,
+                                                            BatchMap bm) {
   VBlock vb;
   const SurItem* it = batch_member(items, bm, vb);
   if (!it) return;
-  const SurItem a = *it;
-  warm_rain_sur_body(a.g, a.rho, a.rt, a.qv, a.qc, a.qr, a.mc, a.w, a.nm, a.counts, a.base, vb);
+  const SurArgs a = it->a;
+  sur_classify_body(a, vb);
+}
+__global__ __launch_bounds__(256) void k_sur_network(SurArgs a) {
+  sur_network_body(a, hw_block());
+}
+__global__ __launch_bounds__(256) void k_sur_network_batch(const SurItem* __restrict__ items,
+                                                           BatchMap bm) {
+  VBlock vb;
+  const SurItem* it = batch_member(items, bm, vb);
+  if (!it) return;
+  const SurArgs a = it->a;
+  sur_network_body(a, vb);
 }
 
 // Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
@@ -678,18 +736,19 @@ void launch_zero_counters(hipStream_t st, unsigned long long* p, int n) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
 }
 
-void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
-                          float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* counts, Base<float> base) {
-  const long cells = static_cast<long>(g.nx) * g.ny * g.nz;
+void launch_warm_rain_sur(hipStream_t st, const SurArgs& a) {
+  const dim3 b1(64, 4, 1), g1((a.g.nx + 63) / 64, (a.g.ny + 3) / 4);
+  const long nseg = sur_segments(a.g);
+  const int g2 = static_cast<int>(nseg < 2048 ? nseg : 2048);
   if (recorder()) {
-    record_launch(reinterpret_cast<const void*>(&k_warm_rain_sur_batch), dim3(256), 0,
-                  SurItem{g, rho, rt, qv, qc, qr, mc, w, nm, counts, base, 0, 0},
-                  static_cast<int>(infer_blocks(cells)), 1);
+    record_launch(reinterpret_cast<const void*>(&k_sur_classify_batch), b1, 0, SurItem{a, 0, 0},
+                  static_cast<int>(g1.x), static_cast<int>(g1.y));
+    record_launch(reinterpret_cast<const void*>(&k_sur_network_batch), dim3(256), 0,
+                  SurItem{a, 0, 0}, g2, 1);
     return;
   }
-  hipLaunchKernelGGL(k_warm_rain_sur, dim3(infer_blocks(cells)), dim3(256), 0, st, g, rho, rt,
-                     qv, qc, qr, mc, w, nm, counts, base);
+  hipLaunchKernelGGL(k_sur_classify, g1, b1, 0, st, a);
+  hipLaunchKernelGGL(k_sur_network, dim3(g2), dim3(256), 0, st, a);
 }
 
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,

@@ -53,9 +53,31 @@ void launch_sur_train_step(hipStream_t st, SurTrain& T, const float* x, const fl
                            const SurNorm& nm, bool update, double* loss_part, int* nloss);
 void launch_sur_infer(hipStream_t st, const float* w, const SurNorm& nm, const float* x, long n,
                       float* y);
-void launch_warm_rain_sur(hipStream_t st, const Geo& g, const float* rho, float* rt, float* qv,
-                          float* qc, float* qr, const MicroCfg<float>& mc, const float* w,
-                          const SurNorm& nm, unsigned long long* counts, Base<float> base);
+// Arguments of the surrogate warm-rain call.  The call runs in two passes:
+// the classify pass walks every cell (a wave per 64 columns, top down),
+// applies the plain scheme to the cells outside the training box and writes
+// quiet cells back, and lists each wave's network cells (cell offsets) in
+// its own queue segment; the network pass then evaluates only the listed
+// cells, packed 256 to a tile.  A segment is the 64 columns of one wave:
+// nseg = ceil(nx / 64) * ny segments of 64 * nz entries.
+struct SurArgs {
+  Geo g;
+  const float* rho;
+  float *rt, *qv, *qc, *qr;
+  MicroCfg<float> mc;
+  const float* w;
+  SurNorm nm;
+  unsigned long long* counts;  // [0] += fallback cells, [1] += network cells
+  Base<float> base;
+  unsigned* queue;  // nseg * 64 * nz cell offsets
+  unsigned* segs;   // [2][nseg]: network cells, fallback cells per segment
+};
+inline long sur_segments(const Geo& g) { return static_cast<long>((g.nx + 63) / 64) * g.ny; }
+// Device words the queue and segment counts of a grid need.
+inline size_t sur_queue_words(const Geo& g) {
+  return static_cast<size_t>(sur_segments(g)) * (64 * static_cast<size_t>(g.nz) + 2);
+}
+void launch_warm_rain_sur(hipStream_t st, const SurArgs& a);
 // Clears n (<= 64) device counters with a kernel (no memset node in a
 // captured graph).
 void launch_zero_counters(hipStream_t st, unsigned long long* p, int n);
@@ -65,14 +87,7 @@ void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, cons
 
 // Items of the batched forms (gwm_kernels.hpp, Recorder).
 struct SurItem {
-  Geo g;
-  const float* rho;
-  float *rt, *qv, *qc, *qr;
-  MicroCfg<float> mc;
-  const float* w;
-  SurNorm nm;
-  unsigned long long* counts;
-  Base<float> base;
+  SurArgs a;
   int gx, gy;
 };
 struct ZeroItem {
