@@ -1667,7 +1667,7 @@ int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks) {
 int gwm_fp32_math(gwm_ctx* ctx, int fn, const void* dev_x, void* dev_y, long n) {
   return guard([&] {
     require(ctx && dev_x && dev_y, "null argument");
-    require(fn >= GWM_MATH_POW_EOS && fn <= GWM_MATH_RCP, "unknown function");
+    require(fn >= GWM_MATH_POW_EOS && fn <= GWM_MATH_POW1PM1, "unknown function");
     require(n >= 0, "n must be >= 0");
     if (n == 0) return;
     HIPCHK(hipSetDevice(ctx->device));

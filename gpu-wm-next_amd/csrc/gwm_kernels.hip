@@ -609,7 +609,7 @@ __global__ void k_metrics(Geo g, const double* zif, const double* zc, const doub
   }
 }
 
-// Copy of owned + halo storage (whole buffer).
+// The FP32 arithmetic helpers, elementwise (gwm_fp32_math).
 __global__ void k_fp32_math(int fn, const float* x, float* y, size_t n) {
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -621,6 +621,7 @@ __global__ void k_fp32_math(int fn, const float* x, float* y, size_t n) {
     case 2: r = qexp(v); break;
     case 3: r = qlog1p(v); break;
     case 4: r = qexpm1(v); break;
+    case 6: r = p_pert(v, 1.0f, 1.0f); break;  // (1 + x)^gamma - 1
     default: r = qrcp(v); break;
   }
   y[t] = r;

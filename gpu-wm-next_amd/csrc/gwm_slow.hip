@@ -170,21 +170,31 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
   };
   auto load_lev = [&](int kk) -> Lev { return Lev{load_raw(own, kk), load_raw(hpos, kk)}; };
   // Stages level kk at LDS entry e.
+  // FP32 divides the cell velocities (and theta') by rho through one
+  // reciprocal (within an ulp of the quotients); FP64 divides each.
   auto stage_raw = [&](const Raw& q, int e, int kk) {
     R rr;
     if (pert) {  // theta' and p' of the departures
       const R rb = LV(kk, SL_BRHO);
       rr = rmax(rb + q.r, R(1.0e-6));
-      lds(Q_TH, kk & (NSL - 1), e) = theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
+      lds(Q_TH, kk & (NSL - 1), e) =
+          RAW ? (q.rt - LV(kk, SL_BTH) * q.r) * qrcp(rr) : theta_pert(q.r, q.rt, rb, LV(kk, SL_BTH));
       lds(Q_P, kk & (NSL - 1), e) = p_pert(q.rt, LV(kk, SL_BRT), LV(kk, SL_BP));
     } else {
       rr = rmax(q.r, R(1.0e-6));
       lds(Q_TH, kk & (NSL - 1), e) = q_theta(q.r, q.rt);
       lds(Q_P, kk & (NSL - 1), e) = q_pressure(q.r, q.rt);
     }
-    lds(Q_UC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.u0 + q.u1), rr);
-    lds(Q_VC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.v0 + q.v1), rr);
-    lds(Q_WC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.w0 + q.w1), rr);
+    if constexpr (RAW) {
+      const R ir = qrcp(rr);
+      lds(Q_UC, kk & (NSL - 1), e) = R(0.5) * (q.u0 + q.u1) * ir;
+      lds(Q_VC, kk & (NSL - 1), e) = R(0.5) * (q.v0 + q.v1) * ir;
+      lds(Q_WC, kk & (NSL - 1), e) = R(0.5) * (q.w0 + q.w1) * ir;
+    } else {
+      lds(Q_UC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.u0 + q.u1), rr);
+      lds(Q_VC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.v0 + q.v1), rr);
+      lds(Q_WC, kk & (NSL - 1), e) = qdiv(R(0.5) * (q.w0 + q.w1), rr);
+    }
     if constexpr (RAW) {
       lds(Q_U, kk & (NSL - 1), e) = q.u0;
       lds(Q_V, kk & (NSL - 1), e) = q.v0;

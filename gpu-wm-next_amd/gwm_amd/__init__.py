@@ -21,7 +21,7 @@ HEADER = os.path.join(REPO, "include", "gwm_hip.h")
 
 F32, F64 = 0, 1
 # gwm_fp32_math functions (GWM_MATH_*)
-MATH_POW_EOS, MATH_POW_EXNER, MATH_EXP, MATH_LOG1P, MATH_EXPM1, MATH_RCP = range(6)
+MATH_POW_EOS, MATH_POW_EXNER, MATH_EXP, MATH_LOG1P, MATH_EXPM1, MATH_RCP, MATH_POW1PM1 = range(7)
 RHO, RHO_THETA, MOM_U, MOM_V, MOM_W, PRESSURE = 0, 1, 2, 3, 4, 5
 TEND_RHO, TEND_RHO_THETA, TEND_U, TEND_V, TEND_W = 6, 7, 8, 9, 10
 PRECIP, TRACER0 = 11, 16

@@ -195,14 +195,16 @@ int gwm_ctx_tuning_report(gwm_ctx* ctx, char* buf, size_t size);
  *   GWM_MATH_EXP        exp(x)
  *   GWM_MATH_LOG1P      log(1+x)
  *   GWM_MATH_EXPM1      exp(x)-1
- *   GWM_MATH_RCP        1/x                                              */
+ *   GWM_MATH_RCP        1/x
+ *   GWM_MATH_POW1PM1    (1+x)^(1/(1-kappa)) - 1  (pressure departure)      */
 enum {
   GWM_MATH_POW_EOS = 0,
   GWM_MATH_POW_EXNER = 1,
   GWM_MATH_EXP = 2,
   GWM_MATH_LOG1P = 3,
   GWM_MATH_EXPM1 = 4,
-  GWM_MATH_RCP = 5
+  GWM_MATH_RCP = 5,
+  GWM_MATH_POW1PM1 = 6
 };
 int gwm_fp32_math(gwm_ctx* ctx, int fn, const void* dev_x, void* dev_y, long n);
 

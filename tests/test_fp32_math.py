@@ -50,6 +50,12 @@ CASES = [
     (G.MATH_EXPM1, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.7), np.expm1, 4.0, 1.0),
     (G.MATH_RCP, lambda r, n: np.exp(r.uniform(np.log(1e-6), np.log(1e6), n)).astype(np.float32),
      lambda x: 1.0 / x, 0.501, 0.26),
+    # the pressure departure's (1 + x)^gamma - 1: binomial series below
+    # |x| = 1/16, log1p then expm1 above (their errors compound)
+    (G.MATH_POW1PM1, lambda r, n: signed_log_uniform(r, n, 1e-9, 0.0624),
+     lambda x: np.expm1(np.float64(GAMMA) * np.log1p(x)), 2.0, 0.5),
+    (G.MATH_POW1PM1, lambda r, n: signed_log_uniform(r, n, 0.0625, 0.3),
+     lambda x: np.expm1(np.float64(GAMMA) * np.log1p(x)), 6.0, 1.5),
 ]
 
 
@@ -76,3 +82,6 @@ def test_fp32_helpers_special_values(ctx):
     one = np.ones(4, np.float32)
     assert np.all(ctx.fp32_math(G.MATH_POW_EOS, one) == 1.0)
     assert np.all(ctx.fp32_math(G.MATH_POW_EXNER, one) == 1.0)
+    # (1 + x)^gamma - 1 is 0 at 0 and gamma x for tiny x, on both sides
+    tiny = np.array([0.0, 1e-20, -1e-20], np.float32)
+    assert np.array_equal(ctx.fp32_math(G.MATH_POW1PM1, tiny), GAMMA * tiny)
