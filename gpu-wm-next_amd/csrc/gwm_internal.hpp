@@ -165,10 +165,11 @@ __device__ inline float qexpm1(float y) {
 }
 __device__ inline double qexpm1(double y) { return expm1(y); }
 // Division by a uniform grid step d with its reciprocal rd = 1 / d (correctly
-// rounded): FP32 multiplies and corrects with the residual, FP64 divides.
+// rounded): FP32 multiplies by rd (within an ulp of the quotient, one
+// instruction instead of three), FP64 divides.
 __device__ inline float qdivc(float x, float d, float rd) {
-  const float q = x * rd;
-  return __builtin_fmaf(__builtin_fmaf(-q, d, x), rd, q);
+  (void)d;
+  return x * rd;
 }
 __device__ inline double qdivc(double x, double d, double rd) {
   (void)rd;

@@ -266,6 +266,13 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
   // and since loads complete in order, waiting for the base (needed first)
   // leaves level k+2 in flight.  Unrolled by two with the buffers swapped, so
   // no register copy waits for loads in flight.
+  // Vertical fluxes through the top faces of the last level computed, which
+  // are the bottom faces of the next one: the mass flux (w face) and the
+  // theta flux (its base part in the perturbation form), and the upwind
+  // fluxes of u, v and w through them.
+  struct Carry {
+    R m, th, thb, u, v, w;
+  } cz{};
   auto level = [&](int k, const Lev& Ln, Lev& Lnn) {
     Q0 Qc;
     if constexpr (PIPE) {
@@ -342,49 +349,69 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
       {
         const R mw = U(i, j, k), me = U(i + 1, j, k);
         const R ms = V(i, j, k), mn = V(i, j + 1, k);
-        const R mb = W(i, j, k), mt = w_up;
+        // below the chunk's first level the bottom face's values are the
+        // top face's of the level before (carried, bit for bit the same)
+        const bool carried = k > k0;
+        const R mb = carried ? cz.m : W(i, j, k), mt = w_up;
         const R idz = IDZC(i, j, k);
         const R div = qdivc(me - mw, dx, rdx) + qdivc(mn - ms, dy, rdy) + (mt - mb) * idz;
         t_rho = -div;
         const R thc = TH(i, j, k);
         const R thw = TH(i - 1, j, k), the = TH(i + 1, j, k);
         const R ths = TH(i, j - 1, k), thn = TH(i, j + 1, k);
-        const R thb = TH(i, j, kb), tht = TH(i, j, kt);
+        const R tht = TH(i, j, kt);
         const R fw = mw * (mw >= R(0) ? thw : thc);
         const R fe = me * (me >= R(0) ? thc : the);
         const R fs = ms * (ms >= R(0) ? ths : thc);
         const R fn = mn * (mn >= R(0) ? thc : thn);
-        const R fb = mb * (mb >= R(0) ? thb : thc);
+        R fb;
+        if (carried) {
+          fb = cz.th;
+        } else {
+          const R thb = TH(i, j, kb);
+          fb = mb * (mb >= R(0) ? thb : thc);
+        }
         const R ft = mt * (mt >= R(0) ? thc : tht);
+        cz.m = mt;
+        cz.th = ft;
         if (pert) {
           // theta = theta_b(level) + theta': the base parts of the horizontal
           // fluxes share theta_b(k); the vertical ones take the upwind level's.
           const R tb = LV(k, SL_BTH);
-          const R tbb = mb >= R(0) ? LV(kb, SL_BTH) : tb;
+          const R mbb = carried ? cz.thb : mb * (mb >= R(0) ? LV(kb, SL_BTH) : tb);
           const R tbt = mt < R(0) ? LV(kt, SL_BTH) : tb;
+          const R mtt = mt * tbt;
+          cz.thb = mtt;
           t_rt = -(qdivc(tb * (me - mw) + (fe - fw), dx, rdx) +
                    qdivc(tb * (mn - ms) + (fn - fs), dy, rdy) +
-                   ((mt * tbt - mb * tbb) + (ft - fb)) * idz);
+                   ((mtt - mbb) + (ft - fb)) * idz);
         } else {
           t_rt = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
         }
       }
 
       // ---- x-face (K17 + K21)
-      auto tend_u = [&](int fi, const auto& Uf) -> R {
+      auto tend_u = [&](int fi, const auto& Uf, R* carry) -> R {
         // closed outer faces and open-side fluxes are selected away, not
         // branched around: every read stays inside the staged tile and ring
         const bool closed = (fi == 0 && ow) || (fi == g.nx && oe);
         const R qw = Uf(fi - 1, j, k), qc = Uf(fi, j, k), qe = Uf(fi + 1, j, k);
         const R qs = Uf(fi, j - 1, k), qn = Uf(fi, j + 1, k);
-        const R qb = Uf(fi, j, kb), qt = Uf(fi, j, kt);
+        const R qt = Uf(fi, j, kt);
         const R fw = upwind(UC(fi - 1, j, k), qw, qc);
         const R fe = upwind(UC(fi, j, k), qc, qe);
         const R fs0 = upwind(R(0.5) * (VC(fi - 1, j - 1, k) + VC(fi, j - 1, k)), qs, qc);
         const R fn0 = upwind(R(0.5) * (VC(fi - 1, j, k) + VC(fi, j, k)), qc, qn);
         const R fs = (j == 0 && os) ? R(0) : fs0, fn = (j + 1 == g.ny && on) ? R(0) : fn0;
-        const R fb0 = upwind(R(0.5) * (WC(fi - 1, j, kb) + WC(fi, j, kb)), qb, qc);
+        R fb0;
+        if (carry && k > k0) {
+          fb0 = *carry;
+        } else {
+          const R qb = Uf(fi, j, kb);
+          fb0 = upwind(R(0.5) * (WC(fi - 1, j, kb) + WC(fi, j, kb)), qb, qc);
+        }
         const R ft0 = upwind(R(0.5) * (WC(fi - 1, j, k) + WC(fi, j, k)), qc, qt);
+        if (carry) *carry = ft0;
         const R fb = k > 0 ? fb0 : R(0), ft = k + 1 < nz ? ft0 : R(0);
         const R idz = TERRAIN ? R(0.5) * (IDZC(fi - 1, j, k) + IDZC(fi, j, k)) : IDZC(fi, j, k);
         const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
@@ -401,18 +428,25 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         return closed ? R(0) : A + (-term);
       };
       // ---- y-face (K18 + K22)
-      auto tend_v = [&](int fj, const auto& Vf) -> R {
+      auto tend_v = [&](int fj, const auto& Vf, R* carry) -> R {
         const bool closed = (fj == 0 && os) || (fj == g.ny && on);
         const R qw = Vf(i - 1, fj, k), qc = Vf(i, fj, k), qe = Vf(i + 1, fj, k);
         const R qs = Vf(i, fj - 1, k), qn = Vf(i, fj + 1, k);
-        const R qb = Vf(i, fj, kb), qt = Vf(i, fj, kt);
+        const R qt = Vf(i, fj, kt);
         const R fw0 = upwind(R(0.5) * (UC(i - 1, fj - 1, k) + UC(i - 1, fj, k)), qw, qc);
         const R fe0 = upwind(R(0.5) * (UC(i, fj - 1, k) + UC(i, fj, k)), qc, qe);
         const R fw = (i == 0 && ow) ? R(0) : fw0, fe = (i + 1 == g.nx && oe) ? R(0) : fe0;
         const R fs = upwind(VC(i, fj - 1, k), qs, qc);
         const R fn = upwind(VC(i, fj, k), qc, qn);
-        const R fb0 = upwind(R(0.5) * (WC(i, fj - 1, kb) + WC(i, fj, kb)), qb, qc);
+        R fb0;
+        if (carry && k > k0) {
+          fb0 = *carry;
+        } else {
+          const R qb = Vf(i, fj, kb);
+          fb0 = upwind(R(0.5) * (WC(i, fj - 1, kb) + WC(i, fj, kb)), qb, qc);
+        }
         const R ft0 = upwind(R(0.5) * (WC(i, fj - 1, k) + WC(i, fj, k)), qc, qt);
+        if (carry) *carry = ft0;
         const R fb = k > 0 ? fb0 : R(0), ft = k + 1 < nz ? ft0 : R(0);
         const R idz = TERRAIN ? R(0.5) * (IDZC(i, fj - 1, k) + IDZC(i, fj, k)) : IDZC(i, fj, k);
         const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
@@ -434,15 +468,24 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         const int kl = k - 1, ku = k;
         const R qw = W(i - 1, j, k), qc = W(i, j, k), qe = W(i + 1, j, k);
         const R qs = W(i, j - 1, k), qn = W(i, j + 1, k);
-        const R qb = W(i, j, k - 1), qt = w_up;
+        const R qt = w_up;
         const R fw0 = upwind(R(0.5) * (UC(i - 1, j, kl) + UC(i - 1, j, ku)), qw, qc);
         const R fe0 = upwind(R(0.5) * (UC(i, j, kl) + UC(i, j, ku)), qc, qe);
         const R fs0 = upwind(R(0.5) * (VC(i, j - 1, kl) + VC(i, j - 1, ku)), qs, qc);
         const R fn0 = upwind(R(0.5) * (VC(i, j, kl) + VC(i, j, ku)), qc, qn);
         const R fw = (i == 0 && ow) ? R(0) : fw0, fe = (i + 1 == g.nx && oe) ? R(0) : fe0;
         const R fs = (j == 0 && os) ? R(0) : fs0, fn = (j + 1 == g.ny && on) ? R(0) : fn0;
-        const R fb = upwind(WC(i, j, kl), qb, qc);
+        // the level before computed this face's bottom flux as its top one
+        // unless it was the chunk's first level below face 1
+        R fb;
+        if (k > k0 && k > 1) {
+          fb = cz.w;
+        } else {
+          const R qb = W(i, j, k - 1);
+          fb = upwind(WC(i, j, kl), qb, qc);
+        }
         const R ft = upwind(WC(i, j, ku), qc, qt);
+        cz.w = ft;
         const R idz = IDZF(i, j, k);
         const R A = -(qdivc(fe - fw, dx, rdx) + qdivc(fn - fs, dy, rdy) + (ft - fb) * idz);
         const R tf = R(0.5) * (TH(i, j, kl) + TH(i, j, ku));
@@ -472,15 +515,15 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
       const unsigned yf = oy0 + static_cast<unsigned>(k) * pv;
       // the extra faces: x face nx (i == nx-1), y face ny (j == ny-1), z face nz
       const unsigned xfe = xf + 1u, yfe = yf + static_cast<unsigned>(g.pxc), zft = ci + pc;
-      const R tu0 = tend_u(i, U), tv0 = tend_v(j, V), tw0 = tend_w();
+      const R tu0 = tend_u(i, U, &cz.u), tv0 = tend_v(j, V, &cz.v), tw0 = tend_w();
       if constexpr (MODE == 0) {
         st32(a.out.rho, ci, t_rho);
         st32(a.out.rt, ci, t_rt);
         st32(a.out.u, xf, tu0);
         st32(a.out.v, yf, tv0);
         st32(a.out.w, ci, tw0);
-        if (i == g.nx - 1) st32(a.out.u, xfe, tend_u(g.nx, Ug));
-        if (j == g.ny - 1) st32(a.out.v, yfe, tend_v(g.ny, Vg));
+        if (i == g.nx - 1) st32(a.out.u, xfe, tend_u(g.nx, Ug, nullptr));
+        if (j == g.ny - 1) st32(a.out.v, yfe, tend_v(g.ny, Vg, nullptr));
         if (k == nz - 1) st32(a.out.w, zft, R(0));
       } else {
         const R mf = Phys<R>::mom_floor;
@@ -500,10 +543,10 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         st32(a.out.w, ci, fin(Qc.w, upd(W(i, j, k), tw0, mf)));
         if (i == g.nx - 1)
           st32(a.out.u, xfe,
-               fin(MODE == 2 ? ld32(a.q0.u, xfe) : R(0), upd(ld32(s.u, xfe), tend_u(g.nx, Ug), mf)));
+               fin(MODE == 2 ? ld32(a.q0.u, xfe) : R(0), upd(ld32(s.u, xfe), tend_u(g.nx, Ug, nullptr), mf)));
         if (j == g.ny - 1)
           st32(a.out.v, yfe,
-               fin(MODE == 2 ? ld32(a.q0.v, yfe) : R(0), upd(ld32(s.v, yfe), tend_v(g.ny, Vg), mf)));
+               fin(MODE == 2 ? ld32(a.q0.v, yfe) : R(0), upd(ld32(s.v, yfe), tend_v(g.ny, Vg, nullptr), mf)));
         if (k == nz - 1)
           st32(a.out.w, zft, fin(MODE == 2 ? ld32(a.q0.w, zft) : R(0), upd(w_top, R(0), mf)));
       }
