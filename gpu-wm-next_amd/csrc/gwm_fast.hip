@@ -354,15 +354,20 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
     wb0 = F.wb1;
 
     if (part_out) {  // level sums of the next substep (flat refs, K6/K7)
-      double sr = 0.0, spp = 0.0;
+      // A row's 64 values are summed in FP64, except in the FP32
+      // perturbation form: its values are departures from the level means,
+      // small and of either sign, whose FP32 sum (error below 4e-7 of the
+      // sum of their magnitudes) leaves the mean's error far below the
+      // departures' own rounding; the partial is stored in FP64.
+      using S = typename std::conditional<PERT && sizeof(R) == 4, float, double>::type;
+      S sr = S(0), spp = S(0);
       if (active) {
         if (a.part_theta) {  // theta_ref of the next stage (dry_core.cu:462-490)
-          sr = static_cast<double>(PERT ? theta_pert(rho_new, rt_new, LV(k, LV_BRHO),
-                                                     LV(k, LV_BTH))
-                                        : q_theta(rho_new, rt_new));
+          sr = static_cast<S>(PERT ? theta_pert(rho_new, rt_new, LV(k, LV_BRHO), LV(k, LV_BTH))
+                                   : q_theta(rho_new, rt_new));
         } else {
-          sr = static_cast<double>(rho_new);
-          spp = static_cast<double>(press(rho_new, th_k, k));
+          sr = static_cast<S>(rho_new);
+          spp = static_cast<S>(press(rho_new, th_k, k));
         }
       }
       sr = wave_sum_to63(sr);
@@ -375,8 +380,8 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         const size_t nw = static_cast<size_t>(g.ny) * tiles_x;
         const size_t slot = (static_cast<size_t>(k) * nw +
                              static_cast<size_t>(tile % tiles_x) * g.ny + static_cast<size_t>(j)) * 2;
-        gstore(part_out, slot, sr);
-        gstore(part_out, slot + 1, spp);
+        gstore(part_out, slot, static_cast<double>(sr));
+        gstore(part_out, slot + 1, static_cast<double>(spp));
       }
     }
 

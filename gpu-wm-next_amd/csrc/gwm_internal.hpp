@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 namespace gwm {
 
@@ -444,6 +445,22 @@ __device__ inline double wave_sum_to63(double v) {
   v += dpp_f64<0x140, 0xF>(v);  // row_mirror
   v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+// The same sum of floats (one DPP move per step instead of two, and the
+// move folds into the add).
+template <int CTRL, int ROWS>
+__device__ inline float dpp_f32(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+}
+__device__ inline float wave_sum_to63(float v) {
+  v += dpp_f32<0xB1, 0xF>(v);
+  v += dpp_f32<0x4E, 0xF>(v);
+  v += dpp_f32<0x141, 0xF>(v);
+  v += dpp_f32<0x140, 0xF>(v);
+  v += dpp_f32<0x142, 0xA>(v);
+  v += dpp_f32<0x143, 0xC>(v);
   return v;
 }
 
