@@ -2838,25 +2838,30 @@ void warm_rain_surrogate(gwm_state* S, const gwm_warm_rain_config& c, gwm_surrog
   float* qc = static_cast<float*>(S->field(GWM_TRACER0 + 1));
   float* qr = static_cast<float*>(S->field(GWM_TRACER0 + 2));
   gwm::launch_zero_counters(S->ctx->stream, M->nfall, 2);
-  {
-    Timed t(S->ctx, "warm_rain_surrogate");
-    gwm_state* O = S->view ? S->owner : S;
-    unsigned* q = O->surrogate_queue();
-    gwm::launch_warm_rain_sur(
-        S->ctx->stream,
-        gwm::SurArgs{g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params, M->norm, M->nfall,
-                     S->base<float>(), q, q + static_cast<size_t>(gwm::sur_segments(g)) * 64 * g.nz});
-  }
   const float vt = static_cast<float>(c.rain_terminal_velocity);
   const float depth = static_cast<float>(c.sedimentation_layer_depth_m);
   const int fallout = metric ? (vt > 0.0f ? 2 : 0) : ((vt > 0.0f && depth > 0.0f) ? 1 : 0);
-  if (fallout) {
-    Timed t(S->ctx, "fallout");
-    gwm::launch_fallout(S->ctx->stream, g, qr,
-                        static_cast<float*>(S->view ? S->owner->precip : S->precip),
-                        static_cast<const float*>(S->grid->idz_c),
-                        static_cast<const float*>(S->grid->lev_idz_c),
-                        static_cast<float>(c.dt), vt, depth, fallout, S->grid->terrain_nonzero);
+  {
+    // source terms and fallout (the classify pass carries the fallout down
+    // each column to its first network cell, a fallout pass finishes it)
+    Timed t(S->ctx, "warm_rain_surrogate");
+    gwm_state* O = S->view ? S->owner : S;
+    unsigned* q = O->surrogate_queue();
+    const size_t nq = static_cast<size_t>(gwm::sur_segments(g)) * 64 * g.nz;
+    unsigned* segs = q + nq;
+    unsigned* cols = segs + 2 * static_cast<size_t>(gwm::sur_segments(g));
+    gwm::SurArgs a{g, rho, rt, qv, qc, qr, micro_cfg(c), M->T.params, M->norm, M->nfall,
+                   S->base<float>(), q, segs};
+    a.precip = static_cast<float*>(O->precip);
+    a.idz_c = static_cast<const float*>(S->grid->idz_c);
+    a.lev_idz_c = static_cast<const float*>(S->grid->lev_idz_c);
+    a.fdt = static_cast<float>(c.dt);
+    a.vt = vt;
+    a.depth = depth;
+    a.fallout = fallout;
+    a.col_stop = reinterpret_cast<int*>(cols);
+    a.col_inc = reinterpret_cast<float*>(cols + static_cast<size_t>(g.nx) * g.ny);
+    gwm::launch_warm_rain_sur(S->ctx->stream, a, S->grid->terrain_nonzero);
   }
   if (nfall_out) {
     unsigned long long h = 0;

@@ -116,14 +116,15 @@ __global__ __launch_bounds__(256) void k_level_partials_batch(const PartialsItem
   level_partials_body(a.g, a.rho, a.aux, a.mode, a.partials, a.base, vb);
 }
 
-// One 256-thread block per level: thread t sums partials t, t+256, ... in
-// order over eight interleaved accumulators (eight loads in flight per
-// round: the kernel is bound by load latency, not bytes), a fixed DPP
-// reduction adds each wave's lanes (total in lane 63) and thread 0 adds the
-// four waves in order, so the result does not depend on scheduling.  The
-// kernel runs between every two fast substeps, so its length is the round
-// trips it waits for.
-constexpr int kFinalizeThreads = 256;
+// One 1024-thread block per level: thread t sums partials t, t+1024, ... in
+// order over four interleaved accumulators (a level's 4096 partials of the
+// 512 x 512 grid in one round of four loads per thread: the kernel is bound
+// by load latency, not bytes), a fixed DPP reduction adds each wave's lanes
+// (total in lane 63) and thread 0 adds the sixteen waves in a fixed tree, so
+// the result does not depend on scheduling.  The kernel runs between every
+// two fast substeps, so its length is the round trips it waits for.
+constexpr int kFinalizeThreads = 1024;
+constexpr int kFinalizeAcc = 4;
 
 template <class R>
 __device__ __forceinline__ void level_finalize_body(int nchunk, const double* partials, int mode,
@@ -133,40 +134,46 @@ __device__ __forceinline__ void level_finalize_body(int nchunk, const double* pa
   const int k = vb.x;
   const int t = threadIdx.x;
   const double2* p = reinterpret_cast<const double2*>(partials) + static_cast<size_t>(k) * nchunk;
-  double a0[8], a1[8];
+  constexpr int A = kFinalizeAcc;
+  double a0[A], a1[A];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) a0[q] = a1[q] = 0.0;
+  for (int q = 0; q < A; ++q) a0[q] = a1[q] = 0.0;
   int c = t;
-  for (; c + 7 * kFinalizeThreads < nchunk; c += 8 * kFinalizeThreads) {
-    double2 v[8];
+  for (; c + (A - 1) * kFinalizeThreads < nchunk; c += A * kFinalizeThreads) {
+    double2 v[A];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = p[c + q * kFinalizeThreads];
+    for (int q = 0; q < A; ++q) v[q] = p[c + q * kFinalizeThreads];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < A; ++q) {
       a0[q] += v[q].x;
       a1[q] += v[q].y;
     }
   }
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < A; ++q) {
     if (c + q * kFinalizeThreads < nchunk) {
       const double2 v = p[c + q * kFinalizeThreads];
       a0[q] += v.x;
       a1[q] += v.y;
     }
   }
-  const double s0 = wave_sum_to63(((a0[0] + a0[1]) + (a0[2] + a0[3])) +
-                                  ((a0[4] + a0[5]) + (a0[6] + a0[7])));
-  const double s1 = wave_sum_to63(((a1[0] + a1[1]) + (a1[2] + a1[3])) +
-                                  ((a1[4] + a1[5]) + (a1[6] + a1[7])));
+  const double s0 = wave_sum_to63((a0[0] + a0[1]) + (a0[2] + a0[3]));
+  const double s1 = wave_sum_to63((a1[0] + a1[1]) + (a1[2] + a1[3]));
   if ((t & 63) == 63) {
     sh[0][t >> 6] = s0;
     sh[1][t >> 6] = s1;
   }
   __syncthreads();
   if (t != 0) return;
-  const double S0 = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-  const double S1 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+  // sixteen waves, pairwise
+  auto tree = [](const double* x) {
+    const double q0 = (x[0] + x[1]) + (x[2] + x[3]), q1 = (x[4] + x[5]) + (x[6] + x[7]);
+    const double q2 = (x[8] + x[9]) + (x[10] + x[11]), q3 = (x[12] + x[13]) + (x[14] + x[15]);
+    return (q0 + q1) + (q2 + q3);
+  };
+  static_assert(kFinalizeThreads == 1024 && kFinalizeAcc == 4, "tree sizes");
+  const double S0 = tree(sh[0]);
+  const double S1 = tree(sh[1]);
   if (mode == 0) {
     out0[k] = static_cast<R>(S0 / count);  // dry_core.cu:485-490
   } else {

@@ -382,6 +382,43 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 // condensate at or below saturation keep their state, the plain scheme's
 // exact answer there.  Two passes (SurArgs): classify, then network.
 
+// Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
+// warm_rain.cu:122-191), one level at a time down a column: `inc` carries
+// the rain entering the level from above.
+struct Fallout {
+  float dt, vt, dz_nom, rdz_nom, frac_nom;
+  bool nominal, metric;
+  __device__ Fallout(float dt_, float vt_, float depth, int mode) : dt(dt_), vt(vt_) {
+    dz_nom = fmaxf(depth, 1.0e-3f);
+    frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
+    rdz_nom = 1.0f / dz_nom;
+    // nothing falls out: the nominal sweep leaves the column alone
+    // (warm_rain.cu:134-136)
+    nominal = mode == 1 && frac_nom > 0.0f;
+    metric = mode == 2;
+  }
+  __device__ bool on() const { return nominal || metric; }
+  template <bool TERRAIN>
+  __device__ float level(float rqr, float& inc, const Geo& g, const float* idz_c,
+                         const float* lev_idz_c, int i, int j, int k) const {
+    if (nominal) {
+      const float old = fmaxf(rqr, 0.0f) * dz_nom;
+      const float out = frac_nom * old;
+      rqr = qdivc(old - out + inc, dz_nom, rdz_nom);
+      inc = out;
+    } else {
+      const float idz = fmaxf(TERRAIN ? idz_c[g.metric(i, j, k)] : lev_idz_c[k], 1.0e-6f);
+      const float dz = qrcp(idz);
+      const float frac = fminf(fmaxf(vt * dt * idz, 0.0f), 1.0f);
+      const float old = fmaxf(rqr, 0.0f) * dz;
+      const float out = frac * old;
+      rqr = (old - out + inc) * idz;
+      inc = out;
+    }
+    return rqr;
+  }
+};
+
 // Features of a cell and its class.  Without cloud or rain and at or below
 // saturation every source term of the plain scheme is zero (no
 // condensation, nothing to evaporate, convert or accrete): such a cell is
@@ -408,7 +445,11 @@ __device__ inline SurCell sur_cell(float rho, float rt0, float rqv, float rqc, f
 // plain scheme, quiet cells their state written back; network cells are
 // appended to the wave's segment in (level, lane) order (ballot + mbcnt,
 // no atomics) and left for pass 2.  Lane 0 stores the segment's network
-// and fallback counts.
+// and fallback counts.  The rain fallout follows each column down to its
+// first network cell (all the way, and into the bucket, in a column
+// without one); the column's stop level and the rain entering it are kept
+// for the fallout pass.
+template <bool TERRAIN>
 __device__ __forceinline__ void sur_classify_body(const SurArgs& a, const VBlock& vb) {
   const Geo g = a.g;
   int bx, by;
@@ -437,6 +478,9 @@ __device__ __forceinline__ void sur_classify_body(const SurArgs& a, const VBlock
     if (g.nz > 1) l2 = load(c - plane);
   }
   unsigned nnet = 0, nfall = 0;  // wave-uniform
+  const Fallout fo(a.fdt, a.vt, a.depth, a.fallout);
+  float inc = 0.0f, inc_stop = 0.0f;
+  int stop = -1;
   for (int k = g.nz - 1; k >= 0; --k, c -= plane) {
     const Lv cur = l1;
     l1 = l2;
@@ -452,13 +496,19 @@ __device__ __forceinline__ void sur_classify_body(const SurArgs& a, const VBlock
       const unsigned below = __builtin_amdgcn_mbcnt_hi(
           static_cast<unsigned>(bal >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u));
       q[nnet + below] = static_cast<unsigned>(c);
+      if (stop < 0) {
+        stop = k;
+        inc_stop = inc;
+      }
     } else if (col) {
       MicroCell<float>& m = s.m;
       const float th0 = m.th;
       if (!s.quiet) micro_apply(m, a.mc);
       a.qv[c] = m.rho * m.qv;
       a.qc[c] = m.rho * m.qc;
-      a.qr[c] = m.rho * m.qr;
+      float rqr = m.rho * m.qr;
+      if (fo.on() && stop < 0) rqr = fo.level<TERRAIN>(rqr, inc, g, a.idz_c, a.lev_idz_c, i, j, k);
+      a.qr[c] = rqr;
       // perturbation form: the departure changes by rho * dtheta only
       a.rt[c] = pert ? cur.rt + m.rho * (m.th - th0) : m.rho * m.th;
     }
@@ -467,6 +517,12 @@ __device__ __forceinline__ void sur_classify_body(const SurArgs& a, const VBlock
   if (lane == 0) {
     a.segs[seg] = nnet;
     a.segs[nseg + seg] = nfall;
+  }
+  if (col && fo.on()) {
+    const size_t cc = static_cast<size_t>(j) * g.nx + i;
+    if (stop < 0) a.precip[cc] += inc;
+    a.col_stop[cc] = stop;
+    a.col_inc[cc] = inc_stop;
   }
 }
 
@@ -559,16 +615,18 @@ __device__ __forceinline__ void sur_network_body(const SurArgs& a, const VBlock&
   }
 }
 
+template <bool TERRAIN>
 __global__ __launch_bounds__(256) void k_sur_classify(SurArgs a) {
-  sur_classify_body(a, hw_block());
+  sur_classify_body<TERRAIN>(a, hw_block());
 }
+template <bool TERRAIN>
 __global__ __launch_bounds__(256) void k_sur_classify_batch(const SurItem* __restrict__ items,
                                                             BatchMap bm) {
   VBlock vb;
   const SurItem* it = batch_member(items, bm, vb);
   if (!it) return;
   const SurArgs a = it->a;
-  sur_classify_body(a, vb);
+  sur_classify_body<TERRAIN>(a, vb);
 }
 __global__ __launch_bounds__(256) void k_sur_network(SurArgs a) {
   sur_network_body(a, hw_block());
@@ -582,53 +640,38 @@ __global__ __launch_bounds__(256) void k_sur_network_batch(const SurItem* __rest
   sur_network_body(a, vb);
 }
 
-// Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
-// warm_rain.cu:122-191), one thread per column, top down.
+// Rain fallout, one thread per column, top down from level nz-1, or (with
+// col_stop) from the column's col_stop level with col_inc entering it.
 template <bool TERRAIN>
-__device__ __forceinline__ void fallout_body(const Geo& g, float* qr, float* precip,
-                                             const float* idz_c, const float* lev_idz_c, float dt,
-                                             float vt, float depth, int fallout,
-                                             const VBlock& vb) {
+__device__ __forceinline__ void fallout_body(const FalloutItem& a, const VBlock& vb) {
+  const Geo& g = a.g;
   int bx, by;
   xcd_block2d(vb, bx, by);
   const int i = bx * blockDim.x + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
-  const float dz_nom = fmaxf(depth, 1.0e-3f);
-  const float frac_nom = fminf(fmaxf(vt * dt / dz_nom, 0.0f), 1.0f);
-  // nothing falls out: the nominal sweep leaves the column alone
-  // (warm_rain.cu:134-136)
-  if (fallout == 1 && !(frac_nom > 0.0f)) return;
-  const float rdz_nom = 1.0f / dz_nom;
+  const Fallout fo(a.dt, a.vt, a.depth, a.fallout);
+  if (!fo.on()) return;
+  const size_t cc = static_cast<size_t>(j) * g.nx + i;
+  int k0 = g.nz - 1;
   float inc = 0.0f;
+  if (a.col_stop) {
+    k0 = a.col_stop[cc];
+    if (k0 < 0) return;  // finished by the classify pass
+    inc = a.col_inc[cc];
+  }
   // unrolled so that the independent loads of several levels are in flight
 #pragma unroll 4
-  for (int k = g.nz - 1; k >= 0; --k) {
+  for (int k = k0; k >= 0; --k) {
     const size_t c = g.cell(i, j, k);
-    float rqr = qr[c];
-    if (fallout == 1) {
-      const float old = fmaxf(rqr, 0.0f) * dz_nom;
-      const float out = frac_nom * old;
-      rqr = qdivc(old - out + inc, dz_nom, rdz_nom);
-      inc = out;
-    } else {
-      const float idz = fmaxf(TERRAIN ? idz_c[g.metric(i, j, k)] : lev_idz_c[k], 1.0e-6f);
-      const float dz = qrcp(idz);
-      const float frac = fminf(fmaxf(vt * dt * idz, 0.0f), 1.0f);
-      const float old = fmaxf(rqr, 0.0f) * dz;
-      const float out = frac * old;
-      rqr = (old - out + inc) * idz;
-      inc = out;
-    }
-    qr[c] = rqr;
+    a.qr[c] = fo.level<TERRAIN>(a.qr[c], inc, g, a.idz_c, a.lev_idz_c, i, j, k);
   }
-  precip[static_cast<size_t>(j) * g.nx + i] += inc;
+  a.precip[cc] += inc;
 }
 
 template <bool TERRAIN>
-__global__ void k_fallout(Geo g, float* qr, float* precip, const float* idz_c,
-                          const float* lev_idz_c, float dt, float vt, float depth, int fallout) {
-  fallout_body<TERRAIN>(g, qr, precip, idz_c, lev_idz_c, dt, vt, depth, fallout, hw_block());
+__global__ void k_fallout(FalloutItem a) {
+  fallout_body<TERRAIN>(a, hw_block());
 }
 template <bool TERRAIN>
 __global__ void k_fallout_batch(const FalloutItem* __restrict__ items, BatchMap bm) {
@@ -636,8 +679,7 @@ __global__ void k_fallout_batch(const FalloutItem* __restrict__ items, BatchMap 
   const FalloutItem* it = batch_member(items, bm, vb);
   if (!it) return;
   const FalloutItem a = *it;
-  fallout_body<TERRAIN>(a.g, a.qr, a.precip, a.idz_c, a.lev_idz_c, a.dt, a.vt, a.depth, a.fallout,
-                        vb);
+  fallout_body<TERRAIN>(a, vb);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -736,39 +778,44 @@ void launch_zero_counters(hipStream_t st, unsigned long long* p, int n) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
 }
 
-void launch_warm_rain_sur(hipStream_t st, const SurArgs& a) {
+void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
+                    const float* lev_idz_c, float dt, float vt, float depth, int fallout,
+                    bool terrain, const int* col_stop, const float* col_inc) {
+  const dim3 b(64, 4, 1), gr((g.nx + 63) / 64, (g.ny + 3) / 4);
+  const FalloutItem it{g, qr, precip, idz_c, lev_idz_c, dt, vt, depth, fallout,
+                       col_stop, col_inc, 0, 0};
+  if (recorder()) {
+    const void* fn = terrain ? reinterpret_cast<const void*>(&k_fallout_batch<true>)
+                             : reinterpret_cast<const void*>(&k_fallout_batch<false>);
+    record_launch(fn, b, 0, it, static_cast<int>(gr.x), static_cast<int>(gr.y));
+    return;
+  }
+  if (terrain)
+    hipLaunchKernelGGL(k_fallout<true>, gr, b, 0, st, it);
+  else
+    hipLaunchKernelGGL(k_fallout<false>, gr, b, 0, st, it);
+}
+
+void launch_warm_rain_sur(hipStream_t st, const SurArgs& a, bool terrain) {
   const dim3 b1(64, 4, 1), g1((a.g.nx + 63) / 64, (a.g.ny + 3) / 4);
   const long nseg = sur_segments(a.g);
   const int g2 = static_cast<int>(nseg < 2048 ? nseg : 2048);
   if (recorder()) {
-    record_launch(reinterpret_cast<const void*>(&k_sur_classify_batch), b1, 0, SurItem{a, 0, 0},
-                  static_cast<int>(g1.x), static_cast<int>(g1.y));
+    const void* fn = terrain ? reinterpret_cast<const void*>(&k_sur_classify_batch<true>)
+                             : reinterpret_cast<const void*>(&k_sur_classify_batch<false>);
+    record_launch(fn, b1, 0, SurItem{a, 0, 0}, static_cast<int>(g1.x), static_cast<int>(g1.y));
     record_launch(reinterpret_cast<const void*>(&k_sur_network_batch), dim3(256), 0,
                   SurItem{a, 0, 0}, g2, 1);
-    return;
+  } else {
+    if (terrain)
+      hipLaunchKernelGGL(k_sur_classify<true>, g1, b1, 0, st, a);
+    else
+      hipLaunchKernelGGL(k_sur_classify<false>, g1, b1, 0, st, a);
+    hipLaunchKernelGGL(k_sur_network, dim3(g2), dim3(256), 0, st, a);
   }
-  hipLaunchKernelGGL(k_sur_classify, g1, b1, 0, st, a);
-  hipLaunchKernelGGL(k_sur_network, dim3(g2), dim3(256), 0, st, a);
-}
-
-void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
-                    const float* lev_idz_c, float dt, float vt, float depth, int fallout,
-                    bool terrain) {
-  const dim3 b(64, 4, 1), gr((g.nx + 63) / 64, (g.ny + 3) / 4);
-  if (recorder()) {
-    const void* fn = terrain ? reinterpret_cast<const void*>(&k_fallout_batch<true>)
-                             : reinterpret_cast<const void*>(&k_fallout_batch<false>);
-    record_launch(fn, b, 0,
-                  FalloutItem{g, qr, precip, idz_c, lev_idz_c, dt, vt, depth, fallout, 0, 0},
-                  static_cast<int>(gr.x), static_cast<int>(gr.y));
-    return;
-  }
-  if (terrain)
-    hipLaunchKernelGGL(k_fallout<true>, gr, b, 0, st, g, qr, precip, idz_c, lev_idz_c, dt, vt,
-                       depth, fallout);
-  else
-    hipLaunchKernelGGL(k_fallout<false>, gr, b, 0, st, g, qr, precip, idz_c, lev_idz_c, dt, vt,
-                       depth, fallout);
+  if (a.fallout)
+    launch_fallout(st, a.g, a.qr, a.precip, a.idz_c, a.lev_idz_c, a.fdt, a.vt, a.depth, a.fallout,
+                   terrain, a.col_stop, a.col_inc);
 }
 
 }  // namespace gwm

@@ -71,19 +71,36 @@ struct SurArgs {
   Base<float> base;
   unsigned* queue;  // nseg * 64 * nz cell offsets
   unsigned* segs;   // [2][nseg]: network cells, fallback cells per segment
+  // Rain fallout of the adjusted rain (0 none, 1 nominal depth, 2 metric;
+  // time step fdt).  The classify pass carries it down each column until
+  // the column's first network cell; the fallout pass then finishes the
+  // columns that have one, from that level with the rain entering it.
+  float* precip;
+  const float* idz_c;
+  const float* lev_idz_c;
+  float fdt, vt, depth;
+  int fallout;
+  int* col_stop;   // [nx * ny] first network level of the column, -1 none
+  float* col_inc;  // [nx * ny] rain entering that level from above
 };
 inline long sur_segments(const Geo& g) { return static_cast<long>((g.nx + 63) / 64) * g.ny; }
-// Device words the queue and segment counts of a grid need.
+// Device words the queue, segment counts and column records of a grid need.
 inline size_t sur_queue_words(const Geo& g) {
-  return static_cast<size_t>(sur_segments(g)) * (64 * static_cast<size_t>(g.nz) + 2);
+  return static_cast<size_t>(sur_segments(g)) * (64 * static_cast<size_t>(g.nz) + 2) +
+         2 * static_cast<size_t>(g.nx) * g.ny;
 }
-void launch_warm_rain_sur(hipStream_t st, const SurArgs& a);
+// The call's passes: classify, network, and (fallout != 0) the fallout of
+// the columns that hold network cells.
+void launch_warm_rain_sur(hipStream_t st, const SurArgs& a, bool terrain);
 // Clears n (<= 64) device counters with a kernel (no memset node in a
 // captured graph).
 void launch_zero_counters(hipStream_t st, unsigned long long* p, int n);
+// Rain fallout of whole columns (col_stop null), or from each column's
+// col_stop level with col_inc entering it (columns with col_stop < 0 are
+// done).
 void launch_fallout(hipStream_t st, const Geo& g, float* qr, float* precip, const float* idz_c,
                     const float* lev_idz_c, float dt, float vt, float depth, int fallout,
-                    bool terrain);
+                    bool terrain, const int* col_stop = nullptr, const float* col_inc = nullptr);
 
 // Items of the batched forms (gwm_kernels.hpp, Recorder).
 struct SurItem {
@@ -103,6 +120,8 @@ struct FalloutItem {
   const float* lev_idz_c;
   float dt, vt, depth;
   int fallout;
+  const int* col_stop;
+  const float* col_inc;
   int gx, gy;
 };
 

@@ -250,3 +250,55 @@ def test_packed_network_cells_match_unpacked(ctx):
         assert np.array_equal(x[~mask], y[~mask]), f
     # the network's (large) changes were applied in the unmasked cells
     assert np.mean(a.get(17)[~mask & ~quiet] != c.get(17)[~mask & ~quiet]) > 0.5
+
+
+@pytest.mark.parametrize("metric", [True, False])
+def test_split_fallout_equals_whole_column_fallout(ctx, metric):
+    """The classify pass carries the rain fallout down each column until the
+    column's first network cell (all the way, into the bucket, in a column
+    without one); a fallout pass finishes the other columns from that level.
+    The result must be the fallout of whole columns applied after the source
+    terms: here, the surrogate call without fallout followed by the plain
+    warm rain with every source term switched off (its fallout alone).  Half
+    the columns get 20 g/kg more rain, above the box's rain bound, so every
+    cell there is a fallback cell and the column finishes in the classify
+    pass; the others hold network cells at various heights."""
+    base = dict(dt=5.0)
+    if not metric:
+        base["sedimentation_layer_depth_m"] = 500.0
+    _, a = small_state(ctx, seed=15)
+    _, b = small_state(ctx, seed=15)
+    rho = a.get(O.RHO)
+    cols = np.random.default_rng(4).random(rho.shape[1:]) < 0.5
+    for st in (a, b):
+        q = st.get(18)
+        q[:, cols] += 0.02 * rho[:, cols]
+        st.set(18, q)
+    sur = G.Surrogate(ctx, seed=5)
+    big = 1e30
+    hi = big * np.ones(7)
+    hi[5] = 0.01  # rain mixing ratio
+    sur.set_norm(x_mean=np.zeros(7), x_std=np.ones(7) * 1e3, y_mean=np.zeros(4),
+                 y_std=np.array([1e-4, 1e-4, 1e-4, 1.0]), box_lo=-big * np.ones(7), box_hi=hi)
+    sur.set_params((SO.init_xavier(7) * 3.0 + 0.5).astype(np.float32))
+    a.warm_rain_surrogate(G.warm_rain_config(rain_terminal_velocity=12.0, **base), sur,
+                          metric_fallout=metric)
+    fb, nw = sur.last_counts()
+    b.warm_rain_surrogate(G.warm_rain_config(rain_terminal_velocity=0.0, **base), sur,
+                          metric_fallout=metric)
+    assert sur.last_counts() == (fb, nw)
+    assert fb == rho.shape[0] * int(cols.sum()) and nw > 0
+    for f in (O.RHO_THETA, 16, 17):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    no_sources = G.warm_rain_config(rain_terminal_velocity=12.0, condensation_relaxation=0.0,
+                                    cloud_autoconversion_rate=0.0, rain_evaporation_rate=0.0,
+                                    enable_latent_heating=False, **base)
+    b.warm_rain(no_sources, metric_fallout=metric)
+    qa, qb = a.get(18).astype(np.float64), b.get(18).astype(np.float64)
+    # the plain call rounds rho * (rho q_r / rho) once more: an ulp or so
+    assert np.allclose(qa, qb, rtol=1e-6, atol=1e-12 * np.max(qb))
+    pa = a.get(O.PRECIP).astype(np.float64).reshape(cols.shape)
+    pb = b.get(O.PRECIP).astype(np.float64).reshape(cols.shape)
+    # rain reaches the ground in columns of both kinds
+    assert np.all(pb[cols] > 0) and np.any(pb[~cols] > 0)
+    assert np.allclose(pa, pb, rtol=1e-6, atol=1e-12 * np.max(pb))
