@@ -609,6 +609,7 @@ template <class R>
 void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int nkc) {
   const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TY - 1) / TY;
   const int tiles = tx * ty;
+  nkc &= 15;  // launch code: the chunk count (a tile-height field is the fast kernel's)
   nkc = nkc < 1 ? 1 : (nkc > a.g.nz ? a.g.nz : nkc);
   const dim3 blk(TX, TY, 1), grid(tiles * nkc);
   // per-level LDS: the longest chunk plus the levels below and above it
@@ -847,6 +848,7 @@ template <class R>
 void launch_tracer_column(hipStream_t st, const TracerArgs<R>& a, bool terrain, int nkc) {
   const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TY - 1) / TY;
   const int tiles = tx * ty;
+  nkc &= 15;  // launch code: the chunk count
   nkc = nkc < 1 ? 1 : (nkc > a.g.nz ? a.g.nz : nkc);
   for (int t0 = 0; t0 < a.ntr; t0 += 3) {
     TracerArgs<R> s = a;
