@@ -294,10 +294,18 @@ def error_leg(G, a, dt, sur):
     perturbation form, both with the surrogate, and the perturbation form
     without microphysics (what the surrogate stands in for); relative L2 of
     each against the FP64 plain step, and the dry-mass change of the
-    perturbation form."""
+    perturbation form.  Beside them, the conditioning of the trajectory:
+    the FP64 plain step started from the same state rounded to FP32 (every
+    field), whose distance from the reference is what FP32 storage of the
+    initial state alone does to the FP64 scheme; and the FP32 forms'
+    errors after 1, 2 and 5 steps."""
+    import numpy as np
     ctx64 = G.Context(0, G.F64)
     ctx32 = G.Context(0, G.F32)
     _, _, ref, zi = make_case(G, G.F64, a.nx, a.ny, a.nz, ctx64, moist=True)
+    _, _, twin, _ = make_case(G, G.F64, a.nx, a.ny, a.nz, ctx64, moist=True)
+    for f in (0, 1, 2, 3, 4, 16, 17, 18):
+        twin.set(f, twin.get(f).astype(np.float32).astype(np.float64))
     states = {}
     _, _, states["fp32_full"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32, moist=True)
     _, _, states["fp32_perturbation"], _ = make_case(G, G.F32, a.nx, a.ny, a.nz, ctx32,
@@ -318,8 +326,11 @@ def error_leg(G, a, dt, sur):
     wr = G.warm_rain_config(dt=dt, rain_terminal_velocity=12.0)
     m0 = column_mass(states["fp32_perturbation"], zi)
     masses = [m0]
-    for _ in range(a.error_steps):
+    fields = (0, 1, 2, 3, 4, 16, 17, 18)
+    by_step = {}
+    for step in range(1, a.error_steps + 1):
         ref.moist_step(cfg, wr)
+        twin.moist_step(cfg, wr)
         for name, st in states.items():
             if name in surs:
                 st.moist_step_surrogate(cfg, wr, surs[name])
@@ -329,13 +340,21 @@ def error_leg(G, a, dt, sur):
             else:
                 st.moist_step(cfg, wr)
         masses.append(column_mass(states["fp32_perturbation"], zi))
+        if step in (1, 2, 5) and step < a.error_steps:
+            e = rel_errors(G, ref, {"fp64_from_fp32_rounded_start": twin,
+                                    "fp32_perturbation": states["fp32_perturbation"],
+                                    "fp32_full": states["fp32_full"]}, fields)
+            by_step[step] = {name: v["worst"] for name, v in e.items()}
     finite = {name: st.summary()["nonfinite_values"] == 0 for name, st in states.items()}
     finite["fp64_plain"] = ref.summary()["nonfinite_values"] == 0
-    errs = rel_errors(G, ref, states, (0, 1, 2, 3, 4, 16, 17, 18))
+    errs = rel_errors(G, ref, dict(states, fp64_from_fp32_rounded_start=twin), fields)
+    by_step[a.error_steps] = {name: errs[name]["worst"]
+                              for name in ("fp64_from_fp32_rounded_start", "fp32_perturbation",
+                                           "fp32_full")}
     drift = [abs(m1 - m0) / m0 for m0, m1 in zip(masses[:-1], masses[1:])]
     return {"dt": dt, "steps": a.error_steps, "reference": "FP64 plain step on the device",
             "rel_l2_floor": "1e-3 of each field's physical scale",
-            "rel_l2": errs, "state_finite": finite,
+            "rel_l2": errs, "worst_by_step": by_step, "state_finite": finite,
             "perturbation_dry_mass_drift_per_step_max": max(drift) if drift else None,
             "perturbation_dry_mass_rel_change_total": abs(masses[-1] - m0) / m0}
 
