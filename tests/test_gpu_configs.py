@@ -19,8 +19,9 @@ multi-step trajectories; the bars here are the conditioning's:
     times the FP64 one); the thermodynamic fields to 1e-5;
   - FP32 perturbation form (flat, doubly periodic) vs FP64 oracle: 1e-4,
     the north-star bar (its departures carry the pressure differences at
-    their own relative precision; the bench measures 1e-6 on the full
-    512x512x64 case);
+    their own relative precision; on the moist 512x512x64 case the bench
+    measures 3.8e-5 after five steps and 3.0e-4 after ten, where the FP64
+    step started from the FP32-rounded state is itself 1.5e-4 away);
   - FP32 full form vs FP64 oracle: 5e-3, the reference formulation's own
     error (one ulp of rho*theta moves p by ~10 mPa against ~1 Pa
     horizontal differences);
