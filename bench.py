@@ -170,7 +170,7 @@ def run(G, precision, a, dt, timing=False, surrogate=None, ctx=None, perturbatio
         st.moist_steps_graph(cfg, wr, n, surrogate)
 
     if calibrate_copy:
-        other = G.State(grid, 3)
+        other = G.State(grid, 3, perturbation=perturbation)
         other.copy_from(st)
         ctx.sync()
         del other
