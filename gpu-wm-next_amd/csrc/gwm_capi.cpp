@@ -510,9 +510,13 @@ void level_theta_ref(gwm_state* S, const R* rho, const R* rt) {
 // are out-of-place and their results do not depend on the shape, so the
 // candidates run on the live arguments and every candidate writes identical
 // values.  Each candidate is timed over two launches after a warm-up.
+// `fine` adds the chunk counts 3, 6 and 12 (the slow and tracer stages,
+// whose resident work-groups per CU make the round count of a launch
+// matter); the fast substep keeps 1, 2, 4, 8, where the extra counts won
+// back-to-back pairs while timing slower launch by launch.
 template <class Launch>
 int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Launch&& launch,
-                int shapes = 1, bool whole_columns = false) {
+                int shapes = 1, bool whole_columns = false, bool fine = false) {
   if (ctx->force_chunks > 0) return ctx->force_chunks;
   char key[160];
   std::snprintf(key, sizeof key, "%s:%dx%dx%d:%s:%d", kname, g.nx, g.ny, g.nz,
@@ -527,8 +531,9 @@ int pick_chunks(gwm_ctx* ctx, const char* kname, const Geo& g, int variant, Laun
   // between rounds does not decide the choice.
   std::vector<int> cands;
   for (int sh = 0; sh < shapes; ++sh)
-    for (int n : {1, 2, 4, 8})
-      if (n == 1 || (!whole_columns && g.nz / n >= 4)) cands.push_back(n | (sh << 4));
+    for (int n : {1, 2, 3, 4, 6, 8, 12})
+      if (n == 1 || (!whole_columns && g.nz / n >= 4 && (fine || (n & (n - 1)) == 0)))
+        cands.push_back(n | (sh << 4));
   std::vector<float> t_min(cands.size(), 1e30f);
   for (int round = 0; round < 3; ++round)
     for (size_t c = 0; c < cands.size(); ++c) {
@@ -668,7 +673,7 @@ void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* c
     gwm::launch_slow_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
   };
   const int nkc = pick_chunks(S->ctx, "slow_column", g,
-                              (S->grid->terrain_nonzero ? 1 : 0) + 2 * mode, run);
+                              (S->grid->terrain_nonzero ? 1 : 0) + 2 * mode, run, 1, false, true);
   Timed t(S->ctx, "slow_stage");
   run(nkc);
 }
@@ -909,7 +914,8 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode,
       gwm::launch_tracer_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
     };
     const int nkc = pick_chunks(S->ctx, "tracer_column", g,
-                                (S->grid->terrain_nonzero ? 1 : 0) + 2 * S->ntr, run);
+                                (S->grid->terrain_nonzero ? 1 : 0) + 2 * S->ntr, run, 1, false,
+                                true);
     Timed tt(S->ctx, "tracer_stage");
     run(nkc);
   };

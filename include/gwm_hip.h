@@ -177,7 +177,8 @@ int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx);
 /* Column kernels split each column into level chunks, and the flat fast
  * kernel runs tiles of 8 or 4 rows (64 columns wide); by default both are
  * chosen per kernel, grid shape and precision by timing the candidates
- * (chunks {1, 2, 4, 8}, times the tile heights) at first use (the
+ * (chunks {1, 2, 4, 8}, and 3, 6, 12 for the slow and tracer stages, times
+ * the tile heights) at first use (the
  * autotuner).  chunks > 0 fixes the launch code: the chunk count in the low
  * 4 bits, bit 4 set for 4-row tiles; 0 returns to the autotuner.  Results
  * do not depend on the choice.  No counterpart in the reference (tuning
