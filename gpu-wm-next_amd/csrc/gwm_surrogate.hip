@@ -382,6 +382,8 @@ __global__ __launch_bounds__(256) void k_sur_infer(const float* w, SurNorm nm, c
 // condensate at or below saturation keep their state, the plain scheme's
 // exact answer there.  Two passes (SurArgs): classify, then network.
 
+namespace {
+
 // Rain fallout of the post-adjustment rain (K25 nominal / K26 metric,
 // warm_rain.cu:122-191), one level at a time down a column: `inc` carries
 // the rain entering the level from above.
@@ -438,6 +440,8 @@ __device__ inline SurCell sur_cell(float rho, float rt0, float rqv, float rqc, f
   s.quiet = s.m.qc <= 0.0f && s.m.qr <= 0.0f && s.m.qv <= q_qsat(s.m.T, s.m.p);
   return s;
 }
+
+}  // namespace
 
 // Pass 1: a work-group of 64 x 4 threads owns a 64 x 4 column tile, a wave
 // one row of it (its queue segment), a lane one column, walked top down
