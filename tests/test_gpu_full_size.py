@@ -14,7 +14,9 @@ leg uses):
 - graph-replayed steps equal eager steps bit for bit;
 - one surrogate warm-rain call with network cells everywhere conserves
   total water plus the surface bucket to FP32 rounding, leaves rho alone
-  and reports every cell in one class.
+  and reports every cell in one class;
+- the surrogate call on the full grid (two queue segments per network-pass
+  work-group) equals the calls on its two halves (one each) bit for bit.
 """
 import numpy as np
 import pytest
@@ -142,3 +144,40 @@ def test_full_size_surrogate_call_conserves_water(ctxs, levels, tracers):
     print("relative change of water + bucket", (w1 - w0) / w0)
     # FP32 storage of rho q in every cell: a few ulp of the total
     assert abs(w1 - w0) <= 1e-6 * w0
+
+
+def test_full_size_surrogate_call_equals_halves(ctxs, levels, tracers):
+    """The network pass takes its queue segments (one per 64-column row
+    segment) two to a work-group on this grid (4096 segments, 2048
+    work-groups) and one on a 512x256 half; each cell's result depends on
+    its own features only, so the full grid's call must equal the calls on
+    its two halves bit for bit (FP32 full form, whose stored values carry
+    over between grids exactly)."""
+    c32, _ = ctxs
+    _, full = moist_state(c32, levels, tracers)
+    sur = G.Surrogate(c32, seed=12)
+    big = 1e30
+    sur.set_norm(x_mean=np.zeros(7), x_std=np.ones(7) * 1e3, y_mean=np.zeros(4),
+                 y_std=np.array([1e-4, 1e-4, 1e-4, 1.0]), box_lo=-big * np.ones(7),
+                 box_hi=big * np.ones(7))
+    sur.set_params((SO.init_xavier(6) * 3.0 + 0.5).astype(np.float32))
+    before = {f: full.get(f) for f in (0, 1, 16, 17, 18)}
+    halves = []
+    for h in range(2):
+        grid = G.Grid(c32, NX, NY // 2, NZ, dx=1000.0, dy=1000.0, z_interfaces=levels)
+        st = G.State(grid, 3)
+        for f, v in before.items():
+            st.set(f, np.ascontiguousarray(v[:, h * NY // 2:(h + 1) * NY // 2, :]))
+        halves.append((grid, st))
+    _, wr = cfgs()
+    full.warm_rain_surrogate(wr, sur, metric_fallout=True)
+    nw = sur.last_counts()[1]
+    nw_halves = 0
+    for _, st in halves:
+        st.warm_rain_surrogate(wr, sur, metric_fallout=True)
+        nw_halves += sur.last_counts()[1]
+    assert nw == nw_halves and nw > 0.5 * NX * NY * NZ
+    for f in (1, 16, 17, 18, G.PRECIP):
+        x = full.get(f)
+        y = np.concatenate([st.get(f) for _, st in halves], axis=1)
+        assert np.array_equal(x, y), f
