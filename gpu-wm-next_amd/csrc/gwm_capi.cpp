@@ -619,6 +619,11 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
     a.col_ref = colref;
     a.first = s == 0;
     a.last = s == ns - 1;
+    // Every other substep marches its columns from the top down, so it
+    // reads first the levels the substep before wrote last (still in the
+    // 256 MB last-level cache); the last substep marches down, so the
+    // upward-marching stage kernels after it do too.  Results do not change.
+    a.down = !colref && ((ns - 1 - s) % 2 == 0);
     a.base = S->base<R>();
     // level sums for the next substep, or (last substep, on request) the
     // theta_ref sums of the written state for the next stage

@@ -52,10 +52,15 @@ constexpr int TX = 64, PX = TX + 2;
 // planes and faces.
 enum { LV_PREF = 0, LV_RREF, LV_IDZF, LV_IDZC, LV_BRHO, LV_BTH, LV_BRT, LV_BP, NLV };
 
-// KIND: bit 0 first substep of the subcycle, bit 1 last substep.
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
+// KIND: bit 0 first substep of the subcycle, bit 1 last substep.  DOWN:
+// march the columns from the top down (whole columns of flat grids only),
+// so that a substep reads first the levels the previous substep, marching
+// up, wrote last; every cell's arithmetic is the upward march's, in the
+// same order, so the results are identical.
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV, bool DOWN>
 __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int tiles, int nkc,
                                           double* part_out, const VBlock& vb) {
+  static_assert(!DOWN || (!TERRAIN && !COLREF), "the downward march is for flat grids");
   constexpr int TY = TYV, PY = TY + 2, PN = PX * PY;
   constexpr int NHALO = PN - TX * TY;  // ring cells (148 at TY 8, 140 at TY 4)
   static_assert(NHALO <= TX * TY, "one thread per ring cell");
@@ -164,6 +169,8 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   const unsigned c_o = static_cast<unsigned>(g.cell(i, j, 0));
   const unsigned xf_o = static_cast<unsigned>(g.xf(i, j, 0));
   const unsigned yf_o = static_cast<unsigned>(g.yf(i, j, 0));
+  // w1 / wb1: the z face the level computes, k+1 marching up, k marching
+  // down (its departure and its base).
   struct Faces {
     R u0, u1, v0, v1, w1, ub0, ub1, vb0, vb1, wb1;
   };
@@ -172,7 +179,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
     const unsigned xf0 = xf_c + static_cast<unsigned>(k) * sxk;
     const unsigned yf0 = yf_c + static_cast<unsigned>(k) * syk;
     const unsigned yf1 = yf0 + static_cast<unsigned>(g.pxc);
-    const unsigned zf1 = zf_c + static_cast<unsigned>(k + 1) * pc;
+    const unsigned zf1 = zf_c + static_cast<unsigned>(DOWN ? k : k + 1) * pc;
     if constexpr (!FIRST) {
       F.u0 = ld32(a.u, xf0);
       F.u1 = ld32(a.u, xf0 + 1);
@@ -200,29 +207,54 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   // ---- prologue: planes k0-1 .. k0+1 staged, plane k0+2 and faces of k0
   // in flight, bottom w face.
   const int kmax = nz - 1 < k1 ? nz - 1 : k1;  // highest plane a chunk stages
+  // rho_k1 / th_k1 hold the next level of the march (k+1 up, k-1 down);
+  // w0 / wb0 the z face carried from the level before (the bottom face k
+  // marching up, the top face k+1 marching down).
   R rho_km1 = R(0);
-  if (k0 > 0) {
-    const Plane P = load_plane(k0 - 1);
-    stage_plane(P, k0 - 1);
-    rho_km1 = P.rc;
-  }
   R rho_k, th_k, rho_k1 = R(0), th_k1 = R(0);
-  {
-    const Plane P = load_plane(k0);
-    stage_plane(P, k0);
-    rho_k = P.rc;
-    th_k = P.tc;
-  }
-  if (k0 + 1 < nz) {
-    const Plane P = load_plane(k0 + 1);
-    stage_plane(P, k0 + 1);
-    rho_k1 = P.rc;
-    th_k1 = P.tc;
-  }
-  Plane P2 = load_plane(k0 + 2 < kmax ? k0 + 2 : kmax);
-  Faces F = load_faces(k0);
+  Plane P2;
+  Faces F;
   R w0, wb0;
-  {
+  if constexpr (DOWN) {
+    // whole columns: planes nz-1 and nz-2 staged, plane nz-3 and the faces
+    // of level nz-1 in flight, the top w face (left as it is).
+    {
+      const Plane P = load_plane(nz - 1);
+      stage_plane(P, nz - 1);
+      rho_k = P.rc;
+      th_k = P.tc;
+    }
+    if (nz > 1) {
+      const Plane P = load_plane(nz - 2);
+      stage_plane(P, nz - 2);
+      rho_k1 = P.rc;
+      th_k1 = P.tc;
+    }
+    P2 = load_plane(nz - 3 > 0 ? nz - 3 : 0);
+    F = load_faces(nz - 1);
+    const size_t zft = g.zf(ic, jc, nz);
+    w0 = FIRST ? R(0) : a.w[zft];
+    wb0 = LAST ? a.wb[zft] : R(0);
+  } else {
+    if (k0 > 0) {
+      const Plane P = load_plane(k0 - 1);
+      stage_plane(P, k0 - 1);
+      rho_km1 = P.rc;
+    }
+    {
+      const Plane P = load_plane(k0);
+      stage_plane(P, k0);
+      rho_k = P.rc;
+      th_k = P.tc;
+    }
+    if (k0 + 1 < nz) {
+      const Plane P = load_plane(k0 + 1);
+      stage_plane(P, k0 + 1);
+      rho_k1 = P.rc;
+      th_k1 = P.tc;
+    }
+    P2 = load_plane(k0 + 2 < kmax ? k0 + 2 : kmax);
+    F = load_faces(k0);
     const size_t zf0 = g.zf(ic, jc, k0);
     w0 = FIRST ? R(0) : a.w[zf0];
     wb0 = LAST ? a.wb[zf0] : R(0);
@@ -238,7 +270,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   }
 
   const bool ow = !g.per_x, os = !g.per_y;
-  if (active && k0 > 0) {
+  if (!DOWN && active && k0 > 0) {
     // K14 at face k0 from levels k0-1 and k0 (flat references).
     const R ppl = sp[(k0 - 1) & 3][ci] - LV(k0 - 1, LV_PREF);
     const R ppu = sp[k0 & 3][ci] - LV(k0, LV_PREF);
@@ -253,10 +285,16 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   // into P3 / Fn first.  The loop is unrolled by two with the buffers
   // swapped, so no register copy waits for loads in flight.
   auto level = [&](int k, const Plane& P2, const Faces& F, Plane& P3, Faces& Fn) {
-    P3 = load_plane(k + 3 < kmax ? k + 3 : kmax);
-    Fn = load_faces(k + 1 < k1 ? k + 1 : k1 - 1);
-    // Stage plane k+2.
-    if (k + 2 <= kmax) stage_plane(P2, k + 2);
+    // Stage the plane two levels ahead in the march (k+2 up, k-2 down).
+    if constexpr (DOWN) {
+      P3 = load_plane(k - 3 > 0 ? k - 3 : 0);
+      Fn = load_faces(k - 1 > 0 ? k - 1 : 0);
+      if (k - 2 >= 0) stage_plane(P2, k - 2);
+    } else {
+      P3 = load_plane(k + 3 < kmax ? k + 3 : kmax);
+      Fn = load_faces(k + 1 < k1 ? k + 1 : k1 - 1);
+      if (k + 2 <= kmax) stage_plane(P2, k + 2);
+    }
     const R rho_k2 = P2.rc, th_k2 = P2.tc;
 
     const R* pk = sp[k & 3];
@@ -311,28 +349,39 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         if (!(j == 0 && os)) v0 = F.v0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy));
         if (!(j + 1 == g.ny && os)) v1 = F.v1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy));
       }
-      // K14: w face k+1 (faces 0 and nz are left as they are).
-      R w1 = F.w1;
-      if (k + 1 < nz) {
+      // K14 (faces 0 and nz are left as they are): marching up, w face k+1
+      // from levels k and k+1; marching down, w face k from levels k-1 and
+      // k (the same arithmetic for a face either way).
+      auto k14 = [&](R wf, R p_lo, R p_hi, R pl, R pu, R r_lo, R r_hi, R rl, R ru, R idz) -> R {
+        const R ppl = p_lo - pl;
+        const R ppu = p_hi - pu;
+        const R rpf = R(0.5) * ((r_lo - rl) + (r_hi - ru));
+        const R tend = -(ppu - ppl) * idz - grav * rpf;
+        return wf + dtf * tend;
+      };
+      R w1 = F.w1;  // the face this level computes
+      if constexpr (DOWN) {
+        if (k > 0)
+          w1 = k14(F.w1, pdn[ci], pk[ci], LV(k - 1, LV_PREF), LV(k, LV_PREF), rho_k1, rho_k,
+                   LV(k - 1, LV_RREF), LV(k, LV_RREF), LV(k, LV_IDZF));
+      } else if (k + 1 < nz) {
         R rl, pl, ru, pu;
         if (COLREF) {
           rl = rref_k; pl = pref_k; ru = rref_k1; pu = pref_k1;
         } else {
           rl = LV(k, LV_RREF); pl = LV(k, LV_PREF); ru = LV(k + 1, LV_RREF); pu = LV(k + 1, LV_PREF);
         }
-        const R ppl = pk[ci] - pl;
-        const R ppu = pup[ci] - pu;
-        const R rpf = R(0.5) * ((rho_k - rl) + (rho_k1 - ru));
         const R idz = TERRAIN ? a.m.idz_f[own_m + (k + 1) * plane_m] : LV(k + 1, LV_IDZF);
-        const R tend = -(ppu - ppl) * idz - grav * rpf;
-        w1 = F.w1 + dtf * tend;
+        w1 = k14(F.w1, pk[ci], pup[ci], pl, pu, rho_k, rho_k1, rl, ru, idz);
       }
+      // the bottom and top w faces of the cell
+      const R wlo = DOWN ? w1 : w0, whi = DOWN ? w0 : w1;
       // K15
       rho_new = rho_k;
       if (a.density) {
         const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : LV(k, LV_IDZC);
         // divergence of (m - m_base): the faces are departures already
-        const R div = qdivc(u1 - u0, dx, rdx) + qdivc(v1 - v0, dy, rdy) + (w1 - w0) * idz;
+        const R div = qdivc(u1 - u0, dx, rdx) + qdivc(v1 - v0, dy, rdy) + (whi - wlo) * idz;
         const R floor = PERT ? R(1.0e-6) - LV(k, LV_BRHO) : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
@@ -345,10 +394,13 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
       // departures, or full faces base + departure on the last substep
       st32(a.u_out, xf0, LAST ? F.ub0 + u0 : u0);
       st32(a.v_out, yf0, LAST ? F.vb0 + v0 : v0);
-      st32(a.w_out, c0, LAST ? wb0 + w0 : w0);
+      // w face k (and face nz on the top level), with its base on the last
+      // substep: the carried face marching up, the computed one marching down
+      const R wb_lo = DOWN ? F.wb1 : wb0, wb_hi = DOWN ? wb0 : F.wb1;
+      st32(a.w_out, c0, LAST ? wb_lo + wlo : wlo);
       if (i == g.nx - 1) st32(a.u_out, xf0 + 1, LAST ? F.ub1 + u1 : u1);
       if (j == g.ny - 1) st32(a.v_out, yf0 + static_cast<unsigned>(g.pxc), LAST ? F.vb1 + v1 : v1);
-      if (k == nz - 1) st32(a.w_out, c0 + pc, LAST ? F.wb1 + w1 : w1);
+      if (k == nz - 1) st32(a.w_out, c0 + pc, LAST ? wb_hi + whi : whi);
       w0 = w1;
     }
     wb0 = F.wb1;
@@ -395,18 +447,26 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   };
   Plane P3;
   Faces Fn;
+  if constexpr (DOWN) {
+    for (int k = nz - 1; k >= 0; k -= 2) {
+      level(k, P2, F, P3, Fn);
+      if (k - 1 >= 0) level(k - 1, P3, Fn, P2, F);
+    }
+    return;
+  }
   for (int k = k0; k < k1; k += 2) {
     level(k, P2, F, P3, Fn);
     if (k + 1 < k1) level(k + 1, P3, Fn, P2, F);
   }
 }
 
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV, bool DOWN>
 __global__ __launch_bounds__(TX * TYV) void k_fast_column(FastArgs<R> a, int tiles_x, int tiles,
                                                           int nkc, double* part_out) {
-  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV>(a, tiles_x, tiles, nkc, part_out, hw_block());
+  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV, DOWN>(a, tiles_x, tiles, nkc, part_out,
+                                                       hw_block());
 }
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV>
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV, bool DOWN>
 __global__ __launch_bounds__(TX * TYV) void k_fast_column_batch(
     const FastItem<R>* __restrict__ items, BatchMap bm) {
   VBlock vb;
@@ -414,11 +474,11 @@ __global__ __launch_bounds__(TX * TYV) void k_fast_column_batch(
   if (!it) return;
   // the member's arguments, read once before the kernel stores anything
   const FastArgs<R> a = it->a;
-  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV>(a, it->tiles_x, it->tiles, it->nkc, it->part_out,
-                                                 vb);
+  fast_body<R, TERRAIN, COLREF, PERT, KIND, TYV, DOWN>(a, it->tiles_x, it->tiles, it->nkc,
+                                                       it->part_out, vb);
 }
 
-template <class R, bool TERRAIN, bool COLREF, bool PERT, int TYV>
+template <class R, bool TERRAIN, bool COLREF, bool PERT, int TYV, bool DOWN = false>
 void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, int nkc, double* part_out) {
   const int tx = (a.g.nx + TX - 1) / TX, ty = (a.g.ny + TYV - 1) / TYV;
   const int tiles = tx * ty;
@@ -428,29 +488,29 @@ void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, int nkc, double* par
   const int kind = (a.first ? 1 : 0) | (a.last ? 2 : 0);
   if (recorder()) {
     const void* fns[4] = {
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 0, TYV>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 1, TYV>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 2, TYV>),
-        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 3, TYV>)};
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 0, TYV, DOWN>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 1, TYV, DOWN>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 2, TYV, DOWN>),
+        reinterpret_cast<const void*>(&k_fast_column_batch<R, TERRAIN, COLREF, PERT, 3, TYV, DOWN>)};
     record_launch(fns[kind], blk, lds, FastItem<R>{a, tx, tiles, nkc, part_out, 0, 0},
                   static_cast<int>(grid.x), 1);
     return;
   }
   switch (kind) {
     case 0:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0, TYV>), grid, blk, lds, st, a,
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 0, TYV, DOWN>), grid, blk, lds, st, a,
                          tx, tiles, nkc, part_out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 1, TYV>), grid, blk, lds, st, a,
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 1, TYV, DOWN>), grid, blk, lds, st, a,
                          tx, tiles, nkc, part_out);
       break;
     case 2:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 2, TYV>), grid, blk, lds, st, a,
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 2, TYV, DOWN>), grid, blk, lds, st, a,
                          tx, tiles, nkc, part_out);
       break;
     default:
-      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 3, TYV>), grid, blk, lds, st, a,
+      hipLaunchKernelGGL((k_fast_column<R, TERRAIN, COLREF, PERT, 3, TYV, DOWN>), grid, blk, lds, st, a,
                          tx, tiles, nkc, part_out);
       break;
   }
@@ -459,9 +519,11 @@ void launch_fast_kind(hipStream_t st, const FastArgs<R>& a, int nkc, double* par
 template <class R>
 void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int code,
                         double* part_out, int* nparts_out) {
-  // code: level chunks (low 4 bits) and tile height (bit 4 set: 4 rows,
-  // else 8; flat launches without the column reference only)
+  // code: level chunks (low 4 bits), tile height (bit 4 set: 4 rows, else
+  // 8; flat launches without the column reference only), bit 6 set: every
+  // substep marches upwards (a.down ignored)
   int nkc = code & 15;
+  const bool down = a.down && (code & 64) == 0;
   // 4-row tiles: flat launches, and the column-reference launches, whose
   // whole columns give few work-groups (one per tile)
   const bool ty4 = ((code >> 4) & 1) != 0 && (a.col_ref || !terrain);
@@ -478,6 +540,15 @@ void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int 
     }
   } else if (terrain) {
     launch_fast_kind<R, true, false, false, 8>(st, a, nkc, part_out);
+  } else if (down && nkc == 1) {
+    // whole columns marched from the top down (flat grids)
+    if (a.base.rho) {
+      if (ty4) launch_fast_kind<R, false, false, true, 4, true>(st, a, 1, part_out);
+      else launch_fast_kind<R, false, false, true, 8, true>(st, a, 1, part_out);
+    } else {
+      if (ty4) launch_fast_kind<R, false, false, false, 4, true>(st, a, 1, part_out);
+      else launch_fast_kind<R, false, false, false, 8, true>(st, a, 1, part_out);
+    }
   } else if (a.base.rho) {  // perturbation form (flat, doubly periodic)
     if (ty4) launch_fast_kind<R, false, false, true, 4>(st, a, nkc, part_out);
     else launch_fast_kind<R, false, false, true, 8>(st, a, nkc, part_out);

@@ -32,6 +32,7 @@ struct FastArgs {
   const R *u, *v, *w;     // face departures m - m_base (read unless first)
   const R *ub, *vb, *wb;  // faces at subcycle entry (read on the last substep)
   int first, last;        // first / last substep of the subcycle
+  int down;               // flat whole-column launches: march the columns top down
   R *rho_out, *u_out, *v_out, *w_out;  // face departures, full faces if last
   R* rt_out;  // rho*theta rebuild (last substep), may be null
   const R *rref_lev, *pref_lev;  // flat reference (nz)

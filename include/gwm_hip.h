@@ -180,8 +180,10 @@ int gwm_ctx_reset_kernel_stats(gwm_ctx* ctx);
  * (chunks {1, 2, 4, 8}, and 3, 6, 12 for the slow and tracer stages, times
  * the tile heights) at first use (the
  * autotuner).  chunks > 0 fixes the launch code: the chunk count in the low
- * 4 bits, bit 4 set for 4-row tiles; 0 returns to the autotuner.  Results
- * do not depend on the choice.  No counterpart in the reference (tuning
+ * 4 bits, bit 4 set for 4-row tiles, bit 6 set to keep every fast substep
+ * marching upwards (by default every other substep of a flat whole-column
+ * launch marches down); 0 returns to the autotuner.  Results do not depend
+ * on the choice.  No counterpart in the reference (tuning
  * control of this implementation). */
 int gwm_ctx_set_chunks(gwm_ctx* ctx, int chunks);
 /* "kernel:NXxNYxNZ:prec:variant=code;..." for every tuned launch shape

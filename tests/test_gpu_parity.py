@@ -523,7 +523,7 @@ def test_level_chunks_do_not_change_results(ctx32, ctx64, prec, ny):
     grid = G.Grid(ctx, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
     ref = None
     try:
-        for chunks in (1, 2, 3, 4, 6, 8, 16 | 1, 16 | 2, 16 | 3, 16 | 8, 0):
+        for chunks in (1, 2, 3, 4, 6, 8, 16 | 1, 16 | 2, 16 | 3, 16 | 8, 64 | 1, 64 | 16 | 1, 0):
             ctx.set_chunks(chunks)
             gs = G.State(grid, 3)
             gs.init_convection(1.0e5, 5, 0.5, 1000.0)
@@ -544,6 +544,37 @@ def test_level_chunks_do_not_change_results(ctx32, ctx64, prec, ny):
     report = ctx.tuning_report()
     assert any(k.startswith(f"fast_column:130x{ny}x12") for k in report), report
     assert all(v & 15 in (1, 2, 3, 4, 6, 8, 12) and v >> 4 in (0, 1) for v in report.values())
+
+
+@pytest.mark.parametrize("perturbation", [False, True])
+@pytest.mark.parametrize("ty4", [0, 16])
+def test_downward_march_does_not_change_results(ctx32, perturbation, ty4):
+    """Every other fast substep of a flat whole-column launch marches its
+    columns from the top down (launch code bit 6 keeps them all upward);
+    each cell's arithmetic is the same either way, so the moist steps give
+    bit-identical states, in the full and the perturbation form, on a grid
+    whose tiles end in partial ones."""
+    nx, ny, nz = 130, 20, 12
+    zi = np.linspace(0.0, 12000.0, nz + 1)
+    grid = G.Grid(ctx32, nx, ny, nz, dx=1000.0, dy=1000.0, z_interfaces=zi)
+    outs = []
+    try:
+        for code in (1 | ty4, 64 | 1 | ty4):
+            ctx32.set_chunks(code)
+            gs = G.State(grid, 3, perturbation=perturbation)
+            gs.init_convection(1.0e5, 5, 0.5, 1000.0)
+            qv = gs.get(O.TRACER0)
+            gs.set(O.TRACER0 + 1, (0.02 * qv).astype(qv.dtype))
+            for _ in range(2):
+                gs.moist_step(G.step_config(dt=2.0, fast_substeps=4),
+                              G.warm_rain_config(dt=2.0, rain_terminal_velocity=12.0))
+            assert gs.summary()["nonfinite_values"] == 0
+            outs.append([gs.get_f64(f) for f in FIELDS + (O.TRACER0, O.TRACER0 + 1,
+                                                          O.TRACER0 + 2, O.PRECIP)])
+    finally:
+        ctx32.set_chunks(0)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("prec", [8, 4])
