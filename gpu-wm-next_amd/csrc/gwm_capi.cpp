@@ -318,16 +318,43 @@ struct gwm_state {
   // stored as departures from per-level bases, the FP64 level means of the
   // state they were built from; the device holds the bases in working
   // precision (rho_b, (rho theta)_b, theta_b, p_b).
+  //
+  // On terrain (pert3) the base is a 3-D field: the FP64 state it was built
+  // from, cell by cell (base_rho64 / base_rt64 over the cell storage); the
+  // device holds its level means as above, the per-cell deviations from
+  // them (b3_dev) and the base's pressure-gradient and gravity forcing per
+  // face (bf_dev), built for the gravity `forcing_g`.
   bool pert = false;
+  bool pert3 = false;
   std::vector<double> base_rho64, base_rt64;
-  void* b_dev = nullptr;  // 4 * nz values: rho_b, rt_b, theta_b, p_b
+  void* b_dev = nullptr;   // 4 * nz values: rho_b, rt_b, theta_b, p_b
+  void* b3_dev = nullptr;  // 3 cell fields: drho, dtheta, dp
+  void* bf_dev = nullptr;  // fu (x faces), fv (y faces), fw (z faces)
+  double forcing_g = 9.81;
   template <class R>
   gwm::Base<R> base() const {
     if (!pert) return gwm::Base<R>{nullptr, nullptr, nullptr, nullptr};
     const gwm_state* o = view ? owner : this;
     const R* b = static_cast<const R*>(o->b_dev);
     const int nz = grid->geo.nz;
-    return gwm::Base<R>{b, b + nz, b + 2 * nz, b + 3 * nz};
+    gwm::Base<R> r{b, b + nz, b + 2 * nz, b + 3 * nz};
+    if (o->pert3) {
+      const Geo& g = grid->geo;
+      const R* d = static_cast<const R*>(o->b3_dev);
+      const R* f = static_cast<const R*>(o->bf_dev);
+      r.drho = d;
+      r.dth = d + g.cell_total();
+      r.dp = d + 2 * g.cell_total();
+      r.fu = f;
+      r.fv = f + g.xf_total();
+      r.fw = f + g.xf_total() + g.yf_total();
+    }
+    return r;
+  }
+  // index of a cell storage position in the FP64 bases
+  size_t bidx(size_t a) const {
+    const Geo& g = grid->geo;
+    return pert3 ? a : a / (static_cast<size_t>(g.pxc) * g.pyc);
   }
   // Captured two-step moist-step graphs, one per buffer parity (the step
   // rotates the buffer sets and tracer buffers with period two).
@@ -351,8 +378,8 @@ struct gwm_state {
       b.push_back(reinterpret_cast<const void*>(v->size()));
       for (const void* q : *v) b.push_back(q);
     }
-    for (const void* q : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
-                          static_cast<void*>(step_part), static_cast<void*>(sur_queue)})
+    for (const void* q : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev, b3_dev,
+                          bf_dev, static_cast<void*>(step_part), static_cast<void*>(sur_queue)})
       b.push_back(q);
     return b;
   }
@@ -432,7 +459,7 @@ struct gwm_state {
     for (auto* v : {&tr, &tr_scr[0], &tr_scr[1], &tr_scr[2], &bref_tr})
       for (void* p : *v)
         if (p) (void)hipFree(p);
-    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev,
+    for (void* p : {pressure, precip, theta_ref, rref_lev, pref_lev, theta_scr, b_dev, b3_dev, bf_dev,
                     static_cast<void*>(step_part), static_cast<void*>(sur_queue)})
       if (p) (void)hipFree(p);
   }
@@ -454,6 +481,9 @@ struct gwm_state {
 };
 
 namespace {
+
+// Uploads the perturbation form's bases (defined with the conversions below).
+void push_base(gwm_state* S);
 
 gwm_state* S_(gwm_state* s) {
   require(s != nullptr, "null state");
@@ -568,8 +598,17 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
   const gwm_grid* G = S->grid;
   const int ns = std::max(1, c.fast_substeps);
   const R dtf = static_cast<R>(c.dt) / static_cast<R>(ns);
-  const bool colref = G->terrain_active;
-  if (!colref) {
+  // Terrain runs use the column reference (K11), other runs the level
+  // references; the perturbation form on terrain neither (K14 on the
+  // departures plus the base forcing, built for the step's gravity).
+  const bool tp = S->pert3;
+  const bool colref = G->terrain_active && !tp;
+  const bool flat = !G->terrain_active;
+  if (tp && S->forcing_g != c.gravity) {
+    S->forcing_g = c.gravity;
+    push_base(S);
+  }
+  if (flat) {
     if (!S->rref_lev) S->rref_lev = dalloc(g.nz * sizeof(R));
     if (!S->pref_lev) S->pref_lev = dalloc(g.nz * sizeof(R));
   }
@@ -579,11 +618,11 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
   int last = z0;
   // Level partials: substep 0 from a pass over the entry state, later
   // substeps from the previous substep's column kernel.
-  double* part = colref ? nullptr : S->level_part();
+  double* part = flat ? S->level_part() : nullptr;
   int nparts = 0;
   for (int s = 0; s < ns; ++s) {
     const int o = outs[s % 2];
-    if (!colref) {
+    if (flat) {
       if (s == 0) {
         Timed t(S->ctx, "level_partials");
         gwm::launch_level_partials<R>(S->ctx->stream, g, rho, theta, 1, part, &nparts,
@@ -623,12 +662,12 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
     // reads first the levels the substep before wrote last (still in the
     // 256 MB last-level cache); the last substep marches down, so the
     // upward-marching stage kernels after it do too.  Results do not change.
-    a.down = !colref && ((ns - 1 - s) % 2 == 0);
+    a.down = flat && ((ns - 1 - s) % 2 == 0);
     a.base = S->base<R>();
     // level sums for the next substep, or (last substep, on request) the
     // theta_ref sums of the written state for the next stage
-    const bool emit_th = !colref && theta_parts && s + 1 == ns;
-    const bool emit = (!colref && s + 1 < ns) || emit_th;
+    const bool emit_th = flat && theta_parts && s + 1 == ns;
+    const bool emit = (flat && s + 1 < ns) || emit_th;
     a.part_theta = emit_th ? 1 : 0;
     auto run = [&](int n) {
       gwm::launch_fast_column<R>(S->ctx->stream, a, G->terrain_nonzero, n,
@@ -638,7 +677,7 @@ int subcycle(gwm_state* S, const gwm_step_config& c, const R* rho_in, const R* t
     // column-reference launches keep whole columns and choose the height
     const int nkc =
         colref ? pick_chunks(S->ctx, "fast_column", g, 2, run, 2, /*whole_columns=*/true)
-               : pick_chunks(S->ctx, "fast_column", g, G->terrain_nonzero ? 1 : 0, run,
+               : pick_chunks(S->ctx, "fast_column", g, G->terrain_nonzero ? (tp ? 3 : 1) : 0, run,
                              G->terrain_nonzero ? 1 : 2);
     Timed t(S->ctx, "fast_column");
     run(nkc);
@@ -678,7 +717,8 @@ void slow_stage(gwm_state* S, const gwm_step_config& c, void* const* in, void* c
     gwm::launch_slow_column<R>(S->ctx->stream, a, S->grid->terrain_nonzero, n);
   };
   const int nkc = pick_chunks(S->ctx, "slow_column", g,
-                              (S->grid->terrain_nonzero ? 1 : 0) + 2 * mode, run, 1, false, true);
+                              (S->grid->terrain_nonzero ? 1 : 0) + 2 * mode + (S->pert3 ? 16 : 0),
+                              run, 1, false, true);
   Timed t(S->ctx, "slow_stage");
   run(nkc);
 }
@@ -988,6 +1028,7 @@ void reduce_field(gwm_state* S, const R* f, int fx, int fy, int fz, int kind, co
   a.rho = rho;
   a.fbase = fbase;
   a.rbase = rbase;
+  a.cb = S->base<R>();
   a.idz3 = S->grid->terrain_nonzero ? static_cast<const R*>(S->grid->idz_c) : nullptr;
   a.idz_lev = static_cast<const R*>(S->grid->lev_idz_c);
   a.fx = fx;
@@ -1057,11 +1098,20 @@ void budget(gwm_state* S, double* o, const ReduceBufs& rb = ReduceBufs{}) {
   // Perturbation form: sums of the departures plus the FP64 base sums, so
   // the mass budget keeps the departures' precision.
   double base_rho = 0.0, base_rt = 0.0;
-  if (S->pert)
+  const gwm_state* bo = S->view ? S->owner : S;
+  if (bo->pert3) {
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          base_rho += bo->base_rho64[g.cell(i, j, k)];
+          base_rt += bo->base_rt64[g.cell(i, j, k)];
+        }
+  } else if (S->pert) {
     for (int k = 0; k < g.nz; ++k) {
-      base_rho += static_cast<double>(g.nx) * g.ny * S->base_rho64[static_cast<size_t>(k)];
-      base_rt += static_cast<double>(g.nx) * g.ny * S->base_rt64[static_cast<size_t>(k)];
+      base_rho += static_cast<double>(g.nx) * g.ny * bo->base_rho64[static_cast<size_t>(k)];
+      base_rt += static_cast<double>(g.nx) * g.ny * bo->base_rt64[static_cast<size_t>(k)];
     }
+  }
   reduce_field<R>(S, rho, g.nx, g.ny, g.nz, 0, nullptr, r, B.rho, nullptr, rb);
   o[GWM_S_TOTAL_DRY_MASS] = r[0] + base_rho;
   extremes(GWM_S_RHO_MIN, GWM_S_RHO_MAX);
@@ -1201,7 +1251,135 @@ std::vector<double> level_means(const gwm_state* S, const T* full) {
   return m;
 }
 
+// Terrain form: the base of a field is the field itself (FP64), over the
+// whole cell storage; halo positions take the value of the owned cell they
+// stand for (periodic wrap, or the nearest owned cell across an open side),
+// so every stored position holds a valid base.
+template <class T>
+std::vector<double> cell_base(const gwm_state* S, const T* full) {
+  const Geo& g = S->g();
+  std::vector<double> b(g.cell_total());
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = -g.h; j < g.ny + g.h; ++j)
+      for (int i = -g.h; i < g.nx + g.h; ++i) {
+        const int oi = g.per_x ? ((i % g.nx) + g.nx) % g.nx : std::clamp(i, 0, g.nx - 1);
+        const int oj = g.per_y ? ((j % g.ny) + g.ny) % g.ny : std::clamp(j, 0, g.ny - 1);
+        b[g.cell(i, j, k)] = static_cast<double>(full[g.cell(oi, oj, k)]);
+      }
+  return b;
+}
+
+// The base of a field from full values: level means (flat) or the values
+// themselves (terrain).
+template <class T>
+std::vector<double> field_base(const gwm_state* S, const T* full) {
+  return S->pert3 ? cell_base(S, full) : level_means(S, full);
+}
+
+// Terrain form: level values, per-cell deviations and the base forcing,
+// from the FP64 bases.  The forcing is the base state's share of the fast
+// and slow pressure-gradient terms, in FP64 on the FP64 metrics (the
+// device metric formulas, grid_metrics.hpp:184-236): the terrain-following
+// horizontal gradient of p_b (dry_pressure_gradient.cu:37-73) on the x and
+// y faces, and -dp_b/dz - g rho_b on the interior z faces (K14 without a
+// reference state).
+void push_base3(gwm_state* S) {
+  const Geo& g = S->g();
+  const int nz = g.nz;
+  const size_t nc = g.cell_total();
+  const std::vector<double>& br = S->base_rho64;
+  const std::vector<double>& bt = S->base_rt64;
+  // level means over the owned cells
+  std::vector<double> lr(static_cast<size_t>(nz), 0.0), lt(static_cast<size_t>(nz), 0.0);
+  for (int k = 0; k < nz; ++k) {
+    double sr = 0.0, st = 0.0;
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        sr += br[g.cell(i, j, k)];
+        st += bt[g.cell(i, j, k)];
+      }
+    lr[static_cast<size_t>(k)] = sr / (static_cast<double>(g.nx) * g.ny);
+    lt[static_cast<size_t>(k)] = st / (static_cast<double>(g.nx) * g.ny);
+  }
+  std::vector<float> lev(4 * static_cast<size_t>(nz));
+  for (int k = 0; k < nz; ++k) {
+    const double r = lr[static_cast<size_t>(k)], t = lt[static_cast<size_t>(k)];
+    lev[k] = static_cast<float>(r);
+    lev[nz + k] = static_cast<float>(t);
+    lev[2 * nz + k] = static_cast<float>(t / r);
+    lev[3 * nz + k] = static_cast<float>(eos64(r, t));
+  }
+  // per-cell base pressure and the deviations
+  std::vector<double> pb(nc);
+  std::vector<float> dev(3 * nc);
+  const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
+  for (size_t a = 0; a < nc; ++a) {
+    const size_t k = a / plane;
+    pb[a] = eos64(br[a], bt[a]);
+    dev[a] = static_cast<float>(br[a] - static_cast<double>(lev[k]));
+    dev[nc + a] = static_cast<float>(bt[a] / br[a] - static_cast<double>(lev[2 * nz + k]));
+    dev[2 * nc + a] = static_cast<float>(pb[a] - static_cast<double>(lev[3 * nz + k]));
+  }
+  // FP64 metrics: z of the cell centres, zc[k] + wc[k] h(i, j)
+  const size_t n2 = static_cast<size_t>(g.nx) * g.ny;
+  std::vector<float> ter(n2, 0.0f);
+  scopy(S->ctx->stream, ter.data(), S->grid->terr, n2 * sizeof(float), hipMemcpyDeviceToHost);
+  auto zc = [&](int i, int j, int k) {
+    const double th = static_cast<double>(ter[static_cast<size_t>(j) * g.nx + i]);
+    const double prod = S->grid->w_c[static_cast<size_t>(k)] * th;
+    return S->grid->z_c[static_cast<size_t>(k)] + prod;
+  };
+  auto wrap_i = [&](int i) { return g.per_x ? ((i % g.nx) + g.nx) % g.nx : std::clamp(i, 0, g.nx - 1); };
+  auto wrap_j = [&](int j) { return g.per_y ? ((j % g.ny) + g.ny) % g.ny : std::clamp(j, 0, g.ny - 1); };
+  auto P = [&](int i, int j, int k) { return pb[g.cell(wrap_i(i), wrap_j(j), k)]; };
+  auto vgrad = [&](int i, int j, int k) -> double {  // dp_b/dz of column (i, j)
+    if (nz == 1) return 0.0;
+    const int ii = wrap_i(i), jj = wrap_j(j);
+    if (k == 0) return (P(ii, jj, 1) - P(ii, jj, 0)) / std::max(zc(ii, jj, 1) - zc(ii, jj, 0), 1.0e-6);
+    if (k + 1 == nz)
+      return (P(ii, jj, k) - P(ii, jj, k - 1)) / std::max(zc(ii, jj, k) - zc(ii, jj, k - 1), 1.0e-6);
+    return (P(ii, jj, k + 1) - P(ii, jj, k - 1)) / std::max(zc(ii, jj, k + 1) - zc(ii, jj, k - 1), 1.0e-6);
+  };
+  auto pgf = [&](int il, int jl, int ir, int jr, int k, double d) -> double {
+    const double dzd = (zc(wrap_i(ir), wrap_j(jr), k) - zc(wrap_i(il), wrap_j(jl), k)) / d;
+    return (P(ir, jr, k) - P(il, jl, k)) / d - dzd * (0.5 * (vgrad(il, jl, k) + vgrad(ir, jr, k)));
+  };
+  const size_t nxf = g.xf_total(), nyf = g.yf_total(), nzf = g.zf_total();
+  std::vector<float> frc(nxf + nyf + nzf, 0.0f);
+  const double grav = S->forcing_g;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i <= g.nx; ++i) {
+        if (!g.per_x && (i == 0 || i == g.nx)) continue;  // closed faces of open sides
+        frc[g.xf(i, j, k)] = static_cast<float>(pgf(i - 1, j, i, j, k, g.dx));
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        if (!g.per_y && (j == 0 || j == g.ny)) continue;
+        frc[nxf + g.yf(i, j, k)] = static_cast<float>(pgf(i, j - 1, i, j, k, g.dy));
+      }
+  for (int k = 1; k < nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double idz = 1.0 / std::max(zc(i, j, k) - zc(i, j, k - 1), 1.0e-6);
+        const size_t lo = g.cell(i, j, k - 1), hi = g.cell(i, j, k);
+        frc[nxf + nyf + g.zf(i, j, k)] =
+            static_cast<float>(-(pb[hi] - pb[lo]) * idz - grav * (0.5 * (br[lo] + br[hi])));
+      }
+  if (!S->b_dev) S->b_dev = dalloc(lev.size() * sizeof(float));
+  if (!S->b3_dev) S->b3_dev = dalloc(dev.size() * sizeof(float));
+  if (!S->bf_dev) S->bf_dev = dalloc(frc.size() * sizeof(float));
+  scopy(S->ctx->stream, S->b_dev, lev.data(), lev.size() * sizeof(float), hipMemcpyHostToDevice);
+  scopy(S->ctx->stream, S->b3_dev, dev.data(), dev.size() * sizeof(float), hipMemcpyHostToDevice);
+  scopy(S->ctx->stream, S->bf_dev, frc.data(), frc.size() * sizeof(float), hipMemcpyHostToDevice);
+}
+
 void push_base(gwm_state* S) {
+  if (S->pert3) {
+    push_base3(S);
+    return;
+  }
   const int nz = S->g().nz;
   std::vector<float> b(4 * static_cast<size_t>(nz));
   for (int k = 0; k < nz; ++k) {
@@ -1216,15 +1394,15 @@ void push_base(gwm_state* S) {
   scopy(S->ctx->stream, S->b_dev, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice);
 }
 
-// Full cell field -> departures from `base` (a level mean of it), FP32, device.
+// Full cell field -> departures from `base` (its level means, or on
+// terrain the 3-D base), FP32, device.
 template <class T>
 void upload_departure(gwm_state* S, void* dst, const T* full, const std::vector<double>& base) {
   const Geo& g = S->g();
   const size_t n = g.cell_total();
-  const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
   std::vector<float> d(n);
   for (size_t a = 0; a < n; ++a)
-    d[a] = static_cast<float>(static_cast<double>(full[a]) - base[a / plane]);
+    d[a] = static_cast<float>(static_cast<double>(full[a]) - base[S->bidx(a)]);
   scopy(S->ctx->stream, dst, d.data(), n * sizeof(float), hipMemcpyHostToDevice);
 }
 
@@ -1233,8 +1411,8 @@ void upload_departure(gwm_state* S, void* dst, const T* full, const std::vector<
 template <class T>
 void upload_pert_fields(gwm_state* S, const T* rho, const T* rt, const T* u, const T* v,
                         const T* w, const std::vector<std::vector<T>>& q) {
-  S->base_rho64 = level_means(S, rho);
-  S->base_rt64 = level_means(S, rt);
+  S->base_rho64 = field_base(S, rho);
+  S->base_rt64 = field_base(S, rt);
   push_base(S);
   upload_departure(S, S->cur_set()[0], rho, S->base_rho64);
   upload_departure(S, S->cur_set()[1], rt, S->base_rt64);
@@ -1303,6 +1481,33 @@ std::vector<R> host_metric(const gwm_state* S, int which) {
                   : which == 2 ? S->grid->idz_c : S->grid->idz_f;
   if (S->ctx->esize() == sizeof(R)) {
     scopy(S->ctx->stream, out.data(), src, n * sizeof(R), hipMemcpyDeviceToHost);
+  } else if (S->pert3) {
+    // FP64 host build of a perturbation-form state on terrain: the FP64
+    // metrics themselves (the device formulas, k_metrics), so its initial
+    // state, which is also its base, is the FP64 context's bit for bit.
+    std::vector<float> ter(n2);
+    scopy(S->ctx->stream, ter.data(), S->grid->terr, n2 * sizeof(float), hipMemcpyDeviceToHost);
+    const gwm_grid* G = S->grid;
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double thd = static_cast<double>(ter[static_cast<size_t>(j) * g.nx + i]);
+        auto zif = [&](int k) { const double prod = G->w_if[k] * thd; return G->z_if[k] + prod; };
+        auto zcm = [&](int k) { const double prod = G->w_c[k] * thd; return G->z_c[k] + prod; };
+        const int kn = which == 0 || which == 3 ? g.nz + 1 : g.nz;
+        for (int k = 0; k < kn; ++k) {
+          double v;
+          if (which == 0) {
+            v = zif(k);
+          } else if (which == 1) {
+            v = zcm(k);
+          } else if (which == 2) {
+            v = 1.0 / std::fmax(zif(k + 1) - zif(k), 1.0e-6);
+          } else {
+            v = (k == 0 || k == g.nz) ? 0.0 : 1.0 / std::fmax(zcm(k) - zcm(k - 1), 1.0e-6);
+          }
+          out[g.metric(i, j, k)] = static_cast<R>(v);
+        }
+      }
   } else {  // FP64 host build of a perturbation-form (FP32) state
     std::vector<float> tmp(n);
     scopy(S->ctx->stream, tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
@@ -1905,17 +2110,23 @@ int gwm_state_create_perturbation(gwm_ctx* ctx, const gwm_grid* grid, int ntr,
   return guard([&] {
     require(ctx && grid && out, "null argument");
     require(ctx->prec == GWM_F32, "the perturbation form is an FP32 state form");
-    require(!grid->terrain_nonzero, "the perturbation form needs flat terrain");
-    require(grid->geo.per_x && grid->geo.per_y,
-            "the perturbation form needs a doubly periodic domain");
+    const bool terrain = grid->terrain_nonzero;
+    require(!terrain || grid->terrain_active,
+            "the perturbation form on terrain needs terrain above 1e-6 m somewhere");
+    require(terrain || (grid->geo.per_x && grid->geo.per_y),
+            "the perturbation form on flat terrain needs a doubly periodic domain");
+    require(!terrain || ntr == 0,
+            "the perturbation form on terrain carries the dry state only (no tracers)");
     gwm_state* s = nullptr;
     const int rc = gwm_state_create(ctx, grid, ntr, &s);
     if (rc != GWM_OK) throw Error(rc, gwm_last_error());
     s->pert = true;
+    s->pert3 = terrain;
     // Until a state is loaded, the departures are zero against a nominal
     // base (rho_b = 1, (rho theta)_b = 300).
-    s->base_rho64.assign(static_cast<size_t>(grid->geo.nz), 1.0);
-    s->base_rt64.assign(static_cast<size_t>(grid->geo.nz), 300.0);
+    const size_t nb = terrain ? grid->geo.cell_total() : static_cast<size_t>(grid->geo.nz);
+    s->base_rho64.assign(nb, 1.0);
+    s->base_rt64.assign(nb, 300.0);
     try {
       push_base(s);
     } catch (...) {
@@ -1949,9 +2160,10 @@ int gwm_state_upload(gwm_state* s, int f, const void* host) {
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     if (s->pert && (f == GWM_RHO || f == GWM_RHO_THETA)) {
       // full values in; the base of the field becomes their level means
+      // (on terrain: the values themselves)
       gwm_state* o = s->view ? s->owner : s;
       const float* h = static_cast<const float*>(host);
-      if (!s->view) (f == GWM_RHO ? o->base_rho64 : o->base_rt64) = level_means(s, h);
+      if (!s->view) (f == GWM_RHO ? o->base_rho64 : o->base_rt64) = field_base(s, h);
       push_base(o);
       upload_departure(s, d, h, f == GWM_RHO ? o->base_rho64 : o->base_rt64);
       return;
@@ -1972,11 +2184,9 @@ int gwm_state_download(gwm_state* s, int f, void* host) {
       // departures -> full values (base added in FP64, rounded once)
       const gwm_state* o = s->view ? s->owner : s;
       const auto& b = f == GWM_RHO ? o->base_rho64 : o->base_rt64;
-      const Geo& g = s->g();
-      const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
       float* h = static_cast<float*>(host);
       for (size_t a = 0; a < s->fsize(f); ++a)
-        h[a] = static_cast<float>(static_cast<double>(h[a]) + b[a / plane]);
+        h[a] = static_cast<float>(static_cast<double>(h[a]) + b[o->bidx(a)]);
     }
   });
 }
@@ -1996,11 +2206,9 @@ int gwm_state_download_f64(gwm_state* s, int f, double* host) {
     scopy(s->ctx->stream, h.data(), d, n * sizeof(float), hipMemcpyDeviceToHost);
     const bool dep = s->pert && (f == GWM_RHO || f == GWM_RHO_THETA);
     const gwm_state* o = s->view ? s->owner : s;
-    const Geo& g = s->g();
-    const size_t plane = static_cast<size_t>(g.pxc) * g.pyc;
     for (size_t a = 0; a < n; ++a)
       host[a] = static_cast<double>(h[a]) +
-                (dep ? (f == GWM_RHO ? o->base_rho64 : o->base_rt64)[a / plane] : 0.0);
+                (dep ? (f == GWM_RHO ? o->base_rho64 : o->base_rt64)[o->bidx(a)] : 0.0);
   });
 }
 
@@ -2146,8 +2354,8 @@ std::vector<CkptItem> ckpt_items(gwm_state* S, std::vector<unsigned char>& bases
   if (S->ntr > 0)
     v.push_back({"precip", S->precip, nullptr, static_cast<size_t>(S->g().nx) * S->g().ny * S->es()});
   if (S->pert) {
-    const size_t nz = static_cast<size_t>(S->g().nz);
-    bases.resize(2 * nz * sizeof(double));
+    const size_t nb = S->base_rho64.size();  // nz, or the cell storage on terrain
+    bases.resize(2 * nb * sizeof(double));
     v.push_back({"bases_fp64", nullptr, &bases, bases.size()});
   }
   if (S->have_boundary) {
@@ -2176,9 +2384,9 @@ void checkpoint_write(gwm_state* S, const std::string& path, const char* meta) {
   std::vector<unsigned char> bases;
   auto items = ckpt_items(S, bases);
   if (S->pert) {
-    const size_t nz = static_cast<size_t>(g.nz);
-    std::memcpy(bases.data(), S->base_rho64.data(), nz * sizeof(double));
-    std::memcpy(bases.data() + nz * sizeof(double), S->base_rt64.data(), nz * sizeof(double));
+    const size_t nb = S->base_rho64.size();
+    std::memcpy(bases.data(), S->base_rho64.data(), nb * sizeof(double));
+    std::memcpy(bases.data() + nb * sizeof(double), S->base_rt64.data(), nb * sizeof(double));
   }
   CkptHeader h{};
   std::memcpy(h.magic, kCkptMagic, 8);
@@ -2326,11 +2534,11 @@ void checkpoint_read(gwm_state* S, const std::string& path) {
       scopy(S->ctx->stream, it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice);
   }
   if (S->pert) {
-    const size_t nz = static_cast<size_t>(g.nz);
+    const size_t nb = S->base_rho64.size();
     S->base_rho64.assign(reinterpret_cast<const double*>(bases.data()),
-                         reinterpret_cast<const double*>(bases.data()) + nz);
-    S->base_rt64.assign(reinterpret_cast<const double*>(bases.data()) + nz,
-                        reinterpret_cast<const double*>(bases.data()) + 2 * nz);
+                         reinterpret_cast<const double*>(bases.data()) + nb);
+    S->base_rt64.assign(reinterpret_cast<const double*>(bases.data()) + nb,
+                        reinterpret_cast<const double*>(bases.data()) + 2 * nb);
     push_base(S);
   }
   S->faces_synced = false;
@@ -2637,6 +2845,8 @@ int gwm_plan_view(gwm_state* s, int slice_k, int steps, double dt, int with_prec
     S_(s);
     require(out && nfields, "null output");
     require(steps >= 0, "steps must be >= 0");
+    require(!(s->view ? s->owner : s)->pert3,
+            "gwm_plan_view: the perturbation form on terrain is not supported");
     *nfields = dispatch(s->ctx, [&](auto r) {
       return plan_view<decltype(r)>(s, slice_k, steps, dt, with_precip, out);
     });
@@ -3689,6 +3899,7 @@ void check_members(gwm_state* const* states, int n, const gwm_step_config* c,
   for (int e = 0; e < n; ++e) {
     S_(states[e]);
     require(!states[e]->view, "not on a stage view");
+    require(!states[e]->pert3, "ensemble members in the perturbation form on terrain are not supported");
     for (int f = 0; f < e; ++f) require(states[f] != states[e], "a member appears twice");
     if (surrogates && surrogates[e])
       require(surrogates[e]->ctx == states[e]->ctx,

@@ -61,6 +61,12 @@ template <class R, bool TERRAIN, bool COLREF, bool PERT, int KIND, int TYV, bool
 __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int tiles, int nkc,
                                           double* part_out, const VBlock& vb) {
   static_assert(!DOWN || (!TERRAIN && !COLREF), "the downward march is for flat grids");
+  // Terrain perturbation form: per-cell bases, base forcing per face, and
+  // K14 without a reference state (the column reference's terms cancel
+  // between its recurrence and K14 up to the recurrence's residual, so K14
+  // acts on the departures and the base's own forcing is added).
+  constexpr bool TP = TERRAIN && PERT;
+  static_assert(!TP || !COLREF, "the terrain perturbation form has no column reference");
   constexpr int TY = TYV, PY = TY + 2, PN = PX * PY;
   constexpr int NHALO = PN - TX * TY;  // ring cells (148 at TY 8, 140 at TY 4)
   static_assert(NHALO <= TX * TY, "one thread per ring cell");
@@ -102,7 +108,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
     const int kk = k0 - 1 + q;
     const int kc = kk < 0 ? 0 : (kk >= nz ? nz - 1 : kk), kf = kk < 0 ? 0 : (kk > nz ? nz : kk);
     R* e = lev + q * NLV;
-    if (!COLREF) {
+    if (!COLREF && !TP) {
       e[LV_PREF] = a.pref_lev[kc];
       e[LV_RREF] = a.rref_lev[kc];
     }
@@ -136,22 +142,43 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
     const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(k) * pc);
     return off < 0 ? R(0) : v;
   };
+  // (terrain perturbation form: the base deviations of the own and ring
+  // cells too; positions outside an open domain read a stored stand-in)
+  auto ldb = [&](const R* f, int off, int k) -> R {
+    return ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(k) * pc);
+  };
   struct Plane {
     R rc, tc, rh, th;
+    R brc, btc, bpc, brh, bth, bph;
   };
   auto load_plane = [&](int k) -> Plane {
-    return Plane{ldc(a.rho, oc, k), ldc(a.theta, oc, k), ldc(a.rho, hc, k), ldc(a.theta, hc, k)};
+    Plane P{ldc(a.rho, oc, k), ldc(a.theta, oc, k), ldc(a.rho, hc, k), ldc(a.theta, hc, k),
+            R(0), R(0), R(0), R(0), R(0), R(0)};
+    if constexpr (TP) {
+      P.brc = ldb(a.base.drho, oc, k);
+      P.btc = ldb(a.base.dth, oc, k);
+      P.bpc = ldb(a.base.dp, oc, k);
+      P.brh = ldb(a.base.drho, hc, k);
+      P.bth = ldb(a.base.dth, hc, k);
+      P.bph = ldb(a.base.dp, hc, k);
+    }
+    return P;
   };
   // Perturbation form: rho' and theta' in, p' out (K10 rho*theta = rho*theta_fixed
-  // written for the departures: rho_b theta' + rho' (theta_b + theta')).
-  auto press = [&](R r, R th, int k) -> R {
+  // written for the departures: rho_b theta' + rho' (theta_b + theta')); the
+  // terrain form adds the cell's base deviations (dr, dt, dp) to the level's.
+  auto press = [&](R r, R th, int k, R dr, R dt, R dp) -> R {
     if (!PERT) return q_pressure(r, r * th);
+    if constexpr (TP) {
+      const R rb = LV(k, LV_BRHO) + dr, tb = LV(k, LV_BTH) + dt;
+      return p_pert(rb * th + r * (tb + th), rb * tb, LV(k, LV_BP) + dp);
+    }
     const R rb = LV(k, LV_BRHO), tb = LV(k, LV_BTH);
     return p_pert(rb * th + r * (tb + th), LV(k, LV_BRT), LV(k, LV_BP));
   };
   auto stage_plane = [&](const Plane& P, int k) {
-    sp[k & 3][ci] = press(P.rc, P.tc, k);
-    if (t < NHALO) sp[k & 3][hci] = press(P.rh, P.th, k);
+    sp[k & 3][ci] = press(P.rc, P.tc, k, P.brc, P.btc, P.bpc);
+    if (t < NHALO) sp[k & 3][hci] = press(P.rh, P.th, k, P.brh, P.bth, P.bph);
   };
 
   // Face departures m - m_base of one level of the own cell: none on the
@@ -173,6 +200,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   // down (its departure and its base).
   struct Faces {
     R u0, u1, v0, v1, w1, ub0, ub1, vb0, vb1, wb1;
+    R fu0, fu1, fv0, fv1, fw1;  // base forcing (terrain perturbation form)
   };
   auto load_faces = [&](int k) -> Faces {
     Faces F{};
@@ -194,6 +222,13 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
       F.vb1 = ld32(a.vb, yf1);
       F.wb1 = ld32(a.wb, zf1);
     }
+    if constexpr (TP) {
+      F.fu0 = ld32(a.base.fu, xf0);
+      F.fu1 = ld32(a.base.fu, xf0 + 1);
+      F.fv0 = ld32(a.base.fv, yf0);
+      F.fv1 = ld32(a.base.fv, yf1);
+      F.fw1 = ld32(a.base.fw, zf1);
+    }
     return F;
   };
 
@@ -212,6 +247,8 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   // marching up, the top face k+1 marching down).
   R rho_km1 = R(0);
   R rho_k, th_k, rho_k1 = R(0), th_k1 = R(0);
+  // own-cell base deviations of levels k and k+1 (terrain perturbation form)
+  R br_k = R(0), bt_k = R(0), br_k1 = R(0), bt_k1 = R(0);
   Plane P2;
   Faces F;
   R w0, wb0;
@@ -246,12 +283,16 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
       stage_plane(P, k0);
       rho_k = P.rc;
       th_k = P.tc;
+      br_k = P.brc;
+      bt_k = P.btc;
     }
     if (k0 + 1 < nz) {
       const Plane P = load_plane(k0 + 1);
       stage_plane(P, k0 + 1);
       rho_k1 = P.rc;
       th_k1 = P.tc;
+      br_k1 = P.brc;
+      bt_k1 = P.btc;
     }
     P2 = load_plane(k0 + 2 < kmax ? k0 + 2 : kmax);
     F = load_faces(k0);
@@ -271,13 +312,21 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
 
   const bool ow = !g.per_x, os = !g.per_y;
   if (!DOWN && active && k0 > 0) {
-    // K14 at face k0 from levels k0-1 and k0 (flat references).
-    const R ppl = sp[(k0 - 1) & 3][ci] - LV(k0 - 1, LV_PREF);
-    const R ppu = sp[k0 & 3][ci] - LV(k0, LV_PREF);
-    const R rpf = R(0.5) * ((rho_km1 - LV(k0 - 1, LV_RREF)) + (rho_k - LV(k0, LV_RREF)));
+    // K14 at face k0 from levels k0-1 and k0 (flat references; the terrain
+    // perturbation form: departures plus the base forcing of the face).
     const R idz = TERRAIN ? a.m.idz_f[own_m + k0 * plane_m] : LV(k0, LV_IDZF);
-    const R tend = -(ppu - ppl) * idz - grav * rpf;
-    w0 = w0 + dtf * tend;
+    if constexpr (TP) {
+      const R ppl = sp[(k0 - 1) & 3][ci], ppu = sp[k0 & 3][ci];
+      const R rpf = R(0.5) * (rho_km1 + rho_k);
+      const R tend = -(ppu - ppl) * idz - grav * rpf;
+      w0 = w0 + dtf * (tend + a.base.fw[g.zf(ic, jc, k0)]);
+    } else {
+      const R ppl = sp[(k0 - 1) & 3][ci] - LV(k0 - 1, LV_PREF);
+      const R ppu = sp[k0 & 3][ci] - LV(k0, LV_PREF);
+      const R rpf = R(0.5) * ((rho_km1 - LV(k0 - 1, LV_RREF)) + (rho_k - LV(k0, LV_RREF)));
+      const R tend = -(ppu - ppl) * idz - grav * rpf;
+      w0 = w0 + dtf * tend;
+    }
   }
 
   // Level k: P2 / F hold plane k+2 and the faces of level k (loaded an
@@ -296,6 +345,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
       if (k + 2 <= kmax) stage_plane(P2, k + 2);
     }
     const R rho_k2 = P2.rc, th_k2 = P2.tc;
+    const R br_k2 = P2.brc, bt_k2 = P2.btc;
 
     const R* pk = sp[k & 3];
     const R* pdn = sp[(k + 3) & 3];  // level k-1
@@ -341,18 +391,30 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         const R gl = vgrad(offl, ml), gr = vgrad(offr, mr);
         return qdivc(pr - pl, d, rd) - dzd * (R(0.5) * (gl + gr));
       };
-      // K12 / K13: faces i, i+1 and j, j+1 of the cell.
+      // K12 / K13: faces i, i+1 and j, j+1 of the cell (the terrain
+      // perturbation form adds the base's forcing of the face).
+      auto fb = [&](R pg, R f) -> R {
+        if constexpr (TP) return pg + f;
+        return pg;
+      };
       R u0 = F.u0, u1 = F.u1, v0 = F.v0, v1 = F.v1;
       if (a.horizontal) {
-        if (!(i == 0 && ow)) u0 = F.u0 + (-dtf * pgf(ci - 1, ci, i - 1, j, i, j, dx, rdx));
-        if (!(i + 1 == g.nx && ow)) u1 = F.u1 + (-dtf * pgf(ci, ci + 1, i, j, i + 1, j, dx, rdx));
-        if (!(j == 0 && os)) v0 = F.v0 + (-dtf * pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy));
-        if (!(j + 1 == g.ny && os)) v1 = F.v1 + (-dtf * pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy));
+        if (!(i == 0 && ow)) u0 = F.u0 + (-dtf * fb(pgf(ci - 1, ci, i - 1, j, i, j, dx, rdx), F.fu0));
+        if (!(i + 1 == g.nx && ow))
+          u1 = F.u1 + (-dtf * fb(pgf(ci, ci + 1, i, j, i + 1, j, dx, rdx), F.fu1));
+        if (!(j == 0 && os)) v0 = F.v0 + (-dtf * fb(pgf(ci - PX, ci, i, j - 1, i, j, dy, rdy), F.fv0));
+        if (!(j + 1 == g.ny && os))
+          v1 = F.v1 + (-dtf * fb(pgf(ci, ci + PX, i, j, i, j + 1, dy, rdy), F.fv1));
       }
       // K14 (faces 0 and nz are left as they are): marching up, w face k+1
       // from levels k and k+1; marching down, w face k from levels k-1 and
       // k (the same arithmetic for a face either way).
       auto k14 = [&](R wf, R p_lo, R p_hi, R pl, R pu, R r_lo, R r_hi, R rl, R ru, R idz) -> R {
+        if constexpr (TP) {  // departures and the base forcing of the face
+          const R rpf = R(0.5) * (r_lo + r_hi);
+          const R tend = -(p_hi - p_lo) * idz - grav * rpf;
+          return wf + dtf * (tend + F.fw1);
+        }
         const R ppl = p_lo - pl;
         const R ppu = p_hi - pu;
         const R rpf = R(0.5) * ((r_lo - rl) + (r_hi - ru));
@@ -368,6 +430,8 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         R rl, pl, ru, pu;
         if (COLREF) {
           rl = rref_k; pl = pref_k; ru = rref_k1; pu = pref_k1;
+        } else if (TP) {
+          rl = pl = ru = pu = R(0);
         } else {
           rl = LV(k, LV_RREF); pl = LV(k, LV_PREF); ru = LV(k + 1, LV_RREF); pu = LV(k + 1, LV_PREF);
         }
@@ -382,14 +446,19 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         const R idz = TERRAIN ? a.m.idz_c[own_m + k * plane_m] : LV(k, LV_IDZC);
         // divergence of (m - m_base): the faces are departures already
         const R div = qdivc(u1 - u0, dx, rdx) + qdivc(v1 - v0, dy, rdy) + (whi - wlo) * idz;
-        const R floor = PERT ? R(1.0e-6) - LV(k, LV_BRHO) : R(1.0e-6);
+        const R floor = PERT ? R(1.0e-6) - (TP ? LV(k, LV_BRHO) + br_k : LV(k, LV_BRHO)) : R(1.0e-6);
         rho_new = rmax(floor, rho_new - dtf * div);
       }
       const unsigned c0 = c_o + static_cast<unsigned>(k) * pc;
       const unsigned xf0 = xf_o + static_cast<unsigned>(k) * sxk;
       const unsigned yf0 = yf_o + static_cast<unsigned>(k) * syk;
       st32(a.rho_out, c0, rho_new);
-      rt_new = PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k) : rho_new * th_k;
+      if constexpr (TP) {
+        const R rb = LV(k, LV_BRHO) + br_k, tb = LV(k, LV_BTH) + bt_k;
+        rt_new = rb * th_k + rho_new * (tb + th_k);
+      } else {
+        rt_new = PERT ? LV(k, LV_BRHO) * th_k + rho_new * (LV(k, LV_BTH) + th_k) : rho_new * th_k;
+      }
       if (a.rt_out) st32(a.rt_out, c0, rt_new);
       // departures, or full faces base + departure on the last substep
       st32(a.u_out, xf0, LAST ? F.ub0 + u0 : u0);
@@ -419,7 +488,7 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
                                    : q_theta(rho_new, rt_new));
         } else {
           sr = static_cast<S>(rho_new);
-          spp = static_cast<S>(press(rho_new, th_k, k));
+          spp = static_cast<S>(press(rho_new, th_k, k, R(0), R(0), R(0)));
         }
       }
       sr = wave_sum_to63(sr);
@@ -441,6 +510,10 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
     th_k = th_k1;
     rho_k1 = rho_k2;
     th_k1 = th_k2;
+    br_k = br_k1;
+    bt_k = bt_k1;
+    br_k1 = br_k2;
+    bt_k1 = bt_k2;
     pref_k = pref_k1;
     rref_k = rref_k1;
     __syncthreads();
@@ -539,6 +612,12 @@ void launch_fast_column(hipStream_t st, const FastArgs<R>& a, bool terrain, int 
       else launch_fast_kind<R, false, true, false, 8>(st, a, 1, nullptr);
     }
   } else if (terrain) {
+    if constexpr (sizeof(R) == 4) {
+      if (a.base.drho) {  // perturbation form on terrain (per-cell bases)
+        launch_fast_kind<R, true, false, true, 8>(st, a, nkc, part_out);
+        return;
+      }
+    }
     launch_fast_kind<R, true, false, false, 8>(st, a, nkc, part_out);
   } else if (down && nkc == 1) {
     // whole columns marched from the top down (flat grids)

@@ -256,12 +256,30 @@ __device__ inline R d_qsat(R T, R p) {  // moist_thermo.hpp:23-36
 // for the departure, so that pressure differences keep their relative
 // precision.
 // ---------------------------------------------------------------------------
+//
+// Terrain form: the base varies within a level (terrain-following levels
+// cross the base's vertical structure), so it is the per-level value above
+// plus a per-cell deviation (drho, dth, dp in the cell layout); every
+// departure relation above holds cell by cell with
+//   rho_b = rho[k] + drho, theta_b = th[k] + dth, p_b = p[k] + dp,
+//   (rho theta)_b = rho_b theta_b,
+// and the base's own pressure-gradient and gravity forcing, fixed for the
+// run, is held per face in FP32 (fu, fv: the terrain-following horizontal
+// pressure gradient of p_b; fw: -dp_b/dz - g rho_b at the z faces), so that
+// only departures pass through FP32 pressure arithmetic.  All six are
+// nullptr on flat grids.
 template <class R>
 struct Base {
   const R* rho;  // rho_b (nz)
   const R* rt;   // (rho theta)_b
   const R* th;   // theta_b = (rho theta)_b / rho_b
   const R* p;    // p_b = EOS(rho_b, (rho theta)_b)
+  const R* drho = nullptr;
+  const R* dth = nullptr;
+  const R* dp = nullptr;
+  const R* fu = nullptr;
+  const R* fv = nullptr;
+  const R* fw = nullptr;
 };
 
 template <class R>

@@ -74,7 +74,13 @@ __device__ __forceinline__ void level_partials_body(const Geo& g, const R* rho, 
     for (int q = 0; q < kPartialRows; ++q) {
       if (j0 + q >= g.ny) break;
       const R r = rv[q];
-      if (pert) {
+      if (pert && base.drho) {
+        // terrain form: theta relative to the level's base value (the
+        // cell's base deviation plus its departure), for theta_ref
+        const size_t c0 = g.cell(i, j0 + q, k);
+        const R dt = base.dth[c0];
+        s0 += static_cast<double>(dt + theta_pert(r, av[q], rb + base.drho[c0], thb + dt));
+      } else if (pert) {
         // departures: theta' of (rho', (rho theta)'), or rho' and p' of
         // (rho', theta' fixed)
         if (mode == 0) {
@@ -226,7 +232,10 @@ __global__ void k_theta(Geo g, const R* rho, const R* rt, R* th, Base<R> base) {
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
   const size_t c = g.cell(i, j, k);
-  th[c] = base.rho ? theta_pert(rho[c], rt[c], base.rho[k], base.th[k]) : q_theta(rho[c], rt[c]);
+  if (base.drho)  // terrain form: the cell's base
+    th[c] = theta_pert(rho[c], rt[c], base.rho[k] + base.drho[c], base.th[k] + base.dth[c]);
+  else
+    th[c] = base.rho ? theta_pert(rho[c], rt[c], base.rho[k], base.th[k]) : q_theta(rho[c], rt[c]);
 }
 
 template <class R>
@@ -236,7 +245,12 @@ __global__ void k_rebuild_rt(Geo g, const R* rho, const R* th, R* rt, Base<R> ba
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
   const size_t c = g.cell(i, j, k);
-  rt[c] = base.rho ? base.rho[k] * th[c] + rho[c] * (base.th[k] + th[c]) : rho[c] * th[c];
+  if (base.drho) {  // terrain form: the cell's base
+    const R rb = base.rho[k] + base.drho[c], tb = base.th[k] + base.dth[c];
+    rt[c] = rb * th[c] + rho[c] * (tb + th[c]);
+  } else {
+    rt[c] = base.rho ? base.rho[k] * th[c] + rho[c] * (base.th[k] + th[c]) : rho[c] * th[c];
+  }
 }
 
 // ===========================================================================
@@ -489,7 +503,15 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
     const size_t idx = (static_cast<size_t>(k) * (a.fy + 2 * a.h) + (j + a.h)) * (a.fx + 2 * a.h) + (i + a.h);
     double v = static_cast<double>(a.f[idx]);
     // perturbation form: extremes of the full value, sums of the departure
-    const double full = a.fbase ? v + static_cast<double>(a.fbase[k]) : v;
+    double fb = 0.0;
+    if (a.fbase) {
+      if (a.cb.drho && a.fbase == a.cb.rt)
+        fb = (static_cast<double>(a.cb.rho[k]) + static_cast<double>(a.cb.drho[idx])) *
+             (static_cast<double>(a.cb.th[k]) + static_cast<double>(a.cb.dth[idx]));
+      else
+        fb = static_cast<double>(a.fbase[k]) + (a.cb.drho ? static_cast<double>(a.cb.drho[idx]) : 0.0);
+    }
+    const double full = a.fbase ? v + fb : v;
     double ext = full;
     if (a.kind == 1) {  // column path
       const R idz = a.idz3 ? a.idz3[(static_cast<size_t>(k) * a.fy + j) * a.fx + i] : a.idz_lev[k];
@@ -497,7 +519,9 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> a) {
       ext = v;
     } else if (a.kind == 2) {  // theta_m / mixing ratio: f / max(rho, 1e-12)
       const double r = static_cast<double>(a.rho[idx]) +
-                       (a.rbase ? static_cast<double>(a.rbase[k]) : 0.0);
+                       (a.rbase ? static_cast<double>(a.rbase[k]) +
+                                      (a.cb.drho ? static_cast<double>(a.cb.drho[idx]) : 0.0)
+                                : 0.0);
       v = full / fmax(r, 1.0e-12);
       ext = v;
     } else if (a.kind == 3) {  // count of positive values; extremes of the value

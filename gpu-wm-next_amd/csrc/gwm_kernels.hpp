@@ -89,9 +89,12 @@ struct ReduceArgs {
   int kind;  // 0 value, 1 value*dz, 2 value/rho, 3 count of values > 0
   double* partials;
   // Perturbation form: per-level bases added for min / max (and for rho in
-  // kind 2); sums stay over the stored departures.
+  // kind 2); sums stay over the stored departures.  On terrain (cb.drho set)
+  // the cell's base is the level value plus its deviation; rho*theta's is
+  // rho_b theta_b of the cell's.
   const R* fbase;
   const R* rbase;
+  Base<R> cb;
 };
 
 template <class R>

@@ -115,6 +115,9 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
 
   const int nz = g.nz;
   constexpr bool pert = PERT;
+  // Terrain perturbation form: per-cell bases (level value + deviation),
+  // the base's pressure-gradient forcing per face, open sides allowed.
+  constexpr bool TP = TERRAIN && PERT;
   const unsigned pc = static_cast<unsigned>(g.pxc) * g.pyc;  // cell / z-face level stride
   const unsigned pu = static_cast<unsigned>(g.pxu) * g.pyc;  // x-face level stride
   const unsigned pv = static_cast<unsigned>(g.pxc) * g.pyv;  // y-face level stride
@@ -151,22 +154,34 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
   // of the SSP base of level k+1) stay in flight while level k is computed.
   struct Raw {
     R r, rt, u0, u1, v0, v1, w0, w1;
+    R br, bt, bp;  // base deviations (terrain perturbation form)
+    bool out;      // a position outside an open domain (zero stand-ins)
   };
   struct Lev {
     Raw o, h;  // own cell, ring cell
   };
   auto ldo = [&](const R* f, int off, unsigned plane, int kk) -> R {
-    // The perturbation form runs on doubly periodic grids only, where every
-    // staged position maps to a stored one: no zero stand-ins to select.
-    if constexpr (PERT) return ld32(f, static_cast<unsigned>(off) + static_cast<unsigned>(kk) * plane);
+    // The flat perturbation form runs on doubly periodic grids only, where
+    // every staged position maps to a stored one: no zero stand-ins to select.
+    if constexpr (PERT && !TERRAIN)
+      return ld32(f, static_cast<unsigned>(off) + static_cast<unsigned>(kk) * plane);
     const R v = ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * plane);
     return off < 0 ? R(0) : v;
   };
   const StagePos hpos = t < NHALO ? ring : own;
+  auto ldb = [&](const R* f, int off, int kk) -> R {  // base deviations (stand-in: position 0)
+    return ld32(f, static_cast<unsigned>(off < 0 ? 0 : off) + static_cast<unsigned>(kk) * pc);
+  };
   auto load_raw = [&](const StagePos& o, int kk) -> Raw {
-    return Raw{ldo(s.rho, o.c, pc, kk),  ldo(s.rt, o.c, pc, kk),  ldo(s.u, o.u0, pu, kk),
-               ldo(s.u, o.u1, pu, kk),   ldo(s.v, o.v0, pv, kk),  ldo(s.v, o.v1, pv, kk),
-               ldo(s.w, o.c, pc, kk),    ldo(s.w, o.c, pc, kk + 1)};
+    Raw q{ldo(s.rho, o.c, pc, kk),  ldo(s.rt, o.c, pc, kk),  ldo(s.u, o.u0, pu, kk),
+          ldo(s.u, o.u1, pu, kk),   ldo(s.v, o.v0, pv, kk),  ldo(s.v, o.v1, pv, kk),
+          ldo(s.w, o.c, pc, kk),    ldo(s.w, o.c, pc, kk + 1), R(0), R(0), R(0), o.c < 0};
+    if constexpr (TP) {
+      q.br = ldb(a.base.drho, o.c, kk);
+      q.bt = ldb(a.base.dth, o.c, kk);
+      q.bp = ldb(a.base.dp, o.c, kk);
+    }
+    return q;
   };
   auto load_lev = [&](int kk) -> Lev { return Lev{load_raw(own, kk), load_raw(hpos, kk)}; };
   // Stages level kk at LDS entry e.
@@ -174,7 +189,16 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
   // reciprocal (within an ulp of the quotients); FP64 divides each.
   auto stage_raw = [&](const Raw& q, int e, int kk) {
     R rr;
-    if (pert) {  // theta' and p' of the departures
+    if constexpr (TP) {
+      // theta staged relative to the level's base value theta_b(k) (the
+      // cell's base deviation plus its departure), p' against the cell's
+      // base; outside an open domain the stand-in is zero theta, as in the
+      // full form (theta = -theta_b(k) here), and p' = 0 (never used)
+      const R rb = LV(kk, SL_BRHO) + q.br, tb = LV(kk, SL_BTH) + q.bt;
+      rr = rmax(rb + q.r, R(1.0e-6));
+      lds(Q_TH, kk & (NSL - 1), e) = q.out ? -LV(kk, SL_BTH) : q.bt + (q.rt - tb * q.r) * qrcp(rr);
+      lds(Q_P, kk & (NSL - 1), e) = q.out ? R(0) : p_pert(q.rt, rb * tb, LV(kk, SL_BP) + q.bp);
+    } else if (pert) {  // theta' and p' of the departures
       const R rb = LV(kk, SL_BRHO);
       rr = rmax(rb + q.r, R(1.0e-6));
       lds(Q_TH, kk & (NSL - 1), e) =
@@ -237,8 +261,9 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
 
   const R dx = static_cast<R>(g.dx), dy = static_cast<R>(g.dy);
   const R rdx = R(1) / dx, rdy = R(1) / dy;
-  // open sides (none in the perturbation form, which is doubly periodic)
-  const bool ow = !PERT && !g.per_x, oe = ow, os = !PERT && !g.per_y, on = os;
+  // open sides (none in the flat perturbation form, which is doubly periodic)
+  const bool ow = (!PERT || TERRAIN) && !g.per_x, oe = ow;
+  const bool os = (!PERT || TERRAIN) && !g.per_y, on = os;
   const size_t plane_m = static_cast<size_t>(g.nx) * g.ny;
   auto mcol = [&](int ii, int jj) -> size_t {
     return static_cast<size_t>(L.mj(jj)) * g.nx + L.mi(ii);
@@ -391,7 +416,9 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
       }
 
       // ---- x-face (K17 + K21)
-      auto tend_u = [&](int fi, const auto& Uf, R* carry) -> R {
+      // (fo: the face's offset, for the base forcing of the terrain
+      // perturbation form)
+      auto tend_u = [&](int fi, const auto& Uf, R* carry, unsigned fo) -> R {
         // closed outer faces and open-side fluxes are selected away, not
         // branched around: every read stays inside the staged tile and ring
         const bool closed = (fi == 0 && ow) || (fi == g.nx && oe);
@@ -422,13 +449,14 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
           const R zr = a.m.zc[mcol(fi, j) + k * plane_m];
           const R dzdx = qdivc(zr - zl, dx, rdx);
           term = qdivc(pr - pl, dx, rdx) - dzdx * (R(0.5) * (vgrad(fi - 1, j) + vgrad(fi, j)));
+          if constexpr (TP) term = term + ld32(a.base.fu, fo);
         } else {
           term = qdivc(pr - pl, dx, rdx);
         }
         return closed ? R(0) : A + (-term);
       };
       // ---- y-face (K18 + K22)
-      auto tend_v = [&](int fj, const auto& Vf, R* carry) -> R {
+      auto tend_v = [&](int fj, const auto& Vf, R* carry, unsigned fo) -> R {
         const bool closed = (fj == 0 && os) || (fj == g.ny && on);
         const R qw = Vf(i - 1, fj, k), qc = Vf(i, fj, k), qe = Vf(i + 1, fj, k);
         const R qs = Vf(i, fj - 1, k), qn = Vf(i, fj + 1, k);
@@ -457,6 +485,7 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
           const R zn = a.m.zc[mcol(i, fj) + k * plane_m];
           const R dzdy = qdivc(zn - zs, dy, rdy);
           term = qdivc(pn - ps, dy, rdy) - dzdy * (R(0.5) * (vgrad(i, fj - 1) + vgrad(i, fj)));
+          if constexpr (TP) term = term + ld32(a.base.fv, fo);
         } else {
           term = qdivc(pn - ps, dy, rdy);
         }
@@ -492,8 +521,12 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         if (pert) {
           // theta_ref holds the level means of theta'; the bases cancel in
           // theta_f - theta_ref,f
-          const R rl = r_km1 + LV(kl, SL_BRHO);
-          const R ru = r_k + LV(ku, SL_BRHO);
+          R rl = r_km1 + LV(kl, SL_BRHO);
+          R ru = r_k + LV(ku, SL_BRHO);
+          if constexpr (TP) {  // the cell's base: level value + deviation
+            rl = r_km1 + (LV(kl, SL_BRHO) + ld32(a.base.drho, oc0 + static_cast<unsigned>(kl) * pc));
+            ru = r_k + (LV(ku, SL_BRHO) + ld32(a.base.drho, oc0 + static_cast<unsigned>(ku) * pc));
+          }
           const R trp = R(0.5) * (LV(kl, SL_TREF) + LV(ku, SL_TREF));
           const R trf = R(0.5) * (LV(kl, SL_BTH) + LV(ku, SL_BTH)) + trp;
           const R rf = R(0.5) * (rl + ru);
@@ -515,27 +548,32 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
       const unsigned yf = oy0 + static_cast<unsigned>(k) * pv;
       // the extra faces: x face nx (i == nx-1), y face ny (j == ny-1), z face nz
       const unsigned xfe = xf + 1u, yfe = yf + static_cast<unsigned>(g.pxc), zft = ci + pc;
-      const R tu0 = tend_u(i, U, &cz.u), tv0 = tend_v(j, V, &cz.v), tw0 = tend_w();
+      const R tu0 = tend_u(i, U, &cz.u, xf), tv0 = tend_v(j, V, &cz.v, yf), tw0 = tend_w();
       if constexpr (MODE == 0) {
         st32(a.out.rho, ci, t_rho);
         st32(a.out.rt, ci, t_rt);
         st32(a.out.u, xf, tu0);
         st32(a.out.v, yf, tv0);
         st32(a.out.w, ci, tw0);
-        if (i == g.nx - 1) st32(a.out.u, xfe, tend_u(g.nx, Ug, nullptr));
-        if (j == g.ny - 1) st32(a.out.v, yfe, tend_v(g.ny, Vg, nullptr));
+        if (i == g.nx - 1) st32(a.out.u, xfe, tend_u(g.nx, Ug, nullptr, xfe));
+        if (j == g.ny - 1) st32(a.out.v, yfe, tend_v(g.ny, Vg, nullptr, yfe));
         if (k == nz - 1) st32(a.out.w, zft, R(0));
       } else {
         const R mf = Phys<R>::mom_floor;
         // floors on the full values (departures: 1e-6 - base)
-        const R flr = pert ? R(1.0e-6) - LV(k, SL_BRHO) : R(1.0e-6);
-        const R flt = pert ? R(1.0e-6) - LV(k, SL_BRT) : R(1.0e-6);
+        R rbk = LV(k, SL_BRHO), tbk = LV(k, SL_BTH), rtbk = LV(k, SL_BRT);
+        if constexpr (TP) {  // the own cell's base
+          rbk = rbk + ld32(a.base.drho, ci);
+          tbk = tbk + ld32(a.base.dth, ci);
+          rtbk = rbk * tbk;
+        }
+        const R flr = pert ? R(1.0e-6) - rbk : R(1.0e-6);
+        const R flt = pert ? R(1.0e-6) - rtbk : R(1.0e-6);
         const R rho_new = fin(Qc.rho, upd(r_k, t_rho, flr));
         const R rt_new = fin(Qc.rt, upd(rt_k, t_rt, flt));
         st32(a.out.rho, ci, rho_new);
         if (pert)
-          st32(a.out.rt, ci,
-               a.out_theta ? theta_pert(rho_new, rt_new, LV(k, SL_BRHO), LV(k, SL_BTH)) : rt_new);
+          st32(a.out.rt, ci, a.out_theta ? theta_pert(rho_new, rt_new, rbk, tbk) : rt_new);
         else
           st32(a.out.rt, ci, a.out_theta ? qdiv(rt_new, rmax(rho_new, R(1.0e-6))) : rt_new);
         st32(a.out.u, xf, fin(Qc.u, upd(U(i, j, k), tu0, mf)));
@@ -543,10 +581,10 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         st32(a.out.w, ci, fin(Qc.w, upd(W(i, j, k), tw0, mf)));
         if (i == g.nx - 1)
           st32(a.out.u, xfe,
-               fin(MODE == 2 ? ld32(a.q0.u, xfe) : R(0), upd(ld32(s.u, xfe), tend_u(g.nx, Ug, nullptr), mf)));
+               fin(MODE == 2 ? ld32(a.q0.u, xfe) : R(0), upd(ld32(s.u, xfe), tend_u(g.nx, Ug, nullptr, xfe), mf)));
         if (j == g.ny - 1)
           st32(a.out.v, yfe,
-               fin(MODE == 2 ? ld32(a.q0.v, yfe) : R(0), upd(ld32(s.v, yfe), tend_v(g.ny, Vg, nullptr), mf)));
+               fin(MODE == 2 ? ld32(a.q0.v, yfe) : R(0), upd(ld32(s.v, yfe), tend_v(g.ny, Vg, nullptr, yfe), mf)));
         if (k == nz - 1)
           st32(a.out.w, zft, fin(MODE == 2 ? ld32(a.q0.w, zft) : R(0), upd(w_top, R(0), mf)));
       }
@@ -614,9 +652,15 @@ void launch_slow_column(hipStream_t st, const SlowArgs<R>& a, bool terrain, int 
   const dim3 blk(TX, TY, 1), grid(tiles * nkc);
   // per-level LDS: the longest chunk plus the levels below and above it
   const size_t lds = static_cast<size_t>((a.g.nz + nkc - 1) / nkc + 3) * SL_N * sizeof(R);
-  if (terrain)
+  if (terrain) {
+    if constexpr (sizeof(R) == 4) {
+      if (a.base.drho) {  // perturbation form on terrain (per-cell bases)
+        launch_slow_mode<R, true, true>(st, a, grid, blk, lds, tx, tiles, nkc);
+        return;
+      }
+    }
     launch_slow_mode<R, true, false>(st, a, grid, blk, lds, tx, tiles, nkc);
-  else if (a.base.rho)  // perturbation form (flat, doubly periodic)
+  } else if (a.base.rho)  // perturbation form (flat, doubly periodic)
     launch_slow_mode<R, false, true>(st, a, grid, blk, lds, tx, tiles, nkc);
   else
     launch_slow_mode<R, false, false>(st, a, grid, blk, lds, tx, tiles, nkc);

@@ -351,7 +351,9 @@ class State:
 
     def __init__(self, grid, n_tracers=0, perturbation=False):
         """perturbation=True: the FP32 perturbation form (rho and rho*theta
-        stored as departures from FP64 level-mean bases)."""
+        stored as departures from FP64 bases: level means on flat, doubly
+        periodic grids; on terrain the 3-D state the base was built from,
+        dry states only, open or periodic sides)."""
         self.grid = grid
         self.ctx = grid.ctx
         self.n_tracers = n_tracers
