@@ -249,9 +249,15 @@ int gwm_state_destroy(gwm_state* state);
  * p' = p_b expm1(gamma log1p((rho theta)'/(rho theta)_b)), so pressure
  * differences keep their relative precision and the dry-mass budget is the
  * FP64 base sum plus a compensated sum of departures.  Needs an FP32
- * context, flat terrain and a doubly periodic grid.  Uploads, downloads and
- * fills of GWM_RHO / GWM_RHO_THETA take and return full values (an upload
- * re-bases that field); every operator accepts the state. */
+ * context; on flat terrain a doubly periodic grid.  On terrain (some
+ * |h| > 1e-6 m) the base is 3-D: the FP64 state loaded, cell by cell, with
+ * its pressure-gradient and gravity forcing held per face, so that K14
+ * acts on the departures without the column reference (whose terms cancel
+ * against its recurrence); open or periodic sides, dry states only
+ * (n_tracers 0; gwm_plan_view and the ensemble entry points refuse it).
+ * Uploads, downloads and fills of GWM_RHO / GWM_RHO_THETA take and return
+ * full values (an upload re-bases that field); every other operator accepts
+ * the state. */
 int gwm_state_create_perturbation(gwm_ctx* ctx, const gwm_grid* grid, int n_tracers,
                                   gwm_state** out);
 int gwm_state_is_perturbation(const gwm_state* state);
