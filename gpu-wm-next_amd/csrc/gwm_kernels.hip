@@ -456,14 +456,42 @@ __global__ void k_exchange_layout(int phase, int orient, int nranks, const int* 
 // ===========================================================================
 // Boundary strips / relaxation on the outermost owned strips of one field
 // (boundary_conditions.cpp:9-161; relaxation is the extension documented in
-// the header).  orient as in k_halo_fill.
+// the header).  orient as in k_halo_fill.  Threads cover the frame of
+// cells within wx (wy) of the open x (y) sides when it is narrower than the
+// field (wx = 0: the whole field): x strips of every row, then y strips of
+// the columns between them; each cell's test and update are the same
+// either way.
 // ===========================================================================
 template <class R>
-__global__ void k_boundary(Geo g, int orient, R* f, const R* ref, int mode, R width) {
+__global__ void k_boundary(Geo g, int orient, R* f, const R* ref, int mode, R width, int wx,
+                           int wy) {
   const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= fx * fy * fz) return;
-  const int i = a % fx, j = (a / fx) % fy, k = a / (fx * fy);
+  int i, j, k;
+  if (wx > 0 || wy > 0) {
+    const int wx0 = wx > 0 ? wx : 0;
+    const int nxs = wx > 0 ? 2 * wx * fy : 0;
+    const int nys = wy > 0 ? 2 * wy * (fx - 2 * wx0) : 0;
+    const int per = nxs + nys;
+    if (a >= per * fz) return;
+    k = a / per;
+    int r = a % per;
+    if (r < nxs) {
+      const int ii = r % (2 * wx);
+      j = r / (2 * wx);
+      i = ii < wx ? ii : fx - 2 * wx + ii;
+    } else {
+      r -= nxs;
+      const int jj = r % (2 * wy);
+      i = wx0 + r / (2 * wy);
+      j = jj < wy ? jj : fy - 2 * wy + jj;
+    }
+  } else {
+    if (a >= fx * fy * fz) return;
+    i = a % fx;
+    j = (a / fx) % fy;
+    k = a / (fx * fy);
+  }
   const bool w = !g.per_x, e = !g.per_x, s = !g.per_y, n = !g.per_y;
   const int px = fx + 2 * g.h, py = fy + 2 * g.h;
   const size_t idx = (static_cast<size_t>(k) * py + (j + g.h)) * px + (i + g.h);
@@ -764,8 +792,22 @@ void launch_boundary(hipStream_t st, const Geo& g, int orient, R* f, const R* re
                      double width) {
   if (refuse_record("boundary")) return;
   const int fx = g.nx + (orient == 1), fy = g.ny + (orient == 2), fz = g.nz + (orient == 3);
-  hipLaunchKernelGGL(k_boundary<R>, dim3(blocks_for(static_cast<size_t>(fx) * fy * fz, 256)),
-                     dim3(256), 0, st, g, orient, f, ref, mode, static_cast<R>(width));
+  // frame widths: the cells a side can reach (the outermost one for strips,
+  // distance < width for relaxation); the whole field when a frame would
+  // cover it
+  const int reach = mode == 1 ? 1 : (width > 0.0 ? static_cast<int>(std::ceil(width)) : 0);
+  int wx = g.per_x ? 0 : reach, wy = g.per_y ? 0 : reach;
+  size_t n = static_cast<size_t>(fx) * fy * fz;
+  if ((wx > 0 || wy > 0) && 2 * wx < fx && 2 * wy < fy) {
+    const size_t nxs = wx > 0 ? 2 * static_cast<size_t>(wx) * fy : 0;
+    const size_t nys = wy > 0 ? 2 * static_cast<size_t>(wy) * (fx - 2 * wx) : 0;
+    n = (nxs + nys) * fz;
+  } else {
+    wx = wy = 0;
+  }
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_boundary<R>, dim3(blocks_for(n, 256)), dim3(256), 0, st, g, orient, f, ref,
+                     mode, static_cast<R>(width), wx, wy);
 }
 template <class R>
 void launch_reduce(hipStream_t st, const ReduceArgs<R>& a, int nblocks) {
