@@ -1359,14 +1359,40 @@ void push_base3(gwm_state* S) {
         if (!g.per_y && (j == 0 || j == g.ny)) continue;
         frc[nxf + g.yf(i, j, k)] = static_cast<float>(pgf(i, j - 1, i, j, k, g.dy));
       }
-  for (int k = 1; k < nz; ++k)
-    for (int j = 0; j < g.ny; ++j)
-      for (int i = 0; i < g.nx; ++i) {
-        const double idz = 1.0 / std::max(zc(i, j, k) - zc(i, j, k - 1), 1.0e-6);
+  // z faces: K14 of the base with the column reference, as the FP64 step
+  // computes it, is the plain -dp_b/dz - g rho_b plus the reference's share
+  // (p_ref,u - p_ref,l)/dz + g (rho_ref,l + rho_ref,u)/2, which its
+  // recurrence (four fixed-point iterations) leaves at the last iteration's
+  // change; that share, a smooth function of height, is kept here from the
+  // base column, so the departures' K14 needs no reference state.
+  const double Rd = 287.05, p0 = 1.0e5, kap = 0.2854;
+  auto rho_of = [&](double p, double th) {  // dry_thermo.hpp:35-40
+    const double ex = std::pow(std::max(p / p0, 1.0e-6), kap);
+    return p / std::max(Rd * th * ex, 1.0e-6);
+  };
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const size_t c0 = g.cell(i, j, 0);
+      double pref = eos64(br[c0], bt[c0]);
+      double rref = rho_of(pref, bt[c0] / std::max(br[c0], 1.0e-6));
+      for (int k = 1; k < nz; ++k) {
         const size_t lo = g.cell(i, j, k - 1), hi = g.cell(i, j, k);
-        frc[nxf + nyf + g.zf(i, j, k)] =
-            static_cast<float>(-(pb[hi] - pb[lo]) * idz - grav * (0.5 * (br[lo] + br[hi])));
+        const double thk = bt[hi] / std::max(br[hi], 1.0e-6);
+        const double dz = zc(i, j, k) - zc(i, j, k - 1);
+        double pn = std::max(1.0, pref - grav * rref * dz);
+        for (int it = 0; it < 4; ++it) {
+          const double rn = rho_of(pn, thk);
+          pn = std::max(pref - grav * 0.5 * (rref + rn) * dz, 1.0);
+        }
+        const double rref1 = rho_of(pn, thk);
+        const double idz = 1.0 / std::max(dz, 1.0e-6);
+        const double share = (pn - pref) * idz + grav * 0.5 * (rref + rref1);
+        frc[nxf + nyf + g.zf(i, j, k)] = static_cast<float>(
+            -(pb[hi] - pb[lo]) * idz - grav * (0.5 * (br[lo] + br[hi])) + share);
+        pref = pn;
+        rref = rref1;
       }
+    }
   if (!S->b_dev) S->b_dev = dalloc(lev.size() * sizeof(float));
   if (!S->b3_dev) S->b3_dev = dalloc(dev.size() * sizeof(float));
   if (!S->bf_dev) S->bf_dev = dalloc(frc.size() * sizeof(float));

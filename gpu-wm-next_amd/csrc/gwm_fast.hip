@@ -63,8 +63,9 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
   static_assert(!DOWN || (!TERRAIN && !COLREF), "the downward march is for flat grids");
   // Terrain perturbation form: per-cell bases, base forcing per face, and
   // K14 without a reference state (the column reference's terms cancel
-  // between its recurrence and K14 up to the recurrence's residual, so K14
-  // acts on the departures and the base's own forcing is added).
+  // between its recurrence and K14 up to the recurrence's residual, which
+  // the z-face base forcing carries, computed on the base column; K14 acts
+  // on the departures and adds that forcing).
   constexpr bool TP = TERRAIN && PERT;
   static_assert(!TP || !COLREF, "the terrain perturbation form has no column reference");
   constexpr int TY = TYV, PY = TY + 2, PN = PX * PY;

@@ -265,8 +265,9 @@ __device__ inline R d_qsat(R T, R p) {  // moist_thermo.hpp:23-36
 //   (rho theta)_b = rho_b theta_b,
 // and the base's own pressure-gradient and gravity forcing, fixed for the
 // run, is held per face in FP32 (fu, fv: the terrain-following horizontal
-// pressure gradient of p_b; fw: -dp_b/dz - g rho_b at the z faces), so that
-// only departures pass through FP32 pressure arithmetic.  All six are
+// pressure gradient of p_b; fw: -dp_b/dz - g rho_b at the z faces plus the
+// share of K14 that the column reference's recurrence leaves on the base
+// column), so that only departures pass through FP32 pressure arithmetic.  All six are
 // nullptr on flat grids.
 template <class R>
 struct Base {

@@ -253,7 +253,8 @@ int gwm_state_destroy(gwm_state* state);
  * |h| > 1e-6 m) the base is 3-D: the FP64 state loaded, cell by cell, with
  * its pressure-gradient and gravity forcing held per face, so that K14
  * acts on the departures without the column reference (whose terms cancel
- * against its recurrence); open or periodic sides, dry states only
+ * against its recurrence up to a residual that the base forcing carries);
+ * open or periodic sides, dry states only
  * (n_tracers 0; gwm_plan_view and the ensemble entry points refuse it).
  * Uploads, downloads and fills of GWM_RHO / GWM_RHO_THETA take and return
  * full values (an upload re-bases that field); every other operator accepts

@@ -6,7 +6,8 @@ it, and holds the base's own pressure-gradient and gravity forcing per face,
 so only departures pass through FP32 pressure arithmetic.  Its K14 acts on
 the departures without a reference state: the column reference's terms
 cancel between its recurrence and K14 up to the recurrence's residual
-(dry_fast_modes.cu:108-158), so the FP64 step with the column reference is
+(dry_fast_modes.cu:108-158), which the z-face base forcing carries,
+computed on the base column; the FP64 step with the column reference is
 the reference here.
 
 The cases are the mountain wave (Agnesi ridge 400 m, half-width 10 km,
@@ -89,8 +90,8 @@ def test_terrain_perturbation_starts_on_the_fp64_state(ctxs):
 @pytest.mark.parametrize("periodic_x", [False, True], ids=["relax_open_x", "periodic"])
 def test_terrain_perturbation_against_fp64(ctxs, periodic_x):
     """Twelve steps (3 s): every field of the perturbation form within
-    relative L2 1e-4 of the FP64 step, and w at least ten times closer than
-    the full FP32 form's (whose one-ulp rho*theta rounding moves p by
+    relative L2 1e-4 of the FP64 step, and w at least a hundred times closer
+    than the full FP32 form's (whose one-ulp rho*theta rounding moves p by
     ~10 mPa against the young wave's differences)."""
     c32, c64 = ctxs
     nx, ny, nz, steps = 96, 8, 40, 12
@@ -103,7 +104,7 @@ def test_terrain_perturbation_against_fp64(ctxs, periodic_x):
     print("perturbation form", e_tp)
     print("full form", e_full)
     assert max(e_tp.values()) <= 1e-4, e_tp
-    assert e_tp["mom_w"] * 10 <= e_full["mom_w"], (e_tp, e_full)
+    assert e_tp["mom_w"] * 100 <= e_full["mom_w"], (e_tp, e_full)
 
 
 def test_terrain_perturbation_checkpoint_restart_is_bit_exact(ctxs):
@@ -139,3 +140,53 @@ def test_terrain_perturbation_refuses_what_it_does_not_carry(ctxs):
     st.init_constant_n(theta_surface=288.0, brunt_vaisala=0.01, p_surface=1.0e5, u=10.0)
     with pytest.raises(G.GwmError, match="terrain"):
         st.plan_view(0, 0, 0.0)
+
+
+def test_terrain_perturbation_single_operators(ctxs):
+    """From the same (initial) state: the slow tendencies (whose pressure
+    gradient is the departures' plus the base forcing of each face) and one
+    fast subcycle (theta and rho*theta rebuilt from the cell's base) against
+    the FP64 operators, beside the full FP32 form.  The momentum tendencies,
+    where pressure enters, within 1e-5 (full form: 5.6e-3 on u); the mass
+    and rho*theta tendencies, FP32 flux differences of the stored momenta
+    in either form, no worse than the full form's."""
+    c32, c64 = ctxs
+    cfg = G.step_config(dt=0.25, fast_substeps=6)
+    st = {}
+    for key, ctx, pert in (("fp64", c64, False), ("tp", c32, True), ("full", c32, False)):
+        _, s = mountain(ctx, 64, 8, 30, False, perturbation=pert)
+        s.slow_tendencies(cfg)
+        st[key] = s
+    for f in (G.TEND_RHO, G.TEND_RHO_THETA, G.TEND_U, G.TEND_W):
+        r = st["fp64"].get(f).astype(np.float64)
+        e = {k: np.linalg.norm(st[k].get(f).astype(np.float64) - r) / max(np.linalg.norm(r), 1e-30)
+             for k in ("tp", "full")}
+        print("tendency", f, e)
+        if f in (G.TEND_U, G.TEND_W):
+            assert e["tp"] <= 1e-5, (f, e)
+        else:
+            assert e["tp"] <= e["full"], (f, e)
+    for s in st.values():
+        s.fast_modes(cfg)
+    e = rel_l2(st["fp64"], st["tp"])
+    print("fast subcycle", e)
+    assert max(e.values()) <= 1e-6, e
+
+
+def test_terrain_perturbation_with_boundary_strips(ctxs):
+    """The strip boundary overwrites the outermost owned strips from the
+    captured reference, which the perturbation form holds as departures
+    (a linear map either way): eight steps against the FP64 step."""
+    c32, c64 = ctxs
+    cfg = G.step_config(dt=0.25, fast_substeps=6)
+    bc = G.BoundaryConfig(G.BOUNDARY_STRIPS, 0.0, G.BOUNDARY_FN(), None)
+    out = []
+    for ctx, pert in ((c64, False), (c32, True)):
+        _, st = mountain(ctx, 64, 8, 30, False, perturbation=pert)
+        for _ in range(8):
+            st.ssprk3_step(cfg, bc)
+        assert st.summary()["nonfinite_values"] == 0
+        out.append(st)
+    e = rel_l2(out[0], out[1])
+    print("strips", e)
+    assert max(e.values()) <= 1e-4, e
