@@ -384,12 +384,22 @@ __device__ __forceinline__ void fast_body(const FastArgs<R>& a, int tiles_x, int
         const R dz = zc[m + (k + 1) * plane_m] - zc[m + (k - 1) * plane_m];
         return qdiv(pup[off] - pdn[off], rmax(dz, R(1.0e-6)));
       };
+      // The own column's dp/dz and height enter all four faces: evaluated
+      // once (the same arithmetic as at each face).
+      R g_own = R(0), z_own = R(0);
+      if constexpr (TERRAIN) {
+        g_own = vgrad(ci, own_m);
+        z_own = a.m.zc[own_m + k * plane_m];
+      }
       auto pgf = [&](int offl, int offr, int il, int jl, int ir, int jr, R d, R rd) -> R {
         const R pl = pk[offl], pr = pk[offr];
         if (!TERRAIN) return qdivc(pr - pl, d, rd);
-        const size_t ml = mcol(il, jl), mr = mcol(ir, jr);
-        const R dzd = qdivc(a.m.zc[mr + k * plane_m] - a.m.zc[ml + k * plane_m], d, rd);
-        const R gl = vgrad(offl, ml), gr = vgrad(offr, mr);
+        const bool own_l = offl == ci;  // the own cell is the left or the right one
+        const size_t mn = own_l ? mcol(ir, jr) : mcol(il, jl);
+        const R zn = a.m.zc[mn + k * plane_m], gn = vgrad(own_l ? offr : offl, mn);
+        const R zl = own_l ? z_own : zn, zr = own_l ? zn : z_own;
+        const R gl = own_l ? g_own : gn, gr = own_l ? gn : g_own;
+        const R dzd = qdivc(zr - zl, d, rd);
         return qdivc(pr - pl, d, rd) - dzd * (R(0.5) * (gl + gr));
       };
       // K12 / K13: faces i, i+1 and j, j+1 of the cell (the terrain

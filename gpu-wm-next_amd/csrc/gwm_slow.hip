@@ -369,6 +369,15 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         return qdiv(P(ii, jj, k + 1) - P(ii, jj, k - 1), rmax(dz, R(1.0e-6)));
       };
 
+      // The own column's dp/dz and height enter its x face and its y face
+      // (and the domain-edge faces): evaluated once, the same arithmetic as
+      // at each face.
+      R g_own = R(0), z_own = R(0);
+      if constexpr (TERRAIN) {
+        g_own = vgrad(i, j);
+        z_own = a.m.zc[mcol(i, j) + k * plane_m];
+      }
+
       // ---- K2 (dry_core.cu:62-144)
       R t_rho, t_rt;
       {
@@ -445,10 +454,15 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         const R pl = P(fi - 1, j, k), pr = P(fi, j, k);
         R term;
         if (TERRAIN) {
-          const R zl = a.m.zc[mcol(fi - 1, j) + k * plane_m];
-          const R zr = a.m.zc[mcol(fi, j) + k * plane_m];
+          // face fi lies between columns fi-1 and fi, one of them the own
+          // (fi == i: the right one; the edge face nx: the left one)
+          const bool own_r = fi == i;
+          const int nb = own_r ? fi - 1 : fi;
+          const R zn = a.m.zc[mcol(nb, j) + k * plane_m], gn = vgrad(nb, j);
+          const R zl = own_r ? zn : z_own, zr = own_r ? z_own : zn;
+          const R gl = own_r ? gn : g_own, gr = own_r ? g_own : gn;
           const R dzdx = qdivc(zr - zl, dx, rdx);
-          term = qdivc(pr - pl, dx, rdx) - dzdx * (R(0.5) * (vgrad(fi - 1, j) + vgrad(fi, j)));
+          term = qdivc(pr - pl, dx, rdx) - dzdx * (R(0.5) * (gl + gr));
           if constexpr (TP) term = term + ld32(a.base.fu, fo);
         } else {
           term = qdivc(pr - pl, dx, rdx);
@@ -481,10 +495,13 @@ __device__ __forceinline__ void slow_body(const SlowArgs<R>& a, int tiles_x, int
         const R ps = P(i, fj - 1, k), pn = P(i, fj, k);
         R term;
         if (TERRAIN) {
-          const R zs = a.m.zc[mcol(i, fj - 1) + k * plane_m];
-          const R zn = a.m.zc[mcol(i, fj) + k * plane_m];
+          const bool own_n = fj == j;
+          const int nb = own_n ? fj - 1 : fj;
+          const R zb = a.m.zc[mcol(i, nb) + k * plane_m], gb = vgrad(i, nb);
+          const R zs = own_n ? zb : z_own, zn = own_n ? z_own : zb;
+          const R gs = own_n ? gb : g_own, gnn = own_n ? g_own : gb;
           const R dzdy = qdivc(zn - zs, dy, rdy);
-          term = qdivc(pn - ps, dy, rdy) - dzdy * (R(0.5) * (vgrad(i, fj - 1) + vgrad(i, fj)));
+          term = qdivc(pn - ps, dy, rdy) - dzdy * (R(0.5) * (gs + gnn));
           if constexpr (TP) term = term + ld32(a.base.fv, fo);
         } else {
           term = qdivc(pn - ps, dy, rdy);
