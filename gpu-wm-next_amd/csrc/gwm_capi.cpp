@@ -764,6 +764,8 @@ void run_boundary(gwm_state* S, void** fs, const gwm_boundary_config* b, double 
     view.ntr = S->ntr;
     view.view = true;
     view.owner = S;
+    view.pert = S->pert;  // reads and writes convert with the owner's bases
+    view.pert3 = S->pert3;
     view.faces_synced = true;
     for (int f = 0; f < 5; ++f) view.vset[f] = fs[f];
     HIPCHK(hipStreamSynchronize(S->ctx->stream));
@@ -824,6 +826,8 @@ void ssprk3(gwm_state* S, const gwm_step_config& c, const gwm_boundary_config* b
       view.ntr = S->ntr;
       view.view = true;
       view.owner = S;
+      view.pert = S->pert;
+      view.pert3 = S->pert3;
       view.faces_synced = true;
       for (int f = 0; f < 5; ++f) view.vset[f] = zs[f];
       HIPCHK(hipStreamSynchronize(S->ctx->stream));
@@ -915,6 +919,8 @@ void tracers_ssprk3(gwm_state* S, const gwm_step_config& c, int bmode,
       view.ntr = S->ntr;
       view.view = true;
       view.owner = S;
+      view.pert = S->pert;
+      view.pert3 = S->pert3;
       view.faces_synced = true;
       for (int q = 0; q < 5; ++q) view.vset[q] = ds[q];
       view.vtr = f;

@@ -190,3 +190,31 @@ def test_terrain_perturbation_with_boundary_strips(ctxs):
     e = rel_l2(out[0], out[1])
     print("strips", e)
     assert max(e.values()) <= 1e-4, e
+
+
+def test_boundary_callback_view_reads_full_values(ctxs):
+    """A boundary plug-in handed a stage view of a perturbation-form state
+    reads full values through it (the view converts with its owner's
+    bases), as from a full-form state: at the step's entry the view's rho
+    is the state's, bit for bit, on the owned cells."""
+    import ctypes
+    c32, _ = ctxs
+    grid, st = mountain(c32, 64, 8, 30, True, perturbation=True)
+    rho0 = st.get(0)
+    shape, h = st.storage_shape(0), grid.halo
+    seen = []
+
+    def fn(hd, t, _user):
+        buf = np.zeros(int(np.prod(shape)), np.float32)
+        rc = G.lib().gwm_state_download(hd, 0, buf.ctypes.data_as(ctypes.c_void_p))
+        owned = buf.reshape(shape)[:, h:shape[1] - h, h:shape[2] - h]
+        seen.append((t, rc, owned.copy()))
+        return 0
+
+    bc = G.BoundaryConfig(G.BOUNDARY_NONE, 0.0, G.BOUNDARY_FN(fn), None)
+    st.ssprk3_step(G.step_config(dt=0.25, fast_substeps=6), bc)
+    assert [x[0] for x in seen] == [0.0, 0.25, 0.125]
+    assert all(x[1] == 0 for x in seen)
+    assert np.array_equal(seen[0][2], rho0)
+    for _, _, r in seen[1:]:
+        assert np.max(np.abs(r - rho0)) <= 1e-4 * np.max(rho0)
