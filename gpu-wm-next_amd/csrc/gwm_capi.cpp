@@ -314,7 +314,7 @@ struct gwm_state {
   // Network-cell queue and segment counts of the surrogate warm-rain call
   // (allocated at its first call, sized for the grid).
   unsigned* sur_queue = nullptr;
-  // Perturbation form (flat, doubly periodic, FP32): rho and rho*theta are
+  // Perturbation form (FP32; flat and doubly periodic, or terrain): rho and rho*theta are
   // stored as departures from per-level bases, the FP64 level means of the
   // state they were built from; the device holds the bases in working
   // precision (rho_b, (rho theta)_b, theta_b, p_b).

@@ -246,7 +246,7 @@ __device__ inline R d_qsat(R T, R p) {  // moist_thermo.hpp:23-36
 }
 
 // ---------------------------------------------------------------------------
-// Perturbation form (flat, doubly periodic grids).  The state holds
+// Perturbation form (flat, doubly periodic grids; terrain below).  The state holds
 // rho' = rho - rho_b(k) and (rho theta)' = rho theta - (rho theta)_b(k)
 // against a per-level base (the FP64 level means of the initial state,
 // rounded to working precision here); rho == nullptr means the full form.
